@@ -1,0 +1,336 @@
+"""MI355X-native Kohn-Sham exchange-correlation path (Chinium's Grid + ExchangeCorrelation).
+
+Python mirror of the reference's C++ interface over the C ABI in
+include/chinium_xc.h (libchinium_xc.so, built in-tree for gfx950):
+
+    Grid(mol, "SG-1", nthreads, output)   src/Grid/Grid.h:90-116
+    grid.setType(1); grid.getAO(0, 1)     src/Grid/Grid.cpp:14-33
+    grid.getDensity([D])                  src/Grid/Grid.cpp:35-49
+    xc.Read("PBE", output); xc.Evaluate("ev", grid)   src/ExchangeCorrelation.cpp:18-248
+    grid.getEnergy(); grid.getFock()      src/Grid/Grid.cpp:112-171
+
+The call order per SCF iteration is the reference's (Restricted/SP.cpp:52-55):
+getDensity -> Evaluate("ev") -> getEnergy -> getFock. There is no CPU
+fallback: without the built library every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libchinium_xc.so")
+DATA_ROOT = os.path.join(PKG_DIR, "data")
+ANGSTROM_TO_BOHR = 1.8897259886  # src/Macro/Unit.h:1
+
+CHX_PREC_FP64 = 0
+
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class ChxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib():
+    """Load libchinium_xc.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libchinium_xc.so is not built; run __graft_entry__.build() "
+                          "(make -C chinium_amd/csrc)")
+    L = ctypes.CDLL(LIB_PATH)
+    i, d, cp, l = ctypes.c_int, ctypes.c_double, ctypes.c_char_p, ctypes.c_long
+    P = ctypes.POINTER
+    sig = {
+        "chx_last_error": (cp, []),
+        "chx_device_count": (i, [P(i)]),
+        "chx_mol_create": (i, [i, _vp, _vp, P(_vp)]),
+        "chx_mol_set_basis_file": (i, [_vp, cp, cp]),
+        "chx_mol_add_shell": (i, [_vp, i, i, i, _vp, _vp]),
+        "chx_mol_nbasis": (i, [_vp, P(i)]),
+        "chx_mol_nshells": (i, [_vp, P(i)]),
+        "chx_mol_get_shells": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "chx_mol_overlap": (i, [_vp, _vp]),
+        "chx_mol_destroy": (None, [_vp]),
+        "chx_grid_create": (i, [_vp, cp, cp, i, i, P(_vp)]),
+        "chx_grid_copy": (i, [_vp, P(_vp)]),
+        "chx_grid_destroy": (None, [_vp]),
+        "chx_grid_num_threads": (i, [_vp, P(i)]),
+        "chx_grid_num_points": (i, [_vp, P(l)]),
+        "chx_grid_get_points": (i, [_vp, _vp]),
+        "chx_grid_set_type": (i, [_vp, i]),
+        "chx_grid_get_ao": (i, [_vp, i, i]),
+        "chx_grid_get_density": (i, [_vp, _vp, i, i]),
+        "chx_grid_get_density_device": (i, [_vp, _vp, i, i]),
+        "chx_grid_num_electrons": (i, [_vp, P(d)]),
+        "chx_grid_energy": (i, [_vp, P(d)]),
+        "chx_grid_fock": (i, [_vp, _vp, i, i]),
+        "chx_grid_fock_device": (i, [_vp, _vp, i, i]),
+        "chx_grid_set_vrho": (i, [_vp, _vp, l]),
+        "chx_grid_stream": (i, [_vp, P(_vp)]),
+        "chx_grid_set_precision": (i, [_vp, i]),
+        "chx_grid_set_screening": (i, [_vp, d]),
+        "chx_grid_stats": (i, [_vp, P(l), P(l), P(d), P(d)]),
+        "chx_grid_profile": (i, [_vp, i]),
+        "chx_grid_kernel_times": (i, [_vp, i, _vp, _vp]),
+        "chx_xc_read": (i, [cp, cp, i, i, P(_vp)]),
+        "chx_xc_create": (i, [i, _vp, i, P(_vp)]),
+        "chx_xc_destroy": (None, [_vp]),
+        "chx_xc_info": (i, [_vp, P(i), P(d), P(i), _vp]),
+        "chx_xc_evaluate": (i, [_vp, _vp, cp]),
+        "chx_selftest": (i, [P(d)]),
+        "chx_lebedev_rule": (i, [i, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise ChxError(rc, lib().chx_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def device_count():
+    n = ctypes.c_int()
+    _check(lib().chx_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def selftest():
+    """Max deviation of one FP64 MFMA tile from the scalar product (0 = layout holds)."""
+    e = ctypes.c_double()
+    _check(lib().chx_selftest(ctypes.byref(e)))
+    return e.value
+
+
+def lebedev_rule(n):
+    """The library's Lebedev-Laikov rule with n points: rows (x, y, z, w), sum w = 1."""
+    out = np.zeros((n, 4))
+    _check(lib().chx_lebedev_rule(int(n), _ptr(out)))
+    return out
+
+
+class Molecule:
+    """Atoms (Z, bohr) and a normalised basis: the Mwfn fields the XC path reads."""
+
+    def __init__(self, Z, xyz_bohr, basis=None, data_root=DATA_ROOT):
+        L = lib()
+        self.Z = np.ascontiguousarray(Z, dtype=np.int32)
+        self.xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+        h = _vp()
+        _check(L.chx_mol_create(len(self.Z), _ptr(self.Z), _ptr(self.xyz), ctypes.byref(h)))
+        self.h = h
+        if basis is not None:
+            _check(L.chx_mol_set_basis_file(self.h, data_root.encode(), basis.encode()))
+
+    def add_shell(self, center, type_, exps, norm_coefs):
+        e = np.ascontiguousarray(exps, dtype=np.float64)
+        c = np.ascontiguousarray(norm_coefs, dtype=np.float64)
+        _check(lib().chx_mol_add_shell(self.h, int(center), int(type_), len(e), _ptr(e), _ptr(c)))
+
+    @property
+    def nbasis(self):
+        n = ctypes.c_int()
+        _check(lib().chx_mol_nbasis(self.h, ctypes.byref(n)))
+        return n.value
+
+    def shells(self):
+        L = lib()
+        ns = ctypes.c_int()
+        _check(L.chx_mol_nshells(self.h, ctypes.byref(ns)))
+        ns = ns.value
+        t = np.zeros(ns, np.int32); npr = np.zeros(ns, np.int32); c = np.zeros(ns, np.int32)
+        _check(L.chx_mol_get_shells(self.h, _ptr(t), _ptr(npr), _ptr(c), None, None))
+        e = np.zeros(int(npr.sum())); k = np.zeros(int(npr.sum()))
+        _check(L.chx_mol_get_shells(self.h, None, None, None, _ptr(e), _ptr(k)))
+        return t, npr, c, e, k
+
+    def overlap(self):
+        n = self.nbasis
+        S = np.zeros((n, n), order="F")
+        _check(lib().chx_mol_overlap(self.h, _ptr(S)))
+        return np.array(S)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.chx_mol_destroy(self.h)
+            self.h = None
+
+
+class Grid:
+    """Device-resident molecular grid; mirrors class Grid (src/Grid/Grid.h:90-116)."""
+
+    def __init__(self, mol=None, grid="", nthreads=1, output=0, data_root=DATA_ROOT, _handle=None):
+        self.h = None
+        if _handle is not None:
+            self.h = _handle
+        else:
+            h = _vp()
+            _check(lib().chx_grid_create(mol.h, grid.encode(), data_root.encode(), int(nthreads), int(output),
+                                         ctypes.byref(h)))
+            self.h = h
+        n = ctypes.c_long()
+        _check(lib().chx_grid_num_points(self.h, ctypes.byref(n)))
+        self.npoints = n.value
+        self.nbasis = mol.nbasis if mol is not None else None
+
+    def copy(self):
+        h = _vp()
+        _check(lib().chx_grid_copy(self.h, ctypes.byref(h)))
+        g = Grid(_handle=h)
+        g.nbasis = self.nbasis
+        return g
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.chx_grid_destroy(self.h)
+            self.h = None
+
+    def getNumThreads(self):
+        n = ctypes.c_int()
+        _check(lib().chx_grid_num_threads(self.h, ctypes.byref(n)))
+        return n.value
+
+    def points(self):
+        P = np.zeros((self.npoints, 4))
+        _check(lib().chx_grid_get_points(self.h, _ptr(P)))
+        return P
+
+    def setType(self, t):
+        _check(lib().chx_grid_set_type(self.h, int(t)))
+
+    def setScreening(self, thr):
+        _check(lib().chx_grid_set_screening(self.h, float(thr)))
+
+    def setPrecision(self, p):
+        _check(lib().chx_grid_set_precision(self.h, int(p)))
+
+    def getAO(self, derivative=0, output=0):
+        _check(lib().chx_grid_get_ao(self.h, int(derivative), int(output)))
+
+    def getDensity(self, Ds):
+        Ds = list(Ds)
+        n = Ds[0].shape[0]
+        buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for D in Ds])
+        _check(lib().chx_grid_get_density(self.h, _ptr(buf), len(Ds), n))
+
+    def getDensityDevice(self, ptr, nspin, nbasis):
+        _check(lib().chx_grid_get_density_device(self.h, ctypes.c_void_p(ptr), int(nspin), int(nbasis)))
+
+    def getNumElectrons(self):
+        v = ctypes.c_double()
+        _check(lib().chx_grid_num_electrons(self.h, ctypes.byref(v)))
+        return v.value
+
+    def getEnergy(self):
+        v = ctypes.c_double()
+        _check(lib().chx_grid_energy(self.h, ctypes.byref(v)))
+        return v.value
+
+    def getFock(self, nspin=1):
+        n = self.nbasis
+        F = np.zeros(nspin * n * n)
+        _check(lib().chx_grid_fock(self.h, _ptr(F), nspin, n))
+        return [F[k * n * n:(k + 1) * n * n].reshape((n, n), order="F").copy() for k in range(nspin)]
+
+    def getFockDevice(self, ptr, nspin, nbasis):
+        _check(lib().chx_grid_fock_device(self.h, ctypes.c_void_p(ptr), int(nspin), int(nbasis)))
+
+    def setVrho(self, vrho):
+        v = np.ascontiguousarray(vrho, dtype=np.float64)
+        _check(lib().chx_grid_set_vrho(self.h, _ptr(v), len(v)))
+
+    def stream(self):
+        s = _vp()
+        _check(lib().chx_grid_stream(self.h, ctypes.byref(s)))
+        return s.value
+
+    STAGES = ("permute_d", "density", "functional", "vxc", "finalize_v")
+
+    def profile(self, enable=True):
+        _check(lib().chx_grid_profile(self.h, int(bool(enable))))
+
+    def kernel_times(self, reset=True):
+        """Accumulated HIP-event milliseconds and launch counts per stage."""
+        ms = np.zeros(5)
+        n = np.zeros(5, dtype=np.int64)
+        _check(lib().chx_grid_kernel_times(self.h, int(bool(reset)), _ptr(ms), _ptr(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
+
+    def stats(self):
+        nb, npnt, mt, fl = ctypes.c_long(), ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().chx_grid_stats(self.h, ctypes.byref(nb), ctypes.byref(npnt), ctypes.byref(mt), ctypes.byref(fl)))
+        return {"batches": nb.value, "points": npnt.value, "mean_tiles": mt.value, "flops_per_build": fl.value}
+
+
+class ExchangeCorrelation:
+    """Mirrors class ExchangeCorrelation (src/ExchangeCorrelation.h:8-17)."""
+
+    def __init__(self):
+        self.h = None
+        self.Codes = []
+        self.EXX = 1.0
+        self.Spin = 1
+        self.Family = ""
+
+    def _load_info(self):
+        L = lib()
+        fam, exx, nc = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        _check(L.chx_xc_info(self.h, ctypes.byref(fam), ctypes.byref(exx), ctypes.byref(nc), None))
+        codes = np.zeros(nc.value, np.int32)
+        _check(L.chx_xc_info(self.h, None, None, None, _ptr(codes)))
+        self.Codes = [int(c) for c in codes]
+        self.EXX = exx.value
+        self.Family = ["LDA", "GGA", "mGGA"][fam.value]
+        self.family = fam.value
+
+    def Read(self, df, output=0, data_root=DATA_ROOT):
+        h = _vp()
+        _check(lib().chx_xc_read(data_root.encode(), df.encode(), int(self.Spin), int(output), ctypes.byref(h)))
+        self._set(h)
+
+    def from_codes(self, codes):
+        c = np.ascontiguousarray(codes, dtype=np.int32)
+        h = _vp()
+        _check(lib().chx_xc_create(len(c), _ptr(c), int(self.Spin), ctypes.byref(h)))
+        self._set(h)
+        return self
+
+    def _set(self, h):
+        if self.h is not None:
+            lib().chx_xc_destroy(self.h)
+        self.h = h
+        self._load_info()
+
+    def Evaluate(self, order, grid):
+        _check(lib().chx_xc_evaluate(self.h, grid.h, order.encode()))
+
+    def __bool__(self):
+        return len(self.Codes) > 0
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.chx_xc_destroy(self.h)
+            self.h = None
+
+
+def xc_build(grid, xc, D):
+    """One XC build in the reference's order; returns (E_xc, V)."""
+    grid.getDensity([D])
+    xc.Evaluate("ev", grid)
+    E = grid.getEnergy()
+    V = grid.getFock()[0]
+    return E, V

@@ -1,0 +1,535 @@
+// C ABI of the XC path (include/chinium_xc.h). Every entry point catches
+// exceptions and returns a status; the message is kept per host thread.
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+
+#include "internal.hpp"
+#include "xc_functionals.hpp"
+
+void mol_set_basis_file(chx_mol* m, const std::string& data_root, const std::string& basis);
+void mol_overlap(const chx_mol* m, double* S);
+
+namespace chx {
+void grid_build(chx_grid* g, const std::string& grid_name, const std::string& data_root);
+void grid_plan(chx_grid* g);
+void launch_selftest(hipStream_t s, double* out);
+}  // namespace chx
+int chx_lebedev_rule_impl(int n, double* xyzw);
+namespace chx {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(CHX_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+}  // namespace chx
+
+using namespace chx;
+
+void chx_grid::mark_begin(int stage, hipEvent_t* a) {
+  (void)stage;
+  *a = nullptr;
+  if (!profile) return;
+  CHX_HIP(hipEventCreate(a));
+  CHX_HIP(hipEventRecord(*a, stream));
+}
+
+void chx_grid::mark_end(int stage, hipEvent_t a) {
+  if (!profile || !a) return;
+  hipEvent_t b;
+  CHX_HIP(hipEventCreate(&b));
+  CHX_HIP(hipEventRecord(b, stream));
+  pending.push_back({stage, a, b});
+}
+
+void chx_grid::collect_times() {
+  if (pending.empty()) return;
+  CHX_HIP(hipStreamSynchronize(stream));
+  for (const Pending& p : pending) {
+    float ms = 0;
+    CHX_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    stage_ms[p.stage] += ms;
+    stage_n[p.stage] += 1;
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  pending.clear();
+}
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F> int guard(F&& f) {
+  try {
+    f();
+    return CHX_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return CHX_ERR_ARG;
+  }
+}
+
+void require(bool ok, int code, const char* msg) {
+  if (!ok) throw Error(code, msg);
+}
+
+void make_stream(chx_grid* g) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(CHX_ERR_DEVICE, "no HIP device available");
+  CHX_HIP(hipGetDevice(&g->device));
+  CHX_HIP(hipStreamCreate(&g->stream));
+}
+
+void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
+  g->d_codes.upload(xc->codes, g->stream);
+  hipEvent_t e = nullptr;
+  g->mark_begin(2, &e);
+  launch_functional(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_w.p, g->d_codes.p, (int)xc->codes.size(),
+                    xc->family, want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p, g->d_coef.p, g->d_en_part.p, g->en_blocks);
+  launch_reduce_en(g->stream, g->en_blocks, g->d_en_part.p, g->d_en.p);
+  g->mark_end(2, e);
+}
+
+// Stage ids of chx_grid_kernel_times.
+enum { kStPermute = 0, kStDensity = 1, kStFunctional = 2, kStVxc = 3, kStFinalize = 4 };
+
+void density_from_device(chx_grid* g, const double* dD) {
+  hipEvent_t e = nullptr;
+  g->mark_begin(kStPermute, &e);
+  launch_permute_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dint.p);
+  g->mark_end(kStPermute, e);
+  g->mark_begin(kStDensity, &e);
+  launch_density_f64(g->stream, (int)g->grp_batch.size(), g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p,
+                     g->d_bt_off.p, g->d_bt_list.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_Dint.p, g->npad,
+                     g->d_part.p);
+  g->mark_end(kStDensity, e);
+  g->have_density = true;
+  g->have_eval = false;
+  g->have_v = false;
+}
+
+void fock_to_device(chx_grid* g, double* dV) {
+  hipEvent_t e = nullptr;
+  launch_fill(g->stream, g->d_Vint.p, (size_t)g->npad * g->npad, 0.0);
+  g->mark_begin(kStVxc, &e);
+  launch_vxc_f64(g->stream, (int)g->vwork.size(), g->d_vwork.p, g->d_pair_batches.p, g->d_alive.p, g->ntiles, g->d_x.p,
+                 g->d_y.p, g->d_z.p, g->d_coef.p, g->view(), g->d_Vint.p, g->npad);
+  g->mark_end(kStVxc, e);
+  g->mark_begin(kStFinalize, &e);
+  launch_finalize_v(g->stream, g->nbasis, g->npad, g->d_Vint.p, g->d_ref2int.p, dV);
+  g->mark_end(kStFinalize, e);
+}
+
+void check_dm_args(const chx_grid* g, int nspin, int nbasis) {
+  require(g->ao_ready, CHX_ERR_STATE, "Grid::getAO must be called before the density");
+  require(nspin == 1, CHX_ERR_UNSUPPORTED, "Wrong number of spin types! (only the closed-shell path is built)");
+  require(nbasis == g->nbasis, CHX_ERR_ARG, "density matrix size does not match the basis");
+}
+}  // namespace
+
+extern "C" {
+
+const char* chx_last_error(void) { return g_last_error.c_str(); }
+
+int chx_device_count(int* n) {
+  return guard([&] {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+  });
+}
+
+// ---- molecule -------------------------------------------------------------
+int chx_mol_create(int natoms, const int* Z, const double* xyz, chx_mol** out) {
+  return guard([&] {
+    require(natoms > 0 && Z && xyz && out, CHX_ERR_ARG, "bad molecule arguments");
+    chx_mol* m = new chx_mol;
+    m->natoms = natoms;
+    m->Z.assign(Z, Z + natoms);
+    m->xyz.assign(xyz, xyz + 3 * natoms);
+    *out = m;
+  });
+}
+
+int chx_mol_set_basis_file(chx_mol* m, const char* data_root, const char* basis_name) {
+  return guard([&] {
+    require(m && data_root && basis_name, CHX_ERR_ARG, "bad arguments");
+    mol_set_basis_file(m, data_root, basis_name);
+  });
+}
+
+int chx_mol_add_shell(chx_mol* m, int center, int type, int nprim, const double* exps, const double* coefs) {
+  return guard([&] {
+    require(m && center >= 0 && center < m->natoms && nprim > 0 && exps && coefs, CHX_ERR_ARG, "bad shell");
+    require(type >= -3 && type <= 1, CHX_ERR_UNSUPPORTED, "angular momentum above f is not supported");
+    ShellH s{type, center, nprim, (int)m->exps.size(), m->nbasis};
+    m->exps.insert(m->exps.end(), exps, exps + nprim);
+    m->coefs.insert(m->coefs.end(), coefs, coefs + nprim);
+    m->shells.push_back(s);
+    m->nbasis += shell_size(type);
+  });
+}
+
+int chx_mol_nbasis(const chx_mol* m, int* n) {
+  return guard([&] { *n = m->nbasis; });
+}
+
+int chx_mol_nshells(const chx_mol* m, int* n) {
+  return guard([&] { *n = (int)m->shells.size(); });
+}
+
+int chx_mol_get_shells(const chx_mol* m, int* type, int* nprim, int* center, double* exps, double* coefs) {
+  return guard([&] {
+    for (size_t i = 0; i < m->shells.size(); ++i) {
+      if (type) type[i] = m->shells[i].type;
+      if (nprim) nprim[i] = m->shells[i].nprim;
+      if (center) center[i] = m->shells[i].center;
+    }
+    if (exps) std::memcpy(exps, m->exps.data(), sizeof(double) * m->exps.size());
+    if (coefs) std::memcpy(coefs, m->coefs.data(), sizeof(double) * m->coefs.size());
+  });
+}
+
+int chx_mol_overlap(const chx_mol* m, double* S) {
+  return guard([&] { mol_overlap(m, S); });
+}
+
+void chx_mol_destroy(chx_mol* m) { delete m; }
+
+// ---- grid -----------------------------------------------------------------
+int chx_grid_create(const chx_mol* m, const char* grid_name, const char* data_root, int nthreads, int output,
+                    chx_grid** out) {
+  chx_grid* g = nullptr;
+  int rc = guard([&] {
+    require(m && grid_name && data_root && out, CHX_ERR_ARG, "bad grid arguments");
+    g = new chx_grid;
+    g->mol = *m;
+    g->nthreads = nthreads > 0 ? nthreads : 1;
+    make_stream(g);
+    grid_build(g, grid_name, data_root);
+    if (output)
+      std::printf("Grid %s: %ld points in %d batches of %d; %d AOs in %d tiles\n", grid_name, g->npoints, g->nbatches,
+                  kBatch, g->nbasis, g->ntiles);
+  });
+  if (rc != CHX_OK) {
+    if (g) chx_grid_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return CHX_OK;
+}
+
+int chx_grid_copy(const chx_grid* s, chx_grid** out) {
+  chx_grid* g = nullptr;
+  int rc = guard([&] {
+    require(s && out, CHX_ERR_ARG, "bad arguments");
+    g = new chx_grid;
+    g->nthreads = s->nthreads; g->type = s->type; g->precision = s->precision; g->screen_thr = s->screen_thr;
+    g->mol = s->mol;
+    g->nbasis = s->nbasis; g->npad = s->npad; g->ntiles = s->ntiles;
+    g->int2ref = s->int2ref; g->ref2int = s->ref2int;
+    g->sh_xyz = s->sh_xyz; g->sh_type = s->sh_type; g->sh_nprim = s->sh_nprim; g->sh_poff = s->sh_poff;
+    g->p_exp = s->p_exp; g->p_coef = s->p_coef; g->tile_seg_off = s->tile_seg_off; g->seg = s->seg;
+    g->npoints = s->npoints; g->nbatches = s->nbatches; g->batch_n = s->batch_n;
+    g->hx = s->hx; g->hy = s->hy; g->hz = s->hz; g->hw = s->hw;
+    make_stream(g);
+    hipStream_t st = g->stream;
+    CHX_HIP(hipStreamSynchronize(s->stream));
+    g->d_sh_xyz.copy_from(s->d_sh_xyz, st); g->d_p_exp.copy_from(s->d_p_exp, st); g->d_p_coef.copy_from(s->d_p_coef, st);
+    g->d_sh_type.copy_from(s->d_sh_type, st); g->d_sh_nprim.copy_from(s->d_sh_nprim, st);
+    g->d_sh_poff.copy_from(s->d_sh_poff, st); g->d_tile_seg_off.copy_from(s->d_tile_seg_off, st);
+    g->d_seg.copy_from(s->d_seg, st); g->d_int2ref.copy_from(s->d_int2ref, st); g->d_ref2int.copy_from(s->d_ref2int, st);
+    g->d_x.copy_from(s->d_x, st); g->d_y.copy_from(s->d_y, st); g->d_z.copy_from(s->d_z, st);
+    g->d_w.copy_from(s->d_w, st); g->d_bn.copy_from(s->d_bn, st);
+    CHX_HIP(hipStreamSynchronize(st));
+    if (s->ao_ready) grid_plan(g);
+  });
+  if (rc != CHX_OK) {
+    if (g) chx_grid_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return CHX_OK;
+}
+
+void chx_grid_destroy(chx_grid* g) {
+  if (!g) return;
+  try {
+    g->collect_times();
+  } catch (...) {
+  }
+  if (g->stream) {
+    (void)hipStreamSynchronize(g->stream);
+    (void)hipStreamDestroy(g->stream);
+  }
+  delete g;
+}
+
+int chx_grid_num_threads(const chx_grid* g, int* n) {
+  return guard([&] { *n = g->nthreads; });
+}
+
+int chx_grid_num_points(const chx_grid* g, long* n) {
+  return guard([&] { *n = g->npoints; });
+}
+
+int chx_grid_get_points(const chx_grid* g, double* xyzw) {
+  return guard([&] {
+    long k = 0;
+    for (int b = 0; b < g->nbatches; ++b)
+      for (int i = 0; i < g->batch_n[b]; ++i, ++k) {
+        const size_t s = (size_t)b * kBatch + i;
+        xyzw[4 * k] = g->hx[s]; xyzw[4 * k + 1] = g->hy[s]; xyzw[4 * k + 2] = g->hz[s]; xyzw[4 * k + 3] = g->hw[s];
+      }
+  });
+}
+
+int chx_grid_set_type(chx_grid* g, int type) {
+  return guard([&] {
+    require(type >= -1 && type <= 2, CHX_ERR_ARG, "grid type must be -1, 0, 1 or 2");
+    require(type <= 1, CHX_ERR_UNSUPPORTED, "meta-GGA grids are not built yet");
+    g->type = type;
+  });
+}
+
+int chx_grid_get_ao(chx_grid* g, int derivative, int output) {
+  return guard([&] {
+    require(derivative == 0, CHX_ERR_UNSUPPORTED, "AO derivatives beyond the SCF order are not built");
+    grid_plan(g);
+    if (output)
+      std::printf("AO screening: %.2f of %d tiles alive per batch, %.3e FLOP per XC build\n", g->mean_tiles, g->ntiles,
+                  g->flops_build);
+  });
+}
+
+int chx_grid_get_density(chx_grid* g, const double* D, int nspin, int nbasis) {
+  return guard([&] {
+    check_dm_args(g, nspin, nbasis);
+    require(D != nullptr, CHX_ERR_ARG, "null density matrix");
+    CHX_HIP(hipMemcpyAsync(g->d_Din.p, D, sizeof(double) * (size_t)nbasis * nbasis, hipMemcpyHostToDevice, g->stream));
+    density_from_device(g, g->d_Din.p);
+  });
+}
+
+int chx_grid_get_density_device(chx_grid* g, const double* dD, int nspin, int nbasis) {
+  return guard([&] {
+    check_dm_args(g, nspin, nbasis);
+    require(dD != nullptr, CHX_ERR_ARG, "null density matrix");
+    density_from_device(g, dD);
+  });
+}
+
+int chx_grid_num_electrons(chx_grid* g, double* N) {
+  return guard([&] {
+    require(g->have_density, CHX_ERR_STATE, "no density on the grid");
+    if (!g->have_eval) {
+      chx_xc none;
+      none.family = g->type < 0 ? 0 : g->type;
+      run_functional(g, &none, false, false);
+    }
+    double en[2];
+    CHX_HIP(hipMemcpyAsync(en, g->d_en.p, sizeof(en), hipMemcpyDeviceToHost, g->stream));
+    CHX_HIP(hipStreamSynchronize(g->stream));
+    *N = en[1];
+  });
+}
+
+int chx_grid_energy(chx_grid* g, double* E) {
+  return guard([&] {
+    require(g->have_eval, CHX_ERR_STATE, "ExchangeCorrelation::Evaluate must precede Grid::getEnergy");
+    double en[2];
+    CHX_HIP(hipMemcpyAsync(en, g->d_en.p, sizeof(en), hipMemcpyDeviceToHost, g->stream));
+    CHX_HIP(hipStreamSynchronize(g->stream));
+    g->last_E = en[0];
+    g->last_N = en[1];
+    *E = en[0];
+  });
+}
+
+int chx_grid_fock(chx_grid* g, double* F, int nspin, int nbasis) {
+  return guard([&] {
+    require(g->have_v, CHX_ERR_STATE, "no potential on the grid (Evaluate with \"v\" or \"ev\" first)");
+    require(nspin == 1, CHX_ERR_UNSUPPORTED, "only the closed-shell path is built");
+    require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
+    fock_to_device(g, g->d_Vout.p);
+    CHX_HIP(hipMemcpyAsync(F, g->d_Vout.p, sizeof(double) * (size_t)nbasis * nbasis, hipMemcpyDeviceToHost, g->stream));
+    CHX_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
+int chx_grid_fock_device(chx_grid* g, double* dF, int nspin, int nbasis) {
+  return guard([&] {
+    require(g->have_v, CHX_ERR_STATE, "no potential on the grid (Evaluate with \"v\" or \"ev\" first)");
+    require(nspin == 1, CHX_ERR_UNSUPPORTED, "only the closed-shell path is built");
+    require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
+    fock_to_device(g, dF);
+  });
+}
+
+int chx_grid_set_vrho(chx_grid* g, const double* vrho, long npoints) {
+  return guard([&] {
+    require(g->ao_ready, CHX_ERR_STATE, "Grid::getAO must be called first");
+    require(npoints == g->npoints && vrho, CHX_ERR_ARG, "v_rho must have one value per grid point");
+    std::vector<double> vw((size_t)g->nbatches * kBatch, 0.0);
+    long k = 0;
+    for (int b = 0; b < g->nbatches; ++b)
+      for (int i = 0; i < g->batch_n[b]; ++i, ++k) {
+        const size_t s = (size_t)b * kBatch + i;
+        vw[s] = vrho[k] * g->hw[s];
+      }
+    DevBuf<double> d;
+    d.upload(vw, g->stream);
+    launch_vrho_coef(g->stream, (long)vw.size(), d.p, g->d_coef.p);
+    CHX_HIP(hipStreamSynchronize(g->stream));
+    g->have_v = true;
+  });
+}
+
+int chx_grid_stream(const chx_grid* g, void** stream) {
+  return guard([&] { *stream = (void*)g->stream; });
+}
+
+int chx_grid_set_precision(chx_grid* g, int precision) {
+  return guard([&] {
+    require(precision == CHX_PREC_FP64, CHX_ERR_UNSUPPORTED, "only the FP64 form is built");
+    g->precision = precision;
+  });
+}
+
+int chx_grid_set_screening(chx_grid* g, double thr) {
+  return guard([&] {
+    require(thr >= 0, CHX_ERR_ARG, "screening threshold must be >= 0");
+    g->screen_thr = thr;
+    g->ao_ready = false;
+  });
+}
+
+int chx_grid_profile(chx_grid* g, int enable) {
+  return guard([&] {
+    g->collect_times();
+    g->profile = enable != 0;
+  });
+}
+
+int chx_grid_kernel_times(chx_grid* g, int reset, double* ms, long* launches) {
+  return guard([&] {
+    g->collect_times();
+    for (int k = 0; k < 5; ++k) {
+      if (ms) ms[k] = g->stage_ms[k];
+      if (launches) launches[k] = g->stage_n[k];
+      if (reset) { g->stage_ms[k] = 0; g->stage_n[k] = 0; }
+    }
+  });
+}
+
+int chx_grid_stats(const chx_grid* g, long* nbatches, long* npoints, double* mean_tiles, double* flops) {
+  return guard([&] {
+    if (nbatches) *nbatches = g->nbatches;
+    if (npoints) *npoints = g->npoints;
+    if (mean_tiles) *mean_tiles = g->mean_tiles;
+    if (flops) *flops = g->flops_build;
+  });
+}
+
+// ---- functional -----------------------------------------------------------
+int chx_xc_create(int ncodes, const int* codes, int spin, chx_xc** out) {
+  return guard([&] {
+    require(ncodes > 0 && codes && out, CHX_ERR_ARG, "bad functional arguments");
+    require(spin == 1, CHX_ERR_UNSUPPORTED, "only the spin-unpolarised functional path is built");
+    chx_xc* x = new chx_xc;
+    x->spin = spin;
+    for (int i = 0; i < ncodes; ++i) {
+      const int f = code_family(codes[i]);
+      if (f < 0) {
+        delete x;
+        throw Error(CHX_ERR_UNSUPPORTED, "functional code " + std::to_string(codes[i]) + " is not implemented");
+      }
+      x->codes.push_back(codes[i]);
+      x->family = std::max(x->family, f);
+      x->exx = std::max(x->exx, code_exx(codes[i]));
+    }
+    *out = x;
+  });
+}
+
+int chx_xc_read(const char* data_root, const char* df_name, int spin, int output, chx_xc** out) {
+  std::vector<int> codes;
+  int rc = guard([&] {
+    require(data_root && df_name && out, CHX_ERR_ARG, "bad arguments");
+    std::ifstream f(std::string(data_root) + "/DensityFunctionals/" + df_name + ".df");
+    if (!f.good()) throw Error(CHX_ERR_FILE, "Density functional file is missing!");
+    std::string line;
+    std::getline(f, line);
+    std::stringstream ss(line);
+    std::string tok;
+    while (ss >> tok) {
+      try {
+        size_t used = 0;
+        const int c = std::stoi(tok, &used);
+        if (used != tok.size()) throw std::invalid_argument(tok);
+        codes.push_back(c);
+      } catch (const std::exception&) {
+        throw Error(CHX_ERR_FILE, "Corrupted density functional file!");
+      }
+    }
+    if (codes.empty()) throw Error(CHX_ERR_FILE, "Corrupted density functional file!");
+  });
+  if (rc != CHX_OK) return rc;
+  rc = chx_xc_create((int)codes.size(), codes.data(), spin, out);
+  if (rc == CHX_OK && output) {
+    std::printf("Density functional %s: %zu code(s), family %d, EXX %.2f\n", df_name, codes.size(), (*out)->family,
+                (*out)->exx);
+  }
+  return rc;
+}
+
+void chx_xc_destroy(chx_xc* xc) { delete xc; }
+
+int chx_xc_info(const chx_xc* xc, int* family, double* exx, int* ncodes, int* codes) {
+  return guard([&] {
+    if (family) *family = xc->family;
+    if (exx) *exx = xc->exx;
+    if (ncodes) *ncodes = (int)xc->codes.size();
+    if (codes) std::memcpy(codes, xc->codes.data(), sizeof(int) * xc->codes.size());
+  });
+}
+
+int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order) {
+  return guard([&] {
+    require(xc && g && order, CHX_ERR_ARG, "bad arguments");
+    const std::string o(order);
+    require(o == "ev" || o == "e" || o == "v", CHX_ERR_UNSUPPORTED, "only the orders \"ev\", \"e\" and \"v\" are built");
+    require(g->have_density, CHX_ERR_STATE, "Grid::getDensity must precede Evaluate");
+    require(g->type >= xc->family, CHX_ERR_STATE, "grid type is below the functional family (call setType)");
+    const bool e = o.find('e') != std::string::npos, v = o.find('v') != std::string::npos;
+    run_functional(g, xc, e, v);
+    g->have_eval = e || g->have_eval;
+    g->have_v = v;
+  });
+}
+
+int chx_lebedev_rule(int npoints, double* xyzw) {
+  return guard([&] {
+    require(xyzw != nullptr, CHX_ERR_ARG, "null output");
+    chx_lebedev_rule_impl(npoints, xyzw);
+  });
+}
+
+int chx_selftest(double* max_err) {
+  return guard([&] {
+    DevBuf<double> d;
+    d.alloc(1);
+    hipStream_t s;
+    CHX_HIP(hipStreamCreate(&s));
+    launch_selftest(s, d.p);
+    CHX_HIP(hipMemcpyAsync(max_err, d.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipStreamSynchronize(s));
+    CHX_HIP(hipStreamDestroy(s));
+  });
+}
+
+}  // extern "C"

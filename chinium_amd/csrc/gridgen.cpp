@@ -1,0 +1,401 @@
+// Grid construction and batch planning (host side).
+//
+// Restates Grid::Grid (src/Grid/GridGeneration.cpp:27-162,232-310) for the
+// GPU: atom-centred Lebedev x radial points, Becke partition weights (FP64,
+// on the device), then this build's own batching: spatially compact batches
+// of 64 points (recursive bisection along the longest box edge) instead of
+// the reference's 128-point halves of the generation order. Batch membership
+// changes only the order of summation.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <numeric>
+#include <sstream>
+
+#include "internal.hpp"
+
+namespace chx {
+namespace {
+
+struct LebOrbit { int type; double a, b, v; };
+struct LebRule { int n, first, count; };
+#include "lebedev_table.inc"
+
+const char* kElemNames[] = {"", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si",
+                            "P", "S", "Cl", "Ar", "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni",
+                            "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr"};
+
+// Octahedral orbit of a generator (Lebedev-Laikov types 1-6).
+void expand_orbit(const LebOrbit& o, std::vector<std::array<double, 4>>& out) {
+  double g[3];
+  switch (o.type) {
+    case 1: g[0] = 1; g[1] = 0; g[2] = 0; break;
+    case 2: g[0] = 0; g[1] = g[2] = std::sqrt(0.5); break;
+    case 3: g[0] = g[1] = g[2] = std::sqrt(1.0 / 3.0); break;
+    case 4: g[0] = g[1] = o.a; g[2] = std::sqrt(1.0 - 2.0 * o.a * o.a); break;
+    case 5: g[0] = o.a; g[1] = std::sqrt(1.0 - o.a * o.a); g[2] = 0; break;
+    case 6: g[0] = o.a; g[1] = o.b; g[2] = std::sqrt(1.0 - o.a * o.a - o.b * o.b); break;
+    default: throw Error(CHX_ERR_ARG, "bad Lebedev orbit");
+  }
+  static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  std::vector<std::array<double, 3>> base;
+  for (const auto& p : perm) {
+    std::array<double, 3> c = {g[p[0]], g[p[1]], g[p[2]]};
+    if (std::find(base.begin(), base.end(), c) == base.end()) base.push_back(c);
+  }
+  for (const auto& c : base)
+    for (int s = 0; s < 8; ++s) {
+      const double sx = (s & 1) ? -1.0 : 1.0, sy = (s & 2) ? -1.0 : 1.0, sz = (s & 4) ? -1.0 : 1.0;
+      if ((c[0] == 0 && sx < 0) || (c[1] == 0 && sy < 0) || (c[2] == 0 && sz < 0)) continue;
+      out.push_back({sx * c[0], sy * c[1], sz * c[2], o.v});
+    }
+}
+
+const std::vector<std::array<double, 4>>& lebedev(int n) {
+  static std::vector<std::vector<std::array<double, 4>>> cache(6000);
+  if (n <= 0 || n >= (int)cache.size()) throw Error(CHX_ERR_FILE, "Lebedev rule not available");
+  auto& r = cache[n];
+  if (!r.empty()) return r;
+  for (const LebRule& lr : kLebRules) {
+    if (lr.n != n) continue;
+    for (int k = 0; k < lr.count; ++k) expand_orbit(kLebOrbits[lr.first + k], r);
+    if ((int)r.size() != n) throw Error(CHX_ERR_FILE, "Lebedev expansion size mismatch");
+    return r;
+  }
+  throw Error(CHX_ERR_FILE, "Lebedev rule not available");
+}
+
+struct Radial { std::vector<double> r, w; std::vector<int> nang; };
+
+// .grid file (Grids/README.md:1-11): "ngroups nshells", radial line ("em R" or
+// "de2 alpha xmin xmax"), then "npoints nshells" per group from the nucleus out.
+Radial read_grid_file(const std::string& path) {
+  std::ifstream f(path);
+  if (!f.good()) throw Error(CHX_ERR_FILE, "Missing element grid file in folder!");
+  std::string line;
+  int ngroups = 0, ntot = 0;
+  if (!std::getline(f, line)) throw Error(CHX_ERR_FILE, "Corrupted grid file");
+  { std::stringstream ss(line); ss >> ngroups >> ntot; }
+  if (ngroups <= 0 || ntot <= 1) throw Error(CHX_ERR_FILE, "Corrupted grid file");
+  if (!std::getline(f, line)) throw Error(CHX_ERR_FILE, "Corrupted grid file");
+  std::stringstream ss(line);
+  std::string formula;
+  ss >> formula;
+  Radial R;
+  R.r.resize(ntot);
+  R.w.resize(ntot);
+  if (formula == "de2") {  // GridGeneration.cpp:90-100
+    double alpha, x0, x1;
+    ss >> alpha >> x0 >> x1;
+    const double h = (x1 - x0) / (double)(ntot - 1);
+    for (int i = 0; i < ntot; ++i) {
+      const double xi = h * i + x0;
+      R.r[i] = std::exp(alpha * xi - std::exp(-xi));
+      R.w[i] = std::exp(3 * alpha * xi - 3 * std::exp(-xi)) * (alpha + std::exp(-xi)) * h;
+    }
+  } else if (formula == "em") {  // GridGeneration.cpp:103-113
+    double Rm;
+    ss >> Rm;
+    for (int i = 0; i < ntot; ++i) {
+      const double ii = i + 1;
+      R.r[i] = Rm * std::pow(ii / (ntot + 1 - ii), 2);
+      R.w[i] = 2 * std::pow(Rm, 3) * (ntot + 1) * std::pow(ii, 5) / std::pow(ntot + 1 - ii, 7);
+    }
+  } else {
+    throw Error(CHX_ERR_FILE, "Unrecognized radial formula!");
+  }
+  for (int k = 0; k < ngroups; ++k) {
+    if (!std::getline(f, line)) throw Error(CHX_ERR_FILE, "Corrupted grid file");
+    std::stringstream gs(line);
+    int npts = 0, nsh = 0;
+    gs >> npts >> nsh;
+    for (int s = 0; s < nsh; ++s) R.nang.push_back(npts);
+  }
+  if ((int)R.nang.size() != ntot) throw Error(CHX_ERR_FILE, "Corrupted grid file");
+  return R;
+}
+
+// Recursive bisection into batches of exactly kBatch points (one remainder).
+void bisect(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& z,
+            std::vector<long>& idx, long lo, long hi, std::vector<std::pair<long, long>>& out) {
+  const long n = hi - lo;
+  if (n <= kBatch) {
+    out.push_back({lo, hi});
+    return;
+  }
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (long i = lo; i < hi; ++i) {
+    const double c[3] = {x[idx[i]], y[idx[i]], z[idx[i]]};
+    for (int d = 0; d < 3; ++d) { mn[d] = std::min(mn[d], c[d]); mx[d] = std::max(mx[d], c[d]); }
+  }
+  int ax = 0;
+  for (int d = 1; d < 3; ++d)
+    if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+  const std::vector<double>& key = ax == 0 ? x : (ax == 1 ? y : z);
+  const long nb = (n + kBatch - 1) / kBatch;
+  const long left = kBatch * (nb / 2);
+  std::nth_element(idx.begin() + lo, idx.begin() + lo + left, idx.begin() + hi,
+                   [&](long a, long b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+  bisect(x, y, z, idx, lo, lo + left, out);
+  bisect(x, y, z, idx, lo + left, hi, out);
+}
+
+uint64_t morton3(double fx, double fy, double fz) {
+  auto spread = [](uint64_t v) {
+    uint64_t r = 0;
+    for (int i = 0; i < 21; ++i) r |= ((v >> i) & 1ull) << (3 * i);
+    return r;
+  };
+  auto q = [](double f) {
+    f = std::min(std::max(f, 0.0), 1.0);
+    return (uint64_t)(f * 2097151.0);
+  };
+  return spread(q(fx)) | (spread(q(fy)) << 1) | (spread(q(fz)) << 2);
+}
+
+}  // namespace
+
+const std::vector<std::array<double, 4>>& lebedev_rule(int n) { return lebedev(n); }
+
+// Build points, weights, batches and the internal AO layout.
+void grid_build(chx_grid* g, const std::string& grid_name, const std::string& data_root) {
+  const chx_mol& m = g->mol;
+  const int nat = m.natoms;
+  if (nat <= 0) throw Error(CHX_ERR_ARG, "molecule has no atoms");
+  if (m.nbasis <= 0) throw Error(CHX_ERR_ARG, "molecule has no basis");
+  // ---- atom-centred points in generation order (atom -> shell -> angular point)
+  std::vector<double> px, py, pz, pw;
+  std::vector<int> owner;
+  for (int a = 0; a < nat; ++a) {
+    const int Z = m.Z[a];
+    if (Z < 1 || Z > 36) throw Error(CHX_ERR_ARG, "unsupported element");
+    const Radial R = read_grid_file(data_root + "/Grids/" + grid_name + "/" + kElemNames[Z] + ".grid");
+    const double* c = &m.xyz[3 * a];
+    for (size_t s = 0; s < R.r.size(); ++s) {
+      const auto& L = lebedev(R.nang[s]);
+      for (const auto& l : L) {
+        px.push_back(l[0] * R.r[s] + c[0]);
+        py.push_back(l[1] * R.r[s] + c[1]);
+        pz.push_back(l[2] * R.r[s] + c[2]);
+        pw.push_back(R.w[s] * 4 * M_PI * l[3]);
+        owner.push_back(a);
+      }
+    }
+  }
+  const long G = (long)px.size();
+  // ---- Becke weights on the device (FP64)
+  {
+    std::vector<double> inv(nat * (size_t)nat, 0.0);
+    for (int b = 0; b < nat; ++b)
+      for (int c = 0; c < nat; ++c) {
+        if (b == c) continue;
+        const double dx = m.xyz[3 * b] - m.xyz[3 * c], dy = m.xyz[3 * b + 1] - m.xyz[3 * c + 1],
+                     dz = m.xyz[3 * b + 2] - m.xyz[3 * c + 2];
+        inv[(size_t)b * nat + c] = std::sqrt(dx * dx + dy * dy + dz * dz);
+      }
+    DevBuf<double> dx, dy, dz, dw, dxyz, dinv;
+    DevBuf<int> down;
+    dx.upload(px, g->stream); dy.upload(py, g->stream); dz.upload(pz, g->stream); dw.upload(pw, g->stream);
+    down.upload(owner, g->stream);
+    dxyz.upload(m.xyz, g->stream);
+    dinv.upload(inv, g->stream);
+    launch_becke(g->stream, G, dx.p, dy.p, dz.p, dw.p, down.p, nat, dxyz.p, dinv.p);
+    CHX_HIP(hipMemcpyAsync(pw.data(), dw.p, sizeof(double) * G, hipMemcpyDeviceToHost, g->stream));
+    CHX_HIP(hipStreamSynchronize(g->stream));
+  }
+  // ---- drop exactly-zero weights, batch the rest spatially
+  std::vector<long> keep;
+  keep.reserve(G);
+  for (long i = 0; i < G; ++i)
+    if (pw[i] != 0.0) keep.push_back(i);
+  std::vector<std::pair<long, long>> parts;
+  if (!keep.empty()) bisect(px, py, pz, keep, 0, (long)keep.size(), parts);
+  g->npoints = (long)keep.size();
+  g->nbatches = (int)parts.size();
+  const size_t nslots = (size_t)g->nbatches * kBatch;
+  g->hx.assign(nslots, 0); g->hy.assign(nslots, 0); g->hz.assign(nslots, 0); g->hw.assign(nslots, 0);
+  g->batch_n.assign(g->nbatches, 0);
+  for (int b = 0; b < g->nbatches; ++b) {
+    const long lo = parts[b].first, n = parts[b].second - parts[b].first;
+    g->batch_n[b] = (int)n;
+    for (int k = 0; k < kBatch; ++k) {
+      const long src = keep[lo + (k < n ? k : 0)];
+      const size_t s = (size_t)b * kBatch + k;
+      g->hx[s] = px[src]; g->hy[s] = py[src]; g->hz[s] = pz[src];
+      g->hw[s] = k < n ? pw[src] : 0.0;
+    }
+  }
+  // ---- internal AO order: atoms along a Morton curve, shells in file order
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (int a = 0; a < nat; ++a)
+    for (int d = 0; d < 3; ++d) { mn[d] = std::min(mn[d], m.xyz[3 * a + d]); mx[d] = std::max(mx[d], m.xyz[3 * a + d]); }
+  const double span = std::max({mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2], 1e-9});
+  std::vector<int> aorder(nat);
+  std::iota(aorder.begin(), aorder.end(), 0);
+  std::vector<uint64_t> key(nat);
+  for (int a = 0; a < nat; ++a)
+    key[a] = morton3((m.xyz[3 * a] - mn[0]) / span, (m.xyz[3 * a + 1] - mn[1]) / span, (m.xyz[3 * a + 2] - mn[2]) / span);
+  std::stable_sort(aorder.begin(), aorder.end(), [&](int p, int q) { return key[p] < key[q]; });
+  g->nbasis = m.nbasis;
+  g->npad = ((m.nbasis + kSuper * kTile - 1) / (kSuper * kTile)) * (kSuper * kTile);
+  g->ntiles = g->npad / kTile;
+  g->int2ref.assign(g->npad, -1);
+  g->ref2int.assign(g->nbasis, -1);
+  g->sh_xyz.clear(); g->sh_type.clear(); g->sh_nprim.clear(); g->sh_poff.clear();
+  g->p_exp.clear(); g->p_coef.clear();
+  std::vector<int> sh_int_ao;  // internal AO offset per internal shell
+  int ao = 0;
+  for (int a : aorder)
+    for (const ShellH& s : m.shells) {
+      if (s.center != a) continue;
+      const int n = shell_size(s.type);
+      for (int k = 0; k < n; ++k) {
+        g->int2ref[ao + k] = s.ao_off + k;
+        g->ref2int[s.ao_off + k] = ao + k;
+      }
+      g->sh_xyz.push_back(m.xyz[3 * a]); g->sh_xyz.push_back(m.xyz[3 * a + 1]); g->sh_xyz.push_back(m.xyz[3 * a + 2]);
+      g->sh_type.push_back(s.type);
+      g->sh_nprim.push_back(s.nprim);
+      g->sh_poff.push_back((int)g->p_exp.size());
+      for (int p = 0; p < s.nprim; ++p) {
+        g->p_exp.push_back(m.exps[s.prim_off + p]);
+        g->p_coef.push_back(m.coefs[s.prim_off + p]);
+      }
+      sh_int_ao.push_back(ao);
+      ao += n;
+    }
+  // ---- tile segments
+  g->tile_seg_off.assign(g->ntiles + 1, 0);
+  g->seg.clear();
+  for (int t = 0; t < g->ntiles; ++t) {
+    g->tile_seg_off[t] = (int)g->seg.size();
+    const int lo = t * kTile, hi = lo + kTile;
+    for (size_t s = 0; s < sh_int_ao.size(); ++s) {
+      const int a0 = sh_int_ao[s], a1 = a0 + shell_size(g->sh_type[s]);
+      const int b0 = std::max(a0, lo), b1 = std::min(a1, hi);
+      if (b0 >= b1) continue;
+      g->seg.push_back(make_int4((int)s, b0 - a0, b1 - b0, b0 - lo));
+    }
+  }
+  g->tile_seg_off[g->ntiles] = (int)g->seg.size();
+  // ---- device copies
+  hipStream_t st = g->stream;
+  g->d_sh_xyz.upload(g->sh_xyz, st); g->d_sh_type.upload(g->sh_type, st); g->d_sh_nprim.upload(g->sh_nprim, st);
+  g->d_sh_poff.upload(g->sh_poff, st); g->d_p_exp.upload(g->p_exp, st); g->d_p_coef.upload(g->p_coef, st);
+  g->d_tile_seg_off.upload(g->tile_seg_off, st); g->d_seg.upload(g->seg, st);
+  g->d_int2ref.upload(g->int2ref, st); g->d_ref2int.upload(g->ref2int, st);
+  g->d_x.upload(g->hx, st); g->d_y.upload(g->hy, st); g->d_z.upload(g->hz, st); g->d_w.upload(g->hw, st);
+  g->d_bn.upload(g->batch_n, st);
+  CHX_HIP(hipStreamSynchronize(st));
+  g->ao_ready = false;
+}
+
+// Screening and work plans (Grid::getAO counterpart).
+void grid_plan(chx_grid* g) {
+  hipStream_t st = g->stream;
+  const int nb = g->nbatches, nt = g->ntiles;
+  g->d_alive.alloc((size_t)std::max(1, nb * nt));
+  launch_screen(st, nb, nt, g->d_x.p, g->d_y.p, g->d_z.p, g->d_bn.p, g->view(), g->screen_thr, g->d_alive.p);
+  g->alive.assign((size_t)nb * nt, 0);
+  if (nb) CHX_HIP(hipMemcpyAsync(g->alive.data(), g->d_alive.p, (size_t)nb * nt, hipMemcpyDeviceToHost, st));
+  CHX_HIP(hipStreamSynchronize(st));
+  // real AO count per tile
+  std::vector<int> tile_nao(nt, 0);
+  for (int t = 0; t < nt; ++t)
+    for (int k = 0; k < kTile; ++k) tile_nao[t] += g->int2ref[t * kTile + k] >= 0 ? 1 : 0;
+  g->bt_off.assign(nb + 1, 0);
+  g->bt_list.clear();
+  g->grp_batch.clear(); g->grp_tiles.clear(); g->grp_ntiles.clear();
+  g->grp_off.assign(nb + 1, 0);
+  double flops = 0, tiles = 0;
+  for (int b = 0; b < nb; ++b) {
+    g->bt_off[b] = (int)g->bt_list.size();
+    g->grp_off[b] = (int)g->grp_batch.size();
+    int nao = 0;
+    std::vector<int> lst;
+    for (int t = 0; t < nt; ++t)
+      if (g->alive[(size_t)b * nt + t]) { lst.push_back(t); nao += tile_nao[t]; }
+    g->bt_list.insert(g->bt_list.end(), lst.begin(), lst.end());
+    for (size_t k = 0; k < lst.size(); k += kGroup) {
+      g->grp_batch.push_back(b);
+      const int cnt = (int)std::min<size_t>(kGroup, lst.size() - k);
+      g->grp_ntiles.push_back(cnt);
+      for (int q = 0; q < kGroup; ++q) g->grp_tiles.push_back(q < cnt ? lst[k + q] : 0);
+    }
+    tiles += (double)lst.size();
+    flops += (double)g->batch_n[b] * (double)nao * (double)nao;
+  }
+  g->bt_off[nb] = (int)g->bt_list.size();
+  g->grp_off[nb] = (int)g->grp_batch.size();
+  g->mean_tiles = nb ? tiles / nb : 0;
+  g->flops_build = 2.0 * 2.0 * flops;  // k = 2 contractions (GGA), 2 flops per multiply-add
+  // Vxc work items over super-tile pairs
+  const int nsup = g->npad / (kSuper * kTile);
+  std::vector<std::vector<int>> pairs((size_t)nsup * nsup);
+  std::vector<int> sup;
+  for (int b = 0; b < nb; ++b) {
+    sup.clear();
+    for (int s = 0; s < nsup; ++s) {
+      bool any = false;
+      for (int h = 0; h < kSuper; ++h) any |= g->alive[(size_t)b * nt + s * kSuper + h] != 0;
+      if (any) sup.push_back(s);
+    }
+    for (size_t i = 0; i < sup.size(); ++i)
+      for (size_t j = i; j < sup.size(); ++j) pairs[(size_t)sup[i] * nsup + sup[j]].push_back(b);
+  }
+  g->vwork.clear();
+  g->pair_batches.clear();
+  for (int I = 0; I < nsup; ++I)
+    for (int J = I; J < nsup; ++J) {
+      const auto& L = pairs[(size_t)I * nsup + J];
+      for (size_t k = 0; k < L.size(); k += kVxcChunk) {
+        const int begin = (int)g->pair_batches.size();
+        const size_t e = std::min(L.size(), k + kVxcChunk);
+        g->pair_batches.insert(g->pair_batches.end(), L.begin() + k, L.begin() + e);
+        g->vwork.push_back(make_int4(I, J, begin, (int)g->pair_batches.size()));
+      }
+    }
+  g->d_bt_off.upload(g->bt_off, st); g->d_bt_list.upload(g->bt_list, st);
+  g->d_grp_batch.upload(g->grp_batch, st); g->d_grp_tiles.upload(g->grp_tiles, st);
+  g->d_grp_ntiles.upload(g->grp_ntiles, st); g->d_grp_off.upload(g->grp_off, st);
+  g->d_vwork.upload(g->vwork, st); g->d_pair_batches.upload(g->pair_batches, st);
+  // per-iteration buffers
+  const size_t nslots = (size_t)nb * kBatch;
+  g->d_Dint.alloc((size_t)g->npad * g->npad);
+  g->d_Vint.alloc((size_t)g->npad * g->npad);
+  g->d_Din.alloc((size_t)g->nbasis * g->nbasis);
+  g->d_Vout.alloc((size_t)g->nbasis * g->nbasis);
+  g->d_part.alloc(std::max<size_t>(1, g->grp_batch.size() * 4 * kBatch));
+  g->d_rho.alloc(std::max<size_t>(1, nslots));
+  g->d_coef.alloc(std::max<size_t>(4, 4 * nslots));
+  g->en_blocks = (int)std::max<size_t>(1, (nslots + 255) / 256);
+  g->d_en_part.alloc(2 * (size_t)g->en_blocks);
+  g->d_en.alloc(2);
+  CHX_HIP(hipStreamSynchronize(st));
+  g->ao_ready = true;
+  g->have_density = g->have_eval = g->have_v = false;
+}
+
+}  // namespace chx
+
+int chx_lebedev_rule_impl(int n, double* xyzw) {
+  const auto& r = chx::lebedev_rule(n);
+  for (size_t i = 0; i < r.size(); ++i)
+    for (int k = 0; k < 4; ++k) xyzw[4 * i + k] = r[i][k];
+  return (int)r.size();
+}
+
+chx::BasisView chx_grid::view() const {
+  chx::BasisView v;
+  v.sh_xyz = d_sh_xyz.p;
+  v.sh_type = d_sh_type.p;
+  v.sh_nprim = d_sh_nprim.p;
+  v.sh_poff = d_sh_poff.p;
+  v.p_exp = d_p_exp.p;
+  v.p_coef = d_p_coef.p;
+  v.tile_seg_off = d_tile_seg_off.p;
+  v.seg = d_seg.p;
+  return v;
+}

@@ -1,0 +1,185 @@
+// Internal host-side structures of the MI355X XC library (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "chinium_xc.h"
+
+namespace chx {
+
+// Batch and tile geometry shared by host planning and the kernels.
+constexpr int kBatch = 64;     // quadrature points per batch (MFMA K for Vxc, M for density)
+constexpr int kTile = 32;      // AOs per screening tile
+constexpr int kGroup = 4;      // AO tiles per density work group (128 output columns)
+constexpr int kSuper = 2;      // AO tiles per Vxc super-tile (64 x 64 output block)
+constexpr int kVxcChunk = 32;  // batches per Vxc work item
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+#define CHX_HIP(x) ::chx::hip_check((x), #x)
+
+struct ShellH {
+  int type;    // 0 s, 1 Cartesian p, -1 pure p, -2 pure d, -3 pure f
+  int center;
+  int nprim;
+  int prim_off;
+  int ao_off;  // offset in the reference AO order
+};
+
+int shell_size(int type);
+
+// A device buffer owned by a handle.
+template <class T> struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    CHX_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const std::vector<T>& h, hipStream_t s) {
+    alloc(h.size());
+    if (!h.empty()) CHX_HIP(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void copy_from(const DevBuf& o, hipStream_t s) {
+    alloc(o.n);
+    if (o.n) CHX_HIP(hipMemcpyAsync(p, o.p, o.n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  }
+};
+
+// Kernel-side view of the basis in the internal AO order.
+struct BasisView {
+  const double* sh_xyz;   // 3 per shell
+  const int* sh_type;
+  const int* sh_nprim;
+  const int* sh_poff;
+  const double* p_exp;
+  const double* p_coef;
+  const int* tile_seg_off;  // ntiles + 1
+  const int4* seg;          // (shell, first component, count, local AO offset in tile)
+};
+
+// Launchers (kernels.hip).
+void launch_becke(hipStream_t s, long npts, const double* px, const double* py, const double* pz, double* w,
+                  const int* owner, int natoms, const double* atom_xyz, const double* inv_rab);
+void launch_screen(hipStream_t s, int nbatches, int ntiles, const double* bx, const double* by, const double* bz,
+                   const int* bn, BasisView bv, double thr, unsigned char* alive);
+void launch_permute_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, double* Dint);
+void launch_density_f64(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
+                        const int* bt_off, const int* bt_list, const double* bx, const double* by, const double* bz,
+                        BasisView bv, const double* Dint, int npad, double* part);
+void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* bw,
+                       const int* codes, int ncodes, int family, int want_e, int want_v,
+                       double* rho_out, double* coef, double* en_part, int en_blocks);
+void launch_reduce_en(hipStream_t s, int nparts, const double* en_part, double* en);
+void launch_fill(hipStream_t s, double* p, size_t n, double v);
+void launch_vxc_f64(hipStream_t s, int nwork, const int4* work, const int* pair_batches, const unsigned char* alive,
+                    int ntiles, const double* bx, const double* by, const double* bz, const double* coef, BasisView bv,
+                    double* Vint, int npad);
+void launch_finalize_v(hipStream_t s, int n, int npad, const double* Vint, const int* ref2int, double* V);
+void launch_vrho_coef(hipStream_t s, long nslots, const double* vrho_w, double* coef);
+
+}  // namespace chx
+
+// Opaque handle definitions.
+struct chx_mol {
+  int natoms = 0;
+  std::vector<int> Z;
+  std::vector<double> xyz;
+  std::vector<chx::ShellH> shells;
+  std::vector<double> exps, coefs;
+  int nbasis = 0;
+};
+
+struct chx_xc {
+  std::vector<int> codes;
+  int family = 0;
+  double exx = 0;
+  int spin = 1;
+};
+
+struct chx_grid {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nthreads = 1;
+  int type = 0;
+  int precision = CHX_PREC_FP64;
+  double screen_thr = 1e-14;
+  chx_mol mol;
+
+  // AO layout: internal order groups AOs of spatially sorted atoms.
+  int nbasis = 0, npad = 0, ntiles = 0;
+  std::vector<int> int2ref;  // npad, -1 for padding
+  std::vector<int> ref2int;  // nbasis
+  std::vector<double> sh_xyz;
+  std::vector<int> sh_type, sh_nprim, sh_poff;
+  std::vector<double> p_exp, p_coef;
+  std::vector<int> tile_seg_off;
+  std::vector<int4> seg;
+
+  // Points: nbatches x kBatch slots (padding slots carry w = 0).
+  long npoints = 0;
+  int nbatches = 0;
+  std::vector<int> batch_n;        // real points per batch
+  std::vector<double> hx, hy, hz, hw;  // slot arrays (host copy)
+
+  // Screening products (built by get_ao).
+  bool ao_ready = false;
+  std::vector<unsigned char> alive;  // nbatches x ntiles
+  std::vector<int> bt_off, bt_list;  // alive tiles per batch (CSR)
+  std::vector<int> grp_batch, grp_tiles, grp_ntiles, grp_off;  // density work groups
+  std::vector<int4> vwork;           // Vxc work items (tile I, tile J, begin, end)
+  std::vector<int> pair_batches;
+  double flops_build = 0, mean_tiles = 0;
+
+  // Device mirrors.
+  chx::DevBuf<double> d_sh_xyz, d_p_exp, d_p_coef;
+  chx::DevBuf<int> d_sh_type, d_sh_nprim, d_sh_poff, d_tile_seg_off;
+  chx::DevBuf<int4> d_seg;
+  chx::DevBuf<int> d_int2ref, d_ref2int;
+  chx::DevBuf<double> d_x, d_y, d_z, d_w;
+  chx::DevBuf<int> d_bn;
+  chx::DevBuf<unsigned char> d_alive;
+  chx::DevBuf<int> d_bt_off, d_bt_list, d_grp_batch, d_grp_tiles, d_grp_ntiles, d_grp_off;
+  chx::DevBuf<int4> d_vwork;
+  chx::DevBuf<int> d_pair_batches;
+  chx::DevBuf<int> d_codes;
+
+  // Per-iteration state.
+  chx::DevBuf<double> d_Din, d_Dint, d_part, d_rho, d_coef, d_en_part, d_en, d_Vint, d_Vout;
+  int en_blocks = 0;
+  bool have_density = false, have_eval = false, have_v = false;
+
+  // Optional per-kernel HIP-event timing (stage ids in chx_grid_kernel_times).
+  bool profile = false;
+  struct Pending { int stage; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  double stage_ms[8] = {0};
+  long stage_n[8] = {0};
+  void mark_begin(int stage, hipEvent_t* a);
+  void mark_end(int stage, hipEvent_t a);
+  void collect_times();
+  int eval_family = -1;
+  double last_E = 0, last_N = 0;
+
+  chx::BasisView view() const;
+};

@@ -1,0 +1,633 @@
+// HIP kernels of the XC path for gfx950 (CDNA4). FP64 plain form.
+//
+// Data layout in HBM
+//   points   : batch-major slots, kBatch (64) slots per batch, SoA x/y/z/w;
+//              padding slots repeat the batch's first point with w = 0.
+//   basis    : internal AO order (atoms sorted along a space-filling order),
+//              split into 32-AO tiles; per tile a list of shell segments.
+//   D, V     : npad x npad, row-major, internal order (npad = nbasis rounded
+//              up to 64). D is symmetric; V holds the upper 64x64 blocks.
+//   partials : density per (batch, tile group): 4 x 64 doubles (rho, drho/2).
+//   coef     : per slot (a, c_x, c_y, c_z) with the Vxc integrand
+//              B_nu = a phi_nu + c . grad phi_nu, a = w v_rho / 2, c = 2 w v_sigma grad rho.
+//
+// The two dense contractions, X = Phi.D (density) and V = Phi^T B + B^T Phi
+// (Vxc), run on v_mfma_f64_16x16x4_f64. AO values are recomputed from the
+// shell data inside each kernel (never stored in HBM); the functional and
+// the reductions run on the vector ALUs.
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+#include "xc_functionals.hpp"
+
+namespace chx {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma_f64(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// AO evaluation of one shell at one point (relative coordinates x, y, z).
+// phi_m = R(r) Q_m(x,y,z), R = sum_p c_p exp(-a_p r^2); components in the
+// reference order (src/Grid/AO/{S,CartP,PureP,PureD,PureF}.hpp).
+template <bool GRAD>
+__device__ __forceinline__ int shell_eval(const BasisView& bv, int sh, double px, double py, double pz,
+                                          double* v, double* gx, double* gy, double* gz) {
+  const double x = px - bv.sh_xyz[3 * sh], y = py - bv.sh_xyz[3 * sh + 1], z = pz - bv.sh_xyz[3 * sh + 2];
+  const double r2 = x * x + y * y + z * z;
+  const int np = bv.sh_nprim[sh], po = bv.sh_poff[sh];
+  double A0 = 0, A1 = 0;
+  for (int p = 0; p < np; ++p) {
+    const double a = bv.p_exp[po + p];
+    const double t = bv.p_coef[po + p] * ::exp(-a * r2);
+    A0 += t;
+    if (GRAD) A1 += a * t;
+  }
+  const double dAx = -2.0 * x * A1, dAy = -2.0 * y * A1, dAz = -2.0 * z * A1;
+  const int type = bv.sh_type[sh];
+  // q and grad q per component
+  double q[7], qx[7], qy[7], qz[7];
+  int n;
+  const double r3 = 1.7320508075688772, r6 = 2.449489742783178, r10 = 3.1622776601683795, r15 = 3.872983346207417;
+  switch (type) {
+    case 0:
+      n = 1; q[0] = 1; qx[0] = qy[0] = qz[0] = 0;
+      break;
+    case 1:
+      n = 3;
+      q[0] = x; qx[0] = 1; qy[0] = 0; qz[0] = 0;
+      q[1] = y; qx[1] = 0; qy[1] = 1; qz[1] = 0;
+      q[2] = z; qx[2] = 0; qy[2] = 0; qz[2] = 1;
+      break;
+    case -1:
+      n = 3;
+      q[0] = y; qx[0] = 0; qy[0] = 1; qz[0] = 0;
+      q[1] = z; qx[1] = 0; qy[1] = 0; qz[1] = 1;
+      q[2] = x; qx[2] = 1; qy[2] = 0; qz[2] = 0;
+      break;
+    case -2:
+      n = 5;
+      q[0] = r3 * x * y;                  qx[0] = r3 * y; qy[0] = r3 * x; qz[0] = 0;
+      q[1] = r3 * y * z;                  qx[1] = 0;      qy[1] = r3 * z; qz[1] = r3 * y;
+      q[2] = z * z - 0.5 * (x * x + y * y); qx[2] = -x;   qy[2] = -y;     qz[2] = 2 * z;
+      q[3] = r3 * x * z;                  qx[3] = r3 * z; qy[3] = 0;      qz[3] = r3 * x;
+      q[4] = 0.5 * r3 * (x * x - y * y);  qx[4] = r3 * x; qy[4] = -r3 * y; qz[4] = 0;
+      break;
+    default: {  // -3, pure f
+      n = 7;
+      const double xx = x * x, yy = y * y, zz = z * z;
+      const double c10 = 0.25 * r10, c6 = 0.25 * r6;
+      q[0] = c10 * y * (3 * xx - yy);     qx[0] = c10 * 6 * x * y;       qy[0] = c10 * 3 * (xx - yy);        qz[0] = 0;
+      q[1] = r15 * x * y * z;             qx[1] = r15 * y * z;           qy[1] = r15 * x * z;                qz[1] = r15 * x * y;
+      q[2] = c6 * y * (4 * zz - xx - yy); qx[2] = -c6 * 2 * x * y;       qy[2] = c6 * (4 * zz - xx - 3 * yy); qz[2] = c6 * 8 * y * z;
+      q[3] = z * (zz - 1.5 * (xx + yy));  qx[3] = -3 * x * z;            qy[3] = -3 * y * z;                 qz[3] = 3 * zz - 1.5 * (xx + yy);
+      q[4] = c6 * x * (4 * zz - xx - yy); qx[4] = c6 * (4 * zz - 3 * xx - yy); qy[4] = -c6 * 2 * x * y;     qz[4] = c6 * 8 * x * z;
+      q[5] = 0.5 * r15 * (xx - yy) * z;   qx[5] = r15 * x * z;           qy[5] = -r15 * y * z;               qz[5] = 0.5 * r15 * (xx - yy);
+      q[6] = c10 * x * (xx - 3 * yy);     qx[6] = c10 * 3 * (xx - yy);   qy[6] = -c10 * 6 * x * y;           qz[6] = 0;
+      break;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    if (m < n) {
+      v[m] = A0 * q[m];
+      if (GRAD) {
+        gx[m] = dAx * q[m] + A0 * qx[m];
+        gy[m] = dAy * q[m] + A0 * qy[m];
+        gz[m] = dAz * q[m] + A0 * qz[m];
+      }
+    }
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Becke partition (GridGeneration.cpp:136-153): w *= P_owner / sum_B P_B,
+// P_B = prod_{C != B} s(mu_BC), s = (1 - p(p(p(mu))))/2, p(m) = 1.5m - 0.5m^3.
+__global__ __launch_bounds__(256) void k_becke(long npts, const double* __restrict__ px, const double* __restrict__ py,
+                                               const double* __restrict__ pz, double* __restrict__ w,
+                                               const int* __restrict__ owner, int natoms,
+                                               const double* __restrict__ axyz, const double* __restrict__ inv_rab) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  const double x = px[i], y = py[i], z = pz[i];
+  const int me = owner[i];
+  double total = 0, mine = 0;
+  for (int b = 0; b < natoms; ++b) {
+    const double dxb = x - axyz[3 * b], dyb = y - axyz[3 * b + 1], dzb = z - axyz[3 * b + 2];
+    const double rb = ::sqrt(dxb * dxb + dyb * dyb + dzb * dzb);
+    double pb = 1.0;
+    for (int c = 0; c < natoms && pb != 0.0; ++c) {
+      if (c == b) continue;
+      const double dxc = x - axyz[3 * c], dyc = y - axyz[3 * c + 1], dzc = z - axyz[3 * c + 2];
+      const double rc = ::sqrt(dxc * dxc + dyc * dyc + dzc * dzc);
+      double m = (rb - rc) / inv_rab[b * natoms + c];  // inv_rab holds R_BC (division as in s_p_mu)
+      m = 1.5 * m - 0.5 * m * m * m;
+      m = 1.5 * m - 0.5 * m * m * m;
+      m = 1.5 * m - 0.5 * m * m * m;
+      pb *= 0.5 * (1.0 - m);
+    }
+    total += pb;
+    if (b == me) mine = pb;
+  }
+  w[i] *= mine / total;
+}
+
+void launch_becke(hipStream_t s, long npts, const double* px, const double* py, const double* pz, double* w,
+                  const int* owner, int natoms, const double* atom_xyz, const double* inv_rab) {
+  if (npts == 0) return;
+  const int blocks = (int)((npts + 255) / 256);
+  hipLaunchKernelGGL(k_becke, dim3(blocks), dim3(256), 0, s, npts, px, py, pz, w, owner, natoms, atom_xyz, inv_rab);
+  CHX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// AO screening: one wavefront per batch, one lane per slot. A tile is alive
+// when some AO in it has |phi| or |grad phi| above thr at some real point.
+__global__ __launch_bounds__(64) void k_screen(int ntiles, const double* __restrict__ bx, const double* __restrict__ by,
+                                               const double* __restrict__ bz, const int* __restrict__ bn, BasisView bv,
+                                               double thr, unsigned char* __restrict__ alive) {
+  const int b = blockIdx.x;
+  const int g = threadIdx.x;
+  const bool real = g < bn[b];
+  const double x = bx[b * kBatch + g], y = by[b * kBatch + g], z = bz[b * kBatch + g];
+  for (int t = 0; t < ntiles; ++t) {
+    bool any = false;
+    for (int s = bv.tile_seg_off[t]; s < bv.tile_seg_off[t + 1] && !any; ++s) {
+      const int4 sg = bv.seg[s];
+      double v[7], gx[7], gy[7], gz[7];
+      shell_eval<true>(bv, sg.x, x, y, z, v, gx, gy, gz);
+      double mx = 0;
+      for (int m = 0; m < 7; ++m) {
+        if (m >= sg.y && m < sg.y + sg.z) {
+          mx = fmax(mx, fabs(v[m]));
+          mx = fmax(mx, fmax(fabs(gx[m]), fmax(fabs(gy[m]), fabs(gz[m]))));
+        }
+      }
+      any = __ballot(real && mx > thr) != 0;
+    }
+    if (g == 0) alive[(size_t)b * ntiles + t] = any ? 1 : 0;
+  }
+}
+
+void launch_screen(hipStream_t s, int nbatches, int ntiles, const double* bx, const double* by, const double* bz,
+                   const int* bn, BasisView bv, double thr, unsigned char* alive) {
+  if (nbatches == 0) return;
+  hipLaunchKernelGGL(k_screen, dim3(nbatches), dim3(64), 0, s, ntiles, bx, by, bz, bn, bv, thr, alive);
+  CHX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// D (reference order, column-major n x n) -> internal order, npad x npad, zero padded.
+__global__ void k_permute_d(int n, int npad, const double* __restrict__ D, const int* __restrict__ int2ref,
+                            double* __restrict__ Dint) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)npad * npad) return;
+  const int r = (int)(idx / npad), c = (int)(idx % npad);
+  const int rr = int2ref[r], cc = int2ref[c];
+  Dint[idx] = (rr >= 0 && cc >= 0) ? D[(size_t)cc * n + rr] : 0.0;
+}
+
+void launch_permute_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, double* Dint) {
+  const long tot = (long)npad * npad;
+  hipLaunchKernelGGL(k_permute_d, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, npad, D, int2ref, Dint);
+  CHX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Density. One block (4 waves) per work group = (batch b, up to 4 alive AO
+// tiles j as output columns). X[g, nu] = sum_{alive tiles i} Phi_i[g, :] D[i, nu]
+// accumulates on MFMA over K = the batch's alive tiles, then
+//   rho[g] += sum_nu X phi_nu,  drho[g]/2 += sum_nu X grad phi_nu.
+// Wave w owns rows g = 16w..16w+15 and all 128 output columns.
+struct DensSmem {
+  union {
+    struct {
+      double A[kTile][kBatch];           // Phi_i, [mu][g]
+      double B[kTile][kGroup * kTile];   // D block, [mu][nu]
+    } k;
+    double E[4][kBatch][kTile + 1];      // phi_j and grad phi_j, [c][g][nu]
+  };
+  double px[kBatch], py[kBatch], pz[kBatch];
+};
+
+__global__ __launch_bounds__(256) void k_density_f64(const int* __restrict__ grp_batch, const int* __restrict__ grp_tiles,
+                                                     const int* __restrict__ grp_ntiles, const int* __restrict__ bt_off,
+                                                     const int* __restrict__ bt_list, const double* __restrict__ bx,
+                                                     const double* __restrict__ by, const double* __restrict__ bz,
+                                                     BasisView bv, const double* __restrict__ Dint, int npad,
+                                                     double* __restrict__ part) {
+  __shared__ DensSmem sm;
+  const int grp = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = grp_batch[grp];
+  const int nt = grp_ntiles[grp];
+  int jt[kGroup];
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) jt[q] = grp_tiles[grp * kGroup + q];
+
+  if (tid < kBatch) {
+    sm.px[tid] = bx[b * kBatch + tid];
+    sm.py[tid] = by[b * kBatch + tid];
+    sm.pz[tid] = bz[b * kBatch + tid];
+  }
+  dbl4 acc[2 * kGroup];
+#pragma unroll
+  for (int c = 0; c < 2 * kGroup; ++c) acc[c] = dbl4{0, 0, 0, 0};
+  __syncthreads();
+
+  const int lrow = lane & 15, lk = lane >> 4;
+  for (int it = bt_off[b]; it < bt_off[b + 1]; ++it) {
+    const int ti = bt_list[it];
+    // zero the Phi tile, then fill the AOs this tile holds
+    for (int e = tid; e < kTile * kBatch; e += 256) (&sm.k.A[0][0])[e] = 0.0;
+    __syncthreads();
+    const int s0 = bv.tile_seg_off[ti], s1 = bv.tile_seg_off[ti + 1];
+    for (int item = tid; item < (s1 - s0) * kBatch; item += 256) {
+      const int g = item % kBatch;
+      const int4 sg = bv.seg[s0 + item / kBatch];
+      double v[7], gx[7], gy[7], gz[7];
+      shell_eval<false>(bv, sg.x, sm.px[g], sm.py[g], sm.pz[g], v, gx, gy, gz);
+#pragma unroll
+      for (int m = 0; m < 7; ++m)
+        if (m < sg.z) sm.k.A[sg.w + m][g] = v[sg.y + m];
+    }
+    // D block rows of tile i, columns of the group's tiles
+    for (int e = tid; e < kTile * kGroup * kTile; e += 256) {
+      const int r = e / (kGroup * kTile), c = e % (kGroup * kTile);
+      const int q = c / kTile;
+      sm.k.B[r][c] = (q < nt) ? Dint[(size_t)(ti * kTile + r) * npad + jt[q] * kTile + (c % kTile)] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kTile / 4; ++ks) {
+      const double a = sm.k.A[4 * ks + lk][16 * wave + lrow];
+#pragma unroll
+      for (int cb = 0; cb < 2 * kGroup; ++cb) {
+        if (cb < 2 * nt) {
+          const double bb = sm.k.B[4 * ks + lk][16 * cb + lrow];
+          acc[cb] = mfma_f64(a, bb, acc[cb]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: contract X with phi_j and grad phi_j, tile by tile.
+  double s[4][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s[c][r] = 0.0;
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) {
+    if (q < nt) {
+      for (int e = tid; e < 4 * kBatch * (kTile + 1); e += 256) (&sm.E[0][0][0])[e] = 0.0;
+      __syncthreads();
+      const int tj = jt[q];
+      const int s0 = bv.tile_seg_off[tj], s1 = bv.tile_seg_off[tj + 1];
+      for (int item = tid; item < (s1 - s0) * kBatch; item += 256) {
+        const int g = item % kBatch;
+        const int4 sg = bv.seg[s0 + item / kBatch];
+        double v[7], gx[7], gy[7], gz[7];
+        shell_eval<true>(bv, sg.x, sm.px[g], sm.py[g], sm.pz[g], v, gx, gy, gz);
+#pragma unroll
+        for (int m = 0; m < 7; ++m)
+          if (m < sg.z) {
+            sm.E[0][g][sg.w + m] = v[sg.y + m];
+            sm.E[1][g][sg.w + m] = gx[sg.y + m];
+            sm.E[2][g][sg.w + m] = gy[sg.y + m];
+            sm.E[3][g][sg.w + m] = gz[sg.y + m];
+          }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const dbl4 x = acc[2 * q + h];
+        const int col = 16 * h + lrow;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int g = 16 * wave + 4 * r + lk;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) s[c][r] += x[r] * sm.E[c][g][col];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // Reduce over the 16 lanes that share rows, then lane k writes (c = k/4, r = k%4).
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double v = s[c][r];
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 4, 64);
+      v += __shfl_xor(v, 8, 64);
+      s[c][r] = v;
+    }
+  double out = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (lrow == 4 * c + r) out = s[c][r];
+  const int c = lrow >> 2, r = lrow & 3;
+  part[((size_t)grp * 4 + c) * kBatch + 16 * wave + 4 * r + lk] = out;
+}
+
+void launch_density_f64(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
+                        const int* bt_off, const int* bt_list, const double* bx, const double* by, const double* bz,
+                        BasisView bv, const double* Dint, int npad, double* part) {
+  if (ngroups == 0) return;
+  hipLaunchKernelGGL(k_density_f64, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_ntiles, bt_off, bt_list,
+                     bx, by, bz, bv, Dint, npad, part);
+  CHX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Functional: one thread per slot. Sums the density partials of the batch's
+// work groups, evaluates the functional and writes the Vxc integrand
+// coefficients; block partial sums of E = sum w rho eps and N = sum w rho.
+__global__ __launch_bounds__(256) void k_functional(int nslots, const int* __restrict__ grp_off,
+                                                    const double* __restrict__ part, const double* __restrict__ bw,
+                                                    const int* __restrict__ codes, int ncodes, int family, int want_e,
+                                                    int want_v, double* __restrict__ rho_out, double* __restrict__ coef,
+                                                    double* __restrict__ en_part) {
+  __shared__ double red[2][256];
+  const int slot = blockIdx.x * 256 + threadIdx.x;
+  double e_acc = 0, n_acc = 0;
+  if (slot < nslots) {
+    const int b = slot / kBatch, g = slot % kBatch;
+    double rho = 0, gx = 0, gy = 0, gz = 0;
+    for (int k = grp_off[b]; k < grp_off[b + 1]; ++k) {
+      const double* p = part + (size_t)k * 4 * kBatch;
+      rho += p[g];
+      gx += p[kBatch + g];
+      gy += p[2 * kBatch + g];
+      gz += p[3 * kBatch + g];
+    }
+    gx *= 2; gy *= 2; gz *= 2;
+    const double w = bw[slot];
+    double sigma = family >= 1 ? gx * gx + gy * gy + gz * gz : 0.0;
+    double eps, vr, vs;
+    eval_codes(codes, ncodes, rho, sigma, &eps, &vr, &vs);
+    rho_out[slot] = rho;
+    if (want_v) {
+      coef[4 * (size_t)slot + 0] = 0.5 * w * vr;
+      const double f = family >= 1 ? 2.0 * w * vs : 0.0;
+      coef[4 * (size_t)slot + 1] = f * gx;
+      coef[4 * (size_t)slot + 2] = f * gy;
+      coef[4 * (size_t)slot + 3] = f * gz;
+    }
+    if (want_e) e_acc = w * rho * eps;
+    n_acc = w * rho;
+  }
+  red[0][threadIdx.x] = e_acc;
+  red[1][threadIdx.x] = n_acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + st];
+      red[1][threadIdx.x] += red[1][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    en_part[2 * blockIdx.x] = red[0][0];
+    en_part[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* bw,
+                       const int* codes, int ncodes, int family, int want_e, int want_v, double* rho_out, double* coef,
+                       double* en_part, int en_blocks) {
+  const int nslots = nbatches * kBatch;
+  hipLaunchKernelGGL(k_functional, dim3(en_blocks), dim3(256), 0, s, nslots, grp_off, part, bw, codes, ncodes, family,
+                     want_e, want_v, rho_out, coef, en_part);
+  CHX_HIP(hipGetLastError());
+}
+
+// Deterministic final sum of the block partials (one block).
+__global__ __launch_bounds__(256) void k_reduce_en(int nparts, const double* __restrict__ en_part, double* __restrict__ en) {
+  __shared__ double red[2][256];
+  double e = 0, n = 0;
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    e += en_part[2 * i];
+    n += en_part[2 * i + 1];
+  }
+  red[0][threadIdx.x] = e;
+  red[1][threadIdx.x] = n;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + st];
+      red[1][threadIdx.x] += red[1][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    en[0] = red[0][0];
+    en[1] = red[1][0];
+  }
+}
+
+void launch_reduce_en(hipStream_t s, int nparts, const double* en_part, double* en) {
+  hipLaunchKernelGGL(k_reduce_en, dim3(1), dim3(256), 0, s, nparts, en_part, en);
+  CHX_HIP(hipGetLastError());
+}
+
+__global__ void k_fill(double* p, size_t n, double v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill(hipStream_t s, double* p, size_t n, double v) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  CHX_HIP(hipGetLastError());
+}
+
+// v_rho supplied by the caller (SAP guess): coefficients a = w v_rho / 2, c = 0.
+__global__ void k_vrho_coef(long nslots, const double* __restrict__ vrho_w, double* __restrict__ coef) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  coef[4 * i] = 0.5 * vrho_w[i];
+  coef[4 * i + 1] = 0;
+  coef[4 * i + 2] = 0;
+  coef[4 * i + 3] = 0;
+}
+
+void launch_vrho_coef(hipStream_t s, long nslots, const double* vrho_w, double* coef) {
+  hipLaunchKernelGGL(k_vrho_coef, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, nslots, vrho_w, coef);
+  CHX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Vxc. One block per work item (super-tiles I <= J of 64 AOs, range of
+// batches where both are alive). V_IJ += sum_g Phi_I^T B_J + B_I^T Phi_J on
+// MFMA, K = points, 32 points per LDS stage. Wave w owns output rows
+// 16w..16w+15 and all 64 columns. The block result is added to V with
+// hardware FP64 atomics once per work item.
+constexpr int kVxcK = 32;
+constexpr int kSupAO = kSuper * kTile;  // 64
+
+struct VxcSmem {
+  double PI[kVxcK][kSupAO + 1];
+  double BI[kVxcK][kSupAO + 1];
+  double PJ[kVxcK][kSupAO + 1];
+  double BJ[kVxcK][kSupAO + 1];
+  double px[kVxcK], py[kVxcK], pz[kVxcK], ca[kVxcK], cx[kVxcK], cy[kVxcK], cz[kVxcK];
+};
+
+__device__ __forceinline__ void vxc_fill_super(VxcSmem& sm, double (*P)[kSupAO + 1], double (*B)[kSupAO + 1],
+                                               int sup, const unsigned char* alive_b, const BasisView& bv, int tid) {
+  for (int half = 0; half < kSuper; ++half) {
+    const int t = sup * kSuper + half;
+    if (!alive_b[t]) continue;
+    const int s0 = bv.tile_seg_off[t], s1 = bv.tile_seg_off[t + 1];
+    for (int item = tid; item < (s1 - s0) * kVxcK; item += 256) {
+      const int g = item % kVxcK;
+      const int4 sg = bv.seg[s0 + item / kVxcK];
+      double v[7], gx[7], gy[7], gz[7];
+      shell_eval<true>(bv, sg.x, sm.px[g], sm.py[g], sm.pz[g], v, gx, gy, gz);
+      const double a = sm.ca[g], cx = sm.cx[g], cy = sm.cy[g], cz = sm.cz[g];
+#pragma unroll
+      for (int m = 0; m < 7; ++m)
+        if (m < sg.z) {
+          const int col = half * kTile + sg.w + m;
+          const double ph = v[sg.y + m];
+          P[g][col] = ph;
+          B[g][col] = a * ph + cx * gx[sg.y + m] + cy * gy[sg.y + m] + cz * gz[sg.y + m];
+        }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_vxc_f64(const int4* __restrict__ work, const int* __restrict__ pair_batches,
+                                                 const unsigned char* __restrict__ alive, int ntiles,
+                                                 const double* __restrict__ bx, const double* __restrict__ by,
+                                                 const double* __restrict__ bz, const double* __restrict__ coef,
+                                                 BasisView bv, double* __restrict__ Vint, int npad) {
+  __shared__ VxcSmem sm;
+  const int4 wk = work[blockIdx.x];
+  const int I = wk.x, J = wk.y;
+  const bool diag = (I == J);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lrow = lane & 15, lk = lane >> 4;
+  dbl4 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c] = dbl4{0, 0, 0, 0};
+
+  for (int it = wk.z; it < wk.w; ++it) {
+    const int b = pair_batches[it];
+    const unsigned char* ab = alive + (size_t)b * ntiles;
+    for (int h = 0; h < kBatch / kVxcK; ++h) {
+      // zero the tiles, load the points and their coefficients
+      for (int e = tid; e < kVxcK * (kSupAO + 1); e += 256) {
+        (&sm.PI[0][0])[e] = 0.0;
+        (&sm.BI[0][0])[e] = 0.0;
+        if (!diag) {
+          (&sm.PJ[0][0])[e] = 0.0;
+          (&sm.BJ[0][0])[e] = 0.0;
+        }
+      }
+      if (tid < kVxcK) {
+        const size_t slot = (size_t)b * kBatch + h * kVxcK + tid;
+        sm.px[tid] = bx[slot];
+        sm.py[tid] = by[slot];
+        sm.pz[tid] = bz[slot];
+        sm.ca[tid] = coef[4 * slot];
+        sm.cx[tid] = coef[4 * slot + 1];
+        sm.cy[tid] = coef[4 * slot + 2];
+        sm.cz[tid] = coef[4 * slot + 3];
+      }
+      __syncthreads();
+      vxc_fill_super(sm, sm.PI, sm.BI, I, ab, bv, tid);
+      if (!diag) vxc_fill_super(sm, sm.PJ, sm.BJ, J, ab, bv, tid);
+      __syncthreads();
+      double (*PJ)[kSupAO + 1] = diag ? sm.PI : sm.PJ;
+      double (*BJ)[kSupAO + 1] = diag ? sm.BI : sm.BJ;
+#pragma unroll
+      for (int ks = 0; ks < kVxcK / 4; ++ks) {
+        const int g = 4 * ks + lk;
+        const double a1 = sm.PI[g][16 * wave + lrow];
+        const double a2 = sm.BI[g][16 * wave + lrow];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          acc[cb] = mfma_f64(a1, BJ[g][16 * cb + lrow], acc[cb]);
+          acc[cb] = mfma_f64(a2, PJ[g][16 * cb + lrow], acc[cb]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // C layout: acc[cb][r] = V[row = 16w + 4r + lk][col = 16 cb + lrow]
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int mu = I * kSupAO + 16 * wave + 4 * r + lk;
+      const int nu = J * kSupAO + 16 * cb + lrow;
+      unsafeAtomicAdd(&Vint[(size_t)mu * npad + nu], acc[cb][r]);
+    }
+}
+
+void launch_vxc_f64(hipStream_t s, int nwork, const int4* work, const int* pair_batches, const unsigned char* alive,
+                    int ntiles, const double* bx, const double* by, const double* bz, const double* coef, BasisView bv,
+                    double* Vint, int npad) {
+  if (nwork == 0) return;
+  hipLaunchKernelGGL(k_vxc_f64, dim3(nwork), dim3(256), 0, s, work, pair_batches, alive, ntiles, bx, by, bz, coef, bv,
+                     Vint, npad);
+  CHX_HIP(hipGetLastError());
+}
+
+// Internal upper 64x64 blocks -> full symmetric V in reference order, column-major.
+__global__ void k_finalize_v(int n, int npad, const double* __restrict__ Vint, const int* __restrict__ ref2int,
+                             double* __restrict__ V) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int col = (int)(idx / n), row = (int)(idx % n);
+  const int mu = ref2int[row], nu = ref2int[col];
+  const double v = (mu / kSupAO <= nu / kSupAO) ? Vint[(size_t)mu * npad + nu] : Vint[(size_t)nu * npad + mu];
+  V[idx] = v;
+}
+
+void launch_finalize_v(hipStream_t s, int n, int npad, const double* Vint, const int* ref2int, double* V) {
+  const long tot = (long)n * n;
+  hipLaunchKernelGGL(k_finalize_v, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, npad, Vint, ref2int, V);
+  CHX_HIP(hipGetLastError());
+}
+
+}  // namespace chx
+
+namespace chx {
+// Self-test of the FP64 MFMA operand/result layout the kernels assume:
+// A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, C[4r + l/16][l%16] in reg r.
+__global__ void k_selftest(double* out) {
+  __shared__ double A[16][4], B[4][16];
+  const int l = threadIdx.x;
+  A[l % 16][l / 16] = 1.0 + 0.25 * l;
+  B[l / 16][l % 16] = 0.5 - 0.125 * l;
+  __syncthreads();
+  dbl4 c = {0, 0, 0, 0};
+  c = mfma_f64(A[l % 16][l / 16], B[l / 16][l % 16], c);
+  double err = 0;
+  for (int r = 0; r < 4; ++r) {
+    const int i = 4 * r + l / 16, j = l % 16;
+    double ref = 0;
+    for (int k = 0; k < 4; ++k) ref += A[i][k] * B[k][j];
+    err = fmax(err, fabs(c[r] - ref));
+  }
+  for (int o = 32; o > 0; o >>= 1) err = fmax(err, __shfl_xor(err, o, 64));
+  if (l == 0) out[0] = err;
+}
+
+void launch_selftest(hipStream_t s, double* out) {
+  hipLaunchKernelGGL(k_selftest, dim3(1), dim3(64), 0, s, out);
+  CHX_HIP(hipGetLastError());
+}
+}  // namespace chx

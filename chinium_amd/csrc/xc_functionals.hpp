@@ -1,0 +1,185 @@
+// Device-side exchange-correlation functionals, spin-unpolarised, FP64.
+//
+// Replaces the libxc calls of src/ExchangeCorrelation.cpp:86-163 (summed over
+// the functional codes of a .df file, :202-217). Each functional is coded as
+// its energy density e = rho*eps(rho, sigma, tau); v_rho, v_sigma and v_tau
+// are exact derivatives obtained by forward-mode dual numbers, so there is no
+// hand-differentiated code to drift from the energy.
+//
+// libxc conventions: eps per particle; sigma = |grad rho|^2;
+// tau = 1/2 sum_i |grad psi_i|^2. Below the density threshold a code
+// contributes nothing; sigma is floored at thr^(8/3).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace chx {
+
+struct D3 {
+  double v, d0, d1, d2;
+};
+__device__ __forceinline__ D3 dconst(double x) { return {x, 0.0, 0.0, 0.0}; }
+__device__ __forceinline__ D3 dchain(const D3& a, double f, double df) { return {f, df * a.d0, df * a.d1, df * a.d2}; }
+__device__ __forceinline__ D3 operator+(const D3& a, const D3& b) { return {a.v + b.v, a.d0 + b.d0, a.d1 + b.d1, a.d2 + b.d2}; }
+__device__ __forceinline__ D3 operator-(const D3& a, const D3& b) { return {a.v - b.v, a.d0 - b.d0, a.d1 - b.d1, a.d2 - b.d2}; }
+__device__ __forceinline__ D3 operator-(const D3& a) { return {-a.v, -a.d0, -a.d1, -a.d2}; }
+__device__ __forceinline__ D3 operator*(const D3& a, const D3& b) {
+  return {a.v * b.v, a.d0 * b.v + a.v * b.d0, a.d1 * b.v + a.v * b.d1, a.d2 * b.v + a.v * b.d2};
+}
+__device__ __forceinline__ D3 operator/(const D3& a, const D3& b) {
+  const double r = 1.0 / b.v;
+  const double q = a.v * r;
+  return {q, (a.d0 - q * b.d0) * r, (a.d1 - q * b.d1) * r, (a.d2 - q * b.d2) * r};
+}
+__device__ __forceinline__ D3 operator+(const D3& a, double b) { return {a.v + b, a.d0, a.d1, a.d2}; }
+__device__ __forceinline__ D3 operator+(double a, const D3& b) { return {a + b.v, b.d0, b.d1, b.d2}; }
+__device__ __forceinline__ D3 operator-(const D3& a, double b) { return {a.v - b, a.d0, a.d1, a.d2}; }
+__device__ __forceinline__ D3 operator-(double a, const D3& b) { return {a - b.v, -b.d0, -b.d1, -b.d2}; }
+__device__ __forceinline__ D3 operator*(const D3& a, double b) { return {a.v * b, a.d0 * b, a.d1 * b, a.d2 * b}; }
+__device__ __forceinline__ D3 operator*(double a, const D3& b) { return {a * b.v, a * b.d0, a * b.d1, a * b.d2}; }
+__device__ __forceinline__ D3 operator/(const D3& a, double b) { return a * (1.0 / b); }
+__device__ __forceinline__ D3 operator/(double a, const D3& b) { return dconst(a) / b; }
+__device__ __forceinline__ D3 dexp(const D3& a) { const double e = ::exp(a.v); return dchain(a, e, e); }
+__device__ __forceinline__ D3 dlog(const D3& a) { return dchain(a, ::log(a.v), 1.0 / a.v); }
+__device__ __forceinline__ D3 dsqrt(const D3& a) { const double s = ::sqrt(a.v); return dchain(a, s, 0.5 / s); }
+__device__ __forceinline__ D3 dpow(const D3& a, double p) { const double f = ::pow(a.v, p); return dchain(a, f, p * f / a.v); }
+__device__ __forceinline__ D3 datan(const D3& a) { return dchain(a, ::atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+__device__ __forceinline__ D3 dasinh(const D3& a) { return dchain(a, ::asinh(a.v), 1.0 / ::sqrt(1.0 + a.v * a.v)); }
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Slater exchange (Dirac): e = -(3/4)(3/pi)^(1/3) rho^(4/3).
+__device__ __forceinline__ D3 e_lda_x(const D3& rho, const D3& rho13) {
+  const double cx = -0.7385587663820224;  // -(3/4)(3/pi)^(1/3)
+  return cx * rho * rho13;
+}
+
+// Wigner-Seitz radius from rho^(1/3): rs = (3/(4 pi))^(1/3) / rho^(1/3).
+__device__ __forceinline__ D3 rs_from_rho13(const D3& rho13) { return 0.6203504908994000 / rho13; }
+
+// VWN paramagnetic interpolation formula (Vosko-Wilk-Nusair 1980, eq. 4.4), Hartree.
+__device__ __forceinline__ D3 vwn_eps(const D3& rs, double A, double b, double c, double x0) {
+  const D3 x = dsqrt(rs);
+  const D3 X = x * x + b * x + c;
+  const double Q = ::sqrt(4.0 * c - b * b);
+  const double X0 = x0 * x0 + b * x0 + c;
+  const D3 at = datan(Q / (2.0 * x + b));
+  const D3 xm = x - x0;
+  return A * (dlog(x * x / X) + (2.0 * b / Q) * at - (b * x0 / X0) * (dlog(xm * xm / X) + (2.0 * (b + 2.0 * x0) / Q) * at));
+}
+__device__ __forceinline__ D3 vwn5_eps(const D3& rs) { return vwn_eps(rs, 0.0310907, 3.72744, 12.9352, -0.10498); }
+__device__ __forceinline__ D3 vwn_rpa_eps(const D3& rs) { return vwn_eps(rs, 0.0310907, 13.0720, 42.7198, -0.409286); }
+
+// PW92 paramagnetic correlation (Perdew-Wang 1992, eq. 10), libxc PW_MOD parameters.
+__device__ __forceinline__ D3 pw92_eps(const D3& rs) {
+  const double A = 0.0310907, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+  const D3 srs = dsqrt(rs);
+  const D3 den = (2.0 * A) * (srs * (b1 + b3 * rs) + rs * (b2 + b4 * rs));
+  return (-2.0 * A) * (1.0 + a1 * rs) * dlog(1.0 + 1.0 / den);
+}
+
+// PBE exchange: e = e_x^LDA * (1 + kappa - kappa / (1 + mu s^2 / kappa)).
+__device__ __forceinline__ D3 e_pbe_x(const D3& rho, const D3& rho13, const D3& sigma, double kappa, double mu) {
+  // s^2 = sigma / (4 (3 pi^2)^(2/3) rho^(8/3))
+  const D3 r83 = rho * rho * rho13 * rho13;
+  const D3 s2 = sigma / (38.28312000250922 * r83);
+  const D3 F = (1.0 + kappa) - kappa / (1.0 + (mu / kappa) * s2);
+  return e_lda_x(rho, rho13) * F;
+}
+
+// PBE correlation with zeta = 0: e = rho (eps_c^PW92 + H(rs, t)).
+__device__ __forceinline__ D3 e_pbe_c(const D3& rho, const D3& rho13, const D3& sigma, double beta) {
+  const double gamma = 0.031090690869654895;  // (1 - ln 2) / pi^2
+  const D3 rs = rs_from_rho13(rho13);
+  const D3 ec = pw92_eps(rs);
+  // t^2 = sigma / (4 ks^2 rho^2), ks^2 = 4 kF / pi, kF = (3 pi^2 rho)^(1/3)
+  const D3 kf = 3.0936677262801355 * rho13;
+  const D3 t2 = sigma / ((16.0 / kPi) * kf * rho * rho);
+  const D3 A = (beta / gamma) / (dexp(-ec / gamma) - 1.0);
+  const D3 At2 = A * t2;
+  const D3 H = gamma * dlog(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  return rho * (ec + H);
+}
+
+// Becke 88 exchange, spin-unpolarised: two equal spin channels rho/2.
+__device__ __forceinline__ D3 e_b88_x(const D3& rho, const D3& rho13, const D3& sigma) {
+  const double beta = 0.0042;
+  // rho_s^(4/3) = (rho/2)^(4/3); x = |grad rho_s| / rho_s^(4/3)
+  const D3 rs43 = 0.39685026299204984 * rho * rho13;  // 2^(-4/3) rho^(4/3)
+  const D3 x = 0.5 * dsqrt(sigma) / rs43;
+  const double cx = 0.9305257363491000;  // (3/2) (3/(4 pi))^(1/3)
+  return -2.0 * rs43 * (cx + beta * x * x / (1.0 + 6.0 * beta * x * dasinh(x)));
+}
+
+// LYP correlation, closed shell (Miehlich-Savin-Stoll-Preuss 1989, eq. 2 at
+// rho_a = rho_b = rho/2): e = -a rho/(1 + d rho^-1/3)
+//   - a b exp(-c rho^-1/3)/(1 + d rho^-1/3) [CF rho - rho^-5/3 sigma (1/24 + 7 delta/72)].
+__device__ __forceinline__ D3 e_lyp_c(const D3& rho, const D3& rho13, const D3& sigma) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double CF = 2.8712340001881915;  // (3/10)(3 pi^2)^(2/3)
+  const D3 rm13 = 1.0 / rho13;
+  const D3 inv_den = 1.0 / (1.0 + d * rm13);
+  const D3 delta = rm13 * (c + d * inv_den);
+  const D3 ex = dexp(-c * rm13);
+  const D3 rm53 = rm13 * rm13 / rho;
+  return -a * rho * inv_den - (a * b) * ex * inv_den * (CF * rho - rm53 * sigma * (1.0 / 24.0 + (7.0 / 72.0) * delta));
+}
+
+// Functional codes handled by the kernels (libxc IDs).
+enum : int {
+  kLdaX = 1, kLdaCVwn = 7, kLdaCVwnRpa = 8, kGgaXPbe = 101, kGgaCPbe = 130, kGgaXB88 = 106,
+  kGgaCLyp = 131, kGgaXPbeG = 321, kGgaCPbeG = 322, kHybB3lyp = 402
+};
+
+__host__ __device__ inline int code_family(int code) {
+  switch (code) {
+    case kLdaX: case kLdaCVwn: case kLdaCVwnRpa: return 0;
+    case kGgaXPbe: case kGgaCPbe: case kGgaXB88: case kGgaCLyp: case kGgaXPbeG: case kGgaCPbeG: case kHybB3lyp: return 1;
+    default: return -1;
+  }
+}
+__host__ __device__ inline double code_exx(int code) { return code == kHybB3lyp ? 0.2 : 0.0; }
+constexpr double kDensThreshold = 1e-15;
+
+__device__ __forceinline__ D3 energy_density(int code, const D3& rho, const D3& rho13, const D3& sigma) {
+  switch (code) {
+    case kLdaX: return e_lda_x(rho, rho13);
+    case kLdaCVwn: return rho * vwn5_eps(rs_from_rho13(rho13));
+    case kLdaCVwnRpa: return rho * vwn_rpa_eps(rs_from_rho13(rho13));
+    case kGgaXPbe: return e_pbe_x(rho, rho13, sigma, 0.804, 0.2195149727645171);
+    case kGgaXPbeG: return e_pbe_x(rho, rho13, sigma, 0.804, 0.219510);
+    case kGgaCPbe: return e_pbe_c(rho, rho13, sigma, 0.06672455060314922);
+    case kGgaCPbeG: return e_pbe_c(rho, rho13, sigma, 0.06672632268006);
+    case kGgaXB88: return e_b88_x(rho, rho13, sigma);
+    case kGgaCLyp: return e_lyp_c(rho, rho13, sigma);
+    case kHybB3lyp:
+      return 0.08 * e_lda_x(rho, rho13) + 0.72 * e_b88_x(rho, rho13, sigma) +
+             0.19 * (rho * vwn_rpa_eps(rs_from_rho13(rho13))) + 0.81 * e_lyp_c(rho, rho13, sigma);
+    default: return dconst(0.0);
+  }
+}
+
+// Sum over codes at one point; returns eps (per particle) and derivatives.
+__device__ __forceinline__ void eval_codes(const int* codes, int ncodes, double rho, double sigma,
+                                           double* eps, double* vrho, double* vsigma) {
+  double e = 0, vr = 0, vs = 0;
+  if (rho >= kDensThreshold) {
+    const double sfloor = 1.0000000000000000e-40;  // (thr^(4/3))^2 for thr = 1e-15
+    const double s = sigma > sfloor ? sigma : sfloor;
+    const D3 r = {rho, 1.0, 0.0, 0.0};
+    const D3 sg = {s, 0.0, 1.0, 0.0};
+    const D3 r13 = dchain(r, ::cbrt(rho), 0.0);
+    const D3 rho13 = {r13.v, r13.v / (3.0 * rho), 0.0, 0.0};
+    for (int k = 0; k < ncodes; ++k) {
+      const D3 ed = energy_density(codes[k], r, rho13, sg);
+      e += ed.v;
+      vr += ed.d0;
+      vs += ed.d1;
+    }
+    e /= rho;
+  }
+  *eps = e;
+  *vrho = vr;
+  *vsigma = vs;
+}
+
+}  // namespace chx

@@ -1,0 +1,149 @@
+/*
+ * chinium_xc.h - C ABI of the MI355X (gfx950) exchange-correlation path.
+ *
+ * Drop-in replacement for the reference's per-SCF-iteration grid/XC calls.
+ * Each entry point names the reference interface it replaces (file:line,
+ * relative to the Chinium source tree). All functions return CHX_OK (0) or a
+ * negative status and never throw; chx_last_error() gives the message, which
+ * for the reference's failure cases is the reference's own text.
+ *
+ * Matrices (D, V, S) are nbasis x nbasis, column-major (Eigen::MatrixXd
+ * layout, src/Macro/Abbreviation.h:4), in the reference's AO order
+ * (centres in input order -> shells in basis-file order -> components,
+ * src/Grid/GridAO.cpp:72-75). Coordinates are in bohr.
+ *
+ * Threading: a chx_grid owns one HIP stream and its device buffers; calls on
+ * one handle are blocking and must not overlap (the reference is not
+ * re-entrant on one Grid either). Different handles may be used from
+ * different host threads.
+ */
+#ifndef CHINIUM_XC_H
+#define CHINIUM_XC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHX_OK 0
+#define CHX_ERR_ARG -1       /* bad argument / wrong size or spin count */
+#define CHX_ERR_FILE -2      /* missing or corrupt data file */
+#define CHX_ERR_STATE -3     /* call out of protocol order */
+#define CHX_ERR_DEVICE -4    /* HIP runtime error / no device */
+#define CHX_ERR_UNSUPPORTED -5
+
+typedef struct chx_mol chx_mol;   /* molecule + normalised basis (the Mwfn fields the path reads) */
+typedef struct chx_grid chx_grid; /* molecular grid, device resident (class Grid) */
+typedef struct chx_xc chx_xc;     /* functional (class ExchangeCorrelation) */
+
+/* Precision of the two dense contractions (Phi.D and Phi^T.B). */
+#define CHX_PREC_FP64 0          /* plain form: everything FP64 */
+
+const char* chx_last_error(void);
+int chx_device_count(int* n);
+
+/* ---- Molecule and basis --------------------------------------------------
+ * Replaces Mwfn::setCenters / Mwfn::setBasis (legacy/MwfnIO/Mwfn.cpp:606-712)
+ * and Normalize (src/Integral/Normalization.cpp:11-18). */
+int chx_mol_create(int natoms, const int* Z, const double* xyz_bohr, chx_mol** out);
+/* Read <data_root>/BasisSets/<basis>.gbs and normalise (libint2 convention). */
+int chx_mol_set_basis_file(chx_mol* m, const char* data_root, const char* basis_name);
+/* Append one shell with already-normalised coefficients (MwfnShell fields
+ * Type, Exponents, NormalizedCoefficients; legacy/MwfnIO/MwfnShell.h:1-9).
+ * type: 0 s, 1 Cartesian p, -1 pure p, -2 pure d, -3 pure f. */
+int chx_mol_add_shell(chx_mol* m, int center, int type, int nprim, const double* exps, const double* norm_coefs);
+int chx_mol_nbasis(const chx_mol* m, int* nbasis);
+int chx_mol_nshells(const chx_mol* m, int* nshells);
+/* Copy out shell data: arrays sized nshells (type, nprim, center) and
+ * sum(nprim) (exps, coefs). Any pointer may be NULL. */
+int chx_mol_get_shells(const chx_mol* m, int* type, int* nprim, int* center, double* exps, double* coefs);
+/* Analytic AO overlap S (column-major), used to build S-orthonormal inputs. */
+int chx_mol_overlap(const chx_mol* m, double* S);
+void chx_mol_destroy(chx_mol* m);
+
+/* ---- Grid ------------------------------------------------------------------
+ * Grid::Grid(Mwfn*, std::string grid, int nthreads, int output)
+ *   src/Grid/GridGeneration.cpp:232-310 (points, Becke weights, batches).
+ * Reads <data_root>/Grids/<grid_name>/<El>.grid. nthreads is recorded for
+ * chx_grid_num_threads (Grid::getNumThreads, src/Grid/Grid.h:98) only. */
+int chx_grid_create(const chx_mol* m, const char* grid_name, const char* data_root,
+                    int nthreads, int output, chx_grid** out);
+/* Grid(const Grid&) / operator= (GridGeneration.cpp:312-331): deep copy. */
+int chx_grid_copy(const chx_grid* src, chx_grid** out);
+void chx_grid_destroy(chx_grid* g);
+int chx_grid_num_threads(const chx_grid* g, int* n);
+/* Number of retained quadrature points (points with zero weight are dropped:
+ * they contribute exactly nothing to any sum) and their coordinates/weights
+ * in the library's point order (xyzw[4*i+k]). */
+int chx_grid_num_points(const chx_grid* g, long* n);
+int chx_grid_get_points(const chx_grid* g, double* xyzw);
+/* Grid::setType (src/Grid/Grid.cpp:14-20): -1 none, 0 LDA, 1 GGA, 2 meta-GGA. */
+int chx_grid_set_type(chx_grid* g, int type);
+/* Grid::getAO (src/Grid/Grid.cpp:22-33). The GPU path recomputes AO values
+ * inside the contraction kernels; this call builds the per-batch AO screening
+ * that those kernels use and must precede the first density call. */
+int chx_grid_get_ao(chx_grid* g, int derivative, int output);
+/* Grid::getDensity(std::vector<EigenMatrix> D) (src/Grid/Grid.cpp:35-49):
+ * D is nspin column-major nbasis x nbasis matrices, back to back (host memory). */
+int chx_grid_get_density(chx_grid* g, const double* D, int nspin, int nbasis);
+/* Same, with D already in device memory (HBM); used to time the path with
+ * inputs resident on the GPU. */
+int chx_grid_get_density_device(chx_grid* g, const double* D_dev, int nspin, int nbasis);
+/* Grid::getNumElectrons (src/Grid/Grid.cpp:51-61). */
+int chx_grid_num_electrons(chx_grid* g, double* N);
+/* Grid::getEnergy (src/Grid/Grid.cpp:112-122). */
+int chx_grid_energy(chx_grid* g, double* E);
+/* Grid::getFock (src/Grid/Grid.cpp:156-171): writes nspin matrices (host). */
+int chx_grid_fock(chx_grid* g, double* F, int nspin, int nbasis);
+/* Same, into device memory. */
+int chx_grid_fock_device(chx_grid* g, double* F_dev, int nspin, int nbasis);
+/* Supply v_rho directly, in the library's point order, as the SAP guess does
+ * when it writes SubGrid::Eps1Rho before Grid::getFock
+ * (src/HartreeFockKohnSham/GuessSCF.cpp:37-60). Other potentials are zeroed. */
+int chx_grid_set_vrho(chx_grid* g, const double* vrho, long npoints);
+/* Device stream the handle launches on (hipStream_t), for event timing. */
+int chx_grid_stream(const chx_grid* g, void** stream);
+/* Options of the GPU path (no reference counterpart). */
+int chx_grid_set_precision(chx_grid* g, int precision);
+/* AO screening threshold: an AO tile is dropped from a batch when every |phi|
+ * and |grad phi| in it stays at or below the threshold at every point of the
+ * batch. Takes effect at the next chx_grid_get_ao. Default 1e-14. */
+int chx_grid_set_screening(chx_grid* g, double threshold);
+/* Statistics of the batch layout: batches, points, AO tiles per batch (mean),
+ * algorithmic FLOPs of one XC build (dense-contraction count, k*2*sum g_b n_b^2). */
+int chx_grid_stats(const chx_grid* g, long* nbatches, long* npoints, double* mean_tiles, double* flops_per_build);
+
+/* Per-kernel HIP-event timing on the handle's stream (no reference counterpart).
+ * Stages: 0 D permute, 1 density contraction, 2 functional + E/N sums,
+ * 3 Vxc contraction, 4 V finalise. Times accumulate until read with reset. */
+int chx_grid_profile(chx_grid* g, int enable);
+int chx_grid_kernel_times(chx_grid* g, int reset, double* ms /*[5]*/, long* launches /*[5]*/);
+
+/* ---- Functional ------------------------------------------------------------
+ * ExchangeCorrelation::Read(std::string df, bool output)
+ *   src/ExchangeCorrelation.cpp:18-84: reads <data_root>/DensityFunctionals/<df>.df */
+int chx_xc_read(const char* data_root, const char* df_name, int spin, int output, chx_xc** out);
+/* Same from explicit libxc IDs (the codes a .df line holds). */
+int chx_xc_create(int ncodes, const int* codes, int spin, chx_xc** out);
+void chx_xc_destroy(chx_xc* xc);
+/* Members Family (0 LDA, 1 GGA, 2 mGGA), EXX, Codes (ExchangeCorrelation.h:9-12). */
+int chx_xc_info(const chx_xc* xc, int* family, double* exx, int* ncodes, int* codes);
+/* ExchangeCorrelation::Evaluate(std::string order, Grid&)
+ *   src/ExchangeCorrelation.cpp:165-248. order: "ev", "e" or "v". */
+int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order);
+
+/* Lebedev-Laikov angular rule with npoints points (x, y, z, w; sum w = 1), as
+ * ld_by_order (src/Grid/sphere_lebedev_rule.cpp:671-846) returns it; the grid
+ * scales w by 4 pi (GridGeneration.cpp:133). Host only. */
+int chx_lebedev_rule(int npoints, double* xyzw);
+
+/* ---- Diagnostics (no reference counterpart) ---------------------------------
+ * Runs one FP64 MFMA tile on the device and returns the largest deviation from
+ * the scalar product; 0 means the operand layout the kernels assume holds. */
+int chx_selftest(double* max_err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHINIUM_XC_H */

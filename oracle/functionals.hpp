@@ -1,0 +1,192 @@
+// FP64 exchange-correlation functionals for the CPU oracle (test infrastructure).
+//
+// The reference evaluates these through libxc (src/ExchangeCorrelation.cpp:86-163,
+// per-code sum at :202-217). libxc is not present, so each functional is written
+// here from its published form, spin-unpolarised (the reference's closed-shell
+// SCF sets Spin=1, src/HartreeFockKohnSham/Restricted/Gateway.cpp:62).
+// Derivatives come from forward-mode dual numbers, so v_rho, v_sigma and v_tau
+// are exact derivatives of the energy density rho*eps that is coded below.
+//
+// libxc conventions kept: eps is per particle; sigma = |grad rho|^2;
+// tau = 1/2 sum_i |grad psi_i|^2; a point with rho below the density threshold
+// contributes nothing; sigma is floored at thr^(8/3) and, for meta-GGAs, capped
+// at 8 rho tau.
+#pragma once
+#include <cmath>
+
+namespace oxf {
+
+template <int N> struct Dual {
+  double v;
+  double d[N];
+  Dual() : v(0) { for (int i = 0; i < N; ++i) d[i] = 0; }
+  Dual(double x) : v(x) { for (int i = 0; i < N; ++i) d[i] = 0; }
+  static Dual var(double x, int k) { Dual r(x); r.d[k] = 1; return r; }
+};
+template <int N> Dual<N> chain(const Dual<N>& a, double f, double df) {
+  Dual<N> r; r.v = f; for (int i = 0; i < N; ++i) r.d[i] = df * a.d[i]; return r;
+}
+template <int N> Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v + b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int N> Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v - b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int N> Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v * b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+template <int N> Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v / b.v; for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v); return r; }
+template <int N> Dual<N> operator-(const Dual<N>& a) { Dual<N> r; r.v = -a.v; for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+template <int N> Dual<N> operator+(const Dual<N>& a, double b) { return a + Dual<N>(b); }
+template <int N> Dual<N> operator+(double a, const Dual<N>& b) { return Dual<N>(a) + b; }
+template <int N> Dual<N> operator-(const Dual<N>& a, double b) { return a - Dual<N>(b); }
+template <int N> Dual<N> operator-(double a, const Dual<N>& b) { return Dual<N>(a) - b; }
+template <int N> Dual<N> operator*(const Dual<N>& a, double b) { return a * Dual<N>(b); }
+template <int N> Dual<N> operator*(double a, const Dual<N>& b) { return Dual<N>(a) * b; }
+template <int N> Dual<N> operator/(const Dual<N>& a, double b) { return a / Dual<N>(b); }
+template <int N> Dual<N> operator/(double a, const Dual<N>& b) { return Dual<N>(a) / b; }
+template <int N> Dual<N> exp(const Dual<N>& a) { double e = std::exp(a.v); return chain(a, e, e); }
+template <int N> Dual<N> log(const Dual<N>& a) { return chain(a, std::log(a.v), 1.0 / a.v); }
+template <int N> Dual<N> sqrt(const Dual<N>& a) { double s = std::sqrt(a.v); return chain(a, s, 0.5 / s); }
+template <int N> Dual<N> pow(const Dual<N>& a, double p) { double f = std::pow(a.v, p); return chain(a, f, p * std::pow(a.v, p - 1.0)); }
+template <int N> Dual<N> atan(const Dual<N>& a) { return chain(a, std::atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+template <int N> Dual<N> asinh(const Dual<N>& a) { return chain(a, std::asinh(a.v), 1.0 / std::sqrt(1.0 + a.v * a.v)); }
+template <int N> Dual<N> tanh(const Dual<N>& a) { double t = std::tanh(a.v); return chain(a, t, 1.0 - t * t); }
+template <int N> Dual<N> expm1(const Dual<N>& a) { return chain(a, std::expm1(a.v), std::exp(a.v)); }
+template <int N> Dual<N> log1p(const Dual<N>& a) { return chain(a, std::log1p(a.v), 1.0 / (1.0 + a.v)); }
+inline double exp(double a) { return std::exp(a); }
+inline double log(double a) { return std::log(a); }
+inline double sqrt(double a) { return std::sqrt(a); }
+inline double pow(double a, double p) { return std::pow(a, p); }
+inline double atan(double a) { return std::atan(a); }
+inline double asinh(double a) { return std::asinh(a); }
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---- LDA exchange (libxc 1, Slater/Dirac): e = -(3/4)(3/pi)^(1/3) rho^(4/3)
+template <class T> T lda_x(const T& rho) {
+  const double cx = -0.75 * std::cbrt(3.0 / kPi);
+  return cx * pow(rho, 4.0 / 3.0);
+}
+
+// ---- VWN paramagnetic interpolation, eps_c(rs) in Hartree.
+// Vosko, Wilk, Nusair, Can. J. Phys. 58, 1200 (1980), eq. 4.4.
+struct VwnPar { double A, b, c, x0; };
+constexpr VwnPar kVwn5 = {0.0310907, 3.72744, 12.9352, -0.10498};       // libxc 7 (fit to Ceperley-Alder)
+constexpr VwnPar kVwnRpa = {0.0310907, 13.0720, 42.7198, -0.409286};    // libxc 8 (fit to RPA)
+template <class T> T vwn_eps(const T& rs, const VwnPar& p) {
+  const T x = sqrt(rs);
+  const T X = x * x + p.b * x + p.c;
+  const double Q = std::sqrt(4.0 * p.c - p.b * p.b);
+  const double X0 = p.x0 * p.x0 + p.b * p.x0 + p.c;
+  const T at = atan(Q / (2.0 * x + p.b));
+  return p.A * (log(x * x / X) + (2.0 * p.b / Q) * at
+                - (p.b * p.x0 / X0) * (log((x - p.x0) * (x - p.x0) / X) + (2.0 * (p.b + 2.0 * p.x0) / Q) * at));
+}
+template <class T> T rs_of(const T& rho) { return pow(3.0 / (4.0 * kPi) / rho, 1.0 / 3.0); }
+template <class T> T lda_c_vwn(const T& rho, const VwnPar& p) { return rho * vwn_eps(rs_of(rho), p); }
+
+// ---- PW92 paramagnetic correlation with libxc's PW_MOD parameters.
+// Perdew, Wang, PRB 45, 13244 (1992), eq. 10; G(rs; A, a1, b1..b4, p=1).
+template <class T> T pw92_eps(const T& rs) {
+  const double A = 0.0310907, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+  const T srs = sqrt(rs);
+  const T den = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+  return -2.0 * A * (1.0 + a1 * rs) * log(1.0 + 1.0 / den);
+}
+
+// ---- PBE exchange. Perdew, Burke, Ernzerhof, PRL 77, 3865 (1996), eq. 14.
+template <class T> T gga_x_pbe(const T& rho, const T& sigma, double kappa, double mu) {
+  const T ex_lda = lda_x(rho);
+  const double c2 = 4.0 * std::pow(3.0 * kPi * kPi, 2.0 / 3.0);
+  const T s2 = sigma / (c2 * pow(rho, 8.0 / 3.0));
+  const T F = 1.0 + kappa - kappa / (1.0 + mu * s2 / kappa);
+  return ex_lda * F;
+}
+
+// ---- PBE correlation (zeta = 0, phi = 1). PRL 77, 3865, eqs. 7-8.
+template <class T> T gga_c_pbe(const T& rho, const T& sigma, double beta) {
+  const double gamma = (1.0 - std::log(2.0)) / (kPi * kPi);
+  const T rs = rs_of(rho);
+  const T ec = pw92_eps(rs);
+  const T kf = pow(3.0 * kPi * kPi * rho, 1.0 / 3.0);
+  const T ks2 = 4.0 * kf / kPi;
+  const T t2 = sigma / (4.0 * ks2 * rho * rho);
+  const T A = (beta / gamma) / (exp(-ec / gamma) - 1.0);
+  const T At2 = A * t2;
+  const T H = gamma * log(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  return rho * (ec + H);
+}
+
+// ---- Becke 88 exchange, spin-unpolarised. Becke, PRA 38, 3098 (1988), eq. 8.
+template <class T> T gga_x_b88(const T& rho, const T& sigma) {
+  const double beta = 0.0042;
+  const T rs_ = 0.5 * rho;                         // rho_sigma
+  const T gs = 0.5 * sqrt(sigma);                  // |grad rho_sigma|
+  const T r43 = pow(rs_, 4.0 / 3.0);
+  const T x = gs / r43;
+  const double cx = 1.5 * std::cbrt(3.0 / (4.0 * kPi));
+  const T per_spin = -r43 * (cx + beta * x * x / (1.0 + 6.0 * beta * x * asinh(x)));
+  return 2.0 * per_spin;
+}
+
+// ---- LYP correlation, closed shell. Miehlich, Savin, Stoll, Preuss,
+// CPL 157, 200 (1989), eq. 2 with rho_a = rho_b = rho/2, |grad rho_a|^2 = sigma/4.
+template <class T> T gga_c_lyp(const T& rho, const T& sigma) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double CF = 0.3 * std::pow(3.0 * kPi * kPi, 2.0 / 3.0);
+  const T rm13 = pow(rho, -1.0 / 3.0);
+  const T den = 1.0 + d * rm13;
+  const T delta = c * rm13 + d * rm13 / den;
+  const T ex = exp(-c * rm13);
+  const T t1 = -a * rho / den;
+  const T t2 = -a * b * ex / den * (CF * rho - pow(rho, -5.0 / 3.0) * sigma * (1.0 / 24.0 + 7.0 * delta / 72.0));
+  return t1 + t2;
+}
+
+// ---- Functional table (libxc IDs used by the reference's .df files).
+struct Component { int kind; double coef; };
+// kind: 1 lda_x, 7 vwn5, 8 vwn_rpa, 101/321 pbe_x, 130/322 pbe_c, 106 b88, 131 lyp
+inline int family_of(int code) {
+  switch (code) {
+    case 1: case 7: case 8: return 0;
+    case 101: case 130: case 321: case 322: case 106: case 131: case 402: return 1;
+    case 497: case 498: return 2;
+    default: return -1;
+  }
+}
+inline double density_threshold(int /*code*/) { return 1e-15; }
+
+// Energy density rho*eps of one libxc code, on dual inputs (rho, sigma, tau).
+template <class T> T energy_density(int code, const T& rho, const T& sigma, const T& /*tau*/) {
+  switch (code) {
+    case 1: return lda_x(rho);
+    case 7: return lda_c_vwn(rho, kVwn5);
+    case 8: return lda_c_vwn(rho, kVwnRpa);
+    case 101: return gga_x_pbe(rho, sigma, 0.804, 0.2195149727645171);
+    case 321: return gga_x_pbe(rho, sigma, 0.804, 0.219510);
+    case 130: return gga_c_pbe(rho, sigma, 0.06672455060314922);
+    case 322: return gga_c_pbe(rho, sigma, 0.06672632268006);
+    case 106: return gga_x_b88(rho, sigma);
+    case 131: return gga_c_lyp(rho, sigma);
+    case 402:  // B3LYP (XC part; 20% exact exchange handled by the caller)
+      return 0.08 * lda_x(rho) + 0.72 * gga_x_b88(rho, sigma) + 0.19 * lda_c_vwn(rho, kVwnRpa) + 0.81 * gga_c_lyp(rho, sigma);
+    default: return T(0.0);
+  }
+}
+
+// Evaluate one code at one point, accumulating like src/ExchangeCorrelation.cpp:202-217.
+inline void eval_point(int code, double rho, double sigma, double tau,
+                       double* eps, double* vrho, double* vsigma, double* vtau) {
+  const double thr = density_threshold(code);
+  if (!(rho >= thr)) return;
+  const double sthr = std::pow(thr, 4.0 / 3.0);
+  double s = sigma > sthr * sthr ? sigma : sthr * sthr;
+  double t = tau;
+  if (family_of(code) == 2) {
+    t = tau > 1e-20 ? tau : 1e-20;
+    if (s > 8.0 * rho * t) s = 8.0 * rho * t;
+  }
+  typedef Dual<3> D3;
+  const D3 e = energy_density(code, D3::var(rho, 0), D3::var(s, 1), D3::var(t, 2));
+  *eps += e.v / rho;
+  *vrho += e.d[0];
+  *vsigma += e.d[1];
+  *vtau += e.d[2];
+}
+
+}  // namespace oxf

@@ -1,0 +1,837 @@
+// FP64 CPU oracle: restatement of the reference's XC path, loop nest by loop nest.
+// TEST INFRASTRUCTURE ONLY (see oracle.h). Compiled like the reference
+// (makefile:41-42: -O3 -fopenmp -march=native) so it doubles as the CPU baseline.
+#include "oracle.h"
+#include "functionals.hpp"
+
+#include <omp.h>
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+const char* kElem[] = {"X", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si",
+                       "P", "S", "Cl", "Ar", "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni",
+                       "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr"};
+const int kNElem = 36;
+
+struct Shell {
+  int type;      // 0 s, 1 Cartesian p, -l pure
+  int center;
+  std::vector<double> exps, coefs;  // coefs normalised
+};
+
+// One angular function as a polynomial: sum of c * x^i y^j z^k.
+struct Mono { double c; int i, j, k; };
+typedef std::vector<Mono> Poly;
+
+std::vector<Poly> angular_functions(int type) {
+  const double s3 = std::sqrt(3.0), s6 = std::sqrt(6.0), s10 = std::sqrt(10.0), s15 = std::sqrt(15.0);
+  switch (type) {
+    case 0: return {{{1, 0, 0, 0}}};
+    case 1: return {{{1, 1, 0, 0}}, {{1, 0, 1, 0}}, {{1, 0, 0, 1}}};                 // AO/CartP.hpp
+    case -1: return {{{1, 0, 1, 0}}, {{1, 0, 0, 1}}, {{1, 1, 0, 0}}};                // AO/PureP.hpp (y,z,x)
+    case -2: return {                                                                 // AO/PureD.hpp
+        {{s3, 1, 1, 0}},
+        {{s3, 0, 1, 1}},
+        {{1, 0, 0, 2}, {-0.5, 2, 0, 0}, {-0.5, 0, 2, 0}},
+        {{s3, 1, 0, 1}},
+        {{s3 / 2, 2, 0, 0}, {-s3 / 2, 0, 2, 0}}};
+    case -3: return {                                                                 // AO/PureF.hpp
+        {{3 * s10 / 4, 2, 1, 0}, {-s10 / 4, 0, 3, 0}},
+        {{s15, 1, 1, 1}},
+        {{s6, 0, 1, 2}, {-s6 / 4, 2, 1, 0}, {-s6 / 4, 0, 3, 0}},
+        {{1, 0, 0, 3}, {-1.5, 2, 0, 1}, {-1.5, 0, 2, 1}},
+        {{s6, 1, 0, 2}, {-s6 / 4, 3, 0, 0}, {-s6 / 4, 1, 2, 0}},
+        {{s15 / 2, 2, 0, 1}, {-s15 / 2, 0, 2, 1}},
+        {{s10 / 4, 3, 0, 0}, {-3 * s10 / 4, 1, 2, 0}}};
+    default: throw std::runtime_error("oracle: angular momentum above f is not supported");
+  }
+}
+
+inline double ipow(double x, int n) { double r = 1; for (int i = 0; i < n; ++i) r *= x; return r; }
+
+// Value, gradient and second derivatives (xx,yy,zz) of a polynomial.
+void poly_eval(const Poly& p, double x, double y, double z, double* q, double* qg, double* qll) {
+  double v = 0, gx = 0, gy = 0, gz = 0, lxx = 0, lyy = 0, lzz = 0;
+  for (const Mono& m : p) {
+    const double xi = ipow(x, m.i), yj = ipow(y, m.j), zk = ipow(z, m.k);
+    v += m.c * xi * yj * zk;
+    if (m.i > 0) gx += m.c * m.i * ipow(x, m.i - 1) * yj * zk;
+    if (m.j > 0) gy += m.c * m.j * xi * ipow(y, m.j - 1) * zk;
+    if (m.k > 0) gz += m.c * m.k * xi * yj * ipow(z, m.k - 1);
+    if (m.i > 1) lxx += m.c * m.i * (m.i - 1) * ipow(x, m.i - 2) * yj * zk;
+    if (m.j > 1) lyy += m.c * m.j * (m.j - 1) * xi * ipow(y, m.j - 2) * zk;
+    if (m.k > 1) lzz += m.c * m.k * (m.k - 1) * xi * yj * ipow(z, m.k - 2);
+  }
+  *q = v; qg[0] = gx; qg[1] = gy; qg[2] = gz; *qll = lxx + lyy + lzz;
+}
+
+struct SubGrid {
+  int n;      // unpadded
+  int npad;   // padded to a multiple of 8 (GridGeneration.cpp:214)
+  std::vector<double> x, y, z, w;
+  // AO tables, [mu][g] (value), [r][mu][g] (gradient), [mu][g] (Laplacian)
+  std::vector<double> ao, ao1, ao2l;
+  std::vector<double> rho, rho1, sigma, tau, lapl;           // per point
+  std::vector<double> eps, vrho, vsigma, vlapl, vtau;
+};
+
+}  // namespace
+
+struct ox_system {
+  int natoms = 0;
+  std::vector<int> Z;
+  std::vector<double> xyz;
+  std::vector<Shell> shells;
+  int nbasis = 0;
+  std::vector<std::vector<Poly>> shell_ang;
+  std::vector<double> points;   // 4*G generation order
+  std::vector<SubGrid> subgrids;
+  std::vector<int> active;
+  std::vector<int> codes;
+  int family = -1;
+  int nthreads = 1;
+  bool ao_ready = false;
+};
+
+namespace {
+
+std::string elem_name(int z) {
+  if (z < 1 || z > kNElem) throw std::runtime_error("oracle: unsupported element");
+  return kElem[z];
+}
+
+int elem_z(std::string s) {
+  for (auto& ch : s) ch = (char)std::toupper((unsigned char)ch);
+  for (int z = 1; z <= kNElem; ++z) {
+    std::string e = kElem[z];
+    for (auto& ch : e) ch = (char)std::toupper((unsigned char)ch);
+    if (e == s) return z;
+  }
+  return -1;
+}
+
+double parse_fortran_double(std::string t) {
+  for (auto& ch : t) if (ch == 'D' || ch == 'd') ch = 'E';
+  return std::stod(t);
+}
+
+// ---- Basis: .gbs reader (restates legacy/MwfnIO/Mwfn.cpp:606-680) + normalisation
+// (libint2 Shell::renorm as called by src/Integral/Normalization.cpp:11-18).
+std::map<int, std::vector<Shell>> read_gbs(const std::string& path) {
+  std::ifstream f(path);
+  if (!f.good()) throw std::runtime_error("Basis set file is missing: " + path);
+  std::map<int, std::vector<Shell>> out;
+  std::string line;
+  int cur = -1;
+  std::vector<Shell> cur_shells;
+  while (std::getline(f, line)) {
+    std::stringstream ss(line);
+    std::string w;
+    if (!(ss >> w)) continue;
+    if (w[0] == '!') continue;
+    if (w[0] == '-') {
+      cur = elem_z(w.substr(1));
+      cur_shells.clear();
+    } else if (w == "****") {
+      if (cur > 0) out[cur] = cur_shells;
+      cur = -1;
+    } else if (w == "S" || w == "SP" || w == "P" || w == "D" || w == "F" || w == "G") {
+      std::string ns; ss >> ns;
+      const int n = std::stoi(ns);
+      Shell a, b;
+      a.type = (w == "S" || w == "SP") ? 0 : (w == "P" ? 1 : (w == "D" ? -2 : (w == "F" ? -3 : -4)));
+      b.type = 1;
+      for (int i = 0; i < n; ++i) {
+        std::getline(f, line);
+        std::stringstream ls(line);
+        std::string e, c1, c2;
+        ls >> e >> c1;
+        a.exps.push_back(parse_fortran_double(e));
+        a.coefs.push_back(parse_fortran_double(c1));
+        if (w == "SP") {
+          ls >> c2;
+          b.exps.push_back(parse_fortran_double(e));
+          b.coefs.push_back(parse_fortran_double(c2));
+        }
+      }
+      cur_shells.push_back(a);
+      if (w == "SP") cur_shells.push_back(b);
+    }
+  }
+  return out;
+}
+
+double dfact(int n) { double r = 1; for (int k = n; k > 1; k -= 2) r *= k; return r; }  // n!!
+
+void normalise(Shell& s) {
+  const int l = std::abs(s.type);
+  const double pi32 = std::pow(M_PI, 1.5);
+  const double dfl = dfact(2 * l - 1);
+  for (size_t p = 0; p < s.exps.size(); ++p) {
+    const double a2 = 2 * s.exps[p];
+    s.coefs[p] *= std::sqrt(std::pow(2.0, l) * std::pow(a2, l + 1.5) / (pi32 * dfl));
+  }
+  double norm = 0;
+  for (size_t p = 0; p < s.exps.size(); ++p)
+    for (size_t q = 0; q < s.exps.size(); ++q)
+      norm += dfl * pi32 * s.coefs[p] * s.coefs[q] / (std::pow(2.0, l) * std::pow(s.exps[p] + s.exps[q], l + 1.5));
+  const double f = 1.0 / std::sqrt(norm);
+  for (double& c : s.coefs) c *= f;
+}
+
+int shell_size(int type) { return type == 1 ? 3 : 2 * std::abs(type) + 1; }
+
+// ---- Lebedev: octahedral expansion of orbit generators.
+std::map<int, std::vector<std::array<double, 4>>> g_lebedev;
+
+void oh_orbit(int type, double a, double b, double v, std::vector<std::array<double, 4>>& out) {
+  auto signs3 = [&](double x, double y, double z) {
+    for (int sx = -1; sx <= 1; sx += 2)
+      for (int sy = -1; sy <= 1; sy += 2)
+        for (int sz = -1; sz <= 1; sz += 2) {
+          if ((x == 0 && sx < 0) || (y == 0 && sy < 0) || (z == 0 && sz < 0)) continue;
+          out.push_back({sx * x, sy * y, sz * z, v});
+        }
+  };
+  auto perms = [&](double p, double q, double r, bool distinct) {
+    // all distinct permutations of (p,q,r)
+    double t[3] = {p, q, r};
+    std::vector<std::array<double, 3>> seen;
+    int idx[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    for (auto& ix : idx) {
+      std::array<double, 3> c = {t[ix[0]], t[ix[1]], t[ix[2]]};
+      bool dup = false;
+      for (auto& s : seen) if (s == c) dup = true;
+      if (dup) continue;
+      seen.push_back(c);
+      signs3(c[0], c[1], c[2]);
+    }
+    (void)distinct;
+  };
+  switch (type) {
+    case 1: perms(1, 0, 0, false); break;
+    case 2: { double r = std::sqrt(0.5); perms(0, r, r, false); break; }
+    case 3: { double r = std::sqrt(1.0 / 3.0); perms(r, r, r, false); break; }
+    case 4: perms(a, a, std::sqrt(1 - 2 * a * a), false); break;
+    case 5: perms(a, std::sqrt(1 - a * a), 0, true); break;
+    case 6: perms(a, b, std::sqrt(1 - a * a - b * b), true); break;
+    default: throw std::runtime_error("oracle: bad Lebedev orbit type");
+  }
+}
+
+void load_lebedev(const std::string& path) {
+  if (!g_lebedev.empty()) return;
+  std::ifstream f(path);
+  if (!f.good()) throw std::runtime_error("Lebedev table missing: " + path);
+  std::string line;
+  int cur = -1;
+  while (std::getline(f, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::stringstream ss(line);
+    std::string w; ss >> w;
+    if (w == "rule") { int n, k; ss >> n >> k; cur = n; g_lebedev[cur] = {}; continue; }
+    int type = std::stoi(w);
+    std::string sa, sb, sv; ss >> sa >> sb >> sv;
+    oh_orbit(type, std::stod(sa), std::stod(sb), std::stod(sv), g_lebedev[cur]);
+  }
+  for (auto& kv : g_lebedev)
+    if ((int)kv.second.size() != kv.first) throw std::runtime_error("oracle: Lebedev expansion size mismatch");
+}
+
+// s(mu) with three Becke smoothing iterations (GridGeneration.cpp:50-71).
+double becke_s(double x0, double y0, double z0, const double* b, const double* c) {
+  const double r01 = std::sqrt((x0 - b[0]) * (x0 - b[0]) + (y0 - b[1]) * (y0 - b[1]) + (z0 - b[2]) * (z0 - b[2]));
+  const double r02 = std::sqrt((x0 - c[0]) * (x0 - c[0]) + (y0 - c[1]) * (y0 - c[1]) + (z0 - c[2]) * (z0 - c[2]));
+  const double r12 = std::sqrt((b[0] - c[0]) * (b[0] - c[0]) + (b[1] - c[1]) * (b[1] - c[1]) + (b[2] - c[2]) * (b[2] - c[2]));
+  const double mu = (r01 - r02) / r12;
+  double p = mu;
+  for (int k = 0; k < 3; ++k) p = 1.5 * p - 0.5 * p * p * p;
+  return 0.5 * (1 - p);
+}
+
+struct AtomGrid { std::vector<double> r, wr; std::vector<int> order; };
+
+AtomGrid read_atom_grid(const std::string& path) {
+  std::ifstream f(path);
+  if (!f.good()) throw std::runtime_error("Missing element grid file in folder!");
+  std::string line;
+  std::getline(f, line);
+  int ngroups, ntot;
+  { std::stringstream ss(line); ss >> ngroups >> ntot; }
+  std::getline(f, line);
+  std::stringstream ss(line);
+  std::string formula; ss >> formula;
+  AtomGrid g;
+  g.r.resize(ntot); g.wr.resize(ntot);
+  if (formula == "de2") {  // GridGeneration.cpp:90-100
+    double alpha, x0, x1; ss >> alpha >> x0 >> x1;
+    const double h = (x1 - x0) / (double)(ntot - 1);
+    for (int i = 0; i < ntot; ++i) {
+      const double xi = h * i + x0;
+      g.r[i] = std::exp(alpha * xi - std::exp(-xi));
+      g.wr[i] = std::exp(3 * alpha * xi - 3 * std::exp(-xi)) * (alpha + std::exp(-xi)) * h;
+    }
+  } else if (formula == "em") {  // GridGeneration.cpp:103-113
+    double R; ss >> R;
+    for (int i = 0; i < ntot; ++i) {
+      const double ii = i + 1;
+      g.r[i] = R * std::pow(ii / (ntot + 1 - ii), 2);
+      g.wr[i] = 2 * std::pow(R, 3) * (ntot + 1) * std::pow(ii, 5) / std::pow(ntot + 1 - ii, 7);
+    }
+  } else {
+    throw std::runtime_error("Unrecognized radial formula!");
+  }
+  for (int k = 0; k < ngroups; ++k) {
+    std::getline(f, line);
+    std::stringstream gs(line);
+    int npts, nsh; gs >> npts >> nsh;
+    for (int s = 0; s < nsh; ++s) g.order.push_back(npts);
+  }
+  if ((int)g.order.size() != ntot) throw std::runtime_error("Corrupted grid file");
+  return g;
+}
+
+// Contiguous recursive halving into <=128-point sub-grids. The reference's Cut
+// (GridGeneration.cpp:177-201) reorders only the first 4 points of a parent and
+// then splits by column index, so membership is exactly this halving.
+void recursive_halving(long lo, long n, std::vector<std::pair<long, long>>& out) {
+  if (n <= 128) { out.push_back({lo, n}); return; }
+  const long left = n / 2;
+  recursive_halving(lo, left, out);
+  recursive_halving(lo + left, n - left, out);
+}
+
+// AO tables for one sub-grid (GridAO.cpp:14-215).
+void compute_ao(ox_system* s, SubGrid& sg, int family) {
+  const int ng = sg.npad, nb = s->nbasis;
+  sg.ao.assign((size_t)ng * nb, 0);
+  sg.ao1.assign((size_t)3 * ng * nb, 0);
+  if (family >= 2) sg.ao2l.assign((size_t)ng * nb, 0);
+  else sg.ao2l.clear();
+  int mu0 = 0;
+  for (size_t is = 0; is < s->shells.size(); ++is) {
+    const Shell& sh = s->shells[is];
+    const double* c = &s->xyz[3 * sh.center];
+    const std::vector<Poly>& ang = s->shell_ang[is];
+    for (int g = 0; g < ng; ++g) {
+      const double x = sg.x[g] - c[0], y = sg.y[g] - c[1], z = sg.z[g] - c[2];
+      const double r2 = x * x + y * y + z * z;
+      double A0 = 0, A1 = 0, A2 = 0;
+      for (size_t p = 0; p < sh.exps.size(); ++p) {
+        const double t = sh.coefs[p] * std::exp(-sh.exps[p] * r2);
+        A0 += t; A1 += t * sh.exps[p]; A2 += t * sh.exps[p] * sh.exps[p];
+      }
+      const double Ag[3] = {-2 * x * A1, -2 * y * A1, -2 * z * A1};
+      const double Alap = -6 * A1 + 4 * r2 * A2;
+      for (size_t m = 0; m < ang.size(); ++m) {
+        double q, qg[3], ql;
+        poly_eval(ang[m], x, y, z, &q, qg, &ql);
+        const int mu = mu0 + (int)m;
+        sg.ao[(size_t)mu * ng + g] = A0 * q;
+        for (int r = 0; r < 3; ++r) sg.ao1[((size_t)r * nb + mu) * ng + g] = Ag[r] * q + A0 * qg[r];
+        if (family >= 2)
+          sg.ao2l[(size_t)mu * ng + g] = Alap * q + 2 * (Ag[0] * qg[0] + Ag[1] * qg[1] + Ag[2] * qg[2]) + A0 * ql;
+      }
+    }
+    mu0 += (int)ang.size();
+  }
+}
+
+// Density on one sub-grid (GridDensity.cpp:12-41, EinSum nests Rho, Rho1, Sigma, Tau, Lapl).
+void density_subgrid(const ox_system* s, SubGrid& sg, const double* D, int family, std::vector<double>& tmp) {
+  const int ng = sg.npad, nb = s->nbasis;
+  tmp.assign(ng, 0);
+  sg.rho.assign(ng, 0);
+  // Rho(g) = sum_nu (sum_mu D(mu,nu) AO(g,mu)) AO(g,nu)
+  for (int nu = 0; nu < nb; ++nu) {
+    std::fill(tmp.begin(), tmp.end(), 0.0);
+    for (int mu = 0; mu < nb; ++mu) {
+      const double d = D[(size_t)nu * nb + mu];
+      const double* a = &sg.ao[(size_t)mu * ng];
+      for (int g = 0; g < ng; ++g) tmp[g] += d * a[g];
+    }
+    const double* a = &sg.ao[(size_t)nu * ng];
+    for (int g = 0; g < ng; ++g) sg.rho[g] += tmp[g] * a[g];
+  }
+  if (family >= 1) {
+    sg.rho1.assign((size_t)3 * ng, 0);
+    for (int r = 0; r < 3; ++r)
+      for (int nu = 0; nu < nb; ++nu) {
+        std::fill(tmp.begin(), tmp.end(), 0.0);
+        for (int mu = 0; mu < nb; ++mu) {
+          const double d = D[(size_t)nu * nb + mu];
+          const double* a1 = &sg.ao1[((size_t)r * nb + mu) * ng];
+          for (int g = 0; g < ng; ++g) tmp[g] += d * a1[g];
+        }
+        const double* a = &sg.ao[(size_t)nu * ng];
+        for (int g = 0; g < ng; ++g) sg.rho1[(size_t)r * ng + g] += tmp[g] * a[g];
+      }
+    for (double& v : sg.rho1) v *= 2;
+    sg.sigma.assign(ng, 0);
+    for (int g = 0; g < ng; ++g)
+      for (int r = 0; r < 3; ++r) sg.sigma[g] += sg.rho1[(size_t)r * ng + g] * sg.rho1[(size_t)r * ng + g];
+  }
+  if (family >= 2) {
+    sg.tau.assign(ng, 0);
+    for (int r = 0; r < 3; ++r)
+      for (int nu = 0; nu < nb; ++nu) {
+        std::fill(tmp.begin(), tmp.end(), 0.0);
+        for (int mu = 0; mu < nb; ++mu) {
+          const double d = D[(size_t)nu * nb + mu];
+          const double* a1 = &sg.ao1[((size_t)r * nb + mu) * ng];
+          for (int g = 0; g < ng; ++g) tmp[g] += d * a1[g];
+        }
+        const double* a1n = &sg.ao1[((size_t)r * nb + nu) * ng];
+        for (int g = 0; g < ng; ++g) sg.tau[g] += tmp[g] * a1n[g];
+      }
+    for (double& v : sg.tau) v *= 0.5;
+    sg.lapl.assign(ng, 0);
+    for (int g = 0; g < ng; ++g) sg.lapl[g] = 4 * sg.tau[g];
+    for (int nu = 0; nu < nb; ++nu) {
+      std::fill(tmp.begin(), tmp.end(), 0.0);
+      for (int mu = 0; mu < nb; ++mu) {
+        const double d = D[(size_t)nu * nb + mu];
+        const double* a = &sg.ao[(size_t)mu * ng];
+        for (int g = 0; g < ng; ++g) tmp[g] += d * a[g];
+      }
+      const double* l = &sg.ao2l[(size_t)nu * ng];
+      for (int g = 0; g < ng; ++g) sg.lapl[g] += tmp[g] * l[g];
+    }
+    for (int nu = 0; nu < nb; ++nu) {
+      std::fill(tmp.begin(), tmp.end(), 0.0);
+      for (int mu = 0; mu < nb; ++mu) {
+        const double d = D[(size_t)nu * nb + mu];
+        const double* l = &sg.ao2l[(size_t)mu * ng];
+        for (int g = 0; g < ng; ++g) tmp[g] += d * l[g];
+      }
+      const double* a = &sg.ao[(size_t)nu * ng];
+      for (int g = 0; g < ng; ++g) sg.lapl[g] += tmp[g] * a[g];
+    }
+  }
+}
+
+void evaluate_subgrid(const ox_system* s, SubGrid& sg) {
+  const int ng = sg.npad;
+  sg.eps.assign(ng, 0); sg.vrho.assign(ng, 0); sg.vsigma.assign(ng, 0);
+  sg.vlapl.assign(ng, 0); sg.vtau.assign(ng, 0);
+  for (int code : s->codes)
+    for (int g = 0; g < ng; ++g) {
+      const double sig = s->family >= 1 ? sg.sigma[g] : 0.0;
+      const double tau = s->family >= 2 ? sg.tau[g] : 0.0;
+      oxf::eval_point(code, sg.rho[g], sig, tau, &sg.eps[g], &sg.vrho[g], &sg.vsigma[g], &sg.vtau[g]);
+    }
+}
+
+// Vxc contribution of one sub-grid (GridFock.cpp:11-49), added to F_ (nb x nb column-major).
+void fock_subgrid(const ox_system* s, const SubGrid& sg, double* Fout, std::vector<double>& F,
+                  std::vector<double>& tmp, std::vector<double>& tmp3) {
+  const int ng = sg.npad, nb = s->nbasis;
+  F.assign((size_t)nb * nb, 0);
+  tmp.assign(ng, 0);
+  // F0(mu,nu) = sum_g W v_rho AO(g,mu) AO(g,nu); F += 0.5 F0
+  for (int g = 0; g < ng; ++g) tmp[g] = sg.w[g] * sg.vrho[g];
+  for (int mu = 0; mu < nb; ++mu) {
+    std::vector<double> wa(ng);
+    const double* am = &sg.ao[(size_t)mu * ng];
+    for (int g = 0; g < ng; ++g) wa[g] = tmp[g] * am[g];
+    for (int nu = 0; nu < nb; ++nu) {
+      const double* an = &sg.ao[(size_t)nu * ng];
+      double acc = 0;
+      for (int g = 0; g < ng; ++g) acc += wa[g] * an[g];
+      F[(size_t)nu * nb + mu] += 0.5 * acc;
+    }
+  }
+  if (s->family >= 1) {
+    // F1(mu,nu) = sum_g W (2 v_sigma) sum_r Rho1(g,r) AO1(g,mu,r) AO(g,nu)
+    tmp3.assign((size_t)3 * ng, 0);
+    for (int r = 0; r < 3; ++r)
+      for (int g = 0; g < ng; ++g) tmp3[(size_t)r * ng + g] = sg.w[g] * 2.0 * sg.vsigma[g] * sg.rho1[(size_t)r * ng + g];
+    std::vector<double> wa(ng);
+    for (int mu = 0; mu < nb; ++mu) {
+      std::fill(wa.begin(), wa.end(), 0.0);
+      for (int r = 0; r < 3; ++r) {
+        const double* a1 = &sg.ao1[((size_t)r * nb + mu) * ng];
+        for (int g = 0; g < ng; ++g) wa[g] += tmp3[(size_t)r * ng + g] * a1[g];
+      }
+      for (int nu = 0; nu < nb; ++nu) {
+        const double* an = &sg.ao[(size_t)nu * ng];
+        double acc = 0;
+        for (int g = 0; g < ng; ++g) acc += wa[g] * an[g];
+        F[(size_t)nu * nb + mu] += acc;
+      }
+    }
+  }
+  if (s->family >= 2) {
+    // F2 = sum_g W (v_tau/2 + 2 v_lapl) AO1.AO1; F += 0.5 F2; then v_lapl AO AO2L
+    for (int g = 0; g < ng; ++g) tmp[g] = sg.w[g] * (0.5 * sg.vtau[g] + 2 * sg.vlapl[g]);
+    std::vector<double> wa(ng);
+    for (int mu = 0; mu < nb; ++mu)
+      for (int nu = 0; nu < nb; ++nu) {
+        double acc = 0;
+        for (int r = 0; r < 3; ++r) {
+          const double* am = &sg.ao1[((size_t)r * nb + mu) * ng];
+          const double* an = &sg.ao1[((size_t)r * nb + nu) * ng];
+          for (int g = 0; g < ng; ++g) acc += tmp[g] * am[g] * an[g];
+        }
+        F[(size_t)nu * nb + mu] += 0.5 * acc;
+      }
+    for (int g = 0; g < ng; ++g) tmp[g] = sg.w[g] * sg.vlapl[g];
+    bool any = false;
+    for (int g = 0; g < ng; ++g) any |= (tmp[g] != 0);
+    if (any)
+      for (int mu = 0; mu < nb; ++mu)
+        for (int nu = 0; nu < nb; ++nu) {
+          const double* am = &sg.ao[(size_t)mu * ng];
+          const double* ln = &sg.ao2l[(size_t)nu * ng];
+          double acc = 0;
+          for (int g = 0; g < ng; ++g) acc += tmp[g] * am[g] * ln[g];
+          F[(size_t)nu * nb + mu] += acc;
+        }
+  }
+  // F += F^T, then accumulate (all AOs kept: BasisList is the identity).
+  for (int nu = 0; nu < nb; ++nu)
+    for (int mu = 0; mu < nb; ++mu)
+      Fout[(size_t)nu * nb + mu] += F[(size_t)nu * nb + mu] + F[(size_t)mu * nb + nu];
+}
+
+template <class Fn> int guarded(Fn fn) {
+  try { fn(); return 0; } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ox_last_error(void) { return g_err.c_str(); }
+
+ox_system* ox_create(const char* data_root, const char* grid_name, const char* basis_name,
+                     int natoms, const int* Z, const double* xyz_bohr, int nthreads) {
+  ox_system* s = new ox_system;
+  int rc = guarded([&] {
+    s->natoms = natoms;
+    s->Z.assign(Z, Z + natoms);
+    s->xyz.assign(xyz_bohr, xyz_bohr + 3 * natoms);
+    s->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads();
+    const std::string root(data_root);
+    load_lebedev(root + "/lebedev_orbits.txt");
+    // basis
+    auto lib = read_gbs(root + "/BasisSets/" + basis_name + ".gbs");
+    for (int a = 0; a < natoms; ++a) {
+      auto it = lib.find(Z[a]);
+      if (it == lib.end()) throw std::runtime_error("Basis set file does not include this element!");
+      for (Shell sh : it->second) {
+        sh.center = a;
+        normalise(sh);
+        s->shells.push_back(sh);
+      }
+    }
+    s->nbasis = 0;
+    for (auto& sh : s->shells) {
+      s->shell_ang.push_back(angular_functions(sh.type));
+      s->nbasis += shell_size(sh.type);
+    }
+    // grid (GridGeneration.cpp:73-162)
+    std::vector<AtomGrid> ag(natoms);
+    long G = 0;
+    for (int a = 0; a < natoms; ++a) {
+      ag[a] = read_atom_grid(root + "/Grids/" + grid_name + "/" + elem_name(Z[a]) + ".grid");
+      for (int n : ag[a].order) {
+        if (!g_lebedev.count(n)) throw std::runtime_error("oracle: Lebedev order not tabulated");
+        G += n;
+      }
+    }
+    s->points.assign((size_t)4 * G, 0);
+    std::vector<long> start(natoms + 1, 0);
+    for (int a = 0; a < natoms; ++a) {
+      long n = 0; for (int k : ag[a].order) n += k;
+      start[a + 1] = start[a] + n;
+    }
+    for (int a = 0; a < natoms; ++a) {
+      long p = start[a];
+      const double* ca = &s->xyz[3 * a];
+      for (size_t ish = 0; ish < ag[a].order.size(); ++ish) {
+        const auto& leb = g_lebedev[ag[a].order[ish]];
+        for (const auto& l : leb) {
+          double* q = &s->points[4 * p];
+          q[0] = l[0] * ag[a].r[ish] + ca[0];
+          q[1] = l[1] * ag[a].r[ish] + ca[1];
+          q[2] = l[2] * ag[a].r[ish] + ca[2];
+          q[3] = ag[a].wr[ish] * 4 * M_PI * l[3];
+          ++p;
+        }
+      }
+    }
+    const int nat = natoms;
+    const double* X = s->xyz.data();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(s->nthreads)
+    for (long p = 0; p < G; ++p) {
+      int owner = 0;
+      while (p >= start[owner + 1]) ++owner;
+      double* q = &s->points[4 * p];
+      double total = 0, mine = 0;
+      for (int b = 0; b < nat; ++b) {
+        double pb = 1;
+        for (int c = 0; c < nat; ++c) {
+          if (b == c) continue;
+          pb *= becke_s(q[0], q[1], q[2], X + 3 * b, X + 3 * c);
+        }
+        total += pb;
+        if (b == owner) mine = pb;
+      }
+      q[3] *= mine / total;
+    }
+    // sub-grids
+    std::vector<std::pair<long, long>> parts;
+    recursive_halving(0, G, parts);
+    s->subgrids.resize(parts.size());
+    for (size_t i = 0; i < parts.size(); ++i) {
+      SubGrid& sg = s->subgrids[i];
+      sg.n = (int)parts[i].second;
+      sg.npad = (sg.n + 7) & ~7;
+      sg.x.resize(sg.npad); sg.y.resize(sg.npad); sg.z.resize(sg.npad); sg.w.assign(sg.npad, 0);
+      for (int g = 0; g < sg.npad; ++g) {
+        const long src = parts[i].first + (g < sg.n ? g : g - sg.n);
+        sg.x[g] = s->points[4 * src]; sg.y[g] = s->points[4 * src + 1]; sg.z[g] = s->points[4 * src + 2];
+        if (g < sg.n) sg.w[g] = s->points[4 * src + 3];
+      }
+    }
+    s->active.clear();
+    for (size_t i = 0; i < s->subgrids.size(); ++i) s->active.push_back((int)i);
+  });
+  if (rc) { delete s; return nullptr; }
+  return s;
+}
+
+void ox_destroy(ox_system* s) { delete s; }
+int ox_nbasis(const ox_system* s) { return s->nbasis; }
+long ox_npoints(const ox_system* s) { return (long)(s->points.size() / 4); }
+int ox_nsubgrids(const ox_system* s) { return (int)s->subgrids.size(); }
+int ox_nshells(const ox_system* s) { return (int)s->shells.size(); }
+
+int ox_get_points(const ox_system* s, double* xyzw) {
+  std::memcpy(xyzw, s->points.data(), s->points.size() * sizeof(double));
+  return 0;
+}
+
+int ox_get_subgrid_sizes(const ox_system* s, int* sizes) {
+  for (size_t i = 0; i < s->subgrids.size(); ++i) sizes[i] = s->subgrids[i].n;
+  return 0;
+}
+
+int ox_get_shells(const ox_system* s, int* type, int* nprim, int* center, double* exps, double* coefs) {
+  size_t k = 0;
+  for (size_t i = 0; i < s->shells.size(); ++i) {
+    type[i] = s->shells[i].type;
+    nprim[i] = (int)s->shells[i].exps.size();
+    center[i] = s->shells[i].center;
+    for (size_t p = 0; p < s->shells[i].exps.size(); ++p, ++k) {
+      exps[k] = s->shells[i].exps[p];
+      coefs[k] = s->shells[i].coefs[p];
+    }
+  }
+  return 0;
+}
+
+int ox_lebedev(const char* data_root, int n, double* xyzw) {
+  return guarded([&] {
+    load_lebedev(std::string(data_root) + "/lebedev_orbits.txt");
+    auto it = g_lebedev.find(n);
+    if (it == g_lebedev.end()) throw std::runtime_error("oracle: Lebedev order not tabulated");
+    for (size_t i = 0; i < it->second.size(); ++i)
+      for (int k = 0; k < 4; ++k) xyzw[4 * i + k] = it->second[i][k];
+  });
+}
+
+int ox_set_functional_codes(ox_system* s, int ncodes, const int* codes) {
+  return guarded([&] {
+    s->codes.assign(codes, codes + ncodes);
+    int fam = 0;
+    for (int c : s->codes) {
+      const int f = oxf::family_of(c);
+      if (f < 0) throw std::runtime_error("oracle: functional code not implemented");
+      fam = std::max(fam, f);
+    }
+    if (s->family != fam) s->ao_ready = false;
+    s->family = fam;
+  });
+}
+
+int ox_set_functional_file(ox_system* s, const char* data_root, const char* df_name) {
+  std::string path = std::string(data_root) + "/DensityFunctionals/" + df_name + ".df";
+  std::ifstream f(path);
+  if (!f.good()) { g_err = "Density functional file is missing!"; return -1; }
+  std::string line;
+  std::getline(f, line);
+  std::stringstream ss(line);
+  std::vector<int> codes;
+  int c;
+  while (ss >> c) codes.push_back(c);
+  if (codes.empty()) { g_err = "Corrupted density functional file!"; return -1; }
+  return ox_set_functional_codes(s, (int)codes.size(), codes.data());
+}
+
+int ox_family(const ox_system* s) { return s->family; }
+
+int ox_set_subgrid_subset(ox_system* s, int n, const int* ids) {
+  s->active.clear();
+  if (n <= 0) {
+    for (size_t i = 0; i < s->subgrids.size(); ++i) s->active.push_back((int)i);
+  } else {
+    for (int i = 0; i < n; ++i) {
+      if (ids[i] < 0 || ids[i] >= (int)s->subgrids.size()) { g_err = "bad sub-grid id"; return -1; }
+      s->active.push_back(ids[i]);
+    }
+  }
+  s->ao_ready = false;
+  return 0;
+}
+
+int ox_get_ao(ox_system* s) {
+  return guarded([&] {
+    if (s->family < 0) throw std::runtime_error("oracle: set a functional first");
+    // free AO tables of inactive sub-grids
+    std::vector<char> act(s->subgrids.size(), 0);
+    for (int i : s->active) act[i] = 1;
+    for (size_t i = 0; i < s->subgrids.size(); ++i)
+      if (!act[i]) { s->subgrids[i].ao.clear(); s->subgrids[i].ao.shrink_to_fit(); s->subgrids[i].ao1.clear(); s->subgrids[i].ao1.shrink_to_fit(); s->subgrids[i].ao2l.clear(); }
+    const int n = (int)s->active.size();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(s->nthreads)
+    for (int i = 0; i < n; ++i) compute_ao(s, s->subgrids[s->active[i]], s->family);
+    s->ao_ready = true;
+  });
+}
+
+int ox_density(ox_system* s, const double* D) {
+  if (!s->ao_ready) { g_err = "oracle: AO tables not ready"; return -1; }
+  const int n = (int)s->active.size();
+#pragma omp parallel num_threads(s->nthreads)
+  {
+    std::vector<double> tmp;
+#pragma omp for schedule(dynamic, 1)
+    for (int i = 0; i < n; ++i) density_subgrid(s, s->subgrids[s->active[i]], D, s->family, tmp);
+  }
+  return 0;
+}
+
+int ox_evaluate(ox_system* s) {
+  const int n = (int)s->active.size();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(s->nthreads)
+  for (int i = 0; i < n; ++i) evaluate_subgrid(s, s->subgrids[s->active[i]]);
+  return 0;
+}
+
+int ox_energy(ox_system* s, double* Exc) {
+  double E = 0;
+  const int n = (int)s->active.size();
+#pragma omp parallel for reduction(+ : E) schedule(static) num_threads(s->nthreads)
+  for (int i = 0; i < n; ++i) {
+    const SubGrid& sg = s->subgrids[s->active[i]];
+    for (int g = 0; g < sg.npad; ++g) E += sg.w[g] * sg.rho[g] * sg.eps[g];
+  }
+  *Exc = E;
+  return 0;
+}
+
+int ox_num_electrons(ox_system* s, double* Nel) {
+  double N = 0;
+  const int n = (int)s->active.size();
+#pragma omp parallel for reduction(+ : N) schedule(static) num_threads(s->nthreads)
+  for (int i = 0; i < n; ++i) {
+    const SubGrid& sg = s->subgrids[s->active[i]];
+    for (int g = 0; g < sg.npad; ++g) N += sg.w[g] * sg.rho[g];
+  }
+  *Nel = N;
+  return 0;
+}
+
+int ox_fock(ox_system* s, double* V) {
+  const int nb = s->nbasis;
+  const int n = (int)s->active.size();
+  std::memset(V, 0, sizeof(double) * nb * nb);
+  const int nt = s->nthreads;
+  std::vector<std::vector<double>> part(nt, std::vector<double>((size_t)nb * nb, 0.0));
+#pragma omp parallel num_threads(nt)
+  {
+    std::vector<double> F, tmp, tmp3;
+    double* mine = part[omp_get_thread_num()].data();
+#pragma omp for schedule(dynamic, 1)
+    for (int i = 0; i < n; ++i) fock_subgrid(s, s->subgrids[s->active[i]], mine, F, tmp, tmp3);
+  }
+  for (int t = 0; t < nt; ++t)
+    for (size_t k = 0; k < (size_t)nb * nb; ++k) V[k] += part[t][k];
+  return 0;
+}
+
+int ox_xc(ox_system* s, const double* D, double* V, double* Exc, double* Nel) {
+  if (ox_density(s, D)) return -1;
+  if (ox_evaluate(s)) return -1;
+  if (ox_energy(s, Exc)) return -1;
+  if (Nel && ox_num_electrons(s, Nel)) return -1;
+  return ox_fock(s, V);
+}
+
+int ox_eval_functional(int ncodes, const int* codes, long n,
+                       const double* rho, const double* sigma, const double* lapl, const double* tau,
+                       double* exc, double* vrho, double* vsigma, double* vlapl, double* vtau) {
+  (void)lapl;
+  for (long i = 0; i < n; ++i) {
+    double e = 0, vr = 0, vs = 0, vt = 0;
+    for (int k = 0; k < ncodes; ++k)
+      oxf::eval_point(codes[k], rho[i], sigma ? sigma[i] : 0.0, tau ? tau[i] : 0.0, &e, &vr, &vs, &vt);
+    if (exc) exc[i] = e;
+    if (vrho) vrho[i] = vr;
+    if (vsigma) vsigma[i] = vs;
+    if (vlapl) vlapl[i] = 0;
+    if (vtau) vtau[i] = vt;
+  }
+  return 0;
+}
+
+int ox_ao_values(const ox_system* s, long n, const double* xyz, double* ao, double* grad, double* lapl) {
+  return guarded([&] {
+    SubGrid sg;
+    sg.npad = (int)n;
+    sg.x.resize(n); sg.y.resize(n); sg.z.resize(n);
+    for (long i = 0; i < n; ++i) { sg.x[i] = xyz[3 * i]; sg.y[i] = xyz[3 * i + 1]; sg.z[i] = xyz[3 * i + 2]; }
+    compute_ao(const_cast<ox_system*>(s), sg, 2);
+    std::memcpy(ao, sg.ao.data(), sizeof(double) * sg.ao.size());
+    if (grad) std::memcpy(grad, sg.ao1.data(), sizeof(double) * sg.ao1.size());
+    if (lapl) std::memcpy(lapl, sg.ao2l.data(), sizeof(double) * sg.ao2l.size());
+  });
+}
+
+long ox_active_points(const ox_system* s) {
+  long n = 0;
+  for (int i : s->active) n += s->subgrids[i].n;
+  return n;
+}
+
+int ox_get_density(const ox_system* s, double* rho, double* grad, double* sigma, double* tau, double* w) {
+  long k = 0;
+  for (int i : s->active) {
+    const SubGrid& sg = s->subgrids[i];
+    for (int g = 0; g < sg.n; ++g, ++k) {
+      if (rho) rho[k] = sg.rho[g];
+      if (grad && !sg.rho1.empty()) for (int r = 0; r < 3; ++r) grad[3 * k + r] = sg.rho1[(size_t)r * sg.npad + g];
+      if (sigma && !sg.sigma.empty()) sigma[k] = sg.sigma[g];
+      if (tau && !sg.tau.empty()) tau[k] = sg.tau[g];
+      if (w) w[k] = sg.w[g];
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+/*
+ * FP64 CPU oracle for the Kohn-Sham exchange-correlation path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: the tests, the
+ * smoke check and the CPU-baseline leg of the benchmark call it; the shipped
+ * library (chinium_amd) never links, loads or calls it.
+ *
+ * It restates, loop nest by loop nest, what the reference does on one SCF
+ * iteration (src/Grid + src/ExchangeCorrelation of Chinium):
+ *   grid generation      src/Grid/GridGeneration.cpp:27-162,203-230
+ *   basis normalisation  src/Integral/Normalization.cpp:11-18 (libint2 renorm)
+ *   AO tables            src/Grid/GridAO.cpp:14-290
+ *   density              src/Grid/GridDensity.cpp:12-45
+ *   functional           src/ExchangeCorrelation.cpp:165-248 (libxc replaced
+ *                        by the project's own FP64 functionals, oracle side)
+ *   energy               src/Grid/GridEnergy.cpp:11-13
+ *   Vxc                  src/Grid/GridFock.cpp:11-49, src/Grid/Grid.cpp:156-171
+ *
+ * Parity status of each part is recorded in oracle/README.md.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ox_system ox_system;
+
+/* Build molecule + basis + grid. Coordinates in bohr, row-major natoms x 3.
+ * Reads <data_root>/Grids/<grid>/<El>.grid and <data_root>/BasisSets/<basis>.gbs.
+ * Returns NULL on failure (message via ox_last_error). */
+ox_system* ox_create(const char* data_root, const char* grid_name, const char* basis_name,
+                     int natoms, const int* Z, const double* xyz_bohr, int nthreads);
+void ox_destroy(ox_system* s);
+const char* ox_last_error(void);
+
+int ox_nbasis(const ox_system* s);
+long ox_npoints(const ox_system* s);      /* unpadded points, generation order */
+int ox_nsubgrids(const ox_system* s);
+/* Points in generation order: xyzw[4*i + {0,1,2,3}]. */
+int ox_get_points(const ox_system* s, double* xyzw);
+/* Sub-grid sizes (unpadded) in order. */
+int ox_get_subgrid_sizes(const ox_system* s, int* sizes);
+/* Normalised contraction coefficients, flattened shell by shell (for tests). */
+int ox_nshells(const ox_system* s);
+int ox_get_shells(const ox_system* s, int* type, int* nprim, int* center, double* exps, double* coefs);
+
+/* Lebedev rule with n points (x, y, z, w), expanded from <data_root>/lebedev_orbits.txt. */
+int ox_lebedev(const char* data_root, int n, double* xyzw);
+
+/* Functional: codes are libxc IDs. Family: 0 LDA, 1 GGA, 2 mGGA. */
+int ox_set_functional_codes(ox_system* s, int ncodes, const int* codes);
+int ox_set_functional_file(ox_system* s, const char* data_root, const char* df_name);
+int ox_family(const ox_system* s);
+
+/* Restrict all following work to a subset of sub-grids (for timing samples).
+ * n <= 0 restores the full set. */
+int ox_set_subgrid_subset(ox_system* s, int n, const int* ids);
+
+/* Cache AO tables (value, gradient, and Laplacian when mGGA) for the active
+ * sub-grids, as the reference does once per SCF (Grid::getAO(0,1)). */
+int ox_get_ao(ox_system* s);
+
+/* One XC build: getDensity(D) -> Evaluate("ev") -> getEnergy -> getFock.
+ * D and V are nbasis x nbasis column-major (symmetric). */
+int ox_xc(ox_system* s, const double* D, double* V, double* Exc, double* Nel);
+
+/* Individual phases, for timing. */
+int ox_density(ox_system* s, const double* D);
+int ox_evaluate(ox_system* s);
+int ox_energy(ox_system* s, double* Exc);
+int ox_num_electrons(ox_system* s, double* Nel);
+int ox_fock(ox_system* s, double* V);
+
+/* Per-point functional, libxc conventions (e per particle, derivatives of rho*e).
+ * Arrays may be NULL when not needed by the family. */
+int ox_eval_functional(int ncodes, const int* codes, long n,
+                       const double* rho, const double* sigma, const double* lapl, const double* tau,
+                       double* exc, double* vrho, double* vsigma, double* vlapl, double* vtau);
+
+/* AO values at arbitrary points (n points): ao[mu*n + i], grad[(r*nbasis + mu)*n + i],
+ * lapl[mu*n + i]. grad/lapl may be NULL. */
+int ox_ao_values(const ox_system* s, long n, const double* xyz, double* ao, double* grad, double* lapl);
+
+/* Density quantities on the active sub-grids after ox_density, in the
+ * sub-grid order with padding removed: rho, grad (3 per point), sigma, tau. */
+long ox_active_points(const ox_system* s);
+int ox_get_density(const ox_system* s, double* rho, double* grad, double* sigma, double* tau, double* w);
+
+#ifdef __cplusplus
+}
+#endif

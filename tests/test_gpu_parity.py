@@ -1,0 +1,138 @@
+"""GPU parity: the FP64 plain form against the FP64 oracle, through the C ABI.
+
+Bar (plain FP64 form): |E_xc - E_oracle| <= 1e-10 Eh, max |V - V_oracle| <= 1e-9
+elementwise, on identical inputs (same molecule, basis, grid file, D).
+"""
+import numpy as np
+import pytest
+
+import chinium_amd as ca
+import oracle_lib as ol
+from chinium_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-10
+V_TOL = 1e-9
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def h2o():
+    w = wl.build_workload("h2o_ccpvdz_sg1_pbe")
+    mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
+    D, S = wl.workload_density(mol, w["nocc"], w["seed"])
+    _say("h2o: building grid on device")
+    grid = ca.Grid(mol, w["grid"], 1, 0)
+    grid.setType(1)
+    grid.getAO(0, 0)
+    orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], w["grid"], w["basis"])
+    return dict(w=w, mol=mol, D=D, S=S, grid=grid, orc=orc)
+
+
+def test_mfma_layout_selftest():
+    _say("selftest: FP64 MFMA 16x16x4 layout")
+    assert ca.selftest() == 0.0
+
+
+def test_grid_points_match_oracle(h2o):
+    """Same point set and weights (zero weights dropped) as the reference generation."""
+    P = h2o["grid"].points()
+    Po = h2o["orc"].points()
+    Po = Po[Po[:, 3] != 0.0]
+    assert len(P) == len(Po)
+    key = lambda A: np.lexsort((A[:, 2], A[:, 1], A[:, 0]))
+    A, B = P[key(P)], Po[key(Po)]
+    assert np.abs(A[:, :3] - B[:, :3]).max() < 1e-12
+    # Becke fractions are computed from r_B - r_C, which cancels for points far
+    # from the molecule (em radial grids reach r ~ 1e3 bohr) and for fractions
+    # near 0 ((1 - p)/2 with p -> 1); allow that rounding, relatively, with an
+    # absolute floor.
+    dw = np.abs(A[:, 3] - B[:, 3])
+    _say("  max rel dw %.2e max abs dw (w<1) %.2e" % ((dw / np.abs(B[:, 3])).max(), dw[np.abs(B[:, 3]) < 1].max()))
+    assert np.all((dw <= 1e-10 * np.abs(B[:, 3])) | (dw <= 1e-12))
+
+
+@pytest.mark.parametrize("func", ["PBE", "B3LYP", "SLATER", "SVWN5", "BLYP", "B88"])
+def test_h2o_xc_parity(h2o, func):
+    _say("parity h2o/cc-pvdz/SG-1 %s" % func)
+    xc = ca.ExchangeCorrelation()
+    xc.Read(func)
+    E, V = ca.xc_build(h2o["grid"], xc, h2o["D"])
+    N = h2o["grid"].getNumElectrons()
+    orc = h2o["orc"]
+    orc.set_functional(func)
+    orc.get_ao()
+    Eo, No, Vo = orc.xc(h2o["D"])
+    _say("  E gpu %.14f oracle %.14f dE %.2e dV %.2e dN %.2e" % (E, Eo, abs(E - Eo), np.abs(V - Vo).max(), abs(N - No)))
+    assert abs(E - Eo) <= E_TOL
+    assert np.abs(V - Vo).max() <= V_TOL
+    assert abs(N - No) <= 1e-10
+    assert np.abs(V - V.T).max() <= 1e-12
+
+
+def test_water_cluster_parity():
+    """(H2O)_4 / def2-SVP / SG-1 / B3LYP: several molecules, AO permutation and screening active."""
+    Z, X = wl.water_cluster_angstrom(4, 5.0, 11)
+    xyz = X * ca.ANGSTROM_TO_BOHR
+    mol = ca.Molecule(Z, xyz, "def2-svp")
+    D, S = wl.workload_density(mol, 20, 3)
+    _say("parity (H2O)4/def2-svp/SG-1 B3LYP: grid")
+    grid = ca.Grid(mol, "SG-1", 1, 0)
+    grid.setType(1)
+    grid.getAO(0, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read("B3LYP")
+    E, V = ca.xc_build(grid, xc, D)
+    orc = ol.OracleSystem(Z, xyz, "SG-1", "def2-svp")
+    orc.set_functional("B3LYP")
+    orc.get_ao()
+    Eo, No, Vo = orc.xc(D)
+    _say("  E gpu %.14f oracle %.14f dE %.2e dV %.2e" % (E, Eo, abs(E - Eo), np.abs(V - Vo).max()))
+    assert abs(E - Eo) <= E_TOL
+    assert np.abs(V - Vo).max() <= V_TOL
+
+
+def test_set_vrho_fock(h2o):
+    """Externally supplied v_rho (SAP guess path): V = sum_g w v phi phi."""
+    grid = h2o["grid"]
+    P = grid.points()
+    r = np.linalg.norm(P[:, :3], axis=1)
+    v = -1.0 / np.sqrt(r * r + 0.5)
+    grid.setVrho(v)
+    V = grid.getFock()[0]
+    ao, _, _ = h2o["orc"].ao_values(P[:, :3])
+    Vref = (ao * (P[:, 3] * v)) @ ao.T
+    _say("  set_vrho dV %.2e" % np.abs(V - Vref).max())
+    assert np.abs(V - Vref).max() <= 1e-9
+
+
+def test_protocol_errors(h2o):
+    grid = ca.Grid(h2o["mol"], "SG-1", 1, 0)
+    grid.setType(1)
+    with pytest.raises(ca.ChxError) as e:
+        grid.getDensity([h2o["D"]])
+    assert e.value.code == -3
+    grid.getAO(0, 0)
+    with pytest.raises(ca.ChxError):
+        grid.getEnergy()
+    grid.getDensity([h2o["D"]])
+    with pytest.raises(ca.ChxError) as e:
+        grid.getDensity([h2o["D"], h2o["D"]])
+    assert e.value.code == -5
+    with pytest.raises(ca.ChxError):
+        grid.getFock()
+
+
+def test_grid_copy_independent(h2o):
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    g2 = h2o["grid"].copy()
+    E1, V1 = ca.xc_build(h2o["grid"], xc, h2o["D"])
+    E2, V2 = ca.xc_build(g2, xc, h2o["D"])
+    # V is accumulated with FP64 atomics, so it is reproducible to rounding only.
+    assert abs(E1 - E2) <= 1e-13
+    assert np.abs(V1 - V2).max() <= 1e-13
