@@ -15,6 +15,7 @@ void mol_overlap(const chx_mol* m, double* S);
 namespace chx {
 void grid_build(chx_grid* g, const std::string& grid_name, const std::string& data_root);
 void grid_plan(chx_grid* g);
+void ensure_stream(chx_grid* g);
 void launch_selftest(hipStream_t s, double* out);
 }  // namespace chx
 int chx_lebedev_rule_impl(int n, double* xyzw);
@@ -77,12 +78,7 @@ void require(bool ok, int code, const char* msg) {
   if (!ok) throw Error(code, msg);
 }
 
-void make_stream(chx_grid* g) {
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(CHX_ERR_DEVICE, "no HIP device available");
-  CHX_HIP(hipGetDevice(&g->device));
-  CHX_HIP(hipStreamCreate(&g->stream));
-}
+void make_stream(chx_grid* g) { ensure_stream(g); }
 
 void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
   g->d_codes.upload(xc->codes, g->stream);
@@ -209,8 +205,7 @@ int chx_grid_create(const chx_mol* m, const char* grid_name, const char* data_ro
     g = new chx_grid;
     g->mol = *m;
     g->nthreads = nthreads > 0 ? nthreads : 1;
-    make_stream(g);
-    grid_build(g, grid_name, data_root);
+    grid_build(g, grid_name, data_root);  // reads the grid files, then creates the stream
     if (output)
       std::printf("Grid %s: %ld points in %d batches of %d; %d AOs in %d tiles\n", grid_name, g->npoints, g->nbatches,
                   kBatch, g->nbasis, g->ntiles);

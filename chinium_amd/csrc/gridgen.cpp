@@ -159,6 +159,14 @@ uint64_t morton3(double fx, double fy, double fz) {
 
 const std::vector<std::array<double, 4>>& lebedev_rule(int n) { return lebedev(n); }
 
+void ensure_stream(chx_grid* g) {
+  if (g->stream) return;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(CHX_ERR_DEVICE, "no HIP device available");
+  CHX_HIP(hipGetDevice(&g->device));
+  CHX_HIP(hipStreamCreate(&g->stream));
+}
+
 // Build points, weights, batches and the internal AO layout.
 void grid_build(chx_grid* g, const std::string& grid_name, const std::string& data_root) {
   const chx_mol& m = g->mol;
@@ -185,7 +193,8 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
     }
   }
   const long G = (long)px.size();
-  // ---- Becke weights on the device (FP64)
+  // ---- Becke weights on the device (FP64); the data files are all read by now
+  ensure_stream(g);
   {
     std::vector<double> inv(nat * (size_t)nat, 0.0);
     for (int b = 0; b < nat; ++b)

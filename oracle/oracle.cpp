@@ -98,6 +98,7 @@ struct ox_system {
   int nbasis = 0;
   std::vector<std::vector<Poly>> shell_ang;
   std::vector<double> points;   // 4*G generation order
+  std::vector<double> raw_w;    // radial x angular weight before the Becke partition
   std::vector<SubGrid> subgrids;
   std::vector<int> active;
   std::vector<int> codes;
@@ -574,6 +575,8 @@ ox_system* ox_create(const char* data_root, const char* grid_name, const char* b
         }
       }
     }
+    s->raw_w.resize(G);
+    for (long p = 0; p < G; ++p) s->raw_w[p] = s->points[4 * p + 3];
     const int nat = natoms;
     const double* X = s->xyz.data();
 #pragma omp parallel for schedule(dynamic, 1024) num_threads(s->nthreads)
@@ -620,6 +623,11 @@ int ox_nbasis(const ox_system* s) { return s->nbasis; }
 long ox_npoints(const ox_system* s) { return (long)(s->points.size() / 4); }
 int ox_nsubgrids(const ox_system* s) { return (int)s->subgrids.size(); }
 int ox_nshells(const ox_system* s) { return (int)s->shells.size(); }
+
+int ox_get_raw_weights(const ox_system* s, double* w) {
+  std::memcpy(w, s->raw_w.data(), s->raw_w.size() * sizeof(double));
+  return 0;
+}
 
 int ox_get_points(const ox_system* s, double* xyzw) {
   std::memcpy(xyzw, s->points.data(), s->points.size() * sizeof(double));

@@ -40,6 +40,8 @@ long ox_npoints(const ox_system* s);      /* unpadded points, generation order *
 int ox_nsubgrids(const ox_system* s);
 /* Points in generation order: xyzw[4*i + {0,1,2,3}]. */
 int ox_get_points(const ox_system* s, double* xyzw);
+/* Radial x angular weights before the Becke partition, generation order. */
+int ox_get_raw_weights(const ox_system* s, double* w);
 /* Sub-grid sizes (unpadded) in order. */
 int ox_get_subgrid_sizes(const ox_system* s, int* sizes);
 /* Normalised contraction coefficients, flattened shell by shell (for tests). */

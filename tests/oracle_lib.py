@@ -33,6 +33,7 @@ def lib():
         "ox_nsubgrids": (ctypes.c_int, [_vp]),
         "ox_nshells": (ctypes.c_int, [_vp]),
         "ox_get_points": (ctypes.c_int, [_vp, _vp]),
+        "ox_get_raw_weights": (ctypes.c_int, [_vp, _vp]),
         "ox_get_subgrid_sizes": (ctypes.c_int, [_vp, _vp]),
         "ox_get_shells": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "ox_lebedev": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _vp]),
@@ -95,6 +96,11 @@ class OracleSystem:
         P = np.zeros((self.npoints, 4))
         lib().ox_get_points(self.h, _ptr(P))
         return P
+
+    def raw_weights(self):
+        w = np.zeros(self.npoints)
+        lib().ox_get_raw_weights(self.h, _ptr(w))
+        return w
 
     def subgrid_sizes(self):
         s = np.zeros(self.nsubgrids, dtype=np.int32)
