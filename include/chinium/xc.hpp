@@ -1,0 +1,198 @@
+// chinium/xc.hpp - header-only C++ drop-in for the reference's Grid and
+// ExchangeCorrelation classes (src/Grid/Grid.h:90-116, src/ExchangeCorrelation.h:8-17),
+// implemented over the C ABI in chinium_xc.h (libchinium_xc.so, gfx950).
+//
+// Same member names, argument meaning, call order and failure behaviour:
+// failures throw std::runtime_error carrying the reference's message
+// (e.g. "Density functional file is missing!"), data files are found through
+// $CHINIUM_PATH. The per-iteration protocol is the reference's
+// (Restricted/SP.cpp:52-55):
+//     grid.getDensity({2 * D}); xc.Evaluate("ev", grid);
+//     double E = grid.getEnergy(); EigenMatrix V = grid.getFock()[0];
+//
+// Matrices: with Eigen present, EigenMatrix is Eigen::MatrixXd as in the
+// reference (src/Macro/Abbreviation.h:4); otherwise a minimal column-major
+// matrix with the same data()/rows()/cols() accessors.
+#pragma once
+
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../chinium_xc.h"
+
+#if __has_include(<Eigen/Core>)
+#include <Eigen/Core>
+namespace chinium {
+using EigenMatrix = Eigen::MatrixXd;
+inline EigenMatrix make_matrix(int r, int c) { return EigenMatrix::Zero(r, c); }
+}  // namespace chinium
+#else
+namespace chinium {
+class EigenMatrix {
+ public:
+  EigenMatrix() = default;
+  EigenMatrix(int r, int c) : r_(r), c_(c), d_((size_t)r * c, 0.0) {}
+  int rows() const { return r_; }
+  int cols() const { return c_; }
+  double* data() { return d_.data(); }
+  const double* data() const { return d_.data(); }
+  double& operator()(int i, int j) { return d_[(size_t)j * r_ + i]; }
+  double operator()(int i, int j) const { return d_[(size_t)j * r_ + i]; }
+
+ private:
+  int r_ = 0, c_ = 0;
+  std::vector<double> d_;
+};
+inline EigenMatrix make_matrix(int r, int c) { return EigenMatrix(r, c); }
+}  // namespace chinium
+#endif
+
+namespace chinium {
+
+inline void check(int rc) {
+  if (rc != CHX_OK) throw std::runtime_error(chx_last_error());
+}
+
+inline std::string data_root() {
+  const char* p = std::getenv("CHINIUM_PATH");
+  if (!p) throw std::runtime_error("CHINIUM_PATH is not set");
+  return p;
+}
+
+// The Mwfn fields the XC path reads: centres (Z, bohr) and normalised shells.
+class Molecule {
+ public:
+  Molecule(const std::vector<int>& Z, const std::vector<double>& xyz_bohr) {
+    check(chx_mol_create((int)Z.size(), Z.data(), xyz_bohr.data(), &h_));
+  }
+  // Mwfn::setBasis + Normalize (legacy/MwfnIO/Mwfn.cpp:686; src/Integral/Normalization.cpp:11).
+  void setBasis(const std::string& name) { check(chx_mol_set_basis_file(h_, data_root().c_str(), name.c_str())); }
+  // One MwfnShell: Type, Exponents, NormalizedCoefficients.
+  void addShell(int center, int type, const std::vector<double>& exps, const std::vector<double>& norm_coefs) {
+    check(chx_mol_add_shell(h_, center, type, (int)exps.size(), exps.data(), norm_coefs.data()));
+  }
+  int getNumBasis() const {
+    int n = 0;
+    check(chx_mol_nbasis(h_, &n));
+    return n;
+  }
+  const chx_mol* handle() const { return h_; }
+  ~Molecule() { chx_mol_destroy(h_); }
+  Molecule(const Molecule&) = delete;
+  Molecule& operator=(const Molecule&) = delete;
+
+ private:
+  chx_mol* h_ = nullptr;
+};
+
+class Grid {
+ public:
+  Grid() = default;
+  // Grid::Grid(Mwfn*, std::string grid, int nthreads, int output)
+  Grid(const Molecule* mol, std::string grid, int nthreads, int output) : nbasis_(mol->getNumBasis()) {
+    if (grid.empty()) return;  // as the reference: an empty name builds nothing
+    check(chx_grid_create(mol->handle(), grid.c_str(), data_root().c_str(), nthreads, output, &h_));
+  }
+  Grid(const Grid& o) : nbasis_(o.nbasis_) {
+    if (o.h_) check(chx_grid_copy(o.h_, &h_));
+  }
+  Grid& operator=(const Grid& o) {
+    if (this != &o) {
+      Grid tmp(o);
+      std::swap(h_, tmp.h_);
+      std::swap(nbasis_, tmp.nbasis_);
+    }
+    return *this;
+  }
+  ~Grid() { chx_grid_destroy(h_); }
+
+  int getNumThreads() const {
+    int n = 0;
+    check(chx_grid_num_threads(h_, &n));
+    return n;
+  }
+  void setType(int type) { check(chx_grid_set_type(h_, type)); }
+  void getAO(int derivative, int output) { check(chx_grid_get_ao(h_, derivative, output)); }
+  void getDensity(std::vector<EigenMatrix> D) {
+    const int nspin = (int)D.size();
+    std::vector<double> buf;
+    buf.reserve((size_t)nspin * nbasis_ * nbasis_);
+    for (const EigenMatrix& m : D) buf.insert(buf.end(), m.data(), m.data() + (size_t)nbasis_ * nbasis_);
+    check(chx_grid_get_density(h_, buf.data(), nspin, nbasis_));
+  }
+  double getNumElectrons() {
+    double n = 0;
+    check(chx_grid_num_electrons(h_, &n));
+    return n;
+  }
+  double getEnergy() {
+    double e = 0;
+    check(chx_grid_energy(h_, &e));
+    return e;
+  }
+  std::vector<EigenMatrix> getFock() {
+    std::vector<EigenMatrix> F(1, make_matrix(nbasis_, nbasis_));
+    check(chx_grid_fock(h_, F[0].data(), 1, nbasis_));
+    return F;
+  }
+  chx_grid* handle() { return h_; }
+
+ private:
+  chx_grid* h_ = nullptr;
+  int nbasis_ = 0;
+};
+
+class ExchangeCorrelation {
+ public:
+  std::vector<int> Codes;
+  double EXX = 1;
+  int Spin = 1;  // 1 - Unpolarized, 2 - Polarized
+  std::string Family;
+
+  void Read(std::string df, bool output) {
+    chx_xc* h = nullptr;
+    check(chx_xc_read(data_root().c_str(), df.c_str(), Spin, output ? 1 : 0, &h));
+    reset(h);
+  }
+  void Evaluate(std::string order, Grid& grid) { check(chx_xc_evaluate(h_, grid.handle(), order.c_str())); }
+  explicit operator bool() const { return Codes.size() > 0; }
+
+  ExchangeCorrelation() = default;
+  ExchangeCorrelation(const ExchangeCorrelation& o) : Codes(o.Codes), EXX(o.EXX), Spin(o.Spin), Family(o.Family) {
+    if (!Codes.empty()) {
+      chx_xc* h = nullptr;
+      check(chx_xc_create((int)Codes.size(), Codes.data(), Spin, &h));
+      h_ = h;
+    }
+  }
+  ExchangeCorrelation& operator=(const ExchangeCorrelation& o) {
+    if (this != &o) {
+      ExchangeCorrelation tmp(o);
+      std::swap(h_, tmp.h_);
+      Codes = o.Codes;
+      EXX = o.EXX;
+      Spin = o.Spin;
+      Family = o.Family;
+    }
+    return *this;
+  }
+  ~ExchangeCorrelation() { chx_xc_destroy(h_); }
+
+ private:
+  chx_xc* h_ = nullptr;
+  void reset(chx_xc* h) {
+    chx_xc_destroy(h_);
+    h_ = h;
+    int family = 0, n = 0;
+    check(chx_xc_info(h_, &family, &EXX, &n, nullptr));
+    Codes.assign(n, 0);
+    check(chx_xc_info(h_, nullptr, nullptr, nullptr, Codes.data()));
+    static const char* names[] = {"LDA", "GGA", "mGGA"};
+    Family = names[family];
+  }
+};
+
+}  // namespace chinium
