@@ -30,6 +30,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="h2o16_def2svp_sg2_b3lyp")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ao-cache", type=int, default=-1, help="-1 auto, 0 recompute AO values, 1 cache them")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -51,6 +52,7 @@ def main():
     grid = ca.Grid(mol, w["grid"], 1, 0)
     xc = ca.ExchangeCorrelation()
     xc.Read(w["functional"])
+    grid.setAOCache(args.ao_cache)
     grid.setType(xc.family)
     grid.getAO(0, 0)
     t_setup = time.time() - t0
@@ -120,6 +122,7 @@ def main():
         "config": {"workload": "(H2O)16 / def2-SVP / SG-2 / B3LYP", "name": args.workload,
                    "points": st["points"], "nbasis": n, "batches": st["batches"],
                    "mean_alive_tiles_per_batch": round(st["mean_tiles"], 2), "form": "plain FP64 (chosen = plain)",
+                   "ao_tables": "cached in HBM" if grid.aoCached() else "recomputed in kernels",
                    "screening_threshold": 1e-14},
         "tflops": round(flops_build / (ms_step * 1e-3) / 1e12, 3),
         "gpoints_per_s": round(st["points"] / (ms_step * 1e-3) / 1e9, 4),

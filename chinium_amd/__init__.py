@@ -77,6 +77,8 @@ def lib():
         "chx_grid_set_screening": (i, [_vp, d]),
         "chx_grid_stats": (i, [_vp, P(l), P(l), P(d), P(d)]),
         "chx_grid_profile": (i, [_vp, i]),
+        "chx_grid_set_ao_cache": (i, [_vp, i]),
+        "chx_grid_ao_cached": (i, [_vp, P(i)]),
         "chx_grid_kernel_times": (i, [_vp, i, _vp, _vp]),
         "chx_xc_read": (i, [cp, cp, i, i, P(_vp)]),
         "chx_xc_create": (i, [i, _vp, i, P(_vp)]),
@@ -258,15 +260,24 @@ class Grid:
         _check(lib().chx_grid_stream(self.h, ctypes.byref(s)))
         return s.value
 
-    STAGES = ("permute_d", "density", "functional", "vxc", "finalize_v")
+    STAGES = ("permute_d", "density", "functional", "vxc", "finalize_v", "make_b")
+
+    def setAOCache(self, mode):
+        """-1 automatic (default), 0 recompute AO values in the kernels, 1 cache them in HBM."""
+        _check(lib().chx_grid_set_ao_cache(self.h, int(mode)))
+
+    def aoCached(self):
+        c = ctypes.c_int()
+        _check(lib().chx_grid_ao_cached(self.h, ctypes.byref(c)))
+        return bool(c.value)
 
     def profile(self, enable=True):
         _check(lib().chx_grid_profile(self.h, int(bool(enable))))
 
     def kernel_times(self, reset=True):
         """Accumulated HIP-event milliseconds and launch counts per stage."""
-        ms = np.zeros(5)
-        n = np.zeros(5, dtype=np.int64)
+        ms = np.zeros(6)
+        n = np.zeros(6, dtype=np.int64)
         _check(lib().chx_grid_kernel_times(self.h, int(bool(reset)), _ptr(ms), _ptr(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
 

@@ -91,7 +91,7 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
 }
 
 // Stage ids of chx_grid_kernel_times.
-enum { kStPermute = 0, kStDensity = 1, kStFunctional = 2, kStVxc = 3, kStFinalize = 4 };
+enum { kStPermute = 0, kStDensity = 1, kStFunctional = 2, kStVxc = 3, kStFinalize = 4, kStMakeB = 5 };
 
 void density_from_device(chx_grid* g, const double* dD) {
   hipEvent_t e = nullptr;
@@ -99,9 +99,13 @@ void density_from_device(chx_grid* g, const double* dD) {
   launch_permute_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dint.p);
   g->mark_end(kStPermute, e);
   g->mark_begin(kStDensity, &e);
-  launch_density_f64(g->stream, (int)g->grp_batch.size(), g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p,
-                     g->d_bt_off.p, g->d_bt_list.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_Dint.p, g->npad,
-                     g->d_part.p);
+  if (g->cached)
+    launch_density_c64(g->stream, (int)g->grp_batch.size(), g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p,
+                       g->d_grp_ntiles.p, g->d_bt_off.p, g->d_bt_list.p, g->d_cache.p, g->d_Dint.p, g->npad, g->d_part.p);
+  else
+    launch_density_f64(g->stream, (int)g->grp_batch.size(), g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p,
+                       g->d_bt_off.p, g->d_bt_list.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_Dint.p, g->npad,
+                       g->d_part.p);
   g->mark_end(kStDensity, e);
   g->have_density = true;
   g->have_eval = false;
@@ -110,6 +114,20 @@ void density_from_device(chx_grid* g, const double* dD) {
 
 void fock_to_device(chx_grid* g, double* dV) {
   hipEvent_t e = nullptr;
+  if (g->cached) {
+    g->mark_begin(kStMakeB, &e);
+    launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
+    g->mark_end(kStMakeB, e);
+    launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
+    g->mark_begin(kStVxc, &e);
+    launch_vxc_c64(g->stream, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p, g->ntiles,
+                   g->d_cache.p, g->d_bmat.p, g->d_Mint.p, g->npad);
+    g->mark_end(kStVxc, e);
+    g->mark_begin(kStFinalize, &e);
+    launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);
+    g->mark_end(kStFinalize, e);
+    return;
+  }
   launch_fill(g->stream, g->d_Vint.p, (size_t)g->npad * g->npad, 0.0);
   g->mark_begin(kStVxc, &e);
   launch_vxc_f64(g->stream, (int)g->vwork.size(), g->d_vwork.p, g->d_pair_batches.p, g->d_alive.p, g->ntiles, g->d_x.p,
@@ -224,6 +242,7 @@ int chx_grid_copy(const chx_grid* s, chx_grid** out) {
     require(s && out, CHX_ERR_ARG, "bad arguments");
     g = new chx_grid;
     g->nthreads = s->nthreads; g->type = s->type; g->precision = s->precision; g->screen_thr = s->screen_thr;
+    g->cache_mode = s->cache_mode;
     g->mol = s->mol;
     g->nbasis = s->nbasis; g->npad = s->npad; g->ntiles = s->ntiles;
     g->int2ref = s->int2ref; g->ref2int = s->ref2int;
@@ -413,12 +432,24 @@ int chx_grid_profile(chx_grid* g, int enable) {
 int chx_grid_kernel_times(chx_grid* g, int reset, double* ms, long* launches) {
   return guard([&] {
     g->collect_times();
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 6; ++k) {
       if (ms) ms[k] = g->stage_ms[k];
       if (launches) launches[k] = g->stage_n[k];
       if (reset) { g->stage_ms[k] = 0; g->stage_n[k] = 0; }
     }
   });
+}
+
+int chx_grid_set_ao_cache(chx_grid* g, int mode) {
+  return guard([&] {
+    require(mode >= -1 && mode <= 1, CHX_ERR_ARG, "cache mode must be -1, 0 or 1");
+    g->cache_mode = mode;
+    g->ao_ready = false;
+  });
+}
+
+int chx_grid_ao_cached(const chx_grid* g, int* cached) {
+  return guard([&] { *cached = g->cached ? 1 : 0; });
 }
 
 int chx_grid_stats(const chx_grid* g, long* nbatches, long* npoints, double* mean_tiles, double* flops) {

@@ -248,7 +248,7 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
     key[a] = morton3((m.xyz[3 * a] - mn[0]) / span, (m.xyz[3 * a + 1] - mn[1]) / span, (m.xyz[3 * a + 2] - mn[2]) / span);
   std::stable_sort(aorder.begin(), aorder.end(), [&](int p, int q) { return key[p] < key[q]; });
   g->nbasis = m.nbasis;
-  g->npad = ((m.nbasis + kSuper * kTile - 1) / (kSuper * kTile)) * (kSuper * kTile);
+  g->npad = ((m.nbasis + kPadAO - 1) / kPadAO) * kPadAO;
   g->ntiles = g->npad / kTile;
   g->int2ref.assign(g->npad, -1);
   g->ref2int.assign(g->nbasis, -1);
@@ -366,6 +366,66 @@ void grid_plan(chx_grid* g) {
         g->vwork.push_back(make_int4(I, J, begin, (int)g->pair_batches.size()));
       }
     }
+  // Cached-AO plan: slots are the entries of bt_list.
+  const size_t nslot = g->bt_list.size();
+  g->slot_batch.assign(nslot, 0);
+  g->slot_tile.assign(nslot, 0);
+  g->slot_of.assign((size_t)nb * nt, -1);
+  for (int b = 0; b < nb; ++b)
+    for (int it = g->bt_off[b]; it < g->bt_off[b + 1]; ++it) {
+      g->slot_batch[it] = b;
+      g->slot_tile[it] = g->bt_list[it];
+      g->slot_of[(size_t)b * nt + g->bt_list[it]] = it;
+    }
+  g->tile_nao = tile_nao;
+  g->grp_slots.assign(g->grp_batch.size() * kGroup, 0);
+  for (size_t k = 0; k < g->grp_batch.size(); ++k)
+    for (int q = 0; q < g->grp_ntiles[k]; ++q)
+      g->grp_slots[k * kGroup + q] = g->slot_of[(size_t)g->grp_batch[k] * nt + g->grp_tiles[k * kGroup + q]];
+  const double cache_bytes = (double)nslot * (4.0 + 1.0) * kBatch * kTile * sizeof(double);
+  size_t free_b = 0, total_b = 0;
+  CHX_HIP(hipMemGetInfo(&free_b, &total_b));
+  g->cached = g->cache_mode == 1 || (g->cache_mode == -1 && cache_bytes < 0.8 * (double)free_b);
+  if (g->cached) {
+    const int nsc = g->npad / (kCacheSuper * kTile);
+    std::vector<std::vector<int>> cp((size_t)nsc * nsc);
+    std::vector<int> alive_sup;
+    for (int b = 0; b < nb; ++b) {
+      alive_sup.clear();
+      for (int sI = 0; sI < nsc; ++sI) {
+        bool any = false;
+        for (int h = 0; h < kCacheSuper; ++h) any |= g->alive[(size_t)b * nt + sI * kCacheSuper + h] != 0;
+        if (any) alive_sup.push_back(sI);
+      }
+      for (int I : alive_sup)
+        for (int J : alive_sup) cp[(size_t)I * nsc + J].push_back(b);
+    }
+    g->cwork.clear();
+    g->cpair_batches.clear();
+    for (int I = 0; I < nsc; ++I)
+      for (int J = 0; J < nsc; ++J) {
+        const auto& L = cp[(size_t)I * nsc + J];
+        for (size_t k = 0; k < L.size(); k += kCacheChunk) {
+          const int begin = (int)g->cpair_batches.size();
+          const size_t e = std::min(L.size(), k + kCacheChunk);
+          g->cpair_batches.insert(g->cpair_batches.end(), L.begin() + k, L.begin() + e);
+          g->cwork.push_back(make_int4(I, J, begin, (int)g->cpair_batches.size()));
+        }
+      }
+    g->d_slot_batch.upload(g->slot_batch, st); g->d_slot_tile.upload(g->slot_tile, st);
+    g->d_slot_of.upload(g->slot_of, st); g->d_tile_nao.upload(g->tile_nao, st);
+    g->d_grp_slots.upload(g->grp_slots, st);
+    g->d_cwork.upload(g->cwork, st); g->d_cpair_batches.upload(g->cpair_batches, st);
+    g->d_cache.alloc(std::max<size_t>(1, nslot * (size_t)4 * kBatch * kTile));
+    g->d_bmat.alloc(std::max<size_t>(1, nslot * (size_t)kBatch * kTile));
+    g->d_Mint.alloc((size_t)g->npad * g->npad);
+    launch_ao_cache(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p, g->d_z.p,
+                    g->view(), g->d_cache.p);
+  } else {
+    g->d_cache.release();
+    g->d_bmat.release();
+    g->d_Mint.release();
+  }
   g->d_bt_off.upload(g->bt_off, st); g->d_bt_list.upload(g->bt_list, st);
   g->d_grp_batch.upload(g->grp_batch, st); g->d_grp_tiles.upload(g->grp_tiles, st);
   g->d_grp_ntiles.upload(g->grp_ntiles, st); g->d_grp_off.upload(g->grp_off, st);

@@ -17,6 +17,9 @@ constexpr int kTile = 32;      // AOs per screening tile
 constexpr int kGroup = 4;      // AO tiles per density work group (128 output columns)
 constexpr int kSuper = 2;      // AO tiles per Vxc super-tile (64 x 64 output block)
 constexpr int kVxcChunk = 32;  // batches per Vxc work item
+constexpr int kPadAO = 128;    // nbasis is padded to a multiple of this (4 tiles)
+constexpr int kCacheSuper = 4; // tiles per Vxc super-tile in the cached path
+constexpr int kCacheChunk = 16;  // batches per cached-path Vxc work item
 
 struct Error : std::runtime_error {
   int code;
@@ -97,6 +100,16 @@ void launch_vxc_f64(hipStream_t s, int nwork, const int4* work, const int* pair_
                     double* Vint, int npad);
 void launch_finalize_v(hipStream_t s, int n, int npad, const double* Vint, const int* ref2int, double* V);
 void launch_vrho_coef(hipStream_t s, long nslots, const double* vrho_w, double* coef);
+// Cached-AO FP64 path (kernels_cached.hip).
+void launch_ao_cache(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* tile_nao,
+                     const double* bx, const double* by, const double* bz, BasisView bv, double* cache);
+void launch_density_c64(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_slots,
+                        const int* grp_ntiles, const int* bt_off, const int* bt_list, const double* cache,
+                        const double* Dint, int npad, double* part);
+void launch_make_b(hipStream_t s, int nbatches, const int* bt_off, const double* coef, const double* cache, double* bmat);
+void launch_vxc_c64(hipStream_t s, int nwork, const int4* work, const int* pair_batches, const int* slot_of, int ntiles,
+                    const double* cache, const double* bmat, double* Mint, int npad);
+void launch_finalize_sym(hipStream_t s, int n, int npad, const double* Mint, const int* ref2int, double* V);
 
 }  // namespace chx
 
@@ -149,6 +162,15 @@ struct chx_grid {
   std::vector<int> grp_batch, grp_tiles, grp_ntiles, grp_off;  // density work groups
   std::vector<int4> vwork;           // Vxc work items (tile I, tile J, begin, end)
   std::vector<int> pair_batches;
+  // Cached-AO path: phi and grad phi per alive (batch, tile) slot kept in HBM.
+  bool cached = false;
+  int cache_mode = -1;               // -1 automatic, 0 never, 1 always
+  std::vector<int> slot_batch, slot_tile, slot_of, tile_nao, grp_slots;
+  std::vector<int4> cwork;
+  std::vector<int> cpair_batches;
+  chx::DevBuf<double> d_cache, d_bmat, d_Mint;
+  chx::DevBuf<int> d_slot_batch, d_slot_tile, d_slot_of, d_tile_nao, d_grp_slots, d_cpair_batches;
+  chx::DevBuf<int4> d_cwork;
   double flops_build = 0, mean_tiles = 0;
 
   // Device mirrors.

@@ -114,11 +114,18 @@ int chx_grid_set_screening(chx_grid* g, double threshold);
  * algorithmic FLOPs of one XC build (dense-contraction count, k*2*sum g_b n_b^2). */
 int chx_grid_stats(const chx_grid* g, long* nbatches, long* npoints, double* mean_tiles, double* flops_per_build);
 
+/* AO tables in HBM (no reference counterpart; the reference caches its own,
+ * src/Grid/Grid.cpp:22-33): -1 (default) caches phi and grad phi of every
+ * alive (batch, tile) at the next chx_grid_get_ao when they fit in 80% of the
+ * free device memory, 0 never (recompute inside the kernels), 1 always. */
+int chx_grid_set_ao_cache(chx_grid* g, int mode);
+int chx_grid_ao_cached(const chx_grid* g, int* cached);
 /* Per-kernel HIP-event timing on the handle's stream (no reference counterpart).
  * Stages: 0 D permute, 1 density contraction, 2 functional + E/N sums,
- * 3 Vxc contraction, 4 V finalise. Times accumulate until read with reset. */
+ * 3 Vxc contraction, 4 V finalise, 5 Vxc integrand B (cached path).
+ * Times accumulate until read with reset. */
 int chx_grid_profile(chx_grid* g, int enable);
-int chx_grid_kernel_times(chx_grid* g, int reset, double* ms /*[5]*/, long* launches /*[5]*/);
+int chx_grid_kernel_times(chx_grid* g, int reset, double* ms /*[6]*/, long* launches /*[6]*/);
 
 /* ---- Functional ------------------------------------------------------------
  * ExchangeCorrelation::Read(std::string df, bool output)

@@ -98,6 +98,38 @@ def test_water_cluster_parity():
     assert np.abs(V - Vo).max() <= V_TOL
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ao_cache_modes(mode):
+    """Both AO paths (recomputed in the kernels, cached in HBM) against the oracle."""
+    Z, X = wl.water_cluster_angstrom(3, 4.5, 5)
+    xyz = X * ca.ANGSTROM_TO_BOHR
+    mol = ca.Molecule(Z, xyz, "def2-svp")
+    D, S = wl.workload_density(mol, 15, 9)
+    _say("AO cache mode %d: (H2O)3/def2-svp/SG-2 PBE" % mode)
+    grid = ca.Grid(mol, "SG-2", 1, 0)
+    grid.setAOCache(mode)
+    grid.setType(1)
+    grid.getAO(0, 0)
+    assert grid.aoCached() == bool(mode)
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    E, V = ca.xc_build(grid, xc, D)
+    orc = ol.OracleSystem(Z, xyz, "SG-2", "def2-svp")
+    orc.set_functional("PBE")
+    orc.get_ao()
+    Eo, No, Vo = orc.xc(D)
+    _say("  E gpu %.14f oracle %.14f dE %.2e dV %.2e" % (E, Eo, abs(E - Eo), np.abs(V - Vo).max()))
+    assert abs(E - Eo) <= E_TOL
+    assert np.abs(V - Vo).max() <= V_TOL
+    # the supplied-potential path uses the same Vxc kernels
+    P = grid.points()
+    v = -np.exp(-0.1 * np.sum(P[:, :3] ** 2, axis=1))
+    grid.setVrho(v)
+    V2 = grid.getFock()[0]
+    ao, _, _ = orc.ao_values(P[:, :3])
+    assert np.abs(V2 - (ao * (P[:, 3] * v)) @ ao.T).max() <= V_TOL
+
+
 def test_set_vrho_fock(h2o):
     """Externally supplied v_rho (SAP guess path): V = sum_g w v phi phi."""
     grid = h2o["grid"]
