@@ -87,6 +87,7 @@ def lib():
         "chx_xc_evaluate": (i, [_vp, _vp, cp]),
         "chx_selftest": (i, [P(d)]),
         "chx_lebedev_rule": (i, [i, _vp]),
+        "chx_xc_eval_points": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -325,6 +326,17 @@ class ExchangeCorrelation:
             lib().chx_xc_destroy(self.h)
         self.h = h
         self._load_info()
+
+    def eval_points(self, rho, sigma=None, tau=None):
+        """exc, v_rho, v_sigma, v_tau at the given points, evaluated on the device."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        n = len(rho)
+        sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+        ta = None if tau is None else np.ascontiguousarray(tau, dtype=np.float64)
+        e, vr, vs, vt = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().chx_xc_eval_points(self.h, n, _ptr(rho), _ptr(sg), _ptr(ta), _ptr(e), _ptr(vr), _ptr(vs),
+                                        _ptr(vt)))
+        return e, vr, vs, vt
 
     def Evaluate(self, order, grid):
         _check(lib().chx_xc_evaluate(self.h, grid.h, order.encode()))

@@ -84,8 +84,9 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
   g->d_codes.upload(xc->codes, g->stream);
   hipEvent_t e = nullptr;
   g->mark_begin(2, &e);
-  launch_functional(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_w.p, g->d_codes.p, (int)xc->codes.size(),
-                    xc->family, want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p, g->d_coef.p, g->d_en_part.p, g->en_blocks);
+  launch_functional(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, g->d_codes.p,
+                    (int)xc->codes.size(), xc->family, want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p, g->d_coef.p,
+                    g->d_en_part.p, g->en_blocks);
   launch_reduce_en(g->stream, g->en_blocks, g->d_en_part.p, g->d_en.p);
   g->mark_end(2, e);
 }
@@ -99,10 +100,16 @@ void density_from_device(chx_grid* g, const double* dD) {
   launch_permute_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dint.p);
   g->mark_end(kStPermute, e);
   g->mark_begin(kStDensity, &e);
-  if (g->cached)
-    launch_density_c64(g->stream, (int)g->grp_batch.size(), g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p,
-                       g->d_grp_ntiles.p, g->d_bt_off.p, g->d_bt_list.p, g->d_cache.p, g->d_Dint.p, g->npad, g->d_part.p);
-  else
+  const int ng = (int)g->grp_batch.size();
+  if (g->cached) {
+    launch_density_c64(g->stream, 0, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
+                       g->d_bt_off.p, g->d_bt_list.p, g->d_cache.p, g->d_Dint.p, g->npad, g->d_part.p);
+    if (g->type >= 2)  // tau: three more contractions with A = d_r phi
+      for (int r = 1; r <= 3; ++r)
+        launch_density_c64(g->stream, r, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
+                           g->d_bt_off.p, g->d_bt_list.p, g->d_cache.p, g->d_Dint.p, g->npad,
+                           g->d_part_tau.p + (size_t)(r - 1) * kBatch);
+  } else
     launch_density_f64(g->stream, (int)g->grp_batch.size(), g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p,
                        g->d_bt_off.p, g->d_bt_list.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_Dint.p, g->npad,
                        g->d_part.p);
@@ -115,12 +122,17 @@ void density_from_device(chx_grid* g, const double* dD) {
 void fock_to_device(chx_grid* g, double* dV) {
   hipEvent_t e = nullptr;
   if (g->cached) {
+    const int nc = g->v_family >= 2 ? 4 : 1;
+    if (g->bmat_comps < nc) {
+      g->d_bmat.alloc(std::max<size_t>(1, g->bt_list.size() * (size_t)nc * kBatch * kTile));
+      g->bmat_comps = nc;
+    }
     g->mark_begin(kStMakeB, &e);
-    launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
+    launch_make_b(g->stream, nc, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
     g->mark_end(kStMakeB, e);
     launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
     g->mark_begin(kStVxc, &e);
-    launch_vxc_c64(g->stream, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p, g->ntiles,
+    launch_vxc_c64(g->stream, nc, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p, g->ntiles,
                    g->d_cache.p, g->d_bmat.p, g->d_Mint.p, g->npad);
     g->mark_end(kStVxc, e);
     g->mark_begin(kStFinalize, &e);
@@ -305,7 +317,7 @@ int chx_grid_get_points(const chx_grid* g, double* xyzw) {
 int chx_grid_set_type(chx_grid* g, int type) {
   return guard([&] {
     require(type >= -1 && type <= 2, CHX_ERR_ARG, "grid type must be -1, 0, 1 or 2");
-    require(type <= 1, CHX_ERR_UNSUPPORTED, "meta-GGA grids are not built yet");
+    if (g->type != type) g->ao_ready = false;  // tau buffers are sized at getAO
     g->type = type;
   });
 }
@@ -400,6 +412,7 @@ int chx_grid_set_vrho(chx_grid* g, const double* vrho, long npoints) {
     launch_vrho_coef(g->stream, (long)vw.size(), d.p, g->d_coef.p);
     CHX_HIP(hipStreamSynchronize(g->stream));
     g->have_v = true;
+    g->v_family = 0;
   });
 }
 
@@ -532,7 +545,10 @@ int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order) {
     require(g->have_density, CHX_ERR_STATE, "Grid::getDensity must precede Evaluate");
     require(g->type >= xc->family, CHX_ERR_STATE, "grid type is below the functional family (call setType)");
     const bool e = o.find('e') != std::string::npos, v = o.find('v') != std::string::npos;
+    require(xc->family < 2 || g->cached, CHX_ERR_UNSUPPORTED,
+            "meta-GGA functionals need the AO tables in HBM (chx_grid_set_ao_cache)");
     run_functional(g, xc, e, v);
+    if (v) g->v_family = xc->family;
     g->have_eval = e || g->have_eval;
     g->have_v = v;
   });
@@ -542,6 +558,30 @@ int chx_lebedev_rule(int npoints, double* xyzw) {
   return guard([&] {
     require(xyzw != nullptr, CHX_ERR_ARG, "null output");
     chx_lebedev_rule_impl(npoints, xyzw);
+  });
+}
+
+int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
+                       double* exc, double* vrho, double* vsigma, double* vtau) {
+  return guard([&] {
+    require(xc && n >= 0 && rho && exc && vrho, CHX_ERR_ARG, "bad arguments");
+    hipStream_t s;
+    CHX_HIP(hipStreamCreate(&s));
+    std::vector<double> zero(n, 0.0);
+    const std::vector<double> hr(rho, rho + n), hs(sigma ? std::vector<double>(sigma, sigma + n) : zero),
+        ht(tau ? std::vector<double>(tau, tau + n) : zero);
+    DevBuf<double> dr, ds, dt, de, dvr, dvs, dvt;
+    DevBuf<int> dc;
+    dr.upload(hr, s); ds.upload(hs, s); dt.upload(ht, s); dc.upload(xc->codes, s);
+    de.alloc(std::max<long>(1, n)); dvr.alloc(std::max<long>(1, n)); dvs.alloc(std::max<long>(1, n));
+    dvt.alloc(std::max<long>(1, n));
+    launch_eval_functional(s, n, dc.p, (int)xc->codes.size(), dr.p, ds.p, dt.p, de.p, dvr.p, dvs.p, dvt.p);
+    auto down = [&](double* h, const DevBuf<double>& d) {
+      if (h && n) CHX_HIP(hipMemcpyAsync(h, d.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    };
+    down(exc, de); down(vrho, dvr); down(vsigma, dvs); down(vtau, dvt);
+    CHX_HIP(hipStreamSynchronize(s));
+    CHX_HIP(hipStreamDestroy(s));
   });
 }
 

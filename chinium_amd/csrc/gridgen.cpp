@@ -382,7 +382,7 @@ void grid_plan(chx_grid* g) {
   for (size_t k = 0; k < g->grp_batch.size(); ++k)
     for (int q = 0; q < g->grp_ntiles[k]; ++q)
       g->grp_slots[k * kGroup + q] = g->slot_of[(size_t)g->grp_batch[k] * nt + g->grp_tiles[k * kGroup + q]];
-  const double cache_bytes = (double)nslot * (4.0 + 1.0) * kBatch * kTile * sizeof(double);
+  const double cache_bytes = (double)nslot * (4.0 + (g->type >= 2 ? 4.0 : 1.0)) * kBatch * kTile * sizeof(double);
   size_t free_b = 0, total_b = 0;
   CHX_HIP(hipMemGetInfo(&free_b, &total_b));
   g->cached = g->cache_mode == 1 || (g->cache_mode == -1 && cache_bytes < 0.8 * (double)free_b);
@@ -400,24 +400,29 @@ void grid_plan(chx_grid* g) {
       for (int I : alive_sup)
         for (int J : alive_sup) cp[(size_t)I * nsc + J].push_back(b);
     }
+    // Chunk-major order: blocks that run at the same time work on the same
+    // batches of different (I, J) pairs, so their operand tiles meet in L2/MALL.
     g->cwork.clear();
     g->cpair_batches.clear();
-    for (int I = 0; I < nsc; ++I)
-      for (int J = 0; J < nsc; ++J) {
-        const auto& L = cp[(size_t)I * nsc + J];
-        for (size_t k = 0; k < L.size(); k += kCacheChunk) {
+    size_t maxlen = 0;
+    for (const auto& L : cp) maxlen = std::max(maxlen, L.size());
+    for (size_t k = 0; k < maxlen; k += kCacheChunk)
+      for (int I = 0; I < nsc; ++I)
+        for (int J = 0; J < nsc; ++J) {
+          const auto& L = cp[(size_t)I * nsc + J];
+          if (k >= L.size()) continue;
           const int begin = (int)g->cpair_batches.size();
           const size_t e = std::min(L.size(), k + kCacheChunk);
           g->cpair_batches.insert(g->cpair_batches.end(), L.begin() + k, L.begin() + e);
           g->cwork.push_back(make_int4(I, J, begin, (int)g->cpair_batches.size()));
         }
-      }
     g->d_slot_batch.upload(g->slot_batch, st); g->d_slot_tile.upload(g->slot_tile, st);
     g->d_slot_of.upload(g->slot_of, st); g->d_tile_nao.upload(g->tile_nao, st);
     g->d_grp_slots.upload(g->grp_slots, st);
     g->d_cwork.upload(g->cwork, st); g->d_cpair_batches.upload(g->cpair_batches, st);
     g->d_cache.alloc(std::max<size_t>(1, nslot * (size_t)4 * kBatch * kTile));
-    g->d_bmat.alloc(std::max<size_t>(1, nslot * (size_t)kBatch * kTile));
+    g->bmat_comps = g->type >= 2 ? 4 : 1;
+    g->d_bmat.alloc(std::max<size_t>(1, nslot * (size_t)g->bmat_comps * kBatch * kTile));
     g->d_Mint.alloc((size_t)g->npad * g->npad);
     launch_ao_cache(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p, g->d_z.p,
                     g->view(), g->d_cache.p);
@@ -438,7 +443,8 @@ void grid_plan(chx_grid* g) {
   g->d_Vout.alloc((size_t)g->nbasis * g->nbasis);
   g->d_part.alloc(std::max<size_t>(1, g->grp_batch.size() * 4 * kBatch));
   g->d_rho.alloc(std::max<size_t>(1, nslots));
-  g->d_coef.alloc(std::max<size_t>(4, 4 * nslots));
+  g->d_coef.alloc(std::max<size_t>(kCoef, kCoef * nslots));
+  g->d_part_tau.alloc(g->type >= 2 ? std::max<size_t>(1, g->grp_batch.size() * 3 * kBatch) : 1);
   g->en_blocks = (int)std::max<size_t>(1, (nslots + 255) / 256);
   g->d_en_part.alloc(2 * (size_t)g->en_blocks);
   g->d_en.alloc(2);

@@ -20,6 +20,7 @@ constexpr int kVxcChunk = 32;  // batches per Vxc work item
 constexpr int kPadAO = 128;    // nbasis is padded to a multiple of this (4 tiles)
 constexpr int kCacheSuper = 4; // tiles per Vxc super-tile in the cached path
 constexpr int kCacheChunk = 16;  // batches per cached-path Vxc work item
+constexpr int kCoef = 5;       // Vxc integrand coefficients per point: a, c_x, c_y, c_z, t
 
 struct Error : std::runtime_error {
   int code;
@@ -90,9 +91,11 @@ void launch_permute_d(hipStream_t s, int n, int npad, const double* D, const int
 void launch_density_f64(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
                         const int* bt_off, const int* bt_list, const double* bx, const double* by, const double* bz,
                         BasisView bv, const double* Dint, int npad, double* part);
-void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* bw,
-                       const int* codes, int ncodes, int family, int want_e, int want_v,
+void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* part_tau,
+                       const double* bw, const int* codes, int ncodes, int family, int want_e, int want_v,
                        double* rho_out, double* coef, double* en_part, int en_blocks);
+void launch_eval_functional(hipStream_t s, long n, const int* codes, int ncodes, const double* rho, const double* sigma,
+                            const double* tau, double* exc, double* vrho, double* vsigma, double* vtau);
 void launch_reduce_en(hipStream_t s, int nparts, const double* en_part, double* en);
 void launch_fill(hipStream_t s, double* p, size_t n, double v);
 void launch_vxc_f64(hipStream_t s, int nwork, const int4* work, const int* pair_batches, const unsigned char* alive,
@@ -103,12 +106,13 @@ void launch_vrho_coef(hipStream_t s, long nslots, const double* vrho_w, double* 
 // Cached-AO FP64 path (kernels_cached.hip).
 void launch_ao_cache(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* tile_nao,
                      const double* bx, const double* by, const double* bz, BasisView bv, double* cache);
-void launch_density_c64(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_slots,
-                        const int* grp_ntiles, const int* bt_off, const int* bt_list, const double* cache,
-                        const double* Dint, int npad, double* part);
-void launch_make_b(hipStream_t s, int nbatches, const int* bt_off, const double* coef, const double* cache, double* bmat);
-void launch_vxc_c64(hipStream_t s, int nwork, const int4* work, const int* pair_batches, const int* slot_of, int ntiles,
-                    const double* cache, const double* bmat, double* Mint, int npad);
+void launch_density_c64(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
+                        const int* grp_slots, const int* grp_ntiles, const int* bt_off, const int* bt_list,
+                        const double* cache, const double* Dint, int npad, double* part);
+void launch_make_b(hipStream_t s, int ncomp, int nbatches, const int* bt_off, const double* coef, const double* cache,
+                   double* bmat);
+void launch_vxc_c64(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
+                    int ntiles, const double* cache, const double* bmat, double* Mint, int npad);
 void launch_finalize_sym(hipStream_t s, int n, int npad, const double* Mint, const int* ref2int, double* V);
 
 }  // namespace chx
@@ -187,7 +191,9 @@ struct chx_grid {
   chx::DevBuf<int> d_codes;
 
   // Per-iteration state.
-  chx::DevBuf<double> d_Din, d_Dint, d_part, d_rho, d_coef, d_en_part, d_en, d_Vint, d_Vout;
+  chx::DevBuf<double> d_Din, d_Dint, d_part, d_part_tau, d_rho, d_coef, d_en_part, d_en, d_Vint, d_Vout;
+  int bmat_comps = 0;  // components per slot held in d_bmat (1 GGA, 4 meta-GGA)
+  int v_family = 0;    // family of the potential on the grid (set by Evaluate / set_vrho)
   int en_blocks = 0;
   bool have_density = false, have_eval = false, have_v = false;
 

@@ -273,35 +273,42 @@ void launch_density_f64(hipStream_t s, int ngroups, const int* grp_batch, const 
 // work groups, evaluates the functional and writes the Vxc integrand
 // coefficients; block partial sums of E = sum w rho eps and N = sum w rho.
 __global__ __launch_bounds__(256) void k_functional(int nslots, const int* __restrict__ grp_off,
-                                                    const double* __restrict__ part, const double* __restrict__ bw,
-                                                    const int* __restrict__ codes, int ncodes, int family, int want_e,
-                                                    int want_v, double* __restrict__ rho_out, double* __restrict__ coef,
+                                                    const double* __restrict__ part, const double* __restrict__ part_tau,
+                                                    const double* __restrict__ bw, const int* __restrict__ codes,
+                                                    int ncodes, int family, int want_e, int want_v,
+                                                    double* __restrict__ rho_out, double* __restrict__ coef,
                                                     double* __restrict__ en_part) {
   __shared__ double red[2][256];
   const int slot = blockIdx.x * 256 + threadIdx.x;
   double e_acc = 0, n_acc = 0;
   if (slot < nslots) {
     const int b = slot / kBatch, g = slot % kBatch;
-    double rho = 0, gx = 0, gy = 0, gz = 0;
+    double rho = 0, gx = 0, gy = 0, gz = 0, tau = 0;
     for (int k = grp_off[b]; k < grp_off[b + 1]; ++k) {
       const double* p = part + (size_t)k * 4 * kBatch;
       rho += p[g];
       gx += p[kBatch + g];
       gy += p[2 * kBatch + g];
       gz += p[3 * kBatch + g];
+      if (family >= 2) {
+        const double* q = part_tau + (size_t)k * 3 * kBatch;
+        tau += q[g] + q[kBatch + g] + q[2 * kBatch + g];
+      }
     }
     gx *= 2; gy *= 2; gz *= 2;
+    tau *= 0.5;
     const double w = bw[slot];
     double sigma = family >= 1 ? gx * gx + gy * gy + gz * gz : 0.0;
-    double eps, vr, vs;
-    eval_codes(codes, ncodes, rho, sigma, &eps, &vr, &vs);
+    double eps, vr, vs, vt;
+    eval_codes(codes, ncodes, rho, sigma, tau, &eps, &vr, &vs, &vt);
     rho_out[slot] = rho;
     if (want_v) {
-      coef[4 * (size_t)slot + 0] = 0.5 * w * vr;
+      coef[kCoef * (size_t)slot + 0] = 0.5 * w * vr;
       const double f = family >= 1 ? 2.0 * w * vs : 0.0;
-      coef[4 * (size_t)slot + 1] = f * gx;
-      coef[4 * (size_t)slot + 2] = f * gy;
-      coef[4 * (size_t)slot + 3] = f * gz;
+      coef[kCoef * (size_t)slot + 1] = f * gx;
+      coef[kCoef * (size_t)slot + 2] = f * gy;
+      coef[kCoef * (size_t)slot + 3] = f * gz;
+      coef[kCoef * (size_t)slot + 4] = family >= 2 ? 0.25 * w * vt : 0.0;
     }
     if (want_e) e_acc = w * rho * eps;
     n_acc = w * rho;
@@ -322,12 +329,36 @@ __global__ __launch_bounds__(256) void k_functional(int nslots, const int* __res
   }
 }
 
-void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* bw,
-                       const int* codes, int ncodes, int family, int want_e, int want_v, double* rho_out, double* coef,
-                       double* en_part, int en_blocks) {
+void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* part_tau,
+                       const double* bw, const int* codes, int ncodes, int family, int want_e, int want_v,
+                       double* rho_out, double* coef, double* en_part, int en_blocks) {
   const int nslots = nbatches * kBatch;
-  hipLaunchKernelGGL(k_functional, dim3(en_blocks), dim3(256), 0, s, nslots, grp_off, part, bw, codes, ncodes, family,
-                     want_e, want_v, rho_out, coef, en_part);
+  hipLaunchKernelGGL(k_functional, dim3(en_blocks), dim3(256), 0, s, nslots, grp_off, part, part_tau, bw, codes, ncodes,
+                     family, want_e, want_v, rho_out, coef, en_part);
+  CHX_HIP(hipGetLastError());
+}
+
+// Functional values and derivatives at arbitrary (rho, sigma, tau), as libxc's
+// xc_*_exc_vxc returns them (src/ExchangeCorrelation.cpp:98,122,157).
+__global__ void k_eval_functional(long n, const int* __restrict__ codes, int ncodes, const double* __restrict__ rho,
+                                  const double* __restrict__ sigma, const double* __restrict__ tau,
+                                  double* __restrict__ exc, double* __restrict__ vrho, double* __restrict__ vsigma,
+                                  double* __restrict__ vtau) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double e, vr, vs, vt;
+  eval_codes(codes, ncodes, rho[i], sigma[i], tau[i], &e, &vr, &vs, &vt);
+  exc[i] = e;
+  vrho[i] = vr;
+  vsigma[i] = vs;
+  vtau[i] = vt;
+}
+
+void launch_eval_functional(hipStream_t s, long n, const int* codes, int ncodes, const double* rho, const double* sigma,
+                            const double* tau, double* exc, double* vrho, double* vsigma, double* vtau) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_eval_functional, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, codes, ncodes, rho,
+                     sigma, tau, exc, vrho, vsigma, vtau);
   CHX_HIP(hipGetLastError());
 }
 
@@ -375,10 +406,11 @@ void launch_fill(hipStream_t s, double* p, size_t n, double v) {
 __global__ void k_vrho_coef(long nslots, const double* __restrict__ vrho_w, double* __restrict__ coef) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots) return;
-  coef[4 * i] = 0.5 * vrho_w[i];
-  coef[4 * i + 1] = 0;
-  coef[4 * i + 2] = 0;
-  coef[4 * i + 3] = 0;
+  coef[kCoef * i] = 0.5 * vrho_w[i];
+  coef[kCoef * i + 1] = 0;
+  coef[kCoef * i + 2] = 0;
+  coef[kCoef * i + 3] = 0;
+  coef[kCoef * i + 4] = 0;
 }
 
 void launch_vrho_coef(hipStream_t s, long nslots, const double* vrho_w, double* coef) {
@@ -460,10 +492,10 @@ __global__ __launch_bounds__(256) void k_vxc_f64(const int4* __restrict__ work, 
         sm.px[tid] = bx[slot];
         sm.py[tid] = by[slot];
         sm.pz[tid] = bz[slot];
-        sm.ca[tid] = coef[4 * slot];
-        sm.cx[tid] = coef[4 * slot + 1];
-        sm.cy[tid] = coef[4 * slot + 2];
-        sm.cz[tid] = coef[4 * slot + 3];
+        sm.ca[tid] = coef[kCoef * slot];
+        sm.cx[tid] = coef[kCoef * slot + 1];
+        sm.cy[tid] = coef[kCoef * slot + 2];
+        sm.cz[tid] = coef[kCoef * slot + 3];
       }
       __syncthreads();
       vxc_fill_super(sm, sm.PI, sm.BI, I, ab, bv, tid);

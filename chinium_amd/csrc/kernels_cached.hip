@@ -10,7 +10,9 @@
 //            rho = sum X phi and grad rho / 2 = sum X grad phi from the cache;
 //   Vxc:     M_IJ = Phi_I^T B_J over 128-AO super-tiles, B = a phi + c . grad phi
 //            built once per iteration by k_make_b; V = M + M^T at finalisation.
-// Layouts: cache[slot][c][g][32] (c = phi, d/dx, d/dy, d/dz), bmat[slot][g][32].
+// Meta-GGA adds tau = 1/2 sum_r (d_r Phi D) . d_r phi (three more density
+// contractions) and sum_r (d_r Phi)^T T_r to M (three more K components).
+// Layouts: cache[slot][c][g][32] (c = phi, d/dx, d/dy, d/dz), bmat[slot][NC][g][32].
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -74,22 +76,32 @@ void launch_ao_cache(hipStream_t s, int nslots, const int* slot_batch, const int
 
 // ---------------------------------------------------------------------------
 // Density from the cache. Block = (batch, group of <= 4 alive tiles as output
-// columns); wave w owns rows g = 16w..16w+15. K loop over the batch's alive
-// tiles: phi tile [g][mu] and D block [mu][nu] double-buffered in LDS.
+// columns), 8 waves: wave w owns rows g = 16(w%4)..+15 and output columns
+// 64(w/4)..+63 (two tiles). K loop over the batch's alive tiles: phi tile
+// [g][mu] and D block [mu][nu] double-buffered in LDS, the next K step
+// prefetched into registers while the MFMAs of the current one run.
 struct DensCSmem {
   double A[2][kBatch][kTile + 1];
   double B[2][kTile][kGroup * kTile];
+  double red[2][4][kBatch];
 };
 
-__global__ __launch_bounds__(256) void k_density_c64(const int* __restrict__ grp_batch, const int* __restrict__ grp_tiles,
+// MODE 0: A = phi, epilogue rho and grad rho / 2 (4 components into part).
+// MODE r = 1..3: A = d phi / d x_r, epilogue sum X_r d_r phi into part (one
+// component: the tau partial of direction r).
+template <int MODE>
+__global__ __launch_bounds__(512) void k_density_c64(const int* __restrict__ grp_batch, const int* __restrict__ grp_tiles,
                                                      const int* __restrict__ grp_slots, const int* __restrict__ grp_ntiles,
                                                      const int* __restrict__ bt_off, const int* __restrict__ bt_list,
                                                      const double* __restrict__ cache, const double* __restrict__ Dint,
-                                                     int npad, double* __restrict__ part) {
+                                                     int npad, double* __restrict__ part, int grp_stride) {
+  constexpr int NOUT = MODE == 0 ? 4 : 1;
   __shared__ DensCSmem sm;
   const int grp = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform, keeps branches scalar
   const int lrow = lane & 15, lk = lane >> 4;
+  const int wr = wave & 3, wc = wave >> 2;  // row block, column half
   const int b = grp_batch[grp];
   const int nt = grp_ntiles[grp];
   int jt[kGroup];
@@ -97,16 +109,15 @@ __global__ __launch_bounds__(256) void k_density_c64(const int* __restrict__ grp
   for (int q = 0; q < kGroup; ++q) jt[q] = grp_tiles[grp * kGroup + q];
   const int it0 = bt_off[b], it1 = bt_off[b + 1];
 
-  // register staging of one K step: 8 phi values and 16 D values per thread
-  double2 ra[4], rb[8];
+  double2 ra[2], rb[4];
   auto load = [&](int it) {
-    const double2* src = reinterpret_cast<const double2*>(cache + (size_t)it * kSlotAO);
+    const double2* src = reinterpret_cast<const double2*>(cache + (size_t)it * kSlotAO + MODE * kSlotB);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) ra[j] = src[tid + 256 * j];
+    for (int j = 0; j < 2; ++j) ra[j] = src[tid + 512 * j];
     const int ti = bt_list[it];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = 2 * (tid + 256 * j);
+    for (int j = 0; j < 4; ++j) {
+      const int e = 2 * (tid + 512 * j);
       const int r = e / (kGroup * kTile), cc = e % (kGroup * kTile), q = cc / kTile;
       rb[j] = (q < nt) ? *reinterpret_cast<const double2*>(Dint + (size_t)(ti * kTile + r) * npad + jt[q] * kTile + (cc % kTile))
                        : double2{0.0, 0.0};
@@ -114,21 +125,22 @@ __global__ __launch_bounds__(256) void k_density_c64(const int* __restrict__ grp
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = 2 * (tid + 256 * j);
+    for (int j = 0; j < 2; ++j) {
+      const int e = 2 * (tid + 512 * j);
       sm.A[buf][e / kTile][e % kTile] = ra[j].x;
       sm.A[buf][e / kTile][e % kTile + 1] = ra[j].y;
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = 2 * (tid + 256 * j);
+    for (int j = 0; j < 4; ++j) {
+      const int e = 2 * (tid + 512 * j);
       *reinterpret_cast<double2*>(&sm.B[buf][e / (kGroup * kTile)][e % (kGroup * kTile)]) = rb[j];
     }
   };
 
-  dbl4 acc[2 * kGroup];
+  dbl4 acc[4];
 #pragma unroll
-  for (int c = 0; c < 2 * kGroup; ++c) acc[c] = dbl4{0, 0, 0, 0};
+  for (int c = 0; c < 4; ++c) acc[c] = dbl4{0, 0, 0, 0};
+  const int ncb = min(4, max(0, 2 * nt - 4 * wc));  // valid column blocks of this wave
   if (it0 < it1) load(it0);
   int buf = 0;
   for (int it = it0; it < it1; ++it) {
@@ -137,37 +149,37 @@ __global__ __launch_bounds__(256) void k_density_c64(const int* __restrict__ grp
     if (it + 1 < it1) load(it + 1);
 #pragma unroll
     for (int ks = 0; ks < kTile / 4; ++ks) {
-      const double a = sm.A[buf][16 * wave + lrow][4 * ks + lk];
+      const double a = sm.A[buf][16 * wr + lrow][4 * ks + lk];
+      // columns of absent tiles hold zeros in LDS, so every wave runs all four
 #pragma unroll
-      for (int cb = 0; cb < 2 * kGroup; ++cb)
-        if (cb < 2 * nt) acc[cb] = mfma_f64(a, sm.B[buf][4 * ks + lk][16 * cb + lrow], acc[cb]);
+      for (int cb = 0; cb < 4; ++cb) acc[cb] = mfma_f64(a, sm.B[buf][4 * ks + lk][64 * wc + 16 * cb + lrow], acc[cb]);
     }
     buf ^= 1;
   }
 
-  // Epilogue: X[g = 16w + 4r + lk][nu = 16h + lrow] of tile q, against phi and grad phi.
+  // Epilogue: X[g = 16wr + 4r + lk][nu = 16h + lrow] of tile q = 2wc + cb/2, against phi and grad phi.
   double s[4][4];
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
     for (int r = 0; r < 4; ++r) s[c][r] = 0.0;
 #pragma unroll
-  for (int q = 0; q < kGroup; ++q) {
-    if (q < nt) {
+  for (int cb = 0; cb < 4; ++cb) {
+    if (cb < ncb) {
+      const int q = 2 * wc + cb / 2, h = cb % 2;
       const double* src = cache + (size_t)grp_slots[grp * kGroup + q] * kSlotAO;
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int r = 0; r < 4; ++r) {
+        const int g = 16 * wr + 4 * r + lk;
+        const double x = acc[cb][r];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int g = 16 * wave + 4 * r + lk;
-          const double x = acc[2 * q + h][r];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) s[c][r] += x * src[((size_t)c * kBatch + g) * kTile + 16 * h + lrow];
-        }
+        for (int c = 0; c < NOUT; ++c)
+          s[c][r] += x * src[((size_t)(MODE == 0 ? c : MODE) * kBatch + g) * kTile + 16 * h + lrow];
+      }
     }
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < NOUT; ++c)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       double v = s[c][r];
@@ -179,143 +191,166 @@ __global__ __launch_bounds__(256) void k_density_c64(const int* __restrict__ grp
     }
   double out = 0;
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < NOUT; ++c)
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (lrow == 4 * c + r) out = s[c][r];
   const int c = lrow >> 2, r = lrow & 3;
-  part[((size_t)grp * 4 + c) * kBatch + 16 * wave + 4 * r + lk] = out;
+  if (c < NOUT) sm.red[wc][c][16 * wr + 4 * r + lk] = out;
+  __syncthreads();
+  if (tid < NOUT * kBatch) {
+    const int cc = tid / kBatch, g = tid % kBatch;
+    part[(size_t)grp * grp_stride + cc * kBatch + g] = sm.red[0][cc][g] + sm.red[1][cc][g];
+  }
 }
 
-void launch_density_c64(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_slots,
-                        const int* grp_ntiles, const int* bt_off, const int* bt_list, const double* cache,
-                        const double* Dint, int npad, double* part) {
+void launch_density_c64(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
+                        const int* grp_slots, const int* grp_ntiles, const int* bt_off, const int* bt_list,
+                        const double* cache, const double* Dint, int npad, double* part) {
   if (ngroups == 0) return;
-  hipLaunchKernelGGL(k_density_c64, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_slots, grp_ntiles, bt_off,
-                     bt_list, cache, Dint, npad, part);
+  const int stride = (mode == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
+  switch (mode) {
+    case 0: hipLaunchKernelGGL(k_density_c64<0>, dim3(ngroups), dim3(512), 0, s, grp_batch, grp_tiles, grp_slots, grp_ntiles, bt_off, bt_list, cache, Dint, npad, part, stride); break;
+    case 1: hipLaunchKernelGGL(k_density_c64<1>, dim3(ngroups), dim3(512), 0, s, grp_batch, grp_tiles, grp_slots, grp_ntiles, bt_off, bt_list, cache, Dint, npad, part, stride); break;
+    case 2: hipLaunchKernelGGL(k_density_c64<2>, dim3(ngroups), dim3(512), 0, s, grp_batch, grp_tiles, grp_slots, grp_ntiles, bt_off, bt_list, cache, Dint, npad, part, stride); break;
+    default: hipLaunchKernelGGL(k_density_c64<3>, dim3(ngroups), dim3(512), 0, s, grp_batch, grp_tiles, grp_slots, grp_ntiles, bt_off, bt_list, cache, Dint, npad, part, stride); break;
+  }
   CHX_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
 // B = a phi + c . grad phi for every slot of a batch (one block per batch).
+// Meta-GGA (NC = 4) adds T_r = t d_r phi, t = w v_tau / 4, so that
+// V = M + M^T carries (1/2) w v_tau grad phi_mu . grad phi_nu.
+template <int NC>
 __global__ __launch_bounds__(256) void k_make_b(const int* __restrict__ bt_off, const double* __restrict__ coef,
                                                 const double* __restrict__ cache, double* __restrict__ bmat) {
-  __shared__ double ca[kBatch], cx[kBatch], cy[kBatch], cz[kBatch];
+  __shared__ double ca[kBatch], cx[kBatch], cy[kBatch], cz[kBatch], ct[kBatch];
   const int b = blockIdx.x, tid = threadIdx.x;
   if (tid < kBatch) {
     const size_t slot = (size_t)b * kBatch + tid;
-    ca[tid] = coef[4 * slot];
-    cx[tid] = coef[4 * slot + 1];
-    cy[tid] = coef[4 * slot + 2];
-    cz[tid] = coef[4 * slot + 3];
+    ca[tid] = coef[kCoef * slot];
+    cx[tid] = coef[kCoef * slot + 1];
+    cy[tid] = coef[kCoef * slot + 2];
+    cz[tid] = coef[kCoef * slot + 3];
+    ct[tid] = coef[kCoef * slot + 4];
   }
   __syncthreads();
   for (int it = bt_off[b]; it < bt_off[b + 1]; ++it) {
     const double* src = cache + (size_t)it * kSlotAO;
-    double* dst = bmat + (size_t)it * kSlotB;
+    double* dst = bmat + (size_t)it * NC * kSlotB;
     for (int e = tid; e < kSlotB; e += 256) {
       const int g = e / kTile;
-      dst[e] = ca[g] * src[e] + cx[g] * src[kSlotB + e] + cy[g] * src[2 * kSlotB + e] + cz[g] * src[3 * kSlotB + e];
+      const double dx = src[kSlotB + e], dy = src[2 * kSlotB + e], dz = src[3 * kSlotB + e];
+      dst[e] = ca[g] * src[e] + cx[g] * dx + cy[g] * dy + cz[g] * dz;
+      if (NC == 4) {
+        dst[kSlotB + e] = ct[g] * dx;
+        dst[2 * kSlotB + e] = ct[g] * dy;
+        dst[3 * kSlotB + e] = ct[g] * dz;
+      }
     }
   }
 }
 
-void launch_make_b(hipStream_t s, int nbatches, const int* bt_off, const double* coef, const double* cache, double* bmat) {
+void launch_make_b(hipStream_t s, int ncomp, int nbatches, const int* bt_off, const double* coef, const double* cache,
+                   double* bmat) {
   if (nbatches == 0) return;
-  hipLaunchKernelGGL(k_make_b, dim3(nbatches), dim3(256), 0, s, bt_off, coef, cache, bmat);
+  if (ncomp == 4)
+    hipLaunchKernelGGL(k_make_b<4>, dim3(nbatches), dim3(256), 0, s, bt_off, coef, cache, bmat);
+  else
+    hipLaunchKernelGGL(k_make_b<1>, dim3(nbatches), dim3(256), 0, s, bt_off, coef, cache, bmat);
   CHX_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
 // Vxc from the cache: M_IJ += sum_g phi_I[g]^T B_J[g] over the work item's
-// batches, 32 points per LDS stage (double-buffered). Wave w owns rows
-// mu = 32w..32w+31 of the 128 x 128 block. One atomic add per element per item.
+// batches, 32 points per LDS stage (double-buffered), 8 waves: wave w owns rows
+// mu = 16w..16w+15 of the 128 x 128 block. One atomic add per element per item.
 struct VxcCSmem {
   double P[2][kHalf][kSupCAO + 1];
   double Bt[2][kHalf][kSupCAO + 1];
 };
 
-__global__ __launch_bounds__(256) void k_vxc_c64(const int4* __restrict__ work, const int* __restrict__ pair_batches,
+// NC = 1: M = phi^T B. NC = 4 (meta-GGA): also sum_r (d_r phi)^T T_r, each
+// component streamed as further K stages.
+template <int NC>
+__global__ __launch_bounds__(512) void k_vxc_c64(const int4* __restrict__ work, const int* __restrict__ pair_batches,
                                                  const int* __restrict__ slot_of, int ntiles,
                                                  const double* __restrict__ cache, const double* __restrict__ bmat,
                                                  double* __restrict__ Mint, int npad) {
   __shared__ VxcCSmem sm;
   const int4 wk = work[blockIdx.x];
   const int I = wk.x, J = wk.y;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lrow = lane & 15, lk = lane >> 4;
-  // staging: per stage 4 tiles x 32 points x 32 AOs for each operand = 4096 doubles
-  // each; thread t moves 8 double2 of each.
-  double2 rp[8], rq[8];
-  auto load = [&](int it, int h) {
+  // Per stage and operand: 4 tiles x 32 points x 32 AOs; thread t moves the
+  // double2 at element 2 (t + 512 j) of tile j.
+  double2 rp[kSupC], rq[kSupC];
+  const int r0 = 2 * tid;  // element within a tile's 32 x 32 block
+  auto load = [&](int k) {  // stage k -> (batch, half, component)
+    const int it = wk.z + k / (2 * NC), h = (k / NC) % 2, c = k % NC;
     const int b = pair_batches[it];
+    const int g = h * kHalf + r0 / kTile, mu = r0 % kTile;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = 2 * (tid + 256 * j);            // over [tile q][g][mu]
-      const int q = e / (kHalf * kTile), r = e % (kHalf * kTile);
-      const int g = h * kHalf + r / kTile, mu = r % kTile;
-      const int si = slot_of[(size_t)b * ntiles + I * kSupC + q];
-      const int sj = slot_of[(size_t)b * ntiles + J * kSupC + q];
-      rp[j] = si >= 0 ? *reinterpret_cast<const double2*>(cache + (size_t)si * kSlotAO + g * kTile + mu) : double2{0.0, 0.0};
-      rq[j] = sj >= 0 ? *reinterpret_cast<const double2*>(bmat + (size_t)sj * kSlotB + g * kTile + mu) : double2{0.0, 0.0};
+    for (int j = 0; j < kSupC; ++j) {
+      const int si = slot_of[(size_t)b * ntiles + I * kSupC + j];
+      const int sj = slot_of[(size_t)b * ntiles + J * kSupC + j];
+      rp[j] = si >= 0 ? *reinterpret_cast<const double2*>(cache + (size_t)si * kSlotAO + (size_t)c * kSlotB + g * kTile + mu)
+                      : double2{0.0, 0.0};
+      rq[j] = sj >= 0 ? *reinterpret_cast<const double2*>(bmat + ((size_t)sj * NC + c) * kSlotB + g * kTile + mu)
+                      : double2{0.0, 0.0};
     }
   };
   auto store = [&](int buf) {
+    const int g = r0 / kTile, m = r0 % kTile;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = 2 * (tid + 256 * j);
-      const int q = e / (kHalf * kTile), r = e % (kHalf * kTile);
-      const int g = r / kTile, col = q * kTile + r % kTile;
+    for (int j = 0; j < kSupC; ++j) {
+      const int col = j * kTile + m;
       sm.P[buf][g][col] = rp[j].x;
       sm.P[buf][g][col + 1] = rp[j].y;
       sm.Bt[buf][g][col] = rq[j].x;
       sm.Bt[buf][g][col + 1] = rq[j].y;
     }
   };
-  dbl4 acc[2][8];
+  dbl4 acc[8];
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = dbl4{0, 0, 0, 0};
-  const int nstage = (wk.w - wk.z) * (kBatch / kHalf);
-  if (nstage > 0) load(wk.z, 0);
+  for (int cb = 0; cb < 8; ++cb) acc[cb] = dbl4{0, 0, 0, 0};
+  const int nstage = (wk.w - wk.z) * (kBatch / kHalf) * NC;
+  if (nstage > 0) load(0);
   int buf = 0;
   for (int k = 0; k < nstage; ++k) {
     store(buf);
     __syncthreads();
-    if (k + 1 < nstage) load(wk.z + (k + 1) / (kBatch / kHalf), (k + 1) % (kBatch / kHalf));
+    if (k + 1 < nstage) load(k + 1);
 #pragma unroll
     for (int ks = 0; ks < kHalf / 4; ++ks) {
       const int g = 4 * ks + lk;
-      double a[2];
+      const double a = sm.P[buf][g][16 * wave + lrow];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) a[rb] = sm.P[buf][g][32 * wave + 16 * rb + lrow];
-#pragma unroll
-      for (int cb = 0; cb < 8; ++cb) {
-        const double bb = sm.Bt[buf][g][16 * cb + lrow];
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb) acc[rb][cb] = mfma_f64(a[rb], bb, acc[rb][cb]);
-      }
+      for (int cb = 0; cb < 8; ++cb) acc[cb] = mfma_f64(a, sm.Bt[buf][g][16 * cb + lrow], acc[cb]);
     }
     buf ^= 1;
   }
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int cb = 0; cb < 8; ++cb)
 #pragma unroll
-    for (int cb = 0; cb < 8; ++cb)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int mu = I * kSupCAO + 32 * wave + 16 * rb + 4 * r + lk;
-        const int nu = J * kSupCAO + 16 * cb + lrow;
-        unsafeAtomicAdd(&Mint[(size_t)mu * npad + nu], acc[rb][cb][r]);
-      }
+    for (int r = 0; r < 4; ++r) {
+      const int mu = I * kSupCAO + 16 * wave + 4 * r + lk;
+      const int nu = J * kSupCAO + 16 * cb + lrow;
+      unsafeAtomicAdd(&Mint[(size_t)mu * npad + nu], acc[cb][r]);
+    }
 }
 
-void launch_vxc_c64(hipStream_t s, int nwork, const int4* work, const int* pair_batches, const int* slot_of, int ntiles,
-                    const double* cache, const double* bmat, double* Mint, int npad) {
+void launch_vxc_c64(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
+                    int ntiles, const double* cache, const double* bmat, double* Mint, int npad) {
   if (nwork == 0) return;
-  hipLaunchKernelGGL(k_vxc_c64, dim3(nwork), dim3(256), 0, s, work, pair_batches, slot_of, ntiles, cache, bmat, Mint,
-                     npad);
+  if (ncomp == 4)
+    hipLaunchKernelGGL(k_vxc_c64<4>, dim3(nwork), dim3(512), 0, s, work, pair_batches, slot_of, ntiles, cache, bmat,
+                       Mint, npad);
+  else
+    hipLaunchKernelGGL(k_vxc_c64<1>, dim3(nwork), dim3(512), 0, s, work, pair_batches, slot_of, ntiles, cache, bmat,
+                       Mint, npad);
   CHX_HIP(hipGetLastError());
 }
 

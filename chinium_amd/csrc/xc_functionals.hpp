@@ -124,24 +124,113 @@ __device__ __forceinline__ D3 e_lyp_c(const D3& rho, const D3& rho13, const D3& 
   return -a * rho * inv_den - (a * b) * ex * inv_den * (CF * rho - rm53 * sigma * (1.0 / 24.0 + (7.0 / 72.0) * delta));
 }
 
+// ---- r2SCAN (Furness, Kaplan, Ning, Perdew, Sun, JPCL 11, 8208 (2020) and
+// erratum), spin-unpolarised: p = s^2, alpha_bar = (tau - tau_W)/(tau_unif + eta tau_W).
+struct R2Poly { double c[8]; double c1, c2, d; };
+__device__ __forceinline__ D3 r2_interp(const D3& a, const R2Poly& f) {
+  if (a.v <= 0.0) return dexp(-f.c1 * a / (1.0 - a));
+  if (a.v <= 2.5) {
+    D3 r = dconst(f.c[7]);
+#pragma unroll
+    for (int i = 6; i >= 0; --i) r = r * a + f.c[i];
+    return r;
+  }
+  return -f.d * dexp(f.c2 / (1.0 - a));
+}
+__device__ __forceinline__ double r2_slope(const R2Poly& f) {
+  double s = 0;
+#pragma unroll
+  for (int i = 1; i < 8; ++i) s += i * f.c[i];
+  return s;
+}
+__device__ __forceinline__ R2Poly r2_fx() {
+  return {{1.0, -0.667, -0.4445555, -0.663086601049, 1.451297044490, -0.887998041597, 0.234528941479, -0.023185843322},
+          0.667, 0.8, 1.24};
+}
+__device__ __forceinline__ R2Poly r2_fc() {
+  return {{1.0, -0.64, -0.4352, -1.535685604549, 3.061560252175, -1.915710236206, 0.516884468372, -0.051848879792},
+          0.64, 1.5, 0.7};
+}
+constexpr double kR2Eta = 0.001;
+constexpr double kR2Dp2_4 = 0.361 * 0.361 * 0.361 * 0.361;
+
+__device__ __forceinline__ void r2_descriptors(const D3& rho, const D3& rho13, const D3& sigma, const D3& tau,
+                                               D3* p, D3* abar) {
+  const double k2 = 9.570780000627305;  // (3 pi^2)^(2/3)
+  const D3 r53 = rho * rho13 * rho13;
+  *p = sigma / ((4.0 * k2) * rho * r53);
+  const D3 tw = sigma / (8.0 * rho);
+  const D3 tu = (0.3 * k2) * r53;
+  *abar = (tau - tw) / (tu + kR2Eta * tw);
+}
+
+__device__ __forceinline__ D3 e_r2scan_x(const D3& rho, const D3& rho13, const D3& sigma, const D3& tau) {
+  const double h0x = 1.174, k1 = 0.065, muak = 10.0 / 81.0, a1 = 4.9479;
+  const R2Poly fxp = r2_fx();
+  const double c2 = -r2_slope(fxp) * (1.0 - h0x);
+  const double ceta = 20.0 / 27.0 + kR2Eta * 5.0 / 3.0;
+  D3 p, ab;
+  r2_descriptors(rho, rho13, sigma, tau, &p, &ab);
+  const D3 x = (ceta * c2 * dexp(-(p * p) / kR2Dp2_4) + muak) * p;
+  const D3 h1x = (1.0 + k1) - k1 / (1.0 + x / k1);
+  const D3 fx = r2_interp(ab, fxp);
+  const D3 gx = p.v > 0.0 ? 1.0 - dexp(-a1 / dpow(p, 0.25)) : dconst(1.0);
+  return e_lda_x(rho, rho13) * (h1x + fx * (h0x - h1x)) * gx;
+}
+
+__device__ __forceinline__ D3 e_r2scan_c(const D3& rho, const D3& rho13, const D3& sigma, const D3& tau) {
+  const double gamma = 0.031090690869654895;
+  const double b1c = 0.0285764, b2c = 0.0889, b3c = 0.125541, chi_inf = 0.128026;
+  const R2Poly fcp = r2_fc();
+  D3 p, ab;
+  r2_descriptors(rho, rho13, sigma, tau, &p, &ab);
+  const D3 rs = rs_from_rho13(rho13);
+  const D3 srs = dsqrt(rs);
+  const double A = 0.0310907, a1 = 0.21370, B1 = 7.5957, B2 = 3.5876, B3 = 1.6382, B4 = 0.49294;
+  const D3 Q = (2.0 * A) * (srs * (B1 + B3 * rs) + rs * (B2 + B4 * rs));
+  const D3 dQ = (2.0 * A) * ((0.5 * B1) / srs + B2 + (1.5 * B3) * srs + (2.0 * B4) * rs);
+  const D3 L = dlog(1.0 + 1.0 / Q);
+  const D3 e1 = (-2.0 * A) * (1.0 + a1 * rs) * L;
+  const D3 de1 = (-2.0 * A * a1) * L + (2.0 * A) * (1.0 + a1 * rs) * dQ / (Q * Q + Q);
+  const D3 den0 = 1.0 + b2c * srs + b3c * rs;
+  const D3 e0 = -b1c / den0;
+  const D3 de0 = b1c * ((0.5 * b2c) / srs + b3c) / (den0 * den0);
+  const D3 beta = 0.066725 * (1.0 + 0.1 * rs) / (1.0 + 0.1778 * rs);
+  const D3 w1 = dexp(-e1 / gamma) - 1.0;
+  const D3 t2 = 1.5073033983379013 * p / rs;  // (3 pi^2 / 16)^(2/3) p / rs
+  const D3 y = beta * t2 / (gamma * w1);
+  const D3 dy = (r2_slope(fcp) / (27.0 * gamma)) / w1 * (20.0 * rs * (de0 - de1) - (45.0 * kR2Eta) * (e0 - e1)) * p *
+                dexp(-(p * p) / kR2Dp2_4);
+  const D3 g = dpow(1.0 + 4.0 * (y - dy), -0.25);
+  const D3 ec1 = e1 + gamma * dlog(1.0 + w1 * (1.0 - g));
+  const D3 w0 = dexp(-e0 / b1c) - 1.0;
+  const D3 ginf = dpow(1.0 + (4.0 * chi_inf) * p, -0.25);
+  const D3 ec0 = e0 + b1c * dlog(1.0 + w0 * (1.0 - ginf));
+  const D3 fc = r2_interp(ab, fcp);
+  return rho * (ec1 + fc * (ec0 - ec1));
+}
+
 // Functional codes handled by the kernels (libxc IDs).
 enum : int {
   kLdaX = 1, kLdaCVwn = 7, kLdaCVwnRpa = 8, kGgaXPbe = 101, kGgaCPbe = 130, kGgaXB88 = 106,
-  kGgaCLyp = 131, kGgaXPbeG = 321, kGgaCPbeG = 322, kHybB3lyp = 402
+  kGgaCLyp = 131, kGgaXPbeG = 321, kGgaCPbeG = 322, kHybB3lyp = 402, kMggaXR2scan = 497, kMggaCR2scan = 498
 };
 
 __host__ __device__ inline int code_family(int code) {
   switch (code) {
     case kLdaX: case kLdaCVwn: case kLdaCVwnRpa: return 0;
     case kGgaXPbe: case kGgaCPbe: case kGgaXB88: case kGgaCLyp: case kGgaXPbeG: case kGgaCPbeG: case kHybB3lyp: return 1;
+    case kMggaXR2scan: case kMggaCR2scan: return 2;
     default: return -1;
   }
 }
 __host__ __device__ inline double code_exx(int code) { return code == kHybB3lyp ? 0.2 : 0.0; }
 constexpr double kDensThreshold = 1e-15;
 
-__device__ __forceinline__ D3 energy_density(int code, const D3& rho, const D3& rho13, const D3& sigma) {
+__device__ __forceinline__ D3 energy_density(int code, const D3& rho, const D3& rho13, const D3& sigma, const D3& tau) {
   switch (code) {
+    case kMggaXR2scan: return e_r2scan_x(rho, rho13, sigma, tau);
+    case kMggaCR2scan: return e_r2scan_c(rho, rho13, sigma, tau);
     case kLdaX: return e_lda_x(rho, rho13);
     case kLdaCVwn: return rho * vwn5_eps(rs_from_rho13(rho13));
     case kLdaCVwnRpa: return rho * vwn_rpa_eps(rs_from_rho13(rho13));
@@ -159,27 +248,35 @@ __device__ __forceinline__ D3 energy_density(int code, const D3& rho, const D3& 
 }
 
 // Sum over codes at one point; returns eps (per particle) and derivatives.
-__device__ __forceinline__ void eval_codes(const int* codes, int ncodes, double rho, double sigma,
-                                           double* eps, double* vrho, double* vsigma) {
-  double e = 0, vr = 0, vs = 0;
+// Meta-GGA codes see tau floored at 1e-20 and sigma capped at 8 rho tau.
+__device__ __forceinline__ void eval_codes(const int* codes, int ncodes, double rho, double sigma, double tau,
+                                           double* eps, double* vrho, double* vsigma, double* vtau) {
+  double e = 0, vr = 0, vs = 0, vt = 0;
   if (rho >= kDensThreshold) {
     const double sfloor = 1.0000000000000000e-40;  // (thr^(4/3))^2 for thr = 1e-15
     const double s = sigma > sfloor ? sigma : sfloor;
     const D3 r = {rho, 1.0, 0.0, 0.0};
-    const D3 sg = {s, 0.0, 1.0, 0.0};
-    const D3 r13 = dchain(r, ::cbrt(rho), 0.0);
-    const D3 rho13 = {r13.v, r13.v / (3.0 * rho), 0.0, 0.0};
+    const double r13v = ::cbrt(rho);
+    const D3 rho13 = {r13v, r13v / (3.0 * rho), 0.0, 0.0};
     for (int k = 0; k < ncodes; ++k) {
-      const D3 ed = energy_density(codes[k], r, rho13, sg);
+      const int code = codes[k];
+      double sk = s, tk = tau;
+      if (code_family(code) == 2) {
+        tk = tau > 1e-20 ? tau : 1e-20;
+        if (sk > 8.0 * rho * tk) sk = 8.0 * rho * tk;
+      }
+      const D3 ed = energy_density(code, r, rho13, D3{sk, 0.0, 1.0, 0.0}, D3{tk, 0.0, 0.0, 1.0});
       e += ed.v;
       vr += ed.d0;
       vs += ed.d1;
+      vt += ed.d2;
     }
     e /= rho;
   }
   *eps = e;
   *vrho = vr;
   *vsigma = vs;
+  *vtau = vt;
 }
 
 }  // namespace chx

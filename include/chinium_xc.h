@@ -78,7 +78,8 @@ int chx_grid_num_threads(const chx_grid* g, int* n);
  * in the library's point order (xyzw[4*i+k]). */
 int chx_grid_num_points(const chx_grid* g, long* n);
 int chx_grid_get_points(const chx_grid* g, double* xyzw);
-/* Grid::setType (src/Grid/Grid.cpp:14-20): -1 none, 0 LDA, 1 GGA, 2 meta-GGA. */
+/* Grid::setType (src/Grid/Grid.cpp:14-20): -1 none, 0 LDA, 1 GGA, 2 meta-GGA.
+ * Meta-GGA (tau) runs on the AO tables kept in HBM (the default when they fit). */
 int chx_grid_set_type(chx_grid* g, int type);
 /* Grid::getAO (src/Grid/Grid.cpp:22-33). The GPU path recomputes AO values
  * inside the contraction kernels; this call builds the per-batch AO screening
@@ -144,6 +145,13 @@ int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order);
  * ld_by_order (src/Grid/sphere_lebedev_rule.cpp:671-846) returns it; the grid
  * scales w by 4 pi (GridGeneration.cpp:133). Host only. */
 int chx_lebedev_rule(int npoints, double* xyzw);
+
+/* Functional values at n points on the device, with libxc's conventions as the
+ * reference receives them from xc_{lda,gga,mgga}_exc_vxc
+ * (src/ExchangeCorrelation.cpp:98,122,157): exc per particle and the summed
+ * derivatives over the functional's codes. Host arrays; sigma/tau may be NULL. */
+int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
+                       double* exc, double* vrho, double* vsigma, double* vtau);
 
 /* ---- Diagnostics (no reference counterpart) ---------------------------------
  * Runs one FP64 MFMA tile on the device and returns the largest deviation from

@@ -138,6 +138,93 @@ template <class T> T gga_c_lyp(const T& rho, const T& sigma) {
   return t1 + t2;
 }
 
+// ---- r2SCAN (Furness, Kaplan, Ning, Perdew, Sun, JPCL 11, 8208 (2020) and
+// erratum JPCL 11, 9248), spin-unpolarised. Shared pieces:
+//   p = |grad rho|^2 / (4 (3 pi^2)^(2/3) rho^(8/3)),
+//   alpha_bar = (tau - tau_W) / (tau_unif + eta tau_W), eta = 0.001,
+//   tau_W = sigma / (8 rho), tau_unif = (3/10)(3 pi^2)^(2/3) rho^(5/3).
+constexpr double kR2Eta = 0.001;
+constexpr double kR2Dp2 = 0.361;
+constexpr double kR2Cx[8] = {1.0, -0.667, -0.4445555, -0.663086601049, 1.451297044490, -0.887998041597,
+                             0.234528941479, -0.023185843322};
+constexpr double kR2Cc[8] = {1.0, -0.64, -0.4352, -1.535685604549, 3.061560252175, -1.915710236206,
+                             0.516884468372, -0.051848879792};
+
+// Interpolation f(alpha_bar): exp(-c1 a / (1 - a)) for a <= 0, the degree-7
+// polynomial for 0 < a <= 2.5, -d exp(c2 / (1 - a)) for a > 2.5.
+template <class T> T r2_interp(const T& a, const double* c, double c1, double c2, double d) {
+  if (a.v <= 0.0) return exp(-c1 * a / (1.0 - a));
+  if (a.v <= 2.5) {
+    T r = T(c[7]);
+    for (int i = 6; i >= 0; --i) r = r * a + c[i];
+    return r;
+  }
+  return -d * exp(c2 / (1.0 - a));
+}
+inline double r2_poly_slope(const double* c) {  // sum_{i>=1} i c_i
+  double s = 0;
+  for (int i = 1; i < 8; ++i) s += i * c[i];
+  return s;
+}
+
+template <class T> void r2_reduced(const T& rho, const T& sigma, const T& tau, T* p, T* abar) {
+  const double k2 = std::pow(3.0 * kPi * kPi, 2.0 / 3.0);
+  const T r83 = pow(rho, 8.0 / 3.0);
+  *p = sigma / (4.0 * k2 * r83);
+  const T tw = sigma / (8.0 * rho);
+  const T tu = 0.3 * k2 * pow(rho, 5.0 / 3.0);
+  *abar = (tau - tw) / (tu + kR2Eta * tw);
+}
+
+template <class T> T mgga_x_r2scan(const T& rho, const T& sigma, const T& tau) {
+  const double h0x = 1.174, k1 = 0.065, muak = 10.0 / 81.0, a1 = 4.9479;
+  const double ceta = 20.0 / 27.0 + kR2Eta * 5.0 / 3.0;
+  const double c2 = -r2_poly_slope(kR2Cx) * (1.0 - h0x);
+  T p, ab;
+  r2_reduced(rho, sigma, tau, &p, &ab);
+  const T x = (ceta * c2 * exp(-p * p / std::pow(kR2Dp2, 4)) + muak) * p;
+  const T h1x = 1.0 + k1 - k1 / (1.0 + x / k1);
+  const T fx = r2_interp(ab, kR2Cx, 0.667, 0.8, 1.24);
+  const T gx = p.v > 0 ? 1.0 - exp(-a1 / pow(p, 0.25)) : T(1.0);
+  return lda_x(rho) * (h1x + fx * (h0x - h1x)) * gx;
+}
+
+template <class T> T mgga_c_r2scan(const T& rho, const T& sigma, const T& tau) {
+  const double gamma = (1.0 - std::log(2.0)) / (kPi * kPi);
+  const double b1c = 0.0285764, b2c = 0.0889, b3c = 0.125541, chi_inf = 0.128026;
+  T p, ab;
+  r2_reduced(rho, sigma, tau, &p, &ab);
+  const T rs = rs_of(rho);
+  const T srs = sqrt(rs);
+  // LSDA1 = PW92 and its rs-derivative; LSDA0 = -b1c / (1 + b2c sqrt(rs) + b3c rs)
+  const double A = 0.0310907, a1 = 0.21370, B1 = 7.5957, B2 = 3.5876, B3 = 1.6382, B4 = 0.49294;
+  const T Q = 2.0 * A * (B1 * srs + B2 * rs + B3 * rs * srs + B4 * rs * rs);
+  const T dQ = 2.0 * A * (0.5 * B1 / srs + B2 + 1.5 * B3 * srs + 2.0 * B4 * rs);
+  const T L = log(1.0 + 1.0 / Q);
+  const T e1 = -2.0 * A * (1.0 + a1 * rs) * L;
+  const T de1 = -2.0 * A * a1 * L + 2.0 * A * (1.0 + a1 * rs) * dQ / (Q * Q + Q);
+  const T den0 = 1.0 + b2c * srs + b3c * rs;
+  const T e0lda = -b1c / den0;
+  const T de0 = b1c * (0.5 * b2c / srs + b3c) / (den0 * den0);
+  // eps_c^1 = eps_LSDA1 + H1 with the r2SCAN gradient correction Delta y
+  const T beta = 0.066725 * (1.0 + 0.1 * rs) / (1.0 + 0.1778 * rs);
+  const T w1 = exp(-e1 / gamma) - 1.0;
+  const T t2 = std::pow(3.0 * kPi * kPi / 16.0, 2.0 / 3.0) * p / rs;
+  const T y = beta * t2 / (gamma * w1);
+  const double dfc2 = r2_poly_slope(kR2Cc);
+  const T dy = dfc2 / (27.0 * gamma * w1) * (20.0 * rs * (de0 - de1) - 45.0 * kR2Eta * (e0lda - e1)) * p *
+               exp(-p * p / std::pow(kR2Dp2, 4));
+  const T g = pow(1.0 + 4.0 * (y - dy), -0.25);
+  const T H1 = gamma * log(1.0 + w1 * (1.0 - g));
+  const T ec1 = e1 + H1;
+  // eps_c^0 = eps_LDA0 + H0 (G_c(zeta = 0) = 1)
+  const T w0 = exp(-e0lda / b1c) - 1.0;
+  const T ginf = pow(1.0 + 4.0 * chi_inf * p, -0.25);
+  const T ec0 = e0lda + b1c * log(1.0 + w0 * (1.0 - ginf));
+  const T fc = r2_interp(ab, kR2Cc, 0.64, 1.5, 0.7);
+  return rho * (ec1 + fc * (ec0 - ec1));
+}
+
 // ---- Functional table (libxc IDs used by the reference's .df files).
 struct Component { int kind; double coef; };
 // kind: 1 lda_x, 7 vwn5, 8 vwn_rpa, 101/321 pbe_x, 130/322 pbe_c, 106 b88, 131 lyp
@@ -152,8 +239,10 @@ inline int family_of(int code) {
 inline double density_threshold(int /*code*/) { return 1e-15; }
 
 // Energy density rho*eps of one libxc code, on dual inputs (rho, sigma, tau).
-template <class T> T energy_density(int code, const T& rho, const T& sigma, const T& /*tau*/) {
+template <class T> T energy_density(int code, const T& rho, const T& sigma, const T& tau) {
   switch (code) {
+    case 497: return mgga_x_r2scan(rho, sigma, tau);
+    case 498: return mgga_c_r2scan(rho, sigma, tau);
     case 1: return lda_x(rho);
     case 7: return lda_c_vwn(rho, kVwn5);
     case 8: return lda_c_vwn(rho, kVwnRpa);

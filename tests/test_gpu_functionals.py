@@ -1,0 +1,88 @@
+"""Device functionals (chx_xc_eval_points, the libxc-shaped per-point call)
+against the golden vectors of the independent sympy/mpmath transcriptions and
+against the oracle, and meta-GGA (r2SCAN) XC builds against the oracle."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import chinium_amd as ca
+import oracle_lib as ol
+from chinium_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("fname,ncol", [("functionals.json", 2), ("r2scan.json", 3)])
+def test_device_functionals_match_golden(fname, ncol):
+    g = json.load(open(os.path.join(GOLD, fname)))
+    P = np.array(g["points"])
+    for code, vals in g["values"].items():
+        print("device functional %s code %s" % (fname, code), flush=True)
+        v = np.array(vals)
+        xc = ca.ExchangeCorrelation().from_codes([int(code)])
+        e, vr, vs, vt = xc.eval_points(P[:, 0], P[:, 1], P[:, 2] if ncol == 3 else None)
+        assert _rel(e, v[:, 0]).max() < 1e-12
+        assert _rel(vr, v[:, 1]).max() < 1e-11
+        m = P[:, 1] > 0
+        assert _rel(vs[m], v[m, 2]).max() < 1e-10
+        if ncol == 3:
+            assert _rel(vt, v[:, 3]).max() < 1e-9
+
+
+def test_device_functionals_match_oracle_random_points():
+    rng = np.random.default_rng(5)
+    n = 4000
+    rho = 10 ** rng.uniform(-12, 3, n)
+    k2 = (3 * np.pi ** 2) ** (2 / 3)
+    s = rng.uniform(0, 3, n)
+    sig = 4 * k2 * rho ** (8 / 3) * s * s
+    tw = sig / (8 * rho)
+    tau = tw + rng.uniform(0, 4, n) * (0.3 * k2 * rho ** (5 / 3) + 0.001 * tw)
+    for codes in ([321, 322], [402], [1, 7], [106, 131], [497, 498]):
+        xc = ca.ExchangeCorrelation().from_codes(codes)
+        d = xc.eval_points(rho, sig, tau)
+        o = ol.eval_functional(codes, rho, sig, tau)
+        # rounding differences are amplified by cancellation at very low density
+        # (rs ~ 1e4 in PW92 / PBE correlation), where the values do not matter
+        tol = np.where(rho >= 1e-6, 1e-11, 1e-7)
+        for name, a, b in zip(("exc", "vrho", "vsigma", "vtau"), d, o):
+            rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+            bad = rel > tol
+            if bad.any():
+                k = np.argmax(np.where(bad, rel, 0))
+                print("  %s %s worst rel %.2e at rho %.3e s %.3f" % (codes, name, rel[k], rho[k], s[k]), flush=True)
+            assert not bad.any(), (codes, name)
+
+
+@pytest.mark.parametrize("case", ["h2o", "ch4_tzvp"])
+def test_r2scan_parity(case):
+    if case == "h2o":
+        w = wl.build_workload("h2o_ccpvdz_sg1_pbe")
+        Z, xyz, basis, grid_name, nocc = w["Z"], w["xyz_bohr"], w["basis"], "SG-1", 5
+    else:
+        Z, X = wl.alkane_angstrom(1, 4)
+        Z, xyz, basis, grid_name, nocc = Z, X * ca.ANGSTROM_TO_BOHR, "def2-tzvp", "SG-1", 5
+    mol = ca.Molecule(Z, xyz, basis)
+    D, S = wl.workload_density(mol, nocc, 11)
+    print("r2SCAN parity %s (%d AOs)" % (case, mol.nbasis), flush=True)
+    grid = ca.Grid(mol, grid_name, 1, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read("R2SCAN")
+    assert xc.Family == "mGGA"
+    grid.setType(2)
+    grid.getAO(0, 0)
+    E, V = ca.xc_build(grid, xc, D)
+    orc = ol.OracleSystem(Z, xyz, grid_name, basis)
+    orc.set_functional("R2SCAN")
+    orc.get_ao()
+    Eo, No, Vo = orc.xc(D)
+    print("  E gpu %.14f oracle %.14f dE %.2e dV %.2e" % (E, Eo, abs(E - Eo), np.abs(V - Vo).max()), flush=True)
+    assert abs(E - Eo) <= 1e-10
+    assert np.abs(V - Vo).max() <= 1e-9

@@ -77,3 +77,36 @@ def test_derivatives_by_finite_difference():
         em, _, _, _ = ol.eval_functional([code], rho, sig * (1 - h))
         fd_s = (ep - em) * rho / (2 * h * sig)
         assert np.allclose(vs, fd_s, rtol=1e-6)
+
+
+R2 = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "r2scan.json")))
+
+
+@pytest.mark.parametrize("code", ["497", "498"])
+def test_r2scan_against_mpmath_vectors(code):
+    P = np.array(R2["points"])
+    ref = np.array(R2["values"][code])
+    e, vr, vs, vt = ol.eval_functional([int(code)], P[:, 0], P[:, 1], P[:, 2])
+    rel = lambda a, b: np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    assert rel(e, ref[:, 0]).max() < 1e-12
+    assert rel(vr, ref[:, 1]).max() < 1e-11
+    assert rel(vs, ref[:, 2]).max() < 1e-10
+    assert rel(vt, ref[:, 3]).max() < 1e-9
+
+
+def test_r2scan_exact_constraints():
+    """Uniform gas (p = 0, alpha = 1): F_x = 1 and eps_c = PW92; one-orbital
+    limit (alpha = 0, p = 0): F_x = h_x^0 = 1.174."""
+    k2 = (3 * np.pi ** 2) ** (2 / 3)
+    rho = np.array([1e-3, 0.05, 0.7, 9.0])
+    tu = 0.3 * k2 * rho ** (5 / 3)
+    zero = np.full(4, 1e-40)
+    ex, _, _, _ = ol.eval_functional([497], rho, zero, tu)
+    el, _, _, _ = ol.eval_functional([1], rho)
+    assert np.allclose(ex, el, rtol=1e-13)
+    ec, _, _, _ = ol.eval_functional([498], rho, zero, tu)
+    pbe_c0, _, _, _ = ol.eval_functional([130], rho, zero)   # PBE correlation at zero gradient is PW92
+    assert np.allclose(ec, pbe_c0, rtol=1e-12)
+    ex0, _, _, _ = ol.eval_functional([497], rho, zero, np.full(4, 1e-30) + 0.0 * rho)
+    # tau floored at 1e-20 keeps alpha at ~0 for these densities
+    assert np.allclose(ex0 / el, 1.174, rtol=1e-6)

@@ -87,3 +87,21 @@ def test_unknown_code_rejected(h2o):
     o, D, S = h2o
     with pytest.raises(RuntimeError):
         o.set_functional(codes=[9999])
+
+
+def test_r2scan_vxc_is_derivative(h2o):
+    o, D, S = h2o
+    o.set_functional("R2SCAN")
+    assert o.family == 2
+    o.get_ao()
+    E, N, V = o.xc(D)
+    rng = np.random.default_rng(8)
+    dD = rng.standard_normal(D.shape)
+    dD = 0.05 * (dD + dD.T)
+    f = lambda t: o.xc(D + t * dD)[0]
+    h = 1e-3
+    d1 = (f(h) - f(-h)) / (2 * h)
+    d2 = (f(h / 2) - f(-h / 2)) / h
+    fd = (4 * d2 - d1) / 3
+    an = np.sum(V * dD)
+    assert abs(fd - an) <= 1e-7 * max(1.0, abs(an))
