@@ -6,9 +6,15 @@ seeded S-orthonormal 384 x 80 C. One step = one Vxc build in the reference's
 order: getDensity(D) -> Evaluate("ev") -> getEnergy -> getFock, with D and V
 in device memory (HBM) and E_xc returned to the host.
 
+Both GPU forms are timed on the same inputs: the plain FP64 form and the
+chosen mixed-precision form (split-BF16 MFMA contractions with FP32
+accumulation; functional, E_xc, N_e and V sums in FP64). `value` is the
+chosen form's time per build; `plain_fp64` holds the FP64 form's figures.
+
 Prints one JSON line (rank 0): the metric, the per-kernel roofline of the
-dominant kernel (HIP events on the library's stream), accuracy against the
-FP64 CPU oracle, and the oracle's own time on this host (CPU baseline).
+dominant kernel (HIP events on the library's stream), accuracy of both forms
+against the FP64 CPU oracle, and the oracle's own time on this host (CPU
+baseline).
 """
 import argparse
 import json
@@ -20,6 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix
+# MI355X dense BF16 matrix peak (2.5 PFLOP/s) / 3: a split product takes three BF16 MFMAs.
+BF16_SPLIT_PEAK_TFLOPS = 2500.0 / 3.0
 HBM_PEAK_GBS = 8000.0
 
 
@@ -48,63 +56,75 @@ def main():
     mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
     n = mol.nbasis
     D, S = wl.workload_density(mol, w["nocc"], w["seed"])
-    t0 = time.time()
-    grid = ca.Grid(mol, w["grid"], 1, 0)
     xc = ca.ExchangeCorrelation()
     xc.Read(w["functional"])
-    grid.setAOCache(args.ao_cache)
-    grid.setType(xc.family)
-    grid.getAO(0, 0)
-    t_setup = time.time() - t0
-    st = grid.stats()
-    stream = grid.stream()
-
     dD = hip.DeviceArray(n * n)
     dD.upload(np.asfortranarray(D).ravel(order="F"))
-    dV = hip.DeviceArray(n * n)
 
-    def step():
-        grid.getDensityDevice(dD.ptr, 1, n)
-        xc.Evaluate("ev", grid)
-        e = grid.getEnergy()
-        grid.getFockDevice(dV.ptr, 1, n)
-        return e
+    def measure(precision):
+        """Builds the grid in the given form and times K XC builds (D and V in HBM)."""
+        t0 = time.time()
+        grid = ca.Grid(mol, w["grid"], 1, 0)
+        grid.setPrecision(precision)
+        grid.setAOCache(args.ao_cache)
+        grid.setType(xc.family)
+        grid.getAO(0, 0)
+        t_setup = time.time() - t0
+        stream = grid.stream()
+        dV = hip.DeviceArray(n * n)
 
-    for _ in range(args.warmup):
-        step()
-    hip.device_synchronize()
-    e0, e1 = hip.Event(), hip.Event()
-    wall0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        E = step()
-    e1.record(stream)
-    hip.device_synchronize()
-    wall = time.perf_counter() - wall0
-    ms_total = e0.elapsed_ms(e1)
-    ms_step = ms_total / args.steps
+        def step():
+            grid.getDensityDevice(dD.ptr, 1, n)
+            xc.Evaluate("ev", grid)
+            e = grid.getEnergy()
+            grid.getFockDevice(dV.ptr, 1, n)
+            return e
 
-    # Second timed pass with per-kernel events (for the roofline).
-    grid.profile(True)
-    grid.kernel_times(reset=True)
-    for _ in range(args.steps):
-        step()
-    kt = grid.kernel_times(reset=True)
-    grid.profile(False)
+        for _ in range(args.warmup):
+            step()
+        hip.device_synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        wall0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            E = step()
+        e1.record(stream)
+        hip.device_synchronize()
+        wall = time.perf_counter() - wall0
+        ms_step = e0.elapsed_ms(e1) / args.steps
+        # second timed pass with per-kernel HIP events on the library's stream
+        grid.profile(True)
+        grid.kernel_times(reset=True)
+        for _ in range(args.steps):
+            step()
+        kt = grid.kernel_times(reset=True)
+        grid.profile(False)
+        V = dV.download().reshape((n, n), order="F")
+        r = dict(grid=grid, E=E, V=V, N=grid.getNumElectrons(), ms=ms_step, wall_ms=1e3 * wall / args.steps,
+                 setup_s=t_setup, kt=kt, stats=grid.stats(), cached=grid.aoCached())
+        del dV
+        return r
 
-    V = dV.download().reshape((n, n), order="F")
-    N = grid.getNumElectrons()
-
-    flops_build = st["flops_per_build"]
-    flops_kernel = flops_build / 2.0  # each of the two contractions: 2 * sum_b g_b n_b^2
-    dens_ms = kt["density"][0] / max(1, kt["density"][1])
-    vxc_ms = kt["vxc"][0] / max(1, kt["vxc"][1])
-    dom, dom_ms = ("density", dens_ms) if dens_ms >= vxc_ms else ("vxc", vxc_ms)
-    achieved = flops_kernel / (dom_ms * 1e-3) / 1e12
-    roofline = {"kernel": "k_density_f64" if dom == "density" else "k_vxc_f64", "bound": "mfma",
-                "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+    def roofline(r, peak, names):
+        """Dominant contraction kernel: algorithmic FLOPs per launch / mean launch time."""
+        flops_kernel = r["stats"]["flops_per_build"] / 2.0  # each contraction: 2 * sum_b g_b n_b^2
+        kt = r["kt"]
+        dens_ms = kt["density"][0] / max(1, kt["density"][1])
+        vxc_ms = kt["vxc"][0] / max(1, kt["vxc"][1])
+        dom, dom_ms = ("density", dens_ms) if dens_ms >= vxc_ms else ("vxc", vxc_ms)
+        achieved = flops_kernel / (dom_ms * 1e-3) / 1e12
+        return {"kernel": names[dom], "bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 2),
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
                 "avg_ms": round(dom_ms, 4), "flops_per_launch": flops_kernel}
+
+    plain = measure(ca.CHX_PREC_FP64)
+    chosen = measure(ca.CHX_PREC_MIXED)
+    st = chosen["stats"]
+    flops_build = st["flops_per_build"]
+    ms_step = chosen["ms"]
+    names64 = {"density": "k_density_c64<0>" if plain["cached"] else "k_density_f64",
+               "vxc": "k_vxc_c64<1>" if plain["cached"] else "k_vxc_f64"}
+    namesmx = {"density": "k_density_mx<0>", "vxc": "k_vxc_mx<1>"}
 
     out = {
         "metric": "Vxc build ms/SCF-iter & TFLOP/s, chosen vs plain FP64 GPU at |dE_xc|<=1e-5 Eh/atom",
@@ -117,23 +137,35 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3-split/f32-acc (f64 functional and sums)",
         "data": "synthetic (seeded geometry and density matrix)",
         "config": {"workload": "(H2O)16 / def2-SVP / SG-2 / B3LYP", "name": args.workload,
                    "points": st["points"], "nbasis": n, "batches": st["batches"],
-                   "mean_alive_tiles_per_batch": round(st["mean_tiles"], 2), "form": "plain FP64 (chosen = plain)",
-                   "ao_tables": "cached in HBM" if grid.aoCached() else "recomputed in kernels",
-                   "screening_threshold": 1e-14},
+                   "mean_alive_tiles_per_batch": round(st["mean_tiles"], 2),
+                   "form": "mixed: split-BF16 MFMA contractions, FP32 accumulation, FP64 functional/E/N/V sums",
+                   "ao_tables": "cached in HBM (bf16 hi/lo pairs)", "screening_threshold": 1e-14},
+        "speedup_vs_plain_fp64": round(plain["ms"] / ms_step, 3),
         "tflops": round(flops_build / (ms_step * 1e-3) / 1e12, 3),
         "gpoints_per_s": round(st["points"] / (ms_step * 1e-3) / 1e9, 4),
         "flops_per_build": flops_build,
-        "kernels_ms": {k: round(v[0] / max(1, v[1]), 4) for k, v in kt.items()},
-        "wall_ms_per_step": round(1e3 * wall / args.steps, 4),
-        "setup_s": round(t_setup, 3),
-        "E_xc": E,
-        "N_e": N,
-        "roofline": roofline,
+        "kernels_ms": {k: round(v[0] / max(1, v[1]), 4) for k, v in chosen["kt"].items()},
+        "wall_ms_per_step": round(chosen["wall_ms"], 4),
+        "setup_s": round(chosen["setup_s"], 3),
+        "E_xc": chosen["E"],
+        "N_e": chosen["N"],
+        "roofline": roofline(chosen, BF16_SPLIT_PEAK_TFLOPS, namesmx),
+        "plain_fp64": {
+            "ms_per_step": round(plain["ms"], 4),
+            "tflops": round(flops_build / (plain["ms"] * 1e-3) / 1e12, 3),
+            "kernels_ms": {k: round(v[0] / max(1, v[1]), 4) for k, v in plain["kt"].items()},
+            "roofline": roofline(plain, FP64_MFMA_PEAK_TFLOPS, names64),
+            "ao_tables": "cached in HBM (FP64)" if plain["cached"] else "recomputed in kernels",
+            "E_xc": plain["E"],
+            "dE_vs_chosen": abs(plain["E"] - chosen["E"]),
+            "max_dV_vs_chosen": float(np.abs(plain["V"] - chosen["V"]).max()),
+        },
     }
+    E, V, N = chosen["E"], chosen["V"], chosen["N"]
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -151,6 +183,8 @@ def main():
                                          "does once per SCF)"}
         out["accuracy_vs_fp64_oracle"] = {"dE_xc": abs(E - Eo), "dE_xc_per_atom": abs(E - Eo) / natoms,
                                           "max_dV": float(np.abs(V - Vo).max()), "dN_e": abs(N - No)}
+        out["plain_fp64"]["accuracy_vs_fp64_oracle"] = {"dE_xc": abs(plain["E"] - Eo),
+                                                        "max_dV": float(np.abs(plain["V"] - Vo).max())}
     print(json.dumps(out), flush=True)
 
 

@@ -24,6 +24,7 @@ DATA_ROOT = os.path.join(PKG_DIR, "data")
 ANGSTROM_TO_BOHR = 1.8897259886  # src/Macro/Unit.h:1
 
 CHX_PREC_FP64 = 0
+CHX_PREC_MIXED = 1  # split-BF16 MFMA contractions, FP32 accumulation
 
 _vp = ctypes.c_void_p
 _lib = None

@@ -96,11 +96,29 @@ enum { kStPermute = 0, kStDensity = 1, kStFunctional = 2, kStVxc = 3, kStFinaliz
 
 void density_from_device(chx_grid* g, const double* dD) {
   hipEvent_t e = nullptr;
+  const int ng = (int)g->grp_batch.size();
+  if (g->precision == CHX_PREC_MIXED) {
+    g->mark_begin(kStPermute, &e);
+    launch_split_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dsplit.p);
+    g->mark_end(kStPermute, e);
+    g->mark_begin(kStDensity, &e);
+    launch_density_mx(g->stream, 0, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
+                      g->d_bt_off.p, g->d_bt_list.p, g->d_cmx.p, g->mx_sstride, g->d_Dsplit.p, g->npad, g->d_part.p);
+    if (g->type >= 2)
+      for (int r = 1; r <= 3; ++r)
+        launch_density_mx(g->stream, r, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
+                          g->d_bt_off.p, g->d_bt_list.p, g->d_cmx.p, g->mx_sstride, g->d_Dsplit.p, g->npad,
+                          g->d_part_tau.p + (size_t)(r - 1) * kBatch);
+    g->mark_end(kStDensity, e);
+    g->have_density = true;
+    g->have_eval = false;
+    g->have_v = false;
+    return;
+  }
   g->mark_begin(kStPermute, &e);
   launch_permute_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dint.p);
   g->mark_end(kStPermute, e);
   g->mark_begin(kStDensity, &e);
-  const int ng = (int)g->grp_batch.size();
   if (g->cached) {
     launch_density_c64(g->stream, 0, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
                        g->d_bt_off.p, g->d_bt_list.p, g->d_cache.p, g->d_Dint.p, g->npad, g->d_part.p);
@@ -121,6 +139,22 @@ void density_from_device(chx_grid* g, const double* dD) {
 
 void fock_to_device(chx_grid* g, double* dV) {
   hipEvent_t e = nullptr;
+  if (g->precision == CHX_PREC_MIXED) {
+    const int nc = g->v_family >= 2 ? 4 : 1;
+    require(nc <= g->mx_nT, CHX_ERR_STATE, "a meta-GGA potential needs a grid of type 2 before getAO");
+    g->mark_begin(kStMakeB, &e);
+    launch_make_b_mx(g->stream, nc, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cmx.p, g->mx_sstride, g->d_bmx.p);
+    g->mark_end(kStMakeB, e);
+    launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
+    g->mark_begin(kStVxc, &e);
+    launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p, g->ntiles,
+                  g->d_cmx.p, g->mx_sstride, g->d_bmx.p, g->d_Mint.p, g->npad);
+    g->mark_end(kStVxc, e);
+    g->mark_begin(kStFinalize, &e);
+    launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);
+    g->mark_end(kStFinalize, e);
+    return;
+  }
   if (g->cached) {
     const int nc = g->v_family >= 2 ? 4 : 1;
     if (g->bmat_comps < nc) {
@@ -422,7 +456,9 @@ int chx_grid_stream(const chx_grid* g, void** stream) {
 
 int chx_grid_set_precision(chx_grid* g, int precision) {
   return guard([&] {
-    require(precision == CHX_PREC_FP64, CHX_ERR_UNSUPPORTED, "only the FP64 form is built");
+    require(precision == CHX_PREC_FP64 || precision == CHX_PREC_MIXED, CHX_ERR_ARG,
+            "precision must be CHX_PREC_FP64 or CHX_PREC_MIXED");
+    if (g->precision != precision) g->ao_ready = false;  // the AO tables are laid out per form
     g->precision = precision;
   });
 }

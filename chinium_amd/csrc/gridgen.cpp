@@ -382,10 +382,23 @@ void grid_plan(chx_grid* g) {
   for (size_t k = 0; k < g->grp_batch.size(); ++k)
     for (int q = 0; q < g->grp_ntiles[k]; ++q)
       g->grp_slots[k * kGroup + q] = g->slot_of[(size_t)g->grp_batch[k] * nt + g->grp_tiles[k * kGroup + q]];
-  const double cache_bytes = (double)nslot * (4.0 + (g->type >= 2 ? 4.0 : 1.0)) * kBatch * kTile * sizeof(double);
+  // The mixed form always runs from the split AO cache (AOg + nT transposed
+  // components per slot, 2 bytes per element) plus B^T of nT components.
+  const bool mixed = g->precision == CHX_PREC_MIXED;
+  const int nT = g->type >= 2 ? 4 : 1;
+  const double cache_bytes =
+      mixed ? (double)nslot * ((double)kMxSlotG + 2.0 * nT * kMxComp) * sizeof(unsigned short)
+            : (double)nslot * (4.0 + (g->type >= 2 ? 4.0 : 1.0)) * kBatch * kTile * sizeof(double);
+  if (mixed) {  // a previous FP64 plan's tables are not needed any more
+    g->d_cache.release();
+    g->d_bmat.release();
+  }
   size_t free_b = 0, total_b = 0;
   CHX_HIP(hipMemGetInfo(&free_b, &total_b));
-  g->cached = g->cache_mode == 1 || (g->cache_mode == -1 && cache_bytes < 0.8 * (double)free_b);
+  if (mixed && cache_bytes > 0.9 * (double)free_b)
+    throw Error(CHX_ERR_DEVICE, "the mixed-precision form needs " + std::to_string(cache_bytes / 1e9) +
+                                    " GB of device memory for its AO cache");
+  g->cached = mixed || g->cache_mode == 1 || (g->cache_mode == -1 && cache_bytes < 0.8 * (double)free_b);
   if (g->cached) {
     const int nsc = g->npad / (kCacheSuper * kTile);
     std::vector<std::vector<int>> cp((size_t)nsc * nsc);
@@ -420,16 +433,37 @@ void grid_plan(chx_grid* g) {
     g->d_slot_of.upload(g->slot_of, st); g->d_tile_nao.upload(g->tile_nao, st);
     g->d_grp_slots.upload(g->grp_slots, st);
     g->d_cwork.upload(g->cwork, st); g->d_cpair_batches.upload(g->cpair_batches, st);
-    g->d_cache.alloc(std::max<size_t>(1, nslot * (size_t)4 * kBatch * kTile));
-    g->bmat_comps = g->type >= 2 ? 4 : 1;
-    g->d_bmat.alloc(std::max<size_t>(1, nslot * (size_t)g->bmat_comps * kBatch * kTile));
     g->d_Mint.alloc((size_t)g->npad * g->npad);
-    launch_ao_cache(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p, g->d_z.p,
-                    g->view(), g->d_cache.p);
+    if (mixed) {
+      g->d_cache.release();
+      g->d_bmat.release();
+      g->bmat_comps = 0;
+      g->mx_nT = nT;
+      g->mx_sstride = kMxSlotG + nT * kMxComp;
+      g->d_cmx.alloc(std::max<size_t>(1, nslot * (size_t)g->mx_sstride));
+      g->bmx_comps = nT;
+      g->d_bmx.alloc(std::max<size_t>(1, nslot * (size_t)nT * kMxComp));
+      g->d_Dsplit.alloc(2 * (size_t)g->npad * g->npad);
+      launch_ao_cache_mx(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p,
+                         g->d_z.p, g->view(), nT, g->mx_sstride, g->d_cmx.p);
+    } else {
+      g->d_cmx.release();
+      g->d_bmx.release();
+      g->d_Dsplit.release();
+      g->mx_nT = g->mx_sstride = g->bmx_comps = 0;
+      g->d_cache.alloc(std::max<size_t>(1, nslot * (size_t)4 * kBatch * kTile));
+      g->bmat_comps = g->type >= 2 ? 4 : 1;
+      g->d_bmat.alloc(std::max<size_t>(1, nslot * (size_t)g->bmat_comps * kBatch * kTile));
+      launch_ao_cache(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p,
+                      g->d_z.p, g->view(), g->d_cache.p);
+    }
   } else {
     g->d_cache.release();
     g->d_bmat.release();
     g->d_Mint.release();
+    g->d_cmx.release();
+    g->d_bmx.release();
+    g->d_Dsplit.release();
   }
   g->d_bt_off.upload(g->bt_off, st); g->d_bt_list.upload(g->bt_list, st);
   g->d_grp_batch.upload(g->grp_batch, st); g->d_grp_tiles.upload(g->grp_tiles, st);

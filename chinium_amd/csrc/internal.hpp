@@ -114,6 +114,22 @@ void launch_make_b(hipStream_t s, int ncomp, int nbatches, const int* bt_off, co
 void launch_vxc_c64(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
                     int ntiles, const double* cache, const double* bmat, double* Mint, int npad);
 void launch_finalize_sym(hipStream_t s, int n, int npad, const double* Mint, const int* ref2int, double* V);
+// Mixed-precision form (kernels_mixed.hip): AO tables cached as bf16 hi/lo
+// pairs, split-BF16 MFMA contractions with FP32 accumulation.
+constexpr int kMxSlotG = 4 * kBatch * 2 * kTile;  // AOg elements per slot
+constexpr int kMxComp = 2 * kTile * kBatch;       // AOt / Bt elements per component
+void launch_ao_cache_mx(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* tile_nao,
+                        const double* bx, const double* by, const double* bz, BasisView bv, int nT, int sstride,
+                        unsigned short* cache);
+void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds);
+void launch_density_mx(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
+                       const int* grp_slots, const int* grp_ntiles, const int* bt_off, const int* bt_list,
+                       const unsigned short* cache, int sstride, const unsigned short* Ds, int npad, double* part);
+void launch_make_b_mx(hipStream_t s, int ncomp, int nbatches, const int* bt_off, const double* coef,
+                      const unsigned short* cache, int sstride, unsigned short* bt);
+void launch_vxc_mx(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
+                   int ntiles, const unsigned short* cache, int sstride, const unsigned short* bt, double* Mint,
+                   int npad);
 
 }  // namespace chx
 
@@ -179,6 +195,9 @@ struct chx_grid {
 
   // Device mirrors.
   chx::DevBuf<double> d_sh_xyz, d_p_exp, d_p_coef;
+  // Mixed form: split AO cache (AOg + nT transposed components per slot), B^T, split D.
+  chx::DevBuf<unsigned short> d_cmx, d_bmx, d_Dsplit;
+  int mx_sstride = 0, mx_nT = 0, bmx_comps = 0;
   chx::DevBuf<int> d_sh_type, d_sh_nprim, d_sh_poff, d_tile_seg_off;
   chx::DevBuf<int4> d_seg;
   chx::DevBuf<int> d_int2ref, d_ref2int;

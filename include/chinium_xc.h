@@ -39,6 +39,11 @@ typedef struct chx_xc chx_xc;     /* functional (class ExchangeCorrelation) */
 
 /* Precision of the two dense contractions (Phi.D and Phi^T.B). */
 #define CHX_PREC_FP64 0          /* plain form: everything FP64 */
+/* Chosen form: Phi.D and Phi^T.B on MFMA as split-BF16 products (hi*hi +
+ * hi*lo + lo*hi, 16 significant bits per operand) with FP32 accumulation, AO
+ * tables cached in that split form; Becke weights, the functional and the
+ * E_xc / N_e / V sums stay FP64. */
+#define CHX_PREC_MIXED 1
 
 const char* chx_last_error(void);
 int chx_device_count(int* n);
