@@ -38,6 +38,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="h2o16_def2svp_sg2_b3lyp")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a plain streaming kernel (torch.neg on float32) that reads and writes this many "
+                         "MiB, three times: the known-size reference for the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--ao-cache", type=int, default=-1, help="-1 auto, 0 recompute AO values, 1 cache them")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -185,6 +188,14 @@ def main():
                                           "max_dV": float(np.abs(V - Vo).max()), "dN_e": abs(N - No)}
         out["plain_fp64"]["accuracy_vs_fp64_oracle"] = {"dE_xc": abs(plain["E"] - Eo),
                                                         "max_dV": float(np.abs(plain["V"] - Vo).max())}
+    if args.copy_probe_mib:
+        import torch
+        a = torch.ones(args.copy_probe_mib * (1 << 18), dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        out["copy_probe"] = {"kernel": "torch.neg float32", "read_bytes": a.numel() * 4, "write_bytes": b.numel() * 4}
     print(json.dumps(out), flush=True)
 
 

@@ -102,13 +102,11 @@ void density_from_device(chx_grid* g, const double* dD) {
     launch_split_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dsplit.p);
     g->mark_end(kStPermute, e);
     g->mark_begin(kStDensity, &e);
-    launch_density_mx(g->stream, 0, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
-                      g->d_bt_off.p, g->d_bt_list.p, g->d_cmx.p, g->mx_sstride, g->d_Dsplit.p, g->npad, g->d_part.p);
-    if (g->type >= 2)
-      for (int r = 1; r <= 3; ++r)
-        launch_density_mx(g->stream, r, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
-                          g->d_bt_off.p, g->d_bt_list.p, g->d_cmx.p, g->mx_sstride, g->d_Dsplit.p, g->npad,
-                          g->d_part_tau.p + (size_t)(r - 1) * kBatch);
+    for (int r = 0; r <= (g->type >= 2 ? 3 : 0); ++r)  // r > 0: tau partials, A = d_r phi
+      launch_density_mx(g->stream, r, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
+                        g->d_bt_list.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_cmx.p,
+                        g->mx_sstride, g->d_Dsplit.p, g->npad,
+                        r == 0 ? g->d_part.p : g->d_part_tau.p + (size_t)(r - 1) * kBatch);
     g->mark_end(kStDensity, e);
     g->have_density = true;
     g->have_eval = false;
@@ -143,12 +141,13 @@ void fock_to_device(chx_grid* g, double* dV) {
     const int nc = g->v_family >= 2 ? 4 : 1;
     require(nc <= g->mx_nT, CHX_ERR_STATE, "a meta-GGA potential needs a grid of type 2 before getAO");
     g->mark_begin(kStMakeB, &e);
-    launch_make_b_mx(g->stream, nc, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cmx.p, g->mx_sstride, g->d_bmx.p);
+    launch_make_b_mx(g->stream, nc, (int)g->bt_list.size(), g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p,
+                     g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_coef.p, g->d_bmx.p);
     g->mark_end(kStMakeB, e);
     launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
     g->mark_begin(kStVxc, &e);
     launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p, g->ntiles,
-                  g->d_cmx.p, g->mx_sstride, g->d_bmx.p, g->d_Mint.p, g->npad);
+                  g->d_cmx.p, g->mx_sstride, g->mx_nT * kMxComp, g->d_bmx.p, g->d_Mint.p, g->npad);
     g->mark_end(kStVxc, e);
     g->mark_begin(kStFinalize, &e);
     launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);
@@ -300,6 +299,7 @@ int chx_grid_copy(const chx_grid* s, chx_grid** out) {
     hipStream_t st = g->stream;
     CHX_HIP(hipStreamSynchronize(s->stream));
     g->d_sh_xyz.copy_from(s->d_sh_xyz, st); g->d_p_exp.copy_from(s->d_p_exp, st); g->d_p_coef.copy_from(s->d_p_coef, st);
+    g->d_segrec.copy_from(s->d_segrec, st); g->max_nprim = s->max_nprim;
     g->d_sh_type.copy_from(s->d_sh_type, st); g->d_sh_nprim.copy_from(s->d_sh_nprim, st);
     g->d_sh_poff.copy_from(s->d_sh_poff, st); g->d_tile_seg_off.copy_from(s->d_tile_seg_off, st);
     g->d_seg.copy_from(s->d_seg, st); g->d_int2ref.copy_from(s->d_int2ref, st); g->d_ref2int.copy_from(s->d_ref2int, st);
@@ -628,9 +628,16 @@ int chx_selftest(double* max_err) {
     hipStream_t s;
     CHX_HIP(hipStreamCreate(&s));
     launch_selftest(s, d.p);
+    DevBuf<unsigned> bad;
+    bad.alloc(1);
+    launch_fill_u32(s, bad.p, 1, 0u);
+    launch_bf16_check(s, 1u << 20, bad.p);
+    unsigned nbad = 0;
     CHX_HIP(hipMemcpyAsync(max_err, d.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipMemcpyAsync(&nbad, bad.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     CHX_HIP(hipStreamSynchronize(s));
     CHX_HIP(hipStreamDestroy(s));
+    *max_err += (double)nbad;
   });
 }
 

@@ -88,4 +88,13 @@ __device__ __forceinline__ int shell_eval(const BasisView& bv, int sh, double px
   return n;
 }
 
+// Block index -> work index such that each XCD (blocks are dealt to the 8
+// XCDs round-robin) runs one contiguous range of the work list; neighbouring
+// work items, which share operand tiles, then meet in the same L2.
+__device__ __forceinline__ int xcd_remap(int i, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd, r = n % kXcd, x = i % kXcd;
+  return x * q + (x < r ? x : r) + i / kXcd;
+}
+
 }  // namespace chx

@@ -10,6 +10,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <fstream>
 #include <numeric>
 #include <sstream>
@@ -293,6 +294,23 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
   hipStream_t st = g->stream;
   g->d_sh_xyz.upload(g->sh_xyz, st); g->d_sh_type.upload(g->sh_type, st); g->d_sh_nprim.upload(g->sh_nprim, st);
   g->d_sh_poff.upload(g->sh_poff, st); g->d_p_exp.upload(g->p_exp, st); g->d_p_coef.upload(g->p_coef, st);
+  std::vector<SegRec> rec(g->seg.size());  // lives until the synchronisation below
+  g->max_nprim = 0;
+  for (size_t k = 0; k < g->seg.size(); ++k) {
+    const int4 sg = g->seg[k];
+    const int sh = sg.x, np = g->sh_nprim[sh], po = g->sh_poff[sh];
+    g->max_nprim = std::max(g->max_nprim, np);
+    SegRec& r = rec[k];
+    std::memset(&r, 0, sizeof(r));
+    r.ax = g->sh_xyz[3 * sh]; r.ay = g->sh_xyz[3 * sh + 1]; r.az = g->sh_xyz[3 * sh + 2];
+    r.meta = (g->sh_type[sh] & 0xff) | (std::min(np, kSegPrim) << 8) | (sg.y << 16) | (sg.z << 24);
+    r.col = sg.w;
+    for (int p = 0; p < np && p < kSegPrim; ++p) {
+      r.e[p] = (float)g->p_exp[po + p];
+      r.c[p] = (float)g->p_coef[po + p];
+    }
+  }
+  g->d_segrec.upload(rec, st);
   g->d_tile_seg_off.upload(g->tile_seg_off, st); g->d_seg.upload(g->seg, st);
   g->d_int2ref.upload(g->int2ref, st); g->d_ref2int.upload(g->ref2int, st);
   g->d_x.upload(g->hx, st); g->d_y.upload(g->hy, st); g->d_z.upload(g->hz, st); g->d_w.upload(g->hw, st);
@@ -382,13 +400,16 @@ void grid_plan(chx_grid* g) {
   for (size_t k = 0; k < g->grp_batch.size(); ++k)
     for (int q = 0; q < g->grp_ntiles[k]; ++q)
       g->grp_slots[k * kGroup + q] = g->slot_of[(size_t)g->grp_batch[k] * nt + g->grp_tiles[k * kGroup + q]];
-  // The mixed form always runs from the split AO cache (AOg + nT transposed
-  // components per slot, 2 bytes per element) plus B^T of nT components.
+  // The mixed form always runs from the split operand cache (nT components
+  // in two layouts per slot, 2 bytes per element) plus B^T of nT components.
   const bool mixed = g->precision == CHX_PREC_MIXED;
   const int nT = g->type >= 2 ? 4 : 1;
   const double cache_bytes =
-      mixed ? (double)nslot * ((double)kMxSlotG + 2.0 * nT * kMxComp) * sizeof(unsigned short)
+      mixed ? (double)nslot * 3.0 * nT * kMxComp * sizeof(unsigned short)
             : (double)nslot * (4.0 + (g->type >= 2 ? 4.0 : 1.0)) * kBatch * kTile * sizeof(double);
+  if (mixed && g->max_nprim > kSegPrim)
+    throw Error(CHX_ERR_UNSUPPORTED, "the mixed-precision form takes at most " + std::to_string(kSegPrim) +
+                                         " primitives per shell");
   if (mixed) {  // a previous FP64 plan's tables are not needed any more
     g->d_cache.release();
     g->d_bmat.release();
@@ -419,13 +440,16 @@ void grid_plan(chx_grid* g) {
     g->cpair_batches.clear();
     size_t maxlen = 0;
     for (const auto& L : cp) maxlen = std::max(maxlen, L.size());
-    for (size_t k = 0; k < maxlen; k += kCacheChunk)
+    // Fewer, longer items mean fewer FP64 atomics into M; the mixed form's
+    // items are cheap per batch, so it takes longer ones.
+    const size_t chunk = g->vxc_chunk > 0 ? (size_t)g->vxc_chunk : (mixed ? 32 : kCacheChunk);
+    for (size_t k = 0; k < maxlen; k += chunk)
       for (int I = 0; I < nsc; ++I)
         for (int J = 0; J < nsc; ++J) {
           const auto& L = cp[(size_t)I * nsc + J];
           if (k >= L.size()) continue;
           const int begin = (int)g->cpair_batches.size();
-          const size_t e = std::min(L.size(), k + kCacheChunk);
+          const size_t e = std::min(L.size(), k + chunk);
           g->cpair_batches.insert(g->cpair_batches.end(), L.begin() + k, L.begin() + e);
           g->cwork.push_back(make_int4(I, J, begin, (int)g->cpair_batches.size()));
         }
@@ -439,7 +463,7 @@ void grid_plan(chx_grid* g) {
       g->d_bmat.release();
       g->bmat_comps = 0;
       g->mx_nT = nT;
-      g->mx_sstride = kMxSlotG + nT * kMxComp;
+      g->mx_sstride = 2 * nT * kMxComp;  // AOg then AOt, nT components each
       g->d_cmx.alloc(std::max<size_t>(1, nslot * (size_t)g->mx_sstride));
       g->bmx_comps = nT;
       g->d_bmx.alloc(std::max<size_t>(1, nslot * (size_t)nT * kMxComp));
@@ -506,5 +530,6 @@ chx::BasisView chx_grid::view() const {
   v.p_coef = d_p_coef.p;
   v.tile_seg_off = d_tile_seg_off.p;
   v.seg = d_seg.p;
+  v.segrec = d_segrec.p;
   return v;
 }

@@ -19,7 +19,7 @@ constexpr int kSuper = 2;      // AO tiles per Vxc super-tile (64 x 64 output bl
 constexpr int kVxcChunk = 32;  // batches per Vxc work item
 constexpr int kPadAO = 128;    // nbasis is padded to a multiple of this (4 tiles)
 constexpr int kCacheSuper = 4; // tiles per Vxc super-tile in the cached path
-constexpr int kCacheChunk = 16;  // batches per cached-path Vxc work item
+constexpr int kCacheChunk = 16;  // batches per cached-path Vxc work item (FP64 form default)
 constexpr int kCoef = 5;       // Vxc integrand coefficients per point: a, c_x, c_y, c_z, t
 
 struct Error : std::runtime_error {
@@ -70,6 +70,17 @@ template <class T> struct DevBuf {
   }
 };
 
+// One shell segment of a tile with everything its evaluation reads, so that a
+// wave fetches it with two scalar loads (mixed-precision form).
+constexpr int kSegPrim = 12;  // primitives per record
+struct SegRec {
+  double ax, ay, az;  // shell centre
+  int meta;           // type & 0xff | nprim << 8 | first component << 16 | count << 24
+  int col;            // AO offset in the tile
+  float e[kSegPrim], c[kSegPrim];
+};
+static_assert(sizeof(SegRec) == 128, "segment record is two 64-byte scalar loads");
+
 // Kernel-side view of the basis in the internal AO order.
 struct BasisView {
   const double* sh_xyz;   // 3 per shell
@@ -80,6 +91,7 @@ struct BasisView {
   const double* p_coef;
   const int* tile_seg_off;  // ntiles + 1
   const int4* seg;          // (shell, first component, count, local AO offset in tile)
+  const SegRec* segrec;     // per segment, same index as seg (mixed-precision form)
 };
 
 // Launchers (kernels.hip).
@@ -114,22 +126,26 @@ void launch_make_b(hipStream_t s, int ncomp, int nbatches, const int* bt_off, co
 void launch_vxc_c64(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
                     int ntiles, const double* cache, const double* bmat, double* Mint, int npad);
 void launch_finalize_sym(hipStream_t s, int n, int npad, const double* Mint, const int* ref2int, double* V);
-// Mixed-precision form (kernels_mixed.hip): AO tables cached as bf16 hi/lo
-// pairs, split-BF16 MFMA contractions with FP32 accumulation.
-constexpr int kMxSlotG = 4 * kBatch * 2 * kTile;  // AOg elements per slot
-constexpr int kMxComp = 2 * kTile * kBatch;       // AOt / Bt elements per component
+// Mixed-precision form (kernels_mixed.hip): MFMA operands cached as bf16
+// hi/lo pairs, split-BF16 MFMA contractions with FP32 accumulation, FP32
+// recomputation of the AO values outside the MFMA operands.
+constexpr int kMxComp = 2 * kTile * kBatch;  // elements per component and layout of a slot
 void launch_ao_cache_mx(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* tile_nao,
                         const double* bx, const double* by, const double* bz, BasisView bv, int nT, int sstride,
                         unsigned short* cache);
 void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds);
+void launch_bf16_check(hipStream_t s, unsigned n, unsigned* bad);
+void launch_fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v);
 void launch_density_mx(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
-                       const int* grp_slots, const int* grp_ntiles, const int* bt_off, const int* bt_list,
+                       const int* grp_ntiles, const int* bt_off, const int* bt_list, const int* tile_nao,
+                       const double* bx, const double* by, const double* bz, BasisView bv,
                        const unsigned short* cache, int sstride, const unsigned short* Ds, int npad, double* part);
-void launch_make_b_mx(hipStream_t s, int ncomp, int nbatches, const int* bt_off, const double* coef,
-                      const unsigned short* cache, int sstride, unsigned short* bt);
+void launch_make_b_mx(hipStream_t s, int ncomp, int nslots, const int* slot_batch, const int* slot_tile,
+                      const int* tile_nao, const double* bx, const double* by, const double* bz, BasisView bv,
+                      const double* coef, unsigned short* bt);
 void launch_vxc_mx(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
-                   int ntiles, const unsigned short* cache, int sstride, const unsigned short* bt, double* Mint,
-                   int npad);
+                   int ntiles, const unsigned short* cache, int sstride, int toff, const unsigned short* bt,
+                   double* Mint, int npad);
 
 }  // namespace chx
 
@@ -186,6 +202,7 @@ struct chx_grid {
   bool cached = false;
   int cache_mode = -1;               // -1 automatic, 0 never, 1 always
   std::vector<int> slot_batch, slot_tile, slot_of, tile_nao, grp_slots;
+  int vxc_chunk = 0;                 // batches per Vxc work item; 0 = the form's default
   std::vector<int4> cwork;
   std::vector<int> cpair_batches;
   chx::DevBuf<double> d_cache, d_bmat, d_Mint;
@@ -195,6 +212,8 @@ struct chx_grid {
 
   // Device mirrors.
   chx::DevBuf<double> d_sh_xyz, d_p_exp, d_p_coef;
+  chx::DevBuf<chx::SegRec> d_segrec;
+  int max_nprim = 0;
   // Mixed form: split AO cache (AOg + nT transposed components per slot), B^T, split D.
   chx::DevBuf<unsigned short> d_cmx, d_bmx, d_Dsplit;
   int mx_sstride = 0, mx_nT = 0, bmx_comps = 0;

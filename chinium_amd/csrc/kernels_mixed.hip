@@ -6,13 +6,14 @@
 // FP32 accumulation (the dropped lo*lo' term is below 2^-16 |x y|). The
 // functional, the E_xc / N_e sums and the final V accumulation stay FP64.
 //
-// The AO tables are cached at Grid::getAO in this split form (as the FP64
-// cached path keeps them in FP64, src/Grid/Grid.cpp:22-33 caches them in the
-// reference), 32 KB + 8 KB per component of the transposed copy per slot:
-//   AOg [c=0..3][g=0..63][hl][mu=0..31]  (c = phi, d/dx, d/dy, d/dz; K = mu contiguous)
-//   AOt [c=0..nT-1][hl][mu=0..31][g=0..63]  (K = g contiguous; nT = 1, or 4 for meta-GGA)
-// since the MFMA wants the contraction index contiguous in both operands: mu
-// for the density X = Phi D, g for Vxc M = Phi^T B.
+// The MFMA operands are cached at Grid::getAO in this split form (as the
+// reference caches its AO tables once, src/Grid/Grid.cpp:22-33), nT = 1
+// component (phi), or 4 (phi, d/dx, d/dy, d/dz) for meta-GGA, in two layouts:
+//   AOg [c][g=0..63][hl][mu=0..31]  (K = mu contiguous: A of the density X = Phi D)
+//   AOt [c][hl][mu=0..31][g=0..63]  (K = g contiguous: A of Vxc M = Phi^T B)
+// 8 KB per component and layout per slot. Values needed outside the MFMA
+// operands (the density epilogue, the Vxc integrand B) are recomputed in FP32
+// from the basis: the hardware exp makes that cheaper than reading them back.
 // The MFMA operand layouts used below: lane l, element e of the 8-vector holds
 //   A[i = l % 16][k = 8 (l / 16) + e],  B[k = 8 (l / 16) + e][j = l % 16],
 // and result register r of lane l is C[4 (l / 16) + r][l % 16].
@@ -28,12 +29,19 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
 
 constexpr int kMxRow = 2 * kTile;             // one AOg row: 32 hi then 32 lo
-constexpr int kMxG = 4 * kBatch * kMxRow;     // AOg elements per slot
-constexpr int kMxT = 2 * kTile * kBatch;      // AOt / Bt elements per component
+constexpr int kMxT = 2 * kTile * kBatch;      // elements per component of AOg, AOt and Bt
 constexpr int kMxSup = 4;                     // tiles per Vxc super-tile (128 AOs)
-constexpr int kLdsRow = 5;                    // uint4 per LDS row: 4 data + 1 pad (no bank conflicts)
+// LDS operand tiles: rows of 4 16-byte chunks (32 bf16), chunk c of row r
+// stored at position c ^ ((r >> 2) & 3). A wave's 16-byte accesses then hit
+// distinct banks both when 16 lanes read one chunk of 16 consecutive rows
+// (MFMA operands) and when they write all 4 chunks of 4 rows (staging).
+__device__ __forceinline__ int swz(int r, int c) { return c ^ ((r >> 2) & 3); }
 
-__device__ __forceinline__ u16 f2bf(float f) {  // round to nearest even
+// float -> bf16, round to nearest even: the hardware conversion
+// (v_cvt_pk_bf16_f32). k_bf16_check compares it with the bit-level rule
+// f2bf_rne on the device.
+__device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
+__device__ __forceinline__ u16 f2bf_rne(float f) {
   unsigned u = __float_as_uint(f);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (u16)(u >> 16);
@@ -42,6 +50,35 @@ __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)
 __device__ __forceinline__ void split_bf(float x, u16& hi, u16& lo) {
   hi = f2bf(x);
   lo = f2bf(x - bf2f(hi));
+}
+
+// Device check of the conversion: n pseudo-random finite floats spread over
+// many binades; counts values where the hardware result differs from f2bf_rne.
+__global__ void k_bf16_check(unsigned n, unsigned* __restrict__ bad) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned h = i * 2654435761u;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  const unsigned e = 64u + (h >> 25);  // biased exponent 64..191
+  const float f = __uint_as_float((h & 0x807fffffu) | (e << 23));
+  if (f2bf(f) != f2bf_rne(f)) atomicAdd(bad, 1u);
+}
+
+__global__ void k_fill_u32(unsigned* p, size_t n, unsigned v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v) {
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  CHX_HIP(hipGetLastError());
+}
+
+void launch_bf16_check(hipStream_t s, unsigned n, unsigned* bad) {
+  hipLaunchKernelGGL(k_bf16_check, dim3((n + 255) / 256), dim3(256), 0, s, n, bad);
+  CHX_HIP(hipGetLastError());
 }
 __device__ __forceinline__ bf16x8 as_bf(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 __device__ __forceinline__ f32x4 mfma3(uint4 ah, uint4 al, uint4 bh, uint4 bl, f32x4 c) {
@@ -89,19 +126,20 @@ __global__ __launch_bounds__(256) void k_ao_cache_mx(const int* __restrict__ slo
   }
   __syncthreads();
   u16* out = cache + (size_t)s * sstride;
-  for (int e = tid; e < 4 * kBatch * kTile; e += 256) {
+  for (int e = tid; e < nT * kBatch * kTile; e += 256) {
     const int c = e / (kBatch * kTile), g = (e / kTile) % kBatch, mu = e % kTile;
     u16 hi, lo;
     split_bf(E[c][g][mu], hi, lo);
     out[(c * kBatch + g) * kMxRow + mu] = hi;
     out[(c * kBatch + g) * kMxRow + kTile + mu] = lo;
   }
+  u16* outt = out + nT * kMxT;
   for (int e = tid; e < nT * kTile * kBatch; e += 256) {
     const int c = e / (kTile * kBatch), mu = (e / kBatch) % kTile, g = e % kBatch;
     u16 hi, lo;
     split_bf(E[c][g][mu], hi, lo);
-    out[kMxG + c * kMxT + mu * kBatch + g] = hi;
-    out[kMxG + c * kMxT + kTile * kBatch + mu * kBatch + g] = lo;
+    outt[c * kMxT + mu * kBatch + g] = hi;
+    outt[c * kMxT + kTile * kBatch + mu * kBatch + g] = lo;
   }
 }
 
@@ -137,30 +175,175 @@ void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* 
 }
 
 // ---------------------------------------------------------------------------
+// Angular parts of one shell in FP32 (same conventions as shell_eval:
+// src/Grid/AO/{S,CartP,PureP,PureD,PureF}.hpp). phi_m = R Q_m and
+// grad phi_m = grad R Q_m + R grad Q_m with R = sum_p c_p exp(-a_p r^2), so an
+// operation sees the radial part once per shell (op.begin / op.end) and, per
+// component, op.comp(column, Q, dQ/dx, dQ/dy, dQ/dz). The shell type is a
+// template argument: each type's polynomials are straight-line code, the
+// constant derivatives fold away, and components first..first+cnt-1 are
+// emitted with compile-time indices.
+template <int TYPE, class Op>
+__device__ __forceinline__ void emit_shell_f32(float x, float y, float z, int first, int cnt, int col, Op& op) {
+  constexpr int n = TYPE == 0 ? 1 : TYPE == -2 ? 5 : TYPE == -3 ? 7 : 3;
+  float q[n], qx[n], qy[n], qz[n];
+  const float r3 = 1.7320508075688772f, r6 = 2.449489742783178f, r10 = 3.1622776601683795f, r15 = 3.872983346207417f;
+  if (TYPE == 0) {
+    q[0] = 1.f; qx[0] = qy[0] = qz[0] = 0.f;
+  } else if (TYPE == 1) {  // Cartesian p: x, y, z
+    q[0] = x; qx[0] = 1.f; qy[0] = 0.f; qz[0] = 0.f;
+    q[1] = y; qx[1] = 0.f; qy[1] = 1.f; qz[1] = 0.f;
+    q[2] = z; qx[2] = 0.f; qy[2] = 0.f; qz[2] = 1.f;
+  } else if (TYPE == -1) {  // pure p: y, z, x
+    q[0] = y; qx[0] = 0.f; qy[0] = 1.f; qz[0] = 0.f;
+    q[1] = z; qx[1] = 0.f; qy[1] = 0.f; qz[1] = 1.f;
+    q[2] = x; qx[2] = 1.f; qy[2] = 0.f; qz[2] = 0.f;
+  } else if (TYPE == -2) {
+    q[0] = r3 * x * y;                     qx[0] = r3 * y; qy[0] = r3 * x;  qz[0] = 0.f;
+    q[1] = r3 * y * z;                     qx[1] = 0.f;    qy[1] = r3 * z;  qz[1] = r3 * y;
+    q[2] = z * z - 0.5f * (x * x + y * y); qx[2] = -x;     qy[2] = -y;      qz[2] = 2.f * z;
+    q[3] = r3 * x * z;                     qx[3] = r3 * z; qy[3] = 0.f;     qz[3] = r3 * x;
+    q[4] = 0.5f * r3 * (x * x - y * y);    qx[4] = r3 * x; qy[4] = -r3 * y; qz[4] = 0.f;
+  } else {
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float c10 = 0.25f * r10, c6 = 0.25f * r6;
+    q[0] = c10 * y * (3.f * xx - yy);     qx[0] = c10 * 6.f * x * y;  qy[0] = c10 * 3.f * (xx - yy);             qz[0] = 0.f;
+    q[1] = r15 * x * y * z;               qx[1] = r15 * y * z;        qy[1] = r15 * x * z;                       qz[1] = r15 * x * y;
+    q[2] = c6 * y * (4.f * zz - xx - yy); qx[2] = -c6 * 2.f * x * y;  qy[2] = c6 * (4.f * zz - xx - 3.f * yy);  qz[2] = c6 * 8.f * y * z;
+    q[3] = z * (zz - 1.5f * (xx + yy));   qx[3] = -3.f * x * z;       qy[3] = -3.f * y * z;                      qz[3] = 3.f * zz - 1.5f * (xx + yy);
+    q[4] = c6 * x * (4.f * zz - xx - yy); qx[4] = c6 * (4.f * zz - 3.f * xx - yy); qy[4] = -c6 * 2.f * x * y;   qz[4] = c6 * 8.f * x * z;
+    q[5] = 0.5f * r15 * (xx - yy) * z;    qx[5] = r15 * x * z;        qy[5] = -r15 * y * z;                      qz[5] = 0.5f * r15 * (xx - yy);
+    q[6] = c10 * x * (xx - 3.f * yy);     qx[6] = c10 * 3.f * (xx - yy); qy[6] = -c10 * 6.f * x * y;             qz[6] = 0.f;
+  }
+  if (first == 0 && cnt == n) {  // the usual case: the whole shell lies in this tile
+#pragma unroll
+    for (int m = 0; m < n; ++m) op.comp(col + m, q[m], qx[m], qy[m], qz[m]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < n; ++m) {
+      const int k = m - first;  // component m of the shell is AO col + k of the tile
+      if (k >= 0 && k < cnt) op.comp(col + k, q[m], qx[m], qy[m], qz[m]);
+    }
+  }
+}
+
+// R = sum_p c_p exp(-a_p r^2) and sum_p a_p c_p exp(-a_p r^2) over NP primitives.
+template <int NP>
+__device__ __forceinline__ void radial_f32(const SegRec& rec, float r2, float& A0, float& A1) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const float e = rec.c[p] * __expf(-rec.e[p] * r2);
+    A0 += e;
+    A1 += rec.e[p] * e;
+  }
+}
+
+// One wave walks one tile at 64 points, lane = point. Every segment record
+// comes in through scalar loads and the shell loop is uniform across the
+// wave: no divergence, no LDS, no block barrier. The offset from the centre
+// is taken in FP64 (points and centres are tens of bohr apart, the offsets
+// that matter are small), everything after it in FP32 with the hardware exp.
+template <class Op>
+__device__ __forceinline__ void walk_tile_f32(const BasisView& bv, int t, double px, double py, double pz, Op& op) {
+  const int s0 = bv.tile_seg_off[t], s1 = bv.tile_seg_off[t + 1];
+  for (int k = s0; k < s1; ++k) {
+    const SegRec rec = bv.segrec[k];
+    const float x = (float)(px - rec.ax), y = (float)(py - rec.ay), z = (float)(pz - rec.az);
+    const float r2 = x * x + y * y + z * z;
+    float A0 = 0.f, A1 = 0.f;
+    switch ((rec.meta >> 8) & 0xff) {  // primitive count: straight-line code per count
+      case 1: radial_f32<1>(rec, r2, A0, A1); break;
+      case 2: radial_f32<2>(rec, r2, A0, A1); break;
+      case 3: radial_f32<3>(rec, r2, A0, A1); break;
+      case 4: radial_f32<4>(rec, r2, A0, A1); break;
+      case 5: radial_f32<5>(rec, r2, A0, A1); break;
+      case 6: radial_f32<6>(rec, r2, A0, A1); break;
+      case 7: radial_f32<7>(rec, r2, A0, A1); break;
+      case 8: radial_f32<8>(rec, r2, A0, A1); break;
+      default: radial_f32<kSegPrim>(rec, r2, A0, A1); break;  // unused slots hold c = 0
+    }
+    // R = A0, grad R = -2 r A1
+    op.begin(A0, -2.f * x * A1, -2.f * y * A1, -2.f * z * A1);
+    const int first = (rec.meta >> 16) & 0xff, cnt = rec.meta >> 24;
+    switch ((int)(signed char)(rec.meta & 0xff)) {
+      case 0: emit_shell_f32<0>(x, y, z, first, cnt, rec.col, op); break;
+      case 1: emit_shell_f32<1>(x, y, z, first, cnt, rec.col, op); break;
+      case -1: emit_shell_f32<-1>(x, y, z, first, cnt, rec.col, op); break;
+      case -2: emit_shell_f32<-2>(x, y, z, first, cnt, rec.col, op); break;
+      default: emit_shell_f32<-3>(x, y, z, first, cnt, rec.col, op); break;
+    }
+    op.end();
+  }
+}
+
+// Density epilogue on one tile: sum_nu X[g][nu] phi_nu and sum_nu X[g][nu] grad phi_nu
+// (MODE 0), or sum_nu X_r[g][nu] d_r phi_nu (MODE r), as
+// R sum X Q + grad R sum X Q and R sum X grad Q per shell.
+template <int MODE>
+struct DensEpi {
+  const float* xq;  // X^T of this tile at this lane's point: xq[col * (kBatch + 1)]
+  float o[4] = {0.f, 0.f, 0.f, 0.f};
+  float R, Rx, Ry, Rz, S0, Sx, Sy, Sz;
+  __device__ void begin(float a, float ax, float ay, float az) {
+    R = a; Rx = ax; Ry = ay; Rz = az;
+    S0 = Sx = Sy = Sz = 0.f;
+  }
+  __device__ void comp(int col, float q, float qx, float qy, float qz) {
+    const float x = xq[col * (kBatch + 1)];
+    S0 += x * q;
+    if (MODE == 0 || MODE == 1) Sx += x * qx;
+    if (MODE == 0 || MODE == 2) Sy += x * qy;
+    if (MODE == 0 || MODE == 3) Sz += x * qz;
+  }
+  __device__ void end() {
+    if (MODE == 0) {
+      o[0] += R * S0;
+      o[1] += Rx * S0 + R * Sx;
+      o[2] += Ry * S0 + R * Sy;
+      o[3] += Rz * S0 + R * Sz;
+    } else if (MODE == 1) {
+      o[0] += Rx * S0 + R * Sx;
+    } else if (MODE == 2) {
+      o[0] += Ry * S0 + R * Sy;
+    } else {
+      o[0] += Rz * S0 + R * Sz;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Density. Block = (batch, group of <= 4 alive tiles as output columns), 4
 // waves in a 2 x 2 grid: wave (wr, wc) owns rows g = 32 wr..+31 and columns
 // 64 wc..+63 of X = Phi_b D_b (64 x 128). K loop over the batch's alive tiles
 // (32 AOs = one MFMA K step): the A operand (phi rows of the wave) comes
 // straight from the cache into registers; the D block (128 nu x 32 mu, hi and
 // lo) is staged in LDS, double-buffered, the next step prefetched into
-// registers while the MFMAs of the current one run.
+// registers while the MFMAs of the current one run. Epilogue: X goes to LDS
+// and wave q contracts tile q's columns with phi / grad phi recomputed in
+// FP32, one lane per point.
 struct DensMxSmem {
-  uint4 B[2][2][kGroup * kTile][kLdsRow];  // [buf][hl][nu][mu chunk]
-  float red[2][4][kBatch];
+  union {
+    uint4 B[2][2][kGroup * kTile][4];  // [buf][hl][nu][mu chunk, swizzled]
+    float X[kGroup * kTile][kBatch + 1];     // epilogue: X^T [nu][g]
+  };
+  float red[kGroup][4][kBatch];
+  double px[kBatch], py[kBatch], pz[kBatch];
 };
 
 // MODE 0: A = phi; epilogue rho and grad rho / 2 (4 components into part).
 // MODE r = 1..3: A = d phi / d x_r; epilogue sum X_r d_r phi (tau partial of direction r).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_density_mx(const int* __restrict__ grp_batch, const int* __restrict__ grp_tiles,
-                                                    const int* __restrict__ grp_slots, const int* __restrict__ grp_ntiles,
-                                                    const int* __restrict__ bt_off, const int* __restrict__ bt_list,
+__global__ __launch_bounds__(256, 3) void k_density_mx(const int* __restrict__ grp_batch, const int* __restrict__ grp_tiles,
+                                                    const int* __restrict__ grp_ntiles, const int* __restrict__ bt_off,
+                                                    const int* __restrict__ bt_list, const int* __restrict__ tile_nao,
+                                                    const double* __restrict__ bx, const double* __restrict__ by,
+                                                    const double* __restrict__ bz, BasisView bv,
                                                     const u16* __restrict__ cache, int sstride,
                                                     const u16* __restrict__ Ds, int npad, double* __restrict__ part,
                                                     int grp_stride) {
   constexpr int NOUT = MODE == 0 ? 4 : 1;
   __shared__ DensMxSmem sm;
-  const int grp = blockIdx.x;
+  const int grp = xcd_remap(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lrow = lane & 15, lk = lane >> 4;
@@ -172,31 +355,35 @@ __global__ __launch_bounds__(256) void k_density_mx(const int* __restrict__ grp_
   for (int q = 0; q < kGroup; ++q) jt[q] = grp_tiles[grp * kGroup + q];
   const int it0 = bt_off[b], it1 = bt_off[b + 1];
   const size_t np2 = (size_t)npad * npad;
+  (void)tile_nao;
+  if (tid < kBatch) {
+    sm.px[tid] = bx[b * kBatch + tid];
+    sm.py[tid] = by[b * kBatch + tid];
+    sm.pz[tid] = bz[b * kBatch + tid];
+  }
 
-  uint4 an[2][2], ac[2][2], rb[4];  // A next / current [row block][hl]; D staging
-  auto load = [&](int it) {
-    const u16* src = cache + (size_t)it * sstride + (size_t)MODE * kBatch * kMxRow;
+  // Two register stages in flight: the loads for K step k + 2 are issued
+  // while the MFMAs of step k run.
+  struct Stage {
+    uint4 a[2][2];  // A rows of this wave [row block][hl]
+    uint4 d[4];     // this thread's share of the D block
+  };
+  auto load = [&](int it, Stage& S) {
+    const u16* src = cache + (size_t)it * sstride + (size_t)MODE * kMxT;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const u16* row = src + (32 * wr + 16 * r + lrow) * kMxRow + 8 * lk;
-      an[r][0] = *reinterpret_cast<const uint4*>(row);
-      an[r][1] = *reinterpret_cast<const uint4*>(row + kTile);
+      S.a[r][0] = *reinterpret_cast<const uint4*>(row);
+      S.a[r][1] = *reinterpret_cast<const uint4*>(row + kTile);
     }
     const int ti = bt_list[it];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int e = tid + 256 * j;
       const int hl = e >> 9, n = (e >> 2) & 127, qc = e & 3, q = n >> 5;
-      rb[j] = q < nt ? *reinterpret_cast<const uint4*>(Ds + hl * np2 + (size_t)(jt[q] * kTile + (n & 31)) * npad +
-                                                       ti * kTile + 8 * qc)
-                     : uint4{0u, 0u, 0u, 0u};
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j;
-      sm.B[buf][e >> 9][(e >> 2) & 127][e & 3] = rb[j];
+      S.d[j] = q < nt ? *reinterpret_cast<const uint4*>(Ds + hl * np2 + (size_t)(jt[q] * kTile + (n & 31)) * npad +
+                                                        ti * kTile + 8 * qc)
+                      : uint4{0u, 0u, 0u, 0u};
     }
   };
 
@@ -205,154 +392,162 @@ __global__ __launch_bounds__(256) void k_density_mx(const int* __restrict__ grp_
   for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-  if (it0 < it1) load(it0);
-  int buf = 0;
-  for (int it = it0; it < it1; ++it) {
-    store(buf);
+  auto step = [&](int it, Stage& S, int buf) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j;
+      const int n = (e >> 2) & 127;
+      sm.B[buf][e >> 9][n][swz(n, e & 3)] = S.d[j];
+    }
+    uint4 ac[2][2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-      ac[r][0] = an[r][0];
-      ac[r][1] = an[r][1];
+      ac[r][0] = S.a[r][0];
+      ac[r][1] = S.a[r][1];
     }
     __syncthreads();
-    if (it + 1 < it1) load(it + 1);
+    if (it + 2 < it1) load(it + 2, S);
     // columns of absent tiles hold zeros in LDS, so every wave runs all blocks
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       const int n = 64 * wc + 16 * cb + lrow;
-      const uint4 bh = sm.B[buf][0][n][lk], bl = sm.B[buf][1][n][lk];
+      const uint4 bh = sm.B[buf][0][n][swz(n, lk)], bl = sm.B[buf][1][n][swz(n, lk)];
 #pragma unroll
       for (int r = 0; r < 2; ++r) acc[r][cb] = mfma3(ac[r][0], ac[r][1], bh, bl, acc[r][cb]);
     }
-    buf ^= 1;
+  };
+  Stage S0, S1;
+  if (it0 < it1) load(it0, S0);
+  if (it0 + 1 < it1) load(it0 + 1, S1);
+  for (int it = it0; it < it1; it += 2) {  // LDS buffer 0 for even steps, 1 for odd
+    step(it, S0, 0);
+    if (it + 1 < it1) step(it + 1, S1, 1);
   }
 
-  // Epilogue: X[g = 32 wr + 16 r + 4 lk + i][n = 64 wc + 16 cb + lrow] against phi / grad phi of tile q = n / 32.
-  float s[NOUT][2][4];
+  // Epilogue. X[g = 32 wr + 16 r + 4 lk + i][n = 64 wc + 16 cb + lrow] -> LDS as X^T.
+  __syncthreads();  // every wave is done with the D blocks
 #pragma unroll
-  for (int c = 0; c < NOUT; ++c)
+  for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) s[c][r][i] = 0.f;
-#pragma unroll
-  for (int cb = 0; cb < 4; ++cb) {
-    const int q = 2 * wc + (cb >> 1);  // wave-uniform
-    if (q < nt) {
-      const u16* src = cache + (size_t)grp_slots[grp * kGroup + q] * sstride;
-      const int col = 16 * (cb & 1) + lrow;
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int g = 32 * wr + 16 * r + 4 * lk + i;
-          const float x = acc[r][cb][i];
-#pragma unroll
-          for (int c = 0; c < NOUT; ++c)
-            s[c][r][i] += x * pair_at(src + ((MODE == 0 ? c : MODE) * kBatch + g) * kMxRow + col, kTile);
-        }
-    }
+      for (int i = 0; i < 4; ++i) sm.X[64 * wc + 16 * cb + lrow][32 * wr + 16 * r + 4 * lk + i] = acc[r][cb][i];
+  __syncthreads();
+  DensEpi<MODE> epi;
+  if (wave < nt) {  // wave q = tile q of the group; lane = point
+    epi.xq = &sm.X[kTile * wave][lane];
+    walk_tile_f32(bv, grp_tiles[grp * kGroup + wave], sm.px[lane], sm.py[lane], sm.pz[lane], epi);
   }
 #pragma unroll
-  for (int c = 0; c < NOUT; ++c)
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float v = s[c][r][i];
-        v += __shfl_xor(v, 1, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 4, 64);
-        v += __shfl_xor(v, 8, 64);
-        s[c][r][i] = v;
-      }
-  if (lrow == 0) {
-#pragma unroll
-    for (int c = 0; c < NOUT; ++c)
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) sm.red[wc][c][32 * wr + 16 * r + 4 * lk + i] = s[c][r][i];
-  }
+  for (int c = 0; c < NOUT; ++c) sm.red[wave][c][lane] = epi.o[c];
   __syncthreads();
   if (tid < NOUT * kBatch) {
     const int c = tid / kBatch, g = tid % kBatch;
-    part[(size_t)grp * grp_stride + c * kBatch + g] = (double)sm.red[0][c][g] + (double)sm.red[1][c][g];
+    double v = 0.0;
+    for (int q = 0; q < nt; ++q) v += (double)sm.red[q][c][g];
+    part[(size_t)grp * grp_stride + c * kBatch + g] = v;
   }
 }
 
 void launch_density_mx(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
-                       const int* grp_slots, const int* grp_ntiles, const int* bt_off, const int* bt_list,
+                       const int* grp_ntiles, const int* bt_off, const int* bt_list, const int* tile_nao,
+                       const double* bx, const double* by, const double* bz, BasisView bv,
                        const unsigned short* cache, int sstride, const unsigned short* Ds, int npad, double* part) {
   if (ngroups == 0) return;
   const int stride = (mode == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
+#define CHX_DENS_MX(M)                                                                                              \
+  hipLaunchKernelGGL(k_density_mx<M>, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_ntiles, bt_off,   \
+                     bt_list, tile_nao, bx, by, bz, bv, cache, sstride, Ds, npad, part, stride)
   switch (mode) {
-    case 0: hipLaunchKernelGGL(k_density_mx<0>, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_slots, grp_ntiles, bt_off, bt_list, cache, sstride, Ds, npad, part, stride); break;
-    case 1: hipLaunchKernelGGL(k_density_mx<1>, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_slots, grp_ntiles, bt_off, bt_list, cache, sstride, Ds, npad, part, stride); break;
-    case 2: hipLaunchKernelGGL(k_density_mx<2>, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_slots, grp_ntiles, bt_off, bt_list, cache, sstride, Ds, npad, part, stride); break;
-    default: hipLaunchKernelGGL(k_density_mx<3>, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_slots, grp_ntiles, bt_off, bt_list, cache, sstride, Ds, npad, part, stride); break;
+    case 0: CHX_DENS_MX(0); break;
+    case 1: CHX_DENS_MX(1); break;
+    case 2: CHX_DENS_MX(2); break;
+    default: CHX_DENS_MX(3); break;
   }
+#undef CHX_DENS_MX
   CHX_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
-// B^T in split form for every slot of a batch (one block per batch):
-// Bt[slot][0][hl][nu][g] = split(a phi + c . grad phi); meta-GGA (NC = 4) adds
-// Bt[slot][r][hl][nu][g] = split(t d_r phi), t = w v_tau / 4.
+// B^T in split form, one wave per slot (lane = point), from phi and grad phi
+// recomputed in FP32:
+//   Bt[slot][0][hl][nu][g] = split(a phi + c . grad phi),
+//   meta-GGA (NC = 4) adds Bt[slot][r][hl][nu][g] = split(t d_r phi), t = w v_tau / 4.
+// Each wave assembles its slot's [c][hl][nu][g] block in LDS (zeros past the
+// tile's last AO) and writes it with 16-byte coalesced stores.
 template <int NC>
-__global__ __launch_bounds__(256) void k_make_b_mx(const int* __restrict__ bt_off, const double* __restrict__ coef,
-                                                   const u16* __restrict__ cache, int sstride, u16* __restrict__ bt) {
-  __shared__ float ca[kBatch], cx[kBatch], cy[kBatch], cz[kBatch], ct[kBatch];
-  const int b = blockIdx.x, tid = threadIdx.x;
-  if (tid < kBatch) {
-    const size_t slot = (size_t)b * kBatch + tid;
-    ca[tid] = (float)coef[kCoef * slot];
-    cx[tid] = (float)coef[kCoef * slot + 1];
-    cy[tid] = (float)coef[kCoef * slot + 2];
-    cz[tid] = (float)coef[kCoef * slot + 3];
-    ct[tid] = (float)coef[kCoef * slot + 4];
+struct MakeB {
+  u16* sv;  // this lane's column of the [c][hl][nu][g] block
+  float ca, cx, cy, cz, ct, R, Rx, Ry, Rz, s1;
+  __device__ void begin(float a, float ax, float ay, float az) {
+    R = a; Rx = ax; Ry = ay; Rz = az;
+    s1 = ca * a + cx * ax + cy * ay + cz * az;  // (a + c . grad) R
+  }
+  __device__ void comp(int col, float q, float qx, float qy, float qz) {
+    u16 hi, lo;
+    split_bf(q * s1 + R * (cx * qx + cy * qy + cz * qz), hi, lo);  // a phi + c . grad phi
+    sv[col * kBatch] = hi;
+    sv[kTile * kBatch + col * kBatch] = lo;
+    if (NC == 4) {
+      split_bf(ct * (Rx * q + R * qx), hi, lo);
+      sv[kMxT + col * kBatch] = hi;
+      sv[kMxT + kTile * kBatch + col * kBatch] = lo;
+      split_bf(ct * (Ry * q + R * qy), hi, lo);
+      sv[2 * kMxT + col * kBatch] = hi;
+      sv[2 * kMxT + kTile * kBatch + col * kBatch] = lo;
+      split_bf(ct * (Rz * q + R * qz), hi, lo);
+      sv[3 * kMxT + col * kBatch] = hi;
+      sv[3 * kMxT + kTile * kBatch + col * kBatch] = lo;
+    }
+  }
+  __device__ void end() {}
+};
+
+template <int NC>
+__global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(int nslots, const int* __restrict__ slot_batch,
+                                                                  const int* __restrict__ slot_tile,
+                                                                  const double* __restrict__ bx,
+                                                                  const double* __restrict__ by,
+                                                                  const double* __restrict__ bz, BasisView bv,
+                                                                  const double* __restrict__ coef,
+                                                                  u16* __restrict__ bt) {
+  constexpr int W = NC == 4 ? 1 : 4;  // waves per block
+  __shared__ uint4 stage[W][NC * kMxT / 8];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
+  const int slot = blockIdx.x * W + wave;
+  uint4* st = stage[wave];
+#pragma unroll
+  for (int j = 0; j < NC * kMxT / 8 / 64; ++j) st[g + 64 * j] = uint4{0u, 0u, 0u, 0u};
+  __syncthreads();
+  if (slot < nslots) {
+    const int b = slot_batch[slot], t = slot_tile[slot];
+    const size_t p = (size_t)b * kBatch + g;
+    const float ca = (float)coef[kCoef * p], cx = (float)coef[kCoef * p + 1], cy = (float)coef[kCoef * p + 2],
+                cz = (float)coef[kCoef * p + 3], ct = (float)coef[kCoef * p + 4];
+    MakeB<NC> op;
+    op.sv = reinterpret_cast<u16*>(st) + g;  // [c][hl][nu][g]
+    op.ca = ca; op.cx = cx; op.cy = cy; op.cz = cz; op.ct = ct;
+    walk_tile_f32(bv, t, bx[p], by[p], bz[p], op);
   }
   __syncthreads();
-  const int g = tid & 63, m8 = tid >> 6;  // point, 8-AO chunk
-  for (int it = bt_off[b]; it < bt_off[b + 1]; ++it) {
-    const u16* src = cache + (size_t)it * sstride;
-    float v[4][8];
+  if (slot < nslots) {
+    uint4* dst = reinterpret_cast<uint4*>(bt + (size_t)slot * NC * kMxT);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const u16* row = src + (c * kBatch + g) * kMxRow + 8 * m8;
-      const uint4 hv = *reinterpret_cast<const uint4*>(row), lv = *reinterpret_cast<const uint4*>(row + kTile);
-      const u16* h = reinterpret_cast<const u16*>(&hv);
-      const u16* l = reinterpret_cast<const u16*>(&lv);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) v[c][e] = bf2f(h[e]) + bf2f(l[e]);
-    }
-    u16* dst = bt + (size_t)it * NC * kMxT;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int o = (8 * m8 + e) * kBatch + g;
-      u16 hi, lo;
-      split_bf(ca[g] * v[0][e] + cx[g] * v[1][e] + cy[g] * v[2][e] + cz[g] * v[3][e], hi, lo);
-      dst[o] = hi;
-      dst[kTile * kBatch + o] = lo;
-      if (NC == 4) {
-#pragma unroll
-        for (int r = 1; r < 4; ++r) {
-          split_bf(ct[g] * v[r][e], hi, lo);
-          dst[r * kMxT + o] = hi;
-          dst[r * kMxT + kTile * kBatch + o] = lo;
-        }
-      }
-    }
+    for (int j = 0; j < NC * kMxT / 8 / 64; ++j) dst[g + 64 * j] = st[g + 64 * j];
   }
 }
 
-void launch_make_b_mx(hipStream_t s, int ncomp, int nbatches, const int* bt_off, const double* coef,
-                      const unsigned short* cache, int sstride, unsigned short* bt) {
-  if (nbatches == 0) return;
+void launch_make_b_mx(hipStream_t s, int ncomp, int nslots, const int* slot_batch, const int* slot_tile,
+                      const int* tile_nao, const double* bx, const double* by, const double* bz, BasisView bv,
+                      const double* coef, unsigned short* bt) {
+  (void)tile_nao;  // columns past a tile's last AO stay zero from the LDS clear
+  if (nslots == 0) return;
   if (ncomp == 4)
-    hipLaunchKernelGGL(k_make_b_mx<4>, dim3(nbatches), dim3(256), 0, s, bt_off, coef, cache, sstride, bt);
+    hipLaunchKernelGGL(k_make_b_mx<4>, dim3(nslots), dim3(64), 0, s, nslots, slot_batch, slot_tile, bx, by, bz, bv,
+                       coef, bt);
   else
-    hipLaunchKernelGGL(k_make_b_mx<1>, dim3(nbatches), dim3(256), 0, s, bt_off, coef, cache, sstride, bt);
+    hipLaunchKernelGGL(k_make_b_mx<1>, dim3((nslots + 3) / 4), dim3(256), 0, s, nslots, slot_batch, slot_tile, bx, by,
+                       bz, bv, coef, bt);
   CHX_HIP(hipGetLastError());
 }
 
@@ -363,48 +558,41 @@ void launch_make_b_mx(hipStream_t s, int ncomp, int nbatches, const int* bt_off,
 // double-buffered. 4 waves in a 2 x 2 grid, each owning a 64 x 64 block of M.
 // One FP64 atomic add per element per work item.
 struct VxcMxSmem {
-  uint4 A[2][2][kMxSup * kTile][kLdsRow];  // [buf][hl][mu][g chunk]
-  uint4 B[2][2][kMxSup * kTile][kLdsRow];
+  uint4 A[2][2][kMxSup * kTile][4];  // [buf][hl][mu][g chunk, swizzled]
+  uint4 B[2][2][kMxSup * kTile][4];
 };
 
 template <int NC>
-__global__ __launch_bounds__(256) void k_vxc_mx(const int4* __restrict__ work, const int* __restrict__ pair_batches,
+__global__ __launch_bounds__(256, 2) void k_vxc_mx(const int4* __restrict__ work, const int* __restrict__ pair_batches,
                                                 const int* __restrict__ slot_of, int ntiles, const u16* __restrict__ cache,
-                                                int sstride, const u16* __restrict__ bt, double* __restrict__ Mint,
-                                                int npad) {
+                                                int sstride, int toff, const u16* __restrict__ bt,
+                                                double* __restrict__ Mint, int npad) {
   __shared__ VxcMxSmem sm;
-  const int4 wk = work[blockIdx.x];
+  const int4 wk = work[xcd_remap(blockIdx.x, gridDim.x)];
   const int I = wk.x, J = wk.y;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lrow = lane & 15, lk = lane >> 4;
   const int wr = wave & 1, wc = wave >> 1;
   // Thread t moves, for tile t / 64 of each super-tile, the 16-byte chunks
-  // e = t + 256 j (j = 0..3): hl = (e >> 7) & 1, row (e >> 2) & 31, chunk e & 3.
+  // e = t % 64 + 64 j (j = 0..3) of that tile: hl = e >> 7, row (e >> 2) & 31, chunk e & 3.
   const int tj = tid >> 6;
-  uint4 ra[4], rq[4];
-  auto load = [&](int k) {  // stage k -> (batch, half, component)
+  struct Stage {
+    uint4 a[4], q[4];
+  };
+  auto load = [&](int k, Stage& S) {  // stage k -> (batch, half, component)
     const int it = wk.z + k / (2 * NC), h = (k / NC) % 2, c = k % NC;
     const int b = pair_batches[it];
     const int si = slot_of[(size_t)b * ntiles + I * kMxSup + tj];
     const int sj = slot_of[(size_t)b * ntiles + J * kMxSup + tj];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int e = (tid & 63) + 64 * j;  // chunk within the tile: 256 chunks = [hl 2][row 32][chunk 4]
-      const int off = ((e >> 7) & 1) * (kTile * kBatch) + ((e >> 2) & 31) * kBatch + h * 32 + 8 * (e & 3);
-      ra[j] = si >= 0 ? *reinterpret_cast<const uint4*>(cache + (size_t)si * sstride + kMxG + c * kMxT + off)
-                      : uint4{0u, 0u, 0u, 0u};
-      rq[j] = sj >= 0 ? *reinterpret_cast<const uint4*>(bt + ((size_t)sj * NC + c) * kMxT + off)
-                      : uint4{0u, 0u, 0u, 0u};
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
       const int e = (tid & 63) + 64 * j;
-      const int hl = (e >> 7) & 1, row = tj * kTile + ((e >> 2) & 31), ch = e & 3;
-      sm.A[buf][hl][row][ch] = ra[j];
-      sm.B[buf][hl][row][ch] = rq[j];
+      const int off = (e >> 7) * (kTile * kBatch) + ((e >> 2) & 31) * kBatch + h * 32 + 8 * (e & 3);
+      S.a[j] = si >= 0 ? *reinterpret_cast<const uint4*>(cache + (size_t)si * sstride + toff + c * kMxT + off)
+                       : uint4{0u, 0u, 0u, 0u};
+      S.q[j] = sj >= 0 ? *reinterpret_cast<const uint4*>(bt + ((size_t)sj * NC + c) * kMxT + off)
+                       : uint4{0u, 0u, 0u, 0u};
     }
   };
   f32x4 acc[4][4];
@@ -413,26 +601,39 @@ __global__ __launch_bounds__(256) void k_vxc_mx(const int4* __restrict__ work, c
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int nstage = (wk.w - wk.z) * 2 * NC;
-  if (nstage > 0) load(0);
-  int buf = 0;
-  for (int k = 0; k < nstage; ++k) {
-    store(buf);
+  // Two register stages in flight: stage k + 2 loads while stage k computes.
+  auto step = [&](int k, Stage& S, int buf) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = (tid & 63) + 64 * j;
+      const int hl = e >> 7, row = tj * kTile + ((e >> 2) & 31), ch = swz(row, e & 3);
+      sm.A[buf][hl][row][ch] = S.a[j];
+      sm.B[buf][hl][row][ch] = S.q[j];
+    }
     __syncthreads();
-    if (k + 1 < nstage) load(k + 1);
+    if (k + 2 < nstage) load(k + 2, S);
     uint4 ah[4], al[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      ah[r] = sm.A[buf][0][64 * wr + 16 * r + lrow][lk];
-      al[r] = sm.A[buf][1][64 * wr + 16 * r + lrow][lk];
+      const int row = 64 * wr + 16 * r + lrow;
+      ah[r] = sm.A[buf][0][row][swz(row, lk)];
+      al[r] = sm.A[buf][1][row][swz(row, lk)];
     }
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
-      const uint4 bh = sm.B[buf][0][64 * wc + 16 * cb + lrow][lk];
-      const uint4 bl = sm.B[buf][1][64 * wc + 16 * cb + lrow][lk];
+      const int col = 64 * wc + 16 * cb + lrow;
+      const uint4 bh = sm.B[buf][0][col][swz(col, lk)];
+      const uint4 bl = sm.B[buf][1][col][swz(col, lk)];
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r][cb] = mfma3(ah[r], al[r], bh, bl, acc[r][cb]);
     }
-    buf ^= 1;
+  };
+  Stage S0, S1;
+  if (nstage > 0) load(0, S0);
+  if (nstage > 1) load(1, S1);
+  for (int k = 0; k < nstage; k += 2) {  // LDS buffer 0 for even stages, 1 for odd
+    step(k, S0, 0);
+    if (k + 1 < nstage) step(k + 1, S1, 1);
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r)
@@ -447,15 +648,15 @@ __global__ __launch_bounds__(256) void k_vxc_mx(const int4* __restrict__ work, c
 }
 
 void launch_vxc_mx(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
-                   int ntiles, const unsigned short* cache, int sstride, const unsigned short* bt, double* Mint,
-                   int npad) {
+                   int ntiles, const unsigned short* cache, int sstride, int toff, const unsigned short* bt,
+                   double* Mint, int npad) {
   if (nwork == 0) return;
   if (ncomp == 4)
     hipLaunchKernelGGL(k_vxc_mx<4>, dim3(nwork), dim3(256), 0, s, work, pair_batches, slot_of, ntiles, cache, sstride,
-                       bt, Mint, npad);
+                       toff, bt, Mint, npad);
   else
     hipLaunchKernelGGL(k_vxc_mx<1>, dim3(nwork), dim3(256), 0, s, work, pair_batches, slot_of, ntiles, cache, sstride,
-                       bt, Mint, npad);
+                       toff, bt, Mint, npad);
   CHX_HIP(hipGetLastError());
 }
 

@@ -160,7 +160,9 @@ int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double
 
 /* ---- Diagnostics (no reference counterpart) ---------------------------------
  * Runs one FP64 MFMA tile on the device and returns the largest deviation from
- * the scalar product; 0 means the operand layout the kernels assume holds. */
+ * the scalar product, plus the number of 2^20 sample floats whose hardware
+ * bf16 conversion differs from round-to-nearest-even; 0 means the operand
+ * layout and the rounding the kernels assume hold. */
 int chx_selftest(double* max_err);
 
 #ifdef __cplusplus
