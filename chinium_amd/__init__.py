@@ -89,6 +89,12 @@ def lib():
         "chx_selftest": (i, [P(d)]),
         "chx_lebedev_rule": (i, [i, _vp]),
         "chx_xc_eval_points": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "chx_pack_create": (i, [_vp, i, P(_vp)]),
+        "chx_pack_xc": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "chx_pack_set_graph": (i, [_vp, i]),
+        "chx_pack_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
+        "chx_pack_stream": (i, [_vp, P(_vp)]),
+        "chx_pack_destroy": (None, [_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -348,6 +354,53 @@ class ExchangeCorrelation:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.chx_xc_destroy(self.h)
+            self.h = None
+
+
+class Pack:
+    """Many molecules per launch (chx_pack_*): XC builds of all the grids with
+    one kernel launch per stage, replayed as a hipGraph. Grids must be in the
+    mixed-precision form with getAO done; they stay referenced by the pack."""
+
+    def __init__(self, grids):
+        self.grids = list(grids)
+        n = len(self.grids)
+        arr = (ctypes.c_void_p * n)(*[g.h for g in self.grids])
+        h = ctypes.c_void_p()
+        _check(lib().chx_pack_create(arr, n, ctypes.byref(h)))
+        self.h = h
+
+    def _ptrs(self, ptrs):
+        assert len(ptrs) == len(self.grids)
+        return (ctypes.c_void_p * len(ptrs))(*ptrs)
+
+    def xc(self, xc, D_ptrs, V_ptrs):
+        """One XC build of every molecule: D_ptrs / V_ptrs are device pointers
+        (column-major nbasis^2 each, reference order). Returns (E_xc, N_e) arrays."""
+        n = len(self.grids)
+        E, N = np.zeros(n), np.zeros(n)
+        _check(lib().chx_pack_xc(self.h, xc.h, self._ptrs(D_ptrs), self._ptrs(V_ptrs), _ptr(E), _ptr(N)))
+        return E, N
+
+    def set_graph(self, on):
+        _check(lib().chx_pack_set_graph(self.h, 1 if on else 0))
+
+    def autotune(self, xc, D_ptrs, V_ptrs, reps=3):
+        """Times each Vxc work-item length, keeps the fastest; returns (chunk, {chunk: ms})."""
+        c = ctypes.c_int()
+        ms = (ctypes.c_float * 4)()
+        _check(lib().chx_pack_autotune(self.h, xc.h, self._ptrs(D_ptrs), self._ptrs(V_ptrs), int(reps),
+                                       ctypes.byref(c), ms))
+        return c.value, dict(zip((16, 32, 64, 128), list(ms)))
+
+    def stream(self):
+        p = ctypes.c_void_p()
+        _check(lib().chx_pack_stream(self.h, ctypes.byref(p)))
+        return p.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.chx_pack_destroy(self.h)
             self.h = None
 
 

@@ -60,6 +60,11 @@ void chx_grid::collect_times() {
 
 namespace {
 thread_local std::string g_last_error;
+}  // namespace
+namespace chx {
+std::string& last_error() { return g_last_error; }
+}  // namespace chx
+namespace {
 
 template <class F> int guard(F&& f) {
   try {
@@ -103,10 +108,7 @@ void density_from_device(chx_grid* g, const double* dD) {
     g->mark_end(kStPermute, e);
     g->mark_begin(kStDensity, &e);
     for (int r = 0; r <= (g->type >= 2 ? 3 : 0); ++r)  // r > 0: tau partials, A = d_r phi
-      launch_density_mx(g->stream, r, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
-                        g->d_bt_list.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_cmx.p,
-                        g->mx_sstride, g->d_Dsplit.p, g->npad,
-                        r == 0 ? g->d_part.p : g->d_part_tau.p + (size_t)(r - 1) * kBatch);
+      launch_density_mx(g->stream, r, ng, g->d_mol.p, nullptr);
     g->mark_end(kStDensity, e);
     g->have_density = true;
     g->have_eval = false;
@@ -141,13 +143,12 @@ void fock_to_device(chx_grid* g, double* dV) {
     const int nc = g->v_family >= 2 ? 4 : 1;
     require(nc <= g->mx_nT, CHX_ERR_STATE, "a meta-GGA potential needs a grid of type 2 before getAO");
     g->mark_begin(kStMakeB, &e);
-    launch_make_b_mx(g->stream, nc, (int)g->bt_list.size(), g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p,
-                     g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_coef.p, g->d_bmx.p);
+    const int nslot = (int)g->bt_list.size();
+    launch_make_b_mx(g->stream, nc, nc == 4 ? nslot : (nslot + 3) / 4, g->d_mol.p, nullptr);
     g->mark_end(kStMakeB, e);
     launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
     g->mark_begin(kStVxc, &e);
-    launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p, g->ntiles,
-                  g->d_cmx.p, g->mx_sstride, g->mx_nT * kMxComp, g->d_bmx.p, g->d_Mint.p, g->npad);
+    launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->d_mol.p, nullptr);
     g->mark_end(kStVxc, e);
     g->mark_begin(kStFinalize, &e);
     launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);

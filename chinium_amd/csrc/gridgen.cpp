@@ -319,6 +319,61 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
   g->ao_ready = false;
 }
 
+// Vxc work items of the cached / mixed paths: (super-tile I, super-tile J,
+// range of pair_batches) over 128-AO super-tiles, a chunk of batches where
+// both are alive. Chunk-major order: blocks that run at the same time work on
+// the same batches of different (I, J) pairs, so their operand tiles meet in
+// L2. Fewer, longer items mean fewer FP64 atomics into M.
+void build_cwork(chx_grid* g) {
+  const int nb = g->nbatches, nt = g->ntiles;
+  const int nsc = g->npad / (kCacheSuper * kTile);
+  std::vector<std::vector<int>> cp((size_t)nsc * nsc);
+  std::vector<int> alive_sup;
+  for (int b = 0; b < nb; ++b) {
+    alive_sup.clear();
+    for (int sI = 0; sI < nsc; ++sI) {
+      bool any = false;
+      for (int h = 0; h < kCacheSuper; ++h) any |= g->alive[(size_t)b * nt + sI * kCacheSuper + h] != 0;
+      if (any) alive_sup.push_back(sI);
+    }
+    for (int I : alive_sup)
+      for (int J : alive_sup) cp[(size_t)I * nsc + J].push_back(b);
+  }
+  g->cwork.clear();
+  g->cpair_batches.clear();
+  size_t maxlen = 0;
+  for (const auto& L : cp) maxlen = std::max(maxlen, L.size());
+  const size_t chunk =
+      g->vxc_chunk > 0 ? (size_t)g->vxc_chunk : (g->precision == CHX_PREC_MIXED ? 32 : kCacheChunk);
+  for (size_t k = 0; k < maxlen; k += chunk)
+    for (int I = 0; I < nsc; ++I)
+      for (int J = 0; J < nsc; ++J) {
+        const auto& L = cp[(size_t)I * nsc + J];
+        if (k >= L.size()) continue;
+        const int begin = (int)g->cpair_batches.size();
+        const size_t e = std::min(L.size(), k + chunk);
+        g->cpair_batches.insert(g->cpair_batches.end(), L.begin() + k, L.begin() + e);
+        g->cwork.push_back(make_int4(I, J, begin, (int)g->cpair_batches.size()));
+      }
+}
+
+// Rebuilds the Vxc work items for another chunk length (autotuning) without
+// touching the AO tables.
+void set_vxc_chunk(chx_grid* g, int chunk) {
+  g->vxc_chunk = chunk;
+  if (!g->ao_ready || !g->cached) return;
+  build_cwork(g);
+  hipStream_t st = g->stream;
+  g->d_cwork.upload(g->cwork, st);
+  g->d_cpair_batches.upload(g->cpair_batches, st);
+  if (g->precision == CHX_PREC_MIXED) {
+    g->hmol.cwork = g->d_cwork.p;
+    g->hmol.pair_batches = g->d_cpair_batches.p;
+    g->d_mol.upload(std::vector<MolDev>{g->hmol}, st);
+  }
+  CHX_HIP(hipStreamSynchronize(st));
+}
+
 // Screening and work plans (Grid::getAO counterpart).
 void grid_plan(chx_grid* g) {
   hipStream_t st = g->stream;
@@ -421,38 +476,7 @@ void grid_plan(chx_grid* g) {
                                     " GB of device memory for its AO cache");
   g->cached = mixed || g->cache_mode == 1 || (g->cache_mode == -1 && cache_bytes < 0.8 * (double)free_b);
   if (g->cached) {
-    const int nsc = g->npad / (kCacheSuper * kTile);
-    std::vector<std::vector<int>> cp((size_t)nsc * nsc);
-    std::vector<int> alive_sup;
-    for (int b = 0; b < nb; ++b) {
-      alive_sup.clear();
-      for (int sI = 0; sI < nsc; ++sI) {
-        bool any = false;
-        for (int h = 0; h < kCacheSuper; ++h) any |= g->alive[(size_t)b * nt + sI * kCacheSuper + h] != 0;
-        if (any) alive_sup.push_back(sI);
-      }
-      for (int I : alive_sup)
-        for (int J : alive_sup) cp[(size_t)I * nsc + J].push_back(b);
-    }
-    // Chunk-major order: blocks that run at the same time work on the same
-    // batches of different (I, J) pairs, so their operand tiles meet in L2/MALL.
-    g->cwork.clear();
-    g->cpair_batches.clear();
-    size_t maxlen = 0;
-    for (const auto& L : cp) maxlen = std::max(maxlen, L.size());
-    // Fewer, longer items mean fewer FP64 atomics into M; the mixed form's
-    // items are cheap per batch, so it takes longer ones.
-    const size_t chunk = g->vxc_chunk > 0 ? (size_t)g->vxc_chunk : (mixed ? 32 : kCacheChunk);
-    for (size_t k = 0; k < maxlen; k += chunk)
-      for (int I = 0; I < nsc; ++I)
-        for (int J = 0; J < nsc; ++J) {
-          const auto& L = cp[(size_t)I * nsc + J];
-          if (k >= L.size()) continue;
-          const int begin = (int)g->cpair_batches.size();
-          const size_t e = std::min(L.size(), k + chunk);
-          g->cpair_batches.insert(g->cpair_batches.end(), L.begin() + k, L.begin() + e);
-          g->cwork.push_back(make_int4(I, J, begin, (int)g->cpair_batches.size()));
-        }
+    build_cwork(g);
     g->d_slot_batch.upload(g->slot_batch, st); g->d_slot_tile.upload(g->slot_tile, st);
     g->d_slot_of.upload(g->slot_of, st); g->d_tile_nao.upload(g->tile_nao, st);
     g->d_grp_slots.upload(g->grp_slots, st);
@@ -506,6 +530,22 @@ void grid_plan(chx_grid* g) {
   g->en_blocks = (int)std::max<size_t>(1, (nslots + 255) / 256);
   g->d_en_part.alloc(2 * (size_t)g->en_blocks);
   g->d_en.alloc(2);
+  if (mixed) {  // descriptor of this grid for the mixed kernels
+    MolDev& m = g->hmol;
+    m = MolDev{};
+    m.grp_batch = g->d_grp_batch.p; m.grp_tiles = g->d_grp_tiles.p; m.grp_ntiles = g->d_grp_ntiles.p;
+    m.bt_off = g->d_bt_off.p; m.bt_list = g->d_bt_list.p;
+    m.bx = g->d_x.p; m.by = g->d_y.p; m.bz = g->d_z.p;
+    m.bv = g->view();
+    m.cache = g->d_cmx.p; m.sstride = g->mx_sstride; m.toff = g->mx_nT * kMxComp;
+    m.Ds = g->d_Dsplit.p; m.npad = g->npad; m.ntiles = g->ntiles; m.nslots = (int)g->bt_list.size();
+    m.part = g->d_part.p; m.part_tau = g->d_part_tau.p;
+    m.slot_batch = g->d_slot_batch.p; m.slot_tile = g->d_slot_tile.p;
+    m.coef = g->d_coef.p; m.bt = g->d_bmx.p;
+    m.cwork = g->d_cwork.p; m.pair_batches = g->d_cpair_batches.p; m.slot_of = g->d_slot_of.p;
+    m.Mint = g->d_Mint.p;
+    g->d_mol.upload(std::vector<MolDev>{m}, st);
+  }
   CHX_HIP(hipStreamSynchronize(st));
   g->ao_ready = true;
   g->have_density = g->have_eval = g->have_v = false;

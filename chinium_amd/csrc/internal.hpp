@@ -126,6 +126,26 @@ void launch_make_b(hipStream_t s, int ncomp, int nbatches, const int* bt_off, co
 void launch_vxc_c64(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
                     int ntiles, const double* cache, const double* bmat, double* Mint, int npad);
 void launch_finalize_sym(hipStream_t s, int n, int npad, const double* Mint, const int* ref2int, double* V);
+// Device descriptor of one grid for the mixed-precision kernels. A kernel
+// launch covers a list of (grid, item) pairs, so one launch can serve many
+// molecules (a pack) as well as a single grid.
+struct MolDev {
+  const int *grp_batch, *grp_tiles, *grp_ntiles, *bt_off, *bt_list;  // density work groups
+  const double *bx, *by, *bz;                                          // points
+  BasisView bv;
+  const unsigned short* cache;  // split operand cache
+  int sstride, toff;            // slot stride; offset of the transposed layout
+  const unsigned short* Ds;     // split D
+  int npad, ntiles, nslots, pad_;
+  double *part, *part_tau;      // density partials
+  const int *slot_batch, *slot_tile;
+  const double* coef;           // Vxc integrand coefficients per point
+  unsigned short* bt;           // B^T per slot
+  const int4* cwork;            // Vxc work items
+  const int *pair_batches, *slot_of;
+  double* Mint;                 // M accumulator, npad^2
+};
+
 // Mixed-precision form (kernels_mixed.hip): MFMA operands cached as bf16
 // hi/lo pairs, split-BF16 MFMA contractions with FP32 accumulation, FP32
 // recomputation of the AO values outside the MFMA operands.
@@ -136,16 +156,11 @@ void launch_ao_cache_mx(hipStream_t s, int nslots, const int* slot_batch, const 
 void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds);
 void launch_bf16_check(hipStream_t s, unsigned n, unsigned* bad);
 void launch_fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v);
-void launch_density_mx(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
-                       const int* grp_ntiles, const int* bt_off, const int* bt_list, const int* tile_nao,
-                       const double* bx, const double* by, const double* bz, BasisView bv,
-                       const unsigned short* cache, int sstride, const unsigned short* Ds, int npad, double* part);
-void launch_make_b_mx(hipStream_t s, int ncomp, int nslots, const int* slot_batch, const int* slot_tile,
-                      const int* tile_nao, const double* bx, const double* by, const double* bz, BasisView bv,
-                      const double* coef, unsigned short* bt);
-void launch_vxc_mx(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
-                   int ntiles, const unsigned short* cache, int sstride, int toff, const unsigned short* bt,
-                   double* Mint, int npad);
+// Launches over items = (grid index into mols, group / first slot / work item);
+// items == nullptr means grid 0 and item = block.
+void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev* mols, const int2* items);
+void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev* mols, const int2* items);
+void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev* mols, const int2* items);
 
 }  // namespace chx
 
@@ -216,6 +231,8 @@ struct chx_grid {
   int max_nprim = 0;
   // Mixed form: split AO cache (AOg + nT transposed components per slot), B^T, split D.
   chx::DevBuf<unsigned short> d_cmx, d_bmx, d_Dsplit;
+  chx::MolDev hmol{};                 // descriptor of this grid (mixed form)
+  chx::DevBuf<chx::MolDev> d_mol;
   int mx_sstride = 0, mx_nT = 0, bmx_comps = 0;
   chx::DevBuf<int> d_sh_type, d_sh_nprim, d_sh_poff, d_tile_seg_off;
   chx::DevBuf<int4> d_seg;

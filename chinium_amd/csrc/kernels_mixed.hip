@@ -333,33 +333,35 @@ struct DensMxSmem {
 // MODE 0: A = phi; epilogue rho and grad rho / 2 (4 components into part).
 // MODE r = 1..3: A = d phi / d x_r; epilogue sum X_r d_r phi (tau partial of direction r).
 template <int MODE>
-__global__ __launch_bounds__(256, 3) void k_density_mx(const int* __restrict__ grp_batch, const int* __restrict__ grp_tiles,
-                                                    const int* __restrict__ grp_ntiles, const int* __restrict__ bt_off,
-                                                    const int* __restrict__ bt_list, const int* __restrict__ tile_nao,
-                                                    const double* __restrict__ bx, const double* __restrict__ by,
-                                                    const double* __restrict__ bz, BasisView bv,
-                                                    const u16* __restrict__ cache, int sstride,
-                                                    const u16* __restrict__ Ds, int npad, double* __restrict__ part,
-                                                    int grp_stride) {
+__global__ __launch_bounds__(256, 3) void k_density_mx(const MolDev* __restrict__ mols, const int2* __restrict__ items) {
   constexpr int NOUT = MODE == 0 ? 4 : 1;
+  constexpr int grp_stride = (MODE == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
   __shared__ DensMxSmem sm;
-  const int grp = xcd_remap(blockIdx.x, gridDim.x);
+  const int item = xcd_remap(blockIdx.x, gridDim.x);
+  const int2 wi = items ? items[item] : make_int2(0, item);
+  const MolDev& m = mols[wi.x];
+  const int grp = wi.y;
+  const u16* __restrict__ cache = m.cache;
+  const u16* __restrict__ Ds = m.Ds;
+  const int sstride = m.sstride, npad = m.npad;
+  const int* __restrict__ bt_list = m.bt_list;
+  const int* __restrict__ grp_tiles = m.grp_tiles;
+  double* __restrict__ part = MODE == 0 ? m.part : m.part_tau + (MODE - 1) * kBatch;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lrow = lane & 15, lk = lane >> 4;
   const int wr = wave & 1, wc = wave >> 1;
-  const int b = grp_batch[grp];
-  const int nt = grp_ntiles[grp];
+  const int b = m.grp_batch[grp];
+  const int nt = m.grp_ntiles[grp];
   int jt[kGroup];
 #pragma unroll
   for (int q = 0; q < kGroup; ++q) jt[q] = grp_tiles[grp * kGroup + q];
-  const int it0 = bt_off[b], it1 = bt_off[b + 1];
+  const int it0 = m.bt_off[b], it1 = m.bt_off[b + 1];
   const size_t np2 = (size_t)npad * npad;
-  (void)tile_nao;
   if (tid < kBatch) {
-    sm.px[tid] = bx[b * kBatch + tid];
-    sm.py[tid] = by[b * kBatch + tid];
-    sm.pz[tid] = bz[b * kBatch + tid];
+    sm.px[tid] = m.bx[b * kBatch + tid];
+    sm.py[tid] = m.by[b * kBatch + tid];
+    sm.pz[tid] = m.bz[b * kBatch + tid];
   }
 
   // Two register stages in flight: the loads for K step k + 2 are issued
@@ -436,7 +438,7 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(const int* __restrict__ g
   DensEpi<MODE> epi;
   if (wave < nt) {  // wave q = tile q of the group; lane = point
     epi.xq = &sm.X[kTile * wave][lane];
-    walk_tile_f32(bv, grp_tiles[grp * kGroup + wave], sm.px[lane], sm.py[lane], sm.pz[lane], epi);
+    walk_tile_f32(m.bv, grp_tiles[grp * kGroup + wave], sm.px[lane], sm.py[lane], sm.pz[lane], epi);
   }
 #pragma unroll
   for (int c = 0; c < NOUT; ++c) sm.red[wave][c][lane] = epi.o[c];
@@ -449,22 +451,14 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(const int* __restrict__ g
   }
 }
 
-void launch_density_mx(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
-                       const int* grp_ntiles, const int* bt_off, const int* bt_list, const int* tile_nao,
-                       const double* bx, const double* by, const double* bz, BasisView bv,
-                       const unsigned short* cache, int sstride, const unsigned short* Ds, int npad, double* part) {
-  if (ngroups == 0) return;
-  const int stride = (mode == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
-#define CHX_DENS_MX(M)                                                                                              \
-  hipLaunchKernelGGL(k_density_mx<M>, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_ntiles, bt_off,   \
-                     bt_list, tile_nao, bx, by, bz, bv, cache, sstride, Ds, npad, part, stride)
+void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev* mols, const int2* items) {
+  if (nitems == 0) return;
   switch (mode) {
-    case 0: CHX_DENS_MX(0); break;
-    case 1: CHX_DENS_MX(1); break;
-    case 2: CHX_DENS_MX(2); break;
-    default: CHX_DENS_MX(3); break;
+    case 0: hipLaunchKernelGGL(k_density_mx<0>, dim3(nitems), dim3(256), 0, s, mols, items); break;
+    case 1: hipLaunchKernelGGL(k_density_mx<1>, dim3(nitems), dim3(256), 0, s, mols, items); break;
+    case 2: hipLaunchKernelGGL(k_density_mx<2>, dim3(nitems), dim3(256), 0, s, mols, items); break;
+    default: hipLaunchKernelGGL(k_density_mx<3>, dim3(nitems), dim3(256), 0, s, mols, items); break;
   }
-#undef CHX_DENS_MX
   CHX_HIP(hipGetLastError());
 }
 
@@ -504,50 +498,43 @@ struct MakeB {
 };
 
 template <int NC>
-__global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(int nslots, const int* __restrict__ slot_batch,
-                                                                  const int* __restrict__ slot_tile,
-                                                                  const double* __restrict__ bx,
-                                                                  const double* __restrict__ by,
-                                                                  const double* __restrict__ bz, BasisView bv,
-                                                                  const double* __restrict__ coef,
-                                                                  u16* __restrict__ bt) {
+__global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(const MolDev* __restrict__ mols,
+                                                                  const int2* __restrict__ items) {
   constexpr int W = NC == 4 ? 1 : 4;  // waves per block
   __shared__ uint4 stage[W][NC * kMxT / 8];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
-  const int slot = blockIdx.x * W + wave;
+  const int2 wi = items ? items[blockIdx.x] : make_int2(0, blockIdx.x * W);  // first slot of the block
+  const MolDev& m = mols[wi.x];
+  const int slot = wi.y + wave, nslots = m.nslots;
+  const double* __restrict__ coef = m.coef;
   uint4* st = stage[wave];
 #pragma unroll
   for (int j = 0; j < NC * kMxT / 8 / 64; ++j) st[g + 64 * j] = uint4{0u, 0u, 0u, 0u};
   __syncthreads();
   if (slot < nslots) {
-    const int b = slot_batch[slot], t = slot_tile[slot];
+    const int b = m.slot_batch[slot], t = m.slot_tile[slot];
     const size_t p = (size_t)b * kBatch + g;
     const float ca = (float)coef[kCoef * p], cx = (float)coef[kCoef * p + 1], cy = (float)coef[kCoef * p + 2],
                 cz = (float)coef[kCoef * p + 3], ct = (float)coef[kCoef * p + 4];
     MakeB<NC> op;
     op.sv = reinterpret_cast<u16*>(st) + g;  // [c][hl][nu][g]
     op.ca = ca; op.cx = cx; op.cy = cy; op.cz = cz; op.ct = ct;
-    walk_tile_f32(bv, t, bx[p], by[p], bz[p], op);
+    walk_tile_f32(m.bv, t, m.bx[p], m.by[p], m.bz[p], op);
   }
   __syncthreads();
   if (slot < nslots) {
-    uint4* dst = reinterpret_cast<uint4*>(bt + (size_t)slot * NC * kMxT);
+    uint4* dst = reinterpret_cast<uint4*>(m.bt + (size_t)slot * NC * kMxT);
 #pragma unroll
     for (int j = 0; j < NC * kMxT / 8 / 64; ++j) dst[g + 64 * j] = st[g + 64 * j];
   }
 }
 
-void launch_make_b_mx(hipStream_t s, int ncomp, int nslots, const int* slot_batch, const int* slot_tile,
-                      const int* tile_nao, const double* bx, const double* by, const double* bz, BasisView bv,
-                      const double* coef, unsigned short* bt) {
-  (void)tile_nao;  // columns past a tile's last AO stay zero from the LDS clear
-  if (nslots == 0) return;
+void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev* mols, const int2* items) {
+  if (nitems == 0) return;
   if (ncomp == 4)
-    hipLaunchKernelGGL(k_make_b_mx<4>, dim3(nslots), dim3(64), 0, s, nslots, slot_batch, slot_tile, bx, by, bz, bv,
-                       coef, bt);
+    hipLaunchKernelGGL(k_make_b_mx<4>, dim3(nitems), dim3(64), 0, s, mols, items);
   else
-    hipLaunchKernelGGL(k_make_b_mx<1>, dim3((nslots + 3) / 4), dim3(256), 0, s, nslots, slot_batch, slot_tile, bx, by,
-                       bz, bv, coef, bt);
+    hipLaunchKernelGGL(k_make_b_mx<1>, dim3(nitems), dim3(256), 0, s, mols, items);
   CHX_HIP(hipGetLastError());
 }
 
@@ -563,13 +550,19 @@ struct VxcMxSmem {
 };
 
 template <int NC>
-__global__ __launch_bounds__(256, 2) void k_vxc_mx(const int4* __restrict__ work, const int* __restrict__ pair_batches,
-                                                const int* __restrict__ slot_of, int ntiles, const u16* __restrict__ cache,
-                                                int sstride, int toff, const u16* __restrict__ bt,
-                                                double* __restrict__ Mint, int npad) {
+__global__ __launch_bounds__(256, 2) void k_vxc_mx(const MolDev* __restrict__ mols, const int2* __restrict__ items) {
   __shared__ VxcMxSmem sm;
-  const int4 wk = work[xcd_remap(blockIdx.x, gridDim.x)];
+  const int item = xcd_remap(blockIdx.x, gridDim.x);
+  const int2 wi = items ? items[item] : make_int2(0, item);
+  const MolDev& m = mols[wi.x];
+  const int4 wk = m.cwork[wi.y];
   const int I = wk.x, J = wk.y;
+  const int* __restrict__ pair_batches = m.pair_batches;
+  const int* __restrict__ slot_of = m.slot_of;
+  const u16* __restrict__ cache = m.cache;
+  const u16* __restrict__ bt = m.bt;
+  const int ntiles = m.ntiles, sstride = m.sstride, toff = m.toff, npad = m.npad;
+  double* __restrict__ Mint = m.Mint;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lrow = lane & 15, lk = lane >> 4;
@@ -647,16 +640,12 @@ __global__ __launch_bounds__(256, 2) void k_vxc_mx(const int4* __restrict__ work
       }
 }
 
-void launch_vxc_mx(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
-                   int ntiles, const unsigned short* cache, int sstride, int toff, const unsigned short* bt,
-                   double* Mint, int npad) {
-  if (nwork == 0) return;
+void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev* mols, const int2* items) {
+  if (nitems == 0) return;
   if (ncomp == 4)
-    hipLaunchKernelGGL(k_vxc_mx<4>, dim3(nwork), dim3(256), 0, s, work, pair_batches, slot_of, ntiles, cache, sstride,
-                       toff, bt, Mint, npad);
+    hipLaunchKernelGGL(k_vxc_mx<4>, dim3(nitems), dim3(256), 0, s, mols, items);
   else
-    hipLaunchKernelGGL(k_vxc_mx<1>, dim3(nwork), dim3(256), 0, s, work, pair_batches, slot_of, ntiles, cache, sstride,
-                       toff, bt, Mint, npad);
+    hipLaunchKernelGGL(k_vxc_mx<1>, dim3(nitems), dim3(256), 0, s, mols, items);
   CHX_HIP(hipGetLastError());
 }
 

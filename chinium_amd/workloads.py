@@ -123,6 +123,19 @@ def alkane_angstrom(nc, seed):
     return np.array(Z), np.array(X)
 
 
+def alkane_set(count=64, seed=2, nmax=40):
+    """The many-molecule workload: `count` linear alkanes C_nH_{2n+2} with n
+    drawn seeded-uniform in 1..nmax. Returns [(n, Z, xyz_angstrom)]; molecule k
+    takes jitter seed 1000 * seed + k. Closed shell with 4n + 1 occupied orbitals."""
+    rng = SplitMix64(seed)
+    out = []
+    for k in range(count):
+        n = min(nmax, 1 + int(rng.uniform() * nmax))
+        Z, X = alkane_angstrom(n, 1000 * seed + k)
+        out.append((n, Z, X))
+    return out
+
+
 def cyclocarbon_angstrom(n=18, b1=1.23, b2=1.35):
     # chord lengths b = 2 R sin(theta/2); alternate angles theta1, theta2 with
     # n/2 (theta1 + theta2) = 2 pi; solve for R by bisection.

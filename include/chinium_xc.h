@@ -158,6 +158,28 @@ int chx_lebedev_rule(int npoints, double* xyzw);
 int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
                        double* exc, double* vrho, double* vsigma, double* vtau);
 
+/* ---- Packs: many molecules per launch (no reference counterpart) ------------
+ * Chinium runs one molecule per process (SelfConsistentField.cpp:27-33). A
+ * pack takes grids that are each built, set to CHX_PREC_MIXED, typed and
+ * planned (chx_grid_get_ao) on their own, and runs one XC build of all of them
+ * with one kernel launch per stage; the sequence is captured as a hipGraph and
+ * replayed while the D / V pointers and the functional stay the same. The
+ * grids must outlive the pack and must not be used on their own meanwhile.
+ * D_dev[m] / V_dev[m]: device arrays, column-major nbasis_m^2, reference
+ * order. E, N (host, may be NULL): E_xc and N_e per molecule. */
+typedef struct chx_pack chx_pack;
+int chx_pack_create(chx_grid* const* grids, int n, chx_pack** out);
+int chx_pack_xc(chx_pack* p, const chx_xc* xc, const double* const* D_dev, double* const* V_dev, double* E,
+                double* N);
+/* 1 (default) replays a captured hipGraph, 0 launches eagerly. */
+int chx_pack_set_graph(chx_pack* p, int enable);
+/* Autotuner: times `reps` builds for each Vxc work-item length (16, 32, 64,
+ * 128 batches), keeps the fastest; chunk_out gets it, ms_out[4] the times. */
+int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev, double* const* V_dev, int reps,
+                      int* chunk_out, float* ms_out);
+int chx_pack_stream(const chx_pack* p, void** stream);
+void chx_pack_destroy(chx_pack* p);
+
 /* ---- Diagnostics (no reference counterpart) ---------------------------------
  * Runs one FP64 MFMA tile on the device and returns the largest deviation from
  * the scalar product, plus the number of 2^20 sample floats whose hardware

@@ -1,0 +1,116 @@
+"""Packs (chx_pack_*): many molecules' XC builds in one launch per stage,
+replayed as a hipGraph, against the same grids built one at a time and
+against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import chinium_amd as ca
+import oracle_lib as ol
+from chinium_amd import hip
+from chinium_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def alkanes():
+    mols = []
+    for k, n in enumerate((1, 3, 2, 6)):
+        Z, X = wl.alkane_angstrom(n, 70 + k)
+        xyz = X * ca.ANGSTROM_TO_BOHR
+        mol = ca.Molecule(Z, xyz, "def2-svp")
+        D, S = wl.workload_density(mol, 4 * n + 1, 30 + k)
+        grid = ca.Grid(mol, "SG-1", 1, 0)
+        grid.setPrecision(ca.CHX_PREC_MIXED)
+        grid.setType(1)
+        grid.getAO(0, 0)
+        mols.append(dict(n=n, Z=Z, xyz=xyz, mol=mol, D=D, grid=grid))
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    # one-at-a-time builds first (the grids are used on their own before packing)
+    for m in mols:
+        m["E1"], m["V1"] = ca.xc_build(m["grid"], xc, m["D"])
+        m["N1"] = m["grid"].getNumElectrons()
+    dD, dV = [], []
+    for m in mols:
+        nb = m["mol"].nbasis
+        a = hip.DeviceArray(nb * nb)
+        a.upload(np.asfortranarray(m["D"]).ravel(order="F"))
+        dD.append(a)
+        dV.append(hip.DeviceArray(nb * nb))
+    _say("pack of %d alkanes (n = %s)" % (len(mols), [m["n"] for m in mols]))
+    pack = ca.Pack([m["grid"] for m in mols])
+    return dict(mols=mols, xc=xc, pack=pack, dD=dD, dV=dV)
+
+
+def _run(a):
+    E, N = a["pack"].xc(a["xc"], [d.ptr for d in a["dD"]], [v.ptr for v in a["dV"]])
+    Vs = [v.download().reshape((m["mol"].nbasis,) * 2, order="F") for v, m in zip(a["dV"], a["mols"])]
+    return E, N, Vs
+
+
+def test_pack_matches_single_grids(alkanes):
+    E, N, Vs = _run(alkanes)
+    for k, m in enumerate(alkanes["mols"]):
+        dE, dV = abs(E[k] - m["E1"]), np.abs(Vs[k] - m["V1"]).max()
+        _say("  C%d: E pack %.12f single %.12f dE %.1e dV %.1e" % (m["n"], E[k], m["E1"], dE, dV))
+        # same kernels and partial sums; V differs only by the order of FP64 atomics
+        assert dE <= 1e-12 * abs(m["E1"])
+        assert abs(N[k] - m["N1"]) <= 1e-12 * m["N1"]
+        assert dV <= 1e-12
+
+
+def test_pack_graph_replays(alkanes):
+    """Replays of the captured graph, with eager work between them, agree with eager launches."""
+    pack = alkanes["pack"]
+    pack.set_graph(False)
+    E0, N0, V0 = _run(alkanes)
+    pack.set_graph(True)
+    for rep in range(3):
+        E, N, Vs = _run(alkanes)
+        m = alkanes["mols"][rep % len(alkanes["mols"])]
+        ca.xc_build(m["grid"], alkanes["xc"], m["D"])  # eager work on a grid stream between replays
+        assert np.all(np.abs(E - E0) <= 1e-12 * np.abs(E0))
+        for a, b in zip(Vs, V0):
+            assert np.abs(a - b).max() <= 1e-12
+
+
+def test_pack_against_oracle(alkanes):
+    E, N, Vs = _run(alkanes)
+    m = alkanes["mols"][3]
+    orc = ol.OracleSystem(m["Z"], m["xyz"], "SG-1", "def2-svp")
+    orc.set_functional("PBE")
+    orc.get_ao()
+    Eo, No, Vo = orc.xc(m["D"])
+    dE = abs(E[3] - Eo) / len(m["Z"])
+    _say("  C%d pack vs oracle: dE/atom %.2e  max dV %.2e" % (m["n"], dE, np.abs(Vs[3] - Vo).max()))
+    assert dE <= 1e-5
+    assert np.abs(Vs[3] - Vo).max() <= 5e-5
+
+
+def test_pack_autotune(alkanes):
+    xc, pack = alkanes["xc"], alkanes["pack"]
+    E0, _, V0 = _run(alkanes)
+    chunk, ms = pack.autotune(xc, [d.ptr for d in alkanes["dD"]], [v.ptr for v in alkanes["dV"]], reps=2)
+    _say("  autotune: chunk %d, ms %s" % (chunk, {k: round(v, 4) for k, v in ms.items()}))
+    assert chunk in (16, 32, 64, 128) and all(v > 0 for v in ms.values())
+    E, _, Vs = _run(alkanes)
+    # the density side does not depend on the Vxc item length; M is summed in
+    # FP32 within an item, so another length changes V at the FP32 level only
+    assert np.all(np.abs(E - E0) <= 1e-12 * np.abs(E0))
+    for a, b in zip(Vs, V0):
+        assert np.abs(a - b).max() <= 2e-6
+
+
+def test_pack_rejects_fp64_grid(alkanes):
+    m = alkanes["mols"][0]
+    g = ca.Grid(m["mol"], "SG-1", 1, 0)
+    g.setType(1)
+    g.getAO(0, 0)
+    with pytest.raises(ca.ChxError) as e:
+        ca.Pack([g])
+    assert e.value.code == -5
