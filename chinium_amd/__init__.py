@@ -89,6 +89,8 @@ def lib():
         "chx_selftest": (i, [P(d)]),
         "chx_lebedev_rule": (i, [i, _vp]),
         "chx_xc_eval_points": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "chx_grid_get_density_skeleton": (i, [_vp, _vp, i, i]),
+        "chx_grid_energy_grad": (i, [_vp, _vp]),
         "chx_pack_create": (i, [_vp, i, P(_vp)]),
         "chx_pack_xc": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "chx_pack_set_graph": (i, [_vp, i]),
@@ -196,12 +198,14 @@ class Grid:
         _check(lib().chx_grid_num_points(self.h, ctypes.byref(n)))
         self.npoints = n.value
         self.nbasis = mol.nbasis if mol is not None else None
+        self.natoms = len(mol.Z) if mol is not None else None
 
     def copy(self):
         h = _vp()
         _check(lib().chx_grid_copy(self.h, ctypes.byref(h)))
         g = Grid(_handle=h)
         g.nbasis = self.nbasis
+        g.natoms = self.natoms
         return g
 
     def __del__(self):
@@ -236,6 +240,19 @@ class Grid:
         n = Ds[0].shape[0]
         buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for D in Ds])
         _check(lib().chx_grid_get_density(self.h, _ptr(buf), len(Ds), n))
+
+    def getDensitySkeleton(self, Ds):
+        """SubGrid::getDensitySkeleton (GridDensity.cpp:79-130), closed shell."""
+        Ds = list(Ds)
+        n = Ds[0].shape[0]
+        buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for D in Ds])
+        _check(lib().chx_grid_get_density_skeleton(self.h, _ptr(buf), len(Ds), n))
+
+    def getEnergyGrad(self):
+        """Grid::getEnergyGrad: XC nuclear gradient (natoms x 3), grid held fixed."""
+        G = np.zeros(3 * self.natoms)
+        _check(lib().chx_grid_energy_grad(self.h, _ptr(G)))
+        return G.reshape(-1, 3)
 
     def getDensityDevice(self, ptr, nspin, nbasis):
         _check(lib().chx_grid_get_density_device(self.h, ctypes.c_void_p(ptr), int(nspin), int(nbasis)))

@@ -301,6 +301,7 @@ int chx_grid_copy(const chx_grid* s, chx_grid** out) {
     CHX_HIP(hipStreamSynchronize(s->stream));
     g->d_sh_xyz.copy_from(s->d_sh_xyz, st); g->d_p_exp.copy_from(s->d_p_exp, st); g->d_p_coef.copy_from(s->d_p_coef, st);
     g->d_segrec.copy_from(s->d_segrec, st); g->max_nprim = s->max_nprim;
+    g->sh_atom = s->sh_atom; g->d_sh_atom.copy_from(s->d_sh_atom, st);
     g->d_sh_type.copy_from(s->d_sh_type, st); g->d_sh_nprim.copy_from(s->d_sh_nprim, st);
     g->d_sh_poff.copy_from(s->d_sh_poff, st); g->d_tile_seg_off.copy_from(s->d_tile_seg_off, st);
     g->d_seg.copy_from(s->d_seg, st); g->d_int2ref.copy_from(s->d_int2ref, st); g->d_ref2int.copy_from(s->d_ref2int, st);
@@ -359,7 +360,11 @@ int chx_grid_set_type(chx_grid* g, int type) {
 
 int chx_grid_get_ao(chx_grid* g, int derivative, int output) {
   return guard([&] {
-    require(derivative == 0, CHX_ERR_UNSUPPORTED, "AO derivatives beyond the SCF order are not built");
+    require(derivative == 0 || derivative == 1, CHX_ERR_UNSUPPORTED, "AO derivatives above first order are not built");
+    // derivative 1 (before the XC gradient, Restricted/Grad.cpp:73): the tables
+    // of a planned grid already serve the gradient, and the density and
+    // potentials of the last iteration must survive, as in the reference.
+    if (derivative == 1 && g->ao_ready) return;
     grid_plan(g);
     if (output)
       std::printf("AO screening: %.2f of %d tiles alive per batch, %.3e FLOP per XC build\n", g->mean_tiles, g->ntiles,
@@ -373,6 +378,51 @@ int chx_grid_get_density(chx_grid* g, const double* D, int nspin, int nbasis) {
     require(D != nullptr, CHX_ERR_ARG, "null density matrix");
     CHX_HIP(hipMemcpyAsync(g->d_Din.p, D, sizeof(double) * (size_t)nbasis * nbasis, hipMemcpyHostToDevice, g->stream));
     density_from_device(g, g->d_Din.p);
+  });
+}
+
+int chx_grid_get_density_skeleton(chx_grid* g, const double* D, int nspin, int nbasis) {
+  return guard([&] {
+    check_dm_args(g, nspin, nbasis);
+    require(D != nullptr, CHX_ERR_ARG, "null density matrix");
+    require(g->precision == CHX_PREC_FP64 && g->cached, CHX_ERR_UNSUPPORTED,
+            "the XC gradient is built for the FP64 form with the AO tables in HBM");
+    CHX_HIP(hipMemcpyAsync(g->d_Din.p, D, sizeof(double) * (size_t)nbasis * nbasis, hipMemcpyHostToDevice, g->stream));
+    launch_permute_d(g->stream, g->nbasis, g->npad, g->d_Din.p, g->d_int2ref.p, g->d_Dskel.alloc_get((size_t)g->npad * g->npad));
+    g->have_skeleton = true;
+  });
+}
+
+int chx_grid_energy_grad(chx_grid* g, double* grad) {
+  return guard([&] {
+    require(grad != nullptr, CHX_ERR_ARG, "null output");
+    require(g->have_skeleton, CHX_ERR_STATE, "Grid::getDensitySkeleton must precede Grid::getEnergyGrad");
+    require(g->have_v, CHX_ERR_STATE, "ExchangeCorrelation::Evaluate(\"ev\") must precede Grid::getEnergyGrad");
+    require(g->v_family <= 1 && g->type <= 1, CHX_ERR_UNSUPPORTED, "the XC gradient is built for LDA and GGA");
+    const int na = g->mol.natoms, ng = (int)g->grp_batch.size(), ns = (int)g->bt_list.size();
+    if (g->slot_grp.size() != (size_t)ns) {  // slot -> (density group, column tile in the group)
+      g->slot_grp.assign(ns, 0);
+      g->slot_q.assign(ns, 0);
+      for (int k = 0; k < ng; ++k)
+        for (int q = 0; q < g->grp_ntiles[k]; ++q) {
+          const int sl = g->grp_slots[(size_t)k * kGroup + q];
+          g->slot_grp[sl] = k;
+          g->slot_q[sl] = q;
+        }
+      g->d_slot_grp.upload(g->slot_grp, g->stream);
+      g->d_slot_q.upload(g->slot_q, g->stream);
+    }
+    launch_make_b(g->stream, 1, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
+    DevBuf<double> xz, dg;
+    xz.alloc(std::max<size_t>(1, (size_t)ng * 2 * kGroup * kTile * kBatch));
+    dg.alloc(3 * (size_t)na);
+    launch_fill(g->stream, dg.p, 3 * (size_t)na, 0.0);
+    launch_xgrad_xz(g->stream, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p, g->d_bt_list.p,
+                    g->d_cache.p, g->d_bmat.p, g->d_Dskel.p, g->npad, xz.p);
+    launch_xgrad_epi(g->stream, ns, g->d_slot_batch.p, g->d_slot_tile.p, g->d_slot_grp.p, g->d_slot_q.p, g->d_x.p,
+                     g->d_y.p, g->d_z.p, g->view(), g->d_coef.p, xz.p, dg.p);
+    CHX_HIP(hipMemcpyAsync(grad, dg.p, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g->stream));
+    CHX_HIP(hipStreamSynchronize(g->stream));
   });
 }
 

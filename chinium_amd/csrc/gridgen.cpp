@@ -253,7 +253,7 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
   g->ntiles = g->npad / kTile;
   g->int2ref.assign(g->npad, -1);
   g->ref2int.assign(g->nbasis, -1);
-  g->sh_xyz.clear(); g->sh_type.clear(); g->sh_nprim.clear(); g->sh_poff.clear();
+  g->sh_xyz.clear(); g->sh_type.clear(); g->sh_nprim.clear(); g->sh_poff.clear(); g->sh_atom.clear();
   g->p_exp.clear(); g->p_coef.clear();
   std::vector<int> sh_int_ao;  // internal AO offset per internal shell
   int ao = 0;
@@ -267,6 +267,7 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
       }
       g->sh_xyz.push_back(m.xyz[3 * a]); g->sh_xyz.push_back(m.xyz[3 * a + 1]); g->sh_xyz.push_back(m.xyz[3 * a + 2]);
       g->sh_type.push_back(s.type);
+      g->sh_atom.push_back(a);
       g->sh_nprim.push_back(s.nprim);
       g->sh_poff.push_back((int)g->p_exp.size());
       for (int p = 0; p < s.nprim; ++p) {
@@ -293,6 +294,7 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
   // ---- device copies
   hipStream_t st = g->stream;
   g->d_sh_xyz.upload(g->sh_xyz, st); g->d_sh_type.upload(g->sh_type, st); g->d_sh_nprim.upload(g->sh_nprim, st);
+  g->d_sh_atom.upload(g->sh_atom, st);
   g->d_sh_poff.upload(g->sh_poff, st); g->d_p_exp.upload(g->p_exp, st); g->d_p_coef.upload(g->p_coef, st);
   std::vector<SegRec> rec(g->seg.size());  // lives until the synchronisation below
   g->max_nprim = 0;
@@ -548,7 +550,9 @@ void grid_plan(chx_grid* g) {
   }
   CHX_HIP(hipStreamSynchronize(st));
   g->ao_ready = true;
-  g->have_density = g->have_eval = g->have_v = false;
+  g->have_density = g->have_eval = g->have_v = g->have_skeleton = false;
+  g->slot_grp.clear();
+  g->slot_q.clear();
 }
 
 }  // namespace chx
@@ -563,6 +567,7 @@ int chx_lebedev_rule_impl(int n, double* xyzw) {
 chx::BasisView chx_grid::view() const {
   chx::BasisView v;
   v.sh_xyz = d_sh_xyz.p;
+  v.sh_atom = d_sh_atom.p;
   v.sh_type = d_sh_type.p;
   v.sh_nprim = d_sh_nprim.p;
   v.sh_poff = d_sh_poff.p;

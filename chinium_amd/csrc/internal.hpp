@@ -60,6 +60,10 @@ template <class T> struct DevBuf {
     CHX_HIP(hipMalloc(&p, count * sizeof(T)));
     n = count;
   }
+  T* alloc_get(size_t count) {
+    alloc(count);
+    return p;
+  }
   void upload(const std::vector<T>& h, hipStream_t s) {
     alloc(h.size());
     if (!h.empty()) CHX_HIP(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
@@ -92,6 +96,7 @@ struct BasisView {
   const int* tile_seg_off;  // ntiles + 1
   const int4* seg;          // (shell, first component, count, local AO offset in tile)
   const SegRec* segrec;     // per segment, same index as seg (mixed-precision form)
+  const int* sh_atom;       // atom of each shell (XC gradient)
 };
 
 // Launchers (kernels.hip).
@@ -146,6 +151,14 @@ struct MolDev {
   double* Mint;                 // M accumulator, npad^2
 };
 
+// XC nuclear gradient (kernels_grad.hip), FP64 cached form.
+void launch_xgrad_xz(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
+                     const int* bt_off, const int* bt_list, const double* cache, const double* bmat,
+                     const double* Dint, int npad, double* xz);
+void launch_xgrad_epi(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* slot_grp,
+                      const int* slot_q, const double* bx, const double* by, const double* bz, BasisView bv,
+                      const double* coef, const double* xz, double* grad);
+
 // Mixed-precision form (kernels_mixed.hip): MFMA operands cached as bf16
 // hi/lo pairs, split-BF16 MFMA contractions with FP32 accumulation, FP32
 // recomputation of the AO values outside the MFMA operands.
@@ -195,7 +208,7 @@ struct chx_grid {
   std::vector<int> int2ref;  // npad, -1 for padding
   std::vector<int> ref2int;  // nbasis
   std::vector<double> sh_xyz;
-  std::vector<int> sh_type, sh_nprim, sh_poff;
+  std::vector<int> sh_type, sh_nprim, sh_poff, sh_atom;
   std::vector<double> p_exp, p_coef;
   std::vector<int> tile_seg_off;
   std::vector<int4> seg;
@@ -234,7 +247,7 @@ struct chx_grid {
   chx::MolDev hmol{};                 // descriptor of this grid (mixed form)
   chx::DevBuf<chx::MolDev> d_mol;
   int mx_sstride = 0, mx_nT = 0, bmx_comps = 0;
-  chx::DevBuf<int> d_sh_type, d_sh_nprim, d_sh_poff, d_tile_seg_off;
+  chx::DevBuf<int> d_sh_type, d_sh_nprim, d_sh_poff, d_tile_seg_off, d_sh_atom;
   chx::DevBuf<int4> d_seg;
   chx::DevBuf<int> d_int2ref, d_ref2int;
   chx::DevBuf<double> d_x, d_y, d_z, d_w;
@@ -250,6 +263,11 @@ struct chx_grid {
   int bmat_comps = 0;  // components per slot held in d_bmat (1 GGA, 4 meta-GGA)
   int v_family = 0;    // family of the potential on the grid (set by Evaluate / set_vrho)
   int en_blocks = 0;
+  // XC gradient (Grid::getDensitySkeleton / getEnergyGrad)
+  chx::DevBuf<double> d_Dskel;
+  std::vector<int> slot_grp, slot_q;
+  chx::DevBuf<int> d_slot_grp, d_slot_q;
+  bool have_skeleton = false;
   bool have_density = false, have_eval = false, have_v = false;
 
   // Optional per-kernel HIP-event timing (stage ids in chx_grid_kernel_times).

@@ -1,0 +1,251 @@
+// XC nuclear gradient, FP64, skeleton form with the grid held fixed
+// (src/HartreeFockKohnSham/Restricted/Grad.cpp:72-84 -> SubGrid::getDensitySkeleton,
+// src/Grid/GridDensity.cpp:79-130 -> SubGrid::getEnergyGrad, src/Grid/GridEnergy.cpp:15-30).
+//
+// With a = w v_rho / 2, c = 2 w v_sigma grad rho and B = a phi + c . grad phi
+// (the Vxc integrand of the same point), the reference's
+//   E[t,A] = sum_g w (v_rho dRho/dA_t + v_sigma dSigma/dA_t)
+// reduces to
+//   E[t,A] = -2 sum_{mu in A} sum_g [ d_t phi_mu (Z_mu + a X_mu) + (c . grad) d_t phi_mu X_mu ],
+//   X_mu = sum_nu D_mu,nu phi_nu,   Z_mu = sum_nu D_mu,nu B_nu,
+// two contractions of the density's shape (k_xgrad_xz, FP64 MFMA over the
+// cached phi and B tables) and a per-slot epilogue with the AO Hessian
+// (k_xgrad_epi, one wave per slot, lane = point).
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace chx {
+
+constexpr int kGSlotAO = 4 * kBatch * kTile;  // FP64 cache: doubles per slot
+constexpr int kGSlotB = kBatch * kTile;
+constexpr int kGCols = kGroup * kTile;        // 128 columns per group
+
+// ---------------------------------------------------------------------------
+// X and Z of one density work group (batch, <= 4 column tiles), written as
+// xz[grp][0 = X, 1 = Z][column][g]. 4 waves, wave w owns rows 16w..16w+15.
+struct XgradSmem {
+  double Ap[kBatch][kTile + 1];  // phi of the K tile, [g][nu]
+  double Ab[kBatch][kTile + 1];  // B of the K tile
+  double Bd[kTile][kGCols];      // D block [nu][mu]
+};
+
+__global__ __launch_bounds__(256) void k_xgrad_xz(const int* __restrict__ grp_batch, const int* __restrict__ grp_tiles,
+                                                  const int* __restrict__ grp_ntiles, const int* __restrict__ bt_off,
+                                                  const int* __restrict__ bt_list, const double* __restrict__ cache,
+                                                  const double* __restrict__ bmat, const double* __restrict__ Dint,
+                                                  int npad, double* __restrict__ xz) {
+  __shared__ XgradSmem sm;
+  const int grp = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int b = grp_batch[grp];
+  const int nt = grp_ntiles[grp];
+  int jt[kGroup];
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) jt[q] = grp_tiles[grp * kGroup + q];
+  dbl4 ax[8], az[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) ax[c] = az[c] = dbl4{0, 0, 0, 0};
+  for (int it = bt_off[b]; it < bt_off[b + 1]; ++it) {
+    const int ti = bt_list[it];
+    const double* p = cache + (size_t)it * kGSlotAO;  // comp 0 = phi, [g][nu]
+    const double* q = bmat + (size_t)it * kGSlotB;    // B, [g][nu]
+    for (int e = tid; e < kBatch * kTile; e += 256) {
+      sm.Ap[e / kTile][e % kTile] = p[e];
+      sm.Ab[e / kTile][e % kTile] = q[e];
+    }
+    for (int e = tid; e < kTile * kGCols; e += 256) {
+      const int r = e / kGCols, c = e % kGCols, qq = c / kTile;
+      sm.Bd[r][c] = qq < nt ? Dint[(size_t)(ti * kTile + r) * npad + jt[qq] * kTile + (c % kTile)] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kTile / 4; ++ks) {
+      const double a1 = sm.Ap[16 * wave + lrow][4 * ks + lk];
+      const double a2 = sm.Ab[16 * wave + lrow][4 * ks + lk];
+#pragma unroll
+      for (int cb = 0; cb < 8; ++cb) {
+        const double bb = sm.Bd[4 * ks + lk][16 * cb + lrow];
+        ax[cb] = mfma_f64(a1, bb, ax[cb]);
+        az[cb] = mfma_f64(a2, bb, az[cb]);
+      }
+    }
+    __syncthreads();
+  }
+  // result register r of lane l: row g = 16 wave + 4 r + l / 16, column 16 cb + l % 16
+  double* out = xz + (size_t)grp * 2 * kGCols * kBatch;
+#pragma unroll
+  for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int g = 16 * wave + 4 * r + lk, col = 16 * cb + lrow;
+      out[(size_t)col * kBatch + g] = ax[cb][r];
+      out[(size_t)(kGCols + col) * kBatch + g] = az[cb][r];
+    }
+}
+
+void launch_xgrad_xz(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
+                     const int* bt_off, const int* bt_list, const double* cache, const double* bmat,
+                     const double* Dint, int npad, double* xz) {
+  if (ngroups == 0) return;
+  hipLaunchKernelGGL(k_xgrad_xz, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_ntiles, bt_off, bt_list,
+                     cache, bmat, Dint, npad, xz);
+  CHX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Angular parts with first and second derivatives, one shell type per
+// instantiation (conventions of src/Grid/AO/{S,CartP,PureP,PureD,PureF}.hpp).
+// Hessian order (xx, xy, xz, yy, yz, zz).
+template <int TYPE>
+__device__ __forceinline__ void angular_h(double x, double y, double z, double* q, double (*g)[3], double (*h)[6]) {
+  const double r3 = 1.7320508075688772, r6 = 2.449489742783178, r10 = 3.1622776601683795, r15 = 3.872983346207417;
+  auto set = [&](int m, double v, double gx, double gy, double gz, double hxx, double hxy, double hxz, double hyy,
+                 double hyz, double hzz) {
+    q[m] = v;
+    g[m][0] = gx; g[m][1] = gy; g[m][2] = gz;
+    h[m][0] = hxx; h[m][1] = hxy; h[m][2] = hxz; h[m][3] = hyy; h[m][4] = hyz; h[m][5] = hzz;
+  };
+  if (TYPE == 0) {
+    set(0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0);
+  } else if (TYPE == 1) {  // x, y, z
+    set(0, x, 1, 0, 0, 0, 0, 0, 0, 0, 0);
+    set(1, y, 0, 1, 0, 0, 0, 0, 0, 0, 0);
+    set(2, z, 0, 0, 1, 0, 0, 0, 0, 0, 0);
+  } else if (TYPE == -1) {  // y, z, x
+    set(0, y, 0, 1, 0, 0, 0, 0, 0, 0, 0);
+    set(1, z, 0, 0, 1, 0, 0, 0, 0, 0, 0);
+    set(2, x, 1, 0, 0, 0, 0, 0, 0, 0, 0);
+  } else if (TYPE == -2) {
+    set(0, r3 * x * y, r3 * y, r3 * x, 0, 0, r3, 0, 0, 0, 0);
+    set(1, r3 * y * z, 0, r3 * z, r3 * y, 0, 0, 0, 0, r3, 0);
+    set(2, z * z - 0.5 * (x * x + y * y), -x, -y, 2 * z, -1, 0, 0, -1, 0, 2);
+    set(3, r3 * x * z, r3 * z, 0, r3 * x, 0, 0, r3, 0, 0, 0);
+    set(4, 0.5 * r3 * (x * x - y * y), r3 * x, -r3 * y, 0, r3, 0, 0, -r3, 0, 0);
+  } else {  // pure f
+    const double xx = x * x, yy = y * y, zz = z * z;
+    const double c10 = 0.25 * r10, c6 = 0.25 * r6;
+    // y (3x^2 - y^2)
+    set(0, c10 * y * (3 * xx - yy), c10 * 6 * x * y, c10 * 3 * (xx - yy), 0,
+        c10 * 6 * y, c10 * 6 * x, 0, -c10 * 6 * y, 0, 0);
+    // xyz
+    set(1, r15 * x * y * z, r15 * y * z, r15 * x * z, r15 * x * y, 0, r15 * z, r15 * y, 0, r15 * x, 0);
+    // y (4z^2 - x^2 - y^2)
+    set(2, c6 * y * (4 * zz - xx - yy), -c6 * 2 * x * y, c6 * (4 * zz - xx - 3 * yy), c6 * 8 * y * z,
+        -c6 * 2 * y, -c6 * 2 * x, 0, -c6 * 6 * y, c6 * 8 * z, c6 * 8 * y);
+    // z (z^2 - 3/2 (x^2 + y^2))
+    set(3, z * (zz - 1.5 * (xx + yy)), -3 * x * z, -3 * y * z, 3 * zz - 1.5 * (xx + yy),
+        -3 * z, 0, -3 * x, -3 * z, -3 * y, 6 * z);
+    // x (4z^2 - x^2 - y^2)
+    set(4, c6 * x * (4 * zz - xx - yy), c6 * (4 * zz - 3 * xx - yy), -c6 * 2 * x * y, c6 * 8 * x * z,
+        -c6 * 6 * x, -c6 * 2 * y, c6 * 8 * z, -c6 * 2 * x, 0, c6 * 8 * x);
+    // (x^2 - y^2) z
+    set(5, 0.5 * r15 * (xx - yy) * z, r15 * x * z, -r15 * y * z, 0.5 * r15 * (xx - yy),
+        r15 * z, 0, r15 * x, -r15 * z, -r15 * y, 0);
+    // x (x^2 - 3y^2)
+    set(6, c10 * x * (xx - 3 * yy), c10 * 3 * (xx - yy), -c10 * 6 * x * y, 0,
+        c10 * 6 * x, -c10 * 6 * y, 0, -c10 * 6 * x, 0, 0);
+  }
+}
+
+// Gradient contribution of components first..first+cnt-1 of one shell at this
+// lane's point: s_t += d_t phi (Z + a X) + (c . grad) d_t phi X.
+template <int TYPE>
+__device__ __forceinline__ void xgrad_shell(double x, double y, double z, double A0, const double* dA, const double* HA,
+                                            int first, int cnt, const double* xcol, const double* zcol, double a,
+                                            const double* c, double* st) {
+  constexpr int n = TYPE == 0 ? 1 : TYPE == -2 ? 5 : TYPE == -3 ? 7 : 3;
+  double q[n], g[n][3], h[n][6];
+  angular_h<TYPE>(x, y, z, q, g, h);
+  const int hi[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+  for (int m = 0; m < n; ++m) {
+    const int k = m - first;
+    if (k < 0 || k >= cnt) continue;
+    const double X = xcol[k * kBatch], Zv = zcol[k * kBatch];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const double dt = dA[t] * q[m] + A0 * g[m][t];  // d_t phi
+      double cd = 0;                                  // (c . grad) d_t phi
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        cd += c[r] * (HA[hi[r][t]] * q[m] + dA[r] * g[m][t] + dA[t] * g[m][r] + A0 * h[m][hi[r][t]]);
+      st[t] += dt * (Zv + a * X) + cd * X;
+    }
+  }
+}
+
+// One wave per slot (batch, tile), lane = point; per shell segment the three
+// sums are reduced over the wave and added to grad[atom][t] (scaled by -2).
+__global__ __launch_bounds__(256) void k_xgrad_epi(int nslots, const int* __restrict__ slot_batch,
+                                                   const int* __restrict__ slot_tile, const int* __restrict__ slot_grp,
+                                                   const int* __restrict__ slot_q, const double* __restrict__ bx,
+                                                   const double* __restrict__ by, const double* __restrict__ bz,
+                                                   BasisView bv, const double* __restrict__ coef,
+                                                   const double* __restrict__ xz, double* __restrict__ grad) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + wave;
+  if (slot >= nslots) return;
+  const int b = slot_batch[slot], t = slot_tile[slot];
+  const size_t p = (size_t)b * kBatch + g;
+  const double px = bx[p], py = by[p], pz = bz[p];
+  const double a = coef[kCoef * p];
+  const double c[3] = {coef[kCoef * p + 1], coef[kCoef * p + 2], coef[kCoef * p + 3]};
+  const double* X = xz + (size_t)slot_grp[slot] * 2 * kGCols * kBatch + (size_t)slot_q[slot] * kTile * kBatch + g;
+  const double* Z = X + (size_t)kGCols * kBatch;
+  const int s0 = bv.tile_seg_off[t], s1 = bv.tile_seg_off[t + 1];
+  for (int k = s0; k < s1; ++k) {
+    const int4 sg = bv.seg[k];
+    const int sh = sg.x;
+    const double x = px - bv.sh_xyz[3 * sh], y = py - bv.sh_xyz[3 * sh + 1], z = pz - bv.sh_xyz[3 * sh + 2];
+    const double r2 = x * x + y * y + z * z;
+    double A0 = 0, A1 = 0, A2 = 0;
+    for (int i = 0, po = bv.sh_poff[sh]; i < bv.sh_nprim[sh]; ++i) {
+      const double e = bv.p_exp[po + i];
+      const double v = bv.p_coef[po + i] * ::exp(-e * r2);
+      A0 += v;
+      A1 += e * v;
+      A2 += e * e * v;
+    }
+    const double d[3] = {x, y, z};
+    double dA[3], HA[6];
+    for (int r = 0; r < 3; ++r) dA[r] = -2 * d[r] * A1;
+    HA[0] = 4 * x * x * A2 - 2 * A1; HA[1] = 4 * x * y * A2; HA[2] = 4 * x * z * A2;
+    HA[3] = 4 * y * y * A2 - 2 * A1; HA[4] = 4 * y * z * A2; HA[5] = 4 * z * z * A2 - 2 * A1;
+    double st[3] = {0, 0, 0};
+    const double* xc = X + (size_t)sg.w * kBatch;
+    const double* zc = Z + (size_t)sg.w * kBatch;
+    switch (bv.sh_type[sh]) {
+      case 0: xgrad_shell<0>(x, y, z, A0, dA, HA, sg.y, sg.z, xc, zc, a, c, st); break;
+      case 1: xgrad_shell<1>(x, y, z, A0, dA, HA, sg.y, sg.z, xc, zc, a, c, st); break;
+      case -1: xgrad_shell<-1>(x, y, z, A0, dA, HA, sg.y, sg.z, xc, zc, a, c, st); break;
+      case -2: xgrad_shell<-2>(x, y, z, A0, dA, HA, sg.y, sg.z, xc, zc, a, c, st); break;
+      default: xgrad_shell<-3>(x, y, z, A0, dA, HA, sg.y, sg.z, xc, zc, a, c, st); break;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double v = st[r];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      st[r] = v;
+    }
+    if (g == 0) {
+      const int atom = bv.sh_atom[sh];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) unsafeAtomicAdd(&grad[3 * atom + r], -2.0 * st[r]);
+    }
+  }
+}
+
+void launch_xgrad_epi(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* slot_grp,
+                      const int* slot_q, const double* bx, const double* by, const double* bz, BasisView bv,
+                      const double* coef, const double* xz, double* grad) {
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_xgrad_epi, dim3((nslots + 3) / 4), dim3(256), 0, s, nslots, slot_batch, slot_tile, slot_grp,
+                     slot_q, bx, by, bz, bv, coef, xz, grad);
+  CHX_HIP(hipGetLastError());
+}
+
+}  // namespace chx

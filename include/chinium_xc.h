@@ -104,6 +104,17 @@ int chx_grid_energy(chx_grid* g, double* E);
 int chx_grid_fock(chx_grid* g, double* F, int nspin, int nbasis);
 /* Same, into device memory. */
 int chx_grid_fock_device(chx_grid* g, double* F_dev, int nspin, int nbasis);
+/* XC nuclear gradient in the reference's skeleton form, the grid held fixed:
+ * chx_grid_get_ao(g, 1, .) (a no-op on a planned grid, which keeps the last
+ * iteration's density and potentials, Restricted/Grad.cpp:73), then
+ * chx_grid_get_density_skeleton replaces SubGrid::getDensitySkeleton
+ * (GridDensity.cpp:79-130) and chx_grid_energy_grad replaces
+ * Grid::getEnergyGrad / SubGrid::getEnergyGrad (GridEnergy.cpp:15-30), using
+ * v_rho, v_sigma and grad rho of the last Evaluate("ev"). grad: natoms x 3,
+ * row-major [atom][x, y, z]. FP64 form with the AO tables in HBM; LDA and
+ * GGA (the reference's getEnergyGrad has no tau term). */
+int chx_grid_get_density_skeleton(chx_grid* g, const double* D, int nspin, int nbasis);
+int chx_grid_energy_grad(chx_grid* g, double* grad);
 /* Supply v_rho directly, in the library's point order, as the SAP guess does
  * when it writes SubGrid::Eps1Rho before Grid::getFock
  * (src/HartreeFockKohnSham/GuessSCF.cpp:37-60). Other potentials are zeroed. */

@@ -78,6 +78,40 @@ void poly_eval(const Poly& p, double x, double y, double z, double* q, double* q
   *q = v; qg[0] = gx; qg[1] = gy; qg[2] = gz; *qll = lxx + lyy + lzz;
 }
 
+// Value, gradient and Hessian (xx, xy, xz, yy, yz, zz) of a polynomial.
+void poly_eval_h(const Poly& p, double x, double y, double z, double* q, double* qg, double* qh) {
+  double v = 0, g[3] = {0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0};
+  auto pw = [](double b, int n) { return n < 0 ? 0.0 : ipow(b, n); };
+  for (const Mono& m : p) {
+    const int e[3] = {m.i, m.j, m.k};
+    const double b[3] = {x, y, z};
+    double f[3], d1[3], d2[3];
+    for (int r = 0; r < 3; ++r) {
+      f[r] = pw(b[r], e[r]);
+      d1[r] = e[r] * pw(b[r], e[r] - 1);
+      d2[r] = e[r] * (e[r] - 1) * pw(b[r], e[r] - 2);
+    }
+    v += m.c * f[0] * f[1] * f[2];
+    g[0] += m.c * d1[0] * f[1] * f[2];
+    g[1] += m.c * f[0] * d1[1] * f[2];
+    g[2] += m.c * f[0] * f[1] * d1[2];
+    h[0] += m.c * d2[0] * f[1] * f[2];
+    h[1] += m.c * d1[0] * d1[1] * f[2];
+    h[2] += m.c * d1[0] * f[1] * d1[2];
+    h[3] += m.c * f[0] * d2[1] * f[2];
+    h[4] += m.c * f[0] * d1[1] * d1[2];
+    h[5] += m.c * f[0] * f[1] * d2[2];
+  }
+  *q = v;
+  for (int r = 0; r < 3; ++r) qg[r] = g[r];
+  for (int k = 0; k < 6; ++k) qh[k] = h[k];
+}
+
+inline int hidx(int r, int t) {  // (r, t) -> position in (xx, xy, xz, yy, yz, zz)
+  static const int map[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  return map[r][t];
+}
+
 struct SubGrid {
   int n;      // unpadded
   int npad;   // padded to a multiple of 8 (GridGeneration.cpp:214)
@@ -781,6 +815,115 @@ int ox_fock(ox_system* s, double* V) {
   for (int t = 0; t < nt; ++t)
     for (size_t k = 0; k < (size_t)nb * nb; ++k) V[k] += part[t][k];
   return 0;
+}
+
+// XC nuclear gradient in the reference's skeleton form, the grid held fixed
+// (Restricted/Grad.cpp:72-84 -> SubGrid::getDensitySkeleton,
+// GridDensity.cpp:79-130 -> SubGrid::getEnergyGrad, GridEnergy.cpp:15-30):
+//   RhoGrad[g,t,A]    = -2 sum_{mu in A, nu} D_mu,nu d_t phi_mu phi_nu
+//   Rho1Grad[g,r,t,A] = -2 sum_{mu in A, nu} D_mu,nu (d_r d_t phi_mu phi_nu + d_t phi_mu d_r phi_nu)
+//   SigmaGrad[g,t,A]  = 2 sum_r Rho1[g,r] Rho1Grad[g,r,t,A]
+//   E[t,A] = sum_g W (Eps1Rho RhoGrad + Eps1Sigma SigmaGrad)
+// with Eps1Rho, Eps1Sigma, Rho1 from the preceding density / evaluation of
+// the same D. LDA and GGA (the reference has no tau term here).
+int ox_xc_grad(ox_system* s, const double* D, double* grad) {
+  return guarded([&] {
+    if (s->family > 1) throw std::runtime_error("oracle: the XC gradient is built for LDA and GGA");
+    if (ox_density(s, D) || ox_evaluate(s)) throw std::runtime_error(g_err);
+    const int nb = s->nbasis, na = s->natoms;
+    std::vector<int> ao_atom(nb);
+    for (size_t is = 0, mu = 0; is < s->shells.size(); ++is)
+      for (size_t m = 0; m < s->shell_ang[is].size(); ++m) ao_atom[mu++] = s->shells[is].center;
+    std::vector<double> G((size_t)3 * na, 0.0);
+    const int n = (int)s->active.size();
+#pragma omp parallel num_threads(s->nthreads)
+    {
+      std::vector<double> Gl((size_t)3 * na, 0.0), ao, ao1, ao2, X, U;
+#pragma omp for schedule(dynamic, 1)
+      for (int i = 0; i < n; ++i) {
+        const SubGrid& sg = s->subgrids[s->active[i]];
+        const int ng = sg.npad;
+        // phi, grad phi, Hessian of every AO at the sub-grid's points
+        ao.assign((size_t)nb * ng, 0); ao1.assign((size_t)3 * nb * ng, 0); ao2.assign((size_t)6 * nb * ng, 0);
+        int mu0 = 0;
+        for (size_t is = 0; is < s->shells.size(); ++is) {
+          const Shell& sh = s->shells[is];
+          const double* c = &s->xyz[3 * sh.center];
+          for (int g = 0; g < ng; ++g) {
+            const double d[3] = {sg.x[g] - c[0], sg.y[g] - c[1], sg.z[g] - c[2]};
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            double A0 = 0, A1 = 0, A2 = 0;
+            for (size_t p = 0; p < sh.exps.size(); ++p) {
+              const double t = sh.coefs[p] * std::exp(-sh.exps[p] * r2);
+              A0 += t; A1 += t * sh.exps[p]; A2 += t * sh.exps[p] * sh.exps[p];
+            }
+            double Rg[3], Rh[6];
+            for (int r = 0; r < 3; ++r) Rg[r] = -2 * d[r] * A1;
+            for (int r = 0; r < 3; ++r)
+              for (int t = r; t < 3; ++t) Rh[hidx(r, t)] = 4 * d[r] * d[t] * A2 - (r == t ? 2 * A1 : 0.0);
+            for (size_t m = 0; m < s->shell_ang[is].size(); ++m) {
+              double q, qg[3], qh[6];
+              poly_eval_h(s->shell_ang[is][m], d[0], d[1], d[2], &q, qg, qh);
+              const size_t mu = mu0 + m;
+              ao[mu * ng + g] = A0 * q;
+              for (int r = 0; r < 3; ++r) ao1[((size_t)r * nb + mu) * ng + g] = Rg[r] * q + A0 * qg[r];
+              for (int r = 0; r < 3; ++r)
+                for (int t = r; t < 3; ++t)
+                  ao2[((size_t)hidx(r, t) * nb + mu) * ng + g] =
+                      Rh[hidx(r, t)] * q + Rg[r] * qg[t] + Rg[t] * qg[r] + A0 * qh[hidx(r, t)];
+            }
+          }
+          mu0 += (int)s->shell_ang[is].size();
+        }
+        // X_mu = sum_nu D phi_nu, U_mu,r = sum_nu D d_r phi_nu
+        X.assign((size_t)nb * ng, 0); U.assign((size_t)3 * nb * ng, 0);
+        for (int mu = 0; mu < nb; ++mu)
+          for (int nu = 0; nu < nb; ++nu) {
+            const double dd = D[(size_t)nu * nb + mu];
+            if (dd == 0.0) continue;
+            for (int g = 0; g < ng; ++g) X[(size_t)mu * ng + g] += dd * ao[(size_t)nu * ng + g];
+            if (s->family >= 1)
+              for (int r = 0; r < 3; ++r)
+                for (int g = 0; g < ng; ++g)
+                  U[((size_t)r * nb + mu) * ng + g] += dd * ao1[((size_t)r * nb + nu) * ng + g];
+          }
+        for (int mu = 0; mu < nb; ++mu) {
+          const int A = ao_atom[mu];
+          for (int t = 0; t < 3; ++t) {
+            double acc = 0;
+            for (int g = 0; g < ng; ++g) {
+              const double dt = ao1[((size_t)t * nb + mu) * ng + g], x = X[(size_t)mu * ng + g];
+              double e = sg.vrho[g] * (-2 * dt * x);  // RhoGrad
+              if (s->family >= 1) {
+                double sg_ = 0;  // SigmaGrad = 2 sum_r Rho1 Rho1Grad
+                for (int r = 0; r < 3; ++r) {
+                  const double r1g = -2 * (ao2[((size_t)hidx(r, t) * nb + mu) * ng + g] * x +
+                                           dt * U[((size_t)r * nb + mu) * ng + g]);
+                  sg_ += 2 * sg.rho1[(size_t)r * ng + g] * r1g;
+                }
+                e += sg.vsigma[g] * sg_;
+              }
+              acc += sg.w[g] * e;
+            }
+            Gl[(size_t)3 * A + t] += acc;
+          }
+        }
+      }
+#pragma omp critical
+      for (size_t k = 0; k < G.size(); ++k) G[k] += Gl[k];
+    }
+    for (size_t k = 0; k < G.size(); ++k) grad[k] = G[k];
+  });
+}
+
+// Moves the basis functions of one atom (not its grid); the AO tables must be
+// rebuilt with ox_get_ao. Used by finite-difference checks of ox_xc_grad.
+int ox_shift_basis(ox_system* s, int atom, double dx, double dy, double dz) {
+  return guarded([&] {
+    if (atom < 0 || atom >= s->natoms) throw std::runtime_error("oracle: bad atom");
+    s->xyz[3 * atom] += dx; s->xyz[3 * atom + 1] += dy; s->xyz[3 * atom + 2] += dz;
+    s->ao_ready = false;
+  });
 }
 
 int ox_xc(ox_system* s, const double* D, double* V, double* Exc, double* Nel) {

@@ -67,6 +67,11 @@ int ox_get_ao(ox_system* s);
 /* One XC build: getDensity(D) -> Evaluate("ev") -> getEnergy -> getFock.
  * D and V are nbasis x nbasis column-major (symmetric). */
 int ox_xc(ox_system* s, const double* D, double* V, double* Exc, double* Nel);
+/* XC nuclear gradient, skeleton form with the grid fixed (Restricted/Grad.cpp:72-84,
+ * GridDensity.cpp:79-130, GridEnergy.cpp:15-30); grad: natoms x 3, LDA/GGA. */
+int ox_xc_grad(ox_system* s, const double* D, double* grad);
+/* Moves one atom's basis functions, not its grid points (finite-difference checks). */
+int ox_shift_basis(ox_system* s, int atom, double dx, double dy, double dz);
 
 /* Individual phases, for timing. */
 int ox_density(ox_system* s, const double* D);

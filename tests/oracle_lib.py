@@ -43,6 +43,8 @@ def lib():
         "ox_set_subgrid_subset": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
         "ox_get_ao": (ctypes.c_int, [_vp]),
         "ox_xc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "ox_xc_grad": (ctypes.c_int, [_vp, _vp, _vp]),
+        "ox_shift_basis": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "ox_density": (ctypes.c_int, [_vp, _vp]),
         "ox_evaluate": (ctypes.c_int, [_vp]),
         "ox_energy": (ctypes.c_int, [_vp, _vp]),
@@ -140,6 +142,17 @@ class OracleSystem:
         N = ctypes.c_double()
         _check(lib().ox_xc(self.h, _ptr(Df), _ptr(V), ctypes.addressof(E), ctypes.addressof(N)))
         return E.value, N.value, np.array(V)
+
+    def xc_grad(self, D):
+        """XC nuclear gradient (natoms x 3), skeleton form with the grid fixed."""
+        Df = np.asfortranarray(D, dtype=np.float64)
+        G = np.zeros(3 * len(self.Z))
+        _check(lib().ox_xc_grad(self.h, _ptr(Df), _ptr(G)))
+        return G.reshape(-1, 3)
+
+    def shift_basis(self, atom, d):
+        """Moves one atom's basis functions (not its grid); call get_ao afterwards."""
+        _check(lib().ox_shift_basis(self.h, int(atom), float(d[0]), float(d[1]), float(d[2])))
 
     def density(self):
         L = lib()

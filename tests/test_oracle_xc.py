@@ -105,3 +105,40 @@ def test_r2scan_vxc_is_derivative(h2o):
     fd = (4 * d2 - d1) / 3
     an = np.sum(V * dD)
     assert abs(fd - an) <= 1e-7 * max(1.0, abs(an))
+
+
+@pytest.mark.parametrize("func", ["PBE", "SVWN5", "B3LYP"])
+def test_xc_gradient_is_derivative(h2o, func):
+    """The skeleton XC gradient (grid fixed, GridEnergy.cpp:15-30) equals the
+    derivative of E_xc with respect to moving one atom's basis functions,
+    by Richardson-extrapolated central differences (error O(h^4))."""
+    o, D, S = h2o
+    o.set_functional(func)
+    o.get_ao()
+    G = o.xc_grad(D)
+    h = 1e-4
+    Gfd = np.zeros_like(G)
+    for A in range(3):
+        for t in range(3):
+            d = np.zeros(3)
+            d[t] = h
+            vals = []
+            for sgn in (1, -1, 2, -2):
+                o.shift_basis(A, sgn * d)
+                o.get_ao()
+                vals.append(o.xc(D)[0])
+                o.shift_basis(A, -sgn * d)
+            d1 = (vals[0] - vals[1]) / (2 * h)
+            d2 = (vals[2] - vals[3]) / (4 * h)
+            Gfd[A, t] = (4 * d1 - d2) / 3
+    o.get_ao()
+    assert np.abs(G - Gfd).max() <= 1e-9
+    assert np.abs(G).max() > 0.1
+
+
+def test_xc_gradient_rejects_meta_gga(h2o):
+    o, D, S = h2o
+    o.set_functional("R2SCAN")
+    o.get_ao()
+    with pytest.raises(RuntimeError):
+        o.xc_grad(D)
