@@ -108,7 +108,7 @@ void density_from_device(chx_grid* g, const double* dD) {
     g->mark_end(kStPermute, e);
     g->mark_begin(kStDensity, &e);
     for (int r = 0; r <= (g->type >= 2 ? 3 : 0); ++r)  // r > 0: tau partials, A = d_r phi
-      launch_density_mx(g->stream, r, ng, g->d_mol.p, nullptr);
+      launch_density_mx(g->stream, r, ng, g->hmol, nullptr, nullptr);
     g->mark_end(kStDensity, e);
     g->have_density = true;
     g->have_eval = false;
@@ -144,11 +144,11 @@ void fock_to_device(chx_grid* g, double* dV) {
     require(nc <= g->mx_nT, CHX_ERR_STATE, "a meta-GGA potential needs a grid of type 2 before getAO");
     g->mark_begin(kStMakeB, &e);
     const int nslot = (int)g->bt_list.size();
-    launch_make_b_mx(g->stream, nc, nc == 4 ? nslot : (nslot + 3) / 4, g->d_mol.p, nullptr);
+    launch_make_b_mx(g->stream, nc, nc == 4 ? nslot : (nslot + 3) / 4, g->hmol, nullptr, nullptr);
     g->mark_end(kStMakeB, e);
     launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
     g->mark_begin(kStVxc, &e);
-    launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->d_mol.p, nullptr);
+    launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->hmol, nullptr, nullptr);
     g->mark_end(kStVxc, e);
     g->mark_begin(kStFinalize, &e);
     launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);

@@ -170,10 +170,12 @@ void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* 
 void launch_bf16_check(hipStream_t s, unsigned n, unsigned* bad);
 void launch_fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v);
 // Launches over items = (grid index into mols, group / first slot / work item);
-// items == nullptr means grid 0 and item = block.
-void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev* mols, const int2* items);
-void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev* mols, const int2* items);
-void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev* mols, const int2* items);
+// items == nullptr means the single grid `one` (passed by value) and item = block.
+void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one, const MolDev* mols,
+                       const int2* items);
+void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols,
+                      const int2* items);
+void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols, const int2* items);
 
 }  // namespace chx
 

@@ -332,14 +332,18 @@ struct DensMxSmem {
 
 // MODE 0: A = phi; epilogue rho and grad rho / 2 (4 components into part).
 // MODE r = 1..3: A = d phi / d x_r; epilogue sum X_r d_r phi (tau partial of direction r).
-template <int MODE>
-__global__ __launch_bounds__(256, 3) void k_density_mx(const MolDev* __restrict__ mols, const int2* __restrict__ items) {
+// PACK = false: one grid, its descriptor passed by value (kernel argument, so
+// every pointer in it is known to be global and is read with scalar loads);
+// PACK = true: descriptors of many grids in device memory, items select them.
+template <int MODE, bool PACK>
+__global__ __launch_bounds__(256, 3) void k_density_mx(MolDev marg, const MolDev* __restrict__ mols,
+                                                       const int2* __restrict__ items) {
   constexpr int NOUT = MODE == 0 ? 4 : 1;
   constexpr int grp_stride = (MODE == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
   __shared__ DensMxSmem sm;
   const int item = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 wi = items ? items[item] : make_int2(0, item);
-  const MolDev& m = mols[wi.x];
+  const int2 wi = PACK ? items[item] : make_int2(0, item);
+  const MolDev& m = PACK ? mols[wi.x] : marg;
   const int grp = wi.y;
   const u16* __restrict__ cache = m.cache;
   const u16* __restrict__ Ds = m.Ds;
@@ -347,6 +351,7 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(const MolDev* __restrict_
   const int* __restrict__ bt_list = m.bt_list;
   const int* __restrict__ grp_tiles = m.grp_tiles;
   double* __restrict__ part = MODE == 0 ? m.part : m.part_tau + (MODE - 1) * kBatch;
+  const BasisView bv = m.bv;  // descriptor fields in registers, read once
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lrow = lane & 15, lk = lane >> 4;
@@ -438,7 +443,7 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(const MolDev* __restrict_
   DensEpi<MODE> epi;
   if (wave < nt) {  // wave q = tile q of the group; lane = point
     epi.xq = &sm.X[kTile * wave][lane];
-    walk_tile_f32(m.bv, grp_tiles[grp * kGroup + wave], sm.px[lane], sm.py[lane], sm.pz[lane], epi);
+    walk_tile_f32(bv, grp_tiles[grp * kGroup + wave], sm.px[lane], sm.py[lane], sm.pz[lane], epi);
   }
 #pragma unroll
   for (int c = 0; c < NOUT; ++c) sm.red[wave][c][lane] = epi.o[c];
@@ -451,14 +456,21 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(const MolDev* __restrict_
   }
 }
 
-void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev* mols, const int2* items) {
+void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one, const MolDev* mols,
+                       const int2* items) {
   if (nitems == 0) return;
+#define CHX_DMX(M)                                                                                  \
+  if (items)                                                                                        \
+    hipLaunchKernelGGL((k_density_mx<M, true>), dim3(nitems), dim3(256), 0, s, one, mols, items);  \
+  else                                                                                              \
+    hipLaunchKernelGGL((k_density_mx<M, false>), dim3(nitems), dim3(256), 0, s, one, mols, items);
   switch (mode) {
-    case 0: hipLaunchKernelGGL(k_density_mx<0>, dim3(nitems), dim3(256), 0, s, mols, items); break;
-    case 1: hipLaunchKernelGGL(k_density_mx<1>, dim3(nitems), dim3(256), 0, s, mols, items); break;
-    case 2: hipLaunchKernelGGL(k_density_mx<2>, dim3(nitems), dim3(256), 0, s, mols, items); break;
-    default: hipLaunchKernelGGL(k_density_mx<3>, dim3(nitems), dim3(256), 0, s, mols, items); break;
+    case 0: CHX_DMX(0); break;
+    case 1: CHX_DMX(1); break;
+    case 2: CHX_DMX(2); break;
+    default: CHX_DMX(3); break;
   }
+#undef CHX_DMX
   CHX_HIP(hipGetLastError());
 }
 
@@ -497,44 +509,57 @@ struct MakeB {
   __device__ void end() {}
 };
 
-template <int NC>
-__global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(const MolDev* __restrict__ mols,
+template <int NC, bool PACK>
+__global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev marg, const MolDev* __restrict__ mols,
                                                                   const int2* __restrict__ items) {
   constexpr int W = NC == 4 ? 1 : 4;  // waves per block
   __shared__ uint4 stage[W][NC * kMxT / 8];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
-  const int2 wi = items ? items[blockIdx.x] : make_int2(0, blockIdx.x * W);  // first slot of the block
-  const MolDev& m = mols[wi.x];
+  const int2 wi = PACK ? items[blockIdx.x] : make_int2(0, blockIdx.x * W);  // first slot of the block
+  const MolDev& m = PACK ? mols[wi.x] : marg;
   const int slot = wi.y + wave, nslots = m.nslots;
   const double* __restrict__ coef = m.coef;
+  const BasisView bv = m.bv;  // descriptor fields in registers, read once
+  const int* __restrict__ slot_batch = m.slot_batch;
+  const int* __restrict__ slot_tile = m.slot_tile;
+  const double *__restrict__ bx = m.bx, *__restrict__ by = m.by, *__restrict__ bz = m.bz;
+  u16* __restrict__ bt = m.bt;
   uint4* st = stage[wave];
 #pragma unroll
   for (int j = 0; j < NC * kMxT / 8 / 64; ++j) st[g + 64 * j] = uint4{0u, 0u, 0u, 0u};
   __syncthreads();
   if (slot < nslots) {
-    const int b = m.slot_batch[slot], t = m.slot_tile[slot];
+    const int b = slot_batch[slot], t = slot_tile[slot];
     const size_t p = (size_t)b * kBatch + g;
     const float ca = (float)coef[kCoef * p], cx = (float)coef[kCoef * p + 1], cy = (float)coef[kCoef * p + 2],
                 cz = (float)coef[kCoef * p + 3], ct = (float)coef[kCoef * p + 4];
     MakeB<NC> op;
     op.sv = reinterpret_cast<u16*>(st) + g;  // [c][hl][nu][g]
     op.ca = ca; op.cx = cx; op.cy = cy; op.cz = cz; op.ct = ct;
-    walk_tile_f32(m.bv, t, m.bx[p], m.by[p], m.bz[p], op);
+    walk_tile_f32(bv, t, bx[p], by[p], bz[p], op);
   }
   __syncthreads();
   if (slot < nslots) {
-    uint4* dst = reinterpret_cast<uint4*>(m.bt + (size_t)slot * NC * kMxT);
+    uint4* dst = reinterpret_cast<uint4*>(bt + (size_t)slot * NC * kMxT);
 #pragma unroll
     for (int j = 0; j < NC * kMxT / 8 / 64; ++j) dst[g + 64 * j] = st[g + 64 * j];
   }
 }
 
-void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev* mols, const int2* items) {
+void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols,
+                      const int2* items) {
   if (nitems == 0) return;
-  if (ncomp == 4)
-    hipLaunchKernelGGL(k_make_b_mx<4>, dim3(nitems), dim3(64), 0, s, mols, items);
-  else
-    hipLaunchKernelGGL(k_make_b_mx<1>, dim3(nitems), dim3(256), 0, s, mols, items);
+  if (ncomp == 4) {
+    if (items)
+      hipLaunchKernelGGL((k_make_b_mx<4, true>), dim3(nitems), dim3(64), 0, s, one, mols, items);
+    else
+      hipLaunchKernelGGL((k_make_b_mx<4, false>), dim3(nitems), dim3(64), 0, s, one, mols, items);
+  } else {
+    if (items)
+      hipLaunchKernelGGL((k_make_b_mx<1, true>), dim3(nitems), dim3(256), 0, s, one, mols, items);
+    else
+      hipLaunchKernelGGL((k_make_b_mx<1, false>), dim3(nitems), dim3(256), 0, s, one, mols, items);
+  }
   CHX_HIP(hipGetLastError());
 }
 
@@ -549,12 +574,13 @@ struct VxcMxSmem {
   uint4 B[2][2][kMxSup * kTile][4];
 };
 
-template <int NC>
-__global__ __launch_bounds__(256, 2) void k_vxc_mx(const MolDev* __restrict__ mols, const int2* __restrict__ items) {
+template <int NC, bool PACK>
+__global__ __launch_bounds__(256, 2) void k_vxc_mx(MolDev marg, const MolDev* __restrict__ mols,
+                                                   const int2* __restrict__ items) {
   __shared__ VxcMxSmem sm;
   const int item = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 wi = items ? items[item] : make_int2(0, item);
-  const MolDev& m = mols[wi.x];
+  const int2 wi = PACK ? items[item] : make_int2(0, item);
+  const MolDev& m = PACK ? mols[wi.x] : marg;
   const int4 wk = m.cwork[wi.y];
   const int I = wk.x, J = wk.y;
   const int* __restrict__ pair_batches = m.pair_batches;
@@ -640,12 +666,19 @@ __global__ __launch_bounds__(256, 2) void k_vxc_mx(const MolDev* __restrict__ mo
       }
 }
 
-void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev* mols, const int2* items) {
+void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols, const int2* items) {
   if (nitems == 0) return;
-  if (ncomp == 4)
-    hipLaunchKernelGGL(k_vxc_mx<4>, dim3(nitems), dim3(256), 0, s, mols, items);
-  else
-    hipLaunchKernelGGL(k_vxc_mx<1>, dim3(nitems), dim3(256), 0, s, mols, items);
+  if (ncomp == 4) {
+    if (items)
+      hipLaunchKernelGGL((k_vxc_mx<4, true>), dim3(nitems), dim3(256), 0, s, one, mols, items);
+    else
+      hipLaunchKernelGGL((k_vxc_mx<4, false>), dim3(nitems), dim3(256), 0, s, one, mols, items);
+  } else {
+    if (items)
+      hipLaunchKernelGGL((k_vxc_mx<1, true>), dim3(nitems), dim3(256), 0, s, one, mols, items);
+    else
+      hipLaunchKernelGGL((k_vxc_mx<1, false>), dim3(nitems), dim3(256), 0, s, one, mols, items);
+  }
   CHX_HIP(hipGetLastError());
 }
 

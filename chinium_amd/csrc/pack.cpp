@@ -107,7 +107,7 @@ void enqueue(chx_pack* p, const double* const* D, double* const* V) {
     launch_split_d(s, g->nbasis, g->npad, D[m], g->d_int2ref.p, g->d_Dsplit.p);
   }
   const int nd = (int)p->d_dens.n;
-  for (int r = 0; r <= (p->type >= 2 ? 3 : 0); ++r) launch_density_mx(s, r, nd, p->d_mols.p, p->d_dens.p);
+  for (int r = 0; r <= (p->type >= 2 ? 3 : 0); ++r) launch_density_mx(s, r, nd, p->grids[0]->hmol, p->d_mols.p, p->d_dens.p);
   for (int m = 0; m < n; ++m) {
     chx_grid* g = p->grids[m];
     launch_functional(s, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, p->d_codes.p,
@@ -116,14 +116,14 @@ void enqueue(chx_pack* p, const double* const* D, double* const* V) {
   }
   const int nc = p->family >= 2 ? 4 : 1;
   if (nc == 4)
-    launch_make_b_mx(s, 4, (int)p->d_mb4.n, p->d_mols.p, p->d_mb4.p);
+    launch_make_b_mx(s, 4, (int)p->d_mb4.n, p->grids[0]->hmol, p->d_mols.p, p->d_mb4.p);
   else
-    launch_make_b_mx(s, 1, (int)p->d_mb1.n, p->d_mols.p, p->d_mb1.p);
+    launch_make_b_mx(s, 1, (int)p->d_mb1.n, p->grids[0]->hmol, p->d_mols.p, p->d_mb1.p);
   for (int m = 0; m < n; ++m) {
     chx_grid* g = p->grids[m];
     launch_fill(s, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
   }
-  launch_vxc_mx(s, nc, (int)p->d_vxc.n, p->d_mols.p, p->d_vxc.p);
+  launch_vxc_mx(s, nc, (int)p->d_vxc.n, p->grids[0]->hmol, p->d_mols.p, p->d_vxc.p);
   for (int m = 0; m < n; ++m) {
     chx_grid* g = p->grids[m];
     launch_finalize_sym(s, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, V[m]);
