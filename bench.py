@@ -112,6 +112,14 @@ def main():
         del dV
         return r
 
+    traffic = {}  # HBM bytes per launch measured with rocprofv3 --pmc for this workload
+    tfile = os.path.join(ROOT, "profiles", "h2o16_def2svp_sg2_b3lyp_hbm_traffic.md")
+    if args.workload == "h2o16_def2svp_sg2_b3lyp" and os.path.exists(tfile):
+        for line in open(tfile):
+            if line.startswith("traffic_bytes "):
+                _, k, v = line.split()
+                traffic[k] = int(v)
+
     def roofline(r, peak, names):
         """Dominant contraction kernel: algorithmic FLOPs per launch / mean launch time."""
         flops_kernel = r["stats"]["flops_per_build"] / 2.0  # each contraction: 2 * sum_b g_b n_b^2
@@ -121,7 +129,8 @@ def main():
         dom, dom_ms = ("density", dens_ms) if dens_ms >= vxc_ms else ("vxc", vxc_ms)
         achieved = flops_kernel / (dom_ms * 1e-3) / 1e12
         return {"kernel": names[dom], "bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 2),
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic.get(names[dom]),
+                "traffic_source": "profiles/h2o16_def2svp_sg2_b3lyp_hbm_traffic.md" if names[dom] in traffic else None,
                 "avg_ms": round(dom_ms, 4), "flops_per_launch": flops_kernel}
 
     plain = measure(ca.CHX_PREC_FP64)
@@ -131,7 +140,7 @@ def main():
     ms_step = chosen["ms"]
     names64 = {"density": "k_density_c64<0>" if plain["cached"] else "k_density_f64",
                "vxc": "k_vxc_c64<1>" if plain["cached"] else "k_vxc_f64"}
-    namesmx = {"density": "k_density_mx<0>", "vxc": "k_vxc_mx<1>"}
+    namesmx = {"density": "k_density_mx<0,false>", "vxc": "k_vxc_mx<1,false>"}
 
     out = {
         "metric": "Vxc build ms/SCF-iter & TFLOP/s, chosen vs plain FP64 GPU at |dE_xc|<=1e-5 Eh/atom",

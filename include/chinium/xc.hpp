@@ -65,9 +65,10 @@ inline std::string data_root() {
 // The Mwfn fields the XC path reads: centres (Z, bohr) and normalised shells.
 class Molecule {
  public:
-  Molecule(const std::vector<int>& Z, const std::vector<double>& xyz_bohr) {
+  Molecule(const std::vector<int>& Z, const std::vector<double>& xyz_bohr) : natoms_((int)Z.size()) {
     check(chx_mol_create((int)Z.size(), Z.data(), xyz_bohr.data(), &h_));
   }
+  int getNumCenters() const { return natoms_; }
   // Mwfn::setBasis + Normalize (legacy/MwfnIO/Mwfn.cpp:686; src/Integral/Normalization.cpp:11).
   void setBasis(const std::string& name) { check(chx_mol_set_basis_file(h_, data_root().c_str(), name.c_str())); }
   // One MwfnShell: Type, Exponents, NormalizedCoefficients.
@@ -86,17 +87,19 @@ class Molecule {
 
  private:
   chx_mol* h_ = nullptr;
+  int natoms_ = 0;
 };
 
 class Grid {
  public:
   Grid() = default;
   // Grid::Grid(Mwfn*, std::string grid, int nthreads, int output)
-  Grid(const Molecule* mol, std::string grid, int nthreads, int output) : nbasis_(mol->getNumBasis()) {
+  Grid(const Molecule* mol, std::string grid, int nthreads, int output)
+      : nbasis_(mol->getNumBasis()), natoms_(mol->getNumCenters()) {
     if (grid.empty()) return;  // as the reference: an empty name builds nothing
     check(chx_grid_create(mol->handle(), grid.c_str(), data_root().c_str(), nthreads, output, &h_));
   }
-  Grid(const Grid& o) : nbasis_(o.nbasis_) {
+  Grid(const Grid& o) : nbasis_(o.nbasis_), natoms_(o.natoms_) {
     if (o.h_) check(chx_grid_copy(o.h_, &h_));
   }
   Grid& operator=(const Grid& o) {
@@ -104,6 +107,7 @@ class Grid {
       Grid tmp(o);
       std::swap(h_, tmp.h_);
       std::swap(nbasis_, tmp.nbasis_);
+      std::swap(natoms_, tmp.natoms_);
     }
     return *this;
   }
@@ -138,11 +142,28 @@ class Grid {
     check(chx_grid_fock(h_, F[0].data(), 1, nbasis_));
     return F;
   }
+  // SubGrid::getDensitySkeleton / Grid::getEnergyGrad (GridDensity.cpp:79-130,
+  // GridEnergy.cpp:15-30): XC nuclear gradient, 3 * natoms, [atom][x, y, z].
+  void getDensitySkeleton(std::vector<EigenMatrix> D) {
+    const int nspin = (int)D.size();
+    std::vector<double> buf;
+    buf.reserve((size_t)nspin * nbasis_ * nbasis_);
+    for (const EigenMatrix& m : D) buf.insert(buf.end(), m.data(), m.data() + (size_t)nbasis_ * nbasis_);
+    check(chx_grid_get_density_skeleton(h_, buf.data(), nspin, nbasis_));
+  }
+  std::vector<double> getEnergyGrad() {
+    std::vector<double> g(3 * (size_t)natoms_);
+    check(chx_grid_energy_grad(h_, g.data()));
+    return g;
+  }
+  // GPU options (no reference counterpart): CHX_PREC_FP64 or CHX_PREC_MIXED.
+  void setPrecision(int precision) { check(chx_grid_set_precision(h_, precision)); }
   chx_grid* handle() { return h_; }
 
  private:
   chx_grid* h_ = nullptr;
   int nbasis_ = 0;
+  int natoms_ = 0;
 };
 
 class ExchangeCorrelation {
