@@ -345,8 +345,9 @@ void build_cwork(chx_grid* g) {
   g->cpair_batches.clear();
   size_t maxlen = 0;
   for (const auto& L : cp) maxlen = std::max(maxlen, L.size());
-  const size_t chunk =
-      g->vxc_chunk > 0 ? (size_t)g->vxc_chunk : (g->precision == CHX_PREC_MIXED ? 32 : kCacheChunk);
+  const bool mixed = g->precision == CHX_PREC_MIXED;
+  size_t chunk = g->vxc_chunk > 0 ? (size_t)g->vxc_chunk : (mixed ? 32 : kCacheChunk);
+  if (mixed) chunk = std::min<size_t>(chunk, kMxMaxChunk);  // the kernel stages one item's slot table
   for (size_t k = 0; k < maxlen; k += chunk)
     for (int I = 0; I < nsc; ++I)
       for (int J = 0; J < nsc; ++J) {
@@ -357,6 +358,21 @@ void build_cwork(chx_grid* g) {
         g->cpair_batches.insert(g->cpair_batches.end(), L.begin() + k, L.begin() + e);
         g->cwork.push_back(make_int4(I, J, begin, (int)g->cpair_batches.size()));
       }
+  g->cslot.clear();
+  if (mixed) {  // per batch of an item: the slots of I's and J's tiles, the all-zero slot if not alive
+    const int zero = (int)g->bt_list.size();
+    g->cslot.resize(g->cpair_batches.size() * 8);
+    for (const int4& w : g->cwork)
+      for (int k = w.z; k < w.w; ++k) {
+        const int b = g->cpair_batches[k];
+        for (int t = 0; t < kCacheSuper; ++t) {
+          const int si = g->slot_of[(size_t)b * nt + w.x * kCacheSuper + t];
+          const int sj = g->slot_of[(size_t)b * nt + w.y * kCacheSuper + t];
+          g->cslot[(size_t)k * 8 + t] = si >= 0 ? si : zero;
+          g->cslot[(size_t)k * 8 + 4 + t] = sj >= 0 ? sj : zero;
+        }
+      }
+  }
 }
 
 // Rebuilds the Vxc work items for another chunk length (autotuning) without
@@ -369,8 +385,10 @@ void set_vxc_chunk(chx_grid* g, int chunk) {
   g->d_cwork.upload(g->cwork, st);
   g->d_cpair_batches.upload(g->cpair_batches, st);
   if (g->precision == CHX_PREC_MIXED) {
+    g->d_cslot.upload(g->cslot, st);
     g->hmol.cwork = g->d_cwork.p;
     g->hmol.pair_batches = g->d_cpair_batches.p;
+    g->hmol.cslot = g->d_cslot.p;
     g->d_mol.upload(std::vector<MolDev>{g->hmol}, st);
   }
   CHX_HIP(hipStreamSynchronize(st));
@@ -464,6 +482,9 @@ void grid_plan(chx_grid* g) {
   const double cache_bytes =
       mixed ? (double)nslot * 3.0 * nT * kMxComp * sizeof(unsigned short)
             : (double)nslot * (4.0 + (g->type >= 2 ? 4.0 : 1.0)) * kBatch * kTile * sizeof(double);
+  if (mixed && g->ntiles > kMxMaxTiles)
+    throw Error(CHX_ERR_UNSUPPORTED, "the mixed-precision form takes at most " + std::to_string(kMxMaxTiles * kTile) +
+                                         " basis functions");
   if (mixed && g->max_nprim > kSegPrim)
     throw Error(CHX_ERR_UNSUPPORTED, "the mixed-precision form takes at most " + std::to_string(kSegPrim) +
                                          " primitives per shell");
@@ -490,9 +511,15 @@ void grid_plan(chx_grid* g) {
       g->bmat_comps = 0;
       g->mx_nT = nT;
       g->mx_sstride = 2 * nT * kMxComp;  // AOg then AOt, nT components each
-      g->d_cmx.alloc(std::max<size_t>(1, nslot * (size_t)g->mx_sstride));
+      // one extra all-zero slot at index nslot stands in for tiles that are not alive
+      g->d_cmx.alloc((nslot + 1) * (size_t)g->mx_sstride);
+      CHX_HIP(hipMemsetAsync(g->d_cmx.p + nslot * (size_t)g->mx_sstride, 0,
+                             sizeof(unsigned short) * g->mx_sstride, st));
       g->bmx_comps = nT;
-      g->d_bmx.alloc(std::max<size_t>(1, nslot * (size_t)nT * kMxComp));
+      g->d_bmx.alloc((nslot + 1) * (size_t)nT * kMxComp);
+      CHX_HIP(hipMemsetAsync(g->d_bmx.p + nslot * (size_t)nT * kMxComp, 0,
+                             sizeof(unsigned short) * nT * kMxComp, st));
+      g->d_cslot.upload(g->cslot, st);
       g->d_Dsplit.alloc(2 * (size_t)g->npad * g->npad);
       launch_ao_cache_mx(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p,
                          g->d_z.p, g->view(), nT, g->mx_sstride, g->d_cmx.p);
@@ -545,6 +572,7 @@ void grid_plan(chx_grid* g) {
     m.slot_batch = g->d_slot_batch.p; m.slot_tile = g->d_slot_tile.p;
     m.coef = g->d_coef.p; m.bt = g->d_bmx.p;
     m.cwork = g->d_cwork.p; m.pair_batches = g->d_cpair_batches.p; m.slot_of = g->d_slot_of.p;
+    m.cslot = g->d_cslot.p;
     m.Mint = g->d_Mint.p;
     g->d_mol.upload(std::vector<MolDev>{m}, st);
   }

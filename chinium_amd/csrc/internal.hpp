@@ -148,8 +148,11 @@ struct MolDev {
   unsigned short* bt;           // B^T per slot
   const int4* cwork;            // Vxc work items
   const int *pair_batches, *slot_of;
+  const int* cslot;             // per pair_batches entry: slots of I's 4 tiles, then J's (zero slot if absent)
   double* Mint;                 // M accumulator, npad^2
 };
+constexpr int kMxMaxTiles = 512;  // alive tiles per batch the mixed density kernel stages (npad <= 16384)
+constexpr int kMxMaxChunk = 128;  // batches per Vxc work item in the mixed form
 
 // XC nuclear gradient (kernels_grad.hip), FP64 cached form.
 void launch_xgrad_xz(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
@@ -235,6 +238,8 @@ struct chx_grid {
   int vxc_chunk = 0;                 // batches per Vxc work item; 0 = the form's default
   std::vector<int4> cwork;
   std::vector<int> cpair_batches;
+  std::vector<int> cslot;              // mixed form: slot table per cpair_batches entry
+  chx::DevBuf<int> d_cslot;
   chx::DevBuf<double> d_cache, d_bmat, d_Mint;
   chx::DevBuf<int> d_slot_batch, d_slot_tile, d_slot_of, d_tile_nao, d_grp_slots, d_cpair_batches;
   chx::DevBuf<int4> d_cwork;

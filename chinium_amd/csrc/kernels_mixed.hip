@@ -27,6 +27,9 @@ namespace chx {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
+// 16 bytes of operand data; a native vector (HIP's u32x4 is a struct whose
+// copies the compiler may keep in private memory)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMxRow = 2 * kTile;             // one AOg row: 32 hi then 32 lo
 constexpr int kMxT = 2 * kTile * kBatch;      // elements per component of AOg, AOt and Bt
@@ -80,8 +83,8 @@ void launch_bf16_check(hipStream_t s, unsigned n, unsigned* bad) {
   hipLaunchKernelGGL(k_bf16_check, dim3((n + 255) / 256), dim3(256), 0, s, n, bad);
   CHX_HIP(hipGetLastError());
 }
-__device__ __forceinline__ bf16x8 as_bf(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
-__device__ __forceinline__ f32x4 mfma3(uint4 ah, uint4 al, uint4 bh, uint4 bl, f32x4 c) {
+__device__ __forceinline__ bf16x8 as_bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+__device__ __forceinline__ f32x4 mfma3(u32x4 ah, u32x4 al, u32x4 bh, u32x4 bl, f32x4 c) {
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al), as_bf(bh), c, 0, 0, 0);
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah), as_bf(bl), c, 0, 0, 0);
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah), as_bf(bh), c, 0, 0, 0);
@@ -323,11 +326,12 @@ struct DensEpi {
 // FP32, one lane per point.
 struct DensMxSmem {
   union {
-    uint4 B[2][2][kGroup * kTile][4];  // [buf][hl][nu][mu chunk, swizzled]
+    u32x4 B[2][2][kGroup * kTile][4];  // [buf][hl][nu][mu chunk, swizzled]
     float X[kGroup * kTile][kBatch + 1];     // epilogue: X^T [nu][g]
   };
   float red[kGroup][4][kBatch];
   double px[kBatch], py[kBatch], pz[kBatch];
+  int tl[kMxMaxTiles];  // the batch's alive tiles (K steps), staged once
 };
 
 // MODE 0: A = phi; epilogue rho and grad rho / 2 (4 components into part).
@@ -368,30 +372,38 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(MolDev marg, const MolDev
     sm.py[tid] = m.by[b * kBatch + tid];
     sm.pz[tid] = m.bz[b * kBatch + tid];
   }
+  for (int j = tid; j < it1 - it0; j += 256) sm.tl[j] = bt_list[it0 + j];
+  // Per thread: the D rows (columns of X) it stages for j = 0..3; columns of
+  // absent tiles read a present tile's rows and are stored as zeros, so that
+  // every load is unconditional (and the prefetch is not serialised).
+  size_t dcol[4];
+  bool dlive[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = tid + 256 * j;
+    const int n = (e >> 2) & 127, q = n >> 5;
+    dlive[j] = q < nt;
+    dcol[j] = (size_t)(e >> 9) * np2 + (size_t)(jt[dlive[j] ? q : 0] * kTile + (n & 31)) * npad + 8 * (e & 3);
+  }
+  __syncthreads();
 
   // Two register stages in flight: the loads for K step k + 2 are issued
   // while the MFMAs of step k run.
   struct Stage {
-    uint4 a[2][2];  // A rows of this wave [row block][hl]
-    uint4 d[4];     // this thread's share of the D block
+    u32x4 a[2][2];  // A rows of this wave [row block][hl]
+    u32x4 d[4];     // this thread's share of the D block
   };
   auto load = [&](int it, Stage& S) {
     const u16* src = cache + (size_t)it * sstride + (size_t)MODE * kMxT;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const u16* row = src + (32 * wr + 16 * r + lrow) * kMxRow + 8 * lk;
-      S.a[r][0] = *reinterpret_cast<const uint4*>(row);
-      S.a[r][1] = *reinterpret_cast<const uint4*>(row + kTile);
+      S.a[r][0] = *reinterpret_cast<const u32x4*>(row);
+      S.a[r][1] = *reinterpret_cast<const u32x4*>(row + kTile);
     }
-    const int ti = bt_list[it];
+    const int ti = sm.tl[it - it0];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j;
-      const int hl = e >> 9, n = (e >> 2) & 127, qc = e & 3, q = n >> 5;
-      S.d[j] = q < nt ? *reinterpret_cast<const uint4*>(Ds + hl * np2 + (size_t)(jt[q] * kTile + (n & 31)) * npad +
-                                                        ti * kTile + 8 * qc)
-                      : uint4{0u, 0u, 0u, 0u};
-    }
+    for (int j = 0; j < 4; ++j) S.d[j] = *reinterpret_cast<const u32x4*>(Ds + dcol[j] + ti * kTile);
   };
 
   f32x4 acc[2][4];
@@ -404,9 +416,9 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(MolDev marg, const MolDev
     for (int j = 0; j < 4; ++j) {
       const int e = tid + 256 * j;
       const int n = (e >> 2) & 127;
-      sm.B[buf][e >> 9][n][swz(n, e & 3)] = S.d[j];
+      sm.B[buf][e >> 9][n][swz(n, e & 3)] = dlive[j] ? S.d[j] : u32x4{0u, 0u, 0u, 0u};
     }
-    uint4 ac[2][2];
+    u32x4 ac[2][2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       ac[r][0] = S.a[r][0];
@@ -418,7 +430,7 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(MolDev marg, const MolDev
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       const int n = 64 * wc + 16 * cb + lrow;
-      const uint4 bh = sm.B[buf][0][n][swz(n, lk)], bl = sm.B[buf][1][n][swz(n, lk)];
+      const u32x4 bh = sm.B[buf][0][n][swz(n, lk)], bl = sm.B[buf][1][n][swz(n, lk)];
 #pragma unroll
       for (int r = 0; r < 2; ++r) acc[r][cb] = mfma3(ac[r][0], ac[r][1], bh, bl, acc[r][cb]);
     }
@@ -513,7 +525,7 @@ template <int NC, bool PACK>
 __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev marg, const MolDev* __restrict__ mols,
                                                                   const int2* __restrict__ items) {
   constexpr int W = NC == 4 ? 1 : 4;  // waves per block
-  __shared__ uint4 stage[W][NC * kMxT / 8];
+  __shared__ u32x4 stage[W][NC * kMxT / 8];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
   const int2 wi = PACK ? items[blockIdx.x] : make_int2(0, blockIdx.x * W);  // first slot of the block
   const MolDev& m = PACK ? mols[wi.x] : marg;
@@ -524,9 +536,9 @@ __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev marg, c
   const int* __restrict__ slot_tile = m.slot_tile;
   const double *__restrict__ bx = m.bx, *__restrict__ by = m.by, *__restrict__ bz = m.bz;
   u16* __restrict__ bt = m.bt;
-  uint4* st = stage[wave];
+  u32x4* st = stage[wave];
 #pragma unroll
-  for (int j = 0; j < NC * kMxT / 8 / 64; ++j) st[g + 64 * j] = uint4{0u, 0u, 0u, 0u};
+  for (int j = 0; j < NC * kMxT / 8 / 64; ++j) st[g + 64 * j] = u32x4{0u, 0u, 0u, 0u};
   __syncthreads();
   if (slot < nslots) {
     const int b = slot_batch[slot], t = slot_tile[slot];
@@ -540,7 +552,7 @@ __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev marg, c
   }
   __syncthreads();
   if (slot < nslots) {
-    uint4* dst = reinterpret_cast<uint4*>(bt + (size_t)slot * NC * kMxT);
+    u32x4* dst = reinterpret_cast<u32x4*>(bt + (size_t)slot * NC * kMxT);
 #pragma unroll
     for (int j = 0; j < NC * kMxT / 8 / 64; ++j) dst[g + 64 * j] = st[g + 64 * j];
   }
@@ -570,9 +582,62 @@ void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, c
 // double-buffered. 4 waves in a 2 x 2 grid, each owning a 64 x 64 block of M.
 // One FP64 atomic add per element per work item.
 struct VxcMxSmem {
-  uint4 A[2][2][kMxSup * kTile][4];  // [buf][hl][mu][g chunk, swizzled]
-  uint4 B[2][2][kMxSup * kTile][4];
+  u32x4 A[2][2][kMxSup * kTile][4];  // [buf][hl][mu][g chunk, swizzled]
+  u32x4 B[2][2][kMxSup * kTile][4];
+  int sl[kMxMaxChunk][8];            // per batch of the item: slots of I's and J's tiles
 };
+
+// Stage k -> (batch, half, component): this thread's 4 chunks of Phi^T_I and
+// of B^T_J (thread t moves, for tile t / 64 of each super-tile, the 16-byte
+// chunks e = t % 64 + 64 j of that tile: hl = e >> 7, row (e >> 2) & 31, chunk e & 3).
+template <int NC>
+__device__ __forceinline__ void vxc_load(const int (*sl)[8], int k, int tj, int lane, const u16* __restrict__ cache,
+                                         const u16* __restrict__ bt, int sstride, int toff, u32x4* A, u32x4* Q) {
+  const int it = k / (2 * NC), h = (k / NC) % 2, c = k % NC;
+  const int si = sl[it][tj], sj = sl[it][4 + tj];
+  const u16* pa = cache + (size_t)si * sstride + toff + c * kMxT + h * 32;
+  const u16* pb = bt + ((size_t)sj * NC + c) * kMxT + h * 32;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = lane + 64 * j;
+    const int off = (e >> 7) * (kTile * kBatch) + ((e >> 2) & 31) * kBatch + 8 * (e & 3);
+    A[j] = *reinterpret_cast<const u32x4*>(pa + off);
+    Q[j] = *reinterpret_cast<const u32x4*>(pb + off);
+  }
+}
+
+template <int NC>
+__device__ __forceinline__ void vxc_step(VxcMxSmem& sm, int k, int nstage, int tj, int lane, int wr, int wc,
+                                         const u16* __restrict__ cache, const u16* __restrict__ bt, int sstride,
+                                         int toff, u32x4* A, u32x4* Q, int buf, f32x4 (&acc)[4][4]) {
+  const int lrow = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = lane + 64 * j;
+    const int hl = e >> 7, row = tj * kTile + ((e >> 2) & 31), ch = swz(row, e & 3);
+    sm.A[buf][hl][row][ch] = A[j];
+    sm.B[buf][hl][row][ch] = Q[j];
+  }
+  __syncthreads();
+  // always issue the loads (past the end: a valid stage again), so that the
+  // stage registers are written unconditionally
+  vxc_load<NC>(sm.sl, k + 2 < nstage ? k + 2 : nstage - 1, tj, lane, cache, bt, sstride, toff, A, Q);
+  u32x4 ah[4], al[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = 64 * wr + 16 * r + lrow;
+    ah[r] = sm.A[buf][0][row][swz(row, lk)];
+    al[r] = sm.A[buf][1][row][swz(row, lk)];
+  }
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const int col = 64 * wc + 16 * cb + lrow;
+    const u32x4 bh = sm.B[buf][0][col][swz(col, lk)];
+    const u32x4 bl = sm.B[buf][1][col][swz(col, lk)];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r][cb] = mfma3(ah[r], al[r], bh, bl, acc[r][cb]);
+  }
+}
 
 template <int NC, bool PACK>
 __global__ __launch_bounds__(256, 2) void k_vxc_mx(MolDev marg, const MolDev* __restrict__ mols,
@@ -583,12 +648,15 @@ __global__ __launch_bounds__(256, 2) void k_vxc_mx(MolDev marg, const MolDev* __
   const MolDev& m = PACK ? mols[wi.x] : marg;
   const int4 wk = m.cwork[wi.y];
   const int I = wk.x, J = wk.y;
-  const int* __restrict__ pair_batches = m.pair_batches;
-  const int* __restrict__ slot_of = m.slot_of;
   const u16* __restrict__ cache = m.cache;
   const u16* __restrict__ bt = m.bt;
-  const int ntiles = m.ntiles, sstride = m.sstride, toff = m.toff, npad = m.npad;
+  const int sstride = m.sstride, toff = m.toff, npad = m.npad;
   double* __restrict__ Mint = m.Mint;
+  {  // the item's slot table (absent tiles point at the all-zero slot)
+    const int* __restrict__ cs = m.cslot + (size_t)wk.z * 8;
+    for (int e = threadIdx.x; e < (wk.w - wk.z) * 8; e += 256) sm.sl[e >> 3][e & 7] = cs[e];
+    __syncthreads();
+  }
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lrow = lane & 15, lk = lane >> 4;
@@ -596,24 +664,6 @@ __global__ __launch_bounds__(256, 2) void k_vxc_mx(MolDev marg, const MolDev* __
   // Thread t moves, for tile t / 64 of each super-tile, the 16-byte chunks
   // e = t % 64 + 64 j (j = 0..3) of that tile: hl = e >> 7, row (e >> 2) & 31, chunk e & 3.
   const int tj = tid >> 6;
-  struct Stage {
-    uint4 a[4], q[4];
-  };
-  auto load = [&](int k, Stage& S) {  // stage k -> (batch, half, component)
-    const int it = wk.z + k / (2 * NC), h = (k / NC) % 2, c = k % NC;
-    const int b = pair_batches[it];
-    const int si = slot_of[(size_t)b * ntiles + I * kMxSup + tj];
-    const int sj = slot_of[(size_t)b * ntiles + J * kMxSup + tj];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = (tid & 63) + 64 * j;
-      const int off = (e >> 7) * (kTile * kBatch) + ((e >> 2) & 31) * kBatch + h * 32 + 8 * (e & 3);
-      S.a[j] = si >= 0 ? *reinterpret_cast<const uint4*>(cache + (size_t)si * sstride + toff + c * kMxT + off)
-                       : uint4{0u, 0u, 0u, 0u};
-      S.q[j] = sj >= 0 ? *reinterpret_cast<const uint4*>(bt + ((size_t)sj * NC + c) * kMxT + off)
-                       : uint4{0u, 0u, 0u, 0u};
-    }
-  };
   f32x4 acc[4][4];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
@@ -621,38 +671,12 @@ __global__ __launch_bounds__(256, 2) void k_vxc_mx(MolDev marg, const MolDev* __
     for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int nstage = (wk.w - wk.z) * 2 * NC;
   // Two register stages in flight: stage k + 2 loads while stage k computes.
-  auto step = [&](int k, Stage& S, int buf) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = (tid & 63) + 64 * j;
-      const int hl = e >> 7, row = tj * kTile + ((e >> 2) & 31), ch = swz(row, e & 3);
-      sm.A[buf][hl][row][ch] = S.a[j];
-      sm.B[buf][hl][row][ch] = S.q[j];
-    }
-    __syncthreads();
-    if (k + 2 < nstage) load(k + 2, S);
-    uint4 ah[4], al[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = 64 * wr + 16 * r + lrow;
-      ah[r] = sm.A[buf][0][row][swz(row, lk)];
-      al[r] = sm.A[buf][1][row][swz(row, lk)];
-    }
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-      const int col = 64 * wc + 16 * cb + lrow;
-      const uint4 bh = sm.B[buf][0][col][swz(col, lk)];
-      const uint4 bl = sm.B[buf][1][col][swz(col, lk)];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r][cb] = mfma3(ah[r], al[r], bh, bl, acc[r][cb]);
-    }
-  };
-  Stage S0, S1;
-  if (nstage > 0) load(0, S0);
-  if (nstage > 1) load(1, S1);
+  u32x4 a0[4], q0[4], a1[4], q1[4];
+  vxc_load<NC>(sm.sl, 0, tj, lane, cache, bt, sstride, toff, a0, q0);  // items hold >= 1 batch: nstage >= 2
+  vxc_load<NC>(sm.sl, 1, tj, lane, cache, bt, sstride, toff, a1, q1);
   for (int k = 0; k < nstage; k += 2) {  // LDS buffer 0 for even stages, 1 for odd
-    step(k, S0, 0);
-    if (k + 1 < nstage) step(k + 1, S1, 1);
+    vxc_step<NC>(sm, k, nstage, tj, lane, wr, wc, cache, bt, sstride, toff, a0, q0, 0, acc);
+    if (k + 1 < nstage) vxc_step<NC>(sm, k + 1, nstage, tj, lane, wr, wc, cache, bt, sstride, toff, a1, q1, 1, acc);
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r)
