@@ -79,6 +79,7 @@ def main():
         t_setup = time.time() - t0
         stream = grid.stream()
         dV = hip.DeviceArray(n * n)
+        tuned = grid.autotune(xc, dD.ptr, dV.ptr, reps=2) if grid.aoCached() else None
 
         def step():
             grid.getDensityDevice(dD.ptr, 1, n)
@@ -108,7 +109,7 @@ def main():
         grid.profile(False)
         V = dV.download().reshape((n, n), order="F")
         r = dict(grid=grid, E=E, V=V, N=grid.getNumElectrons(), ms=ms_step, wall_ms=1e3 * wall / args.steps,
-                 setup_s=t_setup, kt=kt, stats=grid.stats(), cached=grid.aoCached())
+                 setup_s=t_setup, kt=kt, stats=grid.stats(), cached=grid.aoCached(), tuned=tuned)
         del dV
         return r
 
@@ -161,6 +162,8 @@ def main():
                    "form": "mixed: split-BF16 MFMA contractions, FP32 accumulation, FP64 functional/E/N/V sums",
                    "ao_tables": "cached in HBM (bf16 hi/lo pairs)", "screening_threshold": 1e-14},
         "speedup_vs_plain_fp64": round(plain["ms"] / ms_step, 3),
+        "autotune": {"vxc_chunk": chosen["tuned"][0], "ms": {str(k): round(v, 4) for k, v in chosen["tuned"][1].items()},
+                     "plain_fp64_vxc_chunk": plain["tuned"][0] if plain["tuned"] else None},
         "tflops": round(flops_build / (ms_step * 1e-3) / 1e12, 3),
         "gpoints_per_s": round(st["points"] / (ms_step * 1e-3) / 1e9, 4),
         "flops_per_build": flops_build,

@@ -90,6 +90,7 @@ def lib():
         "chx_lebedev_rule": (i, [i, _vp]),
         "chx_xc_eval_points": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "chx_grid_get_density_skeleton": (i, [_vp, _vp, i, i]),
+        "chx_grid_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
         "chx_grid_energy_grad": (i, [_vp, _vp]),
         "chx_pack_create": (i, [_vp, i, P(_vp)]),
         "chx_pack_xc": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -253,6 +254,13 @@ class Grid:
         G = np.zeros(3 * self.natoms)
         _check(lib().chx_grid_energy_grad(self.h, _ptr(G)))
         return G.reshape(-1, 3)
+
+    def autotune(self, xc, D_ptr, V_ptr, reps=3):
+        """Times each Vxc work-item length, keeps the fastest; returns (chunk, {chunk: ms})."""
+        c = ctypes.c_int()
+        ms = (ctypes.c_float * 4)()
+        _check(lib().chx_grid_autotune(self.h, xc.h, D_ptr, V_ptr, int(reps), ctypes.byref(c), ms))
+        return c.value, dict(zip((16, 32, 64, 128), list(ms)))
 
     def getDensityDevice(self, ptr, nspin, nbasis):
         _check(lib().chx_grid_get_density_device(self.h, ctypes.c_void_p(ptr), int(nspin), int(nbasis)))

@@ -15,6 +15,7 @@ void mol_overlap(const chx_mol* m, double* S);
 namespace chx {
 void grid_build(chx_grid* g, const std::string& grid_name, const std::string& data_root);
 void grid_plan(chx_grid* g);
+void set_vxc_chunk(chx_grid* g, int chunk);
 void ensure_stream(chx_grid* g);
 void launch_selftest(hipStream_t s, double* out);
 }  // namespace chx
@@ -423,6 +424,51 @@ int chx_grid_energy_grad(chx_grid* g, double* grad) {
                      g->d_y.p, g->d_z.p, g->view(), g->d_coef.p, xz.p, dg.p);
     CHX_HIP(hipMemcpyAsync(grad, dg.p, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g->stream));
     CHX_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
+int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double* V_dev, int reps, int* chunk_out,
+                      float* ms_out) {
+  return guard([&] {
+    require(g && xc && D_dev && V_dev && reps > 0, CHX_ERR_ARG, "bad arguments");
+    require(g->ao_ready && g->cached, CHX_ERR_STATE, "Grid::getAO (with the AO tables in HBM) must precede tuning");
+    require(g->type >= xc->family, CHX_ERR_STATE, "grid type is below the functional family (call setType)");
+    static const int cand[] = {16, 32, 64, 128};
+    hipEvent_t a, b;
+    CHX_HIP(hipEventCreate(&a));
+    CHX_HIP(hipEventCreate(&b));
+    const bool prof = g->profile;
+    g->profile = false;
+    auto build = [&] {
+      density_from_device(g, D_dev);
+      run_functional(g, xc, true, true);
+      g->v_family = xc->family;
+      g->have_eval = g->have_v = true;
+      fock_to_device(g, V_dev);
+    };
+    int best = -1;
+    float best_ms = 0.f;
+    for (int k = 0; k < 4; ++k) {
+      set_vxc_chunk(g, cand[k]);
+      build();  // warm-up
+      CHX_HIP(hipEventRecord(a, g->stream));
+      for (int r = 0; r < reps; ++r) build();
+      CHX_HIP(hipEventRecord(b, g->stream));
+      CHX_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      CHX_HIP(hipEventElapsedTime(&ms, a, b));
+      ms /= (float)reps;
+      if (ms_out) ms_out[k] = ms;
+      if (best < 0 || ms < best_ms) {
+        best = k;
+        best_ms = ms;
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    g->profile = prof;
+    set_vxc_chunk(g, cand[best]);
+    if (chunk_out) *chunk_out = cand[best];
   });
 }
 

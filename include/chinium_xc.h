@@ -127,6 +127,12 @@ int chx_grid_set_precision(chx_grid* g, int precision);
  * and |grad phi| in it stays at or below the threshold at every point of the
  * batch. Takes effect at the next chx_grid_get_ao. Default 1e-14. */
 int chx_grid_set_screening(chx_grid* g, double threshold);
+/* Autotuner (no reference counterpart): times `reps` XC builds (D_dev ->
+ * V_dev, device pointers, column-major nbasis^2) for each Vxc work-item
+ * length (16, 32, 64, 128 batches), keeps the fastest; chunk_out gets it,
+ * ms_out[4] the times. Needs the AO tables in HBM (cached or mixed form). */
+int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double* V_dev, int reps, int* chunk_out,
+                      float* ms_out);
 /* Statistics of the batch layout: batches, points, AO tiles per batch (mean),
  * algorithmic FLOPs of one XC build (dense-contraction count, k*2*sum g_b n_b^2). */
 int chx_grid_stats(const chx_grid* g, long* nbatches, long* npoints, double* mean_tiles, double* flops_per_build);
