@@ -246,11 +246,36 @@ __device__ __forceinline__ void radial_f32(const SegRec& rec, float r2, float& A
 // wave: no divergence, no LDS, no block barrier. The offset from the centre
 // is taken in FP64 (points and centres are tens of bohr apart, the offsets
 // that matter are small), everything after it in FP32 with the hardware exp.
-template <class Op>
+// The basis tables are read through the constant address space: they never
+// change while a kernel runs, and so are always read with scalar loads (a
+// kernel that also issues LDS DMA loads would otherwise get vector loads).
+__device__ __forceinline__ SegRec load_segrec(const SegRec* recs, int k) {
+  using cvec = const __attribute__((address_space(4))) u32x4;
+  static_assert(sizeof(SegRec) == 8 * sizeof(u32x4), "SegRec is 128 bytes");
+  u32x4 w[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = ((cvec*)(recs + k))[i];
+  SegRec r;
+  __builtin_memcpy(&r, w, sizeof(SegRec));
+  return r;
+}
+
+// PREFETCH: the next record is loaded while the current one is evaluated
+// (for an Op that reads no LDS, whose waits would take the record load along).
+template <bool PREFETCH = false, class Op>
 __device__ __forceinline__ void walk_tile_f32(const BasisView& bv, int t, double px, double py, double pz, Op& op) {
-  const int s0 = bv.tile_seg_off[t], s1 = bv.tile_seg_off[t + 1];
+  using cint = const __attribute__((address_space(4))) int;
+  const int s0 = ((cint*)bv.tile_seg_off)[t], s1 = ((cint*)bv.tile_seg_off)[t + 1];
+  SegRec nxt;
+  if (PREFETCH && s0 < s1) nxt = load_segrec(bv.segrec, s0);
   for (int k = s0; k < s1; ++k) {
-    const SegRec rec = bv.segrec[k];
+    SegRec rec;
+    if (PREFETCH) {
+      rec = nxt;
+      nxt = load_segrec(bv.segrec, k + 1 < s1 ? k + 1 : k);
+    } else {
+      rec = load_segrec(bv.segrec, k);
+    }
     const float x = (float)(px - rec.ax), y = (float)(py - rec.ay), z = (float)(pz - rec.az);
     const float r2 = x * x + y * y + z * z;
     float A0 = 0.f, A1 = 0.f;
@@ -282,17 +307,23 @@ __device__ __forceinline__ void walk_tile_f32(const BasisView& bv, int t, double
 // Density epilogue on one tile: sum_nu X[g][nu] phi_nu and sum_nu X[g][nu] grad phi_nu
 // (MODE 0), or sum_nu X_r[g][nu] d_r phi_nu (MODE r), as
 // R sum X Q + grad R sum X Q and R sum X grad Q per shell.
+typedef float f32x32 __attribute__((ext_vector_type(32)));
 template <int MODE>
 struct DensEpi {
-  const float* xq;  // X^T of this tile at this lane's point: xq[col * (kBatch + 1)]
+  f32x32 xv;  // X of this tile at this lane's point, in registers (uniform column index: no memory access)
   float o[4] = {0.f, 0.f, 0.f, 0.f};
   float R, Rx, Ry, Rz, S0, Sx, Sy, Sz;
+  // xq: X^T of this tile at this lane's point, xq[col * (kBatch + 1)]
+  __device__ void load(const float* xq) {
+#pragma unroll
+    for (int c = 0; c < kTile; ++c) xv[c] = xq[c * (kBatch + 1)];
+  }
   __device__ void begin(float a, float ax, float ay, float az) {
     R = a; Rx = ax; Ry = ay; Rz = az;
     S0 = Sx = Sy = Sz = 0.f;
   }
   __device__ void comp(int col, float q, float qx, float qy, float qz) {
-    const float x = xq[col * (kBatch + 1)];
+    const float x = xv[col];
     S0 += x * q;
     if (MODE == 0 || MODE == 1) Sx += x * qx;
     if (MODE == 0 || MODE == 2) Sy += x * qy;
@@ -318,20 +349,28 @@ struct DensEpi {
 // Density. Block = (batch, group of <= 4 alive tiles as output columns), 4
 // waves in a 2 x 2 grid: wave (wr, wc) owns rows g = 32 wr..+31 and columns
 // 64 wc..+63 of X = Phi_b D_b (64 x 128). K loop over the batch's alive tiles
-// (32 AOs = one MFMA K step): the A operand (phi rows of the wave) comes
-// straight from the cache into registers; the D block (128 nu x 32 mu, hi and
-// lo) is staged in LDS, double-buffered, the next step prefetched into
-// registers while the MFMAs of the current one run. Epilogue: X goes to LDS
-// and wave q contracts tile q's columns with phi / grad phi recomputed in
-// FP32, one lane per point.
+// (32 AOs = one MFMA K step). A K step's operands (A: 64 points x 32 AOs, D:
+// 128 columns x 32 AOs, hi and lo) are 24 fragments of 1 KB that the waves'
+// DMA loads (global_load_lds_dwordx4: lane i -> bytes 16 i of the fragment)
+// bring from memory straight into LDS, in the order in which the MFMA lanes
+// read them (lane i reads bytes 16 i: no bank conflicts, no VGPR staging).
+// Per step: wait for the step's fragments, read the wave's 12 into registers,
+// barrier, start the DMA of the next step into the same buffer, run the
+// wave's 24 MFMAs while it lands (one 24 KB buffer, so three blocks share a
+// CU and cover each other's barriers). Epilogue: X goes to LDS, wave q loads
+// tile q's columns at its lane's point into registers and contracts them with
+// phi / grad phi recomputed in FP32.
+constexpr int kDxFrag = 64;  // u32x4 per fragment
+constexpr int kDxA = 8;      // A fragments per step: [row block][hl]
+constexpr int kDxB = 16;     // D fragments per step: [column block][hl]
 struct DensMxSmem {
   union {
-    u32x4 B[2][2][kGroup * kTile][4];  // [buf][hl][nu][mu chunk, swizzled]
-    float X[kGroup * kTile][kBatch + 1];     // epilogue: X^T [nu][g]
+    u32x4 F[(kDxA + kDxB) * kDxFrag];      // one K step's fragments
+    float X[kGroup * kTile][kBatch + 1];   // epilogue: X^T [nu][g]
   };
   float red[kGroup][4][kBatch];
   double px[kBatch], py[kBatch], pz[kBatch];
-  int tl[kMxMaxTiles];  // the batch's alive tiles (K steps), staged once
+  int tl[kMxMaxTiles];  // the batch's alive tiles (K steps)
 };
 
 // MODE 0: A = phi; epilogue rho and grad rho / 2 (4 components into part).
@@ -340,8 +379,9 @@ struct DensMxSmem {
 // every pointer in it is known to be global and is read with scalar loads);
 // PACK = true: descriptors of many grids in device memory, items select them.
 template <int MODE, bool PACK>
-__global__ __launch_bounds__(256, 3) void k_density_mx(MolDev marg, const MolDev* __restrict__ mols,
+__global__ __launch_bounds__(256, 4) void k_density_mx(MolDev marg, const MolDev* __restrict__ mols,
                                                        const int2* __restrict__ items) {
+  using lds_ptr = __attribute__((address_space(3))) void*;
   constexpr int NOUT = MODE == 0 ? 4 : 1;
   constexpr int grp_stride = (MODE == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
   __shared__ DensMxSmem sm;
@@ -349,11 +389,10 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(MolDev marg, const MolDev
   const int2 wi = PACK ? items[item] : make_int2(0, item);
   const MolDev& m = PACK ? mols[wi.x] : marg;
   const int grp = wi.y;
-  const u16* __restrict__ cache = m.cache;
+  const u16* __restrict__ cache = m.cache + (size_t)MODE * kMxT;
   const u16* __restrict__ Ds = m.Ds;
-  const int sstride = m.sstride, npad = m.npad;
-  const int* __restrict__ bt_list = m.bt_list;
-  const int* __restrict__ grp_tiles = m.grp_tiles;
+  const size_t sstride = (size_t)m.sstride;
+  const int npad = m.npad;
   double* __restrict__ part = MODE == 0 ? m.part : m.part_tau + (MODE - 1) * kBatch;
   const BasisView bv = m.bv;  // descriptor fields in registers, read once
   const int tid = threadIdx.x, lane = tid & 63;
@@ -364,46 +403,40 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(MolDev marg, const MolDev
   const int nt = m.grp_ntiles[grp];
   int jt[kGroup];
 #pragma unroll
-  for (int q = 0; q < kGroup; ++q) jt[q] = grp_tiles[grp * kGroup + q];
-  const int it0 = m.bt_off[b], it1 = m.bt_off[b + 1];
-  const size_t np2 = (size_t)npad * npad;
+  for (int q = 0; q < kGroup; ++q) jt[q] = m.grp_tiles[grp * kGroup + q];
+  const int it0 = m.bt_off[b], nk = m.bt_off[b + 1] - it0;  // the batch's slots it0.. are its K steps
   if (tid < kBatch) {
     sm.px[tid] = m.bx[b * kBatch + tid];
     sm.py[tid] = m.by[b * kBatch + tid];
     sm.pz[tid] = m.bz[b * kBatch + tid];
   }
-  for (int j = tid; j < it1 - it0; j += 256) sm.tl[j] = bt_list[it0 + j];
-  // Per thread: the D rows (columns of X) it stages for j = 0..3; columns of
-  // absent tiles read a present tile's rows and are stored as zeros, so that
-  // every load is unconditional (and the prefetch is not serialised).
-  size_t dcol[4];
-  bool dlive[4];
+  for (int j = tid; j < nk; j += 256) sm.tl[j] = m.bt_list[it0 + j];
+  // The wave's fragments are f = wave + 4 j. A (j = 0, 1): row block f >> 1,
+  // hl f & 1. D (j = 2..5, f' = f - 8): column block c = f' >> 1 (tile c >> 1,
+  // half c & 1), hl f' & 1; columns past the group's tiles read a present
+  // tile's rows (their X columns are never used).
+  int offa[2], offb[4];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int f = wave + 4 * j;
+    offa[j] = (16 * (f >> 1) + lrow) * kMxRow + (f & 1) * kTile + 8 * lk;
+  }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int e = tid + 256 * j;
-    const int n = (e >> 2) & 127, q = n >> 5;
-    dlive[j] = q < nt;
-    dcol[j] = (size_t)(e >> 9) * np2 + (size_t)(jt[dlive[j] ? q : 0] * kTile + (n & 31)) * npad + 8 * (e & 3);
+    const int f = wave + 4 * j, c = f >> 1, q = c >> 1;
+    const int t = q == 1 ? jt[1] : q == 2 ? jt[2] : q == 3 ? jt[3] : jt[0];
+    offb[j] = (f & 1) * npad * npad + ((q < nt ? t : jt[0]) * kTile + 16 * (c & 1) + lrow) * npad + 8 * lk;
   }
   __syncthreads();
-
-  // Two register stages in flight: the loads for K step k + 2 are issued
-  // while the MFMAs of step k run.
-  struct Stage {
-    u32x4 a[2][2];  // A rows of this wave [row block][hl]
-    u32x4 d[4];     // this thread's share of the D block
-  };
-  auto load = [&](int it, Stage& S) {
-    const u16* src = cache + (size_t)it * sstride + (size_t)MODE * kMxT;
+  auto dma = [&](int k) {
+    const u16* a = cache + (size_t)(it0 + k) * sstride;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const u16* row = src + (32 * wr + 16 * r + lrow) * kMxRow + 8 * lk;
-      S.a[r][0] = *reinterpret_cast<const u32x4*>(row);
-      S.a[r][1] = *reinterpret_cast<const u32x4*>(row + kTile);
-    }
-    const int ti = sm.tl[it - it0];
+    for (int j = 0; j < 2; ++j)
+      __builtin_amdgcn_global_load_lds(a + offa[j], (lds_ptr)(sm.F + (wave + 4 * j) * kDxFrag), 16, 0, 0);
+    const int ti = sm.tl[k] * kTile;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) S.d[j] = *reinterpret_cast<const u32x4*>(Ds + dcol[j] + ti * kTile);
+    for (int j = 0; j < 4; ++j)
+      __builtin_amdgcn_global_load_lds(Ds + offb[j] + ti, (lds_ptr)(sm.F + (kDxA + wave + 4 * j) * kDxFrag), 16, 0, 0);
   };
 
   f32x4 acc[2][4];
@@ -411,51 +444,44 @@ __global__ __launch_bounds__(256, 3) void k_density_mx(MolDev marg, const MolDev
   for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-  auto step = [&](int it, Stage& S, int buf) {
+  const int ncb = 2 * (nt - 2 * wc);  // the wave's column blocks that hold group tiles (may be <= 0)
+  dma(0);
+  for (int k = 0; k < nk; ++k) {
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
+    __syncthreads();                     // ... and every other wave's
+    u32x4 a[2][2], bq[4][2];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j;
-      const int n = (e >> 2) & 127;
-      sm.B[buf][e >> 9][n][swz(n, e & 3)] = dlive[j] ? S.d[j] : u32x4{0u, 0u, 0u, 0u};
-    }
-    u32x4 ac[2][2];
+    for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      ac[r][0] = S.a[r][0];
-      ac[r][1] = S.a[r][1];
-    }
-    __syncthreads();
-    if (it + 2 < it1) load(it + 2, S);
-    // columns of absent tiles hold zeros in LDS, so every wave runs all blocks
+      for (int h = 0; h < 2; ++h) a[r][h] = sm.F[((2 * wr + r) * 2 + h) * kDxFrag + lane];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-      const int n = 64 * wc + 16 * cb + lrow;
-      const u32x4 bh = sm.B[buf][0][n][swz(n, lk)], bl = sm.B[buf][1][n][swz(n, lk)];
+    for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int r = 0; r < 2; ++r) acc[r][cb] = mfma3(ac[r][0], ac[r][1], bh, bl, acc[r][cb]);
-    }
-  };
-  Stage S0, S1;
-  if (it0 < it1) load(it0, S0);
-  if (it0 + 1 < it1) load(it0 + 1, S1);
-  for (int it = it0; it < it1; it += 2) {  // LDS buffer 0 for even steps, 1 for odd
-    step(it, S0, 0);
-    if (it + 1 < it1) step(it + 1, S1, 1);
+      for (int h = 0; h < 2; ++h) bq[c][h] = sm.F[(kDxA + (4 * wc + c) * 2 + h) * kDxFrag + lane];
+    __syncthreads();  // every wave holds its fragments: the buffer is free
+    if (k + 1 < nk) dma(k + 1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c < ncb) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) acc[r][c] = mfma3(a[r][0], a[r][1], bq[c][0], bq[c][1], acc[r][c]);
+      }
   }
 
-  // Epilogue. X[g = 32 wr + 16 r + 4 lk + i][n = 64 wc + 16 cb + lrow] -> LDS as X^T.
-  __syncthreads();  // every wave is done with the D blocks
+  // Epilogue. X[g = 32 wr + 16 r + 4 lk + i][n = 64 wc + 16 c + lrow] -> LDS as X^T.
+  __syncthreads();  // every wave is done with the fragments
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
+    for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) sm.X[64 * wc + 16 * cb + lrow][32 * wr + 16 * r + 4 * lk + i] = acc[r][cb][i];
+      for (int i = 0; i < 4; ++i) sm.X[64 * wc + 16 * c + lrow][32 * wr + 16 * r + 4 * lk + i] = acc[r][c][i];
   __syncthreads();
   DensEpi<MODE> epi;
   if (wave < nt) {  // wave q = tile q of the group; lane = point
-    epi.xq = &sm.X[kTile * wave][lane];
-    walk_tile_f32(bv, grp_tiles[grp * kGroup + wave], sm.px[lane], sm.py[lane], sm.pz[lane], epi);
+    const int t = wave == 1 ? jt[1] : wave == 2 ? jt[2] : wave == 3 ? jt[3] : jt[0];
+    epi.load(&sm.X[kTile * wave][lane]);
+    walk_tile_f32<true>(bv, t, sm.px[lane], sm.py[lane], sm.pz[lane], epi);
   }
 #pragma unroll
   for (int c = 0; c < NOUT; ++c) sm.red[wave][c][lane] = epi.o[c];
