@@ -352,8 +352,8 @@ struct DensEpi {
 // (32 AOs = one MFMA K step). A K step's operands (A: 64 points x 32 AOs, D:
 // 128 columns x 32 AOs, hi and lo) are 24 fragments of 1 KB that the waves'
 // DMA loads (global_load_lds_dwordx4: lane i -> bytes 16 i of the fragment)
-// bring from memory straight into LDS, in the order in which the MFMA lanes
-// read them (lane i reads bytes 16 i: no bank conflicts, no VGPR staging).
+// bring from memory straight into LDS (no VGPR staging), coalesced, in a
+// swizzled order from which the MFMA lanes read without bank conflicts.
 // Per step: wait for the step's fragments, read the wave's 12 into registers,
 // barrier, start the DMA of the next step into the same buffer, run the
 // wave's 24 MFMAs while it lands (one 24 KB buffer, so three blocks share a
@@ -414,18 +414,24 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev marg, const MolDev
   // The wave's fragments are f = wave + 4 j. A (j = 0, 1): row block f >> 1,
   // hl f & 1. D (j = 2..5, f' = f - 8): column block c = f' >> 1 (tile c >> 1,
   // half c & 1), hl f' & 1; columns past the group's tiles read a present
-  // tile's rows (their X columns are never used).
+  // tile's rows (their X columns are never used). A fragment is 16 rows of
+  // 64 bytes; DMA lane i fetches row i >> 2, 16-byte chunk (i & 3) ^ sw(row)
+  // (four neighbouring lanes read one contiguous 64-byte run: coalesced), and
+  // the MFMA lane that needs (row, chunk) reads it back at frag_idx(row, chunk)
+  // (the XOR keeps those LDS reads free of bank conflicts).
+  const int drow = lane >> 2, dch = (lane & 3) ^ ((drow >> 2) & 3);
+  const int fidx = 4 * lrow + (lk ^ ((lrow >> 2) & 3));  // this lane's MFMA operand in a fragment
   int offa[2], offb[4];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int f = wave + 4 * j;
-    offa[j] = (16 * (f >> 1) + lrow) * kMxRow + (f & 1) * kTile + 8 * lk;
+    offa[j] = (16 * (f >> 1) + drow) * kMxRow + (f & 1) * kTile + 8 * dch;
   }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int f = wave + 4 * j, c = f >> 1, q = c >> 1;
     const int t = q == 1 ? jt[1] : q == 2 ? jt[2] : q == 3 ? jt[3] : jt[0];
-    offb[j] = (f & 1) * npad * npad + ((q < nt ? t : jt[0]) * kTile + 16 * (c & 1) + lrow) * npad + 8 * lk;
+    offb[j] = (f & 1) * npad * npad + ((q < nt ? t : jt[0]) * kTile + 16 * (c & 1) + drow) * npad + 8 * dch;
   }
   __syncthreads();
   auto dma = [&](int k) {
@@ -453,11 +459,11 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev marg, const MolDev
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-      for (int h = 0; h < 2; ++h) a[r][h] = sm.F[((2 * wr + r) * 2 + h) * kDxFrag + lane];
+      for (int h = 0; h < 2; ++h) a[r][h] = sm.F[((2 * wr + r) * 2 + h) * kDxFrag + fidx];
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int h = 0; h < 2; ++h) bq[c][h] = sm.F[(kDxA + (4 * wc + c) * 2 + h) * kDxFrag + lane];
+      for (int h = 0; h < 2; ++h) bq[c][h] = sm.F[(kDxA + (4 * wc + c) * 2 + h) * kDxFrag + fidx];
     __syncthreads();  // every wave holds its fragments: the buffer is free
     if (k + 1 < nk) dma(k + 1);
 #pragma unroll
