@@ -609,106 +609,90 @@ void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, c
 
 // ---------------------------------------------------------------------------
 // Vxc: M_IJ += sum_g Phi_I[g]^T B_J[g] over the work item's batches, for
-// 128-AO super-tiles I, J. A stage is 32 points of one component: Phi^T_I
-// (128 mu x 32 g) and B^T_J (128 nu x 32 g), hi and lo, staged in LDS and
-// double-buffered. 4 waves in a 2 x 2 grid, each owning a 64 x 64 block of M.
-// One FP64 atomic add per element per work item.
+// 128-AO super-tiles I, J. A K step is 32 points of one component: Phi^T_I
+// (128 mu x 32 g) and B^T_J (128 nu x 32 g), hi and lo, 32 fragments of 1 KB
+// (16 rows of 64 bytes) that the waves' DMA loads bring straight into LDS,
+// coalesced and swizzled as in the density kernel. One buffer: wait for the
+// step's fragments, read the wave's 16 into registers, barrier, start the DMA
+// of the next step, run the wave's 48 MFMAs. 4 waves in a 2 x 2 grid, each
+// owning a 64 x 64 block of M. One FP64 atomic add per element per work item.
+constexpr int kVxA = 16;  // Phi^T fragments per step: [row block (8)][hl]
 struct VxcMxSmem {
-  u32x4 A[2][2][kMxSup * kTile][4];  // [buf][hl][mu][g chunk, swizzled]
-  u32x4 B[2][2][kMxSup * kTile][4];
-  int sl[kMxMaxChunk][8];            // per batch of the item: slots of I's and J's tiles
+  u32x4 F[2 * kVxA * kDxFrag];  // one K step's fragments: Phi^T_I, then B^T_J
+  int sl[kMxMaxChunk][8];       // per batch of the item: slots of I's and J's tiles
 };
 
-// Stage k -> (batch, half, component): this thread's 4 chunks of Phi^T_I and
-// of B^T_J (thread t moves, for tile t / 64 of each super-tile, the 16-byte
-// chunks e = t % 64 + 64 j of that tile: hl = e >> 7, row (e >> 2) & 31, chunk e & 3).
-template <int NC>
-__device__ __forceinline__ void vxc_load(const int (*sl)[8], int k, int tj, int lane, const u16* __restrict__ cache,
-                                         const u16* __restrict__ bt, int sstride, int toff, u32x4* A, u32x4* Q) {
-  const int it = k / (2 * NC), h = (k / NC) % 2, c = k % NC;
-  const int si = sl[it][tj], sj = sl[it][4 + tj];
-  const u16* pa = cache + (size_t)si * sstride + toff + c * kMxT + h * 32;
-  const u16* pb = bt + ((size_t)sj * NC + c) * kMxT + h * 32;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int e = lane + 64 * j;
-    const int off = (e >> 7) * (kTile * kBatch) + ((e >> 2) & 31) * kBatch + 8 * (e & 3);
-    A[j] = *reinterpret_cast<const u32x4*>(pa + off);
-    Q[j] = *reinterpret_cast<const u32x4*>(pb + off);
-  }
-}
-
-template <int NC>
-__device__ __forceinline__ void vxc_step(VxcMxSmem& sm, int k, int nstage, int tj, int lane, int wr, int wc,
-                                         const u16* __restrict__ cache, const u16* __restrict__ bt, int sstride,
-                                         int toff, u32x4* A, u32x4* Q, int buf, f32x4 (&acc)[4][4]) {
-  const int lrow = lane & 15, lk = lane >> 4;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int e = lane + 64 * j;
-    const int hl = e >> 7, row = tj * kTile + ((e >> 2) & 31), ch = swz(row, e & 3);
-    sm.A[buf][hl][row][ch] = A[j];
-    sm.B[buf][hl][row][ch] = Q[j];
-  }
-  __syncthreads();
-  // always issue the loads (past the end: a valid stage again), so that the
-  // stage registers are written unconditionally
-  vxc_load<NC>(sm.sl, k + 2 < nstage ? k + 2 : nstage - 1, tj, lane, cache, bt, sstride, toff, A, Q);
-  u32x4 ah[4], al[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = 64 * wr + 16 * r + lrow;
-    ah[r] = sm.A[buf][0][row][swz(row, lk)];
-    al[r] = sm.A[buf][1][row][swz(row, lk)];
-  }
-#pragma unroll
-  for (int cb = 0; cb < 4; ++cb) {
-    const int col = 64 * wc + 16 * cb + lrow;
-    const u32x4 bh = sm.B[buf][0][col][swz(col, lk)];
-    const u32x4 bl = sm.B[buf][1][col][swz(col, lk)];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r][cb] = mfma3(ah[r], al[r], bh, bl, acc[r][cb]);
-  }
-}
-
 template <int NC, bool PACK>
-__global__ __launch_bounds__(256, 2) void k_vxc_mx(MolDev marg, const MolDev* __restrict__ mols,
+__global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev marg, const MolDev* __restrict__ mols,
                                                    const int2* __restrict__ items) {
+  using lds_ptr = __attribute__((address_space(3))) void*;
   __shared__ VxcMxSmem sm;
   const int item = xcd_remap(blockIdx.x, gridDim.x);
   const int2 wi = PACK ? items[item] : make_int2(0, item);
   const MolDev& m = PACK ? mols[wi.x] : marg;
   const int4 wk = m.cwork[wi.y];
   const int I = wk.x, J = wk.y;
-  const u16* __restrict__ cache = m.cache;
+  const u16* __restrict__ cache = m.cache + m.toff;
   const u16* __restrict__ bt = m.bt;
-  const int sstride = m.sstride, toff = m.toff, npad = m.npad;
+  const size_t sstride = (size_t)m.sstride;
+  const int npad = m.npad;
   double* __restrict__ Mint = m.Mint;
   {  // the item's slot table (absent tiles point at the all-zero slot)
     const int* __restrict__ cs = m.cslot + (size_t)wk.z * 8;
     for (int e = threadIdx.x; e < (wk.w - wk.z) * 8; e += 256) sm.sl[e >> 3][e & 7] = cs[e];
-    __syncthreads();
   }
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lrow = lane & 15, lk = lane >> 4;
   const int wr = wave & 1, wc = wave >> 1;
-  // Thread t moves, for tile t / 64 of each super-tile, the 16-byte chunks
-  // e = t % 64 + 64 j (j = 0..3) of that tile: hl = e >> 7, row (e >> 2) & 31, chunk e & 3.
-  const int tj = tid >> 6;
+  // The wave's fragments are f = wave + 4 j, j = 0..3 of Phi^T_I and of B^T_J:
+  // row block f >> 1 (tile j = f >> 2, rows 16 ((f >> 1) & 1)..+15 of it), hl
+  // f & 1. DMA lane i fetches row i >> 2, chunk (i & 3) ^ sw(row); the MFMA
+  // lane that needs (row, chunk) reads it back at fidx.
+  const int drow = lane >> 2, dch = (lane & 3) ^ ((drow >> 2) & 3);
+  const int fidx = 4 * lrow + (lk ^ ((lrow >> 2) & 3));
+  int off[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int f = wave + 4 * j;
+    off[j] = (f & 1) * (kTile * kBatch) + (16 * ((f >> 1) & 1) + drow) * kBatch + 8 * dch;
+  }
+  __syncthreads();
+  // K step k -> (batch of the item, half of its points, component)
+  auto dma = [&](int k) {
+    const int it = k / (2 * NC), h = (k / NC) % 2, c = k % NC;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const u16* pa = cache + (size_t)sm.sl[it][j] * sstride + c * kMxT + h * 32;
+      const u16* pb = bt + ((size_t)sm.sl[it][4 + j] * NC + c) * kMxT + h * 32;
+      __builtin_amdgcn_global_load_lds(pa + off[j], (lds_ptr)(sm.F + (wave + 4 * j) * kDxFrag), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(pb + off[j], (lds_ptr)(sm.F + (kVxA + wave + 4 * j) * kDxFrag), 16, 0, 0);
+    }
+  };
   f32x4 acc[4][4];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int nstage = (wk.w - wk.z) * 2 * NC;
-  // Two register stages in flight: stage k + 2 loads while stage k computes.
-  u32x4 a0[4], q0[4], a1[4], q1[4];
-  vxc_load<NC>(sm.sl, 0, tj, lane, cache, bt, sstride, toff, a0, q0);  // items hold >= 1 batch: nstage >= 2
-  vxc_load<NC>(sm.sl, 1, tj, lane, cache, bt, sstride, toff, a1, q1);
-  for (int k = 0; k < nstage; k += 2) {  // LDS buffer 0 for even stages, 1 for odd
-    vxc_step<NC>(sm, k, nstage, tj, lane, wr, wc, cache, bt, sstride, toff, a0, q0, 0, acc);
-    if (k + 1 < nstage) vxc_step<NC>(sm, k + 1, nstage, tj, lane, wr, wc, cache, bt, sstride, toff, a1, q1, 1, acc);
+  const int nstage = (wk.w - wk.z) * 2 * NC;  // items hold >= 1 batch
+  dma(0);
+  for (int k = 0; k < nstage; ++k) {
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
+    __syncthreads();                     // ... and every other wave's
+    u32x4 a[4][2], bq[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        a[r][h] = sm.F[((4 * wr + r) * 2 + h) * kDxFrag + fidx];
+        bq[r][h] = sm.F[(kVxA + (4 * wc + r) * 2 + h) * kDxFrag + fidx];
+      }
+    __syncthreads();  // every wave holds its fragments: the buffer is free
+    if (k + 1 < nstage) dma(k + 1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r][c] = mfma3(a[r][0], a[r][1], bq[c][0], bq[c][1], acc[r][c]);
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r)
