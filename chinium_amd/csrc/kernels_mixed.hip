@@ -249,6 +249,20 @@ __device__ __forceinline__ void radial_f32(const SegRec& rec, float r2, float& A
 // The basis tables are read through the constant address space: they never
 // change while a kernel runs, and so are always read with scalar loads (a
 // kernel that also issues LDS DMA loads would otherwise get vector loads).
+// Read-only tables (work lists, per-grid descriptors of a pack) go through the
+// constant address space as well: scalar loads, uniform values.
+template <class T>
+__device__ __forceinline__ T ldc(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "4-byte words");
+  using cu = const __attribute__((address_space(4))) unsigned;
+  unsigned w[sizeof(T) / 4];
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = ((cu*)p)[i];
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
 __device__ __forceinline__ SegRec load_segrec(const SegRec* recs, int k) {
   using cvec = const __attribute__((address_space(4))) u32x4;
   static_assert(sizeof(SegRec) == 8 * sizeof(u32x4), "SegRec is 128 bytes");
@@ -386,8 +400,8 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev marg, const MolDev
   constexpr int grp_stride = (MODE == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
   __shared__ DensMxSmem sm;
   const int item = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 wi = PACK ? items[item] : make_int2(0, item);
-  const MolDev& m = PACK ? mols[wi.x] : marg;
+  const int2 wi = PACK ? ldc(items + item) : make_int2(0, item);
+  const MolDev m = PACK ? ldc(mols + wi.x) : marg;
   const int grp = wi.y;
   const u16* __restrict__ cache = m.cache + (size_t)MODE * kMxT;
   const u16* __restrict__ Ds = m.Ds;
@@ -559,8 +573,8 @@ __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev marg, c
   constexpr int W = NC == 4 ? 1 : 4;  // waves per block
   __shared__ u32x4 stage[W][NC * kMxT / 8];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
-  const int2 wi = PACK ? items[blockIdx.x] : make_int2(0, blockIdx.x * W);  // first slot of the block
-  const MolDev& m = PACK ? mols[wi.x] : marg;
+  const int2 wi = PACK ? ldc(items + blockIdx.x) : make_int2(0, blockIdx.x * W);  // first slot of the block
+  const MolDev m = PACK ? ldc(mols + wi.x) : marg;
   const int slot = wi.y + wave, nslots = m.nslots;
   const double* __restrict__ coef = m.coef;
   const BasisView bv = m.bv;  // descriptor fields in registers, read once
@@ -628,8 +642,8 @@ __global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev marg, const MolDev* __
   using lds_ptr = __attribute__((address_space(3))) void*;
   __shared__ VxcMxSmem sm;
   const int item = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 wi = PACK ? items[item] : make_int2(0, item);
-  const MolDev& m = PACK ? mols[wi.x] : marg;
+  const int2 wi = PACK ? ldc(items + item) : make_int2(0, item);
+  const MolDev m = PACK ? ldc(mols + wi.x) : marg;
   const int4 wk = m.cwork[wi.y];
   const int I = wk.x, J = wk.y;
   const u16* __restrict__ cache = m.cache + m.toff;
