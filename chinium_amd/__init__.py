@@ -92,6 +92,9 @@ def lib():
         "chx_grid_get_density_skeleton": (i, [_vp, _vp, i, i]),
         "chx_grid_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
         "chx_grid_energy_grad": (i, [_vp, _vp]),
+        "chx_grid_get_density_u": (i, [_vp, _vp, i, i, i]),
+        "chx_grid_fock_u": (i, [_vp, _vp, i, i, i]),
+        "chx_xc_eval_points_f": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp]),
         "chx_pack_create": (i, [_vp, i, P(_vp)]),
         "chx_pack_xc": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "chx_pack_set_graph": (i, [_vp, i]),
@@ -262,6 +265,22 @@ class Grid:
         _check(lib().chx_grid_autotune(self.h, xc.h, D_ptr, V_ptr, int(reps), ctypes.byref(c), ms))
         return c.value, dict(zip((16, 32, 64, 128), list(ms)))
 
+    def getDensityU(self, Ds):
+        """Grid::getDensityU (src/Grid/Grid.cpp:63-78): Ds[spin][mat], closed shell."""
+        Ds = [list(x) for x in Ds]
+        n = Ds[0][0].shape[0]
+        buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for x in Ds for D in x])
+        self._nmats_u = len(Ds[0])
+        _check(lib().chx_grid_get_density_u(self.h, _ptr(buf), len(Ds[0]), len(Ds), n))
+
+    def getFockU(self, nspin=1):
+        """Grid::getFockU<u_t> (src/Grid/Grid.cpp:193-216): response matrices [spin][mat]."""
+        n, m = self.nbasis, getattr(self, "_nmats_u", 0)
+        F = np.zeros(nspin * m * n * n)
+        _check(lib().chx_grid_fock_u(self.h, _ptr(F), m, nspin, n))
+        return [[F[(k * m + j) * n * n:(k * m + j + 1) * n * n].reshape((n, n), order="F").copy() for j in range(m)]
+                for k in range(nspin)]
+
     def getDensityDevice(self, ptr, nspin, nbasis):
         _check(lib().chx_grid_get_density_device(self.h, ctypes.c_void_p(ptr), int(nspin), int(nbasis)))
 
@@ -369,6 +388,15 @@ class ExchangeCorrelation:
         _check(lib().chx_xc_eval_points(self.h, n, _ptr(rho), _ptr(sg), _ptr(ta), _ptr(e), _ptr(vr), _ptr(vs),
                                         _ptr(vt)))
         return e, vr, vs, vt
+
+    def eval_points_f(self, rho, sigma=None):
+        """v2rho2, v2rhosigma, v2sigma2 at the given points (LDA/GGA), on the device."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        n = len(rho)
+        sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+        f0, f1, f2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().chx_xc_eval_points_f(self.h, n, _ptr(rho), _ptr(sg), _ptr(f0), _ptr(f1), _ptr(f2)))
+        return f0, f1, f2
 
     def Evaluate(self, order, grid):
         _check(lib().chx_xc_evaluate(self.h, grid.h, order.encode()))

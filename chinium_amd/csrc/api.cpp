@@ -394,6 +394,66 @@ int chx_grid_get_density_skeleton(chx_grid* g, const double* D, int nspin, int n
   });
 }
 
+int chx_grid_get_density_u(chx_grid* g, const double* D, int nmats, int nspin, int nbasis) {
+  return guard([&] {
+    check_dm_args(g, nspin, nbasis);
+    require(D != nullptr && nmats >= 1, CHX_ERR_ARG, "bad perturbed density matrices");
+    require(g->precision == CHX_PREC_FP64, CHX_ERR_UNSUPPORTED, "the response path is built for the FP64 form");
+    const size_t nslots = (size_t)g->nbatches * kBatch, nb2 = (size_t)nbasis * nbasis;
+    const int ng = (int)g->grp_batch.size();
+    double* part = g->d_part_u.alloc_get(std::max<size_t>(1, (size_t)ng * 4 * kBatch));
+    double* rhou = g->d_rhou.alloc_get(std::max<size_t>(1, (size_t)nmats * 4 * nslots));
+    for (int m = 0; m < nmats; ++m) {  // rho^x and grad rho^x of each matrix, per point
+      CHX_HIP(hipMemcpyAsync(g->d_Din.p, D + m * nb2, sizeof(double) * nb2, hipMemcpyHostToDevice, g->stream));
+      launch_permute_d(g->stream, g->nbasis, g->npad, g->d_Din.p, g->d_int2ref.p, g->d_Dint.p);
+      if (g->cached)
+        launch_density_c64(g->stream, 0, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
+                           g->d_bt_off.p, g->d_bt_list.p, g->d_cache.p, g->d_Dint.p, g->npad, part);
+      else
+        launch_density_f64(g->stream, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
+                           g->d_bt_list.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_Dint.p, g->npad, part);
+      launch_point_rho(g->stream, g->nbatches, g->d_grp_off.p, part, rhou + (size_t)m * 4 * nslots);
+    }
+    CHX_HIP(hipStreamSynchronize(g->stream));
+    g->nmats_u = nmats;
+  });
+}
+
+int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis) {
+  return guard([&] {
+    require(F != nullptr, CHX_ERR_ARG, "null output");
+    require(nspin == 1, CHX_ERR_UNSUPPORTED, "only the closed-shell path is built");
+    require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
+    require(g->fx_family >= 0, CHX_ERR_STATE, "Evaluate(\"f\") must precede Grid::getFockU");
+    require(g->nmats_u > 0, CHX_ERR_STATE, "Grid::getDensityU must precede Grid::getFockU");
+    require(nmats == g->nmats_u, CHX_ERR_ARG, "number of matrices differs from the last Grid::getDensityU");
+    const size_t nslots = (size_t)g->nbatches * kBatch, nb2 = (size_t)nbasis * nbasis;
+    double* coef = g->d_coef_u.alloc_get(std::max<size_t>(kCoef, kCoef * nslots));
+    for (int m = 0; m < nmats; ++m) {
+      launch_response_coef(g->stream, g->nbatches, g->d_rhou.p + (size_t)m * 4 * nslots, g->d_fx.p, g->d_w.p,
+                           g->fx_family, coef);
+      if (g->cached) {  // B = a phi + c . grad phi from the tables, M = Phi^T B, V = M + M^T
+        if (g->bmat_comps < 1) {
+          g->d_bmat.alloc(std::max<size_t>(1, g->bt_list.size() * (size_t)kBatch * kTile));
+          g->bmat_comps = 1;
+        }
+        launch_make_b(g->stream, 1, g->nbatches, g->d_bt_off.p, coef, g->d_cache.p, g->d_bmat.p);
+        launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
+        launch_vxc_c64(g->stream, 1, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p,
+                       g->ntiles, g->d_cache.p, g->d_bmat.p, g->d_Mint.p, g->npad);
+        launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, g->d_Vout.p);
+      } else {
+        launch_fill(g->stream, g->d_Vint.p, (size_t)g->npad * g->npad, 0.0);
+        launch_vxc_f64(g->stream, (int)g->vwork.size(), g->d_vwork.p, g->d_pair_batches.p, g->d_alive.p, g->ntiles,
+                       g->d_x.p, g->d_y.p, g->d_z.p, coef, g->view(), g->d_Vint.p, g->npad);
+        launch_finalize_v(g->stream, g->nbasis, g->npad, g->d_Vint.p, g->d_ref2int.p, g->d_Vout.p);
+      }
+      CHX_HIP(hipMemcpyAsync(F + m * nb2, g->d_Vout.p, sizeof(double) * nb2, hipMemcpyDeviceToHost, g->stream));
+    }
+    CHX_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
 int chx_grid_energy_grad(chx_grid* g, double* grad) {
   return guard([&] {
     require(grad != nullptr, CHX_ERR_ARG, "null output");
@@ -674,9 +734,21 @@ int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order) {
   return guard([&] {
     require(xc && g && order, CHX_ERR_ARG, "bad arguments");
     const std::string o(order);
-    require(o == "ev" || o == "e" || o == "v", CHX_ERR_UNSUPPORTED, "only the orders \"ev\", \"e\" and \"v\" are built");
+    require(o == "ev" || o == "e" || o == "v" || o == "f", CHX_ERR_UNSUPPORTED,
+            "only the orders \"ev\", \"e\", \"v\" and \"f\" are built");
     require(g->have_density, CHX_ERR_STATE, "Grid::getDensity must precede Evaluate");
     require(g->type >= xc->family, CHX_ERR_STATE, "grid type is below the functional family (call setType)");
+    if (o == "f") {  // f_xc at the ground-state density, for the response path
+      require(xc->family <= 1, CHX_ERR_UNSUPPORTED, "Evaluate(\"f\") is built for LDA and GGA functionals");
+      require(g->precision == CHX_PREC_FP64, CHX_ERR_UNSUPPORTED, "the response path is built for the FP64 form");
+      g->d_codes.upload(xc->codes, g->stream);
+      const size_t nslots = (size_t)g->nbatches * kBatch;
+      launch_functional_f(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_codes.p, (int)xc->codes.size(),
+                          xc->family, g->d_fx.alloc_get(std::max<size_t>(1, kFx * nslots)));
+      CHX_HIP(hipStreamSynchronize(g->stream));
+      g->fx_family = xc->family;
+      return;
+    }
     const bool e = o.find('e') != std::string::npos, v = o.find('v') != std::string::npos;
     require(xc->family < 2 || g->cached, CHX_ERR_UNSUPPORTED,
             "meta-GGA functionals need the AO tables in HBM (chx_grid_set_ao_cache)");
@@ -713,6 +785,28 @@ int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double
       if (h && n) CHX_HIP(hipMemcpyAsync(h, d.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     };
     down(exc, de); down(vrho, dvr); down(vsigma, dvs); down(vtau, dvt);
+    CHX_HIP(hipStreamSynchronize(s));
+    CHX_HIP(hipStreamDestroy(s));
+  });
+}
+
+int chx_xc_eval_points_f(const chx_xc* xc, long n, const double* rho, const double* sigma, double* v2rho2,
+                         double* v2rhosigma, double* v2sigma2) {
+  return guard([&] {
+    require(xc && n >= 0 && rho && v2rho2, CHX_ERR_ARG, "bad arguments");
+    require(xc->family <= 1, CHX_ERR_UNSUPPORTED, "second derivatives are built for LDA and GGA functionals");
+    hipStream_t s;
+    CHX_HIP(hipStreamCreate(&s));
+    const std::vector<double> hr(rho, rho + n), hs(sigma ? std::vector<double>(sigma, sigma + n) : std::vector<double>(n, 0.0));
+    DevBuf<double> dr, ds, f0, f1, f2;
+    DevBuf<int> dc;
+    dr.upload(hr, s); ds.upload(hs, s); dc.upload(xc->codes, s);
+    f0.alloc(std::max<long>(1, n)); f1.alloc(std::max<long>(1, n)); f2.alloc(std::max<long>(1, n));
+    launch_eval_functional_f(s, n, dc.p, (int)xc->codes.size(), dr.p, ds.p, f0.p, f1.p, f2.p);
+    auto down = [&](double* h, const DevBuf<double>& d) {
+      if (h && n) CHX_HIP(hipMemcpyAsync(h, d.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    };
+    down(v2rho2, f0); down(v2rhosigma, f1); down(v2sigma2, f2);
     CHX_HIP(hipStreamSynchronize(s));
     CHX_HIP(hipStreamDestroy(s));
   });

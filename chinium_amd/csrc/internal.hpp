@@ -123,6 +123,15 @@ void launch_vrho_coef(hipStream_t s, long nslots, const double* vrho_w, double* 
 // Cached-AO FP64 path (kernels_cached.hip).
 void launch_ao_cache(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* tile_nao,
                      const double* bx, const double* by, const double* bz, BasisView bv, double* cache);
+// response path (closed shell, LDA/GGA)
+constexpr int kFx = 8;  // per slot: v2rho2, v2rhosigma, v2sigma2, v_sigma, grad rho (3), -
+void launch_functional_f(hipStream_t s, int nbatches, const int* grp_off, const double* part, const int* codes,
+                         int ncodes, int family, double* fx);
+void launch_point_rho(hipStream_t s, int nbatches, const int* grp_off, const double* part, double* rhou);
+void launch_response_coef(hipStream_t s, int nbatches, const double* rhou, const double* fx, const double* bw,
+                          int family, double* coef);
+void launch_eval_functional_f(hipStream_t s, long n, const int* codes, int ncodes, const double* rho,
+                              const double* sigma, double* v2rho2, double* v2rhosigma, double* v2sigma2);
 void launch_density_c64(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
                         const int* grp_slots, const int* grp_ntiles, const int* bt_off, const int* bt_list,
                         const double* cache, const double* Dint, int npad, double* part);
@@ -276,6 +285,10 @@ struct chx_grid {
   chx::DevBuf<int> d_slot_grp, d_slot_q;
   bool have_skeleton = false;
   bool have_density = false, have_eval = false, have_v = false;
+  // response path: f_xc at the ground-state density, perturbed densities
+  chx::DevBuf<double> d_fx, d_part_u, d_rhou, d_coef_u;
+  int fx_family = -1;  // family of the f_xc on the grid (-1: none)
+  int nmats_u = 0;     // perturbed densities held in d_rhou
 
   // Optional per-kernel HIP-event timing (stage ids in chx_grid_kernel_times).
   bool profile = false;

@@ -362,6 +362,130 @@ void launch_eval_functional(hipStream_t s, long n, const int* codes, int ncodes,
   CHX_HIP(hipGetLastError());
 }
 
+// ---- Response path (closed shell, LDA/GGA). Replaces Evaluate("f")
+// (src/ExchangeCorrelation.cpp:101-104,126-133), SubGrid::getDensityU
+// (src/Grid/GridDensity.cpp:47-77) and SubGrid::getFockU<u_t>
+// (src/Grid/GridFock.cpp:99-160) for nspin = 1.
+
+// f_xc at the grid's ground-state density: per slot (v2rho2, v2rhosigma,
+// v2sigma2, v_sigma, grad rho), from the density partials.
+__global__ __launch_bounds__(256) void k_functional_f(int nslots, const int* __restrict__ grp_off,
+                                                      const double* __restrict__ part, const int* __restrict__ codes,
+                                                      int ncodes, int family, double* __restrict__ fx) {
+  const int slot = blockIdx.x * 256 + threadIdx.x;
+  if (slot >= nslots) return;
+  const int b = slot / kBatch, g = slot % kBatch;
+  double rho = 0, gx = 0, gy = 0, gz = 0;
+  for (int k = grp_off[b]; k < grp_off[b + 1]; ++k) {
+    const double* p = part + (size_t)k * 4 * kBatch;
+    rho += p[g];
+    gx += p[kBatch + g];
+    gy += p[2 * kBatch + g];
+    gz += p[3 * kBatch + g];
+  }
+  gx *= 2; gy *= 2; gz *= 2;
+  const double sigma = family >= 1 ? gx * gx + gy * gy + gz * gz : 0.0;
+  double vs, frr, frs, fss;
+  eval_codes_f(codes, ncodes, rho, sigma, &vs, &frr, &frs, &fss);
+  double* f = fx + (size_t)kFx * slot;
+  f[0] = frr;
+  f[1] = family >= 1 ? frs : 0.0;
+  f[2] = family >= 1 ? fss : 0.0;
+  f[3] = family >= 1 ? vs : 0.0;
+  f[4] = gx;
+  f[5] = gy;
+  f[6] = gz;
+}
+
+// rho^x and grad rho^x of one perturbed density matrix, per slot, from its partials.
+__global__ __launch_bounds__(256) void k_point_rho(int nslots, const int* __restrict__ grp_off,
+                                                   const double* __restrict__ part, double* __restrict__ rhou) {
+  const int slot = blockIdx.x * 256 + threadIdx.x;
+  if (slot >= nslots) return;
+  const int b = slot / kBatch, g = slot % kBatch;
+  double rho = 0, gx = 0, gy = 0, gz = 0;
+  for (int k = grp_off[b]; k < grp_off[b + 1]; ++k) {
+    const double* p = part + (size_t)k * 4 * kBatch;
+    rho += p[g];
+    gx += p[kBatch + g];
+    gy += p[2 * kBatch + g];
+    gz += p[3 * kBatch + g];
+  }
+  rhou[4 * (size_t)slot + 0] = rho;
+  rhou[4 * (size_t)slot + 1] = 2 * gx;
+  rhou[4 * (size_t)slot + 2] = 2 * gy;
+  rhou[4 * (size_t)slot + 3] = 2 * gz;
+}
+
+// Coefficients of the response Fock matrix for one perturbation, in the
+// layout of k_functional's (a phi + c . grad phi, then M + M^T):
+//   sigma^x = 2 grad rho . grad rho^x,
+//   a = 1/2 w (v2rho2 rho^x + v2rhosigma sigma^x),
+//   c = w (2 (v2rhosigma rho^x + v2sigma2 sigma^x) grad rho + 2 v_sigma grad rho^x).
+__global__ __launch_bounds__(256) void k_response_coef(int nslots, const double* __restrict__ rhou,
+                                                       const double* __restrict__ fx, const double* __restrict__ bw,
+                                                       int family, double* __restrict__ coef) {
+  const int slot = blockIdx.x * 256 + threadIdx.x;
+  if (slot >= nslots) return;
+  const double* f = fx + (size_t)kFx * slot;
+  const double* u = rhou + 4 * (size_t)slot;
+  const double w = bw[slot];
+  const double sx = 2.0 * (f[4] * u[1] + f[5] * u[2] + f[6] * u[3]);
+  double* c = coef + kCoef * (size_t)slot;
+  c[0] = 0.5 * w * (f[0] * u[0] + f[1] * sx);
+  if (family >= 1) {
+    const double q = 2.0 * w * (f[1] * u[0] + f[2] * sx), r = 2.0 * w * f[3];
+    c[1] = q * f[4] + r * u[1];
+    c[2] = q * f[5] + r * u[2];
+    c[3] = q * f[6] + r * u[3];
+  } else {
+    c[1] = c[2] = c[3] = 0.0;
+  }
+  c[4] = 0.0;
+}
+
+// f_xc at arbitrary (rho, sigma), as libxc's xc_{lda,gga}_fxc returns it.
+__global__ void k_eval_functional_f(long n, const int* __restrict__ codes, int ncodes, const double* __restrict__ rho,
+                                    const double* __restrict__ sigma, double* __restrict__ v2rho2,
+                                    double* __restrict__ v2rhosigma, double* __restrict__ v2sigma2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double vs;
+  eval_codes_f(codes, ncodes, rho[i], sigma[i], &vs, v2rho2 + i, v2rhosigma + i, v2sigma2 + i);
+}
+
+void launch_functional_f(hipStream_t s, int nbatches, const int* grp_off, const double* part, const int* codes,
+                         int ncodes, int family, double* fx) {
+  const int nslots = nbatches * kBatch;
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_functional_f, dim3((nslots + 255) / 256), dim3(256), 0, s, nslots, grp_off, part, codes, ncodes,
+                     family, fx);
+  CHX_HIP(hipGetLastError());
+}
+
+void launch_point_rho(hipStream_t s, int nbatches, const int* grp_off, const double* part, double* rhou) {
+  const int nslots = nbatches * kBatch;
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_point_rho, dim3((nslots + 255) / 256), dim3(256), 0, s, nslots, grp_off, part, rhou);
+  CHX_HIP(hipGetLastError());
+}
+
+void launch_response_coef(hipStream_t s, int nbatches, const double* rhou, const double* fx, const double* bw,
+                          int family, double* coef) {
+  const int nslots = nbatches * kBatch;
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_response_coef, dim3((nslots + 255) / 256), dim3(256), 0, s, nslots, rhou, fx, bw, family, coef);
+  CHX_HIP(hipGetLastError());
+}
+
+void launch_eval_functional_f(hipStream_t s, long n, const int* codes, int ncodes, const double* rho,
+                              const double* sigma, double* v2rho2, double* v2rhosigma, double* v2sigma2) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_eval_functional_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, codes, ncodes, rho,
+                     sigma, v2rho2, v2rhosigma, v2sigma2);
+  CHX_HIP(hipGetLastError());
+}
+
 // Deterministic final sum of the block partials (one block).
 __global__ __launch_bounds__(256) void k_reduce_en(int nparts, const double* __restrict__ en_part, double* __restrict__ en) {
   __shared__ double red[2][256];

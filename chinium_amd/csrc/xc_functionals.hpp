@@ -47,65 +47,131 @@ __device__ __forceinline__ D3 dasinh(const D3& a) { return dchain(a, ::asinh(a.v
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Second-order forward mode in the two variables (rho, sigma): value,
+// gradient and the three Hessian entries, for the kernel f_xc of the response
+// path (Evaluate("f")). The LDA/GGA energy densities below are templates on
+// the number type, so f_xc is the exact second derivative of the same code.
+struct H2 {
+  double v, g0, g1, h00, h01, h11;
+};
+__device__ __forceinline__ H2 hchain(const H2& a, double f, double f1, double f2) {
+  return {f, f1 * a.g0, f1 * a.g1, f1 * a.h00 + f2 * a.g0 * a.g0, f1 * a.h01 + f2 * a.g0 * a.g1,
+          f1 * a.h11 + f2 * a.g1 * a.g1};
+}
+__device__ __forceinline__ H2 operator+(const H2& a, const H2& b) {
+  return {a.v + b.v, a.g0 + b.g0, a.g1 + b.g1, a.h00 + b.h00, a.h01 + b.h01, a.h11 + b.h11};
+}
+__device__ __forceinline__ H2 operator-(const H2& a, const H2& b) {
+  return {a.v - b.v, a.g0 - b.g0, a.g1 - b.g1, a.h00 - b.h00, a.h01 - b.h01, a.h11 - b.h11};
+}
+__device__ __forceinline__ H2 operator-(const H2& a) { return {-a.v, -a.g0, -a.g1, -a.h00, -a.h01, -a.h11}; }
+__device__ __forceinline__ H2 operator*(const H2& a, const H2& b) {
+  return {a.v * b.v,
+          a.g0 * b.v + a.v * b.g0,
+          a.g1 * b.v + a.v * b.g1,
+          a.h00 * b.v + 2.0 * a.g0 * b.g0 + a.v * b.h00,
+          a.h01 * b.v + a.g0 * b.g1 + a.g1 * b.g0 + a.v * b.h01,
+          a.h11 * b.v + 2.0 * a.g1 * b.g1 + a.v * b.h11};
+}
+__device__ __forceinline__ H2 hinv(const H2& a) {
+  const double r = 1.0 / a.v;
+  return hchain(a, r, -r * r, 2.0 * r * r * r);
+}
+__device__ __forceinline__ H2 operator/(const H2& a, const H2& b) { return a * hinv(b); }
+__device__ __forceinline__ H2 operator+(const H2& a, double b) { return {a.v + b, a.g0, a.g1, a.h00, a.h01, a.h11}; }
+__device__ __forceinline__ H2 operator+(double a, const H2& b) { return b + a; }
+__device__ __forceinline__ H2 operator-(const H2& a, double b) { return {a.v - b, a.g0, a.g1, a.h00, a.h01, a.h11}; }
+__device__ __forceinline__ H2 operator-(double a, const H2& b) { return (-b) + a; }
+__device__ __forceinline__ H2 operator*(const H2& a, double b) {
+  return {a.v * b, a.g0 * b, a.g1 * b, a.h00 * b, a.h01 * b, a.h11 * b};
+}
+__device__ __forceinline__ H2 operator*(double a, const H2& b) { return b * a; }
+__device__ __forceinline__ H2 operator/(const H2& a, double b) { return a * (1.0 / b); }
+__device__ __forceinline__ H2 operator/(double a, const H2& b) { return hinv(b) * a; }
+__device__ __forceinline__ H2 dexp(const H2& a) { const double e = ::exp(a.v); return hchain(a, e, e, e); }
+__device__ __forceinline__ H2 dlog(const H2& a) { const double r = 1.0 / a.v; return hchain(a, ::log(a.v), r, -r * r); }
+__device__ __forceinline__ H2 dsqrt(const H2& a) {
+  const double s = ::sqrt(a.v);
+  return hchain(a, s, 0.5 / s, -0.25 / (s * a.v));
+}
+__device__ __forceinline__ H2 datan(const H2& a) {
+  const double q = 1.0 / (1.0 + a.v * a.v);
+  return hchain(a, ::atan(a.v), q, -2.0 * a.v * q * q);
+}
+__device__ __forceinline__ H2 dasinh(const H2& a) {
+  const double q = 1.0 / ::sqrt(1.0 + a.v * a.v);
+  return hchain(a, ::asinh(a.v), q, -a.v * q * q * q);
+}
+
+
 // Slater exchange (Dirac): e = -(3/4)(3/pi)^(1/3) rho^(4/3).
-__device__ __forceinline__ D3 e_lda_x(const D3& rho, const D3& rho13) {
+template <class T>
+__device__ __forceinline__ T e_lda_x(const T& rho, const T& rho13) {
   const double cx = -0.7385587663820224;  // -(3/4)(3/pi)^(1/3)
   return cx * rho * rho13;
 }
 
 // Wigner-Seitz radius from rho^(1/3): rs = (3/(4 pi))^(1/3) / rho^(1/3).
-__device__ __forceinline__ D3 rs_from_rho13(const D3& rho13) { return 0.6203504908994000 / rho13; }
+template <class T>
+__device__ __forceinline__ T rs_from_rho13(const T& rho13) { return 0.6203504908994000 / rho13; }
 
 // VWN paramagnetic interpolation formula (Vosko-Wilk-Nusair 1980, eq. 4.4), Hartree.
-__device__ __forceinline__ D3 vwn_eps(const D3& rs, double A, double b, double c, double x0) {
-  const D3 x = dsqrt(rs);
-  const D3 X = x * x + b * x + c;
+template <class T>
+__device__ __forceinline__ T vwn_eps(const T& rs, double A, double b, double c, double x0) {
+  const T x = dsqrt(rs);
+  const T X = x * x + b * x + c;
   const double Q = ::sqrt(4.0 * c - b * b);
   const double X0 = x0 * x0 + b * x0 + c;
-  const D3 at = datan(Q / (2.0 * x + b));
-  const D3 xm = x - x0;
+  const T at = datan(Q / (2.0 * x + b));
+  const T xm = x - x0;
   return A * (dlog(x * x / X) + (2.0 * b / Q) * at - (b * x0 / X0) * (dlog(xm * xm / X) + (2.0 * (b + 2.0 * x0) / Q) * at));
 }
-__device__ __forceinline__ D3 vwn5_eps(const D3& rs) { return vwn_eps(rs, 0.0310907, 3.72744, 12.9352, -0.10498); }
-__device__ __forceinline__ D3 vwn_rpa_eps(const D3& rs) { return vwn_eps(rs, 0.0310907, 13.0720, 42.7198, -0.409286); }
+template <class T>
+__device__ __forceinline__ T vwn5_eps(const T& rs) { return vwn_eps(rs, 0.0310907, 3.72744, 12.9352, -0.10498); }
+template <class T>
+__device__ __forceinline__ T vwn_rpa_eps(const T& rs) { return vwn_eps(rs, 0.0310907, 13.0720, 42.7198, -0.409286); }
 
 // PW92 paramagnetic correlation (Perdew-Wang 1992, eq. 10), libxc PW_MOD parameters.
-__device__ __forceinline__ D3 pw92_eps(const D3& rs) {
+template <class T>
+__device__ __forceinline__ T pw92_eps(const T& rs) {
   const double A = 0.0310907, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
-  const D3 srs = dsqrt(rs);
-  const D3 den = (2.0 * A) * (srs * (b1 + b3 * rs) + rs * (b2 + b4 * rs));
+  const T srs = dsqrt(rs);
+  const T den = (2.0 * A) * (srs * (b1 + b3 * rs) + rs * (b2 + b4 * rs));
   return (-2.0 * A) * (1.0 + a1 * rs) * dlog(1.0 + 1.0 / den);
 }
 
 // PBE exchange: e = e_x^LDA * (1 + kappa - kappa / (1 + mu s^2 / kappa)).
-__device__ __forceinline__ D3 e_pbe_x(const D3& rho, const D3& rho13, const D3& sigma, double kappa, double mu) {
+template <class T>
+__device__ __forceinline__ T e_pbe_x(const T& rho, const T& rho13, const T& sigma, double kappa, double mu) {
   // s^2 = sigma / (4 (3 pi^2)^(2/3) rho^(8/3))
-  const D3 r83 = rho * rho * rho13 * rho13;
-  const D3 s2 = sigma / (38.28312000250922 * r83);
-  const D3 F = (1.0 + kappa) - kappa / (1.0 + (mu / kappa) * s2);
+  const T r83 = rho * rho * rho13 * rho13;
+  const T s2 = sigma / (38.28312000250922 * r83);
+  const T F = (1.0 + kappa) - kappa / (1.0 + (mu / kappa) * s2);
   return e_lda_x(rho, rho13) * F;
 }
 
 // PBE correlation with zeta = 0: e = rho (eps_c^PW92 + H(rs, t)).
-__device__ __forceinline__ D3 e_pbe_c(const D3& rho, const D3& rho13, const D3& sigma, double beta) {
+template <class T>
+__device__ __forceinline__ T e_pbe_c(const T& rho, const T& rho13, const T& sigma, double beta) {
   const double gamma = 0.031090690869654895;  // (1 - ln 2) / pi^2
-  const D3 rs = rs_from_rho13(rho13);
-  const D3 ec = pw92_eps(rs);
+  const T rs = rs_from_rho13(rho13);
+  const T ec = pw92_eps(rs);
   // t^2 = sigma / (4 ks^2 rho^2), ks^2 = 4 kF / pi, kF = (3 pi^2 rho)^(1/3)
-  const D3 kf = 3.0936677262801355 * rho13;
-  const D3 t2 = sigma / ((16.0 / kPi) * kf * rho * rho);
-  const D3 A = (beta / gamma) / (dexp(-ec / gamma) - 1.0);
-  const D3 At2 = A * t2;
-  const D3 H = gamma * dlog(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  const T kf = 3.0936677262801355 * rho13;
+  const T t2 = sigma / ((16.0 / kPi) * kf * rho * rho);
+  const T A = (beta / gamma) / (dexp(-ec / gamma) - 1.0);
+  const T At2 = A * t2;
+  const T H = gamma * dlog(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
   return rho * (ec + H);
 }
 
 // Becke 88 exchange, spin-unpolarised: two equal spin channels rho/2.
-__device__ __forceinline__ D3 e_b88_x(const D3& rho, const D3& rho13, const D3& sigma) {
+template <class T>
+__device__ __forceinline__ T e_b88_x(const T& rho, const T& rho13, const T& sigma) {
   const double beta = 0.0042;
   // rho_s^(4/3) = (rho/2)^(4/3); x = |grad rho_s| / rho_s^(4/3)
-  const D3 rs43 = 0.39685026299204984 * rho * rho13;  // 2^(-4/3) rho^(4/3)
-  const D3 x = 0.5 * dsqrt(sigma) / rs43;
+  const T rs43 = 0.39685026299204984 * rho * rho13;  // 2^(-4/3) rho^(4/3)
+  const T x = 0.5 * dsqrt(sigma) / rs43;
   const double cx = 0.9305257363491000;  // (3/2) (3/(4 pi))^(1/3)
   return -2.0 * rs43 * (cx + beta * x * x / (1.0 + 6.0 * beta * x * dasinh(x)));
 }
@@ -113,14 +179,15 @@ __device__ __forceinline__ D3 e_b88_x(const D3& rho, const D3& rho13, const D3& 
 // LYP correlation, closed shell (Miehlich-Savin-Stoll-Preuss 1989, eq. 2 at
 // rho_a = rho_b = rho/2): e = -a rho/(1 + d rho^-1/3)
 //   - a b exp(-c rho^-1/3)/(1 + d rho^-1/3) [CF rho - rho^-5/3 sigma (1/24 + 7 delta/72)].
-__device__ __forceinline__ D3 e_lyp_c(const D3& rho, const D3& rho13, const D3& sigma) {
+template <class T>
+__device__ __forceinline__ T e_lyp_c(const T& rho, const T& rho13, const T& sigma) {
   const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
   const double CF = 2.8712340001881915;  // (3/10)(3 pi^2)^(2/3)
-  const D3 rm13 = 1.0 / rho13;
-  const D3 inv_den = 1.0 / (1.0 + d * rm13);
-  const D3 delta = rm13 * (c + d * inv_den);
-  const D3 ex = dexp(-c * rm13);
-  const D3 rm53 = rm13 * rm13 / rho;
+  const T rm13 = 1.0 / rho13;
+  const T inv_den = 1.0 / (1.0 + d * rm13);
+  const T delta = rm13 * (c + d * inv_den);
+  const T ex = dexp(-c * rm13);
+  const T rm53 = rm13 * rm13 / rho;
   return -a * rho * inv_den - (a * b) * ex * inv_den * (CF * rho - rm53 * sigma * (1.0 / 24.0 + (7.0 / 72.0) * delta));
 }
 
@@ -227,10 +294,10 @@ __host__ __device__ inline int code_family(int code) {
 __host__ __device__ inline double code_exx(int code) { return code == kHybB3lyp ? 0.2 : 0.0; }
 constexpr double kDensThreshold = 1e-15;
 
-__device__ __forceinline__ D3 energy_density(int code, const D3& rho, const D3& rho13, const D3& sigma, const D3& tau) {
+// LDA and GGA codes, on any number type (first or second order).
+template <class T>
+__device__ __forceinline__ T energy_density_lg(int code, const T& rho, const T& rho13, const T& sigma) {
   switch (code) {
-    case kMggaXR2scan: return e_r2scan_x(rho, rho13, sigma, tau);
-    case kMggaCR2scan: return e_r2scan_c(rho, rho13, sigma, tau);
     case kLdaX: return e_lda_x(rho, rho13);
     case kLdaCVwn: return rho * vwn5_eps(rs_from_rho13(rho13));
     case kLdaCVwnRpa: return rho * vwn_rpa_eps(rs_from_rho13(rho13));
@@ -243,7 +310,15 @@ __device__ __forceinline__ D3 energy_density(int code, const D3& rho, const D3& 
     case kHybB3lyp:
       return 0.08 * e_lda_x(rho, rho13) + 0.72 * e_b88_x(rho, rho13, sigma) +
              0.19 * (rho * vwn_rpa_eps(rs_from_rho13(rho13))) + 0.81 * e_lyp_c(rho, rho13, sigma);
-    default: return dconst(0.0);
+    default: return 0.0 * rho;
+  }
+}
+
+__device__ __forceinline__ D3 energy_density(int code, const D3& rho, const D3& rho13, const D3& sigma, const D3& tau) {
+  switch (code) {
+    case kMggaXR2scan: return e_r2scan_x(rho, rho13, sigma, tau);
+    case kMggaCR2scan: return e_r2scan_c(rho, rho13, sigma, tau);
+    default: return energy_density_lg(code, rho, rho13, sigma);
   }
 }
 
@@ -277,6 +352,34 @@ __device__ __forceinline__ void eval_codes(const int* codes, int ncodes, double 
   *vrho = vr;
   *vsigma = vs;
   *vtau = vt;
+}
+
+// Second derivatives of the summed LDA/GGA energy density at one point
+// (libxc's xc_*_fxc, src/ExchangeCorrelation.cpp:101-104,126-133): v2rho2,
+// v2rhosigma, v2sigma2, and v_sigma with them. Same threshold and sigma floor
+// as eval_codes; meta-GGA codes are not allowed on this path.
+__device__ __forceinline__ void eval_codes_f(const int* codes, int ncodes, double rho, double sigma, double* vsigma,
+                                             double* v2rho2, double* v2rhosigma, double* v2sigma2) {
+  double vs = 0, frr = 0, frs = 0, fss = 0;
+  if (rho >= kDensThreshold) {
+    const double sfloor = 1.0000000000000000e-40;
+    const double sg = sigma > sfloor ? sigma : sfloor;
+    const H2 r = {rho, 1.0, 0.0, 0.0, 0.0, 0.0};
+    const double r13 = ::cbrt(rho);
+    const H2 rho13 = hchain(r, r13, r13 / (3.0 * rho), -2.0 * r13 / (9.0 * rho * rho));
+    const H2 s = {sg, 0.0, 1.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < ncodes; ++k) {
+      const H2 ed = energy_density_lg(codes[k], r, rho13, s);
+      vs += ed.g1;
+      frr += ed.h00;
+      frs += ed.h01;
+      fss += ed.h11;
+    }
+  }
+  *vsigma = vs;
+  *v2rho2 = frr;
+  *v2rhosigma = frs;
+  *v2sigma2 = fss;
 }
 
 }  // namespace chx

@@ -15,6 +15,7 @@
 // matrix with the same data()/rows()/cols() accessors.
 #pragma once
 
+#include <algorithm>
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
@@ -156,6 +157,29 @@ class Grid {
     check(chx_grid_energy_grad(h_, g.data()));
     return g;
   }
+  // Response path, closed shell, LDA/GGA, FP64 form: Grid::getDensityU
+  // (src/Grid/Grid.cpp:63-78) takes D[spin][mat]; Grid::getFockU<u_t>
+  // (src/Grid/Grid.cpp:193-216) returns F[spin][mat]. Evaluate("f") first.
+  void getDensityU(std::vector<std::vector<EigenMatrix>> D) {
+    const int nspin = (int)D.size(), nmats = nspin ? (int)D[0].size() : 0;
+    std::vector<double> buf;
+    buf.reserve((size_t)nspin * nmats * nbasis_ * nbasis_);
+    for (const auto& spin : D)
+      for (const EigenMatrix& m : spin) buf.insert(buf.end(), m.data(), m.data() + (size_t)nbasis_ * nbasis_);
+    check(chx_grid_get_density_u(h_, buf.data(), nmats, nspin, nbasis_));
+    nmats_u_ = nmats;
+  }
+  std::vector<std::vector<EigenMatrix>> getFockU() {
+    std::vector<double> buf((size_t)nmats_u_ * nbasis_ * nbasis_);
+    check(chx_grid_fock_u(h_, buf.data(), nmats_u_, 1, nbasis_));
+    std::vector<std::vector<EigenMatrix>> F(1);
+    for (int m = 0; m < nmats_u_; ++m) {
+      F[0].push_back(make_matrix(nbasis_, nbasis_));
+      std::copy(buf.begin() + (size_t)m * nbasis_ * nbasis_, buf.begin() + (size_t)(m + 1) * nbasis_ * nbasis_,
+                F[0].back().data());
+    }
+    return F;
+  }
   // GPU options (no reference counterpart): CHX_PREC_FP64 or CHX_PREC_MIXED.
   void setPrecision(int precision) { check(chx_grid_set_precision(h_, precision)); }
   chx_grid* handle() { return h_; }
@@ -164,6 +188,7 @@ class Grid {
   chx_grid* h_ = nullptr;
   int nbasis_ = 0;
   int natoms_ = 0;
+  int nmats_u_ = 0;
 };
 
 class ExchangeCorrelation {

@@ -115,6 +115,18 @@ int chx_grid_fock_device(chx_grid* g, double* F_dev, int nspin, int nbasis);
  * GGA (the reference's getEnergyGrad has no tau term). */
 int chx_grid_get_density_skeleton(chx_grid* g, const double* D, int nspin, int nbasis);
 int chx_grid_energy_grad(chx_grid* g, double* grad);
+/* Response path (coupled-perturbed KS, src/HartreeFockKohnSham/CoupledPerturbed.cpp:133-134,
+ * Universal.h:92-118), closed shell, LDA/GGA, FP64 form:
+ * chx_xc_evaluate(xc, g, "f") after the ground-state density takes f_xc
+ * (Evaluate("f"), src/ExchangeCorrelation.cpp:101-104,126-133);
+ * chx_grid_get_density_u replaces Grid::getDensityU (src/Grid/Grid.cpp:63-78,
+ * SubGrid::getDensityU, src/Grid/GridDensity.cpp:47-77): D holds nspin x nmats
+ * column-major nbasis^2 matrices, back to back, [spin][mat] (host memory);
+ * chx_grid_fock_u replaces Grid::getFockU<u_t> (src/Grid/Grid.cpp:193-216,
+ * SubGrid::getFockU, src/Grid/GridFock.cpp:99-160) and writes the nspin x
+ * nmats response matrices in the same order. */
+int chx_grid_get_density_u(chx_grid* g, const double* D, int nmats, int nspin, int nbasis);
+int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis);
 /* Supply v_rho directly, in the library's point order, as the SAP guess does
  * when it writes SubGrid::Eps1Rho before Grid::getFock
  * (src/HartreeFockKohnSham/GuessSCF.cpp:37-60). Other potentials are zeroed. */
@@ -160,7 +172,8 @@ void chx_xc_destroy(chx_xc* xc);
 /* Members Family (0 LDA, 1 GGA, 2 mGGA), EXX, Codes (ExchangeCorrelation.h:9-12). */
 int chx_xc_info(const chx_xc* xc, int* family, double* exx, int* ncodes, int* codes);
 /* ExchangeCorrelation::Evaluate(std::string order, Grid&)
- *   src/ExchangeCorrelation.cpp:165-248. order: "ev", "e" or "v". */
+ *   src/ExchangeCorrelation.cpp:165-248. order: "ev", "e", "v", or "f" (second
+ *   derivatives for the response path; LDA/GGA, FP64 form). */
 int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order);
 
 /* Lebedev-Laikov angular rule with npoints points (x, y, z, w; sum w = 1), as
@@ -174,6 +187,11 @@ int chx_lebedev_rule(int npoints, double* xyzw);
  * derivatives over the functional's codes. Host arrays; sigma/tau may be NULL. */
 int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
                        double* exc, double* vrho, double* vsigma, double* vtau);
+/* Second derivatives at n points, as xc_{lda,gga}_fxc returns them
+ * (src/ExchangeCorrelation.cpp:101-104,126-133): v2rho2, v2rhosigma, v2sigma2
+ * summed over the codes. LDA/GGA; host arrays; sigma may be NULL (LDA). */
+int chx_xc_eval_points_f(const chx_xc* xc, long n, const double* rho, const double* sigma, double* v2rho2,
+                         double* v2rhosigma, double* v2sigma2);
 
 /* ---- Packs: many molecules per launch (no reference counterpart) ------------
  * Chinium runs one molecule per process (SelfConsistentField.cpp:27-33). A

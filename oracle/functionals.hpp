@@ -57,6 +57,67 @@ inline double asinh(double a) { return std::asinh(a); }
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Second-order forward mode in N variables (value, gradient, Hessian), for
+// f_xc (libxc's xc_*_fxc, src/ExchangeCorrelation.cpp:101-104,126-133).
+template <int N> struct Hyper {
+  double v, g[N], h[N][N];
+  Hyper() : v(0) { for (int i = 0; i < N; ++i) { g[i] = 0; for (int j = 0; j < N; ++j) h[i][j] = 0; } }
+  Hyper(double x) : Hyper() { v = x; }
+  static Hyper var(double x, int k) { Hyper r(x); r.g[k] = 1; return r; }
+};
+template <int N> Hyper<N> chain(const Hyper<N>& a, double f, double f1, double f2) {
+  Hyper<N> r; r.v = f;
+  for (int i = 0; i < N; ++i) {
+    r.g[i] = f1 * a.g[i];
+    for (int j = 0; j < N; ++j) r.h[i][j] = f1 * a.h[i][j] + f2 * a.g[i] * a.g[j];
+  }
+  return r;
+}
+template <int N> Hyper<N> operator+(const Hyper<N>& a, const Hyper<N>& b) {
+  Hyper<N> r; r.v = a.v + b.v;
+  for (int i = 0; i < N; ++i) { r.g[i] = a.g[i] + b.g[i]; for (int j = 0; j < N; ++j) r.h[i][j] = a.h[i][j] + b.h[i][j]; }
+  return r;
+}
+template <int N> Hyper<N> operator-(const Hyper<N>& a) {
+  Hyper<N> r; r.v = -a.v;
+  for (int i = 0; i < N; ++i) { r.g[i] = -a.g[i]; for (int j = 0; j < N; ++j) r.h[i][j] = -a.h[i][j]; }
+  return r;
+}
+template <int N> Hyper<N> operator-(const Hyper<N>& a, const Hyper<N>& b) { return a + (-b); }
+template <int N> Hyper<N> operator*(const Hyper<N>& a, const Hyper<N>& b) {
+  Hyper<N> r; r.v = a.v * b.v;
+  for (int i = 0; i < N; ++i) {
+    r.g[i] = a.g[i] * b.v + a.v * b.g[i];
+    for (int j = 0; j < N; ++j) r.h[i][j] = a.h[i][j] * b.v + a.g[i] * b.g[j] + a.g[j] * b.g[i] + a.v * b.h[i][j];
+  }
+  return r;
+}
+template <int N> Hyper<N> recip(const Hyper<N>& a) { const double q = 1.0 / a.v; return chain(a, q, -q * q, 2 * q * q * q); }
+template <int N> Hyper<N> operator/(const Hyper<N>& a, const Hyper<N>& b) { return a * recip(b); }
+template <int N> Hyper<N> operator+(const Hyper<N>& a, double b) { return a + Hyper<N>(b); }
+template <int N> Hyper<N> operator+(double a, const Hyper<N>& b) { return Hyper<N>(a) + b; }
+template <int N> Hyper<N> operator-(const Hyper<N>& a, double b) { return a - Hyper<N>(b); }
+template <int N> Hyper<N> operator-(double a, const Hyper<N>& b) { return Hyper<N>(a) - b; }
+template <int N> Hyper<N> operator*(const Hyper<N>& a, double b) { return a * Hyper<N>(b); }
+template <int N> Hyper<N> operator*(double a, const Hyper<N>& b) { return Hyper<N>(a) * b; }
+template <int N> Hyper<N> operator/(const Hyper<N>& a, double b) { return a * (1.0 / b); }
+template <int N> Hyper<N> operator/(double a, const Hyper<N>& b) { return recip(b) * a; }
+template <int N> Hyper<N> exp(const Hyper<N>& a) { const double e = std::exp(a.v); return chain(a, e, e, e); }
+template <int N> Hyper<N> log(const Hyper<N>& a) { return chain(a, std::log(a.v), 1.0 / a.v, -1.0 / (a.v * a.v)); }
+template <int N> Hyper<N> sqrt(const Hyper<N>& a) { const double q = std::sqrt(a.v); return chain(a, q, 0.5 / q, -0.25 / (q * a.v)); }
+template <int N> Hyper<N> pow(const Hyper<N>& a, double p) {
+  return chain(a, std::pow(a.v, p), p * std::pow(a.v, p - 1.0), p * (p - 1.0) * std::pow(a.v, p - 2.0));
+}
+template <int N> Hyper<N> atan(const Hyper<N>& a) {
+  const double q = 1.0 / (1.0 + a.v * a.v);
+  return chain(a, std::atan(a.v), q, -2.0 * a.v * q * q);
+}
+template <int N> Hyper<N> asinh(const Hyper<N>& a) {
+  const double q = 1.0 / std::sqrt(1.0 + a.v * a.v);
+  return chain(a, std::asinh(a.v), q, -a.v * q * q * q);
+}
+
+
 // ---- LDA exchange (libxc 1, Slater/Dirac): e = -(3/4)(3/pi)^(1/3) rho^(4/3)
 template <class T> T lda_x(const T& rho) {
   const double cx = -0.75 * std::cbrt(3.0 / kPi);
@@ -276,6 +337,23 @@ inline void eval_point(int code, double rho, double sigma, double tau,
   *vrho += e.d[0];
   *vsigma += e.d[1];
   *vtau += e.d[2];
+}
+
+// Second derivatives of one LDA/GGA code at one point (libxc xc_*_fxc):
+// accumulates v_sigma, v2rho2, v2rhosigma, v2sigma2 with the same threshold
+// and sigma floor as eval_point.
+inline void eval_point_f(int code, double rho, double sigma, double* vsigma, double* v2rho2, double* v2rhosigma,
+                         double* v2sigma2) {
+  const double thr = density_threshold(code);
+  if (!(rho >= thr)) return;
+  const double sthr = std::pow(thr, 4.0 / 3.0);
+  const double s = sigma > sthr * sthr ? sigma : sthr * sthr;
+  typedef Hyper<2> H;
+  const H e = energy_density(code, H::var(rho, 0), H::var(s, 1), H(0.0));
+  *vsigma += e.g[1];
+  *v2rho2 += e.h[0][0];
+  *v2rhosigma += e.h[0][1];
+  *v2sigma2 += e.h[1][1];
 }
 
 }  // namespace oxf

@@ -817,6 +817,62 @@ int ox_fock(ox_system* s, double* V) {
   return 0;
 }
 
+// Response Fock matrices (Evaluate("f") + Grid::getDensityU + Grid::getFockU<u_t>,
+// src/ExchangeCorrelation.cpp:101-104,126-133, GridDensity.cpp:47-77,
+// GridFock.cpp:99-160), closed shell, LDA/GGA, after ox_density at the
+// reference density. For each perturbed density matrix D^x:
+//   RhoU = sum D^x phi phi, Rho1U = 2 sum D^x grad phi phi, SigmaU = 2 Rho1 . Rho1U,
+//   TMP0 = v2rho2 RhoU + v2rhosigma SigmaU                 -> F0 (weight 1/2),
+//   TMP4 = 2 (v2rhosigma RhoU + v2sigma2 SigmaU) Rho1 + 2 v_sigma Rho1U -> F1,
+//   F = 1/2 F0 + F1, F += F^T.
+// Evaluated through fock_subgrid with vrho := TMP0, vsigma := 1 and
+// rho1 := TMP4 / 2 on each sub-grid.
+int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx) {
+  return guarded([&] {
+    if (s->family > 1) throw std::runtime_error("oracle: the response path is LDA/GGA");
+    const int nb = s->nbasis, n = (int)s->active.size(), nt = s->nthreads;
+    const size_t nb2 = (size_t)nb * nb;
+    std::vector<std::vector<double>> part(nt, std::vector<double>((size_t)nmats * nb2, 0.0));
+#pragma omp parallel num_threads(nt)
+    {
+      std::vector<double> F, tmp, tmp3, fx, rho0, rho10, sig0, vr0, vs0;
+      double* mine = part[omp_get_thread_num()].data();
+#pragma omp for schedule(dynamic, 1)
+      for (int i = 0; i < n; ++i) {
+        SubGrid& sg = s->subgrids[s->active[i]];
+        const int ng = sg.npad;
+        rho0 = sg.rho; rho10 = sg.rho1; sig0 = sg.sigma; vr0 = sg.vrho; vs0 = sg.vsigma;
+        fx.assign((size_t)4 * ng, 0.0);  // v_sigma, v2rho2, v2rhosigma, v2sigma2
+        for (int code : s->codes)
+          for (int g = 0; g < ng; ++g)
+            oxf::eval_point_f(code, rho0[g], s->family >= 1 ? sig0[g] : 0.0, &fx[4 * g], &fx[4 * g + 1],
+                              &fx[4 * g + 2], &fx[4 * g + 3]);
+        for (int m = 0; m < nmats; ++m) {
+          density_subgrid(s, sg, Dx + m * nb2, s->family, tmp);
+          sg.vrho.assign(ng, 0.0);
+          sg.vsigma.assign(ng, 1.0);
+          for (int g = 0; g < ng; ++g) {
+            double sx = 0.0;
+            if (s->family >= 1)
+              for (int r = 0; r < 3; ++r) sx += 2.0 * rho10[(size_t)r * ng + g] * sg.rho1[(size_t)r * ng + g];
+            sg.vrho[g] = fx[4 * g + 1] * sg.rho[g] + fx[4 * g + 2] * sx;
+            if (s->family >= 1) {
+              const double q = fx[4 * g + 2] * sg.rho[g] + fx[4 * g + 3] * sx;
+              for (int r = 0; r < 3; ++r)
+                sg.rho1[(size_t)r * ng + g] = q * rho10[(size_t)r * ng + g] + fx[4 * g] * sg.rho1[(size_t)r * ng + g];
+            }
+          }
+          fock_subgrid(s, sg, mine + m * nb2, F, tmp, tmp3);
+        }
+        sg.rho = rho0; sg.rho1 = rho10; sg.sigma = sig0; sg.vrho = vr0; sg.vsigma = vs0;
+      }
+    }
+    std::memset(Fx, 0, sizeof(double) * nmats * nb2);
+    for (int t = 0; t < nt; ++t)
+      for (size_t k = 0; k < (size_t)nmats * nb2; ++k) Fx[k] += part[t][k];
+  });
+}
+
 // XC nuclear gradient in the reference's skeleton form, the grid held fixed
 // (Restricted/Grad.cpp:72-84 -> SubGrid::getDensitySkeleton,
 // GridDensity.cpp:79-130 -> SubGrid::getEnergyGrad, GridEnergy.cpp:15-30):
@@ -947,6 +1003,22 @@ int ox_eval_functional(int ncodes, const int* codes, long n,
     if (vsigma) vsigma[i] = vs;
     if (vlapl) vlapl[i] = 0;
     if (vtau) vtau[i] = vt;
+  }
+  return 0;
+}
+
+int ox_eval_functional_f(int ncodes, const int* codes, long n, const double* rho, const double* sigma,
+                         double* vsigma, double* v2rho2, double* v2rhosigma, double* v2sigma2) {
+  for (long i = 0; i < n; ++i) {
+    double vs = 0, frr = 0, frs = 0, fss = 0;
+    for (int k = 0; k < ncodes; ++k) {
+      if (oxf::family_of(codes[k]) > 1) { g_err = "oracle: second derivatives are LDA/GGA"; return -1; }
+      oxf::eval_point_f(codes[k], rho[i], sigma ? sigma[i] : 0.0, &vs, &frr, &frs, &fss);
+    }
+    if (vsigma) vsigma[i] = vs;
+    if (v2rho2) v2rho2[i] = frr;
+    if (v2rhosigma) v2rhosigma[i] = frs;
+    if (v2sigma2) v2sigma2[i] = fss;
   }
   return 0;
 }

@@ -89,6 +89,14 @@ int ox_eval_functional(int ncodes, const int* codes, long n,
 /* AO values at arbitrary points (n points): ao[mu*n + i], grad[(r*nbasis + mu)*n + i],
  * lapl[mu*n + i]. grad/lapl may be NULL. */
 int ox_ao_values(const ox_system* s, long n, const double* xyz, double* ao, double* grad, double* lapl);
+/* Second derivatives of the summed LDA/GGA codes at n points (libxc xc_*_fxc):
+ * v_sigma, v2rho2, v2rhosigma, v2sigma2 (any output may be NULL). */
+int ox_eval_functional_f(int ncodes, const int* codes, long n, const double* rho, const double* sigma,
+                         double* vsigma, double* v2rho2, double* v2rhosigma, double* v2sigma2);
+/* Response Fock matrices of nmats perturbed density matrices (column-major
+ * nbasis^2 each, back to back), closed shell, LDA/GGA, after ox_density at the
+ * reference density: Evaluate("f"), getDensityU, getFockU<u_t>. */
+int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx);
 
 /* Density quantities on the active sub-grids after ox_density, in the
  * sub-grid order with padding removed: rho, grad (3 per point), sigma, tau. */

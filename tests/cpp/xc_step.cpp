@@ -45,6 +45,12 @@ int main(int argc, char** argv) {
     out.write(reinterpret_cast<const char*>(&E), sizeof(double));
     out.write(reinterpret_cast<const char*>(&N), sizeof(double));
     out.write(reinterpret_cast<const char*>(V.data()), sizeof(double) * n * n);
+    if (xc.Family != "mGGA") {  // response to the perturbation D^x = D, as RestrictedFinite/SP.cpp:444-455 does
+      xc.Evaluate("f", grid);
+      grid.getDensityU({{D}});
+      chinium::EigenMatrix F = grid.getFockU()[0][0];
+      out.write(reinterpret_cast<const char*>(F.data()), sizeof(double) * n * n);
+    }
     std::printf("E_xc %.12f N_e %.10f\n", E, N);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());

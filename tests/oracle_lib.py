@@ -54,6 +54,8 @@ def lib():
         "ox_ao_values": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp]),
         "ox_active_points": (ctypes.c_long, [_vp]),
         "ox_get_density": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "ox_eval_functional_f": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long] + [_vp] * 6),
+        "ox_fock_u": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -143,6 +145,17 @@ class OracleSystem:
         _check(lib().ox_xc(self.h, _ptr(Df), _ptr(V), ctypes.addressof(E), ctypes.addressof(N)))
         return E.value, N.value, np.array(V)
 
+    def fock_u(self, D, Dxs):
+        """Response Fock matrices for the perturbed density matrices Dxs at the
+        reference density D (Evaluate("f"), getDensityU, getFockU<u_t>)."""
+        n = self.nbasis
+        Df = np.asfortranarray(D, dtype=np.float64)
+        _check(lib().ox_density(self.h, _ptr(Df)))
+        X = np.stack([np.asfortranarray(d, dtype=np.float64).ravel(order="F") for d in Dxs])
+        F = np.zeros_like(X)
+        _check(lib().ox_fock_u(self.h, len(Dxs), _ptr(X), _ptr(F)))
+        return [F[m].reshape((n, n), order="F") for m in range(len(Dxs))]
+
     def xc_grad(self, D):
         """XC nuclear gradient (natoms x 3), skeleton form with the grid fixed."""
         Df = np.asfortranarray(D, dtype=np.float64)
@@ -187,3 +200,16 @@ def eval_functional(codes, rho, sigma=None, tau=None):
     _check(L.ox_eval_functional(len(cc), _ptr(cc), n, _ptr(rho), _ptr(sigma), None, _ptr(tau),
                                 _ptr(e), _ptr(vr), _ptr(vs), _ptr(vl), _ptr(vt)))
     return e, vr, vs, vt
+
+
+def eval_functional_f(codes, rho, sigma=None):
+    """Per-point v_sigma, v2rho2, v2rhosigma, v2sigma2 (libxc xc_*_fxc conventions)."""
+    L = lib()
+    cc = np.ascontiguousarray(codes, dtype=np.int32)
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    n = len(rho)
+    sigma = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+    vs, frr, frs, fss = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(L.ox_eval_functional_f(len(cc), _ptr(cc), n, _ptr(rho), _ptr(sigma), _ptr(vs), _ptr(frr), _ptr(frs),
+                                  _ptr(fss)))
+    return vs, frr, frs, fss

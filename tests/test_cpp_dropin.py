@@ -37,7 +37,9 @@ def test_cpp_caller_matches_oracle(tmp_path):
                     str(tmp_path / "out.bin")], check=True, env=env, timeout=120)
     out = np.fromfile(tmp_path / "out.bin")
     n = mol.nbasis
-    E, N, V = out[0], out[1], out[2:].reshape((n, n), order="F")
+    E, N = out[0], out[1]
+    V = out[2:2 + n * n].reshape((n, n), order="F")
+    F = out[2 + n * n:].reshape((n, n), order="F")  # response Fock matrix for D^x = D
     orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], w["grid"], w["basis"])
     orc.set_functional(w["functional"])
     orc.get_ao()
@@ -45,3 +47,5 @@ def test_cpp_caller_matches_oracle(tmp_path):
     assert abs(E - Eo) <= 1e-10
     assert abs(N - No) <= 1e-10
     assert np.abs(V - Vo).max() <= 1e-9
+    (Fo,) = orc.fock_u(D, [D])
+    assert np.abs(F - Fo).max() <= 1e-9

@@ -1,0 +1,133 @@
+"""GPU response path (Evaluate("f") -> Grid::getDensityU -> Grid::getFockU<u_t>,
+closed shell, LDA/GGA, FP64 form) against the FP64 oracle, whose f_xc and
+response matrices are pinned by finite differences in test_oracle_response.py.
+Bars: f_xc 1e-11 relative; response matrices 1e-9 elementwise."""
+import numpy as np
+import pytest
+
+import chinium_amd as ca
+import oracle_lib as ol
+from chinium_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+F_TOL = 1e-9
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("func", ["PBE", "B3LYP", "SVWN5", "BLYP"])
+def test_fxc_points(func):
+    xc = ca.ExchangeCorrelation()
+    xc.Read(func)
+    rng = np.random.default_rng(9)
+    rho = 10.0 ** rng.uniform(-6, 2, 2000)
+    sigma = 10.0 ** rng.uniform(-4, 0.5, 2000) * 38.28312000250922 * rho ** (8.0 / 3.0)
+    rho[:5] = [0.0, 1e-16, 1e-15, 2e-15, 1e-12]  # threshold neighbourhood
+    f0, f1, f2 = xc.eval_points_f(rho, sigma)
+    vs, g0, g1, g2 = ol.eval_functional_f(xc.Codes, rho, sigma)
+    exc = ol.eval_functional(xc.Codes, rho, sigma)[0]
+    e = np.abs(rho * exc)  # energy density: the natural scale of e'' is e / rho^2, e / (rho sigma), e / sigma^2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        scales = (e / rho ** 2, e / (rho * sigma), e / sigma ** 2)
+    worst = 0.0
+    for a, b, sc in zip((f0, f1, f2), (g0, g1, g2), scales):
+        assert np.all(a[:2] == 0) and np.all(b[:2] == 0)  # below the density threshold
+        # different operation orders: rounding-level agreement against the natural scale
+        err = np.abs(a[2:] - b[2:]) / (np.abs(b[2:]) + sc[2:])
+        worst = max(worst, float(err.max()))
+    _say("  f_xc %s: max diff / (|f| + scale) %.1e" % (func, worst))
+    assert worst <= 1e-11
+
+
+def _h2o():
+    w = wl.build_workload("h2o_ccpvdz_sg1_pbe")
+    mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
+    D, S = wl.workload_density(mol, w["nocc"], w["seed"])
+    return w, mol, D
+
+
+def _response(mol, grid_name, func, D, Dxs, cache):
+    grid = ca.Grid(mol, grid_name, 1, 0)
+    grid.setAOCache(cache)
+    xc = ca.ExchangeCorrelation()
+    xc.Read(func)
+    grid.setType(xc.family)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    xc.Evaluate("f", grid)
+    grid.getDensityU([Dxs])
+    return grid, xc, grid.getFockU()[0]
+
+
+@pytest.mark.parametrize("func,cache", [("PBE", 1), ("B3LYP", 1), ("SVWN5", 1), ("PBE", 0)])
+def test_h2o_response_against_oracle(func, cache):
+    w, mol, D = _h2o()
+    rng = np.random.default_rng(4)
+    Dxs = []
+    for _ in range(3):
+        A = rng.standard_normal(D.shape)
+        Dxs.append(0.1 * (A + A.T))
+    _say("response h2o/cc-pvdz/SG-1 %s, AO cache %d, 3 perturbations" % (func, cache))
+    grid, xc, Fs = _response(mol, "SG-1", func, D, Dxs, cache)
+    orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], "SG-1", w["basis"])
+    orc.set_functional(func)
+    orc.get_ao()
+    Fo = orc.fock_u(D, Dxs)
+    for F, G in zip(Fs, Fo):
+        _say("  max |F - F_oracle| %.2e  max |F| %.3f" % (np.abs(F - G).max(), np.abs(G).max()))
+        assert np.abs(F - G).max() <= F_TOL
+        assert np.abs(F - F.T).max() <= 1e-12
+
+
+def test_water_cluster_response():
+    Z, X = wl.water_cluster_angstrom(4, 5.0, 11)
+    xyz = X * ca.ANGSTROM_TO_BOHR
+    mol = ca.Molecule(Z, xyz, "def2-svp")
+    D, S = wl.workload_density(mol, 20, 3)
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal(D.shape)
+    Dx = 0.05 * (A + A.T)
+    _say("response (H2O)4/def2-svp/SG-1 B3LYP")
+    grid, xc, Fs = _response(mol, "SG-1", "B3LYP", D, [Dx], 1)
+    orc = ol.OracleSystem(Z, xyz, "SG-1", "def2-svp")
+    orc.set_functional("B3LYP")
+    orc.get_ao()
+    (Fo,) = orc.fock_u(D, [Dx])
+    _say("  max |F - F_oracle| %.2e" % np.abs(Fs[0] - Fo).max())
+    assert np.abs(Fs[0] - Fo).max() <= F_TOL
+    # the ground-state build on the same grid is unaffected by the response calls
+    E, V = ca.xc_build(grid, xc, D)
+    Eo, No, Vo = orc.xc(D)
+    assert abs(E - Eo) <= 1e-10 and np.abs(V - Vo).max() <= 1e-9
+
+
+def test_response_protocol_errors():
+    w, mol, D = _h2o()
+    grid = ca.Grid(mol, "SG-1", 1, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    grid.setType(1)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    grid.getDensityU([[D]])
+    with pytest.raises(ca.ChxError) as e:  # Evaluate("f") has not run
+        grid.getFockU()
+    assert e.value.code == -3
+    r2 = ca.ExchangeCorrelation()
+    r2.Read("R2SCAN")
+    grid.setType(2)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    with pytest.raises(ca.ChxError) as e:  # meta-GGA f_xc is not built
+        r2.Evaluate("f", grid)
+    assert e.value.code == -5
+    m = ca.Grid(mol, "SG-1", 1, 0)
+    m.setPrecision(ca.CHX_PREC_MIXED)
+    m.setType(1)
+    m.getAO(0, 0)
+    m.getDensity([D])
+    with pytest.raises(ca.ChxError) as e:  # FP64 form only
+        xc.Evaluate("f", m)
+    assert e.value.code == -5

@@ -1,0 +1,106 @@
+"""Oracle response path (Evaluate("f") -> getDensityU -> getFockU<u_t>) pinned
+by what the mathematics fixes, with no libxc:
+  * f_xc is the derivative of v_xc: v2rho2 = d v_rho / d rho,
+    v2rhosigma = d v_sigma / d rho = d v_rho / d sigma, v2sigma2 = d v_sigma / d sigma
+    (central differences of the oracle's exact first derivatives);
+  * the response Fock matrix is the linear response of V: F[D^x] = dV[D + t D^x]/dt
+    at t = 0 (central differences of the oracle's full XC build);
+  * F is linear in D^x."""
+import numpy as np
+import pytest
+
+import chinium_amd as ca
+import oracle_lib as ol
+from chinium_amd import workloads as wl
+
+FUNCS = {"SVWN5": [1, 7], "VWN_RPA": [8], "PBE": [101, 130], "PBE_libxc": [321, 322], "B3LYP": [402],
+         "BLYP": [106, 131]}
+
+
+def _points(n, seed):
+    rng = np.random.default_rng(seed)
+    rho = 10.0 ** rng.uniform(-3, 1, n)
+    s2 = 10.0 ** rng.uniform(-3, 0.5, n)          # reduced gradient squared
+    sigma = s2 * 38.28312000250922 * rho ** (8.0 / 3.0)
+    return rho, sigma
+
+
+@pytest.mark.parametrize("name", sorted(FUNCS))
+def test_fxc_is_derivative_of_vxc(name):
+    codes = FUNCS[name]
+    rho, sigma = _points(40, 11)
+    vs, frr, frs, fss = ol.eval_functional_f(codes, rho, sigma)
+    _, vr0, vs0, _ = ol.eval_functional(codes, rho, sigma)
+    np.testing.assert_allclose(vs, vs0, rtol=1e-11, atol=0)  # same derivative, other operation order
+
+    def d(fn, x, h):  # Richardson-extrapolated central difference
+        d1 = (fn(x + h) - fn(x - h)) / (2 * h)
+        d2 = (fn(x + h / 2) - fn(x - h / 2)) / h
+        return (4 * d2 - d1) / 3
+
+    hr, hs = 1e-4 * rho, 1e-4 * sigma
+    vr_r = d(lambda r: ol.eval_functional(codes, r, sigma)[1], rho, hr)
+    vs_r = d(lambda r: ol.eval_functional(codes, r, sigma)[2], rho, hr)
+    vr_s = d(lambda s: ol.eval_functional(codes, rho, s)[1], sigma, hs)
+    vs_s = d(lambda s: ol.eval_functional(codes, rho, s)[2], sigma, hs)
+    # truncation (1e-7 relative) plus the rounding of the differenced values
+    # (|v| eps / h, which dominates where v hardly changes over the step)
+    tol = lambda ref, v, h: 1e-7 * np.abs(ref) + 1e-13 * np.abs(v) / h
+    assert np.all(np.abs(frr - vr_r) <= tol(frr, vr0, hr))
+    if codes[0] not in (1, 7, 8):  # GGA: the mixed derivatives agree both ways
+        assert np.all(np.abs(frs - vs_r) <= tol(frs, vs0, hr))
+        assert np.all(np.abs(frs - vr_s) <= tol(frs, vr0, hs))
+        assert np.all(np.abs(fss - vs_s) <= tol(fss, vs0, hs))
+    else:
+        assert np.all(frs == 0) and np.all(fss == 0)
+
+
+def test_fxc_below_threshold_is_zero():
+    vs, frr, frs, fss = ol.eval_functional_f([101, 130], np.array([1e-16, 0.0]), np.array([1e-30, 0.0]))
+    assert np.all(frr == 0) and np.all(frs == 0) and np.all(fss == 0) and np.all(vs == 0)
+
+
+@pytest.fixture(scope="module")
+def h2o():
+    w = wl.build_workload("h2o_ccpvdz_sg1_pbe")
+    mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
+    D, S = wl.workload_density(mol, w["nocc"], w["seed"])
+    o = ol.OracleSystem(w["Z"], w["xyz_bohr"], w["grid"], w["basis"])
+    return o, D
+
+
+@pytest.mark.parametrize("func", ["PBE", "B3LYP", "SVWN5"])
+def test_fock_u_is_linear_response_of_v(h2o, func):
+    o, D = h2o
+    o.set_functional(func)
+    o.get_ao()
+    rng = np.random.default_rng(3)
+    Dx = rng.standard_normal(D.shape)
+    Dx = 0.05 * (Dx + Dx.T)
+    (F,) = o.fock_u(D, [Dx])
+    V = lambda t: o.xc(D + t * Dx)[2]
+    h = 1e-3
+    d1 = (V(h) - V(-h)) / (2 * h)
+    d2 = (V(h / 2) - V(-h / 2)) / h
+    fd = (4 * d2 - d1) / 3
+    assert np.abs(F - F.T).max() <= 1e-13 * np.abs(F).max()
+    assert np.abs(F - fd).max() <= 1e-7 * np.abs(F).max()
+
+
+def test_fock_u_is_linear_in_dx(h2o):
+    o, D = h2o
+    o.set_functional("PBE")
+    o.get_ao()
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal(D.shape); A = A + A.T
+    B = rng.standard_normal(D.shape); B = B + B.T
+    FA, FB, FC = o.fock_u(D, [A, B, 2.0 * A - 3.0 * B])
+    assert np.abs(FC - (2.0 * FA - 3.0 * FB)).max() <= 1e-12 * np.abs(FC).max()
+
+
+def test_fock_u_rejects_meta_gga(h2o):
+    o, D = h2o
+    o.set_functional("R2SCAN")
+    o.get_ao()
+    with pytest.raises(RuntimeError):
+        o.fock_u(D, [D])
