@@ -432,7 +432,9 @@ void grid_plan(chx_grid* g) {
   g->bt_off[nb] = (int)g->bt_list.size();
   g->grp_off[nb] = (int)g->grp_batch.size();
   g->mean_tiles = nb ? tiles / nb : 0;
-  g->flops_build = 2.0 * 2.0 * flops;  // k = 2 contractions (GGA), 2 flops per multiply-add
+  // k contractions per build (GGA: Phi D and Phi^T B; meta-GGA adds three of
+  // each for tau), 2 flops per multiply-add
+  g->flops_build = (g->type >= 2 ? 8.0 : 2.0) * 2.0 * flops;
   // Vxc work items over super-tile pairs
   const int nsup = g->npad / (kSuper * kTile);
   std::vector<std::vector<int>> pairs((size_t)nsup * nsup);
