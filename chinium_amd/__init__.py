@@ -101,6 +101,21 @@ def lib():
         "chx_pack_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
         "chx_pack_stream": (i, [_vp, P(_vp)]),
         "chx_pack_destroy": (None, [_vp]),
+        "chx_mlp_create": (i, [ctypes.c_ulonglong, P(_vp)]),
+        "chx_mlp_destroy": (None, [_vp]),
+        "chx_mlp_get_params": (i, [_vp, _vp]),
+        "chx_mlp_set_params": (i, [_vp, _vp]),
+        "chx_mlp_set_box": (i, [_vp, _vp, _vp, d]),
+        "chx_mlp_get_box": (i, [_vp, _vp, _vp, P(d)]),
+        "chx_mlp_features": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp]),
+        "chx_mlp_set_data": (i, [_vp, l, _vp, _vp, _vp]),
+        "chx_mlp_train": (i, [_vp, i, l, d, d, _vp]),
+        "chx_mlp_loss_grad": (i, [_vp, l, _vp, _vp, _vp, _vp, P(d)]),
+        "chx_mlp_forward": (i, [_vp, l, _vp, i, _vp, _vp]),
+        "chx_mlp_eval_points": (i, [_vp, _vp, l, _vp, _vp, _vp, _vp, _vp, _vp, _vp, P(l)]),
+        "chx_xc_set_surrogate": (i, [_vp, _vp]),
+        "chx_grid_point_density": (i, [_vp, _vp, _vp, _vp, _vp, P(l)]),
+        "chx_grid_fallback_count": (i, [_vp, P(l)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -281,6 +296,20 @@ class Grid:
         return [[F[(k * m + j) * n * n:(k * m + j + 1) * n * n].reshape((n, n), order="F").copy() for j in range(m)]
                 for k in range(nspin)]
 
+    def pointDensity(self):
+        """rho, sigma, tau, weight per slot of the last getDensity (padding slots have weight 0)."""
+        n = ctypes.c_long()
+        _check(lib().chx_grid_point_density(self.h, None, None, None, None, ctypes.byref(n)))
+        out = [np.zeros(n.value) for _ in range(4)]
+        _check(lib().chx_grid_point_density(self.h, *[_ptr(o) for o in out], ctypes.byref(n)))
+        return tuple(out)
+
+    def fallbackCount(self):
+        """Points of the last Evaluate that took the surrogate's analytic fallback."""
+        n = ctypes.c_long()
+        _check(lib().chx_grid_fallback_count(self.h, ctypes.byref(n)))
+        return n.value
+
     def getDensityDevice(self, ptr, nspin, nbasis):
         _check(lib().chx_grid_get_density_device(self.h, ctypes.c_void_p(ptr), int(nspin), int(nbasis)))
 
@@ -401,6 +430,12 @@ class ExchangeCorrelation:
     def Evaluate(self, order, grid):
         _check(lib().chx_xc_evaluate(self.h, grid.h, order.encode()))
 
+    def setSurrogate(self, surrogate):
+        """Attach a trained Surrogate (meta-GGA functionals; None detaches):
+        Evaluate then runs the network, with the analytic fallback."""
+        _check(lib().chx_xc_set_surrogate(self.h, None if surrogate is None else surrogate.h))
+        self._surrogate = surrogate  # keeps the network alive while attached
+
     def __bool__(self):
         return len(self.Codes) > 0
 
@@ -464,3 +499,6 @@ def xc_build(grid, xc, D):
     E = grid.getEnergy()
     V = grid.getFock()[0]
     return E, V
+
+
+from .surrogate import Surrogate  # noqa: E402  (uses lib/_check above)

@@ -1,5 +1,6 @@
 // C ABI of the XC path (include/chinium_xc.h). Every entry point catches
 // exceptions and returns a status; the message is kept per host thread.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -89,6 +90,21 @@ void make_stream(chx_grid* g) { ensure_stream(g); }
 void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
   g->d_codes.upload(xc->codes, g->stream);
   hipEvent_t e = nullptr;
+  g->surrogate_used = false;
+  if (xc->surrogate && xc->family == 2) {  // learned functional, one partial sum per 64-slot block
+    const chx_mlp* m = xc->surrogate;
+    if (g->d_en_part.n < 2 * (size_t)g->nbatches) g->d_en_part.alloc(2 * (size_t)g->nbatches);
+    g->d_nfall.alloc(1);
+    launch_fill_u32(g->stream, (unsigned*)g->d_nfall.p, 1, 0u);
+    g->mark_begin(2, &e);
+    launch_functional_mlp(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, m->Pinf.p,
+                          m->box, g->d_codes.p, (int)xc->codes.size(), want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p,
+                          g->d_coef.p, g->d_en_part.p, g->d_nfall.p);
+    launch_reduce_en(g->stream, g->nbatches, g->d_en_part.p, g->d_en.p);
+    g->mark_end(2, e);
+    g->surrogate_used = true;
+    return;
+  }
   g->mark_begin(2, &e);
   launch_functional(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, g->d_codes.p,
                     (int)xc->codes.size(), xc->family, want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p, g->d_coef.p,
@@ -809,6 +825,314 @@ int chx_xc_eval_points_f(const chx_xc* xc, long n, const double* rho, const doub
     down(v2rho2, f0); down(v2rhosigma, f1); down(v2sigma2, f2);
     CHX_HIP(hipStreamSynchronize(s));
     CHX_HIP(hipStreamDestroy(s));
+  });
+}
+
+// ---- learned functional surrogate ----------------------------------------
+static_assert(CHX_MLP_PARAMS == kMlpParams, "parameter block size");
+
+namespace {
+void mlp_require_stream(chx_mlp* m) {
+  if (!m->stream) CHX_HIP(hipStreamCreate(&m->stream));
+}
+
+void mlp_upload_params(chx_mlp* m, const float* P) {
+  CHX_HIP(hipMemcpyAsync(m->P.p, P, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, m->stream));
+  launch_mlp_pack(m->stream, m->P.p, m->Prun.p, m->Pinf.p);
+}
+
+// Glorot-uniform weights (tanh units), zero biases, output bias 1 (F of the
+// uniform gas exchange); splitmix64 for the seeded stream.
+std::vector<float> mlp_init(unsigned long long seed) {
+  std::vector<float> P(kMlpParams, 0.f);
+  unsigned long long x = seed;
+  auto next = [&] {
+    x += 0x9E3779B97F4A7C15ull;
+    unsigned long long z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto fill = [&](int off, int nout, int nin) {
+    const double a = std::sqrt(6.0 / (nin + nout));
+    for (int k = 0; k < nout * nin; ++k) P[off + k] = (float)((2.0 * next() - 1.0) * a);
+  };
+  fill(kMlpW1, kMlpH, 3);
+  fill(kMlpW2, kMlpH, kMlpH);
+  fill(kMlpW3, kMlpH, kMlpH);
+  fill(kMlpW4, 1, kMlpH);
+  P[kMlpB4] = 1.f;
+  return P;
+}
+}  // namespace
+
+int chx_mlp_create(unsigned long long seed, chx_mlp** out) {
+  return guard([&] {
+    require(out != nullptr, CHX_ERR_ARG, "null output");
+    chx_mlp* m = new chx_mlp;
+    try {
+      mlp_require_stream(m);
+      m->P.alloc(kMlpParams);
+      m->Prun.alloc(kMlpParams);
+      m->Pinf.alloc(kMlpParams);
+      m->G.alloc(kMlpParams);
+      m->M.alloc(kMlpParams);
+      m->V.alloc(kMlpParams);
+      m->loss.alloc(1);
+      const std::vector<float> P = mlp_init(seed);
+      mlp_upload_params(m, P.data());
+      CHX_HIP(hipMemsetAsync(m->G.p, 0, sizeof(float) * kMlpParams, m->stream));
+      CHX_HIP(hipMemsetAsync(m->M.p, 0, sizeof(float) * kMlpParams, m->stream));
+      CHX_HIP(hipMemsetAsync(m->V.p, 0, sizeof(float) * kMlpParams, m->stream));
+      CHX_HIP(hipStreamSynchronize(m->stream));
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+void chx_mlp_destroy(chx_mlp* m) { delete m; }
+
+int chx_mlp_get_params(const chx_mlp* m, float* P) {
+  return guard([&] {
+    require(m && P, CHX_ERR_ARG, "bad arguments");
+    CHX_HIP(hipMemcpyAsync(P, m->P.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream));
+    CHX_HIP(hipStreamSynchronize(m->stream));
+  });
+}
+
+int chx_mlp_set_params(chx_mlp* m, const float* P) {
+  return guard([&] {
+    require(m && P, CHX_ERR_ARG, "bad arguments");
+    mlp_upload_params(m, P);
+    CHX_HIP(hipMemsetAsync(m->M.p, 0, sizeof(float) * kMlpParams, m->stream));  // fresh optimizer state
+    CHX_HIP(hipMemsetAsync(m->V.p, 0, sizeof(float) * kMlpParams, m->stream));
+    CHX_HIP(hipStreamSynchronize(m->stream));
+    m->step = 0;
+  });
+}
+
+int chx_mlp_set_box(chx_mlp* m, const float* lo, const float* hi, double rho_min) {
+  return guard([&] {
+    require(m && lo && hi, CHX_ERR_ARG, "bad arguments");
+    for (int k = 0; k < 3; ++k) {
+      require(std::isfinite(lo[k]) && std::isfinite(hi[k]) && hi[k] > lo[k], CHX_ERR_ARG, "empty descriptor box");
+      m->box.lo[k] = lo[k];
+      m->box.hi[k] = hi[k];
+    }
+    require(rho_min > 0, CHX_ERR_ARG, "rho_min must be positive");
+    m->box.rho_min = rho_min;
+    m->have_box = true;
+  });
+}
+
+int chx_mlp_get_box(const chx_mlp* m, float* lo, float* hi, double* rho_min) {
+  return guard([&] {
+    require(m && m->have_box, CHX_ERR_STATE, "the network has no descriptor box");
+    for (int k = 0; k < 3; ++k) {
+      if (lo) lo[k] = m->box.lo[k];
+      if (hi) hi[k] = m->box.hi[k];
+    }
+    if (rho_min) *rho_min = m->box.rho_min;
+  });
+}
+
+int chx_mlp_features(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau, float* t,
+                     float* F) {
+  return guard([&] {
+    require(xc && n >= 0 && rho && sigma && tau && t && F, CHX_ERR_ARG, "bad arguments");
+    if (n == 0) return;
+    hipStream_t s;
+    CHX_HIP(hipStreamCreate(&s));
+    DevBuf<double> dr, ds, dt;
+    DevBuf<int> dc;
+    DevBuf<float> dtt, dF;
+    dr.upload(std::vector<double>(rho, rho + n), s);
+    ds.upload(std::vector<double>(sigma, sigma + n), s);
+    dt.upload(std::vector<double>(tau, tau + n), s);
+    dc.upload(xc->codes, s);
+    dtt.alloc(3 * (size_t)n);
+    dF.alloc(n);
+    launch_mlp_features(s, n, dr.p, ds.p, dt.p, dc.p, (int)xc->codes.size(), dtt.p, dF.p);
+    CHX_HIP(hipMemcpyAsync(t, dtt.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipMemcpyAsync(F, dF.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipStreamSynchronize(s));
+    CHX_HIP(hipStreamDestroy(s));
+  });
+}
+
+int chx_mlp_set_data(chx_mlp* m, long n, const float* U, const float* T, const float* C) {
+  return guard([&] {
+    require(m && n > 0 && U && T, CHX_ERR_ARG, "bad arguments");
+    m->U.alloc(3 * (size_t)n);
+    m->T.alloc(n);
+    CHX_HIP(hipMemcpyAsync(m->U.p, U, sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->stream));
+    CHX_HIP(hipMemcpyAsync(m->T.p, T, sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+    m->cprefix.assign(n + 1, 0.0);
+    if (C) {
+      m->C.alloc(n);
+      CHX_HIP(hipMemcpyAsync(m->C.p, C, sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+      for (long i = 0; i < n; ++i) {
+        require(C[i] >= 0.f && std::isfinite(C[i]), CHX_ERR_ARG, "sample weights must be finite and >= 0");
+        m->cprefix[i + 1] = m->cprefix[i] + C[i];
+      }
+    } else {
+      m->C.release();
+      for (long i = 0; i < n; ++i) m->cprefix[i + 1] = (double)(i + 1);
+    }
+    CHX_HIP(hipStreamSynchronize(m->stream));
+    m->ndata = n;
+  });
+}
+
+int chx_mlp_train(chx_mlp* m, int epochs, long batch, double lr, double lr_final, double* epoch_loss) {
+  return guard([&] {
+    require(m && epochs >= 0 && batch > 0 && lr > 0 && lr_final >= 0, CHX_ERR_ARG, "bad arguments");
+    require(m->ndata > 0, CHX_ERR_STATE, "no training set (chx_mlp_set_data)");
+    const long n = m->ndata;
+    const long nb = (n + batch - 1) / batch;
+    const long total = (long)epochs * nb;
+    long it = 0;
+    for (int ep = 0; ep < epochs; ++ep) {
+      launch_fill_u32(m->stream, (unsigned*)m->loss.p, 1, 0u);  // 0.0f
+      for (long b = 0; b < nb; ++b, ++it) {
+        const long s0 = b * batch, s1 = std::min(n, s0 + batch);
+        const double csum = m->cprefix[s1] - m->cprefix[s0];
+        if (!(csum > 0)) continue;
+        const double cosw = 0.5 * (1.0 + std::cos(M_PI * (double)it / (double)std::max<long>(1, total)));
+        const double lr_t = lr_final + (lr - lr_final) * cosw;
+        launch_mlp_grad(m->stream, s1 - s0, m->U.p + 3 * s0, m->T.p + s0, m->C.p ? m->C.p + s0 : nullptr,
+                        (float)(1.0 / csum), m->Prun.p, m->G.p, m->loss.p, 512);
+        m->step += 1;
+        launch_mlp_adam(m->stream, m->P.p, m->G.p, m->M.p, m->V.p, m->Prun.p, m->Pinf.p, (float)lr_t, m->step);
+      }
+      float l = 0.f;
+      CHX_HIP(hipMemcpyAsync(&l, m->loss.p, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+      CHX_HIP(hipStreamSynchronize(m->stream));
+      if (epoch_loss) epoch_loss[ep] = (double)l / (double)nb;
+    }
+    CHX_HIP(hipStreamSynchronize(m->stream));
+  });
+}
+
+int chx_mlp_loss_grad(chx_mlp* m, long n, const float* U, const float* T, const float* C, float* G, double* loss) {
+  return guard([&] {
+    require(m && n > 0 && U && T && G, CHX_ERR_ARG, "bad arguments");
+    DevBuf<float> dU, dT, dC;
+    dU.alloc(3 * (size_t)n);
+    dT.alloc(n);
+    CHX_HIP(hipMemcpyAsync(dU.p, U, sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->stream));
+    CHX_HIP(hipMemcpyAsync(dT.p, T, sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+    double csum = (double)n;
+    if (C) {
+      dC.alloc(n);
+      CHX_HIP(hipMemcpyAsync(dC.p, C, sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+      csum = 0;
+      for (long i = 0; i < n; ++i) csum += C[i];
+    }
+    require(csum > 0, CHX_ERR_ARG, "sample weights sum to zero");
+    DevBuf<float> dG;
+    dG.alloc(kMlpParams);
+    launch_fill_u32(m->stream, (unsigned*)dG.p, kMlpParams, 0u);
+    launch_fill_u32(m->stream, (unsigned*)m->loss.p, 1, 0u);
+    launch_mlp_grad(m->stream, n, dU.p, dT.p, C ? dC.p : nullptr, (float)(1.0 / csum), m->Prun.p, dG.p, m->loss.p,
+                    512);
+    float l = 0.f;
+    CHX_HIP(hipMemcpyAsync(G, dG.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream));
+    CHX_HIP(hipMemcpyAsync(&l, m->loss.p, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    CHX_HIP(hipStreamSynchronize(m->stream));
+    if (loss) *loss = l;
+  });
+}
+
+int chx_mlp_forward(const chx_mlp* m, long n, const float* U, int bf16, float* F, float* dF) {
+  return guard([&] {
+    require(m && n >= 0 && U && F, CHX_ERR_ARG, "bad arguments");
+    if (n == 0) return;
+    DevBuf<float> dU, dFv, dG;
+    dU.alloc(3 * (size_t)n);
+    dFv.alloc(n);
+    if (dF) dG.alloc(3 * (size_t)n);
+    CHX_HIP(hipMemcpyAsync(dU.p, U, sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->stream));
+    launch_mlp_forward(m->stream, n, dU.p, bf16 ? m->Pinf.p : m->Prun.p, dFv.p, dF ? dG.p : nullptr);
+    CHX_HIP(hipMemcpyAsync(F, dFv.p, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
+    if (dF) CHX_HIP(hipMemcpyAsync(dF, dG.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, m->stream));
+    CHX_HIP(hipStreamSynchronize(m->stream));
+  });
+}
+
+int chx_mlp_eval_points(const chx_mlp* m, const chx_xc* xc, long n, const double* rho, const double* sigma,
+                        const double* tau, double* exc, double* vrho, double* vsigma, double* vtau, long* nfallback) {
+  return guard([&] {
+    require(m && xc && n >= 0 && rho && sigma && tau && exc && vrho && vsigma && vtau, CHX_ERR_ARG, "bad arguments");
+    require(m->have_box, CHX_ERR_STATE, "the network has no descriptor box");
+    if (n == 0) {
+      if (nfallback) *nfallback = 0;
+      return;
+    }
+    hipStream_t s = m->stream;
+    DevBuf<double> dr, ds, dt, de, dvr, dvs, dvt;
+    DevBuf<int> dc, dn;
+    dr.upload(std::vector<double>(rho, rho + n), s);
+    ds.upload(std::vector<double>(sigma, sigma + n), s);
+    dt.upload(std::vector<double>(tau, tau + n), s);
+    dc.upload(xc->codes, s);
+    de.alloc(n); dvr.alloc(n); dvs.alloc(n); dvt.alloc(n);
+    dn.alloc(1);
+    launch_fill_u32(s, (unsigned*)dn.p, 1, 0u);
+    launch_mlp_eval(s, n, m->Pinf.p, m->box, dc.p, (int)xc->codes.size(), dr.p, ds.p, dt.p, de.p, dvr.p, dvs.p, dvt.p,
+                    dn.p);
+    int nf = 0;
+    CHX_HIP(hipMemcpyAsync(exc, de.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipMemcpyAsync(vrho, dvr.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipMemcpyAsync(vsigma, dvs.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipMemcpyAsync(vtau, dvt.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipMemcpyAsync(&nf, dn.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipStreamSynchronize(s));
+    if (nfallback) *nfallback = nf;
+  });
+}
+
+int chx_xc_set_surrogate(chx_xc* xc, const chx_mlp* m) {
+  return guard([&] {
+    require(xc != nullptr, CHX_ERR_ARG, "null functional");
+    if (m) {
+      require(xc->family == 2, CHX_ERR_UNSUPPORTED, "the surrogate stands in for a meta-GGA functional");
+      require(m->have_box, CHX_ERR_STATE, "the network has no descriptor box");
+    }
+    xc->surrogate = m;
+  });
+}
+
+int chx_grid_point_density(chx_grid* g, double* rho, double* sigma, double* tau, double* w, long* n) {
+  return guard([&] {
+    require(g != nullptr, CHX_ERR_ARG, "null grid");
+    require(g->have_density, CHX_ERR_STATE, "Grid::getDensity must precede the point density");
+    const size_t ns = (size_t)g->nbatches * kBatch;
+    if (n) *n = (long)ns;
+    if (!rho && !sigma && !tau && !w) return;
+    DevBuf<double> buf;
+    buf.alloc(std::max<size_t>(1, 4 * ns));
+    launch_point_density(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->type >= 2 ? 1 : 0,
+                         g->d_w.p, buf.p, buf.p + ns, buf.p + 2 * ns, buf.p + 3 * ns);
+    double* outs[4] = {rho, sigma, tau, w};
+    for (int k = 0; k < 4; ++k)
+      if (outs[k] && ns) CHX_HIP(hipMemcpyAsync(outs[k], buf.p + k * ns, sizeof(double) * ns, hipMemcpyDeviceToHost, g->stream));
+    CHX_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
+int chx_grid_fallback_count(const chx_grid* g, long* n) {
+  return guard([&] {
+    require(g && n, CHX_ERR_ARG, "bad arguments");
+    require(g->surrogate_used, CHX_ERR_STATE, "the last Evaluate did not run the surrogate");
+    int nf = 0;
+    CHX_HIP(hipMemcpyAsync(&nf, g->d_nfall.p, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+    CHX_HIP(hipStreamSynchronize(g->stream));
+    *n = nf;
   });
 }
 

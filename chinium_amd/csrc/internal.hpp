@@ -132,6 +132,36 @@ void launch_response_coef(hipStream_t s, int nbatches, const double* rhou, const
                           int family, double* coef);
 void launch_eval_functional_f(hipStream_t s, long n, const int* codes, int ncodes, const double* rho,
                               const double* sigma, double* v2rho2, double* v2rhosigma, double* v2sigma2);
+// Surrogate network (kernels_mlp.hip): 3 -> 64 -> 64 -> 64 -> 1, tanh. Offsets
+// of the parameter block (floats); matrices are [out][in] (y = W x + b). The
+// run copies the kernels read hold W2 transposed ([in][out]) at the same offset.
+constexpr int kMlpH = 64;
+constexpr int kMlpW1 = 0, kMlpB1 = 192, kMlpW2 = 256, kMlpB2 = 4352, kMlpW3 = 4416, kMlpB3 = 8512, kMlpW4 = 8576,
+              kMlpB4 = 8640, kMlpParams = 8641;
+// Descriptor box of the training data: t = (ln rho, s/(1+s), alpha/(1+alpha));
+// the network input is u = (t - c) / h with c, h the box centre and half width.
+struct MlpBox {
+  float lo[3], hi[3];
+  double rho_min;  // below: analytic functional (low-density tail)
+};
+void launch_mlp_pack(hipStream_t s, const float* P, float* Prun, float* Pinf);
+void launch_mlp_forward(hipStream_t s, long n, const float* U, const float* R, float* F, float* dF);
+void launch_mlp_grad(hipStream_t s, long n, const float* U, const float* T, const float* C, float cs, const float* R,
+                     float* G, float* loss, int max_blocks);
+void launch_mlp_adam(hipStream_t s, float* P, float* G, float* M, float* V, float* Prun, float* Pinf, float lr,
+                     int step);
+void launch_mlp_features(hipStream_t s, long n, const double* rho, const double* sigma, const double* tau,
+                         const int* codes, int ncodes, float* t, float* F);
+void launch_mlp_eval(hipStream_t s, long n, const float* R, const MlpBox& box, const int* codes, int ncodes,
+                     const double* rho, const double* sigma, const double* tau, double* exc, double* vrho,
+                     double* vsigma, double* vtau, int* nfall);
+void launch_functional_mlp(hipStream_t s, int nbatches, const int* grp_off, const double* part,
+                           const double* part_tau, const double* bw, const float* R, const MlpBox& box,
+                           const int* codes, int ncodes, int want_e, int want_v, double* rho_out, double* coef,
+                           double* en_part, int* nfall);
+void launch_point_density(hipStream_t s, int nbatches, const int* grp_off, const double* part,
+                          const double* part_tau, int has_tau, const double* bw, double* rho, double* sigma,
+                          double* tau, double* w);
 void launch_density_c64(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
                         const int* grp_slots, const int* grp_ntiles, const int* bt_off, const int* bt_list,
                         const double* cache, const double* Dint, int npad, double* part);
@@ -201,11 +231,26 @@ struct chx_mol {
   int nbasis = 0;
 };
 
+struct chx_mlp {
+  hipStream_t stream = nullptr;
+  chx::DevBuf<float> P, Prun, Pinf, G, M, V, loss;  // canonical, run (FP32), run (BF16-rounded); Adam state
+  chx::DevBuf<float> U, T, C;                        // training set: inputs, targets, sample weights
+  std::vector<double> cprefix;                       // prefix sums of C (minibatch normalisation)
+  long ndata = 0;
+  int step = 0;
+  bool have_box = false;
+  chx::MlpBox box{};
+  ~chx_mlp() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
 struct chx_xc {
   std::vector<int> codes;
   int family = 0;
   double exx = 0;
   int spin = 1;
+  const chx_mlp* surrogate = nullptr;  // when set, Evaluate runs the network (meta-GGA grids)
 };
 
 struct chx_grid {
@@ -289,6 +334,9 @@ struct chx_grid {
   chx::DevBuf<double> d_fx, d_part_u, d_rhou, d_coef_u;
   int fx_family = -1;  // family of the f_xc on the grid (-1: none)
   int nmats_u = 0;     // perturbed densities held in d_rhou
+  // surrogate functional: points that fell back to the analytic functional in the last Evaluate
+  chx::DevBuf<int> d_nfall;
+  bool surrogate_used = false;
 
   // Optional per-kernel HIP-event timing (stage ids in chx_grid_kernel_times).
   bool profile = false;

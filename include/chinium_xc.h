@@ -215,6 +215,60 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
 int chx_pack_stream(const chx_pack* p, void** stream);
 void chx_pack_destroy(chx_pack* p);
 
+/* ---- Learned functional surrogate (no reference counterpart) ----------------
+ * A 3 -> 64 -> 64 -> 64 -> 1 tanh network on the spin-restricted descriptors
+ * t = (ln rho, s/(1+s), alpha/(1+alpha)) that predicts F = e_xc / e_x^unif,
+ * e_x^unif = -(3/4)(3/pi)^(1/3) rho^(4/3); v_rho, v_sigma, v_tau are its input
+ * gradient chained through t. It stands in for the functional evaluation the
+ * reference does with libxc's xc_mgga_exc_vxc (src/ExchangeCorrelation.cpp:
+ * 146-160) on meta-GGA grids. Network input u = (t - c)/h, c and h the centre
+ * and half width of the training box [lo, hi]; points outside the box or with
+ * rho < rho_min take the analytic functional (the out-of-distribution
+ * fallback). Parameters: CHX_MLP_PARAMS floats, matrices [out][in]:
+ * W1[64][3] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4. Training
+ * runs in FP32 with Adam; inference uses BF16-rounded weights, FP32
+ * arithmetic. */
+#define CHX_MLP_PARAMS 8641
+typedef struct chx_mlp chx_mlp;
+int chx_mlp_create(unsigned long long seed, chx_mlp** out); /* Glorot-uniform init, seeded */
+void chx_mlp_destroy(chx_mlp* m);
+int chx_mlp_get_params(const chx_mlp* m, float* P);
+int chx_mlp_set_params(chx_mlp* m, const float* P);
+int chx_mlp_set_box(chx_mlp* m, const float* lo, const float* hi, double rho_min);
+int chx_mlp_get_box(const chx_mlp* m, float* lo, float* hi, double* rho_min);
+/* Training samples from the analytic functional of xc at n points: t (n x 3)
+ * and F (n). Host arrays. */
+int chx_mlp_features(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau, float* t,
+                     float* F);
+/* Training set (normalised inputs U n x 3, targets T, sample weights C or
+ * NULL for uniform), copied to the device. */
+int chx_mlp_set_data(chx_mlp* m, long n, const float* U, const float* T, const float* C);
+/* Adam over the training set in minibatches of `batch` samples (loss
+ * sum C_i (y_i - T_i)^2 / sum C_i per minibatch), learning rate cosine-decayed
+ * from lr to lr_final over all steps. epoch_loss (may be NULL): mean minibatch
+ * loss of each epoch. */
+int chx_mlp_train(chx_mlp* m, int epochs, long batch, double lr, double lr_final, double* epoch_loss);
+/* One loss and gradient evaluation at the current parameters (FP32):
+ * L = cs sum_i C_i (y_i - T_i)^2, cs = 1 / sum C_i; G gets dL/dP. */
+int chx_mlp_loss_grad(chx_mlp* m, long n, const float* U, const float* T, const float* C, float* G, double* loss);
+/* Raw network at n normalised inputs: F and dF/dU (n x 3, may be NULL).
+ * bf16 = 1 uses the inference weights. */
+int chx_mlp_forward(const chx_mlp* m, long n, const float* U, int bf16, float* F, float* dF);
+/* The surrogate at n points (host arrays), with the analytic fallback of xc's
+ * codes; *nfallback gets the number of points that took it. */
+int chx_mlp_eval_points(const chx_mlp* m, const chx_xc* xc, long n, const double* rho, const double* sigma,
+                        const double* tau, double* exc, double* vrho, double* vsigma, double* vtau, long* nfallback);
+/* Attach (m != NULL) or detach the surrogate: Evaluate on a meta-GGA grid
+ * then runs the network. The network must stay alive while attached and
+ * have a box. Packs reject a functional with a surrogate. */
+int chx_xc_set_surrogate(chx_xc* xc, const chx_mlp* m);
+/* Per-point rho, sigma, tau and weight of the last getDensity, in slot order
+ * (npoints_padded = batches x 64 entries; padding slots have weight 0). Any
+ * output may be NULL; *n gets the slot count. tau is 0 below meta-GGA type. */
+int chx_grid_point_density(chx_grid* g, double* rho, double* sigma, double* tau, double* w, long* n);
+/* Points of the last Evaluate that took the analytic fallback (surrogate). */
+int chx_grid_fallback_count(const chx_grid* g, long* n);
+
 /* ---- Diagnostics (no reference counterpart) ---------------------------------
  * Runs one FP64 MFMA tile on the device and returns the largest deviation from
  * the scalar product, plus the number of 2^20 sample floats whose hardware

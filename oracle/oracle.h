@@ -103,6 +103,20 @@ int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx);
 long ox_active_points(const ox_system* s);
 int ox_get_density(const ox_system* s, double* rho, double* grad, double* sigma, double* tau, double* w);
 
+/* Learned functional surrogate (mlp.cpp): the network in FP64 on the
+ * canonical parameter block (8641 doubles, matrices [out][in]); forward value
+ * and input gradient, loss L = cs sum C_i (y_i - T_i)^2 (C may be NULL) with
+ * its parameter gradient, the descriptors t(rho, sigma, tau) with
+ * dt[9i + 3k + c] = d t_k / d(rho, sigma, tau)_c, and the surrogate point
+ * evaluation with the analytic fallback. */
+int ox_mlp_forward(long n, const double* U, const double* P, double* F, double* dU);
+int ox_mlp_loss_grad(long n, const double* U, const double* T, const double* C, double cs, const double* P,
+                     double* G, double* loss);
+int ox_mlp_descriptors(long n, const double* rho, const double* sigma, const double* tau, double* t, double* dt);
+int ox_mlp_point(const double* P, const double* lo, const double* hi, double rho_min, int ncodes, const int* codes,
+                 long n, const double* rho, const double* sigma, const double* tau, double* exc, double* vrho,
+                 double* vsigma, double* vtau, int* used_net);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,11 @@ def lib():
         "ox_get_density": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_f": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long] + [_vp] * 6),
         "ox_fock_u": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+        "ox_mlp_forward": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp]),
+        "ox_mlp_loss_grad": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
+        "ox_mlp_descriptors": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
+        "ox_mlp_point": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _vp, ctypes.c_long]
+                         + [_vp] * 8),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -213,3 +218,52 @@ def eval_functional_f(codes, rho, sigma=None):
     _check(L.ox_eval_functional_f(len(cc), _ptr(cc), n, _ptr(rho), _ptr(sigma), _ptr(vs), _ptr(frr), _ptr(frs),
                                   _ptr(fss)))
     return vs, frr, frs, fss
+
+
+MLP_PARAMS = 8641
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def mlp_forward(P, U):
+    """FP64 network value F and input gradient dF/dU at normalised inputs U (n x 3)."""
+    P, U = _d(P), _d(U).reshape(-1, 3)
+    n = U.shape[0]
+    F, dU = np.zeros(n), np.zeros((n, 3))
+    _check(lib().ox_mlp_forward(n, _ptr(U), _ptr(P), _ptr(F), _ptr(dU)))
+    return F, dU
+
+
+def mlp_loss_grad(P, U, T, C=None):
+    """L = sum C_i (y_i - T_i)^2 / sum C_i and dL/dP (FP64)."""
+    P, U, T = _d(P), _d(U).reshape(-1, 3), _d(T)
+    n = U.shape[0]
+    Cd = None if C is None else _d(C)
+    cs = 1.0 / (n if C is None else float(Cd.sum()))
+    G, L = np.zeros(MLP_PARAMS), ctypes.c_double()
+    _check(lib().ox_mlp_loss_grad(n, _ptr(U), _ptr(T), _ptr(Cd), cs, _ptr(P), _ptr(G), ctypes.byref(L)))
+    return L.value, G
+
+
+def mlp_descriptors(rho, sigma, tau):
+    """t = (ln rho, s/(1+s), alpha/(1+alpha)) (n x 3) and dt/d(rho, sigma, tau) (n x 3 x 3)."""
+    rho, sigma, tau = _d(rho), _d(sigma), _d(tau)
+    n = rho.size
+    t, dt = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    _check(lib().ox_mlp_descriptors(n, _ptr(rho), _ptr(sigma), _ptr(tau), _ptr(t), _ptr(dt)))
+    return t, dt
+
+
+def mlp_point(P, lo, hi, rho_min, codes, rho, sigma, tau):
+    """Surrogate eps, v_rho, v_sigma, v_tau with the analytic fallback; used_net flags."""
+    P, lo, hi = _d(P), _d(lo), _d(hi)
+    rho, sigma, tau = _d(rho), _d(sigma), _d(tau)
+    c = np.ascontiguousarray(codes, dtype=np.int32)
+    n = rho.size
+    out = [np.zeros(n) for _ in range(4)]
+    used = np.zeros(n, dtype=np.int32)
+    _check(lib().ox_mlp_point(_ptr(P), _ptr(lo), _ptr(hi), rho_min, c.size, _ptr(c), n, _ptr(rho), _ptr(sigma),
+                              _ptr(tau), *[_ptr(o) for o in out], _ptr(used)))
+    return out[0], out[1], out[2], out[3], used.astype(bool)
