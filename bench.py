@@ -42,6 +42,9 @@ def main():
                     help="also run a plain streaming kernel (torch.neg on float32) that reads and writes this many "
                          "MiB, three times: the known-size reference for the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--ao-cache", type=int, default=-1, help="-1 auto, 0 recompute AO values, 1 cache them")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="meta-GGA workloads: skip the learned-functional leg (training + a build with it)")
+    ap.add_argument("--surrogate-epochs", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -194,6 +197,36 @@ def main():
     }
     E, V, N = chosen["E"], chosen["V"], chosen["N"]
 
+    surr = None
+    if xc.family >= 2 and not args.no_surrogate:
+        # learned functional: trained on the r2SCAN samples of the H2O, (H2O)16 and alkane workload
+        # grids (not this molecule), then attached for a mixed build of this workload
+        from chinium_amd.surrogate import Surrogate, workload_samples
+        t0 = time.perf_counter()
+        rho_s, sig_s, tau_s, w_s = workload_samples(n_alkanes=8)
+        t_samples = time.perf_counter() - t0
+        net = Surrogate(11)
+        t0 = time.perf_counter()
+        loss = net.fit(xc, rho_s, sig_s, tau_s, w_s, epochs=args.surrogate_epochs)
+        t_train = time.perf_counter() - t0
+        xc.setSurrogate(net)
+        with_s = measure(ca.CHX_PREC_MIXED)
+        with_s["fallback"] = with_s["grid"].fallbackCount()
+        xc.setSurrogate(None)
+        surr = dict(net=net, r=with_s)
+        out["surrogate"] = {
+            "network": "3-64-64-64-1 tanh on (ln rho, s/(1+s), alpha/(1+alpha)), BF16 weights, FP32 arithmetic",
+            "training": {"samples": int(rho_s.size), "epochs": args.surrogate_epochs, "batch": 16384,
+                         "sample_s": round(t_samples, 3), "train_s": round(t_train, 3),
+                         "loss_first": float(loss[0]), "loss_last": float(loss[-1])},
+            "ms_per_step": round(with_s["ms"], 4),
+            "kernels_ms": {k: round(v[0] / max(1, v[1]), 4) for k, v in with_s["kt"].items()},
+            "functional_ms_analytic": round(chosen["kt"]["functional"][0] / max(1, chosen["kt"]["functional"][1]), 4),
+            "fallback_points": with_s["fallback"],
+            "E_xc": with_s["E"],
+            "speedup_vs_plain_fp64": round(plain["ms"] / with_s["ms"], 3),
+        }
+
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as ol
@@ -212,6 +245,12 @@ def main():
                                           "max_dV": float(np.abs(V - Vo).max()), "dN_e": abs(N - No)}
         out["plain_fp64"]["accuracy_vs_fp64_oracle"] = {"dE_xc": abs(plain["E"] - Eo),
                                                         "max_dV": float(np.abs(plain["V"] - Vo).max())}
+        if surr is not None:
+            rs = surr["r"]
+            out["surrogate"]["accuracy_vs_fp64_oracle"] = {"dE_xc": abs(rs["E"] - Eo),
+                                                           "dE_xc_per_atom": abs(rs["E"] - Eo) / natoms,
+                                                           "max_dV": float(np.abs(rs["V"] - Vo).max()),
+                                                           "dN_e": abs(rs["N"] - No)}
     if args.copy_probe_mib:
         import torch
         a = torch.ones(args.copy_probe_mib * (1 << 18), dtype=torch.float32, device="cuda")
