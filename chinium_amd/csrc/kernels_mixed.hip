@@ -630,6 +630,10 @@ void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, c
 // step's fragments, read the wave's 16 into registers, barrier, start the DMA
 // of the next step, run the wave's 48 MFMAs. 4 waves in a 2 x 2 grid, each
 // owning a 64 x 64 block of M. One FP64 atomic add per element per work item.
+// Meta-GGA (NC = 4): the tau part sum_r (d_r Phi)^T (t d_r Phi) is symmetric
+// and V = M + M^T, so an item with I > J skips it, an item with I < J takes it
+// twice (its tau steps run first, then the accumulators are doubled before the
+// phi steps), and I = J takes it once.
 constexpr int kVxA = 16;  // Phi^T fragments per step: [row block (8)][hl]
 struct VxcMxSmem {
   u32x4 F[2 * kVxA * kDxFrag];  // one K step's fragments: Phi^T_I, then B^T_J
@@ -672,9 +676,18 @@ __global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev marg, const MolDev* __
     off[j] = (f & 1) * (kTile * kBatch) + (16 * ((f >> 1) & 1) + drow) * kBatch + 8 * dch;
   }
   __syncthreads();
-  // K step k -> (batch of the item, half of its points, component)
+  const int nbat = wk.w - wk.z;                       // items hold >= 1 batch
+  const int ntau = (NC == 4 && I <= J) ? 6 * nbat : 0;  // tau steps, first
+  const int nstage = ntau + 2 * nbat;
+  // K step k -> (batch of the item, half of its points, component): tau steps
+  // (it, h, c = 1..3) for k < ntau, then the phi steps (it, h, c = 0)
   auto dma = [&](int k) {
-    const int it = k / (2 * NC), h = (k / NC) % 2, c = k % NC;
+    int it, h, c;
+    if (k < ntau) {
+      it = k / 6; h = (k / 3) % 2; c = 1 + k % 3;
+    } else {
+      it = (k - ntau) >> 1; h = (k - ntau) & 1; c = 0;
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const u16* pa = cache + (size_t)sm.sl[it][j] * sstride + c * kMxT + h * 32;
@@ -688,9 +701,14 @@ __global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev marg, const MolDev* __
   for (int r = 0; r < 4; ++r)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int nstage = (wk.w - wk.z) * 2 * NC;  // items hold >= 1 batch
   dma(0);
   for (int k = 0; k < nstage; ++k) {
+    if (NC == 4 && k == ntau && I < J) {  // 2 M_tau for the item's (I, J) and (J, I) blocks
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] *= 2.f;
+    }
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
     __syncthreads();                     // ... and every other wave's
     u32x4 a[4][2], bq[4][2];
