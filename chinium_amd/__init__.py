@@ -259,6 +259,7 @@ class Grid:
         n = Ds[0].shape[0]
         buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for D in Ds])
         _check(lib().chx_grid_get_density(self.h, _ptr(buf), len(Ds), n))
+        self._nspin = len(Ds)
 
     def getDensitySkeleton(self, Ds):
         """SubGrid::getDensitySkeleton (GridDensity.cpp:79-130), closed shell."""
@@ -312,6 +313,7 @@ class Grid:
 
     def getDensityDevice(self, ptr, nspin, nbasis):
         _check(lib().chx_grid_get_density_device(self.h, ctypes.c_void_p(ptr), int(nspin), int(nbasis)))
+        self._nspin = int(nspin)
 
     def getNumElectrons(self):
         v = ctypes.c_double()
@@ -323,7 +325,10 @@ class Grid:
         _check(lib().chx_grid_energy(self.h, ctypes.byref(v)))
         return v.value
 
-    def getFock(self, nspin=1):
+    def getFock(self, nspin=None):
+        """Grid::getFock (src/Grid/Grid.cpp:156-171): one matrix per spin of the last getDensity."""
+        if nspin is None:
+            nspin = getattr(self, "_nspin", 1)
         n = self.nbasis
         F = np.zeros(nspin * n * n)
         _check(lib().chx_grid_fock(self.h, _ptr(F), nspin, n))
@@ -408,7 +413,15 @@ class ExchangeCorrelation:
         self._load_info()
 
     def eval_points(self, rho, sigma=None, tau=None):
-        """exc, v_rho, v_sigma, v_tau at the given points, evaluated on the device."""
+        """exc, v_rho, v_sigma, v_tau at the given points, evaluated on the device. Spin = 2:
+        rho (n x 2), sigma (n x 3) in libxc's polarised layout; returns exc, v_rho (n x 2), v_sigma (n x 3)."""
+        if self.Spin == 2:
+            rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
+            n = rho.shape[0]
+            sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1, 3)
+            e, vr, vs = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
+            _check(lib().chx_xc_eval_points(self.h, n, _ptr(rho), _ptr(sg), None, _ptr(e), _ptr(vr), _ptr(vs), None))
+            return e, vr, vs
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         n = len(rho)
         sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)

@@ -88,9 +88,22 @@ void require(bool ok, int code, const char* msg) {
 void make_stream(chx_grid* g) { ensure_stream(g); }
 
 void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
+  if (xc->spin != g->nspin)
+    throw Error(CHX_ERR_STATE, "the functional's spin count differs from the density's (ExchangeCorrelation::Spin)");
   g->d_codes.upload(xc->codes, g->stream);
   hipEvent_t e = nullptr;
   g->surrogate_used = false;
+  if (g->nspin == 2) {  // spin-polarised LDA/GGA: alpha coefficients in d_coef, beta in d_coef_b
+    if (xc->family > 1) throw Error(CHX_ERR_UNSUPPORTED, "the spin-polarised path is built for LDA and GGA functionals");
+    g->d_coef_b.alloc(g->d_coef.n);
+    g->mark_begin(2, &e);
+    launch_functional_pol(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_b.p, g->d_w.p, g->d_codes.p,
+                          (int)xc->codes.size(), xc->family, want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p, g->d_coef.p,
+                          g->d_coef_b.p, g->d_en_part.p, g->en_blocks);
+    launch_reduce_en(g->stream, g->en_blocks, g->d_en_part.p, g->d_en.p);
+    g->mark_end(2, e);
+    return;
+  }
   if (xc->surrogate && xc->family == 2) {  // learned functional, one partial sum per 64-slot block
     const chx_mlp* m = xc->surrogate;
     if (g->d_en_part.n < 2 * (size_t)g->nbatches) g->d_en_part.alloc(2 * (size_t)g->nbatches);
@@ -201,10 +214,41 @@ void fock_to_device(chx_grid* g, double* dV) {
   g->mark_end(kStFinalize, e);
 }
 
-void check_dm_args(const chx_grid* g, int nspin, int nbasis) {
+void check_dm_args(const chx_grid* g, int nspin, int nbasis, int max_spin = 1) {
   require(g->ao_ready, CHX_ERR_STATE, "Grid::getAO must be called before the density");
-  require(nspin == 1, CHX_ERR_UNSUPPORTED, "Wrong number of spin types! (only the closed-shell path is built)");
+  require(nspin >= 1 && nspin <= 2, CHX_ERR_ARG, "Wrong number of spin types!");
+  require(nspin <= max_spin, CHX_ERR_UNSUPPORTED, "this call is built for the closed-shell (nspin = 1) path");
   require(nbasis == g->nbasis, CHX_ERR_ARG, "density matrix size does not match the basis");
+}
+
+// Density of one or two spin matrices at dD (device, back to back): beta first,
+// its partials kept in d_part_b, then alpha in d_part (GridDensity.cpp:12-31
+// per spin index).
+void density_spin(chx_grid* g, const double* dD, int nspin) {
+  if (nspin == 2) {
+    require(g->type <= 1, CHX_ERR_UNSUPPORTED, "the spin-polarised path is built for LDA and GGA grids");
+    density_from_device(g, dD + (size_t)g->nbasis * g->nbasis);
+    g->d_part_b.alloc(g->d_part.n);
+    CHX_HIP(hipMemcpyAsync(g->d_part_b.p, g->d_part.p, sizeof(double) * g->d_part.n, hipMemcpyDeviceToDevice,
+                           g->stream));
+  }
+  density_from_device(g, dD);
+  g->nspin = nspin;
+}
+
+// Vxc of each spin: alpha from d_coef, then beta with d_coef_b swapped in
+// (the alpha coefficients are restored afterwards).
+void fock_spin(chx_grid* g, double* dV, int nspin) {
+  require(nspin == g->v_nspin, CHX_ERR_ARG, "Fock spin count differs from the evaluated density's");
+  fock_to_device(g, dV);
+  if (nspin == 2) {
+    const size_t nc = g->d_coef.n, nb2 = (size_t)g->nbasis * g->nbasis;
+    g->d_coef_save.alloc(nc);
+    CHX_HIP(hipMemcpyAsync(g->d_coef_save.p, g->d_coef.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, g->stream));
+    CHX_HIP(hipMemcpyAsync(g->d_coef.p, g->d_coef_b.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, g->stream));
+    fock_to_device(g, dV + nb2);
+    CHX_HIP(hipMemcpyAsync(g->d_coef.p, g->d_coef_save.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, g->stream));
+  }
 }
 }  // namespace
 
@@ -391,10 +435,12 @@ int chx_grid_get_ao(chx_grid* g, int derivative, int output) {
 
 int chx_grid_get_density(chx_grid* g, const double* D, int nspin, int nbasis) {
   return guard([&] {
-    check_dm_args(g, nspin, nbasis);
+    check_dm_args(g, nspin, nbasis, 2);
     require(D != nullptr, CHX_ERR_ARG, "null density matrix");
-    CHX_HIP(hipMemcpyAsync(g->d_Din.p, D, sizeof(double) * (size_t)nbasis * nbasis, hipMemcpyHostToDevice, g->stream));
-    density_from_device(g, g->d_Din.p);
+    const size_t n = (size_t)nspin * nbasis * nbasis;
+    if (g->d_Din.n < n) g->d_Din.alloc(n);
+    CHX_HIP(hipMemcpyAsync(g->d_Din.p, D, sizeof(double) * n, hipMemcpyHostToDevice, g->stream));
+    density_spin(g, g->d_Din.p, nspin);
   });
 }
 
@@ -476,6 +522,7 @@ int chx_grid_energy_grad(chx_grid* g, double* grad) {
     require(g->have_skeleton, CHX_ERR_STATE, "Grid::getDensitySkeleton must precede Grid::getEnergyGrad");
     require(g->have_v, CHX_ERR_STATE, "ExchangeCorrelation::Evaluate(\"ev\") must precede Grid::getEnergyGrad");
     require(g->v_family <= 1 && g->type <= 1, CHX_ERR_UNSUPPORTED, "the XC gradient is built for LDA and GGA");
+    require(g->v_nspin == 1, CHX_ERR_UNSUPPORTED, "the XC gradient is built for the closed-shell path");
     const int na = g->mol.natoms, ng = (int)g->grp_batch.size(), ns = (int)g->bt_list.size();
     if (g->slot_grp.size() != (size_t)ns) {  // slot -> (density group, column tile in the group)
       g->slot_grp.assign(ns, 0);
@@ -517,6 +564,7 @@ int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double
     g->profile = false;
     auto build = [&] {
       density_from_device(g, D_dev);
+      g->nspin = 1;
       run_functional(g, xc, true, true);
       g->v_family = xc->family;
       g->have_eval = g->have_v = true;
@@ -550,9 +598,9 @@ int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double
 
 int chx_grid_get_density_device(chx_grid* g, const double* dD, int nspin, int nbasis) {
   return guard([&] {
-    check_dm_args(g, nspin, nbasis);
+    check_dm_args(g, nspin, nbasis, 2);
     require(dD != nullptr, CHX_ERR_ARG, "null density matrix");
-    density_from_device(g, dD);
+    density_spin(g, dD, nspin);
   });
 }
 
@@ -562,6 +610,8 @@ int chx_grid_num_electrons(chx_grid* g, double* N) {
     if (!g->have_eval) {
       chx_xc none;
       none.family = g->type < 0 ? 0 : g->type;
+      if (g->nspin == 2) none.family = std::min(none.family, 1);
+      none.spin = g->nspin;
       run_functional(g, &none, false, false);
     }
     double en[2];
@@ -586,10 +636,11 @@ int chx_grid_energy(chx_grid* g, double* E) {
 int chx_grid_fock(chx_grid* g, double* F, int nspin, int nbasis) {
   return guard([&] {
     require(g->have_v, CHX_ERR_STATE, "no potential on the grid (Evaluate with \"v\" or \"ev\" first)");
-    require(nspin == 1, CHX_ERR_UNSUPPORTED, "only the closed-shell path is built");
     require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
-    fock_to_device(g, g->d_Vout.p);
-    CHX_HIP(hipMemcpyAsync(F, g->d_Vout.p, sizeof(double) * (size_t)nbasis * nbasis, hipMemcpyDeviceToHost, g->stream));
+    const size_t n = (size_t)nspin * nbasis * nbasis;
+    if (g->d_Vout.n < n) g->d_Vout.alloc(n);
+    fock_spin(g, g->d_Vout.p, nspin);
+    CHX_HIP(hipMemcpyAsync(F, g->d_Vout.p, sizeof(double) * n, hipMemcpyDeviceToHost, g->stream));
     CHX_HIP(hipStreamSynchronize(g->stream));
   });
 }
@@ -597,9 +648,8 @@ int chx_grid_fock(chx_grid* g, double* F, int nspin, int nbasis) {
 int chx_grid_fock_device(chx_grid* g, double* dF, int nspin, int nbasis) {
   return guard([&] {
     require(g->have_v, CHX_ERR_STATE, "no potential on the grid (Evaluate with \"v\" or \"ev\" first)");
-    require(nspin == 1, CHX_ERR_UNSUPPORTED, "only the closed-shell path is built");
     require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
-    fock_to_device(g, dF);
+    fock_spin(g, dF, nspin);
   });
 }
 
@@ -620,6 +670,7 @@ int chx_grid_set_vrho(chx_grid* g, const double* vrho, long npoints) {
     CHX_HIP(hipStreamSynchronize(g->stream));
     g->have_v = true;
     g->v_family = 0;
+    g->v_nspin = 1;
   });
 }
 
@@ -687,7 +738,7 @@ int chx_grid_stats(const chx_grid* g, long* nbatches, long* npoints, double* mea
 int chx_xc_create(int ncodes, const int* codes, int spin, chx_xc** out) {
   return guard([&] {
     require(ncodes > 0 && codes && out, CHX_ERR_ARG, "bad functional arguments");
-    require(spin == 1, CHX_ERR_UNSUPPORTED, "only the spin-unpolarised functional path is built");
+    require(spin == 1 || spin == 2, CHX_ERR_ARG, "spin must be 1 (unpolarised) or 2 (polarised)");
     chx_xc* x = new chx_xc;
     x->spin = spin;
     for (int i = 0; i < ncodes; ++i) {
@@ -695,6 +746,10 @@ int chx_xc_create(int ncodes, const int* codes, int spin, chx_xc** out) {
       if (f < 0) {
         delete x;
         throw Error(CHX_ERR_UNSUPPORTED, "functional code " + std::to_string(codes[i]) + " is not implemented");
+      }
+      if (spin == 2 && f > 1) {
+        delete x;
+        throw Error(CHX_ERR_UNSUPPORTED, "spin-polarised meta-GGA functionals are not built");
       }
       x->codes.push_back(codes[i]);
       x->family = std::max(x->family, f);
@@ -756,6 +811,7 @@ int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order) {
     require(g->type >= xc->family, CHX_ERR_STATE, "grid type is below the functional family (call setType)");
     if (o == "f") {  // f_xc at the ground-state density, for the response path
       require(xc->family <= 1, CHX_ERR_UNSUPPORTED, "Evaluate(\"f\") is built for LDA and GGA functionals");
+      require(g->nspin == 1 && xc->spin == 1, CHX_ERR_UNSUPPORTED, "the response path is built for nspin = 1");
       require(g->precision == CHX_PREC_FP64, CHX_ERR_UNSUPPORTED, "the response path is built for the FP64 form");
       g->d_codes.upload(xc->codes, g->stream);
       const size_t nslots = (size_t)g->nbatches * kBatch;
@@ -769,7 +825,10 @@ int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order) {
     require(xc->family < 2 || g->cached, CHX_ERR_UNSUPPORTED,
             "meta-GGA functionals need the AO tables in HBM (chx_grid_set_ao_cache)");
     run_functional(g, xc, e, v);
-    if (v) g->v_family = xc->family;
+    if (v) {
+      g->v_family = xc->family;
+      g->v_nspin = g->nspin;
+    }
     g->have_eval = e || g->have_eval;
     g->have_v = v;
   });
@@ -784,6 +843,27 @@ int chx_lebedev_rule(int npoints, double* xyzw) {
 
 int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
                        double* exc, double* vrho, double* vsigma, double* vtau) {
+  if (xc && xc->spin == 2)
+    return guard([&] {  // libxc XC_POLARIZED layout: rho[2n], sigma[3n], vrho[2n], vsigma[3n]
+      require(n >= 0 && rho && exc && vrho, CHX_ERR_ARG, "bad arguments");
+      if (n == 0) return;
+      hipStream_t s;
+      CHX_HIP(hipStreamCreate(&s));
+      DevBuf<double> dr, ds, de, dvr, dvs;
+      DevBuf<int> dc;
+      dr.upload(std::vector<double>(rho, rho + 2 * n), s);
+      if (sigma) ds.upload(std::vector<double>(sigma, sigma + 3 * n), s);
+      dc.upload(xc->codes, s);
+      de.alloc(n); dvr.alloc(2 * n); dvs.alloc(3 * n);
+      launch_eval_functional_pol(s, n, dc.p, (int)xc->codes.size(), dr.p, sigma ? ds.p : nullptr, de.p, dvr.p, dvs.p);
+      CHX_HIP(hipMemcpyAsync(exc, de.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+      CHX_HIP(hipMemcpyAsync(vrho, dvr.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
+      if (vsigma) CHX_HIP(hipMemcpyAsync(vsigma, dvs.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+      CHX_HIP(hipStreamSynchronize(s));
+      CHX_HIP(hipStreamDestroy(s));
+      (void)tau;
+      (void)vtau;
+    });
   return guard([&] {
     require(xc && n >= 0 && rho && exc && vrho, CHX_ERR_ARG, "bad arguments");
     hipStream_t s;

@@ -134,6 +134,7 @@ void run(chx_pack* p, const chx_xc* xc, const double* const* D, double* const* V
   const int n = (int)p->grids.size();
   prequire(xc->family <= p->type, CHX_ERR_STATE, "grid type is below the functional family (call setType)");
   prequire(!xc->surrogate, CHX_ERR_UNSUPPORTED, "packs run the analytic functional; detach the surrogate");
+  prequire(xc->spin == 1, CHX_ERR_UNSUPPORTED, "packs run the closed-shell (nspin = 1) path");
   if (xc->codes != p->codes || xc->family != p->family) {
     p->codes = xc->codes;
     p->family = xc->family;

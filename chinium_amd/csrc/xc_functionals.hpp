@@ -382,4 +382,225 @@ __device__ __forceinline__ void eval_codes_f(const int* codes, int ncodes, doubl
   *v2sigma2 = fss;
 }
 
+// ---- spin-polarised LDA/GGA (libxc XC_POLARIZED conventions) --------------
+// Inputs rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb; outputs eps per particle
+// of the total density and the five first derivatives of e = rho eps. The
+// number type P5 carries those five partials.
+struct P5 {
+  double v, d[5];
+};
+__device__ __forceinline__ P5 pconst(double x) { return {x, {0.0, 0.0, 0.0, 0.0, 0.0}}; }
+__device__ __forceinline__ P5 pvar(double x, int k) {
+  P5 r = pconst(x);
+  r.d[k] = 1.0;
+  return r;
+}
+__device__ __forceinline__ P5 pchain(const P5& a, double f, double df) {
+  P5 r;
+  r.v = f;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.d[k] = df * a.d[k];
+  return r;
+}
+__device__ __forceinline__ P5 operator+(const P5& a, const P5& b) {
+  P5 r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+__device__ __forceinline__ P5 operator-(const P5& a) { return pchain(a, -a.v, -1.0); }
+__device__ __forceinline__ P5 operator-(const P5& a, const P5& b) { return a + (-b); }
+__device__ __forceinline__ P5 operator*(const P5& a, const P5& b) {
+  P5 r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+__device__ __forceinline__ P5 operator/(const P5& a, const P5& b) {
+  const double ib = 1.0 / b.v, q = a.v * ib;
+  P5 r;
+  r.v = q;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.d[k] = (a.d[k] - q * b.d[k]) * ib;
+  return r;
+}
+__device__ __forceinline__ P5 operator+(const P5& a, double b) { P5 r = a; r.v += b; return r; }
+__device__ __forceinline__ P5 operator+(double a, const P5& b) { return b + a; }
+__device__ __forceinline__ P5 operator-(const P5& a, double b) { return a + (-b); }
+__device__ __forceinline__ P5 operator-(double a, const P5& b) { return (-b) + a; }
+__device__ __forceinline__ P5 operator*(const P5& a, double b) { return pchain(a, a.v * b, b); }
+__device__ __forceinline__ P5 operator*(double a, const P5& b) { return b * a; }
+__device__ __forceinline__ P5 operator/(const P5& a, double b) { return a * (1.0 / b); }
+__device__ __forceinline__ P5 operator/(double a, const P5& b) { return pconst(a) / b; }
+__device__ __forceinline__ P5 dexp(const P5& a) { const double e = ::exp(a.v); return pchain(a, e, e); }
+__device__ __forceinline__ P5 dlog(const P5& a) { return pchain(a, ::log(a.v), 1.0 / a.v); }
+__device__ __forceinline__ P5 dsqrt(const P5& a) { const double s = ::sqrt(a.v); return pchain(a, s, 0.5 / s); }
+__device__ __forceinline__ P5 datan(const P5& a) { return pchain(a, ::atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+__device__ __forceinline__ P5 dasinh(const P5& a) { return pchain(a, ::asinh(a.v), 1.0 / ::sqrt(1.0 + a.v * a.v)); }
+__device__ __forceinline__ P5 dcbrt(const P5& a) { const double c = ::cbrt(a.v); return pchain(a, c, c / (3.0 * a.v)); }
+// x^p for x > 0 (a power of 1 +- zeta)
+__device__ __forceinline__ P5 dpowp(const P5& a, double p) {
+  const double f = ::pow(a.v, p);
+  return pchain(a, f, p * f / a.v);
+}
+
+// f(zeta) of the spin interpolation and its curvature at 0, 4 / (9 (2^(1/3) - 1)).
+constexpr double kFpp0 = 1.709920934161365617563962776245;
+__device__ __forceinline__ P5 f_zeta(const P5& opz, const P5& omz) {  // opz = 1 + zeta, omz = 1 - zeta
+  return (opz * dcbrt(opz) + omz * dcbrt(omz) - 2.0) / (2.5198420997897464 - 2.0);  // 2^(4/3) - 2
+}
+
+// VWN with spin (Vosko-Wilk-Nusair 1980, eq. 4.4 with their spin-stiffness
+// interpolation, as libxc's lda_c_vwn): eps = eP + a f (1 - z^4)/f''(0) + (eF - eP) f z^4.
+__device__ __forceinline__ P5 vwn5_eps_pol(const P5& rs, const P5& z, const P5& opz, const P5& omz) {
+  const P5 eP = vwn_eps(rs, 0.0310907, 3.72744, 12.9352, -0.10498);
+  const P5 eF = vwn_eps(rs, 0.01554535, 7.06042, 18.0578, -0.32500);
+  const P5 ea = vwn_eps(rs, -1.0 / (6.0 * kPi * kPi), 1.13107, 13.0045, -0.0047584);
+  const P5 f = f_zeta(opz, omz);
+  const P5 z4 = z * z * z * z;
+  return eP + ea * f * (1.0 - z4) / kFpp0 + (eF - eP) * f * z4;
+}
+// VWN RPA fit with the plain f(zeta) interpolation (libxc lda_c_vwn_rpa).
+__device__ __forceinline__ P5 vwn_rpa_eps_pol(const P5& rs, const P5& opz, const P5& omz) {
+  const P5 eP = vwn_eps(rs, 0.0310907, 13.0720, 42.7198, -0.409286);
+  const P5 eF = vwn_eps(rs, 0.01554535, 20.1231, 101.578, -0.743294);
+  return eP + (eF - eP) * f_zeta(opz, omz);
+}
+
+// PW92 G(rs) of Perdew-Wang 1992 eq. 10.
+__device__ __forceinline__ P5 pw92_g(const P5& rs, double A, double a1, double b1, double b2, double b3, double b4) {
+  const P5 srs = dsqrt(rs);
+  const P5 den = (2.0 * A) * (srs * (b1 + b3 * rs) + rs * (b2 + b4 * rs));
+  return (-2.0 * A) * (1.0 + a1 * rs) * dlog(1.0 + 1.0 / den);
+}
+// PW92 with spin (eq. 8), libxc PW_MOD parameters; -alpha_c = G(rs; A_alpha, ...).
+__device__ __forceinline__ P5 pw92_eps_pol(const P5& rs, const P5& z, const P5& opz, const P5& omz) {
+  const P5 e0 = pw92_g(rs, 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  const P5 e1 = pw92_g(rs, 0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  const P5 ma = pw92_g(rs, 0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
+  const P5 f = f_zeta(opz, omz);
+  const P5 z4 = z * z * z * z;
+  return e0 - ma * f * (1.0 - z4) / kFpp0 + (e1 - e0) * f * z4;
+}
+
+// PBE correlation with spin: phi = ((1+z)^(2/3) + (1-z)^(2/3))/2,
+// t^2 = |grad rho|^2 / (4 phi^2 ks^2 rho^2), H = gamma phi^3 ln(1 + beta/gamma t^2 (1+At^2)/(1+At^2+A^2t^4)).
+__device__ __forceinline__ P5 e_pbe_c_pol(const P5& rho, const P5& rho13, const P5& gg, const P5& z, const P5& opz,
+                                          const P5& omz, double beta) {
+  const double gamma = 0.031090690869654895;
+  const P5 rs = rs_from_rho13(rho13);
+  const P5 ec = pw92_eps_pol(rs, z, opz, omz);
+  const P5 c1 = dcbrt(opz), c2 = dcbrt(omz);
+  const P5 phi = 0.5 * (c1 * c1 + c2 * c2);
+  const P5 phi3 = phi * phi * phi;
+  const P5 kf = 3.0936677262801355 * rho13;
+  const P5 t2 = gg / ((16.0 / kPi) * kf * rho * rho * phi * phi);
+  const P5 A = (beta / gamma) / (dexp(-ec / (gamma * phi3)) - 1.0);
+  const P5 At2 = A * t2;
+  const P5 H = gamma * phi3 * dlog(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  return rho * (ec + H);
+}
+
+// LYP correlation, general spin form (Miehlich, Savin, Stoll, Preuss 1989, eq. 2).
+__device__ __forceinline__ P5 e_lyp_c_pol(const P5& ra, const P5& rb, const P5& rho, const P5& rho13, const P5& gaa,
+                                          const P5& gab, const P5& gbb) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double CF = 2.8712340001881915;
+  const P5 rm13 = 1.0 / rho13;
+  const P5 inv_den = 1.0 / (1.0 + d * rm13);
+  const P5 delta = rm13 * (c + d * inv_den);
+  const P5 omega = dexp(-c * rm13) * inv_den * (rm13 * rm13 / (rho * rho * rho));  // rho^(-11/3)
+  const P5 gg = gaa + 2.0 * gab + gbb;
+  auto p83 = [](const P5& x) {  // x^(8/3), 0 for an empty channel
+    if (!(x.v > 0.0)) return pconst(0.0);
+    const P5 c13 = dcbrt(x);
+    return x * x * c13 * c13;
+  };
+  const P5 ra83 = p83(ra), rb83 = p83(rb);
+  const P5 inner = ra * rb * (12.699208415745595 * CF * (ra83 + rb83) + (47.0 / 18.0 - (7.0 / 18.0) * delta) * gg -
+                              (2.5 - delta / 18.0) * (gaa + gbb) - (delta - 11.0) / 9.0 * (ra / rho * gaa + rb / rho * gbb)) -
+                   (2.0 / 3.0) * rho * rho * gg + ((2.0 / 3.0) * rho * rho - ra * ra) * gbb +
+                   ((2.0 / 3.0) * rho * rho - rb * rb) * gaa;
+  return -4.0 * a * inv_den * ra * rb / rho - (a * b) * omega * inner;
+}
+
+// Exchange by spin scaling: E_x[ra, rb] = (E_x[2 ra] + E_x[2 rb]) / 2.
+__device__ __forceinline__ P5 e_x_channel(int code, const P5& r2, const P5& s4) {
+  const P5 r13 = dcbrt(r2);
+  switch (code) {
+    case kLdaX: return 0.5 * e_lda_x(r2, r13);
+    case kGgaXPbe: return 0.5 * e_pbe_x(r2, r13, s4, 0.804, 0.2195149727645171);
+    case kGgaXPbeG: return 0.5 * e_pbe_x(r2, r13, s4, 0.804, 0.219510);
+    case kGgaXB88: return 0.5 * e_b88_x(r2, r13, s4);
+    default: return pconst(0.0);
+  }
+}
+
+// Energy density of one LDA/GGA code with spin. ra_x / rb_x: the exchange
+// channels (zero below the threshold, then skipped).
+__device__ __forceinline__ P5 energy_density_pol(int code, const P5& ra, const P5& rb, const P5& saa, const P5& sab,
+                                                 const P5& sbb, bool a_on, bool b_on) {
+  auto ex = [&](int xc) {
+    P5 e = pconst(0.0);
+    if (a_on) e = e + e_x_channel(xc, 2.0 * ra, 4.0 * saa);
+    if (b_on) e = e + e_x_channel(xc, 2.0 * rb, 4.0 * sbb);
+    return e;
+  };
+  const P5 rho = ra + rb;
+  const P5 rho13 = dcbrt(rho);
+  const P5 z = (ra - rb) / rho;
+  P5 opz = 1.0 + z, omz = 1.0 - z;
+  if (opz.v < 1e-15) opz = pconst(1e-15);  // zeta threshold: fully polarised limit
+  if (omz.v < 1e-15) omz = pconst(1e-15);
+  switch (code) {
+    case kLdaX: case kGgaXPbe: case kGgaXPbeG: case kGgaXB88: return ex(code);
+    case kLdaCVwn: return rho * vwn5_eps_pol(rs_from_rho13(rho13), z, opz, omz);
+    case kLdaCVwnRpa: return rho * vwn_rpa_eps_pol(rs_from_rho13(rho13), opz, omz);
+    case kGgaCPbe: return e_pbe_c_pol(rho, rho13, saa + 2.0 * sab + sbb, z, opz, omz, 0.06672455060314922);
+    case kGgaCPbeG: return e_pbe_c_pol(rho, rho13, saa + 2.0 * sab + sbb, z, opz, omz, 0.06672632268006);
+    case kGgaCLyp: return e_lyp_c_pol(ra, rb, rho, rho13, saa, sab, sbb);
+    case kHybB3lyp:
+      return 0.08 * ex(kLdaX) + 0.72 * ex(kGgaXB88) + 0.19 * (rho * vwn_rpa_eps_pol(rs_from_rho13(rho13), opz, omz)) +
+             0.81 * e_lyp_c_pol(ra, rb, rho, rho13, saa, sab, sbb);
+    default: return pconst(0.0);
+  }
+}
+
+// Sum over codes at one point, spin-polarised LDA/GGA: eps (per particle of
+// rho_a + rho_b), v_rho (a, b), v_sigma (aa, ab, bb). A code contributes
+// nothing where rho_a + rho_b is below the threshold; an exchange channel
+// contributes nothing where its own density is. sigma_aa and sigma_bb are
+// floored at 1e-40, |sigma_ab| capped at sqrt(sigma_aa sigma_bb).
+__device__ __forceinline__ void eval_codes_pol(const int* codes, int ncodes, double ra, double rb, double saa,
+                                               double sab, double sbb, double* eps, double* vr, double* vs) {
+  double e = 0, d[5] = {0, 0, 0, 0, 0};
+  if (ra < 0) ra = 0;
+  if (rb < 0) rb = 0;
+  const double rho = ra + rb;
+  if (rho >= kDensThreshold) {
+    const double sfl = 1e-40;
+    saa = saa > sfl ? saa : sfl;
+    sbb = sbb > sfl ? sbb : sfl;
+    const double cap = ::sqrt(saa * sbb);
+    sab = sab > cap ? cap : (sab < -cap ? -cap : sab);
+    const P5 pa = pvar(ra, 0), pb = pvar(rb, 1), paa = pvar(saa, 2), pab = pvar(sab, 3), pbb = pvar(sbb, 4);
+    const bool a_on = ra >= kDensThreshold, b_on = rb >= kDensThreshold;
+    for (int k = 0; k < ncodes; ++k) {
+      const P5 ed = energy_density_pol(codes[k], pa, pb, paa, pab, pbb, a_on, b_on);
+      e += ed.v;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) d[j] += ed.d[j];
+    }
+    e /= rho;
+  }
+  *eps = e;
+  vr[0] = d[0];
+  vr[1] = d[1];
+  vs[0] = d[2];
+  vs[1] = d[3];
+  vs[2] = d[4];
+}
+
 }  // namespace chx

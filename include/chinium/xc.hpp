@@ -127,6 +127,7 @@ class Grid {
     buf.reserve((size_t)nspin * nbasis_ * nbasis_);
     for (const EigenMatrix& m : D) buf.insert(buf.end(), m.data(), m.data() + (size_t)nbasis_ * nbasis_);
     check(chx_grid_get_density(h_, buf.data(), nspin, nbasis_));
+    nspin_ = nspin;
   }
   double getNumElectrons() {
     double n = 0;
@@ -138,9 +139,14 @@ class Grid {
     check(chx_grid_energy(h_, &e));
     return e;
   }
+  // One matrix per spin of the last getDensity (Grid.cpp:156-171).
   std::vector<EigenMatrix> getFock() {
-    std::vector<EigenMatrix> F(1, make_matrix(nbasis_, nbasis_));
-    check(chx_grid_fock(h_, F[0].data(), 1, nbasis_));
+    std::vector<double> buf((size_t)nspin_ * nbasis_ * nbasis_);
+    check(chx_grid_fock(h_, buf.data(), nspin_, nbasis_));
+    std::vector<EigenMatrix> F(nspin_, make_matrix(nbasis_, nbasis_));
+    for (int s = 0; s < nspin_; ++s)
+      std::copy(buf.begin() + (size_t)s * nbasis_ * nbasis_, buf.begin() + (size_t)(s + 1) * nbasis_ * nbasis_,
+                F[s].data());
     return F;
   }
   // SubGrid::getDensitySkeleton / Grid::getEnergyGrad (GridDensity.cpp:79-130,
@@ -189,6 +195,7 @@ class Grid {
   int nbasis_ = 0;
   int natoms_ = 0;
   int nmats_u_ = 0;
+  int nspin_ = 1;  // spin count of the last getDensity
 };
 
 class ExchangeCorrelation {

@@ -91,7 +91,9 @@ int chx_grid_set_type(chx_grid* g, int type);
  * that those kernels use and must precede the first density call. */
 int chx_grid_get_ao(chx_grid* g, int derivative, int output);
 /* Grid::getDensity(std::vector<EigenMatrix> D) (src/Grid/Grid.cpp:35-49):
- * D is nspin column-major nbasis x nbasis matrices, back to back (host memory). */
+ * D is nspin column-major nbasis x nbasis matrices, back to back (host memory).
+ * nspin = 2 (alpha, beta; Unrestricted/SP.cpp:65-68) is built for LDA and GGA
+ * grids (type <= 1); the functional must then be polarised (spin = 2). */
 int chx_grid_get_density(chx_grid* g, const double* D, int nspin, int nbasis);
 /* Same, with D already in device memory (HBM); used to time the path with
  * inputs resident on the GPU. */
@@ -100,7 +102,8 @@ int chx_grid_get_density_device(chx_grid* g, const double* D_dev, int nspin, int
 int chx_grid_num_electrons(chx_grid* g, double* N);
 /* Grid::getEnergy (src/Grid/Grid.cpp:112-122). */
 int chx_grid_energy(chx_grid* g, double* E);
-/* Grid::getFock (src/Grid/Grid.cpp:156-171): writes nspin matrices (host). */
+/* Grid::getFock (src/Grid/Grid.cpp:156-171): writes nspin matrices (host),
+ * nspin equal to the evaluated density's. */
 int chx_grid_fock(chx_grid* g, double* F, int nspin, int nbasis);
 /* Same, into device memory. */
 int chx_grid_fock_device(chx_grid* g, double* F_dev, int nspin, int nbasis);
@@ -166,7 +169,9 @@ int chx_grid_kernel_times(chx_grid* g, int reset, double* ms /*[6]*/, long* laun
  * ExchangeCorrelation::Read(std::string df, bool output)
  *   src/ExchangeCorrelation.cpp:18-84: reads <data_root>/DensityFunctionals/<df>.df */
 int chx_xc_read(const char* data_root, const char* df_name, int spin, int output, chx_xc** out);
-/* Same from explicit libxc IDs (the codes a .df line holds). */
+/* Same from explicit libxc IDs (the codes a .df line holds). spin: 1
+ * unpolarised, 2 polarised (ExchangeCorrelation::Spin, libxc XC_POLARIZED);
+ * polarised meta-GGA codes are not built (CHX_ERR_UNSUPPORTED). */
 int chx_xc_create(int ncodes, const int* codes, int spin, chx_xc** out);
 void chx_xc_destroy(chx_xc* xc);
 /* Members Family (0 LDA, 1 GGA, 2 mGGA), EXX, Codes (ExchangeCorrelation.h:9-12). */
@@ -184,7 +189,10 @@ int chx_lebedev_rule(int npoints, double* xyzw);
 /* Functional values at n points on the device, with libxc's conventions as the
  * reference receives them from xc_{lda,gga,mgga}_exc_vxc
  * (src/ExchangeCorrelation.cpp:98,122,157): exc per particle and the summed
- * derivatives over the functional's codes. Host arrays; sigma/tau may be NULL. */
+ * derivatives over the functional's codes. Host arrays; sigma/tau may be NULL.
+ * A polarised functional (spin = 2) takes libxc's XC_POLARIZED layout: rho[2n]
+ * (a, b), sigma[3n] (aa, ab, bb), and writes vrho[2n], vsigma[3n] (tau, vtau
+ * unused). */
 int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
                        double* exc, double* vrho, double* vsigma, double* vtau);
 /* Second derivatives at n points, as xc_{lda,gga}_fxc returns them

@@ -12,6 +12,7 @@
 // contributes nothing; sigma is floored at thr^(8/3) and, for meta-GGAs, capped
 // at 8 rho tau.
 #pragma once
+#include <algorithm>
 #include <cmath>
 
 namespace oxf {
@@ -354,6 +355,145 @@ inline void eval_point_f(int code, double rho, double sigma, double* vsigma, dou
   *v2rho2 += e.h[0][0];
   *v2rhosigma += e.h[0][1];
   *v2sigma2 += e.h[1][1];
+}
+
+// ---- Spin-polarised LDA/GGA (libxc XC_POLARIZED): variables rho_a, rho_b,
+// sigma_aa, sigma_ab, sigma_bb (Dual<5>). Exchange by spin scaling
+// E_x[ra, rb] = (E_x[2 ra] + E_x[2 rb]) / 2; correlation with the published
+// spin interpolations.
+typedef Dual<5> P5d;
+
+inline P5d spin_f(const P5d& z) {  // f(zeta) = ((1+z)^(4/3) + (1-z)^(4/3) - 2) / (2^(4/3) - 2)
+  const P5d a = 1.0 + z, b = 1.0 - z;
+  auto p43 = [](const P5d& x) { return x.v > 0 ? pow(x, 4.0 / 3.0) : P5d(0.0); };
+  return (p43(a) + p43(b) - 2.0) / (std::pow(2.0, 4.0 / 3.0) - 2.0);
+}
+inline double spin_fpp0() { return 4.0 / (9.0 * (std::cbrt(2.0) - 1.0)); }
+
+// VWN with spin: Vosko, Wilk, Nusair (1980); libxc 7 uses the spin-stiffness
+// interpolation, libxc 8 (RPA) the plain f(zeta) one.
+constexpr VwnPar kVwn5F = {0.01554535, 7.06042, 18.0578, -0.32500};
+constexpr VwnPar kVwnRpaF = {0.01554535, 20.1231, 101.578, -0.743294};
+inline P5d vwn5_pol_eps(const P5d& rs, const P5d& z) {
+  const VwnPar stiff = {-1.0 / (6.0 * kPi * kPi), 1.13107, 13.0045, -0.0047584};
+  const P5d eP = vwn_eps(rs, kVwn5), eF = vwn_eps(rs, kVwn5F), ac = vwn_eps(rs, stiff);
+  const P5d f = spin_f(z), z4 = z * z * z * z;
+  return eP + ac * f * (1.0 - z4) / spin_fpp0() + (eF - eP) * f * z4;
+}
+inline P5d vwnrpa_pol_eps(const P5d& rs, const P5d& z) {
+  const P5d eP = vwn_eps(rs, kVwnRpa), eF = vwn_eps(rs, kVwnRpaF);
+  return eP + (eF - eP) * spin_f(z);
+}
+
+// PW92 with spin, PW_MOD parameters (eqs. 8, 10).
+inline P5d pw92_G(const P5d& rs, const double* q) {  // q = A, a1, b1, b2, b3, b4
+  const P5d srs = sqrt(rs);
+  const P5d den = 2.0 * q[0] * (q[2] * srs + q[3] * rs + q[4] * rs * srs + q[5] * rs * rs);
+  return -2.0 * q[0] * (1.0 + q[1] * rs) * log(1.0 + 1.0 / den);
+}
+inline P5d pw92_pol_eps(const P5d& rs, const P5d& z) {
+  static const double q0[6] = {0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294};
+  static const double q1[6] = {0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517};
+  static const double qa[6] = {0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671};
+  const P5d e0 = pw92_G(rs, q0), e1 = pw92_G(rs, q1), alpha_c = -pw92_G(rs, qa);
+  const P5d f = spin_f(z), z4 = z * z * z * z;
+  return e0 + alpha_c * f * (1.0 - z4) / spin_fpp0() + (e1 - e0) * f * z4;
+}
+
+// PBE correlation with spin (PRL 77, 3865, eqs. 3, 7-8).
+inline P5d pbe_c_pol(const P5d& rho, const P5d& z, const P5d& gg, double beta) {
+  const double gamma = (1.0 - std::log(2.0)) / (kPi * kPi);
+  const P5d ec = pw92_pol_eps(rs_of(rho), z);
+  auto p23 = [](const P5d& x) { return x.v > 0 ? pow(x, 2.0 / 3.0) : P5d(0.0); };
+  const P5d phi = 0.5 * (p23(1.0 + z) + p23(1.0 - z));
+  const P5d kf = pow(3.0 * kPi * kPi * rho, 1.0 / 3.0);
+  const P5d ks2 = 4.0 * kf / kPi;
+  const P5d t2 = gg / (4.0 * phi * phi * ks2 * rho * rho);
+  const P5d g3 = gamma * phi * phi * phi;
+  const P5d A = (beta / gamma) / (exp(-ec / g3) - 1.0);
+  const P5d At2 = A * t2;
+  return rho * (ec + g3 * log(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2)));
+}
+
+// LYP, general spin form (CPL 157, 200, eq. 2).
+inline P5d lyp_pol(const P5d& ra, const P5d& rb, const P5d& gaa, const P5d& gab, const P5d& gbb) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double CF = 0.3 * std::pow(3.0 * kPi * kPi, 2.0 / 3.0);
+  const P5d rho = ra + rb;
+  const P5d rm13 = pow(rho, -1.0 / 3.0);
+  const P5d den = 1.0 + d * rm13;
+  const P5d omega = exp(-c * rm13) / den * pow(rho, -11.0 / 3.0);
+  const P5d delta = c * rm13 + d * rm13 / den;
+  const P5d g = gaa + gbb + 2.0 * gab;
+  auto p83 = [](const P5d& x) { return x.v > 0 ? pow(x, 8.0 / 3.0) : P5d(0.0); };
+  const P5d bracket = ra * rb * (std::pow(2.0, 11.0 / 3.0) * CF * (p83(ra) + p83(rb)) +
+                                 (47.0 / 18.0 - 7.0 / 18.0 * delta) * g - (2.5 - delta / 18.0) * (gaa + gbb) -
+                                 (delta - 11.0) / 9.0 * (ra / rho * gaa + rb / rho * gbb)) -
+                      2.0 / 3.0 * rho * rho * g + (2.0 / 3.0 * rho * rho - ra * ra) * gbb +
+                      (2.0 / 3.0 * rho * rho - rb * rb) * gaa;
+  return -a * 4.0 / den * ra * rb / rho - a * b * omega * bracket;
+}
+
+// One exchange channel of spin density r_s with |grad r_s|^2 = g_s: E_x[2 r_s, 4 g_s] / 2.
+inline P5d x_channel(int code, const P5d& r_s, const P5d& g_s) {
+  const P5d r2 = 2.0 * r_s, g4 = 4.0 * g_s;
+  switch (code) {
+    case 1: return 0.5 * lda_x(r2);
+    case 101: return 0.5 * gga_x_pbe(r2, g4, 0.804, 0.2195149727645171);
+    case 321: return 0.5 * gga_x_pbe(r2, g4, 0.804, 0.219510);
+    case 106: return 0.5 * gga_x_b88(r2, g4);
+    default: return P5d(0.0);
+  }
+}
+
+inline P5d energy_density_pol(int code, const P5d& ra, const P5d& rb, const P5d& gaa, const P5d& gab,
+                              const P5d& gbb, bool a_on, bool b_on) {
+  auto xs = [&](int c) {
+    P5d e(0.0);
+    if (a_on) e = e + x_channel(c, ra, gaa);
+    if (b_on) e = e + x_channel(c, rb, gbb);
+    return e;
+  };
+  const P5d rho = ra + rb;
+  P5d z = (ra - rb) / rho;
+  if (z.v > 1.0 - 1e-15) z = P5d(1.0 - 1e-15);   // zeta threshold
+  if (z.v < -1.0 + 1e-15) z = P5d(-1.0 + 1e-15);
+  const P5d g = gaa + gbb + 2.0 * gab;
+  switch (code) {
+    case 1: case 101: case 321: case 106: return xs(code);
+    case 7: return rho * vwn5_pol_eps(rs_of(rho), z);
+    case 8: return rho * vwnrpa_pol_eps(rs_of(rho), z);
+    case 130: return pbe_c_pol(rho, z, g, 0.06672455060314922);
+    case 322: return pbe_c_pol(rho, z, g, 0.06672632268006);
+    case 131: return lyp_pol(ra, rb, gaa, gab, gbb);
+    case 402:
+      return 0.08 * xs(1) + 0.72 * xs(106) + 0.19 * (rho * vwnrpa_pol_eps(rs_of(rho), z)) + 0.81 * lyp_pol(ra, rb, gaa, gab, gbb);
+    default: return P5d(0.0);
+  }
+}
+
+// One code at one point, spin-polarised, accumulating eps (per particle of the
+// total density), vrho[2] and vsigma[3]. Same thresholds as the unpolarised
+// path on the total density; an exchange channel below the threshold is
+// skipped; sigma_aa, sigma_bb >= 1e-40 and |sigma_ab| <= sqrt(sigma_aa sigma_bb).
+inline void eval_point_pol(int code, double ra, double rb, double saa, double sab, double sbb, double* eps,
+                           double* vrho, double* vsigma) {
+  ra = ra > 0 ? ra : 0;
+  rb = rb > 0 ? rb : 0;
+  const double thr = density_threshold(code);
+  if (!(ra + rb >= thr)) return;
+  saa = std::max(saa, 1e-40);
+  sbb = std::max(sbb, 1e-40);
+  const double cap = std::sqrt(saa * sbb);
+  sab = std::min(std::max(sab, -cap), cap);
+  const P5d e = energy_density_pol(code, P5d::var(ra, 0), P5d::var(rb, 1), P5d::var(saa, 2), P5d::var(sab, 3),
+                                   P5d::var(sbb, 4), ra >= thr, rb >= thr);
+  *eps += e.v / (ra + rb);
+  vrho[0] += e.d[0];
+  vrho[1] += e.d[1];
+  vsigma[0] += e.d[2];
+  vsigma[1] += e.d[3];
+  vsigma[2] += e.d[4];
 }
 
 }  // namespace oxf

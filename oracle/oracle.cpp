@@ -873,6 +873,93 @@ int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx) {
   });
 }
 
+// Spin-polarised XC build (Unrestricted/SP.cpp:65-68 -> Grid::getDensity({Da, Db}),
+// Evaluate("ev") with the functional polarised, getEnergy, getFock), LDA/GGA:
+// per sub-grid, rho_s and grad rho_s of each spin (GridDensity.cpp:22-31 at
+// nspin = 2, sigma packed aa, ab, bb), the polarised functional per point, then
+// per spin u (GridFock.cpp:16-30): F = 1/2 F0[w v_rho_u] + F1 with the scaled
+// sigma derivatives (2 on aa / bb, 1 on ab) against grad rho_v, F += F^T.
+// Evaluated through fock_subgrid with vrho := v_rho_u, vsigma := 1 and
+// rho1 := (2 v_sigma_uu grad rho_u + v_sigma_ab grad rho_other) / 2.
+int ox_xc_pol(ox_system* s, const double* Da, const double* Db, double* Va, double* Vb, double* Exc, double* Nel) {
+  return guarded([&] {
+    if (s->family > 1) throw std::runtime_error("oracle: the spin-polarised path is LDA/GGA");
+    if (!s->ao_ready) throw std::runtime_error("oracle: AO tables not ready");
+    const int nb = s->nbasis, n = (int)s->active.size(), nt = s->nthreads;
+    const size_t nb2 = (size_t)nb * nb;
+    std::vector<std::vector<double>> part(nt, std::vector<double>(2 * nb2, 0.0));
+    double E = 0, N = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : E, N)
+    {
+      std::vector<double> F, tmp, tmp3, ra, rb, ga, gb, eps, vr, vs;
+      double* mine = part[omp_get_thread_num()].data();
+#pragma omp for schedule(dynamic, 1)
+      for (int i = 0; i < n; ++i) {
+        SubGrid& sg = s->subgrids[s->active[i]];
+        const int ng = sg.npad;
+        density_subgrid(s, sg, Da, s->family, tmp);
+        ra = sg.rho;
+        ga = s->family >= 1 ? sg.rho1 : std::vector<double>((size_t)3 * ng, 0.0);
+        density_subgrid(s, sg, Db, s->family, tmp);
+        rb = sg.rho;
+        gb = s->family >= 1 ? sg.rho1 : std::vector<double>((size_t)3 * ng, 0.0);
+        eps.assign(ng, 0.0);
+        vr.assign((size_t)2 * ng, 0.0);
+        vs.assign((size_t)3 * ng, 0.0);
+        for (int g = 0; g < ng; ++g) {
+          double saa = 0, sab = 0, sbb = 0;
+          for (int r = 0; r < 3; ++r) {
+            const double a = ga[(size_t)r * ng + g], b = gb[(size_t)r * ng + g];
+            saa += a * a; sab += a * b; sbb += b * b;
+          }
+          for (int code : s->codes) oxf::eval_point_pol(code, ra[g], rb[g], saa, sab, sbb, &eps[g], &vr[2 * g], &vs[3 * g]);
+          E += sg.w[g] * (ra[g] + rb[g]) * eps[g];
+          N += sg.w[g] * (ra[g] + rb[g]);
+        }
+        for (int u = 0; u < 2; ++u) {
+          const std::vector<double>& gu = u == 0 ? ga : gb;
+          const std::vector<double>& go = u == 0 ? gb : ga;
+          sg.vrho.assign(ng, 0.0);
+          sg.vsigma.assign(ng, 1.0);
+          sg.rho1.assign((size_t)3 * ng, 0.0);
+          for (int g = 0; g < ng; ++g) {
+            sg.vrho[g] = vr[2 * g + u];
+            const double suu = vs[3 * g + (u == 0 ? 0 : 2)], sab = vs[3 * g + 1];
+            for (int r = 0; r < 3; ++r)
+              sg.rho1[(size_t)r * ng + g] = 0.5 * (2.0 * suu * gu[(size_t)r * ng + g] + sab * go[(size_t)r * ng + g]);
+          }
+          fock_subgrid(s, sg, mine + u * nb2, F, tmp, tmp3);
+        }
+      }
+    }
+    std::memset(Va, 0, sizeof(double) * nb2);
+    std::memset(Vb, 0, sizeof(double) * nb2);
+    for (int t = 0; t < nt; ++t)
+      for (size_t k = 0; k < nb2; ++k) {
+        Va[k] += part[t][k];
+        Vb[k] += part[t][nb2 + k];
+      }
+    if (Exc) *Exc = E;
+    if (Nel) *Nel = N;
+  });
+}
+
+int ox_eval_functional_pol(int ncodes, const int* codes, long n, const double* rho, const double* sigma, double* exc,
+                           double* vrho, double* vsigma) {
+  for (long i = 0; i < n; ++i) {
+    double e = 0, vr[2] = {0, 0}, vs[3] = {0, 0, 0};
+    const double s0 = sigma ? sigma[3 * i] : 0.0, s1 = sigma ? sigma[3 * i + 1] : 0.0, s2 = sigma ? sigma[3 * i + 2] : 0.0;
+    for (int k = 0; k < ncodes; ++k) {
+      if (oxf::family_of(codes[k]) > 1) { g_err = "oracle: the spin-polarised functionals are LDA/GGA"; return -1; }
+      oxf::eval_point_pol(codes[k], rho[2 * i], rho[2 * i + 1], s0, s1, s2, &e, vr, vs);
+    }
+    if (exc) exc[i] = e;
+    if (vrho) { vrho[2 * i] = vr[0]; vrho[2 * i + 1] = vr[1]; }
+    if (vsigma) for (int k = 0; k < 3; ++k) vsigma[3 * i + k] = vs[k];
+  }
+  return 0;
+}
+
 // XC nuclear gradient in the reference's skeleton form, the grid held fixed
 // (Restricted/Grad.cpp:72-84 -> SubGrid::getDensitySkeleton,
 // GridDensity.cpp:79-130 -> SubGrid::getEnergyGrad, GridEnergy.cpp:15-30):

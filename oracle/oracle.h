@@ -103,6 +103,14 @@ int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx);
 long ox_active_points(const ox_system* s);
 int ox_get_density(const ox_system* s, double* rho, double* grad, double* sigma, double* tau, double* w);
 
+/* Spin-polarised XC build, LDA/GGA: getDensity({Da, Db}) -> Evaluate("ev")
+ * (polarised functional) -> getEnergy -> getFock, Va / Vb per spin. */
+int ox_xc_pol(ox_system* s, const double* Da, const double* Db, double* Va, double* Vb, double* Exc, double* Nel);
+/* Spin-polarised functional at n points, libxc XC_POLARIZED layout: rho[2n]
+ * (a, b), sigma[3n] (aa, ab, bb), vrho[2n], vsigma[3n]; LDA/GGA codes. */
+int ox_eval_functional_pol(int ncodes, const int* codes, long n, const double* rho, const double* sigma, double* exc,
+                           double* vrho, double* vsigma);
+
 /* Learned functional surrogate (mlp.cpp): the network in FP64 on the
  * canonical parameter block (8641 doubles, matrices [out][in]); forward value
  * and input gradient, loss L = cs sum C_i (y_i - T_i)^2 (C may be NULL) with

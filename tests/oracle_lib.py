@@ -56,6 +56,8 @@ def lib():
         "ox_get_density": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_f": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long] + [_vp] * 6),
         "ox_fock_u": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+        "ox_xc_pol": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "ox_eval_functional_pol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
         "ox_mlp_forward": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp]),
         "ox_mlp_loss_grad": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
         "ox_mlp_descriptors": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
@@ -149,6 +151,16 @@ class OracleSystem:
         N = ctypes.c_double()
         _check(lib().ox_xc(self.h, _ptr(Df), _ptr(V), ctypes.addressof(E), ctypes.addressof(N)))
         return E.value, N.value, np.array(V)
+
+    def xc_pol(self, Da, Db):
+        """Spin-polarised build (LDA/GGA): E, N, Va, Vb."""
+        n = self.nbasis
+        A = np.asfortranarray(Da, dtype=np.float64)
+        B = np.asfortranarray(Db, dtype=np.float64)
+        Va, Vb = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+        E, N = ctypes.c_double(), ctypes.c_double()
+        _check(lib().ox_xc_pol(self.h, _ptr(A), _ptr(B), _ptr(Va), _ptr(Vb), ctypes.addressof(E), ctypes.addressof(N)))
+        return E.value, N.value, np.array(Va), np.array(Vb)
 
     def fock_u(self, D, Dxs):
         """Response Fock matrices for the perturbed density matrices Dxs at the
@@ -267,3 +279,14 @@ def mlp_point(P, lo, hi, rho_min, codes, rho, sigma, tau):
     _check(lib().ox_mlp_point(_ptr(P), _ptr(lo), _ptr(hi), rho_min, c.size, _ptr(c), n, _ptr(rho), _ptr(sigma),
                               _ptr(tau), *[_ptr(o) for o in out], _ptr(used)))
     return out[0], out[1], out[2], out[3], used.astype(bool)
+
+
+def eval_functional_pol(codes, rho, sigma):
+    """Spin-polarised eps, vrho (n x 2), vsigma (n x 3); rho (n x 2: a, b), sigma (n x 3: aa, ab, bb)."""
+    cc = np.ascontiguousarray(codes, dtype=np.int32)
+    rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1, 3)
+    n = rho.shape[0]
+    e, vr, vs = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
+    _check(lib().ox_eval_functional_pol(len(cc), _ptr(cc), n, _ptr(rho), _ptr(sigma), _ptr(e), _ptr(vr), _ptr(vs)))
+    return e, vr, vs

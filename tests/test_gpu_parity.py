@@ -154,11 +154,17 @@ def test_protocol_errors(h2o):
     with pytest.raises(ca.ChxError):
         grid.getEnergy()
     grid.getDensity([h2o["D"]])
-    with pytest.raises(ca.ChxError) as e:
-        grid.getDensity([h2o["D"], h2o["D"]])
-    assert e.value.code == -5
+    with pytest.raises(ca.ChxError) as e:  # three spin matrices
+        grid.getDensity([h2o["D"], h2o["D"], h2o["D"]])
+    assert e.value.code == -1
     with pytest.raises(ca.ChxError):
         grid.getFock()
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    xc.Evaluate("ev", grid)
+    with pytest.raises(ca.ChxError) as e:  # the potential holds one spin
+        grid.getFock(nspin=2)
+    assert e.value.code == -1
 
 
 def test_grid_copy_independent(h2o):
