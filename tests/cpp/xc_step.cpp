@@ -3,7 +3,9 @@
 // on include/chinium/xc.hpp. Usage:
 //   xc_step <grid> <basis> <functional> <geometry.txt> <D.bin> <out.bin>
 // geometry.txt: one "Z x y z" line per atom (bohr); D.bin: nbasis^2 doubles
-// (column-major); out.bin receives E_xc, N_e, then V (column-major).
+// (column-major); out.bin receives E_xc, N_e, then V (column-major); for
+// LDA/GGA also the response matrix of D^x = D, then the spin-unrestricted
+// E_xc and V_alpha, V_beta of Da = 0.6 D, Db = 0.4 D.
 #include <cstdio>
 #include <fstream>
 #include <vector>
@@ -50,6 +52,21 @@ int main(int argc, char** argv) {
       grid.getDensityU({{D}});
       chinium::EigenMatrix F = grid.getFockU()[0][0];
       out.write(reinterpret_cast<const char*>(F.data()), sizeof(double) * n * n);
+      // spin-unrestricted call sequence (Unrestricted/SP.cpp:65-68) with Da = 0.6 D, Db = 0.4 D
+      chinium::ExchangeCorrelation uxc;
+      uxc.Spin = 2;
+      uxc.Read(argv[3], false);
+      chinium::EigenMatrix Da = chinium::make_matrix(n, n), Db = chinium::make_matrix(n, n);
+      for (int k = 0; k < n * n; ++k) {
+        Da.data()[k] = 0.6 * D.data()[k];
+        Db.data()[k] = 0.4 * D.data()[k];
+      }
+      grid.getDensity({Da, Db});
+      uxc.Evaluate("ev", grid);
+      const double Eu = grid.getEnergy();
+      const std::vector<chinium::EigenMatrix> Vu = grid.getFock();
+      out.write(reinterpret_cast<const char*>(&Eu), sizeof(double));
+      for (const chinium::EigenMatrix& m : Vu) out.write(reinterpret_cast<const char*>(m.data()), sizeof(double) * n * n);
     }
     std::printf("E_xc %.12f N_e %.10f\n", E, N);
   } catch (const std::exception& e) {

@@ -39,7 +39,10 @@ def test_cpp_caller_matches_oracle(tmp_path):
     n = mol.nbasis
     E, N = out[0], out[1]
     V = out[2:2 + n * n].reshape((n, n), order="F")
-    F = out[2 + n * n:].reshape((n, n), order="F")  # response Fock matrix for D^x = D
+    F = out[2 + n * n:2 + 2 * n * n].reshape((n, n), order="F")  # response Fock matrix for D^x = D
+    Eu = out[2 + 2 * n * n]
+    Va = out[3 + 2 * n * n:3 + 3 * n * n].reshape((n, n), order="F")
+    Vb = out[3 + 3 * n * n:].reshape((n, n), order="F")
     orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], w["grid"], w["basis"])
     orc.set_functional(w["functional"])
     orc.get_ao()
@@ -49,3 +52,6 @@ def test_cpp_caller_matches_oracle(tmp_path):
     assert np.abs(V - Vo).max() <= 1e-9
     (Fo,) = orc.fock_u(D, [D])
     assert np.abs(F - Fo).max() <= 1e-9
+    Euo, _, Vao, Vbo = orc.xc_pol(0.6 * D, 0.4 * D)
+    assert abs(Eu - Euo) <= 1e-10
+    assert np.abs(Va - Vao).max() <= 1e-9 and np.abs(Vb - Vbo).max() <= 1e-9
