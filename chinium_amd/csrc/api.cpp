@@ -178,7 +178,10 @@ void fock_to_device(chx_grid* g, double* dV) {
     g->mark_end(kStMakeB, e);
     launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
     g->mark_begin(kStVxc, &e);
-    launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->hmol, nullptr, nullptr);
+    if (g->mx_wide)
+      launch_vxc_mx_wide(g->stream, nc, (int)g->cwork.size(), g->hmol);
+    else
+      launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->hmol, nullptr, nullptr);
     g->mark_end(kStVxc, e);
     g->mark_begin(kStFinalize, &e);
     launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);

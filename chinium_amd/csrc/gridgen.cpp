@@ -328,14 +328,16 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
 // L2. Fewer, longer items mean fewer FP64 atomics into M.
 void build_cwork(chx_grid* g) {
   const int nb = g->nbatches, nt = g->ntiles;
-  const int nsc = g->npad / (kCacheSuper * kTile);
+  const bool wide = g->precision == CHX_PREC_MIXED && g->mx_wide;
+  const int sup = wide ? 2 * kCacheSuper : kCacheSuper;  // tiles per super-tile
+  const int nsc = g->npad / (sup * kTile);
   std::vector<std::vector<int>> cp((size_t)nsc * nsc);
   std::vector<int> alive_sup;
   for (int b = 0; b < nb; ++b) {
     alive_sup.clear();
     for (int sI = 0; sI < nsc; ++sI) {
       bool any = false;
-      for (int h = 0; h < kCacheSuper; ++h) any |= g->alive[(size_t)b * nt + sI * kCacheSuper + h] != 0;
+      for (int h = 0; h < sup; ++h) any |= g->alive[(size_t)b * nt + sI * sup + h] != 0;
       if (any) alive_sup.push_back(sI);
     }
     for (int I : alive_sup)
@@ -361,15 +363,15 @@ void build_cwork(chx_grid* g) {
   g->cslot.clear();
   if (mixed) {  // per batch of an item: the slots of I's and J's tiles, the all-zero slot if not alive
     const int zero = (int)g->bt_list.size();
-    g->cslot.resize(g->cpair_batches.size() * 8);
+    g->cslot.resize(g->cpair_batches.size() * 2 * sup);
     for (const int4& w : g->cwork)
       for (int k = w.z; k < w.w; ++k) {
         const int b = g->cpair_batches[k];
-        for (int t = 0; t < kCacheSuper; ++t) {
-          const int si = g->slot_of[(size_t)b * nt + w.x * kCacheSuper + t];
-          const int sj = g->slot_of[(size_t)b * nt + w.y * kCacheSuper + t];
-          g->cslot[(size_t)k * 8 + t] = si >= 0 ? si : zero;
-          g->cslot[(size_t)k * 8 + 4 + t] = sj >= 0 ? sj : zero;
+        for (int t = 0; t < sup; ++t) {
+          const int si = g->slot_of[(size_t)b * nt + w.x * sup + t];
+          const int sj = g->slot_of[(size_t)b * nt + w.y * sup + t];
+          g->cslot[(size_t)k * 2 * sup + t] = si >= 0 ? si : zero;
+          g->cslot[(size_t)k * 2 * sup + sup + t] = sj >= 0 ? sj : zero;
         }
       }
   }
@@ -500,6 +502,8 @@ void grid_plan(chx_grid* g) {
     throw Error(CHX_ERR_DEVICE, "the mixed-precision form needs " + std::to_string(cache_bytes / 1e9) +
                                     " GB of device memory for its AO cache");
   g->cached = mixed || g->cache_mode == 1 || (g->cache_mode == -1 && cache_bytes < 0.8 * (double)free_b);
+  // 256-AO Vxc super-tiles where they tile npad exactly and the basis is large
+  g->mx_wide = mixed && g->npad % (2 * kCacheSuper * kTile) == 0 && g->npad >= 1024 ? 1 : 0;
   if (g->cached) {
     build_cwork(g);
     g->d_slot_batch.upload(g->slot_batch, st); g->d_slot_tile.upload(g->slot_tile, st);

@@ -193,7 +193,7 @@ struct MolDev {
   unsigned short* bt;           // B^T per slot
   const int4* cwork;            // Vxc work items
   const int *pair_batches, *slot_of;
-  const int* cslot;             // per pair_batches entry: slots of I's 4 tiles, then J's (zero slot if absent)
+  const int* cslot;             // per pair_batches entry: slots of I's 4 (wide: 8) tiles, then J's (zero slot if absent)
   double* Mint;                 // M accumulator, npad^2
 };
 constexpr int kMxMaxTiles = 512;  // alive tiles per batch the mixed density kernel stages (npad <= 16384)
@@ -224,6 +224,8 @@ void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one, c
 void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols,
                       const int2* items);
 void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols, const int2* items);
+// 256-AO super-tiles (npad a multiple of 256): cslot holds 16 slots per batch
+void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one);
 
 }  // namespace chx
 
@@ -296,6 +298,7 @@ struct chx_grid {
   int cache_mode = -1;               // -1 automatic, 0 never, 1 always
   std::vector<int> slot_batch, slot_tile, slot_of, tile_nao, grp_slots;
   int vxc_chunk = 0;                 // batches per Vxc work item; 0 = the form's default
+  int mx_wide = 0;                   // mixed form: Vxc over 256-AO super-tiles (large bases)
   std::vector<int4> cwork;
   std::vector<int> cpair_batches;
   std::vector<int> cslot;              // mixed form: slot table per cpair_batches entry

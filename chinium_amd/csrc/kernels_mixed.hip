@@ -738,6 +738,119 @@ __global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev marg, const MolDev* __
       }
 }
 
+// Vxc for large bases: the same contraction over 256-AO super-tiles, so that
+// every operand byte staged in LDS feeds twice the MFMAs of the 128 x 128
+// kernel (64 instead of 32 FLOP per byte). A K step is 32 points of one
+// component: Phi^T_I and B^T_J, 256 rows each, hi and lo, 64 fragments of
+// 1 KB brought in by the waves' DMA (same coalesced lane order and swizzle).
+// 8 waves in a 2 x 4 grid, each owning a 128 x 64 block of M (96 MFMAs per
+// step). The slot table holds 16 slots per batch (I's 8 tiles, then J's).
+// Same one-buffer step and the same symmetric tau handling as k_vxc_mx.
+constexpr int kMxSupW = 8;   // tiles per wide super-tile (256 AOs)
+constexpr int kVxAW = 32;    // Phi^T fragments per step: [row block (16)][hl]
+struct VxcMxWideSmem {
+  u32x4 F[2 * kVxAW * kDxFrag];  // one K step's fragments: Phi^T_I, then B^T_J
+  int sl[kMxMaxChunk][16];
+};
+
+template <int NC>
+__global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
+  using lds_ptr = __attribute__((address_space(3))) void*;
+  __shared__ VxcMxWideSmem sm;
+  const int item = xcd_remap(blockIdx.x, gridDim.x);
+  const int4 wk = m.cwork[item];
+  const int I = wk.x, J = wk.y;
+  const u16* __restrict__ cache = m.cache + m.toff;
+  const u16* __restrict__ bt = m.bt;
+  const size_t sstride = (size_t)m.sstride;
+  const int npad = m.npad;
+  double* __restrict__ Mint = m.Mint;
+  {
+    const int* __restrict__ cs = m.cslot + (size_t)wk.z * 16;
+    for (int e = threadIdx.x; e < (wk.w - wk.z) * 16; e += 512) sm.sl[e >> 4][e & 15] = cs[e];
+  }
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int wr = wave & 1, wc = wave >> 1;
+  // The wave's fragments are f = wave + 8 j, j = 0..3, of Phi^T_I and of B^T_J:
+  // hl f & 1 = wave & 1, rows 16 ((wave >> 1) & 1)..+15 of tile (wave >> 2) + 2 j.
+  const int drow = lane >> 2, dch = (lane & 3) ^ ((drow >> 2) & 3);
+  const int fidx = 4 * lrow + (lk ^ ((lrow >> 2) & 3));
+  const int off = (wave & 1) * (kTile * kBatch) + (16 * ((wave >> 1) & 1) + drow) * kBatch + 8 * dch;
+  const int t0 = wave >> 2;
+  __syncthreads();
+  const int nbat = wk.w - wk.z;
+  const int ntau = (NC == 4 && I <= J) ? 6 * nbat : 0;
+  const int nstage = ntau + 2 * nbat;
+  auto dma = [&](int k) {
+    int it, h, c;
+    if (k < ntau) {
+      it = k / 6; h = (k / 3) % 2; c = 1 + k % 3;
+    } else {
+      it = (k - ntau) >> 1; h = (k - ntau) & 1; c = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int t = t0 + 2 * j;
+      const u16* pa = cache + (size_t)sm.sl[it][t] * sstride + c * kMxT + h * 32;
+      const u16* pb = bt + ((size_t)sm.sl[it][8 + t] * NC + c) * kMxT + h * 32;
+      __builtin_amdgcn_global_load_lds(pa + off, (lds_ptr)(sm.F + (wave + 8 * j) * kDxFrag), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(pb + off, (lds_ptr)(sm.F + (kVxAW + wave + 8 * j) * kDxFrag), 16, 0, 0);
+    }
+  };
+  f32x4 acc[8][4];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  dma(0);
+  for (int k = 0; k < nstage; ++k) {
+    if (NC == 4 && k == ntau && I < J) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] *= 2.f;
+    }
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
+    __syncthreads();                     // ... and every other wave's
+    u32x4 a[8][2], bq[4][2];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) a[r][h] = sm.F[((8 * wr + r) * 2 + h) * kDxFrag + fidx];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) bq[r][h] = sm.F[(kVxAW + (4 * wc + r) * 2 + h) * kDxFrag + fidx];
+    __syncthreads();  // every wave holds its fragments: the buffer is free
+    if (k + 1 < nstage) dma(k + 1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[r][c] = mfma3(a[r][0], a[r][1], bq[c][0], bq[c][1], acc[r][c]);
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int mu = I * kMxSupW * kTile + 128 * wr + 16 * r + 4 * lk + i;
+        const int nu = J * kMxSupW * kTile + 64 * wc + 16 * cb + lrow;
+        unsafeAtomicAdd(&Mint[(size_t)mu * npad + nu], (double)acc[r][cb][i]);
+      }
+}
+
+void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one) {
+  if (nitems == 0) return;
+  if (ncomp == 4)
+    hipLaunchKernelGGL((k_vxc_mx_wide<4>), dim3(nitems), dim3(512), 0, s, one);
+  else
+    hipLaunchKernelGGL((k_vxc_mx_wide<1>), dim3(nitems), dim3(512), 0, s, one);
+  CHX_HIP(hipGetLastError());
+}
+
 void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols, const int2* items) {
   if (nitems == 0) return;
   if (ncomp == 4) {

@@ -190,6 +190,10 @@ int chx_pack_create(chx_grid* const* grids, int n, chx_pack** out) {
       prequire(m == 0 || g->type == grids[0]->type, CHX_ERR_ARG, "all grids of a pack need the same type");
       for (int k = 0; k < m; ++k) prequire(grids[k] != g, CHX_ERR_ARG, "a grid appears twice in the pack");
       CHX_HIP(hipStreamSynchronize(g->stream));
+      if (g->mx_wide) {  // the pack's Vxc launch runs 128-AO super-tiles: re-plan the grid's items
+        g->mx_wide = 0;
+        set_vxc_chunk(g, g->vxc_chunk);
+      }
       p->grids.push_back(g);
     }
     p->type = grids[0]->type;

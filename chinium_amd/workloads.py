@@ -175,12 +175,17 @@ WORKLOADS = {
     "h2o16_def2svp_sg2_b3lyp": ("water16", "def2-svp", "SG-2", "B3LYP", 80, 1),
     # cyclo[18]carbon: bonds fixed at 1.23 / 1.35 A, radius solved for (3.7145 A)
     "c18_def2tzvp_sg3_r2scan": ("c18", "def2-tzvp", "SG-3", "R2SCAN", 54, 3),
+    # (H2O)128 in a 15 A cube, O-O >= 2.7 A: the full-size cluster (3072 AOs)
+    "h2o128_def2svp_sg2_b3lyp": ("water128", "def2-svp", "SG-2", "B3LYP", 640, 5),
+    "h2o128_def2svp_sg2_r2scan": ("water128", "def2-svp", "SG-2", "R2SCAN", 640, 5),
 }
 
 DESCRIPTIONS = {
     "h2o_ccpvdz_sg1_pbe": "H2O / cc-pVDZ / SG-1 / PBE",
     "h2o16_def2svp_sg2_b3lyp": "(H2O)16 / def2-SVP / SG-2 / B3LYP",
     "c18_def2tzvp_sg3_r2scan": "cyclo[18]carbon / def2-TZVP / SG-3 / r2SCAN",
+    "h2o128_def2svp_sg2_b3lyp": "(H2O)128 / def2-SVP / SG-2 / B3LYP",
+    "h2o128_def2svp_sg2_r2scan": "(H2O)128 / def2-SVP / SG-2 / r2SCAN",
 }
 
 
@@ -191,6 +196,8 @@ def build_workload(name):
         Z, X = water_monomer_angstrom()
     elif kind == "water16":
         Z, X = water_cluster_angstrom(16, 7.5, seed)
+    elif kind == "water128":
+        Z, X = water_cluster_angstrom(128, 15.0, seed)
     elif kind == "c18":
         Z, X = cyclocarbon_angstrom(18)
     else:
