@@ -111,7 +111,7 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
     launch_fill_u32(g->stream, (unsigned*)g->d_nfall.p, 1, 0u);
     g->mark_begin(2, &e);
     launch_functional_mlp(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, m->Pinf.p,
-                          m->box, g->d_codes.p, (int)xc->codes.size(), want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p,
+                          m->Wpk.p, m->box, g->d_codes.p, (int)xc->codes.size(), want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p,
                           g->d_coef.p, g->d_en_part.p, g->d_nfall.p);
     launch_reduce_en(g->stream, g->nbatches, g->d_en_part.p, g->d_en.p);
     g->mark_end(2, e);
@@ -922,6 +922,7 @@ void mlp_require_stream(chx_mlp* m) {
 void mlp_upload_params(chx_mlp* m, const float* P) {
   CHX_HIP(hipMemcpyAsync(m->P.p, P, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, m->stream));
   launch_mlp_pack(m->stream, m->P.p, m->Prun.p, m->Pinf.p);
+  launch_mlp_pack_mx(m->stream, m->P.p, m->Wpk.p);
 }
 
 // Glorot-uniform weights (tanh units), zero biases, output bias 1 (F of the
@@ -963,6 +964,7 @@ int chx_mlp_create(unsigned long long seed, chx_mlp** out) {
       m->M.alloc(kMlpParams);
       m->V.alloc(kMlpParams);
       m->loss.alloc(1);
+      m->Wpk.alloc(kMlpPacked);
       const std::vector<float> P = mlp_init(seed);
       mlp_upload_params(m, P.data());
       CHX_HIP(hipMemsetAsync(m->G.p, 0, sizeof(float) * kMlpParams, m->stream));
@@ -1097,6 +1099,7 @@ int chx_mlp_train(chx_mlp* m, int epochs, long batch, double lr, double lr_final
       CHX_HIP(hipStreamSynchronize(m->stream));
       if (epoch_loss) epoch_loss[ep] = (double)l / (double)nb;
     }
+    launch_mlp_pack_mx(m->stream, m->P.p, m->Wpk.p);
     CHX_HIP(hipStreamSynchronize(m->stream));
   });
 }
@@ -1140,7 +1143,8 @@ int chx_mlp_forward(const chx_mlp* m, long n, const float* U, int bf16, float* F
     dFv.alloc(n);
     if (dF) dG.alloc(3 * (size_t)n);
     CHX_HIP(hipMemcpyAsync(dU.p, U, sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->stream));
-    launch_mlp_forward(m->stream, n, dU.p, bf16 ? m->Pinf.p : m->Prun.p, dFv.p, dF ? dG.p : nullptr);
+    launch_mlp_forward(m->stream, n, dU.p, bf16 ? m->Pinf.p : m->Prun.p, bf16 == 2 ? m->Wpk.p : nullptr, dFv.p,
+                       dF ? dG.p : nullptr);
     CHX_HIP(hipMemcpyAsync(F, dFv.p, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
     if (dF) CHX_HIP(hipMemcpyAsync(dF, dG.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, m->stream));
     CHX_HIP(hipStreamSynchronize(m->stream));
@@ -1166,7 +1170,7 @@ int chx_mlp_eval_points(const chx_mlp* m, const chx_xc* xc, long n, const double
     de.alloc(n); dvr.alloc(n); dvs.alloc(n); dvt.alloc(n);
     dn.alloc(1);
     launch_fill_u32(s, (unsigned*)dn.p, 1, 0u);
-    launch_mlp_eval(s, n, m->Pinf.p, m->box, dc.p, (int)xc->codes.size(), dr.p, ds.p, dt.p, de.p, dvr.p, dvs.p, dvt.p,
+    launch_mlp_eval(s, n, m->Pinf.p, m->Wpk.p, m->box, dc.p, (int)xc->codes.size(), dr.p, ds.p, dt.p, de.p, dvr.p, dvs.p, dvt.p,
                     dn.p);
     int nf = 0;
     CHX_HIP(hipMemcpyAsync(exc, de.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));

@@ -151,20 +151,24 @@ struct MlpBox {
   double rho_min;  // below: analytic functional (low-density tail)
 };
 void launch_mlp_pack(hipStream_t s, const float* P, float* Prun, float* Pinf);
-void launch_mlp_forward(hipStream_t s, long n, const float* U, const float* R, float* F, float* dF);
+void launch_mlp_forward(hipStream_t s, long n, const float* U, const float* R, const unsigned* Wpk, float* F,
+                        float* dF);
+constexpr int kMlpPacked = 4 * 4 * 2 * 64 * 4;  // u32 words of the packed BF16 MFMA weight fragments
+void launch_mlp_pack_mx(hipStream_t s, const float* P, unsigned* Wpk);
 void launch_mlp_grad(hipStream_t s, long n, const float* U, const float* T, const float* C, float cs, const float* R,
                      float* G, float* loss, int max_blocks);
 void launch_mlp_adam(hipStream_t s, float* P, float* G, float* M, float* V, float* Prun, float* Pinf, float lr,
                      int step);
 void launch_mlp_features(hipStream_t s, long n, const double* rho, const double* sigma, const double* tau,
                          const int* codes, int ncodes, float* t, float* F);
-void launch_mlp_eval(hipStream_t s, long n, const float* R, const MlpBox& box, const int* codes, int ncodes,
+void launch_mlp_eval(hipStream_t s, long n, const float* R, const unsigned* Wpk, const MlpBox& box, const int* codes,
+                     int ncodes,
                      const double* rho, const double* sigma, const double* tau, double* exc, double* vrho,
                      double* vsigma, double* vtau, int* nfall);
 void launch_functional_mlp(hipStream_t s, int nbatches, const int* grp_off, const double* part,
-                           const double* part_tau, const double* bw, const float* R, const MlpBox& box,
-                           const int* codes, int ncodes, int want_e, int want_v, double* rho_out, double* coef,
-                           double* en_part, int* nfall);
+                           const double* part_tau, const double* bw, const float* R, const unsigned* Wpk,
+                           const MlpBox& box, const int* codes, int ncodes, int want_e, int want_v, double* rho_out,
+                           double* coef, double* en_part, int* nfall);
 void launch_point_density(hipStream_t s, int nbatches, const int* grp_off, const double* part,
                           const double* part_tau, int has_tau, const double* bw, double* rho, double* sigma,
                           double* tau, double* w);
@@ -242,6 +246,7 @@ struct chx_mol {
 struct chx_mlp {
   hipStream_t stream = nullptr;
   chx::DevBuf<float> P, Prun, Pinf, G, M, V, loss;  // canonical, run (FP32), run (BF16-rounded); Adam state
+  chx::DevBuf<unsigned> Wpk;                         // packed BF16 MFMA fragments of the inference weights
   chx::DevBuf<float> U, T, C;                        // training set: inputs, targets, sample weights
   std::vector<double> cprefix;                       // prefix sums of C (minibatch normalisation)
   long ndata = 0;

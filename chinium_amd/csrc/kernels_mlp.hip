@@ -9,10 +9,12 @@
 // (src/ExchangeCorrelation.cpp:86-163), whose result this replaces.
 //
 // Network: 3 -> 64 -> 64 -> 64 -> 1, tanh hidden units, linear output. One
-// wave handles 64 samples (lane = sample). Weights are uniform across the wave
-// and are read through the constant address space (scalar loads); the kernels
-// read a run copy of the parameters with W2 transposed, so that every weight
-// row a loop walks is contiguous.
+// wave handles 64 samples (lane = sample). Training runs the network per lane
+// in FP32 with the weights uniform across the wave, read through the constant
+// address space (scalar loads) from a run copy that holds W2 transposed, so
+// that every weight row a loop walks is contiguous. Inference (Evaluate and
+// point evaluation) runs the three hidden products and the two backward ones
+// as transposed split-BF16 MFMA GEMMs over the wave's 64 points.
 #include "internal.hpp"
 #include "device_common.hpp"
 #include "xc_functionals.hpp"
@@ -79,6 +81,162 @@ __device__ __forceinline__ float mlp_value_grad(const float* R, const float u[3]
   return y;
 }
 
+// ---- MFMA inference (one wave = 64 points) -------------------------------
+// Every layer is computed transposed, C[unit][point] = W X, on
+// v_mfma_f32_16x16x32_bf16 (operand layout: A[m = l%16][k = 8(l/16)+e],
+// B[k = 8(l/16)+e][n = l%16], C register r = C[4(l/16)+r][l%16]). Lane
+// l = 16 g + c holds X[mt][nt][r] = activation of unit 16 mt + 4 g + r at
+// point 16 nt + c: the C registers of a layer. The K slots of every product
+// are permuted so that these registers are also the next product's B operand:
+// K step ks, slot 8 g + e carries unit U(ks, 8g+e) = 16 (2 ks + e/4) + 4 g + e%4,
+// i.e. B element e of (nt, ks) is X[2 ks + e/4][nt][e%4]. The packed weights
+// (k_mlp_pack_mx) hold A[m][slot] in that order, BF16 (the inference weights
+// are BF16 already). Activations enter as split BF16 (hi + lo), so each tile is
+// two MFMAs per K step. Forward: W2, W3; backward for dF/du: W3^T, W2^T.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float fast_tanh(float x) {  // 1 - 2 / (e^(2x) + 1)
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * x) + 1.f);
+}
+__device__ __forceinline__ unsigned short bf_bits(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
+__device__ __forceinline__ float bf_val(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
+
+// B operand of K step ks for point tile nt, split: hi and lo halves.
+__device__ __forceinline__ void pack_b(const float (&X)[4][4][4], int nt, int ks, u32x4_t& hi, u32x4_t& lo) {
+  unsigned h[4], l[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float x0 = X[2 * ks + (2 * q) / 4][nt][(2 * q) % 4], x1 = X[2 * ks + (2 * q + 1) / 4][nt][(2 * q + 1) % 4];
+    const unsigned short h0 = bf_bits(x0), h1 = bf_bits(x1);
+    const unsigned short l0 = bf_bits(x0 - bf_val(h0)), l1 = bf_bits(x1 - bf_val(h1));
+    h[q] = (unsigned)h0 | ((unsigned)h1 << 16);
+    l[q] = (unsigned)l0 | ((unsigned)l1 << 16);
+  }
+  hi = u32x4_t{h[0], h[1], h[2], h[3]};
+  lo = u32x4_t{l[0], l[1], l[2], l[3]};
+}
+
+// acc = W_mat X (64 x 64 times 64 x 64 points), packed weights Wpk.
+__device__ __forceinline__ void mx_layer(const u32x4_t* __restrict__ Wpk, int mat, const float (&X)[4][4][4],
+                                         f32x4_t (&acc)[4][4], int lane) {
+  u32x4_t bh[4][2], bl[4][2];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) pack_b(X, nt, ks, bh[nt][ks], bl[nt][ks]);
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const bf16x8_t a0 = __builtin_bit_cast(bf16x8_t, Wpk[((mat * 4 + mt) * 2 + 0) * 64 + lane]);
+    const bf16x8_t a1 = __builtin_bit_cast(bf16x8_t, Wpk[((mat * 4 + mt) * 2 + 1) * 64 + lane]);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      f32x4_t c = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, __builtin_bit_cast(bf16x8_t, bl[nt][0]), c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, __builtin_bit_cast(bf16x8_t, bl[nt][1]), c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, __builtin_bit_cast(bf16x8_t, bh[nt][0]), c, 0, 0, 0);
+      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, __builtin_bit_cast(bf16x8_t, bh[nt][1]), c, 0, 0, 0);
+    }
+  }
+}
+
+// Network value and input gradient of the wave's 64 points (lane = point);
+// every lane of the wave must call it. R: inference parameters (BF16-rounded,
+// for W1, biases, w4); Wpk: packed BF16 W2, W3, W3^T, W2^T fragments.
+__device__ __forceinline__ float mlp_value_grad_mx(const u32x4_t* __restrict__ Wpk, const float* __restrict__ R,
+                                                   const float u[3], float du[3]) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  float un[4][3];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) un[nt][k] = __shfl(u[k], 16 * nt + c);
+  float X1[4][4][4], X2[4][4][4];
+  f32x4_t acc[4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int unit = 16 * mt + 4 * g + r;
+      const float w0 = R[kMlpW1 + 3 * unit], w1 = R[kMlpW1 + 3 * unit + 1], w2 = R[kMlpW1 + 3 * unit + 2];
+      const float b = R[kMlpB1 + unit];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) X1[mt][nt][r] = fast_tanh(b + w0 * un[nt][0] + w1 * un[nt][1] + w2 * un[nt][2]);
+    }
+  mx_layer(Wpk, 0, X1, acc, lane);  // W2
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float b = R[kMlpB2 + 16 * mt + 4 * g + r];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) X2[mt][nt][r] = fast_tanh(acc[mt][nt][r] + b);
+    }
+  mx_layer(Wpk, 1, X2, acc, lane);  // W3
+  float yp[4] = {0.f, 0.f, 0.f, 0.f};
+  float D[4][4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int unit = 16 * mt + 4 * g + r;
+      const float b = R[kMlpB3 + unit], w = R[kMlpW4 + unit];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const float h = fast_tanh(acc[mt][nt][r] + b);
+        yp[nt] += w * h;
+        D[mt][nt][r] = w * (1.f - h * h);  // d3
+      }
+    }
+  mx_layer(Wpk, 2, D, acc, lane);  // W3^T d3
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) D[mt][nt][r] = acc[mt][nt][r] * (1.f - X2[mt][nt][r] * X2[mt][nt][r]);  // d2
+  mx_layer(Wpk, 3, D, acc, lane);  // W2^T d2
+  float dp[4][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int unit = 16 * mt + 4 * g + r;
+      const float w0 = R[kMlpW1 + 3 * unit], w1 = R[kMlpW1 + 3 * unit + 1], w2 = R[kMlpW1 + 3 * unit + 2];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const float d1 = acc[mt][nt][r] * (1.f - X1[mt][nt][r] * X1[mt][nt][r]);
+        dp[nt][0] += w0 * d1;
+        dp[nt][1] += w1 * d1;
+        dp[nt][2] += w2 * d1;
+      }
+    }
+  // sums over the four lane groups (units 4 g + r); lane 16 g + c then reads point tile nt = g
+  float y = 0.f;
+  du[0] = du[1] = du[2] = 0.f;
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    float v = yp[nt];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    float d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      d[k] = dp[nt][k];
+      d[k] += __shfl_xor(d[k], 16);
+      d[k] += __shfl_xor(d[k], 32);
+    }
+    if (nt == g) {
+      y = v;
+      du[0] = d[0];
+      du[1] = d[1];
+      du[2] = d[2];
+    }
+  }
+  return y + R[kMlpB4];
+}
+
 // Descriptors t = (ln rho, s/(1+s), alpha/(1+alpha)) and their derivatives
 // dt[k] = (dt_k/drho, dt_k/dsigma, dt_k/dtau). s and alpha are formed as the
 // analytic meta-GGA sees them: tau floored at 1e-20, sigma floored at 1e-40
@@ -127,9 +285,10 @@ __device__ __forceinline__ void mlp_descriptors(double rho, double sigma, double
 // Surrogate at one point: eps (per particle) and v_rho, v_sigma, v_tau, or the
 // analytic functional where the point is below rho_min or outside the box
 // (returns false then). Every lane runs the network.
-__device__ __forceinline__ bool mlp_point(const float* R, const MlpBox& box, const int* codes, int ncodes,
-                                          double rho, double sigma, double tau, double* eps, double* vr, double* vs,
-                                          double* vt) {
+template <bool MX>
+__device__ __forceinline__ bool mlp_point(const float* R, const u32x4_t* Wpk, const MlpBox& box, const int* codes,
+                                          int ncodes, double rho, double sigma, double tau, double* eps, double* vr,
+                                          double* vs, double* vt) {
   const double r = rho > 1e-300 ? rho : 1e-300;
   double t[3], dt[3][3];
   mlp_descriptors(r, sigma, tau, t, dt);
@@ -144,7 +303,7 @@ __device__ __forceinline__ bool mlp_point(const float* R, const MlpBox& box, con
     u[k] = (float)((t[k] - 0.5 * (lo + hi)) * hinv[k]);
   }
   float du[3];
-  const float F = mlp_value_grad(R, u, du);
+  const float F = MX ? mlp_value_grad_mx(Wpk, R, u, du) : mlp_value_grad(R, u, du);
   if (in) {
     double g[3];  // dF/dt
 #pragma unroll
@@ -188,20 +347,42 @@ __global__ void k_mlp_pack(const float* __restrict__ P, float* __restrict__ Prun
   Pinf[run_index(k)] = bf16_round(p);
 }
 
-// Raw network on normalised inputs U (n x 3): F (n) and dF/dU (n x 3).
+// Raw network on normalised inputs U (n x 3): F (n) and dF/dU (n x 3); the
+// per-lane FP32 network, or with Wpk the MFMA network of the inference path.
 __global__ __launch_bounds__(64) void k_mlp_forward(long n, const float* __restrict__ U, const float* R,
-                                                    float* __restrict__ F, float* __restrict__ dF) {
-  const long i = (long)blockIdx.x * kLanes + threadIdx.x;
-  if (i >= n) return;
+                                                    const u32x4_t* __restrict__ Wpk, float* __restrict__ F,
+                                                    float* __restrict__ dF) {
+  const long i0 = (long)blockIdx.x * kLanes + threadIdx.x;
+  const long i = i0 < n ? i0 : n - 1;  // the MFMA network needs every lane of the wave
   const float u[3] = {U[3 * i], U[3 * i + 1], U[3 * i + 2]};
   float du[3];
-  const float y = mlp_value_grad(R, u, du);
+  const float y = Wpk ? mlp_value_grad_mx(Wpk, R, u, du) : mlp_value_grad(R, u, du);
+  if (i0 >= n) return;
   F[i] = y;
   if (dF) {
     dF[3 * i] = du[0];
     dF[3 * i + 1] = du[1];
     dF[3 * i + 2] = du[2];
   }
+}
+
+// Packed BF16 A-operand fragments of the MFMA network: for matrix mat (0 W2,
+// 1 W3, 2 W3^T, 3 W2^T), m tile mt, K step ks, lane l = 16 g + c, element e:
+// A[m = 16 mt + c][slot 8 g + e] with the slot's unit U = 16 (2 ks + e/4) + 4 g + e%4.
+__global__ void k_mlp_pack_mx(const float* __restrict__ P, unsigned short* __restrict__ Wpk) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (((mat * 4 + mt) * 2 + ks) * 64 + lane) * 8 + e
+  if (idx >= 4 * 4 * 2 * 64 * 8) return;
+  const int e = idx & 7, lane = (idx >> 3) & 63, ks = (idx >> 9) & 1, mt = (idx >> 10) & 3, mat = idx >> 12;
+  const int g = lane >> 4, c = lane & 15;
+  const int m = 16 * mt + c, un = 16 * (2 * ks + e / 4) + 4 * g + e % 4;
+  float w;
+  switch (mat) {
+    case 0: w = P[kMlpW2 + kMlpH * m + un]; break;
+    case 1: w = P[kMlpW3 + kMlpH * m + un]; break;
+    case 2: w = P[kMlpW3 + kMlpH * un + m]; break;
+    default: w = P[kMlpW2 + kMlpH * un + m]; break;
+  }
+  Wpk[idx] = bf_bits(w);
 }
 
 // Loss L = cs sum_i C_i (y_i - T_i)^2 over the samples [0, n) (C_i = 1 when C
@@ -398,7 +579,8 @@ __global__ void k_mlp_features(long n, const double* __restrict__ rho, const dou
 }
 
 // Surrogate at n arbitrary points; *nfall counts the analytic fallbacks.
-__global__ __launch_bounds__(64) void k_mlp_eval(long n, const float* R, MlpBox box, const int* __restrict__ codes,
+__global__ __launch_bounds__(64) void k_mlp_eval(long n, const float* R, const u32x4_t* __restrict__ Wpk, MlpBox box,
+                                                 const int* __restrict__ codes,
                                                  int ncodes, const double* __restrict__ rho,
                                                  const double* __restrict__ sigma, const double* __restrict__ tau,
                                                  double* __restrict__ exc, double* __restrict__ vrho,
@@ -408,7 +590,7 @@ __global__ __launch_bounds__(64) void k_mlp_eval(long n, const float* R, MlpBox 
   const bool live = i0 < n;
   const long i = live ? i0 : n - 1;  // tail lanes run a copy of the last point and store nothing
   double e, vr, vs, vt;
-  const bool net = mlp_point(R, box, codes, ncodes, rho[i], sigma[i], tau[i], &e, &vr, &vs, &vt);
+  const bool net = mlp_point<true>(R, Wpk, box, codes, ncodes, rho[i], sigma[i], tau[i], &e, &vr, &vs, &vt);
   if (live) {
     exc[i] = e;
     vrho[i] = vr;
@@ -425,7 +607,8 @@ __global__ __launch_bounds__(64) void k_mlp_eval(long n, const float* R, MlpBox 
 __global__ __launch_bounds__(64) void k_functional_mlp(int nslots, const int* __restrict__ grp_off,
                                                        const double* __restrict__ part,
                                                        const double* __restrict__ part_tau,
-                                                       const double* __restrict__ bw, const float* R, MlpBox box,
+                                                       const double* __restrict__ bw, const float* R,
+                                                       const u32x4_t* __restrict__ Wpk, MlpBox box,
                                                        const int* __restrict__ codes, int ncodes, int want_e,
                                                        int want_v, double* __restrict__ rho_out,
                                                        double* __restrict__ coef, double* __restrict__ en_part,
@@ -449,7 +632,7 @@ __global__ __launch_bounds__(64) void k_functional_mlp(int nslots, const int* __
   const double w = bw[slot];
   const double sigma = gx * gx + gy * gy + gz * gz;
   double eps, vr, vs, vt;
-  const bool net = mlp_point(R, box, codes, ncodes, rho, sigma, tau, &eps, &vr, &vs, &vt);
+  const bool net = mlp_point<true>(R, Wpk, box, codes, ncodes, rho, sigma, tau, &eps, &vr, &vs, &vt);
   double e_acc = 0, n_acc = 0;
   if (live) {
     rho_out[slot] = rho;
@@ -507,9 +690,17 @@ void launch_mlp_pack(hipStream_t st, const float* P, float* Prun, float* Pinf) {
   CHX_HIP(hipGetLastError());
 }
 
-void launch_mlp_forward(hipStream_t st, long n, const float* U, const float* R, float* F, float* dF) {
+void launch_mlp_forward(hipStream_t st, long n, const float* U, const float* R, const unsigned* Wpk, float* F,
+                        float* dF) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + kLanes - 1) / kLanes)), dim3(kLanes), 0, st, n, U, R, F, dF);
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + kLanes - 1) / kLanes)), dim3(kLanes), 0, st, n, U, R,
+                     reinterpret_cast<const u32x4_t*>(Wpk), F, dF);
+  CHX_HIP(hipGetLastError());
+}
+
+void launch_mlp_pack_mx(hipStream_t st, const float* P, unsigned* Wpk) {
+  hipLaunchKernelGGL(k_mlp_pack_mx, dim3(4 * 4 * 2 * 64 * 8 / 256), dim3(256), 0, st, P,
+                     reinterpret_cast<unsigned short*>(Wpk));
   CHX_HIP(hipGetLastError());
 }
 
@@ -539,24 +730,25 @@ void launch_mlp_features(hipStream_t st, long n, const double* rho, const double
   CHX_HIP(hipGetLastError());
 }
 
-void launch_mlp_eval(hipStream_t st, long n, const float* R, const MlpBox& box, const int* codes, int ncodes,
-                     const double* rho, const double* sigma, const double* tau, double* exc, double* vrho,
+void launch_mlp_eval(hipStream_t st, long n, const float* R, const unsigned* Wpk, const MlpBox& box, const int* codes,
+                     int ncodes, const double* rho, const double* sigma, const double* tau, double* exc, double* vrho,
                      double* vsigma, double* vtau, int* nfall) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)((n + kLanes - 1) / kLanes)), dim3(kLanes), 0, st, n, R, box, codes,
-                     ncodes, rho, sigma, tau, exc, vrho, vsigma, vtau, nfall);
+  hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)((n + kLanes - 1) / kLanes)), dim3(kLanes), 0, st, n, R,
+                     reinterpret_cast<const u32x4_t*>(Wpk), box, codes, ncodes, rho, sigma, tau, exc, vrho, vsigma,
+                     vtau, nfall);
   CHX_HIP(hipGetLastError());
 }
 
 void launch_functional_mlp(hipStream_t st, int nbatches, const int* grp_off, const double* part,
-                           const double* part_tau, const double* bw, const float* R, const MlpBox& box,
-                           const int* codes, int ncodes, int want_e, int want_v, double* rho_out, double* coef,
-                           double* en_part, int* nfall) {
+                           const double* part_tau, const double* bw, const float* R, const unsigned* Wpk,
+                           const MlpBox& box, const int* codes, int ncodes, int want_e, int want_v, double* rho_out,
+                           double* coef, double* en_part, int* nfall) {
   const int nslots = nbatches * kBatch;
   if (nslots == 0) return;
   hipLaunchKernelGGL(k_functional_mlp, dim3((unsigned)((nslots + kLanes - 1) / kLanes)), dim3(kLanes), 0, st, nslots,
-                     grp_off, part, part_tau, bw, R, box, codes, ncodes, want_e, want_v, rho_out, coef, en_part,
-                     nfall);
+                     grp_off, part, part_tau, bw, R, reinterpret_cast<const u32x4_t*>(Wpk), box, codes, ncodes, want_e,
+                     want_v, rho_out, coef, en_part, nfall);
   CHX_HIP(hipGetLastError());
 }
 

@@ -121,12 +121,14 @@ class Surrogate:
         return L.value, G
 
     # inference
-    def forward(self, U, bf16=True):
-        """Network value and dF/dU at normalised inputs (GPU)."""
+    def forward(self, U, bf16=True, mfma=False):
+        """Network value and dF/dU at normalised inputs (GPU): FP32 training weights, the BF16 inference
+        weights (bf16), or the inference path itself (mfma: split-BF16 activations on MFMA)."""
         U = _f32(U).reshape(-1, 3)
         n = U.shape[0]
         F, dF = np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
-        _check(lib().chx_mlp_forward(self.h, n, _ptr(U), int(bool(bf16)), _ptr(F), _ptr(dF)))
+        mode = 2 if mfma else int(bool(bf16))
+        _check(lib().chx_mlp_forward(self.h, n, _ptr(U), mode, _ptr(F), _ptr(dF)))
         return F, dF
 
     def eval_points(self, xc, rho, sigma, tau):

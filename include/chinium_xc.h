@@ -234,8 +234,8 @@ void chx_pack_destroy(chx_pack* p);
  * rho < rho_min take the analytic functional (the out-of-distribution
  * fallback). Parameters: CHX_MLP_PARAMS floats, matrices [out][in]:
  * W1[64][3] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4. Training
- * runs in FP32 with Adam; inference uses BF16-rounded weights, FP32
- * arithmetic. */
+ * runs in FP32 with Adam; inference uses BF16-rounded weights with split-BF16
+ * activations on MFMA and FP32 accumulation. */
 #define CHX_MLP_PARAMS 8641
 typedef struct chx_mlp chx_mlp;
 int chx_mlp_create(unsigned long long seed, chx_mlp** out); /* Glorot-uniform init, seeded */
@@ -260,7 +260,9 @@ int chx_mlp_train(chx_mlp* m, int epochs, long batch, double lr, double lr_final
  * L = cs sum_i C_i (y_i - T_i)^2, cs = 1 / sum C_i; G gets dL/dP. */
 int chx_mlp_loss_grad(chx_mlp* m, long n, const float* U, const float* T, const float* C, float* G, double* loss);
 /* Raw network at n normalised inputs: F and dF/dU (n x 3, may be NULL).
- * bf16 = 1 uses the inference weights. */
+ * bf16 = 0: FP32 training weights, per-lane FP32 network; 1: the BF16
+ * inference weights on that network; 2: the inference path itself (BF16
+ * weights, split-BF16 activations on MFMA). */
 int chx_mlp_forward(const chx_mlp* m, long n, const float* U, int bf16, float* F, float* dF);
 /* The surrogate at n points (host arrays), with the analytic fallback of xc's
  * codes; *nfallback gets the number of points that took it. */
