@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="meta-GGA workloads: skip the learned-functional leg (training + a build with it)")
     ap.add_argument("--surrogate-epochs", type=int, default=200)
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 leg (workloads whose FP64 AO tables do not fit in HBM); "
+                         "speedup_vs_plain_fp64 is then null")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -144,14 +147,16 @@ def main():
                 "traffic_source": "profiles/h2o16_def2svp_sg2_b3lyp_hbm_traffic.md" if names[dom] in traffic else None,
                 "avg_ms": round(dom_ms, 4), "flops_per_launch": flops_kernel}
 
-    plain = measure(ca.CHX_PREC_FP64)
+    plain = None if args.no_plain else measure(ca.CHX_PREC_FP64)
+    if plain is not None:
+        del plain["grid"]  # the FP64 form's tables are not needed any more
     chosen = measure(ca.CHX_PREC_MIXED)
     st = chosen["stats"]
     flops_build = st["flops_per_build"]
     ms_step = chosen["ms"]
     nc = 4 if xc.family >= 2 else 1
-    names64 = {"density": "k_density_c64<0>" if plain["cached"] else "k_density_f64",
-               "vxc": ("k_vxc_c64<%d>" % nc) if plain["cached"] else "k_vxc_f64"}
+    names64 = {"density": "k_density_c64<0>" if plain and plain["cached"] else "k_density_f64",
+               "vxc": ("k_vxc_c64<%d>" % nc) if plain and plain["cached"] else "k_vxc_f64"}
     namesmx = {"density": "k_density_mx<0,false>", "vxc": "k_vxc_mx<%d,false>" % nc}
 
     out = {
@@ -172,9 +177,9 @@ def main():
                    "mean_alive_tiles_per_batch": round(st["mean_tiles"], 2),
                    "form": "mixed: split-BF16 MFMA contractions, FP32 accumulation, FP64 functional/E/N/V sums",
                    "ao_tables": "cached in HBM (bf16 hi/lo pairs)", "screening_threshold": 1e-14},
-        "speedup_vs_plain_fp64": round(plain["ms"] / ms_step, 3),
+        "speedup_vs_plain_fp64": round(plain["ms"] / ms_step, 3) if plain else None,
         "autotune": {"vxc_chunk": chosen["tuned"][0], "ms": {str(k): round(v, 4) for k, v in chosen["tuned"][1].items()},
-                     "plain_fp64_vxc_chunk": plain["tuned"][0] if plain["tuned"] else None},
+                     "plain_fp64_vxc_chunk": plain["tuned"][0] if plain and plain["tuned"] else None},
         "tflops": round(flops_build / (ms_step * 1e-3) / 1e12, 3),
         "gpoints_per_s": round(st["points"] / (ms_step * 1e-3) / 1e9, 4),
         "flops_per_build": flops_build,
@@ -184,16 +189,16 @@ def main():
         "E_xc": chosen["E"],
         "N_e": chosen["N"],
         "roofline": roofline(chosen, BF16_SPLIT_PEAK_TFLOPS, namesmx),
-        "plain_fp64": {
-            "ms_per_step": round(plain["ms"], 4),
-            "tflops": round(flops_build / (plain["ms"] * 1e-3) / 1e12, 3),
-            "kernels_ms": {k: round(v[0] / max(1, v[1]), 4) for k, v in plain["kt"].items()},
-            "roofline": roofline(plain, FP64_MFMA_PEAK_TFLOPS, names64),
-            "ao_tables": "cached in HBM (FP64)" if plain["cached"] else "recomputed in kernels",
-            "E_xc": plain["E"],
-            "dE_vs_chosen": abs(plain["E"] - chosen["E"]),
-            "max_dV_vs_chosen": float(np.abs(plain["V"] - chosen["V"]).max()),
-        },
+    }
+    out["plain_fp64"] = None if plain is None else {
+        "ms_per_step": round(plain["ms"], 4),
+        "tflops": round(flops_build / (plain["ms"] * 1e-3) / 1e12, 3),
+        "kernels_ms": {k: round(v[0] / max(1, v[1]), 4) for k, v in plain["kt"].items()},
+        "roofline": roofline(plain, FP64_MFMA_PEAK_TFLOPS, names64),
+        "ao_tables": "cached in HBM (FP64)" if plain["cached"] else "recomputed in kernels",
+        "E_xc": plain["E"],
+        "dE_vs_chosen": abs(plain["E"] - chosen["E"]),
+        "max_dV_vs_chosen": float(np.abs(plain["V"] - chosen["V"]).max()),
     }
     E, V, N = chosen["E"], chosen["V"], chosen["N"]
 
@@ -210,6 +215,7 @@ def main():
         loss = net.fit(xc, rho_s, sig_s, tau_s, w_s, epochs=args.surrogate_epochs)
         t_train = time.perf_counter() - t0
         xc.setSurrogate(net)
+        chosen.pop("grid", None)  # one grid's AO tables in HBM at a time
         with_s = measure(ca.CHX_PREC_MIXED)
         with_s["fallback"] = with_s["grid"].fallbackCount()
         xc.setSurrogate(None)
@@ -224,7 +230,7 @@ def main():
             "functional_ms_analytic": round(chosen["kt"]["functional"][0] / max(1, chosen["kt"]["functional"][1]), 4),
             "fallback_points": with_s["fallback"],
             "E_xc": with_s["E"],
-            "speedup_vs_plain_fp64": round(plain["ms"] / with_s["ms"], 3),
+            "speedup_vs_plain_fp64": round(plain["ms"] / with_s["ms"], 3) if plain else None,
         }
 
     if not args.no_cpu_baseline:
@@ -243,8 +249,9 @@ def main():
                                          "does once per SCF)"}
         out["accuracy_vs_fp64_oracle"] = {"dE_xc": abs(E - Eo), "dE_xc_per_atom": abs(E - Eo) / natoms,
                                           "max_dV": float(np.abs(V - Vo).max()), "dN_e": abs(N - No)}
-        out["plain_fp64"]["accuracy_vs_fp64_oracle"] = {"dE_xc": abs(plain["E"] - Eo),
-                                                        "max_dV": float(np.abs(plain["V"] - Vo).max())}
+        if plain is not None:
+            out["plain_fp64"]["accuracy_vs_fp64_oracle"] = {"dE_xc": abs(plain["E"] - Eo),
+                                                            "max_dV": float(np.abs(plain["V"] - Vo).max())}
         if surr is not None:
             rs = surr["r"]
             out["surrogate"]["accuracy_vs_fp64_oracle"] = {"dE_xc": abs(rs["E"] - Eo),
