@@ -576,8 +576,8 @@ __device__ __forceinline__ P5 energy_density_pol(int code, const P5& ra, const P
 __device__ __forceinline__ void eval_codes_pol(const int* codes, int ncodes, double ra, double rb, double saa,
                                                double sab, double sbb, double* eps, double* vr, double* vs) {
   double e = 0, d[5] = {0, 0, 0, 0, 0};
-  if (ra < 0) ra = 0;
-  if (rb < 0) rb = 0;
+  ra = ra > 0 ? ra : 0.0;  // negative or NaN channel: empty
+  rb = rb > 0 ? rb : 0.0;
   const double rho = ra + rb;
   if (rho >= kDensThreshold) {
     const double sfl = 1e-40;

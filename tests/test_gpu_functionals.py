@@ -87,3 +87,42 @@ def test_r2scan_parity(case):
     print("  E gpu %.14f oracle %.14f dE %.2e dV %.2e" % (E, Eo, abs(E - Eo), np.abs(V - Vo).max()), flush=True)
     assert abs(E - Eo) <= 1e-10
     assert np.abs(V - Vo).max() <= 1e-9
+
+
+def test_device_nonphysical_inputs():
+    """The device functionals treat negative / NaN inputs as the oracle does (bit for bit on the
+    values that are exactly zero or floored), unpolarised and polarised."""
+    nan = float("nan")
+    rho = np.array([-1e-3, nan, 0.0, 1e-20, 0.3, 0.3])
+    sig = np.array([0.1, 0.1, 0.1, 0.1, nan, -0.5])
+    tau = np.array([0.1, 0.1, 0.1, 0.1, nan, -0.5])
+    for codes in ([1, 7], [101, 130], [402], [497, 498]):
+        xc = ca.ExchangeCorrelation().from_codes(codes)
+        d = xc.eval_points(rho, sig, tau)
+        o = ol.eval_functional(codes, rho, sig, tau)
+        for a, b in zip(d, o):
+            assert np.all(np.isfinite(a)), codes
+            assert np.array_equal(a[:4], b[:4])
+            # floored sigma / tau: rounding-level agreement (derivatives that vanish there agree to 1e-17)
+            assert np.all(np.abs(a[4:] - b[4:]) <= 1e-11 * np.abs(b[4:]) + 1e-14)
+    xc = ca.ExchangeCorrelation()
+    xc.Spin = 2
+    xc.from_codes([101, 130])
+    R = np.array([[nan, 0.1], [-1.0, 0.2], [0.0, 0.0]])
+    S = np.array([[0.1, 0.0, 0.1]] * 3)
+    e, vr, vs = xc.eval_points(R, S)
+    eo, vro, vso = ol.eval_functional_pol([101, 130], R, S)
+    assert np.all(np.isfinite(e)) and np.all(np.isfinite(vr)) and np.all(np.isfinite(vs))
+    assert e[2] == 0 and not np.any(vr[2])
+    assert np.all(np.abs(e - eo) <= 1e-11 * np.abs(eo))  # an empty channel is a fully polarised point
+
+
+def test_density_matrix_of_wrong_size():
+    w = wl.build_workload("h2o_ccpvdz_sg1_pbe")
+    mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
+    g = ca.Grid(mol, "SG-1", 1, 0)
+    g.setType(1)
+    g.getAO(0, 0)
+    with pytest.raises(ca.ChxError) as e:
+        g.getDensity([np.zeros((mol.nbasis + 1, mol.nbasis + 1))])
+    assert e.value.code == -1

@@ -110,3 +110,23 @@ def test_r2scan_exact_constraints():
     ex0, _, _, _ = ol.eval_functional([497], rho, zero, np.full(4, 1e-30) + 0.0 * rho)
     # tau floored at 1e-20 keeps alpha at ~0 for these densities
     assert np.allclose(ex0 / el, 1.174, rtol=1e-6)
+
+
+def test_nonphysical_inputs_contribute_nothing():
+    """Negative or NaN densities contribute nothing; a NaN or negative sigma / tau is floored
+    (sigma at the threshold floor, tau at 1e-20) rather than propagated."""
+    nan = float("nan")
+    rho = np.array([-1e-3, nan, 0.0, 1e-20, 0.3, 0.3])
+    sig = np.array([0.1, 0.1, 0.1, 0.1, nan, -0.5])
+    tau = np.array([0.1, 0.1, 0.1, 0.1, nan, -0.5])
+    for codes in ([1, 7], [101, 130], [402], [497, 498]):
+        e, vr, vs, vt = ol.eval_functional(codes, rho, sig, tau)
+        for a in (e, vr, vs, vt):
+            assert np.all(np.isfinite(a)), codes
+        assert not np.any(e[:4]) and not np.any(vr[:4])
+        e0, vr0, _, _ = ol.eval_functional(codes, rho[4:5], np.zeros(1), np.full(1, 1e-20))
+        assert e[4] == e0[0] and e[5] == e0[0]  # floored sigma / tau = the zero-gradient values
+    e, vr, vs = ol.eval_functional_pol([101, 130], np.array([[nan, 0.1], [-1.0, 0.2], [0.0, 0.0]]),
+                                       np.array([[0.1, 0.0, 0.1]] * 3))
+    assert np.all(np.isfinite(e)) and np.all(np.isfinite(vr)) and np.all(np.isfinite(vs))
+    assert e[2] == 0 and not np.any(vr[2])
