@@ -594,7 +594,7 @@ __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev marg, c
     MakeB<NC> op;
     op.sv = reinterpret_cast<u16*>(st) + g;  // [c][hl][nu][g]
     op.ca = ca; op.cx = cx; op.cy = cy; op.cz = cz; op.ct = ct;
-    walk_tile_f32(bv, t, bx[p], by[p], bz[p], op);
+    walk_tile_f32<true>(bv, t, bx[p], by[p], bz[p], op);
   }
   __syncthreads();
   if (slot < nslots) {
