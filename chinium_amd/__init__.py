@@ -98,6 +98,7 @@ def lib():
         "chx_pack_create": (i, [_vp, i, P(_vp)]),
         "chx_pack_xc": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "chx_pack_set_graph": (i, [_vp, i]),
+        "chx_pack_set_lanes": (i, [_vp, i]),
         "chx_pack_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
         "chx_pack_stream": (i, [_vp, P(_vp)]),
         "chx_pack_destroy": (None, [_vp]),
@@ -459,8 +460,8 @@ class ExchangeCorrelation:
 
 
 class Pack:
-    """Many molecules per launch (chx_pack_*): XC builds of all the grids with
-    one kernel launch per stage, replayed as a hipGraph. Grids must be in the
+    """Many molecules per XC build (chx_pack_*): every grid's kernels on
+    staggered lanes (streams), replayed as one hipGraph. Grids must be in the
     mixed-precision form with getAO done; they stay referenced by the pack."""
 
     def __init__(self, grids):
@@ -485,6 +486,10 @@ class Pack:
 
     def set_graph(self, on):
         _check(lib().chx_pack_set_graph(self.h, 1 if on else 0))
+
+    def set_lanes(self, lanes):
+        """Splits the molecules into `lanes` groups on streams of their own, staggered by a stage."""
+        _check(lib().chx_pack_set_lanes(self.h, int(lanes)))
 
     def autotune(self, xc, D_ptrs, V_ptrs, reps=3):
         """Times each Vxc work-item length, keeps the fastest; returns (chunk, {chunk: ms})."""

@@ -138,7 +138,7 @@ void density_from_device(chx_grid* g, const double* dD) {
     g->mark_end(kStPermute, e);
     g->mark_begin(kStDensity, &e);
     for (int r = 0; r <= (g->type >= 2 ? 3 : 0); ++r)  // r > 0: tau partials, A = d_r phi
-      launch_density_mx(g->stream, r, ng, g->hmol, nullptr, nullptr);
+      launch_density_mx(g->stream, r, ng, g->hmol);
     g->mark_end(kStDensity, e);
     g->have_density = true;
     g->have_eval = false;
@@ -174,14 +174,14 @@ void fock_to_device(chx_grid* g, double* dV) {
     require(nc <= g->mx_nT, CHX_ERR_STATE, "a meta-GGA potential needs a grid of type 2 before getAO");
     g->mark_begin(kStMakeB, &e);
     const int nslot = (int)g->bt_list.size();
-    launch_make_b_mx(g->stream, nc, nc == 4 ? nslot : (nslot + 3) / 4, g->hmol, nullptr, nullptr);
+    launch_make_b_mx(g->stream, nc, nc == 4 ? nslot : (nslot + 3) / 4, g->hmol);
     g->mark_end(kStMakeB, e);
     launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
     g->mark_begin(kStVxc, &e);
     if (g->mx_wide)
       launch_vxc_mx_wide(g->stream, nc, (int)g->cwork.size(), g->hmol);
     else
-      launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->hmol, nullptr, nullptr);
+      launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->hmol);
     g->mark_end(kStVxc, e);
     g->mark_begin(kStFinalize, &e);
     launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);
@@ -927,8 +927,8 @@ void mlp_upload_params(chx_mlp* m, const float* P) {
 
 // Glorot-uniform weights (tanh units), zero biases, output bias 1 (F of the
 // uniform gas exchange); splitmix64 for the seeded stream.
-std::vector<float> mlp_init(unsigned long long seed) {
-  std::vector<float> P(kMlpParams, 0.f);
+void mlp_init(unsigned long long seed, std::vector<float>& P) {
+  P.assign(kMlpParams, 0.f);
   unsigned long long x = seed;
   auto next = [&] {
     x += 0x9E3779B97F4A7C15ull;
@@ -947,7 +947,6 @@ std::vector<float> mlp_init(unsigned long long seed) {
   fill(kMlpW3, kMlpH, kMlpH);
   fill(kMlpW4, 1, kMlpH);
   P[kMlpB4] = 1.f;
-  return P;
 }
 }  // namespace
 
@@ -965,7 +964,8 @@ int chx_mlp_create(unsigned long long seed, chx_mlp** out) {
       m->V.alloc(kMlpParams);
       m->loss.alloc(1);
       m->Wpk.alloc(kMlpPacked);
-      const std::vector<float> P = mlp_init(seed);
+      std::vector<float> P;
+      mlp_init(seed, P);
       mlp_upload_params(m, P.data());
       CHX_HIP(hipMemsetAsync(m->G.p, 0, sizeof(float) * kMlpParams, m->stream));
       CHX_HIP(hipMemsetAsync(m->M.p, 0, sizeof(float) * kMlpParams, m->stream));

@@ -391,7 +391,6 @@ void set_vxc_chunk(chx_grid* g, int chunk) {
     g->hmol.cwork = g->d_cwork.p;
     g->hmol.pair_batches = g->d_cpair_batches.p;
     g->hmol.cslot = g->d_cslot.p;
-    g->d_mol.upload(std::vector<MolDev>{g->hmol}, st);
   }
   CHX_HIP(hipStreamSynchronize(st));
 }
@@ -580,7 +579,6 @@ void grid_plan(chx_grid* g) {
     m.cwork = g->d_cwork.p; m.pair_batches = g->d_cpair_batches.p; m.slot_of = g->d_slot_of.p;
     m.cslot = g->d_cslot.p;
     m.Mint = g->d_Mint.p;
-    g->d_mol.upload(std::vector<MolDev>{m}, st);
   }
   CHX_HIP(hipStreamSynchronize(st));
   g->ao_ready = true;

@@ -221,13 +221,11 @@ void launch_ao_cache_mx(hipStream_t s, int nslots, const int* slot_batch, const 
 void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds);
 void launch_bf16_check(hipStream_t s, unsigned n, unsigned* bad);
 void launch_fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v);
-// Launches over items = (grid index into mols, group / first slot / work item);
-// items == nullptr means the single grid `one` (passed by value) and item = block.
-void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one, const MolDev* mols,
-                       const int2* items);
-void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols,
-                      const int2* items);
-void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols, const int2* items);
+// One launch over the grid `one` (descriptor by value): density work groups,
+// B^T slot blocks, Vxc work items.
+void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one);
+void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one);
+void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one);
 // 256-AO super-tiles (npad a multiple of 256): cslot holds 16 slots per batch
 void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one);
 
@@ -319,8 +317,7 @@ struct chx_grid {
   int max_nprim = 0;
   // Mixed form: split AO cache (AOg + nT transposed components per slot), B^T, split D.
   chx::DevBuf<unsigned short> d_cmx, d_bmx, d_Dsplit;
-  chx::MolDev hmol{};                 // descriptor of this grid (mixed form)
-  chx::DevBuf<chx::MolDev> d_mol;
+  chx::MolDev hmol{};                 // descriptor of this grid (mixed form), passed to kernels by value
   int mx_sstride = 0, mx_nT = 0, bmx_comps = 0;
   chx::DevBuf<int> d_sh_type, d_sh_nprim, d_sh_poff, d_tile_seg_off, d_sh_atom;
   chx::DevBuf<int4> d_seg;

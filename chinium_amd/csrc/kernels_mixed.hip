@@ -249,19 +249,6 @@ __device__ __forceinline__ void radial_f32(const SegRec& rec, float r2, float& A
 // The basis tables are read through the constant address space: they never
 // change while a kernel runs, and so are always read with scalar loads (a
 // kernel that also issues LDS DMA loads would otherwise get vector loads).
-// Read-only tables (work lists, per-grid descriptors of a pack) go through the
-// constant address space as well: scalar loads, uniform values.
-template <class T>
-__device__ __forceinline__ T ldc(const T* p) {
-  static_assert(sizeof(T) % 4 == 0, "4-byte words");
-  using cu = const __attribute__((address_space(4))) unsigned;
-  unsigned w[sizeof(T) / 4];
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = ((cu*)p)[i];
-  T r;
-  __builtin_memcpy(&r, w, sizeof(T));
-  return r;
-}
 
 __device__ __forceinline__ SegRec load_segrec(const SegRec* recs, int k) {
   using cvec = const __attribute__((address_space(4))) u32x4;
@@ -389,20 +376,16 @@ struct DensMxSmem {
 
 // MODE 0: A = phi; epilogue rho and grad rho / 2 (4 components into part).
 // MODE r = 1..3: A = d phi / d x_r; epilogue sum X_r d_r phi (tau partial of direction r).
-// PACK = false: one grid, its descriptor passed by value (kernel argument, so
-// every pointer in it is known to be global and is read with scalar loads);
-// PACK = true: descriptors of many grids in device memory, items select them.
-template <int MODE, bool PACK>
-__global__ __launch_bounds__(256, 4) void k_density_mx(MolDev marg, const MolDev* __restrict__ mols,
-                                                       const int2* __restrict__ items) {
+// The grid's descriptor comes by value (a kernel argument: every pointer in it
+// is known to address global memory, and uniform table reads become scalar
+// loads; a descriptor read from device memory would leave them flat).
+template <int MODE>
+__global__ __launch_bounds__(256, 4) void k_density_mx(MolDev m) {
   using lds_ptr = __attribute__((address_space(3))) void*;
   constexpr int NOUT = MODE == 0 ? 4 : 1;
   constexpr int grp_stride = (MODE == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
   __shared__ DensMxSmem sm;
-  const int item = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 wi = PACK ? ldc(items + item) : make_int2(0, item);
-  const MolDev m = PACK ? ldc(mols + wi.x) : marg;
-  const int grp = wi.y;
+  const int grp = xcd_remap(blockIdx.x, gridDim.x);
   const u16* __restrict__ cache = m.cache + (size_t)MODE * kMxT;
   const u16* __restrict__ Ds = m.Ds;
   const size_t sstride = (size_t)m.sstride;
@@ -514,14 +497,9 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev marg, const MolDev
   }
 }
 
-void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one, const MolDev* mols,
-                       const int2* items) {
+void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one) {
   if (nitems == 0) return;
-#define CHX_DMX(M)                                                                                  \
-  if (items)                                                                                        \
-    hipLaunchKernelGGL((k_density_mx<M, true>), dim3(nitems), dim3(256), 0, s, one, mols, items);  \
-  else                                                                                              \
-    hipLaunchKernelGGL((k_density_mx<M, false>), dim3(nitems), dim3(256), 0, s, one, mols, items);
+#define CHX_DMX(M) hipLaunchKernelGGL((k_density_mx<M>), dim3(nitems), dim3(256), 0, s, one);
   switch (mode) {
     case 0: CHX_DMX(0); break;
     case 1: CHX_DMX(1); break;
@@ -567,15 +545,12 @@ struct MakeB {
   __device__ void end() {}
 };
 
-template <int NC, bool PACK>
-__global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev marg, const MolDev* __restrict__ mols,
-                                                                  const int2* __restrict__ items) {
+template <int NC>
+__global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev m) {
   constexpr int W = NC == 4 ? 1 : 4;  // waves per block
   __shared__ u32x4 stage[W][NC * kMxT / 8];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
-  const int2 wi = PACK ? ldc(items + blockIdx.x) : make_int2(0, blockIdx.x * W);  // first slot of the block
-  const MolDev m = PACK ? ldc(mols + wi.x) : marg;
-  const int slot = wi.y + wave, nslots = m.nslots;
+  const int slot = blockIdx.x * W + wave, nslots = m.nslots;
   const double* __restrict__ coef = m.coef;
   const BasisView bv = m.bv;  // descriptor fields in registers, read once
   const int* __restrict__ slot_batch = m.slot_batch;
@@ -604,20 +579,12 @@ __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev marg, c
   }
 }
 
-void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols,
-                      const int2* items) {
+void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one) {
   if (nitems == 0) return;
-  if (ncomp == 4) {
-    if (items)
-      hipLaunchKernelGGL((k_make_b_mx<4, true>), dim3(nitems), dim3(64), 0, s, one, mols, items);
-    else
-      hipLaunchKernelGGL((k_make_b_mx<4, false>), dim3(nitems), dim3(64), 0, s, one, mols, items);
-  } else {
-    if (items)
-      hipLaunchKernelGGL((k_make_b_mx<1, true>), dim3(nitems), dim3(256), 0, s, one, mols, items);
-    else
-      hipLaunchKernelGGL((k_make_b_mx<1, false>), dim3(nitems), dim3(256), 0, s, one, mols, items);
-  }
+  if (ncomp == 4)
+    hipLaunchKernelGGL((k_make_b_mx<4>), dim3(nitems), dim3(64), 0, s, one);
+  else
+    hipLaunchKernelGGL((k_make_b_mx<1>), dim3(nitems), dim3(256), 0, s, one);
   CHX_HIP(hipGetLastError());
 }
 
@@ -640,15 +607,12 @@ struct VxcMxSmem {
   int sl[kMxMaxChunk][8];       // per batch of the item: slots of I's and J's tiles
 };
 
-template <int NC, bool PACK>
-__global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev marg, const MolDev* __restrict__ mols,
-                                                   const int2* __restrict__ items) {
+template <int NC>
+__global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev m) {
   using lds_ptr = __attribute__((address_space(3))) void*;
   __shared__ VxcMxSmem sm;
   const int item = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 wi = PACK ? ldc(items + item) : make_int2(0, item);
-  const MolDev m = PACK ? ldc(mols + wi.x) : marg;
-  const int4 wk = m.cwork[wi.y];
+  const int4 wk = m.cwork[item];
   const int I = wk.x, J = wk.y;
   const u16* __restrict__ cache = m.cache + m.toff;
   const u16* __restrict__ bt = m.bt;
@@ -851,19 +815,12 @@ void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one)
   CHX_HIP(hipGetLastError());
 }
 
-void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one, const MolDev* mols, const int2* items) {
+void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one) {
   if (nitems == 0) return;
-  if (ncomp == 4) {
-    if (items)
-      hipLaunchKernelGGL((k_vxc_mx<4, true>), dim3(nitems), dim3(256), 0, s, one, mols, items);
-    else
-      hipLaunchKernelGGL((k_vxc_mx<4, false>), dim3(nitems), dim3(256), 0, s, one, mols, items);
-  } else {
-    if (items)
-      hipLaunchKernelGGL((k_vxc_mx<1, true>), dim3(nitems), dim3(256), 0, s, one, mols, items);
-    else
-      hipLaunchKernelGGL((k_vxc_mx<1, false>), dim3(nitems), dim3(256), 0, s, one, mols, items);
-  }
+  if (ncomp == 4)
+    hipLaunchKernelGGL((k_vxc_mx<4>), dim3(nitems), dim3(256), 0, s, one);
+  else
+    hipLaunchKernelGGL((k_vxc_mx<1>), dim3(nitems), dim3(256), 0, s, one);
   CHX_HIP(hipGetLastError());
 }
 

@@ -1,16 +1,18 @@
-// Many molecules in one launch per stage (mixed-precision form).
+// Many molecules per XC build (mixed-precision form).
 //
 // A pack holds grids that were each built and planned on their own (points,
-// Becke weights, screening, split operand cache). Per XC build it runs, on one
-// stream: split D per molecule -> one density launch over every molecule's
-// work groups -> functional per molecule -> one B^T launch over every slot ->
-// one Vxc launch over every work item -> V = M + M^T per molecule. The big
-// kernels take a list of (molecule, item) pairs and a device array of
-// per-molecule descriptors, so small molecules fill the GPU together with
-// large ones. The whole sequence is captured once as a hipGraph and replayed
-// each SCF iteration (the small per-molecule kernels then cost no launch
-// overhead); a capture is redone when the D/V pointers or the functional
-// change. chx_pack_autotune picks the Vxc work-item length by timing.
+// Becke weights, screening, split operand cache, Vxc work items). One XC build
+// of all of them runs the single-grid kernels of every molecule (descriptor
+// passed by value, so each kernel addresses its tables as global memory) on a
+// few lanes: the molecules are cut into contiguous groups of about equal work,
+// each group on a stream of its own, and a group's density launches wait for
+// the previous group's. One group's functional, B^T and Vxc stages then run
+// beside the next group's density, which keeps the GPU filled between the
+// small per-molecule launches. The whole build is captured once as a hipGraph
+// (streams forked from and joined back to the pack's stream) and replayed each
+// SCF iteration; a capture is redone when the D/V pointers, the functional,
+// the lanes or the Vxc work length change. chx_pack_autotune picks the Vxc
+// work-item length by timing.
 //
 // No reference counterpart: Chinium runs one molecule per process
 // (src/HartreeFockKohnSham/SelfConsistentField.cpp:27-33).
@@ -31,8 +33,6 @@ struct chx_pack {
   std::vector<chx_grid*> grids;  // not owned
   hipStream_t stream = nullptr;
   int type = 0;
-  DevBuf<MolDev> d_mols;
-  DevBuf<int2> d_dens, d_mb1, d_mb4, d_vxc;
   DevBuf<int> d_codes;
   DevBuf<double> d_en;  // E, N per molecule
   std::vector<int> codes;
@@ -43,9 +43,14 @@ struct chx_pack {
   std::vector<const double*> gD;  // pointers the graph was captured with
   std::vector<double*> gV;
   std::vector<double> h_en;
+  int lanes = 16;
+  std::vector<int> cut;  // lane k holds molecules [cut[k], cut[k + 1])
+  std::vector<hipStream_t> lane_st;
+  std::vector<hipEvent_t> ev_dens, ev_done;
+  hipEvent_t ev_fork = nullptr;
   ~chx_pack() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
+    free_lanes();
     if (stream) (void)hipStreamDestroy(stream);
   }
   void drop_graph() {
@@ -53,6 +58,16 @@ struct chx_pack {
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
+  }
+  void free_lanes() {
+    for (hipStream_t t : lane_st) (void)hipStreamDestroy(t);
+    for (hipEvent_t e : ev_dens) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_done) (void)hipEventDestroy(e);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    lane_st.clear();
+    ev_dens.clear();
+    ev_done.clear();
+    ev_fork = nullptr;
   }
 };
 
@@ -74,60 +89,86 @@ void prequire(bool ok, int code, const std::string& msg) {
   if (!ok) throw Error(code, msg);
 }
 
-// Work lists over all molecules: molecules in order, each one's items in its
-// own (XCD-friendly) order.
-void build_items(chx_pack* p) {
-  std::vector<MolDev> mols;
-  std::vector<int2> dens, mb1, mb4, vxc;
-  for (size_t m = 0; m < p->grids.size(); ++m) {
-    const chx_grid* g = p->grids[m];
-    mols.push_back(g->hmol);
-    const int mi = (int)m;
-    for (int k = 0; k < (int)g->grp_batch.size(); ++k) dens.push_back(make_int2(mi, k));
-    const int ns = (int)g->bt_list.size();
-    for (int k = 0; k < ns; k += 4) mb1.push_back(make_int2(mi, k));
-    for (int k = 0; k < ns; ++k) mb4.push_back(make_int2(mi, k));
-    for (int k = 0; k < (int)g->cwork.size(); ++k) vxc.push_back(make_int2(mi, k));
+// Lane cut points (equal shares of density groups + Vxc work items, at least
+// one molecule per lane) and the lanes' streams and events.
+void build_lanes(chx_pack* p) {
+  const int n = (int)p->grids.size();
+  std::vector<double> cum(n + 1, 0.0);
+  for (int m = 0; m < n; ++m)
+    cum[m + 1] = cum[m] + (double)p->grids[m]->grp_batch.size() + (double)p->grids[m]->cwork.size();
+  const int L = std::max(1, std::min(p->lanes, n));
+  p->cut.assign(L + 1, n);
+  int c = 0;
+  for (int k = 0; k < L; ++k) {
+    p->cut[k] = c;
+    if (k == L - 1) break;
+    const double goal = cum[n] * (k + 1) / L;
+    const int m0 = c;
+    while (c < n - (L - 1 - k) && (c == m0 || cum[c + 1] <= goal)) ++c;
   }
-  p->d_mols.upload(mols, p->stream);
-  p->d_dens.upload(dens, p->stream);
-  p->d_mb1.upload(mb1, p->stream);
-  p->d_mb4.upload(mb4, p->stream);
-  p->d_vxc.upload(vxc, p->stream);
-  CHX_HIP(hipStreamSynchronize(p->stream));
   p->drop_graph();
+  p->free_lanes();
+  if (L > 1) {
+    p->lane_st.resize(L);
+    p->ev_dens.resize(L);
+    p->ev_done.resize(L);
+    for (int k = 0; k < L; ++k) {
+      CHX_HIP(hipStreamCreate(&p->lane_st[k]));
+      CHX_HIP(hipEventCreateWithFlags(&p->ev_dens[k], hipEventDisableTiming));
+      CHX_HIP(hipEventCreateWithFlags(&p->ev_done[k], hipEventDisableTiming));
+    }
+    CHX_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+  }
 }
 
-// One XC build of every molecule, enqueued on the pack's stream.
-void enqueue(chx_pack* p, const double* const* D, double* const* V) {
-  hipStream_t s = p->stream;
-  const int n = (int)p->grids.size();
-  for (int m = 0; m < n; ++m) {
+// Molecules [m0, m1) on stream s: split D and clear M first; the density
+// launches wait for `after` (if set) and signal `dens_done` (if set); then per
+// molecule the functional, E/N sums, B^T, Vxc and V = M + M^T. The kernels and
+// their arguments are those of the single-grid build.
+void enqueue_lane(chx_pack* p, int m0, int m1, hipStream_t s, hipEvent_t after, hipEvent_t dens_done,
+                  const double* const* D, double* const* V) {
+  for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
     launch_split_d(s, g->nbasis, g->npad, D[m], g->d_int2ref.p, g->d_Dsplit.p);
+    launch_fill(s, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
   }
-  const int nd = (int)p->d_dens.n;
-  for (int r = 0; r <= (p->type >= 2 ? 3 : 0); ++r) launch_density_mx(s, r, nd, p->grids[0]->hmol, p->d_mols.p, p->d_dens.p);
-  for (int m = 0; m < n; ++m) {
+  if (after) CHX_HIP(hipStreamWaitEvent(s, after, 0));
+  for (int m = m0; m < m1; ++m) {
+    chx_grid* g = p->grids[m];
+    for (int r = 0; r <= (p->type >= 2 ? 3 : 0); ++r) launch_density_mx(s, r, (int)g->grp_batch.size(), g->hmol);
+  }
+  if (dens_done) CHX_HIP(hipEventRecord(dens_done, s));
+  const int nc = p->family >= 2 ? 4 : 1;
+  for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
     launch_functional(s, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, p->d_codes.p,
                       (int)p->codes.size(), p->family, 1, 1, g->d_rho.p, g->d_coef.p, g->d_en_part.p, g->en_blocks);
     launch_reduce_en(s, g->en_blocks, g->d_en_part.p, p->d_en.p + 2 * m);
-  }
-  const int nc = p->family >= 2 ? 4 : 1;
-  if (nc == 4)
-    launch_make_b_mx(s, 4, (int)p->d_mb4.n, p->grids[0]->hmol, p->d_mols.p, p->d_mb4.p);
-  else
-    launch_make_b_mx(s, 1, (int)p->d_mb1.n, p->grids[0]->hmol, p->d_mols.p, p->d_mb1.p);
-  for (int m = 0; m < n; ++m) {
-    chx_grid* g = p->grids[m];
-    launch_fill(s, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
-  }
-  launch_vxc_mx(s, nc, (int)p->d_vxc.n, p->grids[0]->hmol, p->d_mols.p, p->d_vxc.p);
-  for (int m = 0; m < n; ++m) {
-    chx_grid* g = p->grids[m];
+    const int nslot = (int)g->bt_list.size();
+    launch_make_b_mx(s, nc, nc == 4 ? nslot : (nslot + 3) / 4, g->hmol);
+    if (g->mx_wide)
+      launch_vxc_mx_wide(s, nc, (int)g->cwork.size(), g->hmol);
+    else
+      launch_vxc_mx(s, nc, (int)g->cwork.size(), g->hmol);
     launch_finalize_sym(s, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, V[m]);
   }
+}
+
+// One XC build of every molecule: one lane runs on the pack's stream; more
+// lanes fork from it, are staggered by their density launches and join back.
+void enqueue(chx_pack* p, const double* const* D, double* const* V) {
+  const int L = (int)p->cut.size() - 1;
+  if (L == 1) {
+    enqueue_lane(p, 0, (int)p->grids.size(), p->stream, nullptr, nullptr, D, V);
+    return;
+  }
+  CHX_HIP(hipEventRecord(p->ev_fork, p->stream));
+  for (int k = 0; k < L; ++k) {
+    CHX_HIP(hipStreamWaitEvent(p->lane_st[k], p->ev_fork, 0));
+    enqueue_lane(p, p->cut[k], p->cut[k + 1], p->lane_st[k], k ? p->ev_dens[k - 1] : nullptr, p->ev_dens[k], D, V);
+    CHX_HIP(hipEventRecord(p->ev_done[k], p->lane_st[k]));
+  }
+  for (int k = 0; k < L; ++k) CHX_HIP(hipStreamWaitEvent(p->stream, p->ev_done[k], 0));
 }
 
 void run(chx_pack* p, const chx_xc* xc, const double* const* D, double* const* V, double* E, double* N) {
@@ -190,17 +231,13 @@ int chx_pack_create(chx_grid* const* grids, int n, chx_pack** out) {
       prequire(m == 0 || g->type == grids[0]->type, CHX_ERR_ARG, "all grids of a pack need the same type");
       for (int k = 0; k < m; ++k) prequire(grids[k] != g, CHX_ERR_ARG, "a grid appears twice in the pack");
       CHX_HIP(hipStreamSynchronize(g->stream));
-      if (g->mx_wide) {  // the pack's Vxc launch runs 128-AO super-tiles: re-plan the grid's items
-        g->mx_wide = 0;
-        set_vxc_chunk(g, g->vxc_chunk);
-      }
       p->grids.push_back(g);
     }
     p->type = grids[0]->type;
     CHX_HIP(hipSetDevice(grids[0]->device));
     CHX_HIP(hipStreamCreate(&p->stream));
     p->d_en.alloc(2 * (size_t)n);
-    build_items(p);
+    build_lanes(p);
   });
   if (rc != CHX_OK) {
     delete p;
@@ -237,7 +274,7 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
     float best_ms = 0.f;
     for (int k = 0; k < 4; ++k) {
       for (chx_grid* g : p->grids) set_vxc_chunk(g, cand[k]);
-      build_items(p);
+      build_lanes(p);
       run(p, xc, D_dev, V_dev, nullptr, nullptr);  // capture and warm up
       CHX_HIP(hipEventRecord(a, p->stream));
       for (int r = 0; r < reps; ++r) run(p, xc, D_dev, V_dev, nullptr, nullptr);
@@ -255,8 +292,18 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     for (chx_grid* g : p->grids) set_vxc_chunk(g, cand[best]);
-    build_items(p);
+    build_lanes(p);
     if (chunk_out) *chunk_out = cand[best];
+  });
+}
+
+int chx_pack_set_lanes(chx_pack* p, int lanes) {
+  return pguard([&] {
+    prequire(p != nullptr && lanes >= 1 && lanes <= 16, CHX_ERR_ARG, "lanes must be in 1..16");
+    CHX_HIP(hipStreamSynchronize(p->stream));
+    for (hipStream_t t : p->lane_st) CHX_HIP(hipStreamSynchronize(t));
+    p->lanes = lanes;
+    build_lanes(p);
   });
 }
 

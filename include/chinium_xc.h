@@ -201,13 +201,14 @@ int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double
 int chx_xc_eval_points_f(const chx_xc* xc, long n, const double* rho, const double* sigma, double* v2rho2,
                          double* v2rhosigma, double* v2sigma2);
 
-/* ---- Packs: many molecules per launch (no reference counterpart) ------------
+/* ---- Packs: many molecules per XC build (no reference counterpart) -----------
  * Chinium runs one molecule per process (SelfConsistentField.cpp:27-33). A
  * pack takes grids that are each built, set to CHX_PREC_MIXED, typed and
- * planned (chx_grid_get_ao) on their own, and runs one XC build of all of them
- * with one kernel launch per stage; the sequence is captured as a hipGraph and
- * replayed while the D / V pointers and the functional stay the same. The
- * grids must outlive the pack and must not be used on their own meanwhile.
+ * planned (chx_grid_get_ao) on their own, and runs one XC build of all of them:
+ * every molecule's single-grid kernels, on staggered lanes (see
+ * chx_pack_set_lanes); the sequence is captured as a hipGraph and replayed
+ * while the D / V pointers and the functional stay the same. The grids must
+ * outlive the pack and must not be used on their own while a pack call runs.
  * D_dev[m] / V_dev[m]: device arrays, column-major nbasis_m^2, reference
  * order. E, N (host, may be NULL): E_xc and N_e per molecule. */
 typedef struct chx_pack chx_pack;
@@ -220,6 +221,13 @@ int chx_pack_set_graph(chx_pack* p, int enable);
  * 128 batches), keeps the fastest; chunk_out gets it, ms_out[4] the times. */
 int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev, double* const* V_dev, int reps,
                       int* chunk_out, float* ms_out);
+/* Lanes (1..16, default 16, at most one per molecule): the molecules are
+ * split into that many contiguous groups of about equal work, each on a stream
+ * of its own; a group's density launches wait for the previous group's, so the
+ * later stages of one group run beside the density of the next. Results do not
+ * depend on it. chx_pack_stream stays the stream the whole build is ordered
+ * on. */
+int chx_pack_set_lanes(chx_pack* p, int lanes);
 int chx_pack_stream(const chx_pack* p, void** stream);
 void chx_pack_destroy(chx_pack* p);
 

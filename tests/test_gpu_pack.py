@@ -1,6 +1,6 @@
-"""Packs (chx_pack_*): many molecules' XC builds in one launch per stage,
-replayed as a hipGraph, against the same grids built one at a time and
-against the FP64 oracle."""
+"""Packs (chx_pack_*): many molecules' XC builds on staggered lanes, replayed
+as one hipGraph, against the same grids built one at a time and against the
+FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -104,6 +104,62 @@ def test_pack_autotune(alkanes):
     assert np.all(np.abs(E - E0) <= 1e-12 * np.abs(E0))
     for a, b in zip(Vs, V0):
         assert np.abs(a - b).max() <= 2e-6
+
+
+def test_pack_lanes(alkanes):
+    """Molecules split over lanes (streams forked and joined inside the graph)
+    give the same results for any lane count, replayed and eager; the default
+    (more lanes than molecules) clamps to one molecule per lane."""
+    pack = alkanes["pack"]
+    E0, N0, V0 = _run(alkanes)
+    try:
+        for lanes, graph in ((1, True), (1, False), (2, True), (3, True), (3, False), (8, True)):
+            pack.set_lanes(lanes)
+            pack.set_graph(graph)
+            for rep in range(2):
+                E, N, Vs = _run(alkanes)
+                dV = max(np.abs(a - b).max() for a, b in zip(Vs, V0))
+                _say("  lanes %d graph %d rep %d: max dE %.1e max dV %.1e"
+                     % (lanes, graph, rep, np.abs(E - E0).max(), dV))
+                assert np.all(np.abs(E - E0) <= 1e-12 * np.abs(E0))
+                assert np.all(np.abs(N - N0) <= 1e-12 * N0)
+                assert dV <= 1e-12
+        with pytest.raises(ca.ChxError) as e:
+            pack.set_lanes(0)
+        assert e.value.code == -1
+    finally:
+        pack.set_graph(True)
+        pack.set_lanes(16)
+
+
+def test_pack_with_wide_member(alkanes):
+    """A pack member large enough for the 256-AO Vxc super-tiles ((H2O)40 /
+    def2-SVP, 960 AOs, npad 1024) keeps its own plan inside the pack: the pack
+    gives its single-grid result, beside a small alkane."""
+    Z, X = wl.water_cluster_angstrom(40, 10.5, 21)
+    mol = ca.Molecule(Z, X * ca.ANGSTROM_TO_BOHR, "def2-svp")
+    D, S = wl.workload_density(mol, 200, 5)
+    grid = ca.Grid(mol, "SG-1", 1, 0)
+    grid.setPrecision(ca.CHX_PREC_MIXED)
+    grid.setType(1)
+    grid.getAO(0, 0)
+    xc = alkanes["xc"]
+    E1, V1 = ca.xc_build(grid, xc, D)
+    small = alkanes["mols"][1]
+    Es1, Vs1 = ca.xc_build(small["grid"], xc, small["D"])  # at the Vxc item length the autotuner left
+    nb = mol.nbasis
+    dD, dV = hip.DeviceArray(nb * nb), hip.DeviceArray(nb * nb)
+    dD.upload(np.asfortranarray(D).ravel(order="F"))
+    pack = ca.Pack([grid, small["grid"]])
+    E, N = pack.xc(xc, [dD.ptr, alkanes["dD"][1].ptr], [dV.ptr, alkanes["dV"][1].ptr])
+    V = dV.download().reshape((nb, nb), order="F")
+    Vs = alkanes["dV"][1].download().reshape((small["mol"].nbasis,) * 2, order="F")
+    _say("  (H2O)40 in a pack: dE %.1e dV %.1e; C%d dE %.1e dV %.1e"
+         % (abs(E[0] - E1), np.abs(V - V1).max(), small["n"], abs(E[1] - Es1), np.abs(Vs - Vs1).max()))
+    assert abs(E[0] - E1) <= 1e-12 * abs(E1)
+    assert np.abs(V - V1).max() <= 1e-12
+    assert abs(E[1] - Es1) <= 1e-12 * abs(Es1)
+    assert np.abs(Vs - Vs1).max() <= 1e-12
 
 
 def test_pack_rejects_fp64_grid(alkanes):
