@@ -157,10 +157,15 @@ void density_from_device(chx_grid* g, const double* dD) {
         launch_density_c64(g->stream, r, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
                            g->d_bt_off.p, g->d_bt_list.p, g->d_cache.p, g->d_Dint.p, g->npad,
                            g->d_part_tau.p + (size_t)(r - 1) * kBatch);
-  } else
-    launch_density_f64(g->stream, (int)g->grp_batch.size(), g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p,
-                       g->d_bt_off.p, g->d_bt_list.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_Dint.p, g->npad,
-                       g->d_part.p);
+  } else {  // AO values recomputed in the kernels
+    launch_density_f64(g->stream, 0, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
+                       g->d_bt_list.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_Dint.p, g->npad, g->d_part.p);
+    if (g->type >= 2)
+      for (int r = 1; r <= 3; ++r)
+        launch_density_f64(g->stream, r, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
+                           g->d_bt_list.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_Dint.p, g->npad,
+                           g->d_part_tau.p + (size_t)(r - 1) * kBatch);
+  }
   g->mark_end(kStDensity, e);
   g->have_density = true;
   g->have_eval = false;
@@ -209,8 +214,8 @@ void fock_to_device(chx_grid* g, double* dV) {
   }
   launch_fill(g->stream, g->d_Vint.p, (size_t)g->npad * g->npad, 0.0);
   g->mark_begin(kStVxc, &e);
-  launch_vxc_f64(g->stream, (int)g->vwork.size(), g->d_vwork.p, g->d_pair_batches.p, g->d_alive.p, g->ntiles, g->d_x.p,
-                 g->d_y.p, g->d_z.p, g->d_coef.p, g->view(), g->d_Vint.p, g->npad);
+  launch_vxc_f64(g->stream, g->v_family >= 2 ? 4 : 1, (int)g->vwork.size(), g->d_vwork.p, g->d_pair_batches.p,
+                 g->d_alive.p, g->ntiles, g->d_x.p, g->d_y.p, g->d_z.p, g->d_coef.p, g->view(), g->d_Vint.p, g->npad);
   g->mark_end(kStVxc, e);
   g->mark_begin(kStFinalize, &e);
   launch_finalize_v(g->stream, g->nbasis, g->npad, g->d_Vint.p, g->d_ref2int.p, dV);
@@ -475,7 +480,7 @@ int chx_grid_get_density_u(chx_grid* g, const double* D, int nmats, int nspin, i
         launch_density_c64(g->stream, 0, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_slots.p, g->d_grp_ntiles.p,
                            g->d_bt_off.p, g->d_bt_list.p, g->d_cache.p, g->d_Dint.p, g->npad, part);
       else
-        launch_density_f64(g->stream, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
+        launch_density_f64(g->stream, 0, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
                            g->d_bt_list.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(), g->d_Dint.p, g->npad, part);
       launch_point_rho(g->stream, g->nbatches, g->d_grp_off.p, part, rhou + (size_t)m * 4 * nslots);
     }
@@ -509,8 +514,8 @@ int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis) {
         launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, g->d_Vout.p);
       } else {
         launch_fill(g->stream, g->d_Vint.p, (size_t)g->npad * g->npad, 0.0);
-        launch_vxc_f64(g->stream, (int)g->vwork.size(), g->d_vwork.p, g->d_pair_batches.p, g->d_alive.p, g->ntiles,
-                       g->d_x.p, g->d_y.p, g->d_z.p, coef, g->view(), g->d_Vint.p, g->npad);
+        launch_vxc_f64(g->stream, 1, (int)g->vwork.size(), g->d_vwork.p, g->d_pair_batches.p, g->d_alive.p,
+                       g->ntiles, g->d_x.p, g->d_y.p, g->d_z.p, coef, g->view(), g->d_Vint.p, g->npad);
         launch_finalize_v(g->stream, g->nbasis, g->npad, g->d_Vint.p, g->d_ref2int.p, g->d_Vout.p);
       }
       CHX_HIP(hipMemcpyAsync(F + m * nb2, g->d_Vout.p, sizeof(double) * nb2, hipMemcpyDeviceToHost, g->stream));
@@ -825,8 +830,6 @@ int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order) {
       return;
     }
     const bool e = o.find('e') != std::string::npos, v = o.find('v') != std::string::npos;
-    require(xc->family < 2 || g->cached, CHX_ERR_UNSUPPORTED,
-            "meta-GGA functionals need the AO tables in HBM (chx_grid_set_ao_cache)");
     run_functional(g, xc, e, v);
     if (v) {
       g->v_family = xc->family;

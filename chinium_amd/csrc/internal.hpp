@@ -105,9 +105,11 @@ void launch_becke(hipStream_t s, long npts, const double* px, const double* py, 
 void launch_screen(hipStream_t s, int nbatches, int ntiles, const double* bx, const double* by, const double* bz,
                    const int* bn, BasisView bv, double thr, unsigned char* alive);
 void launch_permute_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, double* Dint);
-void launch_density_f64(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
-                        const int* bt_off, const int* bt_list, const double* bx, const double* by, const double* bz,
-                        BasisView bv, const double* Dint, int npad, double* part);
+// mode 0: rho, grad rho / 2 into part ([group][4][64]); mode r = 1..3: the tau
+// partial of direction r (part = component r - 1 of [group][3][64]).
+void launch_density_f64(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
+                        const int* grp_ntiles, const int* bt_off, const int* bt_list, const double* bx,
+                        const double* by, const double* bz, BasisView bv, const double* Dint, int npad, double* part);
 void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* part_tau,
                        const double* bw, const int* codes, int ncodes, int family, int want_e, int want_v,
                        double* rho_out, double* coef, double* en_part, int en_blocks);
@@ -115,9 +117,10 @@ void launch_eval_functional(hipStream_t s, long n, const int* codes, int ncodes,
                             const double* tau, double* exc, double* vrho, double* vsigma, double* vtau);
 void launch_reduce_en(hipStream_t s, int nparts, const double* en_part, double* en);
 void launch_fill(hipStream_t s, double* p, size_t n, double v);
-void launch_vxc_f64(hipStream_t s, int nwork, const int4* work, const int* pair_batches, const unsigned char* alive,
-                    int ntiles, const double* bx, const double* by, const double* bz, const double* coef, BasisView bv,
-                    double* Vint, int npad);
+// ncomp 1: a phi + c . grad phi; ncomp 4 adds the tau term (t = coef[4]).
+void launch_vxc_f64(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches,
+                    const unsigned char* alive, int ntiles, const double* bx, const double* by, const double* bz,
+                    const double* coef, BasisView bv, double* Vint, int npad);
 void launch_finalize_v(hipStream_t s, int n, int npad, const double* Vint, const int* ref2int, double* V);
 void launch_vrho_coef(hipStream_t s, long nslots, const double* vrho_w, double* coef);
 // Cached-AO FP64 path (kernels_cached.hip).

@@ -118,10 +118,14 @@ void launch_permute_d(hipStream_t s, int n, int npad, const double* D, const int
 
 // ---------------------------------------------------------------------------
 // Density. One block (4 waves) per work group = (batch b, up to 4 alive AO
-// tiles j as output columns). X[g, nu] = sum_{alive tiles i} Phi_i[g, :] D[i, nu]
+// tiles j as output columns). X[g, nu] = sum_{alive tiles i} A_i[g, :] D[i, nu]
 // accumulates on MFMA over K = the batch's alive tiles, then
-//   rho[g] += sum_nu X phi_nu,  drho[g]/2 += sum_nu X grad phi_nu.
-// Wave w owns rows g = 16w..16w+15 and all 128 output columns.
+// MODE 0 (A = phi): rho[g] += sum_nu X phi_nu, drho[g]/2 += sum_nu X grad phi_nu
+//   (4 components per point into part, [group][4][64]);
+// MODE r = 1..3 (A = d_r phi): sum_nu X d_r phi_nu, the tau partial of
+//   direction r (part points at component r - 1 of [group][3][64]).
+// Wave w owns rows g = 16w..16w+15 and all 128 output columns. AO values are
+// recomputed here (no tables in HBM).
 struct DensSmem {
   union {
     struct {
@@ -133,6 +137,7 @@ struct DensSmem {
   double px[kBatch], py[kBatch], pz[kBatch];
 };
 
+template <int MODE>
 __global__ __launch_bounds__(256) void k_density_f64(const int* __restrict__ grp_batch, const int* __restrict__ grp_tiles,
                                                      const int* __restrict__ grp_ntiles, const int* __restrict__ bt_off,
                                                      const int* __restrict__ bt_list, const double* __restrict__ bx,
@@ -169,10 +174,11 @@ __global__ __launch_bounds__(256) void k_density_f64(const int* __restrict__ grp
       const int g = item % kBatch;
       const int4 sg = bv.seg[s0 + item / kBatch];
       double v[7], gx[7], gy[7], gz[7];
-      shell_eval<false>(bv, sg.x, sm.px[g], sm.py[g], sm.pz[g], v, gx, gy, gz);
+      shell_eval<MODE != 0>(bv, sg.x, sm.px[g], sm.py[g], sm.pz[g], v, gx, gy, gz);
+      const double* a = MODE == 0 ? v : MODE == 1 ? gx : MODE == 2 ? gy : gz;
 #pragma unroll
       for (int m = 0; m < 7; ++m)
-        if (m < sg.z) sm.k.A[sg.w + m][g] = v[sg.y + m];
+        if (m < sg.z) sm.k.A[sg.w + m][g] = a[sg.y + m];
     }
     // D block rows of tile i, columns of the group's tiles
     for (int e = tid; e < kTile * kGroup * kTile; e += 256) {
@@ -195,7 +201,9 @@ __global__ __launch_bounds__(256) void k_density_f64(const int* __restrict__ grp
     __syncthreads();
   }
 
-  // Epilogue: contract X with phi_j and grad phi_j, tile by tile.
+  // Epilogue: contract X with phi_j and grad phi_j (MODE 0) or d_r phi_j
+  // (MODE r), tile by tile.
+  constexpr int NOUT = MODE == 0 ? 4 : 1;
   double s[4][4];
 #pragma unroll
   for (int c = 0; c < 4; ++c)
@@ -231,7 +239,7 @@ __global__ __launch_bounds__(256) void k_density_f64(const int* __restrict__ grp
         for (int r = 0; r < 4; ++r) {
           const int g = 16 * wave + 4 * r + lk;
 #pragma unroll
-          for (int c = 0; c < 4; ++c) s[c][r] += x[r] * sm.E[c][g][col];
+          for (int c = 0; c < NOUT; ++c) s[c][r] += x[r] * sm.E[MODE == 0 ? c : MODE][g][col];
         }
       }
       __syncthreads();
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(256) void k_density_f64(const int* __restrict__ grp
   }
   // Reduce over the 16 lanes that share rows, then lane k writes (c = k/4, r = k%4).
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < NOUT; ++c)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       double v = s[c][r];
@@ -251,20 +259,29 @@ __global__ __launch_bounds__(256) void k_density_f64(const int* __restrict__ grp
     }
   double out = 0;
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < NOUT; ++c)
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (lrow == 4 * c + r) out = s[c][r];
   const int c = lrow >> 2, r = lrow & 3;
-  part[((size_t)grp * 4 + c) * kBatch + 16 * wave + 4 * r + lk] = out;
+  constexpr int grp_stride = (MODE == 0 ? 4 : 3) * kBatch;
+  if (c < NOUT) part[(size_t)grp * grp_stride + c * kBatch + 16 * wave + 4 * r + lk] = out;
 }
 
-void launch_density_f64(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
-                        const int* bt_off, const int* bt_list, const double* bx, const double* by, const double* bz,
-                        BasisView bv, const double* Dint, int npad, double* part) {
+void launch_density_f64(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
+                        const int* grp_ntiles, const int* bt_off, const int* bt_list, const double* bx,
+                        const double* by, const double* bz, BasisView bv, const double* Dint, int npad, double* part) {
   if (ngroups == 0) return;
-  hipLaunchKernelGGL(k_density_f64, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_ntiles, bt_off, bt_list,
-                     bx, by, bz, bv, Dint, npad, part);
+#define CHX_DF64(M)                                                                                       \
+  hipLaunchKernelGGL(k_density_f64<M>, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_ntiles, \
+                     bt_off, bt_list, bx, by, bz, bv, Dint, npad, part);
+  switch (mode) {
+    case 0: CHX_DF64(0); break;
+    case 1: CHX_DF64(1); break;
+    case 2: CHX_DF64(2); break;
+    default: CHX_DF64(3); break;
+  }
+#undef CHX_DF64
   CHX_HIP(hipGetLastError());
 }
 
@@ -657,10 +674,14 @@ void launch_vrho_coef(hipStream_t s, long nslots, const double* vrho_w, double* 
 
 // ---------------------------------------------------------------------------
 // Vxc. One block per work item (super-tiles I <= J of 64 AOs, range of
-// batches where both are alive). V_IJ += sum_g Phi_I^T B_J + B_I^T Phi_J on
-// MFMA, K = points, 32 points per LDS stage. Wave w owns output rows
-// 16w..16w+15 and all 64 columns. The block result is added to V with
-// hardware FP64 atomics once per work item.
+// batches where both are alive). V_IJ += sum_g P_I^T B_J + B_I^T P_J on
+// MFMA, K = points, 32 points per LDS stage. Pass 0 has P = phi and
+// B = a phi + c . grad phi; meta-GGA (NC = 4) adds passes r = 1..3 with
+// P = d_r phi and B = t d_r phi (t = w v_tau / 4, so the pair of products
+// gives w v_tau / 2 d_r phi_mu d_r phi_nu). AO values are recomputed for each
+// pass (no tables in HBM). Wave w owns output rows 16w..16w+15 and all 64
+// columns. The block result is added to V with hardware FP64 atomics once per
+// work item.
 constexpr int kVxcK = 32;
 constexpr int kSupAO = kSuper * kTile;  // 64
 
@@ -669,11 +690,12 @@ struct VxcSmem {
   double BI[kVxcK][kSupAO + 1];
   double PJ[kVxcK][kSupAO + 1];
   double BJ[kVxcK][kSupAO + 1];
-  double px[kVxcK], py[kVxcK], pz[kVxcK], ca[kVxcK], cx[kVxcK], cy[kVxcK], cz[kVxcK];
+  double px[kVxcK], py[kVxcK], pz[kVxcK], ca[kVxcK], cx[kVxcK], cy[kVxcK], cz[kVxcK], ct[kVxcK];
 };
 
 __device__ __forceinline__ void vxc_fill_super(VxcSmem& sm, double (*P)[kSupAO + 1], double (*B)[kSupAO + 1],
-                                               int sup, const unsigned char* alive_b, const BasisView& bv, int tid) {
+                                               int sup, const unsigned char* alive_b, const BasisView& bv, int tid,
+                                               int pass) {
   for (int half = 0; half < kSuper; ++half) {
     const int t = sup * kSuper + half;
     if (!alive_b[t]) continue;
@@ -683,19 +705,26 @@ __device__ __forceinline__ void vxc_fill_super(VxcSmem& sm, double (*P)[kSupAO +
       const int4 sg = bv.seg[s0 + item / kVxcK];
       double v[7], gx[7], gy[7], gz[7];
       shell_eval<true>(bv, sg.x, sm.px[g], sm.py[g], sm.pz[g], v, gx, gy, gz);
-      const double a = sm.ca[g], cx = sm.cx[g], cy = sm.cy[g], cz = sm.cz[g];
+      const double a = sm.ca[g], cx = sm.cx[g], cy = sm.cy[g], cz = sm.cz[g], t = sm.ct[g];
+      const double* d = pass == 1 ? gx : pass == 2 ? gy : gz;
 #pragma unroll
       for (int m = 0; m < 7; ++m)
         if (m < sg.z) {
           const int col = half * kTile + sg.w + m;
-          const double ph = v[sg.y + m];
-          P[g][col] = ph;
-          B[g][col] = a * ph + cx * gx[sg.y + m] + cy * gy[sg.y + m] + cz * gz[sg.y + m];
+          if (pass == 0) {
+            const double ph = v[sg.y + m];
+            P[g][col] = ph;
+            B[g][col] = a * ph + cx * gx[sg.y + m] + cy * gy[sg.y + m] + cz * gz[sg.y + m];
+          } else {
+            P[g][col] = d[sg.y + m];
+            B[g][col] = t * d[sg.y + m];
+          }
         }
     }
   }
 }
 
+template <int NC>
 __global__ __launch_bounds__(256) void k_vxc_f64(const int4* __restrict__ work, const int* __restrict__ pair_batches,
                                                  const unsigned char* __restrict__ alive, int ntiles,
                                                  const double* __restrict__ bx, const double* __restrict__ by,
@@ -715,15 +744,7 @@ __global__ __launch_bounds__(256) void k_vxc_f64(const int4* __restrict__ work, 
     const int b = pair_batches[it];
     const unsigned char* ab = alive + (size_t)b * ntiles;
     for (int h = 0; h < kBatch / kVxcK; ++h) {
-      // zero the tiles, load the points and their coefficients
-      for (int e = tid; e < kVxcK * (kSupAO + 1); e += 256) {
-        (&sm.PI[0][0])[e] = 0.0;
-        (&sm.BI[0][0])[e] = 0.0;
-        if (!diag) {
-          (&sm.PJ[0][0])[e] = 0.0;
-          (&sm.BJ[0][0])[e] = 0.0;
-        }
-      }
+      // load the points and their coefficients
       if (tid < kVxcK) {
         const size_t slot = (size_t)b * kBatch + h * kVxcK + tid;
         sm.px[tid] = bx[slot];
@@ -733,10 +754,21 @@ __global__ __launch_bounds__(256) void k_vxc_f64(const int4* __restrict__ work, 
         sm.cx[tid] = coef[kCoef * slot + 1];
         sm.cy[tid] = coef[kCoef * slot + 2];
         sm.cz[tid] = coef[kCoef * slot + 3];
+        sm.ct[tid] = NC == 4 ? coef[kCoef * slot + 4] : 0.0;
+      }
+      for (int pass = 0; pass < NC; ++pass) {
+      // zero the tiles (AOs outside the alive tiles stay zero), then fill them
+      for (int e = tid; e < kVxcK * (kSupAO + 1); e += 256) {
+        (&sm.PI[0][0])[e] = 0.0;
+        (&sm.BI[0][0])[e] = 0.0;
+        if (!diag) {
+          (&sm.PJ[0][0])[e] = 0.0;
+          (&sm.BJ[0][0])[e] = 0.0;
+        }
       }
       __syncthreads();
-      vxc_fill_super(sm, sm.PI, sm.BI, I, ab, bv, tid);
-      if (!diag) vxc_fill_super(sm, sm.PJ, sm.BJ, J, ab, bv, tid);
+      vxc_fill_super(sm, sm.PI, sm.BI, I, ab, bv, tid, pass);
+      if (!diag) vxc_fill_super(sm, sm.PJ, sm.BJ, J, ab, bv, tid, pass);
       __syncthreads();
       double (*PJ)[kSupAO + 1] = diag ? sm.PI : sm.PJ;
       double (*BJ)[kSupAO + 1] = diag ? sm.BI : sm.BJ;
@@ -752,6 +784,7 @@ __global__ __launch_bounds__(256) void k_vxc_f64(const int4* __restrict__ work, 
         }
       }
       __syncthreads();
+      }
     }
   }
   // C layout: acc[cb][r] = V[row = 16w + 4r + lk][col = 16 cb + lrow]
@@ -765,12 +798,16 @@ __global__ __launch_bounds__(256) void k_vxc_f64(const int4* __restrict__ work, 
     }
 }
 
-void launch_vxc_f64(hipStream_t s, int nwork, const int4* work, const int* pair_batches, const unsigned char* alive,
-                    int ntiles, const double* bx, const double* by, const double* bz, const double* coef, BasisView bv,
-                    double* Vint, int npad) {
+void launch_vxc_f64(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches,
+                    const unsigned char* alive, int ntiles, const double* bx, const double* by, const double* bz,
+                    const double* coef, BasisView bv, double* Vint, int npad) {
   if (nwork == 0) return;
-  hipLaunchKernelGGL(k_vxc_f64, dim3(nwork), dim3(256), 0, s, work, pair_batches, alive, ntiles, bx, by, bz, coef, bv,
-                     Vint, npad);
+  if (ncomp == 4)
+    hipLaunchKernelGGL(k_vxc_f64<4>, dim3(nwork), dim3(256), 0, s, work, pair_batches, alive, ntiles, bx, by, bz,
+                       coef, bv, Vint, npad);
+  else
+    hipLaunchKernelGGL(k_vxc_f64<1>, dim3(nwork), dim3(256), 0, s, work, pair_batches, alive, ntiles, bx, by, bz,
+                       coef, bv, Vint, npad);
   CHX_HIP(hipGetLastError());
 }
 

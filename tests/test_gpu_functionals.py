@@ -62,8 +62,12 @@ def test_device_functionals_match_oracle_random_points():
             assert not bad.any(), (codes, name)
 
 
+@pytest.mark.parametrize("cache", [1, 0])
 @pytest.mark.parametrize("case", ["h2o", "ch4_tzvp"])
-def test_r2scan_parity(case):
+def test_r2scan_parity(case, cache):
+    """r2SCAN through the FP64 form against the oracle, with the AO tables in HBM (cache 1) and
+    with the AO values recomputed in the kernels (cache 0: tau partials and the tau term of V
+    take three more passes over d_r phi)."""
     if case == "h2o":
         w = wl.build_workload("h2o_ccpvdz_sg1_pbe")
         Z, xyz, basis, grid_name, nocc = w["Z"], w["xyz_bohr"], w["basis"], "SG-1", 5
@@ -72,13 +76,15 @@ def test_r2scan_parity(case):
         Z, xyz, basis, grid_name, nocc = Z, X * ca.ANGSTROM_TO_BOHR, "def2-tzvp", "SG-1", 5
     mol = ca.Molecule(Z, xyz, basis)
     D, S = wl.workload_density(mol, nocc, 11)
-    print("r2SCAN parity %s (%d AOs)" % (case, mol.nbasis), flush=True)
+    print("r2SCAN parity %s (%d AOs), AO cache %d" % (case, mol.nbasis, cache), flush=True)
     grid = ca.Grid(mol, grid_name, 1, 0)
+    grid.setAOCache(cache)
     xc = ca.ExchangeCorrelation()
     xc.Read("R2SCAN")
     assert xc.Family == "mGGA"
     grid.setType(2)
     grid.getAO(0, 0)
+    assert grid.aoCached() == bool(cache)
     E, V = ca.xc_build(grid, xc, D)
     orc = ol.OracleSystem(Z, xyz, grid_name, basis)
     orc.set_functional("R2SCAN")
