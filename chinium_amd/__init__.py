@@ -57,6 +57,9 @@ def lib():
         "chx_mol_nshells": (i, [_vp, P(i)]),
         "chx_mol_get_shells": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "chx_mol_overlap": (i, [_vp, _vp]),
+        "chx_mol_int1e": (i, [_vp, _vp, _vp, _vp]),
+        "chx_mol_eri": (i, [_vp, _vp]),
+        "chx_mol_nuclear_repulsion": (i, [_vp, P(ctypes.c_double)]),
         "chx_mol_destroy": (None, [_vp]),
         "chx_grid_create": (i, [_vp, cp, cp, i, i, P(_vp)]),
         "chx_grid_copy": (i, [_vp, P(_vp)]),
@@ -195,6 +198,25 @@ class Molecule:
         S = np.zeros((n, n), order="F")
         _check(lib().chx_mol_overlap(self.h, _ptr(S)))
         return np.array(S)
+
+    def int1e(self):
+        """(S, T, V): overlap, kinetic and nuclear-attraction matrices (chx_mol_int1e)."""
+        n = self.nbasis
+        S, T, V = (np.zeros((n, n), order="F") for _ in range(3))
+        _check(lib().chx_mol_int1e(self.h, _ptr(S), _ptr(T), _ptr(V)))
+        return np.array(S), np.array(T), np.array(V)
+
+    def eri(self):
+        """Electron-repulsion integrals as an n^4 array, eri[mu, nu, la, si] = (mu nu|la si)."""
+        n = self.nbasis
+        g = np.zeros(n ** 4)
+        _check(lib().chx_mol_eri(self.h, _ptr(g)))
+        return g.reshape((n, n, n, n), order="F")
+
+    def nuclear_repulsion(self):
+        e = ctypes.c_double()
+        _check(lib().chx_mol_nuclear_repulsion(self.h, ctypes.byref(e)))
+        return e.value
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

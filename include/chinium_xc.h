@@ -65,6 +65,16 @@ int chx_mol_nshells(const chx_mol* m, int* nshells);
 int chx_mol_get_shells(const chx_mol* m, int* type, int* nprim, int* center, double* exps, double* coefs);
 /* Analytic AO overlap S (column-major), used to build S-orthonormal inputs. */
 int chx_mol_overlap(const chx_mol* m, double* S);
+/* Integrals for the SCF loop around the XC path (small molecules; host code,
+ * McMurchie-Davidson over the same AOs as the grid). They replace libint2's
+ * Int2C1E (src/Integral/Representation.cpp:62-63) and the integrals behind
+ * Int4C2E::ContractInts (src/Integral/Int4C2E.cpp:673-683).
+ * chx_mol_int1e: overlap S, kinetic T and nuclear attraction V (n x n each).
+ * chx_mol_eri: (mu nu|la si) at mu + n (nu + n (la + n si)), n <= 96.
+ * chx_mol_nuclear_repulsion: sum over atom pairs of Z_A Z_B / R_AB. */
+int chx_mol_int1e(const chx_mol* m, double* S, double* T, double* V);
+int chx_mol_eri(const chx_mol* m, double* eri);
+int chx_mol_nuclear_repulsion(const chx_mol* m, double* E);
 void chx_mol_destroy(chx_mol* m);
 
 /* ---- Grid ------------------------------------------------------------------

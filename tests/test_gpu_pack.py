@@ -162,6 +162,48 @@ def test_pack_with_wide_member(alkanes):
     assert np.abs(Vs - Vs1).max() <= 1e-12
 
 
+def test_pack_meta_gga():
+    """r2SCAN (meta-GGA: four density launches per molecule, tau term in V) through a pack of
+    three small molecules on two lanes, against the single-grid builds and the oracle."""
+    xc = ca.ExchangeCorrelation()
+    xc.Read("R2SCAN")
+    mols = []
+    for k, n in enumerate((1, 4, 2)):
+        Z, X = wl.alkane_angstrom(n, 90 + k)
+        xyz = X * ca.ANGSTROM_TO_BOHR
+        mol = ca.Molecule(Z, xyz, "def2-svp")
+        D, S = wl.workload_density(mol, 4 * n + 1, 60 + k)
+        grid = ca.Grid(mol, "SG-1", 1, 0)
+        grid.setPrecision(ca.CHX_PREC_MIXED)
+        grid.setType(2)
+        grid.getAO(0, 0)
+        E1, V1 = ca.xc_build(grid, xc, D)
+        nb = mol.nbasis
+        dD, dV = hip.DeviceArray(nb * nb), hip.DeviceArray(nb * nb)
+        dD.upload(np.asfortranarray(D).ravel(order="F"))
+        mols.append(dict(n=n, Z=Z, xyz=xyz, mol=mol, D=D, grid=grid, E1=E1, V1=V1, dD=dD, dV=dV))
+    pack = ca.Pack([m["grid"] for m in mols])
+    pack.set_lanes(2)
+    for rep in range(2):
+        E, N = pack.xc(xc, [m["dD"].ptr for m in mols], [m["dV"].ptr for m in mols])
+        for k, m in enumerate(mols):
+            V = m["dV"].download().reshape((m["mol"].nbasis,) * 2, order="F")
+            _say("  r2SCAN pack rep %d C%d: dE %.1e dV %.1e" % (rep, m["n"], abs(E[k] - m["E1"]),
+                                                               np.abs(V - m["V1"]).max()))
+            assert abs(E[k] - m["E1"]) <= 1e-12 * abs(m["E1"])
+            assert np.abs(V - m["V1"]).max() <= 1e-12
+    m = mols[1]
+    orc = ol.OracleSystem(m["Z"], m["xyz"], "SG-1", "def2-svp")
+    orc.set_functional("R2SCAN")
+    orc.get_ao()
+    Eo, No, Vo = orc.xc(m["D"])
+    V = m["dV"].download().reshape((m["mol"].nbasis,) * 2, order="F")
+    _say("  r2SCAN pack C%d vs oracle: dE/atom %.2e max dV %.2e" % (m["n"], abs(E[1] - Eo) / len(m["Z"]),
+                                                                   np.abs(V - Vo).max()))
+    assert abs(E[1] - Eo) / len(m["Z"]) <= 1e-5
+    assert np.abs(V - Vo).max() <= 5e-5
+
+
 def test_pack_rejects_fp64_grid(alkanes):
     m = alkanes["mols"][0]
     g = ca.Grid(m["mol"], "SG-1", 1, 0)

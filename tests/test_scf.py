@@ -1,0 +1,104 @@
+"""Integrals and the restricted SCF loop (chx_mol_int1e / chx_mol_eri, chinium_amd.scf) on the
+CPU: Hartree-Fock energies and closed-form or quadrature checks of the integrals."""
+import numpy as np
+import pytest
+
+import chinium_amd as ca
+import oracle_lib as ol
+from chinium_amd import workloads as wl
+from chinium_amd.scf import RestrictedSCF
+
+# Water geometry (bohr) of T. D. Crawford's programming project 3, whose published STO-3G RHF
+# energy is -74.942079928192 Eh with nuclear repulsion 8.002367061810450 Eh.
+H2O_XYZ = np.array([[0.0, -0.143225816552, 0.0], [1.638036840407, 1.136548822547, 0.0],
+                    [-1.638036840407, 1.136548822547, 0.0]])
+
+
+def _rotation(a, b, c):
+    Rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    Rx = np.array([[1, 0, 0], [0, np.cos(b), -np.sin(b)], [0, np.sin(b), np.cos(b)]])
+    Ry = np.array([[np.cos(c), 0, np.sin(c)], [0, 1, 0], [-np.sin(c), 0, np.cos(c)]])
+    return Rz @ Rx @ Ry
+
+
+def test_sto3g_water_rhf():
+    mol = ca.Molecule([8, 1, 1], H2O_XYZ, "sto-3g")
+    assert mol.nbasis == 7
+    assert abs(mol.nuclear_repulsion() - 8.002367061810450) <= 1e-12
+    r = RestrictedSCF(mol, 5).run()
+    print("STO-3G water RHF %.12f in %d iterations" % (r["E"], r["iterations"]))
+    assert abs(r["E"] - (-74.942079928192)) <= 1e-9
+
+
+def test_h2_sto3g_closed_form():
+    """H2 / STO-3G at R = 1.4 bohr: two AOs, so the RHF orbital is fixed by symmetry,
+    sigma_g = (1 + 2)/sqrt(2 + 2 S12), and E follows from the integrals in closed form
+    (Szabo, Ostlund, ch. 3: E = -1.1167 Eh)."""
+    mol = ca.Molecule([1, 1], np.array([[0, 0, 0], [0, 0, 1.4]]), "sto-3g")
+    S, T, V = mol.int1e()
+    g = mol.eri()
+    c = np.array([1.0, 1.0]) / np.sqrt(2 + 2 * S[0, 1])
+    h = c @ (T + V) @ c
+    J = np.einsum("i,j,k,l,ijkl->", c, c, c, c, g)
+    E = 2 * h + J + 1 / 1.4
+    r = RestrictedSCF(mol, 1).run()
+    print("H2 STO-3G: closed form %.10f, SCF %.10f, S12 %.6f" % (E, r["E"], S[0, 1]))
+    assert abs(r["E"] - E) <= 1e-10
+    assert abs(S[0, 1] - 0.6593) <= 1e-4  # Szabo-Ostlund eq. 3.229
+    assert abs(E - (-1.1167)) <= 1e-4
+
+
+@pytest.mark.parametrize("case", ["h2o_ccpvdz", "ch4_tzvp"])
+def test_one_electron_integrals(case):
+    """S against the analytic overlap; T and V against quadrature of 1/2 grad phi . grad phi and
+    -sum_C Z_C phi phi / |r - C| on an SG-3 grid (pure d and f functions included)."""
+    if case == "h2o_ccpvdz":
+        w = wl.build_workload("h2o_ccpvdz_sg1_pbe")
+        Z, xyz, basis = w["Z"], w["xyz_bohr"], w["basis"]
+    else:
+        Z, X = wl.alkane_angstrom(1, 4)
+        Z, xyz, basis = Z, X * ca.ANGSTROM_TO_BOHR, "def2-tzvp"
+    mol = ca.Molecule(Z, xyz, basis)
+    S, T, V = mol.int1e()
+    assert np.abs(S - mol.overlap()).max() <= 1e-13
+    orc = ol.OracleSystem(Z, xyz, "SG-3", basis)
+    P = orc.points()
+    ao, g, _ = orc.ao_values(P[:, :3])
+    wq = P[:, 3]
+    Tq = 0.5 * sum((g[r] * wq) @ g[r].T for r in range(3))
+    vn = np.zeros(len(P))
+    for zc, c in zip(Z, np.asarray(xyz)):
+        vn -= zc / np.linalg.norm(P[:, :3] - c, axis=1)
+    Vq = (ao * (wq * vn)) @ ao.T
+    dT, dV = np.abs(T - Tq).max(), np.abs(V - Vq).max()
+    print("%s: %d AOs, max|T - T_quad| %.1e, max|V - V_quad| %.1e" % (case, mol.nbasis, dT, dV))
+    assert dT <= 2e-4 * np.abs(T).max()
+    assert dV <= 2e-4 * np.abs(V).max()
+
+
+def test_rotation_invariance():
+    """The RHF energy (cc-pVDZ: pure d) and the core-Hamiltonian levels (def2-TZVP: pure f) do not
+    change when the molecule is rotated and translated."""
+    R, t = _rotation(0.7, 1.1, -0.4), np.array([0.3, -1.2, 2.0])
+    w = wl.build_workload("h2o_ccpvdz_sg1_pbe")
+    E = [RestrictedSCF(ca.Molecule(w["Z"], x, w["basis"]), 5).run()["E"]
+         for x in (w["xyz_bohr"], w["xyz_bohr"] @ R.T + t)]
+    print("H2O/cc-pVDZ RHF %.10f, rotated %+.1e" % (E[0], E[1] - E[0]))
+    assert abs(E[1] - E[0]) <= 1e-10
+    Z, X = wl.alkane_angstrom(1, 4)
+    lev = []
+    for x in (X * ca.ANGSTROM_TO_BOHR, X * ca.ANGSTROM_TO_BOHR @ R.T + t):
+        S, T, V = ca.Molecule(Z, x, "def2-tzvp").int1e()
+        s, U = np.linalg.eigh(S)
+        Zm = U @ np.diag(s ** -0.5) @ U.T
+        lev.append(np.linalg.eigvalsh(Zm @ (T + V) @ Zm))
+    assert np.abs(lev[1] - lev[0]).max() <= 1e-9
+
+
+def test_eri_limits():
+    Z, X = wl.water_cluster_angstrom(5, 6.0, 3)
+    mol = ca.Molecule(Z, X * ca.ANGSTROM_TO_BOHR, "cc-pvdz")
+    assert mol.nbasis > 96
+    with pytest.raises(ca.ChxError) as e:
+        mol.eri()
+    assert e.value.code == -5
