@@ -271,9 +271,9 @@ def main():
 
 def bench_pack(args):
     """64 linear alkanes C_nH_2n+2 (n seeded-uniform in 1..40) / def2-SVP / SG-1 / PBE, mixed form:
-    all molecules' XC builds as one pack (every molecule's kernels on 16 staggered lanes, replayed as
-    one hipGraph, Vxc work-item length chosen by the autotuner) against the same grids built one after
-    another through the single-grid calls."""
+    all molecules' XC builds as one pack (every molecule's kernels on staggered lanes, replayed as
+    one hipGraph, Vxc work-item length and lane count chosen by the autotuner) against the same grids
+    built one after another through the single-grid calls."""
     import numpy as np
 
     import chinium_amd as ca
@@ -358,7 +358,7 @@ def bench_pack(args):
                    "name": "alkanes64", "molecules": len(items), "points": points,
                    "nbasis_total": sum(it["mol"].nbasis for it in items),
                    "form": "mixed, per-molecule kernels on staggered lanes, hipGraph replay",
-                   "lanes": min(16, len(items)), "vxc_chunk": chunk},
+                   "lanes": pack.lanes(), "vxc_chunk": chunk},
         "sequential_ms_per_step": round(ms_seq, 4),
         "speedup_vs_sequential": round(ms_seq / ms_pack, 3),
         "autotune_ms": {str(k): round(v, 4) for k, v in tune_ms.items()},

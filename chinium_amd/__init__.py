@@ -102,6 +102,7 @@ def lib():
         "chx_pack_xc": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "chx_pack_set_graph": (i, [_vp, i]),
         "chx_pack_set_lanes": (i, [_vp, i]),
+        "chx_pack_lanes": (i, [_vp, P(i)]),
         "chx_pack_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
         "chx_pack_stream": (i, [_vp, P(_vp)]),
         "chx_pack_destroy": (None, [_vp]),
@@ -513,8 +514,14 @@ class Pack:
         """Splits the molecules into `lanes` groups on streams of their own, staggered by a stage."""
         _check(lib().chx_pack_set_lanes(self.h, int(lanes)))
 
+    def lanes(self):
+        v = ctypes.c_int()
+        _check(lib().chx_pack_lanes(self.h, ctypes.byref(v)))
+        return v.value
+
     def autotune(self, xc, D_ptrs, V_ptrs, reps=3):
-        """Times each Vxc work-item length, keeps the fastest; returns (chunk, {chunk: ms})."""
+        """Times each Vxc work-item length, keeps the fastest, then picks the lane count;
+        returns (chunk, {chunk: ms})."""
         c = ctypes.c_int()
         ms = (ctypes.c_float * 4)()
         _check(lib().chx_pack_autotune(self.h, xc.h, self._ptrs(D_ptrs), self._ptrs(V_ptrs), int(reps),

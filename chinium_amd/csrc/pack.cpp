@@ -289,11 +289,39 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
         best_ms = ms;
       }
     }
+    for (chx_grid* g : p->grids) set_vxc_chunk(g, cand[best]);
+    // then the lane count at that length (4, 8, 16, at most one lane per molecule)
+    const int n = (int)p->grids.size();
+    int best_lanes = -1;
+    float best_lanes_ms = 0.f;
+    for (int lanes : {4, 8, 16}) {
+      if (lanes > n && lanes != 4) continue;
+      p->lanes = std::min(lanes, n);
+      build_lanes(p);
+      run(p, xc, D_dev, V_dev, nullptr, nullptr);
+      CHX_HIP(hipEventRecord(a, p->stream));
+      for (int r = 0; r < reps; ++r) run(p, xc, D_dev, V_dev, nullptr, nullptr);
+      CHX_HIP(hipEventRecord(b, p->stream));
+      CHX_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      CHX_HIP(hipEventElapsedTime(&ms, a, b));
+      if (best_lanes < 0 || ms < best_lanes_ms) {
+        best_lanes = p->lanes;
+        best_lanes_ms = ms;
+      }
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    for (chx_grid* g : p->grids) set_vxc_chunk(g, cand[best]);
+    p->lanes = best_lanes;
     build_lanes(p);
     if (chunk_out) *chunk_out = cand[best];
+  });
+}
+
+int chx_pack_lanes(const chx_pack* p, int* lanes) {
+  return pguard([&] {
+    prequire(p && lanes, CHX_ERR_ARG, "bad arguments");
+    *lanes = (int)p->cut.size() - 1;
   });
 }
 

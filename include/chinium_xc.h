@@ -228,7 +228,8 @@ int chx_pack_xc(chx_pack* p, const chx_xc* xc, const double* const* D_dev, doubl
 /* 1 (default) replays a captured hipGraph, 0 launches eagerly. */
 int chx_pack_set_graph(chx_pack* p, int enable);
 /* Autotuner: times `reps` builds for each Vxc work-item length (16, 32, 64,
- * 128 batches), keeps the fastest; chunk_out gets it, ms_out[4] the times. */
+ * 128 batches), keeps the fastest (chunk_out gets it, ms_out[4] the times),
+ * then times 4, 8 and 16 lanes at that length and keeps the fastest. */
 int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev, double* const* V_dev, int reps,
                       int* chunk_out, float* ms_out);
 /* Lanes (1..16, default 16, at most one per molecule): the molecules are
@@ -238,6 +239,8 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
  * depend on it. chx_pack_stream stays the stream the whole build is ordered
  * on. */
 int chx_pack_set_lanes(chx_pack* p, int lanes);
+/* Lanes in use (after clamping to the molecule count). */
+int chx_pack_lanes(const chx_pack* p, int* lanes);
 int chx_pack_stream(const chx_pack* p, void** stream);
 void chx_pack_destroy(chx_pack* p);
 

@@ -96,8 +96,9 @@ def test_pack_autotune(alkanes):
     xc, pack = alkanes["xc"], alkanes["pack"]
     E0, _, V0 = _run(alkanes)
     chunk, ms = pack.autotune(xc, [d.ptr for d in alkanes["dD"]], [v.ptr for v in alkanes["dV"]], reps=2)
-    _say("  autotune: chunk %d, ms %s" % (chunk, {k: round(v, 4) for k, v in ms.items()}))
+    _say("  autotune: chunk %d, lanes %d, ms %s" % (chunk, pack.lanes(), {k: round(v, 4) for k, v in ms.items()}))
     assert chunk in (16, 32, 64, 128) and all(v > 0 for v in ms.values())
+    assert 1 <= pack.lanes() <= len(alkanes["mols"])
     E, _, Vs = _run(alkanes)
     # the density side does not depend on the Vxc item length; M is summed in
     # FP32 within an item, so another length changes V at the FP32 level only
