@@ -137,8 +137,8 @@ void density_from_device(chx_grid* g, const double* dD) {
     launch_split_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dsplit.p);
     g->mark_end(kStPermute, e);
     g->mark_begin(kStDensity, &e);
-    for (int r = 0; r <= (g->type >= 2 ? 3 : 0); ++r)  // r > 0: tau partials, A = d_r phi
-      launch_density_mx(g->stream, r, ng, g->hmol);
+    launch_density_mx(g->stream, 0, ng, g->hmol);
+    if (g->type >= 2) launch_density_mx(g->stream, 4, ng, g->hmol);  // tau partials, A = d_r phi, one launch
     g->mark_end(kStDensity, e);
     g->have_density = true;
     g->have_eval = false;

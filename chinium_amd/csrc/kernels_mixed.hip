@@ -497,8 +497,153 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev m) {
   }
 }
 
+// The three tau partials in one block: per K step the three A = d_r phi
+// operands (24 fragments) and the D block (16 fragments, shared by the three
+// products) go to LDS together, so D is staged and read once instead of three
+// times and a step takes two barriers instead of six. Three accumulator sets;
+// the epilogue runs the MODE r contraction for r = 1, 2, 3 in turn through the
+// X staging area. Same indexing as k_density_mx.
+struct DensMxTauSmem {
+  union {
+    u32x4 F[(3 * kDxA + kDxB) * kDxFrag];  // one K step's fragments: A_1, A_2, A_3, D
+    float X[kGroup * kTile][kBatch + 1];   // epilogue: X_r^T [nu][g]
+  };
+  float red[kGroup][3][kBatch];
+  double px[kBatch], py[kBatch], pz[kBatch];
+  int tl[kMxMaxTiles];
+};
+
+template <int R>
+__device__ __forceinline__ void dens_tau_epilogue(DensMxTauSmem& sm, const f32x4 (&acc)[2][4], const BasisView& bv,
+                                                  const int (&jt)[kGroup], int nt, int wave, int lane, int wr,
+                                                  int wc, int lrow, int lk) {
+  __syncthreads();  // the previous user of the X area (fragments or X_{R-1}) is done
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sm.X[64 * wc + 16 * c + lrow][32 * wr + 16 * r + 4 * lk + i] = acc[r][c][i];
+  __syncthreads();
+  DensEpi<R> epi;
+  if (wave < nt) {
+    const int t = wave == 1 ? jt[1] : wave == 2 ? jt[2] : wave == 3 ? jt[3] : jt[0];
+    epi.load(&sm.X[kTile * wave][lane]);
+    walk_tile_f32<true>(bv, t, sm.px[lane], sm.py[lane], sm.pz[lane], epi);
+  }
+  sm.red[wave][R - 1][lane] = epi.o[0];
+}
+
+__global__ __launch_bounds__(256, 2) void k_density_mx_tau(MolDev m) {
+  using lds_ptr = __attribute__((address_space(3))) void*;
+  __shared__ DensMxTauSmem sm;
+  const int grp = xcd_remap(blockIdx.x, gridDim.x);
+  const u16* __restrict__ cache = m.cache + (size_t)kMxT;  // component 1 (d_x phi); d_y, d_z follow
+  const u16* __restrict__ Ds = m.Ds;
+  const size_t sstride = (size_t)m.sstride;
+  const int npad = m.npad;
+  double* __restrict__ part = m.part_tau;  // [group][3][64]
+  const BasisView bv = m.bv;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int wr = wave & 1, wc = wave >> 1;
+  const int b = m.grp_batch[grp];
+  const int nt = m.grp_ntiles[grp];
+  int jt[kGroup];
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) jt[q] = m.grp_tiles[grp * kGroup + q];
+  const int it0 = m.bt_off[b], nk = m.bt_off[b + 1] - it0;
+  if (tid < kBatch) {
+    sm.px[tid] = m.bx[b * kBatch + tid];
+    sm.py[tid] = m.by[b * kBatch + tid];
+    sm.pz[tid] = m.bz[b * kBatch + tid];
+  }
+  for (int j = tid; j < nk; j += 256) sm.tl[j] = m.bt_list[it0 + j];
+  const int drow = lane >> 2, dch = (lane & 3) ^ ((drow >> 2) & 3);
+  const int fidx = 4 * lrow + (lk ^ ((lrow >> 2) & 3));
+  int offa[2], offb[4];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int f = wave + 4 * j;
+    offa[j] = (16 * (f >> 1) + drow) * kMxRow + (f & 1) * kTile + 8 * dch;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int f = wave + 4 * j, c = f >> 1, q = c >> 1;
+    const int t = q == 1 ? jt[1] : q == 2 ? jt[2] : q == 3 ? jt[3] : jt[0];
+    offb[j] = (f & 1) * npad * npad + ((q < nt ? t : jt[0]) * kTile + 16 * (c & 1) + drow) * npad + 8 * dch;
+  }
+  __syncthreads();
+  auto dma = [&](int k) {
+    const u16* a = cache + (size_t)(it0 + k) * sstride;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        __builtin_amdgcn_global_load_lds(a + (size_t)r * kMxT + offa[j],
+                                         (lds_ptr)(sm.F + (r * kDxA + wave + 4 * j) * kDxFrag), 16, 0, 0);
+    const int ti = sm.tl[k] * kTile;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      __builtin_amdgcn_global_load_lds(Ds + offb[j] + ti, (lds_ptr)(sm.F + (3 * kDxA + wave + 4 * j) * kDxFrag), 16,
+                                       0, 0);
+  };
+
+  f32x4 acc[3][2][4];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[d][r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int ncb = 2 * (nt - 2 * wc);
+  dma(0);
+  for (int k = 0; k < nk; ++k) {
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
+    __syncthreads();                     // ... and every other wave's
+    u32x4 a[3][2][2], bq[4][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) a[d][r][h] = sm.F[(d * kDxA + (2 * wr + r) * 2 + h) * kDxFrag + fidx];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) bq[c][h] = sm.F[(3 * kDxA + (4 * wc + c) * 2 + h) * kDxFrag + fidx];
+    __syncthreads();  // every wave holds its fragments: the buffer is free
+    if (k + 1 < nk) dma(k + 1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c < ncb) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+            acc[d][r][c] = mfma3(a[d][r][0], a[d][r][1], bq[c][0], bq[c][1], acc[d][r][c]);
+      }
+  }
+  dens_tau_epilogue<1>(sm, acc[0], bv, jt, nt, wave, lane, wr, wc, lrow, lk);
+  dens_tau_epilogue<2>(sm, acc[1], bv, jt, nt, wave, lane, wr, wc, lrow, lk);
+  dens_tau_epilogue<3>(sm, acc[2], bv, jt, nt, wave, lane, wr, wc, lrow, lk);
+  __syncthreads();
+  if (tid < 3 * kBatch) {
+    const int c = tid / kBatch, g = tid % kBatch;
+    double v = 0.0;
+    for (int q = 0; q < nt; ++q) v += (double)sm.red[q][c][g];
+    part[(size_t)grp * 3 * kBatch + c * kBatch + g] = v;
+  }
+}
+
 void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one) {
   if (nitems == 0) return;
+  if (mode == 4) {  // the three tau partials in one launch
+    hipLaunchKernelGGL(k_density_mx_tau, dim3(nitems), dim3(256), 0, s, one);
+    CHX_HIP(hipGetLastError());
+    return;
+  }
 #define CHX_DMX(M) hipLaunchKernelGGL((k_density_mx<M>), dim3(nitems), dim3(256), 0, s, one);
   switch (mode) {
     case 0: CHX_DMX(0); break;

@@ -135,7 +135,8 @@ void enqueue_lane(chx_pack* p, int m0, int m1, hipStream_t s, hipEvent_t after, 
   if (after) CHX_HIP(hipStreamWaitEvent(s, after, 0));
   for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
-    for (int r = 0; r <= (p->type >= 2 ? 3 : 0); ++r) launch_density_mx(s, r, (int)g->grp_batch.size(), g->hmol);
+    launch_density_mx(s, 0, (int)g->grp_batch.size(), g->hmol);
+    if (p->type >= 2) launch_density_mx(s, 4, (int)g->grp_batch.size(), g->hmol);
   }
   if (dens_done) CHX_HIP(hipEventRecord(dens_done, s));
   const int nc = p->family >= 2 ? 4 : 1;
