@@ -17,8 +17,10 @@
 // ch. 9). The full ERI tensor is formed, so this is for small bases.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -293,10 +295,14 @@ int chx_mol_eri(const chx_mol* m, double* eri) {
           for (const Prim& B : ao[nu]) v.push_back(pair_term(A, B));
       }
     const double pre = 2 * std::pow(M_PI, 2.5);
-    RTab R;
     const long np = (long)pairs.size();
-    for (long ij = 0; ij < np; ++ij)
-      for (long kl = 0; kl <= ij; ++kl) {
+    // rows ij of the unique quartets (ij >= kl) handed out to host threads;
+    // each quartet writes its own eight entries, so the threads never collide
+    std::atomic<long> next{0};
+    auto worker = [&] {
+      RTab R;
+      for (long ij = next++; ij < np; ij = next++)
+        for (long kl = 0; kl <= ij; ++kl) {
         double val = 0;
         for (const PairTerm& a : pairs[ij])
           for (const PairTerm& b : pairs[kl]) {
@@ -322,6 +328,12 @@ int chx_mol_eri(const chx_mol* m, double* eri) {
                                 {k, l, i, j}, {l, k, i, j}, {k, l, j, i}, {l, k, j, i}};
         for (const auto& q : idx) eri[q[0] + n * (q[1] + n * (q[2] + n * q[3]))] = val;
       }
+    };
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; ++t) pool.emplace_back(worker);
+    worker();
+    for (std::thread& t : pool) t.join();
   });
 }
 
