@@ -31,6 +31,8 @@ class RestrictedSCF:
     def __init__(self, mol, nocc, xc=None, grid=None):
         if xc is not None and grid is None:
             raise ValueError("a Kohn-Sham run needs a grid")
+        if not 0 < int(nocc) <= mol.nbasis:
+            raise ValueError("nocc must be in 1..nbasis")
         self.mol, self.nocc, self.xc, self.grid = mol, int(nocc), xc, grid
         self.S, T, V = mol.int1e()
         self.Hcore = T + V

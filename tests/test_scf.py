@@ -95,6 +95,18 @@ def test_rotation_invariance():
     assert np.abs(lev[1] - lev[0]).max() <= 1e-9
 
 
+def test_scf_argument_errors():
+    mol = ca.Molecule([8, 1, 1], H2O_XYZ, "sto-3g")
+    with pytest.raises(ValueError):
+        RestrictedSCF(mol, 0)
+    with pytest.raises(ValueError):
+        RestrictedSCF(mol, 8)
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    with pytest.raises(ValueError):
+        RestrictedSCF(mol, 5, xc)  # Kohn-Sham without a grid
+
+
 def test_eri_limits():
     Z, X = wl.water_cluster_angstrom(5, 6.0, 3)
     mol = ca.Molecule(Z, X * ca.ANGSTROM_TO_BOHR, "cc-pvdz")
