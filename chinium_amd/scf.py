@@ -12,8 +12,8 @@ caller of the XC path. Each iteration:
 * extrapolates F by DIIS on G = 2 (F D S - S D F) (SP.cpp:58).
 
 The reference runs ADIIS to 1e-1 and then CDIIS to 1e-6 on the DIIS error
-(SP.cpp:73-77); this loop uses CDIIS from the core-Hamiltonian guess, with a
-damped first phase. The integrals come from chx_mol_int1e / chx_mol_eri
+(SP.cpp:73-77); this loop uses CDIIS alone (up to 8 vectors) from the
+core-Hamiltonian guess. The integrals come from chx_mol_int1e / chx_mol_eri
 (McMurchie-Davidson on the host, full ERI tensor), so the loop is meant for
 small bases (n <= 96); the XC part runs wherever the grid is.
 """
