@@ -51,7 +51,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(the path would shard batches across GPUs and all-reduce V; see DESIGN.md)" % args.gpus,
+              "(the path would shard the grid batches across GPUs, broadcast D and all-reduce V)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
