@@ -660,8 +660,10 @@ void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one) {
 // recomputed in FP32:
 //   Bt[slot][0][hl][nu][g] = split(a phi + c . grad phi),
 //   meta-GGA (NC = 4) adds Bt[slot][r][hl][nu][g] = split(t d_r phi), t = w v_tau / 4.
-// Each wave assembles its slot's [c][hl][nu][g] block in LDS (zeros past the
-// tile's last AO) and writes it with 16-byte coalesced stores.
+// Each lane stores its point's values straight into the slot's [c][hl][nu][g]
+// block (for each AO, the wave's 64 lanes write one contiguous 128-byte row);
+// columns past the tile's last AO are written as zeros. No LDS stage and no
+// barrier, so occupancy is set by registers alone.
 template <int NC>
 struct MakeB {
   u16* sv;  // this lane's column of the [c][hl][nu][g] block
@@ -693,35 +695,34 @@ struct MakeB {
 template <int NC>
 __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev m) {
   constexpr int W = NC == 4 ? 1 : 4;  // waves per block
-  __shared__ u32x4 stage[W][NC * kMxT / 8];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
-  const int slot = blockIdx.x * W + wave, nslots = m.nslots;
-  const double* __restrict__ coef = m.coef;
+  const int slot = blockIdx.x * W + wave;
+  if (slot >= m.nslots) return;
   const BasisView bv = m.bv;  // descriptor fields in registers, read once
-  const int* __restrict__ slot_batch = m.slot_batch;
-  const int* __restrict__ slot_tile = m.slot_tile;
-  const double *__restrict__ bx = m.bx, *__restrict__ by = m.by, *__restrict__ bz = m.bz;
-  u16* __restrict__ bt = m.bt;
-  u32x4* st = stage[wave];
-#pragma unroll
-  for (int j = 0; j < NC * kMxT / 8 / 64; ++j) st[g + 64 * j] = u32x4{0u, 0u, 0u, 0u};
-  __syncthreads();
-  if (slot < nslots) {
-    const int b = slot_batch[slot], t = slot_tile[slot];
-    const size_t p = (size_t)b * kBatch + g;
-    const float ca = (float)coef[kCoef * p], cx = (float)coef[kCoef * p + 1], cy = (float)coef[kCoef * p + 2],
-                cz = (float)coef[kCoef * p + 3], ct = (float)coef[kCoef * p + 4];
-    MakeB<NC> op;
-    op.sv = reinterpret_cast<u16*>(st) + g;  // [c][hl][nu][g]
-    op.ca = ca; op.cx = cx; op.cy = cy; op.cz = cz; op.ct = ct;
-    walk_tile_f32<true>(bv, t, bx[p], by[p], bz[p], op);
+  const int b = m.slot_batch[slot], t = m.slot_tile[slot];
+  const size_t p = (size_t)b * kBatch + g;
+  const double* __restrict__ coef = m.coef;
+  u16* __restrict__ col0 = m.bt + (size_t)slot * NC * kMxT + g;  // [c][hl][nu][g], this lane's point
+  MakeB<NC> op;
+  op.sv = col0;
+  op.ca = (float)coef[kCoef * p];
+  op.cx = (float)coef[kCoef * p + 1];
+  op.cy = (float)coef[kCoef * p + 2];
+  op.cz = (float)coef[kCoef * p + 3];
+  op.ct = (float)coef[kCoef * p + 4];
+  walk_tile_f32<true>(bv, t, m.bx[p], m.by[p], m.bz[p], op);
+  // zeros past the tile's last AO (segments hold (shell, first component, count, local offset))
+  int nao = 0;
+  for (int k = bv.tile_seg_off[t]; k < bv.tile_seg_off[t + 1]; ++k) {
+    const int4 sg = bv.seg[k];
+    nao = max(nao, sg.w + sg.z);
   }
-  __syncthreads();
-  if (slot < nslots) {
-    u32x4* dst = reinterpret_cast<u32x4*>(bt + (size_t)slot * NC * kMxT);
+  for (int col = nao; col < kTile; ++col)
 #pragma unroll
-    for (int j = 0; j < NC * kMxT / 8 / 64; ++j) dst[g + 64 * j] = st[g + 64 * j];
-  }
+    for (int c = 0; c < NC; ++c) {
+      col0[c * kMxT + col * kBatch] = 0;
+      col0[c * kMxT + kTile * kBatch + col * kBatch] = 0;
+    }
 }
 
 void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one) {
