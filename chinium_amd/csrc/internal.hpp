@@ -225,8 +225,8 @@ void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* 
 void launch_bf16_check(hipStream_t s, unsigned n, unsigned* bad);
 void launch_fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v);
 // One launch over the grid `one` (descriptor by value): density work groups,
-// B^T slot blocks, Vxc work items. Density mode 0: rho, grad rho; 1..3: the tau
-// partial of direction r; 4: the three tau partials in one launch.
+// B^T slot blocks, Vxc work items. Density mode 0: rho, grad rho; 4: the three
+// tau partials in one launch.
 void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one);
 void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one);
 void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one);

@@ -375,7 +375,9 @@ struct DensMxSmem {
 };
 
 // MODE 0: A = phi; epilogue rho and grad rho / 2 (4 components into part).
-// MODE r = 1..3: A = d phi / d x_r; epilogue sum X_r d_r phi (tau partial of direction r).
+// MODE r = 1..3 (A = d phi / d x_r, epilogue sum X_r d_r phi, the tau partial
+// of direction r) is the single-direction form of k_density_mx_tau below,
+// which computes all three in one launch and is the one launched.
 // The grid's descriptor comes by value (a kernel argument: every pointer in it
 // is known to address global memory, and uniform table reads become scalar
 // loads; a descriptor read from device memory would leave them flat).
@@ -639,19 +641,12 @@ __global__ __launch_bounds__(256, 2) void k_density_mx_tau(MolDev m) {
 
 void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one) {
   if (nitems == 0) return;
-  if (mode == 4) {  // the three tau partials in one launch
+  if (mode == 0)
+    hipLaunchKernelGGL((k_density_mx<0>), dim3(nitems), dim3(256), 0, s, one);
+  else if (mode == 4)  // the three tau partials in one launch
     hipLaunchKernelGGL(k_density_mx_tau, dim3(nitems), dim3(256), 0, s, one);
-    CHX_HIP(hipGetLastError());
-    return;
-  }
-#define CHX_DMX(M) hipLaunchKernelGGL((k_density_mx<M>), dim3(nitems), dim3(256), 0, s, one);
-  switch (mode) {
-    case 0: CHX_DMX(0); break;
-    case 1: CHX_DMX(1); break;
-    case 2: CHX_DMX(2); break;
-    default: CHX_DMX(3); break;
-  }
-#undef CHX_DMX
+  else
+    throw Error(CHX_ERR_ARG, "density mode must be 0 (rho, grad rho) or 4 (tau partials)");
   CHX_HIP(hipGetLastError());
 }
 
