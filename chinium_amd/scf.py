@@ -94,3 +94,83 @@ class RestrictedSCF:
             except np.linalg.LinAlgError:
                 F = Fnew
         raise RuntimeError("Convergence failed!")
+
+
+class UnrestrictedSCF:
+    """na alpha and nb beta electrons; xc = None gives UHF (EXX = 1). With xc set (Spin = 2 before
+    Read, LDA/GGA), grid must be a Grid of the molecule, typed and planned.
+
+    Mirrors the unrestricted update (src/HartreeFockKohnSham/Unrestricted/SP.cpp:42-80): orbitals of
+    F_a and F_b, D_s = C_s,occ C_s,occ^T, J from D_a + D_b and K_s = EXX (mu la|nu si) D_s,la,si per
+    spin, the XC sequence grid.getDensity({D_a, D_b}) -> Evaluate("ev") -> getEnergy -> getFock
+    ({V_a, V_b}), E = 1/2 sum_s D_s (Hcore + Fhf_s) + E_xc, and DIIS on the errors
+    S^{-1/2} (F_s D_s S - S D_s F_s) S^{-1/2} of both spins together."""
+
+    def __init__(self, mol, na, nb, xc=None, grid=None):
+        if xc is not None and grid is None:
+            raise ValueError("a Kohn-Sham run needs a grid")
+        if not (0 < int(na) <= mol.nbasis and 0 <= int(nb) <= int(na)):
+            raise ValueError("need 0 <= nb <= na <= nbasis and na >= 1")
+        self.mol, self.na, self.nb, self.xc, self.grid = mol, int(na), int(nb), xc, grid
+        self.S, T, V = mol.int1e()
+        self.Hcore = T + V
+        self.eri = mol.eri()
+        self.enuc = mol.nuclear_repulsion()
+        s, U = np.linalg.eigh(self.S)
+        self.Z = U @ np.diag(s ** -0.5) @ U.T
+        self.exx = 1.0 if xc is None else float(xc.EXX)
+
+    def _density(self, F, nocc):
+        eps, Cp = np.linalg.eigh(self.Z.T @ F @ self.Z)
+        C = self.Z @ Cp
+        return eps, C, C[:, :nocc] @ C[:, :nocc].T
+
+    def update(self, Fa, Fb):
+        epsa, Ca, Da = self._density(Fa, self.na)
+        epsb, Cb, Db = self._density(Fb, self.nb)
+        J = np.einsum("ijkl,kl->ij", self.eri, Da + Db, optimize=True)
+        Ka = self.exx * np.einsum("ikjl,kl->ij", self.eri, Da, optimize=True)
+        Kb = self.exx * np.einsum("ikjl,kl->ij", self.eri, Db, optimize=True)
+        Fhfa, Fhfb = self.Hcore + J - Ka, self.Hcore + J - Kb
+        Exc, Va, Vb = 0.0, 0.0, 0.0
+        if self.xc is not None:
+            self.grid.getDensity([Da, Db])
+            self.xc.Evaluate("ev", self.grid)
+            Exc = self.grid.getEnergy()
+            Va, Vb = self.grid.getFock()
+        Fna, Fnb = Fhfa + Va, Fhfb + Vb
+        E = 0.5 * float(np.sum(Da * (self.Hcore + Fhfa)) + np.sum(Db * (self.Hcore + Fhfb))) + Exc
+        Ga = self.Z @ (Fna @ Da @ self.S - self.S @ Da @ Fna) @ self.Z
+        Gb = self.Z @ (Fnb @ Db @ self.S - self.S @ Db @ Fnb) @ self.Z
+        return Fna, Fnb, E, Ga, Gb, (epsa, epsb), (Da, Db), Exc
+
+    def run(self, max_iter=150, e_tol=1e-10, g_tol=1e-7, ndiis=8):
+        """Returns dict(E total, E_xc, iterations, eps (alpha, beta), D (alpha, beta))."""
+        Fa, Fb = self.Hcore.copy(), self.Hcore.copy()
+        Fs, Gs = [], []
+        E_old = None
+        n = self.mol.nbasis
+        for it in range(1, max_iter + 1):
+            Fna, Fnb, E, Ga, Gb, eps, D, Exc = self.update(Fa, Fb)
+            err = max(np.abs(Ga).max(), np.abs(Gb).max())
+            if E_old is not None and abs(E - E_old) < e_tol and err < g_tol:
+                return dict(E=E + self.enuc, E_elec=E, E_xc=Exc, iterations=it, eps=eps, D=D, diis_error=err)
+            E_old = E
+            Fs.append(np.hstack([Fna, Fnb]))
+            Gs.append(np.hstack([Ga, Gb]))
+            Fs, Gs = Fs[-ndiis:], Gs[-ndiis:]
+            m = len(Fs)
+            B = -np.ones((m + 1, m + 1))
+            B[m, m] = 0.0
+            for a in range(m):
+                for b in range(m):
+                    B[a, b] = np.sum(Gs[a] * Gs[b])
+            rhs = np.zeros(m + 1)
+            rhs[m] = -1.0
+            try:
+                w = np.linalg.solve(B, rhs)[:m]
+                F = sum(wi * Fi for wi, Fi in zip(w, Fs))
+            except np.linalg.LinAlgError:
+                F = np.hstack([Fna, Fnb])
+            Fa, Fb = F[:, :n], F[:, n:]
+        raise RuntimeError("Convergence failed!")

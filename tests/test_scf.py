@@ -114,3 +114,20 @@ def test_eri_limits():
     with pytest.raises(ca.ChxError) as e:
         mol.eri()
     assert e.value.code == -5
+
+
+def test_uhf_h_atom_and_closed_shell():
+    """UHF: one electron has no two-electron energy (J and K cancel), so E is the lowest level of
+    the core Hamiltonian (H / STO-3G: -0.4665819 Eh); for closed-shell water UHF = RHF."""
+    from chinium_amd.scf import UnrestrictedSCF
+    mol = ca.Molecule([1], np.zeros((1, 3)), "sto-3g")
+    S, T, V = mol.int1e()
+    r = UnrestrictedSCF(mol, 1, 0).run()
+    print("H atom UHF %.10f, Hcore %.10f" % (r["E"], (T + V)[0, 0] / S[0, 0]))
+    assert abs(r["E"] - (T + V)[0, 0] / S[0, 0]) <= 1e-12
+    assert abs(r["E"] - (-0.4665819)) <= 1e-6
+    mol = ca.Molecule([8, 1, 1], H2O_XYZ, "sto-3g")
+    ru = UnrestrictedSCF(mol, 5, 5).run()
+    assert abs(ru["E"] - (-74.942079928192)) <= 1e-9
+    with pytest.raises(ValueError):
+        UnrestrictedSCF(mol, 4, 5)
