@@ -25,6 +25,8 @@ ANGSTROM_TO_BOHR = 1.8897259886  # src/Macro/Unit.h:1
 
 CHX_PREC_FP64 = 0
 CHX_PREC_MIXED = 1  # split-BF16 MFMA contractions, FP32 accumulation
+# status codes of the C ABI (chinium_xc.h)
+CHX_ERR_ARG, CHX_ERR_FILE, CHX_ERR_STATE, CHX_ERR_DEVICE, CHX_ERR_UNSUPPORTED = -1, -2, -3, -4, -5
 
 _vp = ctypes.c_void_p
 _lib = None
@@ -278,19 +280,27 @@ class Grid:
     def getAO(self, derivative=0, output=0):
         _check(lib().chx_grid_get_ao(self.h, int(derivative), int(output)))
 
-    def getDensity(self, Ds):
+    def _stack(self, Ds):
+        """One or two nbasis x nbasis matrices, back to back in column-major order (the C ABI's layout).
+        Wrong counts or shapes raise here, before the library copies nspin * n * n doubles."""
         Ds = list(Ds)
-        n = Ds[0].shape[0]
-        buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for D in Ds])
-        _check(lib().chx_grid_get_density(self.h, _ptr(buf), len(Ds), n))
-        self._nspin = len(Ds)
+        if len(Ds) not in (1, 2):
+            raise ChxError(CHX_ERR_ARG, "Wrong number of spin types!")
+        n = self.nbasis
+        for D in Ds:
+            if np.shape(D) != (n, n):
+                raise ChxError(CHX_ERR_ARG, "density matrix size does not match the basis")
+        return np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for D in Ds]), len(Ds)
+
+    def getDensity(self, Ds):
+        buf, ns = self._stack(Ds)
+        _check(lib().chx_grid_get_density(self.h, _ptr(buf), ns, self.nbasis))
+        self._nspin = ns
 
     def getDensitySkeleton(self, Ds):
         """SubGrid::getDensitySkeleton (GridDensity.cpp:79-130), closed shell."""
-        Ds = list(Ds)
-        n = Ds[0].shape[0]
-        buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for D in Ds])
-        _check(lib().chx_grid_get_density_skeleton(self.h, _ptr(buf), len(Ds), n))
+        buf, ns = self._stack(Ds)
+        _check(lib().chx_grid_get_density_skeleton(self.h, _ptr(buf), ns, self.nbasis))
 
     def getEnergyGrad(self):
         """Grid::getEnergyGrad: XC nuclear gradient (natoms x 3), grid held fixed."""
@@ -308,7 +318,11 @@ class Grid:
     def getDensityU(self, Ds):
         """Grid::getDensityU (src/Grid/Grid.cpp:63-78): Ds[spin][mat], closed shell."""
         Ds = [list(x) for x in Ds]
-        n = Ds[0][0].shape[0]
+        n = self.nbasis
+        if len(Ds) not in (1, 2) or any(len(x) != len(Ds[0]) for x in Ds):
+            raise ChxError(CHX_ERR_ARG, "getDensityU takes [spin][mat] lists with equal matrix counts per spin")
+        if any(np.shape(D) != (n, n) for x in Ds for D in x):
+            raise ChxError(CHX_ERR_ARG, "density matrix size does not match the basis")
         buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for x in Ds for D in x])
         self._nmats_u = len(Ds[0])
         _check(lib().chx_grid_get_density_u(self.h, _ptr(buf), len(Ds[0]), len(Ds), n))
@@ -390,6 +404,11 @@ class Grid:
         n = np.zeros(6, dtype=np.int64)
         _check(lib().chx_grid_kernel_times(self.h, int(bool(reset)), _ptr(ms), _ptr(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
+
+    def vxcKernel(self):
+        """Name of the kernel the mixed form runs for the Vxc contraction on this basis size."""
+        npad = -(-self.nbasis // 128) * 128
+        return "k_vxc_mx_wide" if npad % 256 == 0 and npad >= 1024 else "k_vxc_mx"
 
     def stats(self):
         nb, npnt, mt, fl = ctypes.c_long(), ctypes.c_long(), ctypes.c_double(), ctypes.c_double()

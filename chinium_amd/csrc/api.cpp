@@ -106,7 +106,10 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
   }
   if (xc->surrogate && xc->family == 2) {  // learned functional, one partial sum per 64-slot block
     const chx_mlp* m = xc->surrogate;
-    if (g->d_en_part.n < 2 * (size_t)g->nbatches) g->d_en_part.alloc(2 * (size_t)g->nbatches);
+    if (g->d_en_part.n < 2 * (size_t)g->nbatches) {
+      g->d_en_part.alloc(2 * (size_t)g->nbatches);
+      ++g->plan_gen;
+    }
     g->d_nfall.alloc(1);
     launch_fill_u32(g->stream, (unsigned*)g->d_nfall.p, 1, 0u);
     g->mark_begin(2, &e);
@@ -178,8 +181,7 @@ void fock_to_device(chx_grid* g, double* dV) {
     const int nc = g->v_family >= 2 ? 4 : 1;
     require(nc <= g->mx_nT, CHX_ERR_STATE, "a meta-GGA potential needs a grid of type 2 before getAO");
     g->mark_begin(kStMakeB, &e);
-    const int nslot = (int)g->bt_list.size();
-    launch_make_b_mx(g->stream, nc, nc == 4 ? nslot : (nslot + 3) / 4, g->hmol);
+    launch_make_b_mx(g->stream, nc, g->hmol);
     g->mark_end(kStMakeB, e);
     launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
     g->mark_begin(kStVxc, &e);
@@ -195,17 +197,13 @@ void fock_to_device(chx_grid* g, double* dV) {
   }
   if (g->cached) {
     const int nc = g->v_family >= 2 ? 4 : 1;
-    if (g->bmat_comps < nc) {
-      g->d_bmat.alloc(std::max<size_t>(1, g->bt_list.size() * (size_t)nc * kBatch * kTile));
-      g->bmat_comps = nc;
-    }
     g->mark_begin(kStMakeB, &e);
-    launch_make_b(g->stream, nc, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
+    launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
     g->mark_end(kStMakeB, e);
     launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
     g->mark_begin(kStVxc, &e);
     launch_vxc_c64(g->stream, nc, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p, g->ntiles,
-                   g->d_cache.p, g->d_bmat.p, g->d_Mint.p, g->npad);
+                   g->d_cache.p, g->d_bmat.p, g->d_coef.p, g->d_Mint.p, g->npad);
     g->mark_end(kStVxc, e);
     g->mark_begin(kStFinalize, &e);
     launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);
@@ -503,14 +501,10 @@ int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis) {
       launch_response_coef(g->stream, g->nbatches, g->d_rhou.p + (size_t)m * 4 * nslots, g->d_fx.p, g->d_w.p,
                            g->fx_family, coef);
       if (g->cached) {  // B = a phi + c . grad phi from the tables, M = Phi^T B, V = M + M^T
-        if (g->bmat_comps < 1) {
-          g->d_bmat.alloc(std::max<size_t>(1, g->bt_list.size() * (size_t)kBatch * kTile));
-          g->bmat_comps = 1;
-        }
-        launch_make_b(g->stream, 1, g->nbatches, g->d_bt_off.p, coef, g->d_cache.p, g->d_bmat.p);
+        launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, coef, g->d_cache.p, g->d_bmat.p);
         launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
         launch_vxc_c64(g->stream, 1, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p,
-                       g->ntiles, g->d_cache.p, g->d_bmat.p, g->d_Mint.p, g->npad);
+                       g->ntiles, g->d_cache.p, g->d_bmat.p, coef, g->d_Mint.p, g->npad);
         launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, g->d_Vout.p);
       } else {
         launch_fill(g->stream, g->d_Vint.p, (size_t)g->npad * g->npad, 0.0);
@@ -544,7 +538,7 @@ int chx_grid_energy_grad(chx_grid* g, double* grad) {
       g->d_slot_grp.upload(g->slot_grp, g->stream);
       g->d_slot_q.upload(g->slot_q, g->stream);
     }
-    launch_make_b(g->stream, 1, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
+    launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
     DevBuf<double> xz, dg;
     xz.alloc(std::max<size_t>(1, (size_t)ng * 2 * kGroup * kTile * kBatch));
     dg.alloc(3 * (size_t)na);

@@ -382,6 +382,7 @@ void build_cwork(chx_grid* g) {
 void set_vxc_chunk(chx_grid* g, int chunk) {
   g->vxc_chunk = chunk;
   if (!g->ao_ready || !g->cached) return;
+  ++g->plan_gen;
   build_cwork(g);
   hipStream_t st = g->stream;
   g->d_cwork.upload(g->cwork, st);
@@ -484,7 +485,7 @@ void grid_plan(chx_grid* g) {
   const int nT = g->type >= 2 ? 4 : 1;
   const double cache_bytes =
       mixed ? (double)nslot * 3.0 * nT * kMxComp * sizeof(unsigned short)
-            : (double)nslot * (4.0 + (g->type >= 2 ? 4.0 : 1.0)) * kBatch * kTile * sizeof(double);
+            : (double)nslot * (4.0 + 1.0) * kBatch * kTile * sizeof(double);  // phi, grad phi and B
   if (mixed && g->ntiles > kMxMaxTiles)
     throw Error(CHX_ERR_UNSUPPORTED, "the mixed-precision form takes at most " + std::to_string(kMxMaxTiles * kTile) +
                                          " basis functions");
@@ -513,7 +514,6 @@ void grid_plan(chx_grid* g) {
     if (mixed) {
       g->d_cache.release();
       g->d_bmat.release();
-      g->bmat_comps = 0;
       g->mx_nT = nT;
       g->mx_sstride = 2 * nT * kMxComp;  // AOg then AOt, nT components each
       // one extra all-zero slot at index nslot stands in for tiles that are not alive
@@ -521,9 +521,9 @@ void grid_plan(chx_grid* g) {
       CHX_HIP(hipMemsetAsync(g->d_cmx.p + nslot * (size_t)g->mx_sstride, 0,
                              sizeof(unsigned short) * g->mx_sstride, st));
       g->bmx_comps = nT;
+      // B^T of every slot, plus the all-zero slot that make_b rewrites for the
+      // component count of each build's potential
       g->d_bmx.alloc((nslot + 1) * (size_t)nT * kMxComp);
-      CHX_HIP(hipMemsetAsync(g->d_bmx.p + nslot * (size_t)nT * kMxComp, 0,
-                             sizeof(unsigned short) * nT * kMxComp, st));
       g->d_cslot.upload(g->cslot, st);
       g->d_Dsplit.alloc(2 * (size_t)g->npad * g->npad);
       launch_ao_cache_mx(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p,
@@ -534,8 +534,7 @@ void grid_plan(chx_grid* g) {
       g->d_Dsplit.release();
       g->mx_nT = g->mx_sstride = g->bmx_comps = 0;
       g->d_cache.alloc(std::max<size_t>(1, nslot * (size_t)4 * kBatch * kTile));
-      g->bmat_comps = g->type >= 2 ? 4 : 1;
-      g->d_bmat.alloc(std::max<size_t>(1, nslot * (size_t)g->bmat_comps * kBatch * kTile));
+      g->d_bmat.alloc(std::max<size_t>(1, nslot * (size_t)kBatch * kTile));
       launch_ao_cache(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p,
                       g->d_z.p, g->view(), g->d_cache.p);
     }
@@ -582,6 +581,7 @@ void grid_plan(chx_grid* g) {
   }
   CHX_HIP(hipStreamSynchronize(st));
   g->ao_ready = true;
+  ++g->plan_gen;
   g->have_density = g->have_eval = g->have_v = g->have_skeleton = false;
   g->slot_grp.clear();
   g->slot_q.clear();

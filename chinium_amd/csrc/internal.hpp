@@ -178,10 +178,12 @@ void launch_point_density(hipStream_t s, int nbatches, const int* grp_off, const
 void launch_density_c64(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
                         const int* grp_slots, const int* grp_ntiles, const int* bt_off, const int* bt_list,
                         const double* cache, const double* Dint, int npad, double* part);
-void launch_make_b(hipStream_t s, int ncomp, int nbatches, const int* bt_off, const double* coef, const double* cache,
+// B = a phi + c . grad phi per slot (one component for every family)
+void launch_make_b(hipStream_t s, int nbatches, const int* bt_off, const double* coef, const double* cache,
                    double* bmat);
+// ncomp 4 adds the tau term, its operand t d_r phi formed from the cache and coef[4] as it is loaded
 void launch_vxc_c64(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
-                    int ntiles, const double* cache, const double* bmat, double* Mint, int npad);
+                    int ntiles, const double* cache, const double* bmat, const double* coef, double* Mint, int npad);
 void launch_finalize_sym(hipStream_t s, int n, int npad, const double* Mint, const int* ref2int, double* V);
 // Device descriptor of one grid for the mixed-precision kernels. A kernel
 // launch covers a list of (grid, item) pairs, so one launch can serve many
@@ -228,7 +230,8 @@ void launch_fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v);
 // B^T slot blocks, Vxc work items. Density mode 0: rho, grad rho; 4: the three
 // tau partials in one launch.
 void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one);
-void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one);
+// B^T of every slot plus the all-zero slot (index nslots) for ncomp components
+void launch_make_b_mx(hipStream_t s, int ncomp, const MolDev& one);
 void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one);
 // 256-AO super-tiles (npad a multiple of 256): cslot holds 16 slots per batch
 void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one);
@@ -295,6 +298,9 @@ struct chx_grid {
 
   // Screening products (built by get_ao).
   bool ao_ready = false;
+  // Bumped whenever the plan, the work lists or a buffer a captured pack graph
+  // addresses may change (grid_plan, set_vxc_chunk, reallocations).
+  unsigned long plan_gen = 0;
   std::vector<unsigned char> alive;  // nbatches x ntiles
   std::vector<int> bt_off, bt_list;  // alive tiles per batch (CSR)
   std::vector<int> grp_batch, grp_tiles, grp_ntiles, grp_off;  // density work groups
@@ -336,7 +342,6 @@ struct chx_grid {
 
   // Per-iteration state.
   chx::DevBuf<double> d_Din, d_Dint, d_part, d_part_tau, d_rho, d_coef, d_en_part, d_en, d_Vint, d_Vout;
-  int bmat_comps = 0;  // components per slot held in d_bmat (1 GGA, 4 meta-GGA)
   int v_family = 0;    // family of the potential on the grid (set by Evaluate / set_vrho)
   int en_blocks = 0;
   // XC gradient (Grid::getDensitySkeleton / getEnergyGrad)

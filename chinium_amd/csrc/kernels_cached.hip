@@ -219,13 +219,14 @@ void launch_density_c64(hipStream_t s, int mode, int ngroups, const int* grp_bat
 }
 
 // ---------------------------------------------------------------------------
-// B = a phi + c . grad phi for every slot of a batch (one block per batch).
-// Meta-GGA (NC = 4) adds T_r = t d_r phi, t = w v_tau / 4, so that
-// V = M + M^T carries (1/2) w v_tau grad phi_mu . grad phi_nu.
-template <int NC>
+// B = a phi + c . grad phi for every slot of a batch (one block per batch),
+// one component per slot for every family: the meta-GGA operand
+// T_r = t d_r phi (t = w v_tau / 4, so that V = M + M^T carries
+// (1/2) w v_tau grad phi_mu . grad phi_nu) is formed from the cached d_r phi
+// while the Vxc kernel loads it, instead of being stored.
 __global__ __launch_bounds__(256) void k_make_b(const int* __restrict__ bt_off, const double* __restrict__ coef,
                                                 const double* __restrict__ cache, double* __restrict__ bmat) {
-  __shared__ double ca[kBatch], cx[kBatch], cy[kBatch], cz[kBatch], ct[kBatch];
+  __shared__ double ca[kBatch], cx[kBatch], cy[kBatch], cz[kBatch];
   const int b = blockIdx.x, tid = threadIdx.x;
   if (tid < kBatch) {
     const size_t slot = (size_t)b * kBatch + tid;
@@ -233,32 +234,23 @@ __global__ __launch_bounds__(256) void k_make_b(const int* __restrict__ bt_off, 
     cx[tid] = coef[kCoef * slot + 1];
     cy[tid] = coef[kCoef * slot + 2];
     cz[tid] = coef[kCoef * slot + 3];
-    ct[tid] = coef[kCoef * slot + 4];
   }
   __syncthreads();
   for (int it = bt_off[b]; it < bt_off[b + 1]; ++it) {
     const double* src = cache + (size_t)it * kSlotAO;
-    double* dst = bmat + (size_t)it * NC * kSlotB;
+    double* dst = bmat + (size_t)it * kSlotB;
     for (int e = tid; e < kSlotB; e += 256) {
       const int g = e / kTile;
       const double dx = src[kSlotB + e], dy = src[2 * kSlotB + e], dz = src[3 * kSlotB + e];
       dst[e] = ca[g] * src[e] + cx[g] * dx + cy[g] * dy + cz[g] * dz;
-      if (NC == 4) {
-        dst[kSlotB + e] = ct[g] * dx;
-        dst[2 * kSlotB + e] = ct[g] * dy;
-        dst[3 * kSlotB + e] = ct[g] * dz;
-      }
     }
   }
 }
 
-void launch_make_b(hipStream_t s, int ncomp, int nbatches, const int* bt_off, const double* coef, const double* cache,
+void launch_make_b(hipStream_t s, int nbatches, const int* bt_off, const double* coef, const double* cache,
                    double* bmat) {
   if (nbatches == 0) return;
-  if (ncomp == 4)
-    hipLaunchKernelGGL(k_make_b<4>, dim3(nbatches), dim3(256), 0, s, bt_off, coef, cache, bmat);
-  else
-    hipLaunchKernelGGL(k_make_b<1>, dim3(nbatches), dim3(256), 0, s, bt_off, coef, cache, bmat);
+  hipLaunchKernelGGL(k_make_b, dim3(nbatches), dim3(256), 0, s, bt_off, coef, cache, bmat);
   CHX_HIP(hipGetLastError());
 }
 
@@ -272,12 +264,14 @@ struct VxcCSmem {
 };
 
 // NC = 1: M = phi^T B. NC = 4 (meta-GGA): also sum_r (d_r phi)^T T_r, each
-// component streamed as further K stages.
+// component streamed as further K stages, T_r = t d_r phi formed from the
+// cached d_r phi and the point's t = coef[4] as it is loaded.
 template <int NC>
 __global__ __launch_bounds__(512) void k_vxc_c64(const int4* __restrict__ work, const int* __restrict__ pair_batches,
                                                  const int* __restrict__ slot_of, int ntiles,
                                                  const double* __restrict__ cache, const double* __restrict__ bmat,
-                                                 double* __restrict__ Mint, int npad) {
+                                                 const double* __restrict__ coef, double* __restrict__ Mint,
+                                                 int npad) {
   __shared__ VxcCSmem sm;
   const int4 wk = work[blockIdx.x];
   const int I = wk.x, J = wk.y;
@@ -292,14 +286,22 @@ __global__ __launch_bounds__(512) void k_vxc_c64(const int4* __restrict__ work, 
     const int it = wk.z + k / (2 * NC), h = (k / NC) % 2, c = k % NC;
     const int b = pair_batches[it];
     const int g = h * kHalf + r0 / kTile, mu = r0 % kTile;
+    const double t = c ? coef[kCoef * ((size_t)b * kBatch + g) + 4] : 0.0;
 #pragma unroll
     for (int j = 0; j < kSupC; ++j) {
       const int si = slot_of[(size_t)b * ntiles + I * kSupC + j];
       const int sj = slot_of[(size_t)b * ntiles + J * kSupC + j];
       rp[j] = si >= 0 ? *reinterpret_cast<const double2*>(cache + (size_t)si * kSlotAO + (size_t)c * kSlotB + g * kTile + mu)
                       : double2{0.0, 0.0};
-      rq[j] = sj >= 0 ? *reinterpret_cast<const double2*>(bmat + ((size_t)sj * NC + c) * kSlotB + g * kTile + mu)
-                      : double2{0.0, 0.0};
+      if (sj < 0) {
+        rq[j] = double2{0.0, 0.0};
+      } else if (c == 0) {
+        rq[j] = *reinterpret_cast<const double2*>(bmat + (size_t)sj * kSlotB + g * kTile + mu);
+      } else {
+        const double2 d = *reinterpret_cast<const double2*>(cache + (size_t)sj * kSlotAO + (size_t)c * kSlotB +
+                                                             g * kTile + mu);
+        rq[j] = double2{t * d.x, t * d.y};
+      }
     }
   };
   auto store = [&](int buf) {
@@ -343,14 +345,14 @@ __global__ __launch_bounds__(512) void k_vxc_c64(const int4* __restrict__ work, 
 }
 
 void launch_vxc_c64(hipStream_t s, int ncomp, int nwork, const int4* work, const int* pair_batches, const int* slot_of,
-                    int ntiles, const double* cache, const double* bmat, double* Mint, int npad) {
+                    int ntiles, const double* cache, const double* bmat, const double* coef, double* Mint, int npad) {
   if (nwork == 0) return;
   if (ncomp == 4)
     hipLaunchKernelGGL(k_vxc_c64<4>, dim3(nwork), dim3(512), 0, s, work, pair_batches, slot_of, ntiles, cache, bmat,
-                       Mint, npad);
+                       coef, Mint, npad);
   else
     hipLaunchKernelGGL(k_vxc_c64<1>, dim3(nwork), dim3(512), 0, s, work, pair_batches, slot_of, ntiles, cache, bmat,
-                       Mint, npad);
+                       coef, Mint, npad);
   CHX_HIP(hipGetLastError());
 }
 

@@ -687,17 +687,30 @@ struct MakeB {
   __device__ void end() {}
 };
 
+// The wave of slot index nslots clears the all-zero slot that the Vxc items
+// read for tiles that are not alive in a batch: it sits at nslots * NC
+// components, so its place moves with the potential's component count and is
+// rewritten on every build.
 template <int NC>
 __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev m) {
   constexpr int W = NC == 4 ? 1 : 4;  // waves per block
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
   const int slot = blockIdx.x * W + wave;
-  if (slot >= m.nslots) return;
+  if (slot > m.nslots) return;
+  u16* __restrict__ col0 = m.bt + (size_t)slot * NC * kMxT + g;  // [c][hl][nu][g], this lane's point
+  if (slot == m.nslots) {
+    for (int col = 0; col < kTile; ++col)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        col0[c * kMxT + col * kBatch] = 0;
+        col0[c * kMxT + kTile * kBatch + col * kBatch] = 0;
+      }
+    return;
+  }
   const BasisView bv = m.bv;  // descriptor fields in registers, read once
   const int b = m.slot_batch[slot], t = m.slot_tile[slot];
   const size_t p = (size_t)b * kBatch + g;
   const double* __restrict__ coef = m.coef;
-  u16* __restrict__ col0 = m.bt + (size_t)slot * NC * kMxT + g;  // [c][hl][nu][g], this lane's point
   MakeB<NC> op;
   op.sv = col0;
   op.ca = (float)coef[kCoef * p];
@@ -720,12 +733,12 @@ __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev m) {
     }
 }
 
-void launch_make_b_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one) {
-  if (nitems == 0) return;
+void launch_make_b_mx(hipStream_t s, int ncomp, const MolDev& one) {
+  const int waves = one.nslots + 1;  // every slot and the all-zero slot
   if (ncomp == 4)
-    hipLaunchKernelGGL((k_make_b_mx<4>), dim3(nitems), dim3(64), 0, s, one);
+    hipLaunchKernelGGL((k_make_b_mx<4>), dim3(waves), dim3(64), 0, s, one);
   else
-    hipLaunchKernelGGL((k_make_b_mx<1>), dim3(nitems), dim3(256), 0, s, one);
+    hipLaunchKernelGGL((k_make_b_mx<1>), dim3((waves + 3) / 4), dim3(256), 0, s, one);
   CHX_HIP(hipGetLastError());
 }
 

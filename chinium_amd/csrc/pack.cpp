@@ -42,6 +42,7 @@ struct chx_pack {
   hipGraphExec_t exec = nullptr;
   std::vector<const double*> gD;  // pointers the graph was captured with
   std::vector<double*> gV;
+  std::vector<unsigned long> gen;  // each grid's plan generation when the lanes and graph were built
   std::vector<double> h_en;
   int lanes = 16;
   std::vector<int> cut;  // lane k holds molecules [cut[k], cut[k + 1])
@@ -108,6 +109,8 @@ void build_lanes(chx_pack* p) {
   }
   p->drop_graph();
   p->free_lanes();
+  p->gen.resize(n);
+  for (int m = 0; m < n; ++m) p->gen[m] = p->grids[m]->plan_gen;
   if (L > 1) {
     p->lane_st.resize(L);
     p->ev_dens.resize(L);
@@ -145,8 +148,7 @@ void enqueue_lane(chx_pack* p, int m0, int m1, hipStream_t s, hipEvent_t after, 
     launch_functional(s, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, p->d_codes.p,
                       (int)p->codes.size(), p->family, 1, 1, g->d_rho.p, g->d_coef.p, g->d_en_part.p, g->en_blocks);
     launch_reduce_en(s, g->en_blocks, g->d_en_part.p, p->d_en.p + 2 * m);
-    const int nslot = (int)g->bt_list.size();
-    launch_make_b_mx(s, nc, nc == 4 ? nslot : (nslot + 3) / 4, g->hmol);
+    launch_make_b_mx(s, nc, g->hmol);
     if (g->mx_wide)
       launch_vxc_mx_wide(s, nc, (int)g->cwork.size(), g->hmol);
     else
@@ -184,6 +186,17 @@ void run(chx_pack* p, const chx_xc* xc, const double* const* D, double* const* V
     CHX_HIP(hipStreamSynchronize(p->stream));
     p->drop_graph();
   }
+  // A member grid that was re-planned, re-tuned or had a buffer reallocated on
+  // its own since the lanes were cut: its work counts and device pointers in
+  // any captured graph are stale, so cut the lanes again (which drops the graph).
+  for (int m = 0; m < n; ++m)
+    if (p->grids[m]->plan_gen != p->gen[m]) {
+      prequire(p->grids[m]->ao_ready && p->grids[m]->precision == CHX_PREC_MIXED &&
+                   p->grids[m]->type == p->type,
+               CHX_ERR_STATE, "a packed grid was changed to another form or type; create a new pack");
+      build_lanes(p);
+      break;
+    }
   const std::vector<const double*> vd(D, D + n);
   const std::vector<double*> vv(V, V + n);
   if (p->use_graph) {

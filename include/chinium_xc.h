@@ -219,6 +219,10 @@ int chx_xc_eval_points_f(const chx_xc* xc, long n, const double* rho, const doub
  * chx_pack_set_lanes); the sequence is captured as a hipGraph and replayed
  * while the D / V pointers and the functional stay the same. The grids must
  * outlive the pack and must not be used on their own while a pack call runs.
+ * A member grid that is re-planned (chx_grid_get_ao), re-tuned or whose
+ * buffers were reallocated between pack calls is picked up at the next pack
+ * call: the lanes are cut again and the graph is recaptured. A member grid
+ * switched to another form or type makes the pack call fail (CHX_ERR_STATE).
  * D_dev[m] / V_dev[m]: device arrays, column-major nbasis_m^2, reference
  * order. E, N (host, may be NULL): E_xc and N_e per molecule. */
 typedef struct chx_pack chx_pack;

@@ -115,6 +115,8 @@ inline int hidx(int r, int t) {  // (r, t) -> position in (xx, xy, xz, yy, yz, z
 struct SubGrid {
   int n;      // unpadded
   int npad;   // padded to a multiple of 8 (GridGeneration.cpp:214)
+  long first = 0;         // first point in generation order
+  bool weighted = false;  // w holds the Becke-partitioned weights (computed on first use)
   std::vector<double> x, y, z, w;
   // AO tables, [mu][g] (value), [r][mu][g] (gradient), [mu][g] (Laplacian)
   std::vector<double> ao, ao1, ao2l;
@@ -133,6 +135,7 @@ struct ox_system {
   std::vector<std::vector<Poly>> shell_ang;
   std::vector<double> points;   // 4*G generation order
   std::vector<double> raw_w;    // radial x angular weight before the Becke partition
+  std::vector<long> atom_start; // first point of each atom's grid (natoms + 1)
   std::vector<SubGrid> subgrids;
   std::vector<int> active;
   std::vector<int> codes;
@@ -552,6 +555,52 @@ extern "C" {
 
 const char* ox_last_error(void) { return g_err.c_str(); }
 
+// Plain Becke partition (GridGeneration.cpp:136-153): w = w_rad w_ang P_A / sum_B P_B
+// for the points of the given sub-grids that are not weighted yet, written
+// into the sub-grid and into the point list. Done lazily so that a timing
+// sample of a few sub-grids of a large system does not pay for all points.
+static void weigh_subgrids(ox_system* s, const std::vector<int>& ids) {
+  std::vector<long> pts;
+  for (int i : ids) {
+    const SubGrid& sg = s->subgrids[i];
+    if (sg.weighted) continue;
+    for (int g = 0; g < sg.n; ++g) pts.push_back(sg.first + g);
+  }
+  const int nat = s->natoms;
+  const double* X = s->xyz.data();
+  const long np = (long)pts.size();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(s->nthreads)
+  for (long k = 0; k < np; ++k) {
+    const long p = pts[k];
+    int owner = 0;
+    while (p >= s->atom_start[owner + 1]) ++owner;
+    double* q = &s->points[4 * p];
+    double total = 0, mine = 0;
+    for (int b = 0; b < nat; ++b) {
+      double pb = 1;
+      for (int c = 0; c < nat; ++c) {
+        if (b == c) continue;
+        pb *= becke_s(q[0], q[1], q[2], X + 3 * b, X + 3 * c);
+      }
+      total += pb;
+      if (b == owner) mine = pb;
+    }
+    q[3] = s->raw_w[p] * (mine / total);
+  }
+  for (int i : ids) {
+    SubGrid& sg = s->subgrids[i];
+    if (sg.weighted) continue;
+    for (int g = 0; g < sg.n; ++g) sg.w[g] = s->points[4 * (sg.first + g) + 3];
+    sg.weighted = true;
+  }
+}
+
+static void weigh_all(ox_system* s) {
+  std::vector<int> ids(s->subgrids.size());
+  for (size_t i = 0; i < ids.size(); ++i) ids[i] = (int)i;
+  weigh_subgrids(s, ids);
+}
+
 ox_system* ox_create(const char* data_root, const char* grid_name, const char* basis_name,
                      int natoms, const int* Z, const double* xyz_bohr, int nthreads) {
   ox_system* s = new ox_system;
@@ -611,26 +660,9 @@ ox_system* ox_create(const char* data_root, const char* grid_name, const char* b
     }
     s->raw_w.resize(G);
     for (long p = 0; p < G; ++p) s->raw_w[p] = s->points[4 * p + 3];
-    const int nat = natoms;
-    const double* X = s->xyz.data();
-#pragma omp parallel for schedule(dynamic, 1024) num_threads(s->nthreads)
-    for (long p = 0; p < G; ++p) {
-      int owner = 0;
-      while (p >= start[owner + 1]) ++owner;
-      double* q = &s->points[4 * p];
-      double total = 0, mine = 0;
-      for (int b = 0; b < nat; ++b) {
-        double pb = 1;
-        for (int c = 0; c < nat; ++c) {
-          if (b == c) continue;
-          pb *= becke_s(q[0], q[1], q[2], X + 3 * b, X + 3 * c);
-        }
-        total += pb;
-        if (b == owner) mine = pb;
-      }
-      q[3] *= mine / total;
-    }
-    // sub-grids
+    s->atom_start = start;
+    // sub-grids (the Becke partition of their weights is applied on first use,
+    // see weigh_subgrids; membership depends only on the generation order)
     std::vector<std::pair<long, long>> parts;
     recursive_halving(0, G, parts);
     s->subgrids.resize(parts.size());
@@ -638,11 +670,11 @@ ox_system* ox_create(const char* data_root, const char* grid_name, const char* b
       SubGrid& sg = s->subgrids[i];
       sg.n = (int)parts[i].second;
       sg.npad = (sg.n + 7) & ~7;
+      sg.first = parts[i].first;
       sg.x.resize(sg.npad); sg.y.resize(sg.npad); sg.z.resize(sg.npad); sg.w.assign(sg.npad, 0);
       for (int g = 0; g < sg.npad; ++g) {
         const long src = parts[i].first + (g < sg.n ? g : g - sg.n);
         sg.x[g] = s->points[4 * src]; sg.y[g] = s->points[4 * src + 1]; sg.z[g] = s->points[4 * src + 2];
-        if (g < sg.n) sg.w[g] = s->points[4 * src + 3];
       }
     }
     s->active.clear();
@@ -663,7 +695,8 @@ int ox_get_raw_weights(const ox_system* s, double* w) {
   return 0;
 }
 
-int ox_get_points(const ox_system* s, double* xyzw) {
+int ox_get_points(ox_system* s, double* xyzw) {
+  weigh_all(s);
   std::memcpy(xyzw, s->points.data(), s->points.size() * sizeof(double));
   return 0;
 }
@@ -749,6 +782,7 @@ int ox_get_ao(ox_system* s) {
     for (int i : s->active) act[i] = 1;
     for (size_t i = 0; i < s->subgrids.size(); ++i)
       if (!act[i]) { s->subgrids[i].ao.clear(); s->subgrids[i].ao.shrink_to_fit(); s->subgrids[i].ao1.clear(); s->subgrids[i].ao1.shrink_to_fit(); s->subgrids[i].ao2l.clear(); }
+    weigh_subgrids(s, s->active);
     const int n = (int)s->active.size();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(s->nthreads)
     for (int i = 0; i < n; ++i) compute_ao(s, s->subgrids[s->active[i]], s->family);
@@ -1127,6 +1161,19 @@ long ox_active_points(const ox_system* s) {
   long n = 0;
   for (int i : s->active) n += s->subgrids[i].n;
   return n;
+}
+
+int ox_get_active_xyz(const ox_system* s, double* xyz) {
+  long k = 0;
+  for (int i : s->active) {
+    const SubGrid& sg = s->subgrids[i];
+    for (int g = 0; g < sg.n; ++g, ++k) {
+      xyz[3 * k] = sg.x[g];
+      xyz[3 * k + 1] = sg.y[g];
+      xyz[3 * k + 2] = sg.z[g];
+    }
+  }
+  return 0;
 }
 
 int ox_get_density(const ox_system* s, double* rho, double* grad, double* sigma, double* tau, double* w) {

@@ -39,7 +39,7 @@ int ox_nbasis(const ox_system* s);
 long ox_npoints(const ox_system* s);      /* unpadded points, generation order */
 int ox_nsubgrids(const ox_system* s);
 /* Points in generation order: xyzw[4*i + {0,1,2,3}]. */
-int ox_get_points(const ox_system* s, double* xyzw);
+int ox_get_points(ox_system* s, double* xyzw);
 /* Radial x angular weights before the Becke partition, generation order. */
 int ox_get_raw_weights(const ox_system* s, double* w);
 /* Sub-grid sizes (unpadded) in order. */
@@ -102,6 +102,8 @@ int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx);
  * sub-grid order with padding removed: rho, grad (3 per point), sigma, tau. */
 long ox_active_points(const ox_system* s);
 int ox_get_density(const ox_system* s, double* rho, double* grad, double* sigma, double* tau, double* w);
+/* Coordinates of the active sub-grids' unpadded points, in the same order. */
+int ox_get_active_xyz(const ox_system* s, double* xyz);
 
 /* Spin-polarised XC build, LDA/GGA: getDensity({Da, Db}) -> Evaluate("ev")
  * (polarised functional) -> getEnergy -> getFock, Va / Vb per spin. */

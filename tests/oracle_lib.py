@@ -54,6 +54,7 @@ def lib():
         "ox_ao_values": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp]),
         "ox_active_points": (ctypes.c_long, [_vp]),
         "ox_get_density": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "ox_get_active_xyz": (ctypes.c_int, [_vp, _vp]),
         "ox_eval_functional_f": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long] + [_vp] * 6),
         "ox_fock_u": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
         "ox_xc_pol": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -190,6 +191,16 @@ class OracleSystem:
         rho = np.zeros(n); grad = np.zeros((n, 3)); sig = np.zeros(n); tau = np.zeros(n); w = np.zeros(n)
         L.ox_get_density(self.h, _ptr(rho), _ptr(grad), _ptr(sig), _ptr(tau), _ptr(w))
         return rho, grad, sig, tau, w
+
+    def density_at(self, D):
+        """Density of D on the active sub-grids (after get_ao): xyz, rho, sigma, tau, w per unpadded point."""
+        n = self.nbasis
+        Df = np.asfortranarray(D, dtype=np.float64)
+        _check(lib().ox_density(self.h, _ptr(Df)))
+        rho, grad, sig, tau, w = self.density()
+        xyz = np.zeros((len(rho), 3))
+        _check(lib().ox_get_active_xyz(self.h, _ptr(xyz)))
+        return xyz, rho, sig, tau, w
 
     def ao_values(self, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)

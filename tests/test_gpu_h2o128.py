@@ -1,0 +1,108 @@
+"""The full-size cluster: (H2O)128 / def2-SVP / SG-2 / r2SCAN (3072 AOs, 2.9 M
+points, 384 atoms). The mixed form against the plain FP64 form on the same grid
+and D, and both forms' point densities (rho, sigma, tau; GridDensity.cpp:12-41)
+against the FP64 oracle's on a seeded sample of its 128-point sub-grids, points
+matched by coordinate. Bars: the FP64 form's point density to 1e-10 relative
+where rho > 1e-3 (parity), the mixed form's to 2^-12 (its split-BF16 operands
+carry 16 bits, so a dropped term, a wrong sign or a wrong index fails by orders
+of magnitude), and rho, sigma, tau everywhere to 1e-12 (FP64) or 2^-14 (mixed)
+of their largest value; |dE_xc| <= 1e-5 Eh/atom, max |dV| <= 5e-5, V
+symmetric, N_e = 1280 to the grid's integration error."""
+import numpy as np
+import pytest
+
+import chinium_amd as ca
+import oracle_lib as ol
+from chinium_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+SAMPLE = 12  # oracle sub-grids (all 3072 AOs each)
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def h2o128():
+    w = wl.build_workload("h2o128_def2svp_sg2_r2scan")
+    mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
+    assert mol.nbasis == 3072
+    D, S = wl.workload_density(mol, w["nocc"], w["seed"])
+    xc = ca.ExchangeCorrelation()
+    xc.Read(w["functional"])
+    out = {}
+    for prec in (ca.CHX_PREC_FP64, ca.CHX_PREC_MIXED):
+        name = "mixed" if prec else "fp64"
+        _say("  (H2O)128 r2SCAN %s form: grid" % name)
+        grid = ca.Grid(mol, w["grid"], 1, 0)
+        grid.setPrecision(prec)
+        grid.setType(xc.family)
+        grid.getAO(0, 0)
+        E, V = ca.xc_build(grid, xc, D)
+        rho, sig, tau, wt = grid.pointDensity()
+        out[name] = dict(E=E, V=V, N=grid.getNumElectrons(), rho=rho, sig=sig, tau=tau, w=wt,
+                         cached=grid.aoCached())
+        if prec == ca.CHX_PREC_FP64:
+            out["xyzw"] = grid.points()
+        _say("  %s: E_xc %.10f N_e %.8f (AO tables %s)" % (name, E, out[name]["N"],
+                                                          "cached" if out[name]["cached"] else "recomputed"))
+        del grid
+    # oracle density on a seeded sample of sub-grids
+    orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], w["grid"], w["basis"])
+    orc.set_functional(w["functional"])
+    rng = wl.SplitMix64(77)
+    keys = sorted((rng.next_u64(), i) for i in range(orc.nsubgrids))
+    orc.set_subset(sorted(i for _, i in keys[:SAMPLE]))
+    orc.get_ao()
+    out["oracle"] = orc.density_at(D)
+    out["natoms"] = len(w["Z"])
+    return out
+
+
+def _match(xyzw, w_slot, oxyz):
+    """Index into the GPU's slot arrays of each oracle point: both grids come from the same radial and
+    Lebedev formulas, so a point's coordinates agree to rounding (nearest neighbour within 1e-9 bohr)."""
+    from scipy.spatial import cKDTree
+    real = np.nonzero(w_slot)[0]  # slots with nonzero weight, in point order
+    assert len(real) == len(xyzw)
+    d, k = cKDTree(xyzw[:, :3]).query(oxyz)
+    return real, np.where(d <= 1e-9, k, -1)
+
+
+def test_h2o128_mixed_vs_fp64(h2o128):
+    a, b = h2o128["fp64"], h2o128["mixed"]
+    na = h2o128["natoms"]
+    dE, dV = abs(b["E"] - a["E"]) / na, np.abs(b["V"] - a["V"]).max()
+    _say("  |dE_xc|/atom %.2e  max|dV| %.2e  N_e %.6f / %.6f" % (dE, dV, a["N"], b["N"]))
+    assert abs(a["N"] - 1280.0) <= 1e-2
+    assert abs(b["N"] - a["N"]) <= 1e-6 * a["N"]
+    assert dE <= 1e-5
+    assert dV <= 5e-5
+    assert np.abs(b["V"] - b["V"].T).max() <= 1e-12 * np.abs(b["V"]).max()
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_h2o128_point_density_vs_oracle(h2o128, form):
+    oxyz, orho, osig, otau, ow = h2o128["oracle"]
+    r = h2o128[form]
+    real, idx = _match(h2o128["xyzw"], r["w"], oxyz)
+    keep = ow != 0  # points whose Becke weight is exactly 0 are dropped by the GPU grid
+    assert np.all(idx[keep] >= 0), "oracle points missing from the GPU grid"
+    sl = real[idx[keep]]
+    assert np.allclose(r["w"][sl], ow[keep], rtol=1e-12, atol=0)
+    # each quantity against its largest value over the sample (sigma vanishes at critical points and every
+    # quantity at the grid's outer shells, where a relative test says nothing), and rho point by point where
+    # it is appreciable
+    tol = 1e-12 if form == "fp64" else 2.0 ** -14
+    worst = {}
+    for name, g, o in (("rho", r["rho"][sl], orho[keep]), ("sigma", r["sig"][sl], osig[keep]),
+                       ("tau", r["tau"][sl], otau[keep])):
+        worst[name] = float(np.abs(g - o).max() / np.abs(o).max())
+        assert worst[name] <= tol, (name, worst[name])
+    big = orho[keep] > 1e-3
+    rel = float((np.abs(r["rho"][sl] - orho[keep])[big] / orho[keep][big]).max())
+    _say("  %s point density vs oracle (%d points, %d with rho > 1e-3): max err / max %s; rho rel %.2e"
+         % (form, int(keep.sum()), int(big.sum()), worst, rel))
+    assert rel <= (1e-10 if form == "fp64" else 2.0 ** -12)

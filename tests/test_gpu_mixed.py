@@ -118,3 +118,45 @@ def test_precision_switch_needs_getao():
     assert 0 < abs(Emx - E64) <= 3 * E_TOL_PER_ATOM
     with pytest.raises(ca.ChxError):
         grid.setPrecision(7)
+
+
+def test_mixed_meta_grid_after_gga_potentials():
+    """A meta-GGA grid (type 2) in the mixed form that serves a meta-GGA build, then a potential with one
+    component (SAP-style v_rho, GuessSCF.cpp:37-60), then a GGA build, and the meta-GGA build again. A Vxc
+    item reads the all-zero B^T slot for tiles that are not alive in a batch; it sits at the end of each
+    build's B^T block, whose size follows the potential's component count. (H2O)4 / def2-SVP has three
+    AO tiles, so dead tiles occur at the grid's outer points."""
+    Z, X = wl.water_cluster_angstrom(4, 4.0, 11)
+    xyz = X * ca.ANGSTROM_TO_BOHR
+    mol = ca.Molecule(Z, xyz, "def2-svp")
+    D, S = wl.workload_density(mol, 20, 3)
+    grid = _mixed_grid(mol, "SG-1", 2)
+    oracles = {}
+    for func in ("R2SCAN", "PBE"):
+        o = ol.OracleSystem(Z, xyz, "SG-1", "def2-svp")
+        o.set_functional(func)
+        o.get_ao()
+        oracles[func] = o.xc(D)
+    na = len(Z)
+
+    def check(func):
+        xc = ca.ExchangeCorrelation()
+        xc.Read(func)
+        E, V = ca.xc_build(grid, xc, D)
+        Eo, No, Vo = oracles[func]
+        _say("  type-2 grid, %s: dE/atom %.2e max dV %.2e" % (func, abs(E - Eo) / na, np.abs(V - Vo).max()))
+        assert abs(E - Eo) <= E_TOL_PER_ATOM * na
+        assert np.abs(V - Vo).max() <= V_TOL
+
+    check("R2SCAN")
+    P = grid.points()
+    r = np.linalg.norm(P[:, :3] - xyz.mean(axis=0), axis=1)
+    v = -1.0 / np.sqrt(r * r + 0.5)
+    grid.setVrho(v)
+    V = grid.getFock()[0]
+    ao, _, _ = ol.OracleSystem(Z, xyz, "SG-1", "def2-svp").ao_values(P[:, :3])
+    Vref = (ao * (P[:, 3] * v)) @ ao.T
+    _say("  type-2 grid, v_rho only: max dV %.2e" % np.abs(V - Vref).max())
+    assert np.abs(V - Vref).max() <= V_TOL
+    check("PBE")
+    check("R2SCAN")
