@@ -1176,6 +1176,15 @@ int ox_get_active_xyz(const ox_system* s, double* xyz) {
   return 0;
 }
 
+int ox_get_active_raw_weights(const ox_system* s, double* w) {
+  long k = 0;
+  for (int i : s->active) {
+    const SubGrid& sg = s->subgrids[i];
+    for (int g = 0; g < sg.n; ++g, ++k) w[k] = s->raw_w[sg.first + g];
+  }
+  return 0;
+}
+
 int ox_get_density(const ox_system* s, double* rho, double* grad, double* sigma, double* tau, double* w) {
   long k = 0;
   for (int i : s->active) {

@@ -104,6 +104,8 @@ long ox_active_points(const ox_system* s);
 int ox_get_density(const ox_system* s, double* rho, double* grad, double* sigma, double* tau, double* w);
 /* Coordinates of the active sub-grids' unpadded points, in the same order. */
 int ox_get_active_xyz(const ox_system* s, double* xyz);
+/* radial x angular weights (before the Becke partition) of the active sub-grids' points */
+int ox_get_active_raw_weights(const ox_system* s, double* w);
 
 /* Spin-polarised XC build, LDA/GGA: getDensity({Da, Db}) -> Evaluate("ev")
  * (polarised functional) -> getEnergy -> getFock, Va / Vb per spin. */

@@ -55,6 +55,7 @@ def lib():
         "ox_active_points": (ctypes.c_long, [_vp]),
         "ox_get_density": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "ox_get_active_xyz": (ctypes.c_int, [_vp, _vp]),
+        "ox_get_active_raw_weights": (ctypes.c_int, [_vp, _vp]),
         "ox_eval_functional_f": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long] + [_vp] * 6),
         "ox_fock_u": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
         "ox_xc_pol": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -201,6 +202,12 @@ class OracleSystem:
         xyz = np.zeros((len(rho), 3))
         _check(lib().ox_get_active_xyz(self.h, _ptr(xyz)))
         return xyz, rho, sig, tau, w
+
+    def active_raw_weights(self):
+        """Radial x angular weights (before the Becke partition) of the active sub-grids' points."""
+        w = np.zeros(lib().ox_active_points(self.h))
+        _check(lib().ox_get_active_raw_weights(self.h, _ptr(w)))
+        return w
 
     def ao_values(self, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
