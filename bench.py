@@ -35,7 +35,7 @@ FP64_MFMA_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix
 BF16_SPLIT_PEAK_TFLOPS = 2500.0 / 3.0
 HBM_PEAK_GBS = 8000.0
 DEFAULT_WORKLOAD = "h2o128_def2svp_sg2_r2scan"
-CPU_SAMPLE_SUBGRIDS = 256
+CPU_SAMPLE_SUBGRIDS = 16
 
 
 def _say(msg):
@@ -64,7 +64,8 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=CPU_SAMPLE_SUBGRIDS,
-                    help="sub-grids of the CPU baseline's sample on workloads with more than 4x as many")
+                    help="sub-grids of the CPU baseline's sample on large workloads (sub-grids x nbasis^2 above 2e9): "
+                         "every sub-grid carries all nbasis AOs, so each costs the same and a small sample fixes the rate")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain streaming kernel (torch.neg on float32) that reads and writes this many "
                          "MiB, three times: the known-size reference for the FETCH_SIZE / WRITE_SIZE counters")
@@ -74,6 +75,10 @@ def main():
     ap.add_argument("--surrogate-epochs", type=int, default=200)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg")
     ap.add_argument("--no-host-io", action="store_true", help="skip the host-memory D/V leg")
+    ap.add_argument("--scf", action="store_true",
+                    help="instead of the build timing: the full Kohn-Sham SCF of the workload (r2SCAN: fitted J from "
+                         "the device Coulomb engine, ADIIS -> CDIIS), mixed form from the core guess, then the plain "
+                         "FP64 form from the mixed form's converged Fock matrix; one JSON line with both energies")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -83,6 +88,9 @@ def main():
 
     if args.workload == "alkanes64":
         bench_pack(args)
+        return
+    if args.scf:
+        bench_scf(args)
         return
 
     import numpy as np
@@ -335,7 +343,7 @@ def cpu_baseline(args, w, D, E, V, N, natoms):
     orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], w["grid"], w["basis"], nthreads)
     orc.set_functional(w["functional"])
     nsub = orc.nsubgrids
-    sample = nsub if nsub <= 4 * args.cpu_sample else args.cpu_sample
+    sample = nsub if nsub * float(orc.nbasis) ** 2 <= 2e9 else min(nsub, args.cpu_sample)
     if sample < nsub:  # the first `sample` sub-grids of a seeded shuffle
         rng = wl.SplitMix64(4242)
         keys = sorted((rng.next_u64(), i) for i in range(nsub))
@@ -358,6 +366,60 @@ def cpu_baseline(args, w, D, E, V, N, natoms):
     acc = {"dE_xc": abs(E - Eo), "dE_xc_per_atom": abs(E - Eo) / natoms, "max_dV": float(np.abs(V - Vo).max()),
            "dN_e": abs(N - No)}
     return base, acc
+
+
+def bench_scf(args):
+    """Kohn-Sham SCF to convergence in both forms on the same molecule, grid type and fitted J:
+    |E_mixed - E_FP64| per atom is the north star's SCF-level accuracy figure."""
+    import numpy as np
+
+    import chinium_amd as ca
+    from chinium_amd import workloads as wl
+    from chinium_amd.scf import RestrictedDFSCF
+
+    w = wl.build_workload(args.workload)
+    natoms = len(w["Z"])
+    mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
+    xc = ca.ExchangeCorrelation()
+    xc.Read(w["functional"])
+    t0 = time.perf_counter()
+    jf = ca.CoulombFit(mol)
+    info = jf.info()
+    jf.int1e()
+    jf.metric()
+    t_ints = time.perf_counter() - t0
+    _say("Coulomb engine: %s; integrals and metric in %.1f s" % (info, t_ints))
+    out = {"metric": "Kohn-Sham SCF total energy, chosen (mixed) vs plain FP64 GPU XC form",
+           "unit": "Eh", "n_gpus": 1, "data": "synthetic (seeded geometry)",
+           "config": {"workload": wl.DESCRIPTIONS.get(args.workload, args.workload), "name": args.workload,
+                      "nbasis": mol.nbasis, "atoms": natoms, "nocc": w["nocc"], "coulomb": "density-fitted J, def2-svp-j, "
+                      "Coulomb metric", "aux_functions": info["naux"], "primitive_pairs": info["pairs"],
+                      "primitive_pairs_unscreened": info["pairs_unscreened"]},
+           "integrals_s": round(t_ints, 2)}
+    F0 = None
+    for prec, tol, name in ((ca.CHX_PREC_MIXED, 1e-5, "mixed"), (ca.CHX_PREC_FP64, 1e-6, "fp64")):
+        t0 = time.perf_counter()
+        grid = ca.Grid(mol, w["grid"], 1, 0)
+        grid.setPrecision(prec)
+        grid.setType(xc.family)
+        grid.getAO(0, 0)
+        t_grid = time.perf_counter() - t0
+        scf = RestrictedDFSCF(mol, w["nocc"], xc, grid, coulomb=jf)
+        t0 = time.perf_counter()
+        r = scf.run(F0=F0, tol=tol, log=_say)
+        t_scf = time.perf_counter() - t0
+        F0 = r["F"]
+        hist = r["history"]
+        out[name] = {"E": r["E"], "E_xc": r["E_xc"], "iterations": r["iterations"], "diis_error": r["diis_error"],
+                     "tol": tol, "start": "core guess" if name == "mixed" else "the mixed form's converged Fock matrix",
+                     "last_dE": hist[-1][1] - hist[-2][1] if len(hist) > 1 else None,
+                     "scf_s": round(t_scf, 2), "grid_s": round(t_grid, 2),
+                     "s_per_iter": {k: round(float(np.median(v)), 4) for k, v in scf.timing.items()}}
+        _say("%s: E %.10f in %d iterations (%.1f s)" % (name, r["E"], r["iterations"], t_scf))
+        del scf, grid
+    out["dE_per_atom"] = abs(out["mixed"]["E"] - out["fp64"]["E"]) / natoms
+    out["value"] = out["dE_per_atom"]
+    print(json.dumps(out), flush=True)
 
 
 def bench_pack(args):

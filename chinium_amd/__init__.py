@@ -61,6 +61,13 @@ def lib():
         "chx_mol_overlap": (i, [_vp, _vp]),
         "chx_mol_int1e": (i, [_vp, _vp, _vp, _vp]),
         "chx_mol_eri": (i, [_vp, _vp]),
+        "chx_jfit_create": (i, [_vp, cp, cp, P(_vp)]),
+        "chx_jfit_info": (i, [_vp, P(i), P(l), P(l), P(i)]),
+        "chx_jfit_int1e": (i, [_vp, _vp, _vp, _vp]),
+        "chx_jfit_metric": (i, [_vp, _vp]),
+        "chx_jfit_gamma": (i, [_vp, _vp, _vp]),
+        "chx_jfit_coulomb": (i, [_vp, _vp, _vp]),
+        "chx_jfit_destroy": (None, [_vp]),
         "chx_mol_nuclear_repulsion": (i, [_vp, P(ctypes.c_double)]),
         "chx_mol_destroy": (None, [_vp]),
         "chx_grid_create": (i, [_vp, cp, cp, i, i, P(_vp)]),
@@ -408,12 +415,75 @@ class Grid:
     def vxcKernel(self):
         """Name of the kernel the mixed form runs for the Vxc contraction on this basis size."""
         npad = -(-self.nbasis // 128) * 128
-        return "k_vxc_mx_wide" if npad % 256 == 0 and npad >= 1024 else "k_vxc_mx"
+        return "k_vxc_mx_wide" if npad >= 1024 else "k_vxc_mx"
 
     def stats(self):
         nb, npnt, mt, fl = ctypes.c_long(), ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
         _check(lib().chx_grid_stats(self.h, ctypes.byref(nb), ctypes.byref(npnt), ctypes.byref(mt), ctypes.byref(fl)))
         return {"batches": nb.value, "points": npnt.value, "mean_tiles": mt.value, "flops_per_build": fl.value}
+
+
+class CoulombFit:
+    """Density-fitted Coulomb matrix and one-electron integrals on the device (chx_jfit_*), in an
+    auxiliary basis from $CHINIUM_PATH/BasisSets (default def2-svp-j): gamma = (P|mn) D_mn,
+    (P|Q) c = gamma, J_mn = (mn|P) c_P. Matrices are n x n in the reference AO order."""
+
+    def __init__(self, mol, aux_basis="def2-svp-j", data_root=DATA_ROOT):
+        self.mol = mol  # keeps the molecule alive
+        self.h = None
+        h = _vp()
+        _check(lib().chx_jfit_create(mol.h, data_root.encode(), aux_basis.encode(), ctypes.byref(h)))
+        self.h = h
+        self.n = mol.nbasis
+
+    def info(self):
+        """dict(naux, pairs kept, pairs before screening, auxiliary primitives)."""
+        na, npr, nraw, nap = ctypes.c_int(), ctypes.c_long(), ctypes.c_long(), ctypes.c_int()
+        _check(lib().chx_jfit_info(self.h, ctypes.byref(na), ctypes.byref(npr), ctypes.byref(nraw), ctypes.byref(nap)))
+        return dict(naux=na.value, pairs=npr.value, pairs_unscreened=nraw.value, aux_prims=nap.value)
+
+    @property
+    def naux(self):
+        return self.info()["naux"]
+
+    def int1e(self):
+        """(S, T, V), computed once and kept."""
+        if getattr(self, "_int1e", None) is None:
+            n = self.n
+            S, T, V = (np.zeros((n, n), order="F") for _ in range(3))
+            _check(lib().chx_jfit_int1e(self.h, _ptr(S), _ptr(T), _ptr(V)))
+            self._int1e = (np.array(S), np.array(T), np.array(V))
+        return self._int1e
+
+    def metric(self):
+        """(P|Q), computed once and kept."""
+        if getattr(self, "_metric", None) is None:
+            na = self.naux
+            M = np.zeros((na, na), order="F")
+            _check(lib().chx_jfit_metric(self.h, _ptr(M)))
+            self._metric = np.array(M)
+        return self._metric
+
+    def gamma(self, D):
+        Df = np.asfortranarray(D, dtype=np.float64)
+        if Df.shape != (self.n, self.n):
+            raise ValueError("D must be %d x %d" % (self.n, self.n))
+        g = np.zeros(self.naux)
+        _check(lib().chx_jfit_gamma(self.h, _ptr(Df), _ptr(g)))
+        return g
+
+    def coulomb(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        if c.shape != (self.naux,):
+            raise ValueError("c must have %d entries" % self.naux)
+        J = np.zeros((self.n, self.n), order="F")
+        _check(lib().chx_jfit_coulomb(self.h, _ptr(c), _ptr(J)))
+        return np.array(J)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.chx_jfit_destroy(self.h)
+            self.h = None
 
 
 class ExchangeCorrelation:
@@ -458,20 +528,20 @@ class ExchangeCorrelation:
     def eval_points(self, rho, sigma=None, tau=None):
         """exc, v_rho, v_sigma, v_tau at the given points, evaluated on the device. Spin = 2:
         rho (n x 2), sigma (n x 3) in libxc's polarised layout; returns exc, v_rho (n x 2), v_sigma (n x 3)."""
+        f = lib().chx_xc_eval_points
         if self.Spin == 2:
             rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
             n = rho.shape[0]
             sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1, 3)
             e, vr, vs = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
-            _check(lib().chx_xc_eval_points(self.h, n, _ptr(rho), _ptr(sg), None, _ptr(e), _ptr(vr), _ptr(vs), None))
+            _check(f(self.h, n, _ptr(rho), _ptr(sg), None, _ptr(e), _ptr(vr), _ptr(vs), None))
             return e, vr, vs
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         n = len(rho)
         sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
         ta = None if tau is None else np.ascontiguousarray(tau, dtype=np.float64)
         e, vr, vs, vt = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
-        _check(lib().chx_xc_eval_points(self.h, n, _ptr(rho), _ptr(sg), _ptr(ta), _ptr(e), _ptr(vr), _ptr(vs),
-                                        _ptr(vt)))
+        _check(f(self.h, n, _ptr(rho), _ptr(sg), _ptr(ta), _ptr(e), _ptr(vr), _ptr(vs), _ptr(vt)))
         return e, vr, vs, vt
 
     def eval_points_f(self, rho, sigma=None):

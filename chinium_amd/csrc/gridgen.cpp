@@ -321,15 +321,76 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
   g->ao_ready = false;
 }
 
+// Vxc work items of the mixed form for large bases: the batches are cut into
+// chunks of `chunk` consecutive batches (the batch order is spatial, so a
+// chunk covers a compact region), and the union of the alive tiles of a
+// chunk's batches is cut into groups of 8 tiles in tile order. An item is an
+// ordered pair of groups (I, J) of one chunk: a 256 x 256 block of M at those
+// tiles' rows and columns, accumulated over the chunk's batches. Tiles of a
+// group that are not alive in a batch read the all-zero slot in the table,
+// and the kernel skips their loads and products, so no MFMA multiplies a
+// tile-pair block that the screening dropped. Items are chunk-major: the
+// items that run together on one XCD share the chunk's operand slots in its
+// L2.
+static void build_union_plan(chx_grid* g, int chunk) {
+  const int nb = g->nbatches, nt = g->ntiles, G = 2 * kCacheSuper;
+  const int zero = (int)g->bt_list.size();
+  g->cwork.clear();
+  g->cpair_batches.clear();
+  g->cslot.clear();
+  g->vchunk.clear();
+  g->vtab.clear();
+  g->vtile.clear();
+  std::vector<char> in_u(nt);
+  std::vector<int> U;
+  double useful = 0, dense = 0;
+  for (int b0 = 0; b0 < nb; b0 += chunk) {
+    const int b1 = std::min(nb, b0 + chunk);
+    std::fill(in_u.begin(), in_u.end(), 0);
+    for (int b = b0; b < b1; ++b)
+      for (int it = g->bt_off[b]; it < g->bt_off[b + 1]; ++it) in_u[g->bt_list[it]] = 1;
+    U.clear();
+    for (int t = 0; t < nt; ++t)
+      if (in_u[t]) U.push_back(t);
+    if (U.empty()) continue;
+    const int nu = (int)U.size(), ng = (nu + G - 1) / G, npos = ng * G;
+    const int ci = (int)g->vchunk.size();
+    g->vchunk.push_back(make_int4((int)g->vtab.size(), b1 - b0, npos, (int)g->vtile.size()));
+    for (int u = 0; u < npos; ++u) g->vtile.push_back(u < nu ? U[u] : -1);
+    for (int b = b0; b < b1; ++b) {
+      for (int u = 0; u < npos; ++u) {
+        const int si = u < nu ? g->slot_of[(size_t)b * nt + U[u]] : -1;
+        g->vtab.push_back(si >= 0 ? si : zero);
+      }
+      const double n = g->bt_off[b + 1] - g->bt_off[b];
+      useful += n * n;
+    }
+    dense += (double)(b1 - b0) * nu * nu;
+    for (int I = 0; I < ng; ++I)
+      for (int J = 0; J < ng; ++J) g->cwork.push_back(make_int4(I, J, ci, 0));
+  }
+  g->vxc_exec_frac = dense > 0 ? useful / dense : 1.0;
+}
+
 // Vxc work items of the cached / mixed paths: (super-tile I, super-tile J,
 // range of pair_batches) over 128-AO super-tiles, a chunk of batches where
 // both are alive. Chunk-major order: blocks that run at the same time work on
 // the same batches of different (I, J) pairs, so their operand tiles meet in
-// L2. Fewer, longer items mean fewer FP64 atomics into M.
+// L2. Fewer, longer items mean fewer FP64 atomics into M. Large bases in the
+// mixed form take the chunk tile-group plan above.
 void build_cwork(chx_grid* g) {
   const int nb = g->nbatches, nt = g->ntiles;
-  const bool wide = g->precision == CHX_PREC_MIXED && g->mx_wide;
-  const int sup = wide ? 2 * kCacheSuper : kCacheSuper;  // tiles per super-tile
+  const bool mixed = g->precision == CHX_PREC_MIXED;
+  size_t chunk = g->vxc_chunk > 0 ? (size_t)g->vxc_chunk : (mixed ? 32 : kCacheChunk);
+  if (mixed) chunk = std::min<size_t>(chunk, kMxMaxChunk);  // the kernel stages one item's slot table
+  if (mixed && g->mx_wide) {
+    build_union_plan(g, (int)chunk);
+    return;
+  }
+  g->vchunk.clear();
+  g->vtab.clear();
+  g->vtile.clear();
+  const int sup = kCacheSuper;  // tiles per super-tile
   const int nsc = g->npad / (sup * kTile);
   std::vector<std::vector<int>> cp((size_t)nsc * nsc);
   std::vector<int> alive_sup;
@@ -347,9 +408,6 @@ void build_cwork(chx_grid* g) {
   g->cpair_batches.clear();
   size_t maxlen = 0;
   for (const auto& L : cp) maxlen = std::max(maxlen, L.size());
-  const bool mixed = g->precision == CHX_PREC_MIXED;
-  size_t chunk = g->vxc_chunk > 0 ? (size_t)g->vxc_chunk : (mixed ? 32 : kCacheChunk);
-  if (mixed) chunk = std::min<size_t>(chunk, kMxMaxChunk);  // the kernel stages one item's slot table
   for (size_t k = 0; k < maxlen; k += chunk)
     for (int I = 0; I < nsc; ++I)
       for (int J = 0; J < nsc; ++J) {
@@ -377,6 +435,24 @@ void build_cwork(chx_grid* g) {
   }
 }
 
+// Uploads the mixed form's Vxc plan and points the descriptor at it.
+static void upload_mx_plan(chx_grid* g, hipStream_t st) {
+  // the plan the grid does not use stays a one-element placeholder (the copies
+  // are asynchronous, so their sources outlive the call)
+  static const std::vector<int> one{0};
+  static const std::vector<int4> one4{make_int4(0, 0, 0, 0)};
+  g->d_cslot.upload(g->cslot.empty() ? one : g->cslot, st);
+  g->d_vchunk.upload(g->vchunk.empty() ? one4 : g->vchunk, st);
+  g->d_vtab.upload(g->vtab.empty() ? one : g->vtab, st);
+  g->d_vtile.upload(g->vtile.empty() ? one : g->vtile, st);
+  g->hmol.cwork = g->d_cwork.p;
+  g->hmol.pair_batches = g->d_cpair_batches.p;
+  g->hmol.cslot = g->d_cslot.p;
+  g->hmol.vchunk = g->d_vchunk.p;
+  g->hmol.vtab = g->d_vtab.p;
+  g->hmol.vtile = g->d_vtile.p;
+}
+
 // Rebuilds the Vxc work items for another chunk length (autotuning) without
 // touching the AO tables.
 void set_vxc_chunk(chx_grid* g, int chunk) {
@@ -387,12 +463,7 @@ void set_vxc_chunk(chx_grid* g, int chunk) {
   hipStream_t st = g->stream;
   g->d_cwork.upload(g->cwork, st);
   g->d_cpair_batches.upload(g->cpair_batches, st);
-  if (g->precision == CHX_PREC_MIXED) {
-    g->d_cslot.upload(g->cslot, st);
-    g->hmol.cwork = g->d_cwork.p;
-    g->hmol.pair_batches = g->d_cpair_batches.p;
-    g->hmol.cslot = g->d_cslot.p;
-  }
+  if (g->precision == CHX_PREC_MIXED) upload_mx_plan(g, st);
   CHX_HIP(hipStreamSynchronize(st));
 }
 
@@ -502,8 +573,8 @@ void grid_plan(chx_grid* g) {
     throw Error(CHX_ERR_DEVICE, "the mixed-precision form needs " + std::to_string(cache_bytes / 1e9) +
                                     " GB of device memory for its AO cache");
   g->cached = mixed || g->cache_mode == 1 || (g->cache_mode == -1 && cache_bytes < 0.8 * (double)free_b);
-  // 256-AO Vxc super-tiles where they tile npad exactly and the basis is large
-  g->mx_wide = mixed && g->npad % (2 * kCacheSuper * kTile) == 0 && g->npad >= 1024 ? 1 : 0;
+  // large bases: Vxc items over tile groups of batch chunks, 256 x 256 blocks
+  g->mx_wide = mixed && g->npad >= 1024 ? 1 : 0;
   if (g->cached) {
     build_cwork(g);
     g->d_slot_batch.upload(g->slot_batch, st); g->d_slot_tile.upload(g->slot_tile, st);
@@ -524,7 +595,6 @@ void grid_plan(chx_grid* g) {
       // B^T of every slot, plus the all-zero slot that make_b rewrites for the
       // component count of each build's potential
       g->d_bmx.alloc((nslot + 1) * (size_t)nT * kMxComp);
-      g->d_cslot.upload(g->cslot, st);
       g->d_Dsplit.alloc(2 * (size_t)g->npad * g->npad);
       launch_ao_cache_mx(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p,
                          g->d_z.p, g->view(), nT, g->mx_sstride, g->d_cmx.p);
@@ -575,9 +645,9 @@ void grid_plan(chx_grid* g) {
     m.part = g->d_part.p; m.part_tau = g->d_part_tau.p;
     m.slot_batch = g->d_slot_batch.p; m.slot_tile = g->d_slot_tile.p;
     m.coef = g->d_coef.p; m.bt = g->d_bmx.p;
-    m.cwork = g->d_cwork.p; m.pair_batches = g->d_cpair_batches.p; m.slot_of = g->d_slot_of.p;
-    m.cslot = g->d_cslot.p;
+    m.slot_of = g->d_slot_of.p;
     m.Mint = g->d_Mint.p;
+    upload_mx_plan(g, st);
   }
   CHX_HIP(hipStreamSynchronize(st));
   g->ao_ready = true;

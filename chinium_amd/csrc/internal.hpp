@@ -40,6 +40,20 @@ struct ShellH {
 
 int shell_size(int type);
 
+// A component polynomial of a shell as monomials c x^i y^j z^k, in the
+// reference's component order (molecule.cpp).
+struct MonoTerm {
+  double c;
+  int i, j, k;
+};
+std::vector<std::vector<MonoTerm>> component_monomials(int type);
+// A shell of an auxiliary (fitting) basis: angular momentum l (pure), centre,
+// exponents, normalised contraction coefficients.
+struct AuxShell {
+  int l, center;
+  std::vector<double> e, c;
+};
+
 // A device buffer owned by a handle.
 template <class T> struct DevBuf {
   T* p = nullptr;
@@ -202,8 +216,15 @@ struct MolDev {
   unsigned short* bt;           // B^T per slot
   const int4* cwork;            // Vxc work items
   const int *pair_batches, *slot_of;
-  const int* cslot;             // per pair_batches entry: slots of I's 4 (wide: 8) tiles, then J's (zero slot if absent)
+  const int* cslot;             // per pair_batches entry: slots of I's 4 tiles, then J's (zero slot if absent)
   double* Mint;                 // M accumulator, npad^2
+  // Vxc over chunk tile groups (large bases): per chunk of consecutive batches
+  // (table offset, batches, table width = padded alive-tile union, tile-list
+  // offset); per (chunk batch, union position) the slot (zero slot if the tile
+  // is not alive in that batch); per union position the tile (-1: padding).
+  const int4* vchunk;
+  const int* vtab;
+  const int* vtile;
 };
 constexpr int kMxMaxTiles = 512;  // alive tiles per batch the mixed density kernel stages (npad <= 16384)
 constexpr int kMxMaxChunk = 128;  // batches per Vxc work item in the mixed form
@@ -233,12 +254,17 @@ void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one);
 // B^T of every slot plus the all-zero slot (index nslots) for ncomp components
 void launch_make_b_mx(hipStream_t s, int ncomp, const MolDev& one);
 void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one);
-// 256-AO super-tiles (npad a multiple of 256): cslot holds 16 slots per batch
+// 256 x 256 output blocks over groups of 8 tiles of each batch chunk's alive-tile
+// union (vchunk / vtab / vtile); items (group I, group J, chunk)
 void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one);
 
 }  // namespace chx
 
 // Opaque handle definitions.
+namespace chx {
+std::vector<AuxShell> aux_basis_shells(const chx_mol* m, const std::string& data_root, const std::string& basis);
+}
+
 struct chx_mol {
   int natoms = 0;
   std::vector<int> Z;
@@ -316,6 +342,12 @@ struct chx_grid {
   std::vector<int> cpair_batches;
   std::vector<int> cslot;              // mixed form: slot table per cpair_batches entry
   chx::DevBuf<int> d_cslot;
+  // mixed form, large bases: Vxc items over groups of each batch chunk's alive-tile union
+  std::vector<int4> vchunk;
+  std::vector<int> vtab, vtile;
+  chx::DevBuf<int4> d_vchunk;
+  chx::DevBuf<int> d_vtab, d_vtile;
+  double vxc_exec_frac = 1.0;          // useful / executed tile-pair products of the mixed Vxc plan
   chx::DevBuf<double> d_cache, d_bmat, d_Mint;
   chx::DevBuf<int> d_slot_batch, d_slot_tile, d_slot_of, d_tile_nao, d_grp_slots, d_cpair_batches;
   chx::DevBuf<int4> d_cwork;

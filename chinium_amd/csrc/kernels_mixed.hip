@@ -639,14 +639,168 @@ __global__ __launch_bounds__(256, 2) void k_density_mx_tau(MolDev m) {
   }
 }
 
+// Meta-GGA density in one launch: the four products Phi D and d_r Phi D share
+// one D block per K step (48 fragments: A_0..A_3, then D), so D is staged
+// once per build instead of once for rho and once for the tau partials.
+// Epilogues: rho and grad rho / 2 into part (as k_density_mx<0>), then the
+// three tau partials into part_tau (as k_density_mx_tau). Same indexing as
+// k_density_mx.
+struct DensMxMggaSmem {
+  union {
+    u32x4 F[(4 * kDxA + kDxB) * kDxFrag];  // one K step's fragments: A_0..A_3, D
+    float X[kGroup * kTile][kBatch + 1];
+  };
+  float red[kGroup][4][kBatch];
+  double px[kBatch], py[kBatch], pz[kBatch];
+  int tl[kMxMaxTiles];
+};
+
+template <int R>
+__device__ __forceinline__ void dens_mgga_epilogue(DensMxMggaSmem& sm, const f32x4 (&acc)[2][4], const BasisView& bv,
+                                                   const int (&jt)[kGroup], int nt, int wave, int lane, int wr, int wc,
+                                                   int lrow, int lk) {
+  __syncthreads();  // the previous user of the X area is done
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sm.X[64 * wc + 16 * c + lrow][32 * wr + 16 * r + 4 * lk + i] = acc[r][c][i];
+  __syncthreads();
+  DensEpi<R> epi;
+  if (wave < nt) {
+    const int t = wave == 1 ? jt[1] : wave == 2 ? jt[2] : wave == 3 ? jt[3] : jt[0];
+    epi.load(&sm.X[kTile * wave][lane]);
+    walk_tile_f32<true>(bv, t, sm.px[lane], sm.py[lane], sm.pz[lane], epi);
+  }
+  if (R == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sm.red[wave][c][lane] = epi.o[c];
+  } else {
+    sm.red[wave][R - 1][lane] = epi.o[0];
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void k_density_mx_mgga(MolDev m) {
+  using lds_ptr = __attribute__((address_space(3))) void*;
+  __shared__ DensMxMggaSmem sm;
+  const int grp = xcd_remap(blockIdx.x, gridDim.x);
+  const u16* __restrict__ cache = m.cache;  // components 0 (phi), 1..3 (d_r phi)
+  const u16* __restrict__ Ds = m.Ds;
+  const size_t sstride = (size_t)m.sstride;
+  const int npad = m.npad;
+  const BasisView bv = m.bv;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int wr = wave & 1, wc = wave >> 1;
+  const int b = m.grp_batch[grp];
+  const int nt = m.grp_ntiles[grp];
+  int jt[kGroup];
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) jt[q] = m.grp_tiles[grp * kGroup + q];
+  const int it0 = m.bt_off[b], nk = m.bt_off[b + 1] - it0;
+  if (tid < kBatch) {
+    sm.px[tid] = m.bx[b * kBatch + tid];
+    sm.py[tid] = m.by[b * kBatch + tid];
+    sm.pz[tid] = m.bz[b * kBatch + tid];
+  }
+  for (int j = tid; j < nk; j += 256) sm.tl[j] = m.bt_list[it0 + j];
+  const int drow = lane >> 2, dch = (lane & 3) ^ ((drow >> 2) & 3);
+  const int fidx = 4 * lrow + (lk ^ ((lrow >> 2) & 3));
+  int offa[2], offb[4];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int f = wave + 4 * j;
+    offa[j] = (16 * (f >> 1) + drow) * kMxRow + (f & 1) * kTile + 8 * dch;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int f = wave + 4 * j, c = f >> 1, q = c >> 1;
+    const int t = q == 1 ? jt[1] : q == 2 ? jt[2] : q == 3 ? jt[3] : jt[0];
+    offb[j] = (f & 1) * npad * npad + ((q < nt ? t : jt[0]) * kTile + 16 * (c & 1) + drow) * npad + 8 * dch;
+  }
+  __syncthreads();
+  auto dma = [&](int k) {
+    const u16* a = cache + (size_t)(it0 + k) * sstride;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        __builtin_amdgcn_global_load_lds(a + (size_t)r * kMxT + offa[j],
+                                         (lds_ptr)(sm.F + (r * kDxA + wave + 4 * j) * kDxFrag), 16, 0, 0);
+    const int ti = sm.tl[k] * kTile;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      __builtin_amdgcn_global_load_lds(Ds + offb[j] + ti, (lds_ptr)(sm.F + (4 * kDxA + wave + 4 * j) * kDxFrag), 16,
+                                       0, 0);
+  };
+
+  f32x4 acc[4][2][4];
+#pragma unroll
+  for (int d = 0; d < 4; ++d)
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[d][r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int ncb = 2 * (nt - 2 * wc);
+  dma(0);
+  for (int k = 0; k < nk; ++k) {
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
+    __syncthreads();                     // ... and every other wave's
+    u32x4 a[4][2][2], bq[4][2];
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) a[d][r][h] = sm.F[(d * kDxA + (2 * wr + r) * 2 + h) * kDxFrag + fidx];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) bq[c][h] = sm.F[(4 * kDxA + (4 * wc + c) * 2 + h) * kDxFrag + fidx];
+    __syncthreads();  // every wave holds its fragments: the buffer is free
+    if (k + 1 < nk) dma(k + 1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c < ncb) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+            acc[d][r][c] = mfma3(a[d][r][0], a[d][r][1], bq[c][0], bq[c][1], acc[d][r][c]);
+      }
+  }
+  dens_mgga_epilogue<0>(sm, acc[0], bv, jt, nt, wave, lane, wr, wc, lrow, lk);
+  __syncthreads();
+  if (tid < 4 * kBatch) {
+    const int c = tid / kBatch, g = tid % kBatch;
+    double v = 0.0;
+    for (int q = 0; q < nt; ++q) v += (double)sm.red[q][c][g];
+    m.part[(size_t)grp * 4 * kBatch + c * kBatch + g] = v;
+  }
+  dens_mgga_epilogue<1>(sm, acc[1], bv, jt, nt, wave, lane, wr, wc, lrow, lk);
+  dens_mgga_epilogue<2>(sm, acc[2], bv, jt, nt, wave, lane, wr, wc, lrow, lk);
+  dens_mgga_epilogue<3>(sm, acc[3], bv, jt, nt, wave, lane, wr, wc, lrow, lk);
+  __syncthreads();
+  if (tid < 3 * kBatch) {
+    const int c = tid / kBatch, g = tid % kBatch;
+    double v = 0.0;
+    for (int q = 0; q < nt; ++q) v += (double)sm.red[q][c][g];
+    m.part_tau[(size_t)grp * 3 * kBatch + c * kBatch + g] = v;
+  }
+}
+
 void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one) {
   if (nitems == 0) return;
   if (mode == 0)
     hipLaunchKernelGGL((k_density_mx<0>), dim3(nitems), dim3(256), 0, s, one);
   else if (mode == 4)  // the three tau partials in one launch
     hipLaunchKernelGGL(k_density_mx_tau, dim3(nitems), dim3(256), 0, s, one);
+  else if (mode == 5)  // rho, grad rho and the tau partials in one launch
+    hipLaunchKernelGGL(k_density_mx_mgga, dim3(nitems), dim3(256), 0, s, one);
   else
-    throw Error(CHX_ERR_ARG, "density mode must be 0 (rho, grad rho) or 4 (tau partials)");
+    throw Error(CHX_ERR_ARG, "density mode must be 0 (rho, grad rho), 4 (tau partials) or 5 (both)");
   CHX_HIP(hipGetLastError());
 }
 
@@ -856,19 +1010,25 @@ __global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev m) {
       }
 }
 
-// Vxc for large bases: the same contraction over 256-AO super-tiles, so that
-// every operand byte staged in LDS feeds twice the MFMAs of the 128 x 128
-// kernel (64 instead of 32 FLOP per byte). A K step is 32 points of one
-// component: Phi^T_I and B^T_J, 256 rows each, hi and lo, 64 fragments of
-// 1 KB brought in by the waves' DMA (same coalesced lane order and swizzle).
-// 8 waves in a 2 x 4 grid, each owning a 128 x 64 block of M (96 MFMAs per
-// step). The slot table holds 16 slots per batch (I's 8 tiles, then J's).
-// Same one-buffer step and the same symmetric tau handling as k_vxc_mx.
-constexpr int kMxSupW = 8;   // tiles per wide super-tile (256 AOs)
+// Vxc for large bases: M_IJ += Phi_I^T B_J over 256 x 256 blocks whose rows
+// and columns are groups of 8 tiles of a batch chunk's alive-tile union (the
+// item is (group I, group J, chunk)), so every operand byte staged in LDS
+// feeds twice the MFMAs of the 128 x 128 kernel (64 instead of 32 FLOP per
+// byte). A K step is 32 points of one component: Phi^T_I and B^T_J, 256 rows
+// each, hi and lo, 64 fragments of 1 KB brought in by the waves' DMA (same
+// coalesced lane order and swizzle). 8 waves in a 2 x 4 grid, each owning a
+// 128 x 64 block of M (up to 96 MFMAs per step). Per batch a 16-bit mask
+// (I's 8 tiles, then J's) marks the tiles alive in it: a dead tile's
+// fragments are not loaded and the products that involve it are skipped, so
+// the MFMAs follow each batch's screening. Same one-buffer step and the same
+// symmetric tau handling as k_vxc_mx.
+constexpr int kMxSupW = 8;   // tiles per group (256 AOs)
 constexpr int kVxAW = 32;    // Phi^T fragments per step: [row block (16)][hl]
 struct VxcMxWideSmem {
   u32x4 F[2 * kVxAW * kDxFrag];  // one K step's fragments: Phi^T_I, then B^T_J
-  int sl[kMxMaxChunk][16];
+  int sl[kMxMaxChunk][16];       // per batch: slots of I's 8 tiles, then J's
+  unsigned mk[kMxMaxChunk];      // per batch: alive bits of those 16 tiles
+  int tile[16];                  // I's tiles, then J's (-1: padding)
 };
 
 template <int NC>
@@ -878,14 +1038,31 @@ __global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
   const int item = xcd_remap(blockIdx.x, gridDim.x);
   const int4 wk = m.cwork[item];
   const int I = wk.x, J = wk.y;
+  const int4 ch = m.vchunk[wk.z];  // (table offset, batches, table width, tile-list offset)
+  const int nbat = ch.y;
+  const int zero = m.nslots;       // the all-zero slot: tile not alive in that batch
   const u16* __restrict__ cache = m.cache + m.toff;
   const u16* __restrict__ bt = m.bt;
   const size_t sstride = (size_t)m.sstride;
   const int npad = m.npad;
   double* __restrict__ Mint = m.Mint;
   {
-    const int* __restrict__ cs = m.cslot + (size_t)wk.z * 16;
-    for (int e = threadIdx.x; e < (wk.w - wk.z) * 16; e += 512) sm.sl[e >> 4][e & 15] = cs[e];
+    const int* __restrict__ tab = m.vtab + ch.x;
+    for (int e = threadIdx.x; e < nbat * 16; e += 512) {
+      const int it = e >> 4, t = e & 15;
+      sm.sl[it][t] = tab[(size_t)it * ch.z + (t < 8 ? I * kMxSupW + t : J * kMxSupW + t - 8)];
+    }
+    if (threadIdx.x < 16) {
+      const int t = threadIdx.x;
+      sm.tile[t] = m.vtile[ch.w + (t < 8 ? I * kMxSupW + t : J * kMxSupW + t - 8)];
+    }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < nbat; it += 512) {
+    unsigned b = 0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) b |= (sm.sl[it][t] != zero ? 1u : 0u) << t;
+    sm.mk[it] = b;
   }
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -898,9 +1075,10 @@ __global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
   const int off = (wave & 1) * (kTile * kBatch) + (16 * ((wave >> 1) & 1) + drow) * kBatch + 8 * dch;
   const int t0 = wave >> 2;
   __syncthreads();
-  const int nbat = wk.w - wk.z;
   const int ntau = (NC == 4 && I <= J) ? 6 * nbat : 0;
   const int nstage = ntau + 2 * nbat;
+  // K step k -> batch of the item: tau steps (it, h, c = 1..3) first, then (it, h, c = 0)
+  auto step_batch = [&](int k) { return k < ntau ? k / 6 : (k - ntau) >> 1; };
   auto dma = [&](int k) {
     int it, h, c;
     if (k < ntau) {
@@ -908,13 +1086,18 @@ __global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
     } else {
       it = (k - ntau) >> 1; h = (k - ntau) & 1; c = 0;
     }
+    const unsigned mk = __builtin_amdgcn_readfirstlane(sm.mk[it]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int t = t0 + 2 * j;
-      const u16* pa = cache + (size_t)sm.sl[it][t] * sstride + c * kMxT + h * 32;
-      const u16* pb = bt + ((size_t)sm.sl[it][8 + t] * NC + c) * kMxT + h * 32;
-      __builtin_amdgcn_global_load_lds(pa + off, (lds_ptr)(sm.F + (wave + 8 * j) * kDxFrag), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds(pb + off, (lds_ptr)(sm.F + (kVxAW + wave + 8 * j) * kDxFrag), 16, 0, 0);
+      if ((mk >> t) & 1u) {
+        const u16* pa = cache + (size_t)sm.sl[it][t] * sstride + c * kMxT + h * 32;
+        __builtin_amdgcn_global_load_lds(pa + off, (lds_ptr)(sm.F + (wave + 8 * j) * kDxFrag), 16, 0, 0);
+      }
+      if ((mk >> (8 + t)) & 1u) {
+        const u16* pb = bt + ((size_t)sm.sl[it][8 + t] * NC + c) * kMxT + h * 32;
+        __builtin_amdgcn_global_load_lds(pb + off, (lds_ptr)(sm.F + (kVxAW + wave + 8 * j) * kDxFrag), 16, 0, 0);
+      }
     }
   };
   f32x4 acc[8][4];
@@ -930,6 +1113,9 @@ __global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[r][c] *= 2.f;
     }
+    // this wave's row tiles 4 wr..4 wr+3 (bits 0..3 of rmk) and column tiles 2 wc, 2 wc+1 (cmk) alive in the step's batch
+    const unsigned mk = __builtin_amdgcn_readfirstlane(sm.mk[step_batch(k)]);
+    const unsigned rmk = (mk >> (4 * wr)) & 15u, cmk = (mk >> (8 + 2 * wc)) & 3u;
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
     __syncthreads();                     // ... and every other wave's
     u32x4 a[8][2], bq[4][2];
@@ -945,19 +1131,28 @@ __global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
     if (k + 1 < nstage) dma(k + 1);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
+      if ((cmk >> (c >> 1)) & 1u) {
 #pragma unroll
-      for (int r = 0; r < 8; ++r) acc[r][c] = mfma3(a[r][0], a[r][1], bq[c][0], bq[c][1], acc[r][c]);
+        for (int r = 0; r < 8; ++r)
+          if ((rmk >> (r >> 1)) & 1u) acc[r][c] = mfma3(a[r][0], a[r][1], bq[c][0], bq[c][1], acc[r][c]);
+      }
   }
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < 8; ++r) {
+    const int ti = sm.tile[4 * wr + (r >> 1)];
+    if (ti < 0) continue;
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
+    for (int cb = 0; cb < 4; ++cb) {
+      const int tj = sm.tile[8 + 2 * wc + (cb >> 1)];
+      if (tj < 0) continue;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int mu = I * kMxSupW * kTile + 128 * wr + 16 * r + 4 * lk + i;
-        const int nu = J * kMxSupW * kTile + 64 * wc + 16 * cb + lrow;
+        const int mu = ti * kTile + 16 * (r & 1) + 4 * lk + i;
+        const int nu = tj * kTile + 16 * (cb & 1) + lrow;
         unsafeAtomicAdd(&Mint[(size_t)mu * npad + nu], (double)acc[r][cb][i]);
       }
+    }
+  }
 }
 
 void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one) {

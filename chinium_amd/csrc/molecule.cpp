@@ -158,6 +158,33 @@ void os_1d(double a, double b, double A, double B, int li, int lj, double* S /* 
 }
 
 }  // namespace
+
+std::vector<std::vector<MonoTerm>> component_monomials(int type) {
+  std::vector<std::vector<MonoTerm>> out;
+  for (const auto& comp : monomials(type)) {
+    out.emplace_back();
+    for (const Mono& t : comp) out.back().push_back(MonoTerm{t.c, t.i, t.j, t.k});
+  }
+  return out;
+}
+
+std::vector<AuxShell> aux_basis_shells(const chx_mol* m, const std::string& data_root, const std::string& basis) {
+  auto lib = parse_gbs(data_root + "/BasisSets/" + basis + ".gbs");
+  std::vector<AuxShell> out;
+  for (int a = 0; a < m->natoms; ++a) {
+    auto it = lib.find(m->Z[a]);
+    if (it == lib.end()) throw Error(CHX_ERR_FILE, "Basis set file does not include this element!");
+    for (const RawShell& rs : it->second) {
+      const int l = rs.type == 1 ? 1 : std::abs(rs.type);
+      if (l > 4) throw Error(CHX_ERR_UNSUPPORTED, "auxiliary shells above g are not supported");
+      std::vector<double> c = rs.c;
+      normalise(l, rs.e, c);
+      out.push_back(AuxShell{l, a, rs.e, c});
+    }
+  }
+  return out;
+}
+
 }  // namespace chx
 
 using namespace chx;

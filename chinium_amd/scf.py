@@ -96,6 +96,20 @@ class RestrictedSCF:
         raise RuntimeError("Convergence failed!")
 
 
+    def run_reference(self, adiis_tol=1e-1, tol=1e-6, max_iter=300, log=None):
+        """The reference's driver on this update: ADIIS to max|G| < adiis_tol, then CDIIS to
+        max|G| < tol (SP.cpp:73-79), from the core guess."""
+        def update(F):
+            Fnew, E, G, eps, C, Exc = self.update(F)
+            return Fnew, E, G, C[:, :self.nocc] @ C[:, :self.nocc].T, eps, C, Exc
+
+        last, hist = run_reference_diis(update, self.Hcore.copy(), lambda a, b: float(np.sum(a * b)),
+                                        lambda G: float(np.abs(G).max()), adiis_tol, tol, max_iter, log, self.enuc)
+        Fnew, E, G, D, eps, C, Exc = last
+        return dict(E=E + self.enuc, E_elec=E, E_xc=Exc, iterations=len(hist), eps=eps, C=C,
+                    diis_error=float(np.abs(G).max()), history=hist)
+
+
 class UnrestrictedSCF:
     """na alpha and nb beta electrons; xc = None gives UHF (EXX = 1). With xc set (Spin = 2 before
     Read, LDA/GGA), grid must be a Grid of the molecule, typed and planned.
@@ -174,3 +188,196 @@ class UnrestrictedSCF:
                 F = np.hstack([Fna, Fnb])
             Fa, Fb = F[:, :n], F[:, n:]
         raise RuntimeError("Convergence failed!")
+
+
+# ---------------------------------------------------------------------------
+# DIIS as the reference runs it (src/DIIS/DIIS.cpp:88-188, CDIIS.cpp, AEDIIS.cpp)
+
+class DIIS:
+    """History of (update F, residual G, density D, energy E) and one extrapolation rule:
+    "ADIIS" (AEDIIS.cpp: weights on the simplex minimising x.A + x.B x / 2 with
+    A_i = <D_i - D_n, F_n>, B_ij = <D_i - D_n, F_j - F_n>) or "CDIIS" (CDIIS.cpp:
+    the Pulay equations on <G_i, G_j> with a ridge factor that grows by 0.001 each time
+    the space is cut for weights above 5 with a near-singular matrix). Fewer than two
+    entries: the plain update (DIIS.cpp:151-163, no damping)."""
+
+    def __init__(self, kind, max_size=20, tol=1e-6, max_iter=300):
+        self.kind, self.max_size, self.tol, self.max_iter = kind, max_size, tol, max_iter
+        self.F, self.G, self.D, self.E = [], [], [], []
+
+    def steal(self, other):
+        """DIIS::Steal: take the other's history, dropping the oldest beyond max_size."""
+        k = max(0, len(other.F) - self.max_size)
+        self.F, self.G, self.D, self.E = other.F[k:], other.G[k:], other.D[k:], other.E[k:]
+
+    def append(self, F, G, D, E):
+        if len(self.F) == self.max_size:
+            self.F.pop(0); self.G.pop(0); self.D.pop(0); self.E.pop(0)
+        self.F.append(F); self.G.append(G); self.D.append(D); self.E.append(E)
+
+    def weights(self, dot):
+        n = len(self.F)
+        if self.kind == "CDIIS":
+            ridge = 1.0
+            for size in range(n, 1, -1):
+                H = np.zeros((size + 1, size + 1))
+                for i in range(size):
+                    H[i, size] = H[size, i] = 1.0
+                    for j in range(i, size):
+                        H[i, j] = H[j, i] = dot(self.G[n - size + i], self.G[n - size + j])
+                H[np.diag_indices(size + 1)] *= ridge
+                b = np.zeros(size + 1)
+                b[size] = 1.0
+                x = np.linalg.lstsq(H, b, rcond=None)[0][:size]
+                w = np.zeros(n)
+                w[n - size:] = x
+                if np.abs(x).max() < 5 or np.linalg.det(H) > 1e-12:
+                    return w
+                ridge += 0.001
+            return w
+        # ADIIS: minimise on the simplex, x = t^2 / sum t^2
+        from scipy.optimize import minimize
+        Dn, Fn = self.D[-1], self.F[-1]
+        A = np.array([dot(self.D[i] - Dn, Fn) for i in range(n)])
+        B = np.array([[dot(self.D[i] - Dn, self.F[j] - Fn) for j in range(n)] for i in range(n)])
+        Bs = 0.5 * (B + B.T)
+
+        def f(t):
+            s = t @ t
+            x = t * t / s
+            val = x @ A + 0.5 * x @ Bs @ x
+            gx = A + Bs @ x
+            gt = 2 * t / s * (gx - x @ gx)
+            return val, gt
+
+        x0 = np.full(n, 0.1 / max(1, n - 1))
+        x0[-1] = 0.9
+        r = minimize(f, np.sqrt(x0), jac=True, method="L-BFGS-B", options=dict(maxiter=1000, gtol=1e-10))
+        t = r.x
+        return t * t / (t @ t)
+
+
+class RestrictedDFSCF:
+    """Restricted Kohn-Sham SCF (RestrictedDIIS, src/HartreeFockKohnSham/Restricted/SP.cpp:20-81) with
+    every matrix on the device: S, T, V and J from the Coulomb engine (CoulombFit: J fitted in an
+    auxiliary basis with the Coulomb metric), the XC part from the grid, and the dense algebra
+    (diagonalisation, products) in torch on cuda:0. For functionals without exact exchange
+    (EXX = 0, e.g. r2SCAN), which is what the engine serves.
+
+    Update (SP.cpp:36-66): orbitals of F in the S^{-1/2} basis, D = C_occ C_occ^T,
+    J = J[2D] (fitted), XC sequence grid.getDensity({2D}) -> Evaluate("ev") -> getEnergy ->
+    getFock, E = sum D (Hcore + Fhf) + E_xc, G = 2 (F D S - S D F). Then ADIIS until
+    max|G| < adiis_tol, CDIIS (taking over ADIIS's history) until max|G| < tol
+    (SP.cpp:73-79)."""
+
+    def __init__(self, mol, nocc, xc, grid, coulomb=None, device="cuda"):
+        import torch
+
+        if float(xc.EXX) != 0.0:
+            raise ValueError("the density-fitted SCF serves functionals without exact exchange")
+        if not 0 < int(nocc) <= mol.nbasis:
+            raise ValueError("nocc must be in 1..nbasis")
+        from . import CoulombFit
+
+        self.torch, self.dev = torch, torch.device(device)
+        self.mol, self.nocc, self.xc, self.grid = mol, int(nocc), xc, grid
+        t0 = time.perf_counter()
+        self.jf = coulomb if coulomb is not None else CoulombFit(mol)
+        S, T, V = self.jf.int1e()
+        M = self.jf.metric()
+        self.t_setup = time.perf_counter() - t0
+        tt = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=self.dev)
+        self.S, self.Hcore = tt(S), tt(T + V)
+        self.Lm = torch.linalg.cholesky(tt(M))
+        s, U = torch.linalg.eigh(self.S)
+        self.Z = (U * s.rsqrt()) @ U.T  # S^{-1/2}
+        self.enuc = mol.nuclear_repulsion()
+        self.timing = dict(j=[], xc=[], linalg=[])
+
+    def coulomb(self, P):
+        """Fitted J[P] (torch, device) and the fitted Coulomb energy 1/2 gamma^T c."""
+        torch = self.torch
+        g = self.jf.gamma(P.cpu().numpy())
+        gt = torch.as_tensor(g, dtype=torch.float64, device=self.dev)
+        c = torch.cholesky_solve(gt[:, None], self.Lm)[:, 0]
+        J = self.jf.coulomb(c.cpu().numpy())
+        return torch.as_tensor(J, dtype=torch.float64, device=self.dev), 0.5 * float(gt @ c)
+
+    def update(self, F):
+        torch = self.torch
+        t0 = time.perf_counter()
+        eps, Cp = torch.linalg.eigh(self.Z.T @ F @ self.Z)
+        C = self.Z @ Cp
+        Co = C[:, :self.nocc]
+        D = Co @ Co.T
+        t1 = time.perf_counter()
+        J, _ = self.coulomb(2.0 * D)
+        t2 = time.perf_counter()
+        D2 = (2.0 * D).cpu().numpy()
+        self.grid.getDensity([D2])
+        self.xc.Evaluate("ev", self.grid)
+        Exc = self.grid.getEnergy()
+        Vxc = torch.as_tensor(self.grid.getFock()[0], dtype=torch.float64, device=self.dev)
+        t3 = time.perf_counter()
+        Fhf = self.Hcore + J
+        Fnew = Fhf + Vxc
+        E = float(torch.sum(D * (self.Hcore + Fhf))) + Exc
+        G = 2.0 * (Fnew @ D @ self.S - self.S @ D @ Fnew)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        self.timing["linalg"].append((t1 - t0) + (t4 - t3))
+        self.timing["j"].append(t2 - t1)
+        self.timing["xc"].append(t3 - t2)
+        return Fnew, E, G, D, eps, Exc
+
+    def run(self, F0=None, adiis_tol=1e-1, tol=1e-6, max_iter=300, log=None):
+        """Returns dict(E total, E_elec, E_xc, iterations, max|G|, history, F, eps). Raises
+        RuntimeError("Convergence failed!") as the reference when a phase runs out of iterations.
+        F0: starting Fock matrix (default Hcore, the core guess)."""
+        torch = self.torch
+        F = self.Hcore.clone() if F0 is None else torch.as_tensor(F0, dtype=torch.float64, device=self.dev)
+        last, hist = run_reference_diis(self.update, F, lambda a, b: float(torch.sum(a * b)),
+                                        lambda G: float(torch.abs(G).max()), adiis_tol, tol, max_iter,
+                                        log, self.enuc)
+        Fnew, E, G, D, eps, Exc = last
+        return dict(E=E + self.enuc, E_elec=E, E_xc=Exc, iterations=len(hist),
+                    diis_error=float(torch.abs(G).max()), history=hist, F=Fnew.cpu().numpy(),
+                    eps=eps.cpu().numpy())
+
+
+def run_reference_diis(update, F, dot, maxabs, adiis_tol=1e-1, tol=1e-6, max_iter=300, log=None, enuc=0.0):
+    """The reference's SCF driver (SP.cpp:73-79): ADIIS until max|G| < adiis_tol, then CDIIS,
+    which takes over ADIIS's history (CDIIS::Steal) and restarts hot from its last entry, until
+    max|G| < tol. update(F) -> (Fnew, E, G, D, ...). Returns (last update's tuple, history of
+    (phase, E, max|G|)). Raises RuntimeError("Convergence failed!") when a phase runs out."""
+    hist = []
+    last = None
+
+    def phase(d, hot):
+        nonlocal F, last
+        for it in range(d.max_iter):
+            if not (it == 0 and hot and d.F):  # a hot restart skips the first computation (DIIS.cpp:116-122)
+                out = update(F)
+                last = out
+                Fnew, E, G, D = out[:4]
+                d.append(Fnew, G, D, E)
+                hist.append((d.kind, E, maxabs(G)))
+                if log:
+                    log("  %s %3d  E %.10f  max|G| %.3e" % (d.kind, len(hist), E + enuc, hist[-1][2]))
+            if maxabs(d.G[-1]) < d.tol:
+                return True
+            if len(d.F) < 2:
+                F = d.F[-1]
+            else:
+                w = d.weights(dot)
+                F = sum(float(wi) * Fi for wi, Fi in zip(w, d.F))
+        return False
+
+    a = DIIS("ADIIS", 20, adiis_tol, max_iter)
+    if not phase(a, False):
+        raise RuntimeError("Convergence failed!")
+    c = DIIS("CDIIS", 20, tol, max_iter)
+    c.steal(a)
+    if not phase(c, True):
+        raise RuntimeError("Convergence failed!")
+    return last, hist

@@ -211,6 +211,27 @@ int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double
 int chx_xc_eval_points_f(const chx_xc* xc, long n, const double* rho, const double* sigma, double* v2rho2,
                          double* v2rhosigma, double* v2sigma2);
 
+/* ---- Coulomb matrix and one-electron integrals for the SCF loop ---------------
+ * The SCF update around the XC path (src/HartreeFockKohnSham/Restricted/SP.cpp:
+ * 43-56) takes J from Int4C2E::ContractInts (src/Integral/Int4C2E.cpp:673-683)
+ * and S, T, V from Int2C1E (src/Integral/Representation.cpp:62-63). This engine
+ * computes them on the device: S, T, V exactly, and J by density fitting in an
+ * auxiliary basis read from $CHINIUM_PATH/BasisSets/<aux>.gbs (e.g. def2-svp-j)
+ * with the Coulomb metric: gamma_P = sum (P|mn) D_mn, (P|Q) c = gamma,
+ * J_mn = sum_P (mn|P) c_P. Orbital shells up to d, auxiliary shells up to g.
+ * Matrices are column-major nbasis x nbasis (reference AO order), host memory;
+ * the metric is naux x naux. The fitting solve itself (a Cholesky factor of the
+ * metric, once) is the caller's. */
+typedef struct chx_jfit chx_jfit;
+int chx_jfit_create(const chx_mol* m, const char* data_root, const char* aux_basis, chx_jfit** out);
+/* auxiliary functions, primitive pairs kept and before screening, auxiliary primitives */
+int chx_jfit_info(const chx_jfit* j, int* naux, long* npairs, long* npairs_unscreened, int* naux_prims);
+int chx_jfit_int1e(chx_jfit* j, double* S, double* T, double* V);
+int chx_jfit_metric(chx_jfit* j, double* Vpq);
+int chx_jfit_gamma(chx_jfit* j, const double* D, double* gamma);
+int chx_jfit_coulomb(chx_jfit* j, const double* cfit, double* J);
+void chx_jfit_destroy(chx_jfit* j);
+
 /* ---- Packs: many molecules per XC build (no reference counterpart) -----------
  * Chinium runs one molecule per process (SelfConsistentField.cpp:27-33). A
  * pack takes grids that are each built, set to CHX_PREC_MIXED, typed and

@@ -21,6 +21,49 @@ def _rotation(a, b, c):
     return Rz @ Rx @ Ry
 
 
+def test_reference_diis_driver():
+    """ADIIS to 1e-1, then CDIIS (taking over ADIIS's history) to 1e-6, as SP.cpp:73-79 runs them:
+    the published STO-3G water energy, and the same fixed point as the plain CDIIS loop for
+    cc-pVDZ, from the core guess."""
+    mol = ca.Molecule([8, 1, 1], H2O_XYZ, "sto-3g")
+    r = RestrictedSCF(mol, 5).run_reference()
+    phases = [h[0] for h in r["history"]]
+    print("STO-3G water RHF (ADIIS -> CDIIS) %.12f in %d iterations (%d ADIIS)" % (
+        r["E"], r["iterations"], phases.count("ADIIS")))
+    assert abs(r["E"] - (-74.942079928192)) <= 1e-9
+    assert r["diis_error"] < 1e-6
+    assert phases[0] == "ADIIS" and phases[-1] == "CDIIS"
+    adiis = [h[2] for h in r["history"] if h[0] == "ADIIS"]
+    assert all(e >= 1e-1 for e in adiis[:-1]) and adiis[-1] < 1e-1
+    mol = ca.Molecule([8, 1, 1], H2O_XYZ, "cc-pvdz")
+    a = RestrictedSCF(mol, 5).run_reference()
+    b = RestrictedSCF(mol, 5).run()
+    print("cc-pVDZ water RHF: reference driver %.12f (%d it), plain CDIIS %.12f (%d it)" % (
+        a["E"], a["iterations"], b["E"], b["iterations"]))
+    assert abs(a["E"] - b["E"]) <= 1e-9
+
+
+def test_adiis_weights_on_the_simplex():
+    """ADIIS weights lie on the simplex and minimise the reference's quadratic model: no vertex
+    or random simplex point does better."""
+    from chinium_amd.scf import DIIS
+    rng = np.random.default_rng(3)
+    d = DIIS("ADIIS")
+    for _ in range(5):
+        A = rng.normal(size=(6, 6))
+        d.append(A + A.T, rng.normal(size=(6, 6)), rng.normal(size=(6, 6)), 0.0)
+    dot = lambda a, b: float(np.sum(a * b))
+    w = d.weights(dot)
+    assert np.all(w >= 0) and abs(w.sum() - 1) <= 1e-12
+    n = len(d.F)
+    Dn, Fn = d.D[-1], d.F[-1]
+    A = np.array([dot(d.D[i] - Dn, Fn) for i in range(n)])
+    B = np.array([[dot(d.D[i] - Dn, d.F[j] - Fn) for j in range(n)] for i in range(n)])
+    f = lambda x: x @ A + 0.5 * x @ B @ x
+    trial = [np.eye(n)[i] for i in range(n)] + [rng.dirichlet(np.ones(n)) for _ in range(200)]
+    assert f(w) <= min(f(x) for x in trial) + 1e-9
+
+
 def test_sto3g_water_rhf():
     mol = ca.Molecule([8, 1, 1], H2O_XYZ, "sto-3g")
     assert mol.nbasis == 7
