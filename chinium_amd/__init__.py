@@ -101,6 +101,7 @@ def lib():
         "chx_selftest": (i, [P(d)]),
         "chx_lebedev_rule": (i, [i, _vp]),
         "chx_xc_eval_points": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "chx_xc_eval_points_noalloc": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "chx_grid_get_density_skeleton": (i, [_vp, _vp, i, i]),
         "chx_grid_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
         "chx_grid_energy_grad": (i, [_vp, _vp]),
@@ -525,10 +526,11 @@ class ExchangeCorrelation:
         self.h = h
         self._load_info()
 
-    def eval_points(self, rho, sigma=None, tau=None):
+    def eval_points(self, rho, sigma=None, tau=None, noalloc=False):
         """exc, v_rho, v_sigma, v_tau at the given points, evaluated on the device. Spin = 2:
-        rho (n x 2), sigma (n x 3) in libxc's polarised layout; returns exc, v_rho (n x 2), v_sigma (n x 3)."""
-        f = lib().chx_xc_eval_points
+        rho (n x 2), sigma (n x 3) in libxc's polarised layout; returns exc, v_rho (n x 2), v_sigma (n x 3).
+        noalloc: through chx_xc_eval_points_noalloc (fixed device staging, at most CHX_NOALLOC_MAX_POINTS)."""
+        f = lib().chx_xc_eval_points_noalloc if noalloc else lib().chx_xc_eval_points
         if self.Spin == 2:
             rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
             n = rho.shape[0]

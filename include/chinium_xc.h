@@ -205,6 +205,18 @@ int chx_lebedev_rule(int npoints, double* xyzw);
  * unused). */
 int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
                        double* exc, double* vrho, double* vsigma, double* vtau);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (serving loops, code under allocation-free policies): the same
+ * evaluation as chx_xc_eval_points, same arguments and results, with its
+ * device staging (inputs, outputs, code list) in a fixed array the library
+ * holds on each device until the process ends (about 5.8 MB) instead of
+ * buffers allocated per call. It calls no hipMalloc / hipFree and creates no
+ * stream (it runs on the null stream). At most CHX_NOALLOC_MAX_POINTS points
+ * per call (CHX_ERR_ARG beyond). Calls that overlap on one device share that
+ * array and race. */
+#define CHX_NOALLOC_MAX_POINTS 65536
+int chx_xc_eval_points_noalloc(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
+                               double* exc, double* vrho, double* vsigma, double* vtau);
 /* Second derivatives at n points, as xc_{lda,gga}_fxc returns them
  * (src/ExchangeCorrelation.cpp:101-104,126-133): v2rho2, v2rhosigma, v2sigma2
  * summed over the codes. LDA/GGA; host arrays; sigma may be NULL (LDA). */
