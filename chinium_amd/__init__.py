@@ -87,6 +87,8 @@ def lib():
         "chx_grid_set_vrho": (i, [_vp, _vp, l]),
         "chx_grid_stream": (i, [_vp, P(_vp)]),
         "chx_grid_set_precision": (i, [_vp, i]),
+        "chx_grid_set_core_radius": (i, [_vp, d]),
+        "chx_grid_core_batches": (i, [_vp, P(l)]),
         "chx_grid_set_screening": (i, [_vp, d]),
         "chx_grid_stats": (i, [_vp, P(l), P(l), P(d), P(d)]),
         "chx_grid_profile": (i, [_vp, i]),
@@ -284,6 +286,15 @@ class Grid:
 
     def setPrecision(self, p):
         _check(lib().chx_grid_set_precision(self.h, int(p)))
+
+    def setCoreRadius(self, r):
+        """Mixed form: batches within r bohr of a heavy nucleus take three-term operands (0: off)."""
+        _check(lib().chx_grid_set_core_radius(self.h, float(r)))
+
+    def coreBatches(self):
+        n = ctypes.c_long()
+        _check(lib().chx_grid_core_batches(self.h, ctypes.byref(n)))
+        return n.value
 
     def getAO(self, derivative=0, output=0):
         _check(lib().chx_grid_get_ao(self.h, int(derivative), int(output)))

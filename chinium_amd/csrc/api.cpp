@@ -137,11 +137,15 @@ void density_from_device(chx_grid* g, const double* dD) {
   const int ng = (int)g->grp_batch.size();
   if (g->precision == CHX_PREC_MIXED) {
     g->mark_begin(kStPermute, &e);
-    launch_split_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dsplit.p);
+    const int ncg = (int)g->core_grps.size();
+    launch_split_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dsplit.p, ncg ? g->d_Ds3.p : nullptr);
     g->mark_end(kStPermute, e);
     g->mark_begin(kStDensity, &e);
-    launch_density_mx(g->stream, 0, ng, g->hmol);
-    if (g->type >= 2) launch_density_mx(g->stream, 4, ng, g->hmol);  // tau partials, A = d_r phi, one launch
+    // meta-GGA: rho, grad rho and the tau partials (A = d_r phi) in one launch that stages D once
+    launch_density_mx(g->stream, g->type >= 2 ? 5 : 0, ng, g->hmol);
+    // core batches' rho and grad rho again with three-term operands (overwriting their partials; tau
+    // keeps two terms: near the nuclei its error is far below rho's effect on E_xc)
+    launch_density_mx_core(g->stream, 0, ncg, g->hmol);
     g->mark_end(kStDensity, e);
     g->have_density = true;
     g->have_eval = false;
@@ -356,6 +360,7 @@ int chx_grid_copy(const chx_grid* s, chx_grid** out) {
     g = new chx_grid;
     g->nthreads = s->nthreads; g->type = s->type; g->precision = s->precision; g->screen_thr = s->screen_thr;
     g->cache_mode = s->cache_mode;
+    g->core_radius = s->core_radius;
     g->mol = s->mol;
     g->nbasis = s->nbasis; g->npad = s->npad; g->ntiles = s->ntiles;
     g->int2ref = s->int2ref; g->ref2int = s->ref2int;
@@ -686,6 +691,21 @@ int chx_grid_set_precision(chx_grid* g, int precision) {
             "precision must be CHX_PREC_FP64 or CHX_PREC_MIXED");
     if (g->precision != precision) g->ao_ready = false;  // the AO tables are laid out per form
     g->precision = precision;
+  });
+}
+
+int chx_grid_set_core_radius(chx_grid* g, double radius) {
+  return guard([&] {
+    require(g && radius >= 0, CHX_ERR_ARG, "core radius must be >= 0");
+    if (g->core_radius != radius) g->ao_ready = false;  // the plan holds the core batches
+    g->core_radius = radius;
+  });
+}
+
+int chx_grid_core_batches(const chx_grid* g, long* n) {
+  return guard([&] {
+    require(g && n, CHX_ERR_ARG, "bad arguments");
+    *n = g->precision == CHX_PREC_MIXED && g->ao_ready ? g->ncore_batches : 0;
   });
 }
 

@@ -220,8 +220,33 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
   keep.reserve(G);
   for (long i = 0; i < G; ++i)
     if (pw[i] != 0.0) keep.push_back(i);
+  // The points within kCoreRadius of a nucleus with Z > 2 (atomic cores, where
+  // the density is largest) are batched apart, per atom, so that the mixed
+  // form's three-term core treatment covers few batches; the rest is bisected
+  // as one set. Only the batch membership depends on this.
   std::vector<std::pair<long, long>> parts;
-  if (!keep.empty()) bisect(px, py, pz, keep, 0, (long)keep.size(), parts);
+  {
+    const double rc2 = kCoreRadius * kCoreRadius;
+    std::vector<std::vector<long>> core(nat);
+    std::vector<long> rest;
+    rest.reserve(keep.size());
+    for (long i : keep) {
+      const int a = owner[i];
+      const double dx = px[i] - m.xyz[3 * a], dy = py[i] - m.xyz[3 * a + 1], dz = pz[i] - m.xyz[3 * a + 2];
+      if (m.Z[a] > 2 && dx * dx + dy * dy + dz * dz < rc2) core[a].push_back(i);
+      else rest.push_back(i);
+    }
+    keep.clear();
+    std::vector<long> seg = {0};
+    for (int a = 0; a < nat; ++a) {
+      keep.insert(keep.end(), core[a].begin(), core[a].end());
+      seg.push_back((long)keep.size());
+    }
+    keep.insert(keep.end(), rest.begin(), rest.end());
+    seg.push_back((long)keep.size());
+    for (size_t k = 0; k + 1 < seg.size(); ++k)
+      if (seg[k + 1] > seg[k]) bisect(px, py, pz, keep, seg[k], seg[k + 1], parts);
+  }
   g->npoints = (long)keep.size();
   g->nbatches = (int)parts.size();
   const size_t nslots = (size_t)g->nbatches * kBatch;
@@ -325,8 +350,9 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
 // chunks of `chunk` consecutive batches (the batch order is spatial, so a
 // chunk covers a compact region), and the union of the alive tiles of a
 // chunk's batches is cut into groups of 8 tiles in tile order. An item is an
-// ordered pair of groups (I, J) of one chunk: a 256 x 256 block of M at those
-// tiles' rows and columns, accumulated over the chunk's batches. Tiles of a
+// pair of groups I <= J of one chunk: the 256 x 256 blocks of M at those
+// tiles' rows and columns, (I, J) and for I < J (J, I), accumulated over the
+// chunk's batches. Tiles of a
 // group that are not alive in a batch read the all-zero slot in the table,
 // and the kernel skips their loads and products, so no MFMA multiplies a
 // tile-pair block that the screening dropped. Items are chunk-major: the
@@ -366,8 +392,8 @@ static void build_union_plan(chx_grid* g, int chunk) {
       useful += n * n;
     }
     dense += (double)(b1 - b0) * nu * nu;
-    for (int I = 0; I < ng; ++I)
-      for (int J = 0; J < ng; ++J) g->cwork.push_back(make_int4(I, J, ci, 0));
+    for (int I = 0; I < ng; ++I)  // an item covers the blocks (I, J) and (J, I)
+      for (int J = I; J < ng; ++J) g->cwork.push_back(make_int4(I, J, ci, 0));
   }
   g->vxc_exec_frac = dense > 0 ? useful / dense : 1.0;
 }
@@ -596,8 +622,46 @@ void grid_plan(chx_grid* g) {
       // component count of each build's potential
       g->d_bmx.alloc((nslot + 1) * (size_t)nT * kMxComp);
       g->d_Dsplit.alloc(2 * (size_t)g->npad * g->npad);
+      // core batches: any real point within core_radius of a nucleus with Z > 2
+      std::vector<int> core_off(nb, -1), slot_core(nslot, -1);
+      g->core_grps.clear();
+      g->ncore_batches = 0;
+      int ncore_slots = 0;
+      if (g->core_radius > 0) {
+        const double r2c = g->core_radius * g->core_radius;
+        std::vector<int> heavy;
+        for (int a = 0; a < g->mol.natoms; ++a)
+          if (g->mol.Z[a] > 2) heavy.push_back(a);
+        for (int b = 0; b < nb && !heavy.empty(); ++b) {
+          bool core = false;
+          for (int q = 0; q < g->batch_n[b] && !core; ++q) {
+            const size_t p = (size_t)b * kBatch + q;
+            for (int a : heavy) {
+              const double dx = g->hx[p] - g->mol.xyz[3 * a], dy = g->hy[p] - g->mol.xyz[3 * a + 1],
+                           dz = g->hz[p] - g->mol.xyz[3 * a + 2];
+              if (dx * dx + dy * dy + dz * dz < r2c) {
+                core = true;
+                break;
+              }
+            }
+          }
+          if (!core) continue;
+          ++g->ncore_batches;
+          core_off[b] = ncore_slots;
+          for (int it = g->bt_off[b]; it < g->bt_off[b + 1]; ++it) slot_core[it] = ncore_slots++;
+          for (int k = g->grp_off[b]; k < g->grp_off[b + 1]; ++k) g->core_grps.push_back(k);
+        }
+      }
+      static const std::vector<int> neg1{-1}, zero1{0};  // placeholders outlive the asynchronous copies
+      g->d_core_off.upload(core_off.empty() ? neg1 : core_off, st);
+      g->d_slot_core.upload(slot_core.empty() ? neg1 : slot_core, st);
+      g->d_core_grps.upload(g->core_grps.empty() ? zero1 : g->core_grps, st);
+      g->d_cmx3.alloc(std::max<size_t>(1, (size_t)ncore_slots * nT * kBatch * kTile));
+      g->d_Ds3.alloc(ncore_slots ? (size_t)g->npad * g->npad : 1);
       launch_ao_cache_mx(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p,
-                         g->d_z.p, g->view(), nT, g->mx_sstride, g->d_cmx.p);
+                         g->d_z.p, g->view(), nT, g->mx_sstride, g->d_cmx.p, ncore_slots ? g->d_slot_core.p : nullptr,
+                         g->d_cmx3.p);
+      CHX_HIP(hipStreamSynchronize(st));  // the host vectors above go out of scope
     } else {
       g->d_cmx.release();
       g->d_bmx.release();
@@ -647,6 +711,10 @@ void grid_plan(chx_grid* g) {
     m.coef = g->d_coef.p; m.bt = g->d_bmx.p;
     m.slot_of = g->d_slot_of.p;
     m.Mint = g->d_Mint.p;
+    m.cache3 = g->d_cmx3.p;
+    m.Ds3 = g->d_Ds3.p;
+    m.core_off = g->d_core_off.p;
+    m.core_grps = g->d_core_grps.p;
     upload_mx_plan(g, st);
   }
   CHX_HIP(hipStreamSynchronize(st));

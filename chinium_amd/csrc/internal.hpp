@@ -21,6 +21,7 @@ constexpr int kPadAO = 128;    // nbasis is padded to a multiple of this (4 tile
 constexpr int kCacheSuper = 4; // tiles per Vxc super-tile in the cached path
 constexpr int kCacheChunk = 16;  // batches per cached-path Vxc work item (FP64 form default)
 constexpr int kCoef = 5;       // Vxc integrand coefficients per point: a, c_x, c_y, c_z, t
+constexpr double kCoreRadius = 0.5;  // bohr: points this close to a nucleus with Z > 2 are batched apart
 
 struct Error : std::runtime_error {
   int code;
@@ -225,6 +226,13 @@ struct MolDev {
   const int4* vchunk;
   const int* vtab;
   const int* vtile;
+  // core batches (points within the core radius of a nucleus with Z > 2): their
+  // density runs with operands split in three bf16 terms; cache3 holds the
+  // third term of their slots' AOg blocks [core slot][c][g][mu], Ds3 that of D,
+  // core_off the first core slot of each batch (-1: not a core batch),
+  // core_grps the density groups of the core batches
+  const unsigned short *cache3, *Ds3;
+  const int *core_off, *core_grps;
 };
 constexpr int kMxMaxTiles = 512;  // alive tiles per batch the mixed density kernel stages (npad <= 16384)
 constexpr int kMxMaxChunk = 128;  // batches per Vxc work item in the mixed form
@@ -241,16 +249,22 @@ void launch_xgrad_epi(hipStream_t s, int nslots, const int* slot_batch, const in
 // hi/lo pairs, split-BF16 MFMA contractions with FP32 accumulation, FP32
 // recomputation of the AO values outside the MFMA operands.
 constexpr int kMxComp = 2 * kTile * kBatch;  // elements per component and layout of a slot
+// slot_core / cache3 (may be null): the third bf16 term of the AOg block of each
+// slot with slot_core >= 0, at cache3 + slot_core * nT * kBatch * kTile
 void launch_ao_cache_mx(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* tile_nao,
                         const double* bx, const double* by, const double* bz, BasisView bv, int nT, int sstride,
-                        unsigned short* cache);
-void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds);
+                        unsigned short* cache, const int* slot_core, unsigned short* cache3);
+// Ds3 (may be null): the third bf16 term of D, D - hi - lo
+void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds,
+                    unsigned short* Ds3);
 void launch_bf16_check(hipStream_t s, unsigned n, unsigned* bad);
 void launch_fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v);
 // One launch over the grid `one` (descriptor by value): density work groups,
 // B^T slot blocks, Vxc work items. Density mode 0: rho, grad rho; 4: the three
 // tau partials in one launch.
 void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one);
+// the core groups again with three-term operands: mode 0 (rho, grad rho) or r = 1..3 (tau partial r)
+void launch_density_mx_core(hipStream_t s, int mode, int ncore_groups, const MolDev& one);
 // B^T of every slot plus the all-zero slot (index nslots) for ncomp components
 void launch_make_b_mx(hipStream_t s, int ncomp, const MolDev& one);
 void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one);
@@ -348,6 +362,12 @@ struct chx_grid {
   chx::DevBuf<int4> d_vchunk;
   chx::DevBuf<int> d_vtab, d_vtile;
   double vxc_exec_frac = 1.0;          // useful / executed tile-pair products of the mixed Vxc plan
+  // mixed form: core batches (density with three-term operands)
+  double core_radius = chx::kCoreRadius;  // bohr; 0 turns the core treatment off
+  int ncore_batches = 0;
+  std::vector<int> core_grps;
+  chx::DevBuf<unsigned short> d_cmx3, d_Ds3;
+  chx::DevBuf<int> d_core_off, d_core_grps, d_slot_core;
   chx::DevBuf<double> d_cache, d_bmat, d_Mint;
   chx::DevBuf<int> d_slot_batch, d_slot_tile, d_slot_of, d_tile_nao, d_grp_slots, d_cpair_batches;
   chx::DevBuf<int4> d_cwork;

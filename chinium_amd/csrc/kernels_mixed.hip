@@ -98,7 +98,8 @@ __device__ __forceinline__ float pair_at(const u16* p, int off) { return bf2f(p[
 __global__ __launch_bounds__(256) void k_ao_cache_mx(const int* __restrict__ slot_batch, const int* __restrict__ slot_tile,
                                                      const int* __restrict__ tile_nao, const double* __restrict__ bx,
                                                      const double* __restrict__ by, const double* __restrict__ bz,
-                                                     BasisView bv, int nT, int sstride, u16* __restrict__ cache) {
+                                                     BasisView bv, int nT, int sstride, u16* __restrict__ cache,
+                                                     const int* __restrict__ slot_core, u16* __restrict__ cache3) {
   __shared__ float E[4][kBatch][kTile + 1];
   __shared__ double px[kBatch], py[kBatch], pz[kBatch];
   const int s = blockIdx.x, tid = threadIdx.x;
@@ -144,21 +145,32 @@ __global__ __launch_bounds__(256) void k_ao_cache_mx(const int* __restrict__ slo
     outt[c * kMxT + mu * kBatch + g] = hi;
     outt[c * kMxT + kTile * kBatch + mu * kBatch + g] = lo;
   }
+  const int cs = slot_core ? slot_core[s] : -1;
+  if (cs >= 0) {  // core slot: the third term x - hi - lo, AOg order without the hl plane
+    u16* o3 = cache3 + (size_t)cs * nT * kBatch * kTile;
+    for (int e = tid; e < nT * kBatch * kTile; e += 256) {
+      const int c = e / (kBatch * kTile), g = (e / kTile) % kBatch, mu = e % kTile;
+      const float x = E[c][g][mu];
+      u16 hi, lo;
+      split_bf(x, hi, lo);
+      o3[e] = f2bf((x - bf2f(hi)) - bf2f(lo));
+    }
+  }
 }
 
 void launch_ao_cache_mx(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* tile_nao,
                         const double* bx, const double* by, const double* bz, BasisView bv, int nT, int sstride,
-                        unsigned short* cache) {
+                        unsigned short* cache, const int* slot_core, unsigned short* cache3) {
   if (nslots == 0) return;
   hipLaunchKernelGGL(k_ao_cache_mx, dim3(nslots), dim3(256), 0, s, slot_batch, slot_tile, tile_nao, bx, by, bz, bv, nT,
-                     sstride, cache);
+                     sstride, cache, slot_core, cache3);
   CHX_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
 // D (reference order, column-major) -> internal order, split: Ds[hl][mu][nu].
 __global__ void k_split_d(int n, int npad, const double* __restrict__ D, const int* __restrict__ int2ref,
-                          u16* __restrict__ Ds) {
+                          u16* __restrict__ Ds, u16* __restrict__ Ds3) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long np2 = (long)npad * npad;
   if (idx >= np2) return;
@@ -166,14 +178,17 @@ __global__ void k_split_d(int n, int npad, const double* __restrict__ D, const i
   const int rr = int2ref[r], cc = int2ref[c];
   const double v = (rr >= 0 && cc >= 0) ? D[(size_t)cc * n + rr] : 0.0;
   u16 hi, lo;
-  split_bf((float)v, hi, lo);
+  const float x = (float)v;
+  split_bf(x, hi, lo);
   Ds[idx] = hi;
   Ds[np2 + idx] = lo;
+  if (Ds3) Ds3[idx] = f2bf((x - bf2f(hi)) - bf2f(lo));
 }
 
-void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds) {
+void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds,
+                    unsigned short* Ds3) {
   const long tot = (long)npad * npad;
-  hipLaunchKernelGGL(k_split_d, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, npad, D, int2ref, Ds);
+  hipLaunchKernelGGL(k_split_d, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, npad, D, int2ref, Ds, Ds3);
   CHX_HIP(hipGetLastError());
 }
 
@@ -364,10 +379,13 @@ struct DensEpi {
 constexpr int kDxFrag = 64;  // u32x4 per fragment
 constexpr int kDxA = 8;      // A fragments per step: [row block][hl]
 constexpr int kDxB = 16;     // D fragments per step: [column block][hl]
+constexpr int kDxA3 = 4;     // P3: third-term A fragments per step: [row block]
+constexpr int kDxB3 = 8;     // P3: third-term D fragments per step: [column block]
+template <bool P3>
 struct DensMxSmem {
   union {
-    u32x4 F[(kDxA + kDxB) * kDxFrag];      // one K step's fragments
-    float X[kGroup * kTile][kBatch + 1];   // epilogue: X^T [nu][g]
+    u32x4 F[(kDxA + kDxB + (P3 ? kDxA3 + kDxB3 : 0)) * kDxFrag];  // one K step's fragments
+    float X[kGroup * kTile][kBatch + 1];                          // epilogue: X^T [nu][g]
   };
   float red[kGroup][4][kBatch];
   double px[kBatch], py[kBatch], pz[kBatch];
@@ -376,18 +394,22 @@ struct DensMxSmem {
 
 // MODE 0: A = phi; epilogue rho and grad rho / 2 (4 components into part).
 // MODE r = 1..3 (A = d phi / d x_r, epilogue sum X_r d_r phi, the tau partial
-// of direction r) is the single-direction form of k_density_mx_tau below,
-// which computes all three in one launch and is the one launched.
+// of direction r) is the single-direction form of k_density_mx_tau below.
+// P3 (core batches only): each operand carries a third bf16 term (x - hi - lo,
+// 24 significant bits in all), A's from the core slots' side cache and D's
+// from Ds3, and a product takes the six terms hi hi, hi lo, lo hi, hi lo2,
+// lo2 hi and lo lo: FP32-level operands where the density is largest.
+// Blocks walk the core group list; the others run the two-term kernel.
 // The grid's descriptor comes by value (a kernel argument: every pointer in it
 // is known to address global memory, and uniform table reads become scalar
 // loads; a descriptor read from device memory would leave them flat).
-template <int MODE>
-__global__ __launch_bounds__(256, 4) void k_density_mx(MolDev m) {
+template <int MODE, bool P3>
+__global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
   using lds_ptr = __attribute__((address_space(3))) void*;
   constexpr int NOUT = MODE == 0 ? 4 : 1;
   constexpr int grp_stride = (MODE == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
-  __shared__ DensMxSmem sm;
-  const int grp = xcd_remap(blockIdx.x, gridDim.x);
+  __shared__ DensMxSmem<P3> sm;
+  const int grp = P3 ? m.core_grps[xcd_remap(blockIdx.x, gridDim.x)] : xcd_remap(blockIdx.x, gridDim.x);
   const u16* __restrict__ cache = m.cache + (size_t)MODE * kMxT;
   const u16* __restrict__ Ds = m.Ds;
   const size_t sstride = (size_t)m.sstride;
@@ -417,10 +439,11 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev m) {
   // 64 bytes; DMA lane i fetches row i >> 2, 16-byte chunk (i & 3) ^ sw(row)
   // (four neighbouring lanes read one contiguous 64-byte run: coalesced), and
   // the MFMA lane that needs (row, chunk) reads it back at frag_idx(row, chunk)
-  // (the XOR keeps those LDS reads free of bank conflicts).
+  // (the XOR keeps those LDS reads free of bank conflicts). P3 adds the third
+  // terms: A3 row block = wave, D3 column blocks wave and wave + 4.
   const int drow = lane >> 2, dch = (lane & 3) ^ ((drow >> 2) & 3);
   const int fidx = 4 * lrow + (lk ^ ((lrow >> 2) & 3));  // this lane's MFMA operand in a fragment
-  int offa[2], offb[4];
+  int offa[2], offb[4], offb3[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int f = wave + 4 * j;
@@ -432,6 +455,18 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev m) {
     const int t = q == 1 ? jt[1] : q == 2 ? jt[2] : q == 3 ? jt[3] : jt[0];
     offb[j] = (f & 1) * npad * npad + ((q < nt ? t : jt[0]) * kTile + 16 * (c & 1) + drow) * npad + 8 * dch;
   }
+  const u16* __restrict__ c3 = nullptr;
+  if (P3) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = wave + 4 * j, q = c >> 1;
+      const int t = q == 1 ? jt[1] : q == 2 ? jt[2] : q == 3 ? jt[3] : jt[0];
+      offb3[j] = ((q < nt ? t : jt[0]) * kTile + 16 * (c & 1) + drow) * npad + 8 * dch;
+    }
+    // this batch's core slots in the side cache: [core slot][c][g][mu]
+    c3 = m.cache3 + ((size_t)m.core_off[b] * (m.toff / 2) + (size_t)MODE * kBatch * kTile) +
+         (16 * wave + drow) * kTile + 8 * dch;
+  }
   __syncthreads();
   auto dma = [&](int k) {
     const u16* a = cache + (size_t)(it0 + k) * sstride;
@@ -442,6 +477,14 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev m) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       __builtin_amdgcn_global_load_lds(Ds + offb[j] + ti, (lds_ptr)(sm.F + (kDxA + wave + 4 * j) * kDxFrag), 16, 0, 0);
+    if (P3) {
+      __builtin_amdgcn_global_load_lds(c3 + (size_t)k * (m.toff / 2),
+                                       (lds_ptr)(sm.F + (kDxA + kDxB + wave) * kDxFrag), 16, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        __builtin_amdgcn_global_load_lds(m.Ds3 + offb3[j] + ti,
+                                         (lds_ptr)(sm.F + (kDxA + kDxB + kDxA3 + wave + 4 * j) * kDxFrag), 16, 0, 0);
+    }
   };
 
   f32x4 acc[2][4];
@@ -454,7 +497,7 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev m) {
   for (int k = 0; k < nk; ++k) {
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
     __syncthreads();                     // ... and every other wave's
-    u32x4 a[2][2], bq[4][2];
+    u32x4 a[2][2], bq[4][2], a3[2], b3[4];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -463,13 +506,27 @@ __global__ __launch_bounds__(256, 4) void k_density_mx(MolDev m) {
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int h = 0; h < 2; ++h) bq[c][h] = sm.F[(kDxA + (4 * wc + c) * 2 + h) * kDxFrag + fidx];
+    if (P3) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) a3[r] = sm.F[(kDxA + kDxB + 2 * wr + r) * kDxFrag + fidx];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b3[c] = sm.F[(kDxA + kDxB + kDxA3 + 4 * wc + c) * kDxFrag + fidx];
+    }
     __syncthreads();  // every wave holds its fragments: the buffer is free
     if (k + 1 < nk) dma(k + 1);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       if (c < ncb) {
 #pragma unroll
-        for (int r = 0; r < 2; ++r) acc[r][c] = mfma3(a[r][0], a[r][1], bq[c][0], bq[c][1], acc[r][c]);
+        for (int r = 0; r < 2; ++r) {
+          f32x4 v = acc[r][c];
+          if (P3) {  // the small terms first
+            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r][1]), as_bf(bq[c][1]), v, 0, 0, 0);
+            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a3[r]), as_bf(bq[c][0]), v, 0, 0, 0);
+            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r][0]), as_bf(b3[c]), v, 0, 0, 0);
+          }
+          acc[r][c] = mfma3(a[r][0], a[r][1], bq[c][0], bq[c][1], v);
+        }
       }
   }
 
@@ -791,10 +848,23 @@ __global__ __launch_bounds__(256, 2) void k_density_mx_mgga(MolDev m) {
   }
 }
 
+void launch_density_mx_core(hipStream_t s, int mode, int ncore_groups, const MolDev& one) {
+  if (ncore_groups == 0) return;
+  const dim3 g(ncore_groups), b(256);
+  switch (mode) {
+    case 0: hipLaunchKernelGGL((k_density_mx<0, true>), g, b, 0, s, one); break;
+    case 1: hipLaunchKernelGGL((k_density_mx<1, true>), g, b, 0, s, one); break;
+    case 2: hipLaunchKernelGGL((k_density_mx<2, true>), g, b, 0, s, one); break;
+    case 3: hipLaunchKernelGGL((k_density_mx<3, true>), g, b, 0, s, one); break;
+    default: throw Error(CHX_ERR_ARG, "core density mode must be 0..3");
+  }
+  CHX_HIP(hipGetLastError());
+}
+
 void launch_density_mx(hipStream_t s, int mode, int nitems, const MolDev& one) {
   if (nitems == 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL((k_density_mx<0>), dim3(nitems), dim3(256), 0, s, one);
+    hipLaunchKernelGGL((k_density_mx<0, false>), dim3(nitems), dim3(256), 0, s, one);
   else if (mode == 4)  // the three tau partials in one launch
     hipLaunchKernelGGL(k_density_mx_tau, dim3(nitems), dim3(256), 0, s, one);
   else if (mode == 5)  // rho, grad rho and the tau partials in one launch
@@ -1011,21 +1081,25 @@ __global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev m) {
 }
 
 // Vxc for large bases: M_IJ += Phi_I^T B_J over 256 x 256 blocks whose rows
-// and columns are groups of 8 tiles of a batch chunk's alive-tile union (the
-// item is (group I, group J, chunk)), so every operand byte staged in LDS
-// feeds twice the MFMAs of the 128 x 128 kernel (64 instead of 32 FLOP per
-// byte). A K step is 32 points of one component: Phi^T_I and B^T_J, 256 rows
-// each, hi and lo, 64 fragments of 1 KB brought in by the waves' DMA (same
-// coalesced lane order and swizzle). 8 waves in a 2 x 4 grid, each owning a
-// 128 x 64 block of M (up to 96 MFMAs per step). Per batch a 16-bit mask
-// (I's 8 tiles, then J's) marks the tiles alive in it: a dead tile's
-// fragments are not loaded and the products that involve it are skipped, so
-// the MFMAs follow each batch's screening. Same one-buffer step and the same
-// symmetric tau handling as k_vxc_mx.
+// and columns are groups of 8 tiles of a batch chunk's alive-tile union, so
+// every operand byte staged in LDS feeds twice the MFMAs of the 128 x 128
+// kernel (64 instead of 32 FLOP per byte). An item is a pair of groups I <= J
+// of one chunk and covers both blocks: first M_IJ (the tau part, symmetric,
+// taken twice for I < J and once for I = J, then the phi part), then for
+// I < J the mirror block M_JI (phi part only), so every item does about the
+// same work per batch and the items that run together on an XCD stream the
+// chunk's slots in step. A K step is 32 points of one component: Phi^T of the
+// row group and B^T of the column group, 256 rows each, hi and lo, 64
+// fragments of 1 KB brought in by the waves' DMA (same coalesced lane order
+// and swizzle). 8 waves in a 2 x 4 grid, each owning a 128 x 64 block (up to
+// 96 MFMAs per step). Per batch a 16-bit mask (I's 8 tiles, then J's) marks
+// the tiles alive in it: a dead tile's fragments are not loaded and the
+// products that involve it are skipped, so the MFMAs follow each batch's
+// screening.
 constexpr int kMxSupW = 8;   // tiles per group (256 AOs)
 constexpr int kVxAW = 32;    // Phi^T fragments per step: [row block (16)][hl]
 struct VxcMxWideSmem {
-  u32x4 F[2 * kVxAW * kDxFrag];  // one K step's fragments: Phi^T_I, then B^T_J
+  u32x4 F[2 * kVxAW * kDxFrag];  // one K step's fragments: Phi^T of the row group, then B^T of the column group
   int sl[kMxMaxChunk][16];       // per batch: slots of I's 8 tiles, then J's
   unsigned mk[kMxMaxChunk];      // per batch: alive bits of those 16 tiles
   int tile[16];                  // I's tiles, then J's (-1: padding)
@@ -1037,7 +1111,7 @@ __global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
   __shared__ VxcMxWideSmem sm;
   const int item = xcd_remap(blockIdx.x, gridDim.x);
   const int4 wk = m.cwork[item];
-  const int I = wk.x, J = wk.y;
+  const int I = wk.x, J = wk.y;    // I <= J
   const int4 ch = m.vchunk[wk.z];  // (table offset, batches, table width, tile-list offset)
   const int nbat = ch.y;
   const int zero = m.nslots;       // the all-zero slot: tile not alive in that batch
@@ -1068,34 +1142,42 @@ __global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lrow = lane & 15, lk = lane >> 4;
   const int wr = wave & 1, wc = wave >> 1;
-  // The wave's fragments are f = wave + 8 j, j = 0..3, of Phi^T_I and of B^T_J:
-  // hl f & 1 = wave & 1, rows 16 ((wave >> 1) & 1)..+15 of tile (wave >> 2) + 2 j.
+  // The wave's fragments are f = wave + 8 j, j = 0..3, of the row group's Phi^T
+  // and of the column group's B^T: hl f & 1 = wave & 1, rows
+  // 16 ((wave >> 1) & 1)..+15 of tile (wave >> 2) + 2 j.
   const int drow = lane >> 2, dch = (lane & 3) ^ ((drow >> 2) & 3);
   const int fidx = 4 * lrow + (lk ^ ((lrow >> 2) & 3));
   const int off = (wave & 1) * (kTile * kBatch) + (16 * ((wave >> 1) & 1) + drow) * kBatch + 8 * dch;
   const int t0 = wave >> 2;
   __syncthreads();
-  const int ntau = (NC == 4 && I <= J) ? 6 * nbat : 0;
-  const int nstage = ntau + 2 * nbat;
-  // K step k -> batch of the item: tau steps (it, h, c = 1..3) first, then (it, h, c = 0)
-  auto step_batch = [&](int k) { return k < ntau ? k / 6 : (k - ntau) >> 1; };
-  auto dma = [&](int k) {
-    int it, h, c;
+  const int ntau = NC == 4 ? 6 * nbat : 0;  // tau steps of M_IJ, first
+  const int kmirror = ntau + 2 * nbat;      // first step of M_JI (I < J)
+  const int nstage = kmirror + (I < J ? 2 * nbat : 0);
+  // K step k -> (batch, half of its points, component, mirror block?)
+  auto decode = [&](int k, int& it, int& h, int& c, bool& mir) {
+    mir = k >= kmirror;
     if (k < ntau) {
       it = k / 6; h = (k / 3) % 2; c = 1 + k % 3;
     } else {
-      it = (k - ntau) >> 1; h = (k - ntau) & 1; c = 0;
+      const int q = k - (mir ? kmirror : ntau);
+      it = q >> 1; h = q & 1; c = 0;
     }
+  };
+  auto dma = [&](int k) {
+    int it, h, c;
+    bool mir;
+    decode(k, it, h, c, mir);
+    const int ra = mir ? 8 : 0, cb = mir ? 0 : 8;  // table columns of the row and column groups
     const unsigned mk = __builtin_amdgcn_readfirstlane(sm.mk[it]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int t = t0 + 2 * j;
-      if ((mk >> t) & 1u) {
-        const u16* pa = cache + (size_t)sm.sl[it][t] * sstride + c * kMxT + h * 32;
+      if ((mk >> (ra + t)) & 1u) {
+        const u16* pa = cache + (size_t)sm.sl[it][ra + t] * sstride + c * kMxT + h * 32;
         __builtin_amdgcn_global_load_lds(pa + off, (lds_ptr)(sm.F + (wave + 8 * j) * kDxFrag), 16, 0, 0);
       }
-      if ((mk >> (8 + t)) & 1u) {
-        const u16* pb = bt + ((size_t)sm.sl[it][8 + t] * NC + c) * kMxT + h * 32;
+      if ((mk >> (cb + t)) & 1u) {
+        const u16* pb = bt + ((size_t)sm.sl[it][cb + t] * NC + c) * kMxT + h * 32;
         __builtin_amdgcn_global_load_lds(pb + off, (lds_ptr)(sm.F + (kVxAW + wave + 8 * j) * kDxFrag), 16, 0, 0);
       }
     }
@@ -1105,54 +1187,70 @@ __global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
   for (int r = 0; r < 8; ++r)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // one FP64 atomic add per element of the wave's 128 x 64 block; rows from
+  // table columns ro.., columns from co..
+  auto flush = [&](int ro, int co) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int ti = sm.tile[ro + 4 * wr + (r >> 1)];
+      if (ti < 0) continue;
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const int tj = sm.tile[co + 2 * wc + (cb >> 1)];
+        if (tj < 0) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int mu = ti * kTile + 16 * (r & 1) + 4 * lk + i;
+          const int nu = tj * kTile + 16 * (cb & 1) + lrow;
+          unsafeAtomicAdd(&Mint[(size_t)mu * npad + nu], (double)acc[r][cb][i]);
+        }
+      }
+    }
+  };
   dma(0);
   for (int k = 0; k < nstage; ++k) {
-    if (NC == 4 && k == ntau && I < J) {
+    if (NC == 4 && k == ntau && I < J) {  // 2 M_tau: the item's (I, J) block and, by symmetry, (J, I)
 #pragma unroll
       for (int r = 0; r < 8; ++r)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[r][c] *= 2.f;
     }
-    // this wave's row tiles 4 wr..4 wr+3 (bits 0..3 of rmk) and column tiles 2 wc, 2 wc+1 (cmk) alive in the step's batch
-    const unsigned mk = __builtin_amdgcn_readfirstlane(sm.mk[step_batch(k)]);
-    const unsigned rmk = (mk >> (4 * wr)) & 15u, cmk = (mk >> (8 + 2 * wc)) & 3u;
+    if (k == kmirror) {  // M_IJ done: write it and start M_JI
+      flush(0, 8);
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    int it, h, c;
+    bool mir;
+    decode(k, it, h, c, mir);
+    // this wave's row tiles 4 wr..4 wr+3 (bits of rmk) and column tiles 2 wc, 2 wc+1 (cmk) alive in the batch
+    const unsigned mk = __builtin_amdgcn_readfirstlane(sm.mk[it]);
+    const unsigned rmk = (mk >> ((mir ? 8 : 0) + 4 * wr)) & 15u, cmk = (mk >> ((mir ? 0 : 8) + 2 * wc)) & 3u;
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
     __syncthreads();                     // ... and every other wave's
     u32x4 a[8][2], bq[4][2];
 #pragma unroll
     for (int r = 0; r < 8; ++r)
 #pragma unroll
-      for (int h = 0; h < 2; ++h) a[r][h] = sm.F[((8 * wr + r) * 2 + h) * kDxFrag + fidx];
+      for (int hh = 0; hh < 2; ++hh) a[r][hh] = sm.F[((8 * wr + r) * 2 + hh) * kDxFrag + fidx];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int h = 0; h < 2; ++h) bq[r][h] = sm.F[(kVxAW + (4 * wc + r) * 2 + h) * kDxFrag + fidx];
+      for (int hh = 0; hh < 2; ++hh) bq[r][hh] = sm.F[(kVxAW + (4 * wc + r) * 2 + hh) * kDxFrag + fidx];
     __syncthreads();  // every wave holds its fragments: the buffer is free
     if (k + 1 < nstage) dma(k + 1);
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if ((cmk >> (c >> 1)) & 1u) {
+    for (int cc = 0; cc < 4; ++cc)
+      if ((cmk >> (cc >> 1)) & 1u) {
 #pragma unroll
         for (int r = 0; r < 8; ++r)
-          if ((rmk >> (r >> 1)) & 1u) acc[r][c] = mfma3(a[r][0], a[r][1], bq[c][0], bq[c][1], acc[r][c]);
+          if ((rmk >> (r >> 1)) & 1u) acc[r][cc] = mfma3(a[r][0], a[r][1], bq[cc][0], bq[cc][1], acc[r][cc]);
       }
   }
-#pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int ti = sm.tile[4 * wr + (r >> 1)];
-    if (ti < 0) continue;
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-      const int tj = sm.tile[8 + 2 * wc + (cb >> 1)];
-      if (tj < 0) continue;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int mu = ti * kTile + 16 * (r & 1) + 4 * lk + i;
-        const int nu = tj * kTile + 16 * (cb & 1) + lrow;
-        unsafeAtomicAdd(&Mint[(size_t)mu * npad + nu], (double)acc[r][cb][i]);
-      }
-    }
-  }
+  if (I < J) flush(8, 0);
+  else flush(0, 8);
 }
 
 void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one) {

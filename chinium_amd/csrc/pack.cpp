@@ -132,14 +132,16 @@ void enqueue_lane(chx_pack* p, int m0, int m1, hipStream_t s, hipEvent_t after, 
                   const double* const* D, double* const* V) {
   for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
-    launch_split_d(s, g->nbasis, g->npad, D[m], g->d_int2ref.p, g->d_Dsplit.p);
+    launch_split_d(s, g->nbasis, g->npad, D[m], g->d_int2ref.p, g->d_Dsplit.p,
+                   g->core_grps.empty() ? nullptr : g->d_Ds3.p);
     launch_fill(s, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
   }
   if (after) CHX_HIP(hipStreamWaitEvent(s, after, 0));
   for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
-    launch_density_mx(s, 0, (int)g->grp_batch.size(), g->hmol);
-    if (p->type >= 2) launch_density_mx(s, 4, (int)g->grp_batch.size(), g->hmol);
+    launch_density_mx(s, p->type >= 2 ? 5 : 0, (int)g->grp_batch.size(), g->hmol);
+    // core batches' rho and grad rho again with three-term operands
+    launch_density_mx_core(s, 0, (int)g->core_grps.size(), g->hmol);
   }
   if (dens_done) CHX_HIP(hipEventRecord(dens_done, s));
   const int nc = p->family >= 2 ? 4 : 1;

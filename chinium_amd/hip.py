@@ -78,5 +78,10 @@ class Event:
             _hip.hipEventDestroy(self.e)
 
 
+def copy_device(dst, src, nbytes):
+    """Device-to-device copy of nbytes (the runtime's copy kernel): a known-size streaming probe."""
+    _ok(hip().hipMemcpy(dst, src, int(nbytes), 3), "hipMemcpy D2D")
+
+
 def device_synchronize():
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")

@@ -199,21 +199,38 @@ class DIIS:
     A_i = <D_i - D_n, F_n>, B_ij = <D_i - D_n, F_j - F_n>) or "CDIIS" (CDIIS.cpp:
     the Pulay equations on <G_i, G_j> with a ridge factor that grows by 0.001 each time
     the space is cut for weights above 5 with a near-singular matrix). Fewer than two
-    entries: the plain update (DIIS.cpp:151-163, no damping)."""
+    entries: the plain update (DIIS.cpp:151-163, no damping). The inner products are
+    cached per pair of entries, so an iteration computes only the new ones."""
 
     def __init__(self, kind, max_size=20, tol=1e-6, max_iter=300):
         self.kind, self.max_size, self.tol, self.max_iter = kind, max_size, tol, max_iter
-        self.F, self.G, self.D, self.E = [], [], [], []
+        self.F, self.G, self.D, self.E, self.ids = [], [], [], [], []
+        self._next = 0
+        self._dots = {}
 
     def steal(self, other):
         """DIIS::Steal: take the other's history, dropping the oldest beyond max_size."""
         k = max(0, len(other.F) - self.max_size)
         self.F, self.G, self.D, self.E = other.F[k:], other.G[k:], other.D[k:], other.E[k:]
+        self.ids = other.ids[k:]
+        self._next, self._dots = other._next, dict(other._dots)
 
     def append(self, F, G, D, E):
         if len(self.F) == self.max_size:
-            self.F.pop(0); self.G.pop(0); self.D.pop(0); self.E.pop(0)
+            self.F.pop(0); self.G.pop(0); self.D.pop(0); self.E.pop(0); self.ids.pop(0)
         self.F.append(F); self.G.append(G); self.D.append(D); self.E.append(E)
+        self.ids.append(self._next)
+        self._next += 1
+        live = set(self.ids)
+        self._dots = {k: v for k, v in self._dots.items() if k[1] in live and k[2] in live}
+
+    def _dot(self, dot, kind, i, j):
+        key = (kind, self.ids[i], self.ids[j])
+        if key not in self._dots:
+            a = self.D[i] if kind == "DF" else self.G[i]
+            b = self.F[j] if kind == "DF" else self.G[j]
+            self._dots[key] = dot(a, b)
+        return self._dots[key]
 
     def weights(self, dot):
         n = len(self.F)
@@ -224,7 +241,7 @@ class DIIS:
                 for i in range(size):
                     H[i, size] = H[size, i] = 1.0
                     for j in range(i, size):
-                        H[i, j] = H[j, i] = dot(self.G[n - size + i], self.G[n - size + j])
+                        H[i, j] = H[j, i] = self._dot(dot, "GG", n - size + i, n - size + j)
                 H[np.diag_indices(size + 1)] *= ridge
                 b = np.zeros(size + 1)
                 b[size] = 1.0
@@ -237,9 +254,9 @@ class DIIS:
             return w
         # ADIIS: minimise on the simplex, x = t^2 / sum t^2
         from scipy.optimize import minimize
-        Dn, Fn = self.D[-1], self.F[-1]
-        A = np.array([dot(self.D[i] - Dn, Fn) for i in range(n)])
-        B = np.array([[dot(self.D[i] - Dn, self.F[j] - Fn) for j in range(n)] for i in range(n)])
+        DF = np.array([[self._dot(dot, "DF", i, j) for j in range(n)] for i in range(n)])  # <D_i, F_j>
+        A = DF[:, n - 1] - DF[n - 1, n - 1]
+        B = DF - DF[:, [n - 1]] - DF[[n - 1], :] + DF[n - 1, n - 1]
         Bs = 0.5 * (B + B.T)
 
         def f(t):
@@ -259,10 +276,10 @@ class DIIS:
 
 class RestrictedDFSCF:
     """Restricted Kohn-Sham SCF (RestrictedDIIS, src/HartreeFockKohnSham/Restricted/SP.cpp:20-81) with
-    every matrix on the device: S, T, V and J from the Coulomb engine (CoulombFit: J fitted in an
-    auxiliary basis with the Coulomb metric), the XC part from the grid, and the dense algebra
-    (diagonalisation, products) in torch on cuda:0. For functionals without exact exchange
-    (EXX = 0, e.g. r2SCAN), which is what the engine serves.
+    S, T, V and J from the device Coulomb engine (CoulombFit: J fitted in an auxiliary basis with the
+    Coulomb metric), the XC part from the grid, and the O(n^3) algebra (diagonalisation, products)
+    on the device through rocBLAS / rocSOLVER (chinium_amd.linalg). For functionals without exact
+    exchange (EXX = 0, e.g. r2SCAN), which is what the engine serves.
 
     Update (SP.cpp:36-66): orbitals of F in the S^{-1/2} basis, D = C_occ C_occ^T,
     J = J[2D] (fitted), XC sequence grid.getDensity({2D}) -> Evaluate("ev") -> getEnergy ->
@@ -270,60 +287,75 @@ class RestrictedDFSCF:
     max|G| < adiis_tol, CDIIS (taking over ADIIS's history) until max|G| < tol
     (SP.cpp:73-79)."""
 
-    def __init__(self, mol, nocc, xc, grid, coulomb=None, device="cuda"):
-        import torch
+    def __init__(self, mol, nocc, xc, grid, coulomb=None):
+        from . import CoulombFit
+        from . import linalg as la
 
         if float(xc.EXX) != 0.0:
             raise ValueError("the density-fitted SCF serves functionals without exact exchange")
         if not 0 < int(nocc) <= mol.nbasis:
             raise ValueError("nocc must be in 1..nbasis")
-        from . import CoulombFit
-
-        self.torch, self.dev = torch, torch.device(device)
+        self.la = la
         self.mol, self.nocc, self.xc, self.grid = mol, int(nocc), xc, grid
+        n = self.n = mol.nbasis
         t0 = time.perf_counter()
         self.jf = coulomb if coulomb is not None else CoulombFit(mol)
         S, T, V = self.jf.int1e()
         M = self.jf.metric()
+        self.S, self.Hcore = S, T + V
+        self.info = la.DMat(1)
+        self.Lm = la.DMat.of(M)
+        la.potrf(self.Lm, self.info)
+        self.gam = la.DMat(M.shape[0])
+        # S^{-1/2} from the eigenpairs of S
+        self.w, self.work = la.DMat(n), la.DMat(n)
+        U = la.DMat.of(S)
+        la.syevd(U, self.w, self.work, self.info)
+        s = self.w.get()
+        Us = U.get() * s ** -0.25
+        self.Z = la.DMat.of(Us @ Us.T)
+        self.Sd = la.DMat.of(S)
+        self.A, self.B, self.C = la.DMat(n, n), la.DMat(n, n), la.DMat(n, n)
+        self.Dd = la.DMat(n, n)
         self.t_setup = time.perf_counter() - t0
-        tt = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=self.dev)
-        self.S, self.Hcore = tt(S), tt(T + V)
-        self.Lm = torch.linalg.cholesky(tt(M))
-        s, U = torch.linalg.eigh(self.S)
-        self.Z = (U * s.rsqrt()) @ U.T  # S^{-1/2}
         self.enuc = mol.nuclear_repulsion()
         self.timing = dict(j=[], xc=[], linalg=[])
 
     def coulomb(self, P):
-        """Fitted J[P] (torch, device) and the fitted Coulomb energy 1/2 gamma^T c."""
-        torch = self.torch
-        g = self.jf.gamma(P.cpu().numpy())
-        gt = torch.as_tensor(g, dtype=torch.float64, device=self.dev)
-        c = torch.cholesky_solve(gt[:, None], self.Lm)[:, 0]
-        J = self.jf.coulomb(c.cpu().numpy())
-        return torch.as_tensor(J, dtype=torch.float64, device=self.dev), 0.5 * float(gt @ c)
+        """Fitted J[P] and the fitted Coulomb energy 1/2 gamma^T c."""
+        g = self.jf.gamma(P)
+        self.gam.set(g)
+        self.la.potrs(self.Lm, self.gam)
+        c = self.gam.get()
+        return self.jf.coulomb(c), 0.5 * float(g @ c)
 
     def update(self, F):
-        torch = self.torch
+        la = self.la
         t0 = time.perf_counter()
-        eps, Cp = torch.linalg.eigh(self.Z.T @ F @ self.Z)
-        C = self.Z @ Cp
-        Co = C[:, :self.nocc]
-        D = Co @ Co.T
+        self.A.set(F)
+        la.gemm(self.Z, self.A, self.B, ta=True)           # Z^T F
+        la.gemm(self.B, self.Z, self.A)                    # Z^T F Z
+        la.syevd(self.A, self.w, self.work, self.info)     # eigenvectors C' over A
+        la.gemm(self.Z, self.A, self.C)                    # C = Z C'
+        la.gemm(self.C, self.C, self.Dd, tb=True, k=self.nocc)  # D = C_occ C_occ^T
+        D = self.Dd.get()
+        eps = self.w.get()
         t1 = time.perf_counter()
         J, _ = self.coulomb(2.0 * D)
         t2 = time.perf_counter()
-        D2 = (2.0 * D).cpu().numpy()
-        self.grid.getDensity([D2])
+        self.grid.getDensity([2.0 * D])
         self.xc.Evaluate("ev", self.grid)
         Exc = self.grid.getEnergy()
-        Vxc = torch.as_tensor(self.grid.getFock()[0], dtype=torch.float64, device=self.dev)
+        Vxc = self.grid.getFock()[0]
         t3 = time.perf_counter()
         Fhf = self.Hcore + J
         Fnew = Fhf + Vxc
-        E = float(torch.sum(D * (self.Hcore + Fhf))) + Exc
-        G = 2.0 * (Fnew @ D @ self.S - self.S @ D @ Fnew)
-        torch.cuda.synchronize()
+        E = float(np.sum(D * (self.Hcore + Fhf))) + Exc
+        self.A.set(Fnew)
+        la.gemm(self.A, self.Dd, self.B)                   # F D
+        la.gemm(self.B, self.Sd, self.A)                   # F D S (its transpose is S D F)
+        X = self.A.get()
+        G = 2.0 * (X - X.T)
         t4 = time.perf_counter()
         self.timing["linalg"].append((t1 - t0) + (t4 - t3))
         self.timing["j"].append(t2 - t1)
@@ -334,15 +366,12 @@ class RestrictedDFSCF:
         """Returns dict(E total, E_elec, E_xc, iterations, max|G|, history, F, eps). Raises
         RuntimeError("Convergence failed!") as the reference when a phase runs out of iterations.
         F0: starting Fock matrix (default Hcore, the core guess)."""
-        torch = self.torch
-        F = self.Hcore.clone() if F0 is None else torch.as_tensor(F0, dtype=torch.float64, device=self.dev)
-        last, hist = run_reference_diis(self.update, F, lambda a, b: float(torch.sum(a * b)),
-                                        lambda G: float(torch.abs(G).max()), adiis_tol, tol, max_iter,
-                                        log, self.enuc)
+        F = self.Hcore.copy() if F0 is None else np.asarray(F0, dtype=np.float64)
+        last, hist = run_reference_diis(self.update, F, lambda a, b: float(np.vdot(a, b)),
+                                        lambda G: float(np.abs(G).max()), adiis_tol, tol, max_iter, log, self.enuc)
         Fnew, E, G, D, eps, Exc = last
-        return dict(E=E + self.enuc, E_elec=E, E_xc=Exc, iterations=len(hist),
-                    diis_error=float(torch.abs(G).max()), history=hist, F=Fnew.cpu().numpy(),
-                    eps=eps.cpu().numpy())
+        return dict(E=E + self.enuc, E_elec=E, E_xc=Exc, iterations=len(hist), diis_error=float(np.abs(G).max()),
+                    history=hist, F=Fnew, eps=eps)
 
 
 def run_reference_diis(update, F, dot, maxabs, adiis_tol=1e-1, tol=1e-6, max_iter=300, log=None, enuc=0.0):

@@ -148,6 +148,14 @@ int chx_grid_set_vrho(chx_grid* g, const double* vrho, long npoints);
 int chx_grid_stream(const chx_grid* g, void** stream);
 /* Options of the GPU path (no reference counterpart). */
 int chx_grid_set_precision(chx_grid* g, int precision);
+/* Mixed form: batches with a point within `radius` bohr of a nucleus with
+ * Z > 2 (the atomic cores, where the density is largest; the grid batches
+ * the points within 0.5 bohr of such a nucleus apart) form rho and grad rho
+ * with operands split in three bf16 terms (FP32-level) instead of two; 0
+ * turns this off. Default 0.5; set before getAO. No reference counterpart. */
+int chx_grid_set_core_radius(chx_grid* g, double radius);
+/* number of such core batches in the current plan (0 for the FP64 form) */
+int chx_grid_core_batches(const chx_grid* g, long* n);
 /* AO screening threshold: an AO tile is dropped from a batch when every |phi|
  * and |grad phi| in it stays at or below the threshold at every point of the
  * batch. Takes effect at the next chx_grid_get_ao. Default 1e-14. */

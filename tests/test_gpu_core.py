@@ -1,0 +1,36 @@
+"""Core batches (mixed form): the density of batches near nuclei with Z > 2 runs with three-term
+operands. At a converged RHF density of water (core-heavy; fitting-free) the mixed E_xc with the core
+treatment is far closer to the FP64 form's than without it; the count of core batches is small."""
+import numpy as np
+import pytest
+
+import chinium_amd as ca
+from chinium_amd import workloads as wl
+from chinium_amd.scf import RestrictedSCF
+
+pytestmark = pytest.mark.gpu
+
+
+def test_core_batches_cut_the_mixed_error():
+    w = wl.build_workload("h2o_ccpvdz_sg1_pbe")
+    mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
+    C = RestrictedSCF(mol, 5).run()["C"]
+    D = 2.0 * C[:, :5] @ C[:, :5].T
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    E = {}
+    for name, prec, rc in (("fp64", ca.CHX_PREC_FP64, 0.0), ("mixed", ca.CHX_PREC_MIXED, 0.0),
+                           ("mixed_core", ca.CHX_PREC_MIXED, 0.5)):
+        g = ca.Grid(mol, "SG-1", 1, 0)
+        g.setPrecision(prec)
+        g.setCoreRadius(rc)
+        g.setType(1)
+        g.getAO(0, 0)
+        E[name], _ = ca.xc_build(g, xc, D)
+        if name == "mixed_core":
+            nc, nb = g.coreBatches(), g.stats()["batches"]
+    d0, d1 = abs(E["mixed"] - E["fp64"]), abs(E["mixed_core"] - E["fp64"])
+    print("  water RHF density, PBE: |mixed - FP64| %.2e Eh, with core batches %.2e Eh (%d of %d batches)" % (
+        d0, d1, nc, nb))
+    assert 0 < nc <= 0.1 * nb
+    assert d1 <= 3e-6 and d1 <= 0.2 * d0
