@@ -49,8 +49,8 @@ def _traffic_table(workload):
     if os.path.exists(path):
         for line in open(path):
             if line.startswith("traffic_bytes "):
-                _, k, v = line.split()
-                out[k] = int(v)
+                k, v = line[len("traffic_bytes "):].rsplit(None, 1)  # kernel names may hold spaces
+                out[k.strip()] = int(v)
     return out, os.path.relpath(path, ROOT)
 
 

@@ -309,6 +309,8 @@ class Grid:
         for D in Ds:
             if np.shape(D) != (n, n):
                 raise ChxError(CHX_ERR_ARG, "density matrix size does not match the basis")
+        if len(Ds) == 1:  # no copy when D is already a column-major float64 array
+            return np.asfortranarray(Ds[0], dtype=np.float64), 1
         return np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for D in Ds]), len(Ds)
 
     def getDensity(self, Ds):
@@ -387,9 +389,9 @@ class Grid:
         if nspin is None:
             nspin = getattr(self, "_nspin", 1)
         n = self.nbasis
-        F = np.zeros(nspin * n * n)
+        F = np.empty(nspin * n * n)
         _check(lib().chx_grid_fock(self.h, _ptr(F), nspin, n))
-        return [F[k * n * n:(k + 1) * n * n].reshape((n, n), order="F").copy() for k in range(nspin)]
+        return [F[k * n * n:(k + 1) * n * n].reshape((n, n), order="F") for k in range(nspin)]
 
     def getFockDevice(self, ptr, nspin, nbasis):
         _check(lib().chx_grid_fock_device(self.h, ctypes.c_void_p(ptr), int(nspin), int(nbasis)))

@@ -142,7 +142,8 @@ void density_from_device(chx_grid* g, const double* dD) {
     g->mark_end(kStPermute, e);
     g->mark_begin(kStDensity, &e);
     // meta-GGA: rho, grad rho and the tau partials (A = d_r phi) in one launch that stages D once
-    launch_density_mx(g->stream, g->type >= 2 ? 5 : 0, ng, g->hmol);
+    // (rho, grad rho alone: the core batches' groups are left to the core launch)
+    launch_density_mx(g->stream, g->type >= 2 ? 5 : 0, g->type >= 2 ? ng : ng - ncg, g->hmol);
     // core batches' rho and grad rho again with three-term operands (overwriting their partials; tau
     // keeps two terms: near the nuclei its error is far below rho's effect on E_xc)
     launch_density_mx_core(g->stream, 0, ncg, g->hmol);
@@ -696,7 +697,7 @@ int chx_grid_set_precision(chx_grid* g, int precision) {
 
 int chx_grid_set_core_radius(chx_grid* g, double radius) {
   return guard([&] {
-    require(g && radius >= 0, CHX_ERR_ARG, "core radius must be >= 0");
+    require(g && radius >= 0 && radius <= 1.0, CHX_ERR_ARG, "core radius must be in [0, 1] bohr");
     if (g->core_radius != radius) g->ao_ready = false;  // the plan holds the core batches
     g->core_radius = radius;
   });

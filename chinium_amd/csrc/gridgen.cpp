@@ -14,6 +14,7 @@
 #include <fstream>
 #include <numeric>
 #include <sstream>
+#include <unordered_map>
 
 #include "internal.hpp"
 
@@ -117,6 +118,38 @@ Radial read_grid_file(const std::string& path) {
   if ((int)R.nang.size() != ntot) throw Error(CHX_ERR_FILE, "Corrupted grid file");
   return R;
 }
+
+// Nuclei with Z > 2 in a uniform hash of 1-bohr cells, to find the nucleus
+// (if any) within r <= 0.5 bohr of a point by looking at 27 cells.
+struct HeavyHash {
+  std::unordered_map<uint64_t, std::vector<int>> cells;
+  const std::vector<double>* xyz = nullptr;
+  static uint64_t key(long ix, long iy, long iz) {
+    return ((uint64_t)(ix & 0x1fffff) << 42) | ((uint64_t)(iy & 0x1fffff) << 21) | (uint64_t)(iz & 0x1fffff);
+  }
+  HeavyHash(const chx_mol& m) : xyz(&m.xyz) {
+    for (int a = 0; a < m.natoms; ++a)
+      if (m.Z[a] > 2)
+        cells[key((long)std::floor(m.xyz[3 * a]), (long)std::floor(m.xyz[3 * a + 1]), (long)std::floor(m.xyz[3 * a + 2]))]
+            .push_back(a);
+  }
+  // a heavy atom within sqrt(r2) of (x, y, z), sqrt(r2) <= 1; -1 if none
+  int near(double x, double y, double z, double r2) const {
+    if (cells.empty()) return -1;
+    const long ix = (long)std::floor(x), iy = (long)std::floor(y), iz = (long)std::floor(z);
+    for (long dx = -1; dx <= 1; ++dx)
+      for (long dy = -1; dy <= 1; ++dy)
+        for (long dz = -1; dz <= 1; ++dz) {
+          auto it = cells.find(key(ix + dx, iy + dy, iz + dz));
+          if (it == cells.end()) continue;
+          for (int a : it->second) {
+            const double ex = x - (*xyz)[3 * a], ey = y - (*xyz)[3 * a + 1], ez = z - (*xyz)[3 * a + 2];
+            if (ex * ex + ey * ey + ez * ez < r2) return a;
+          }
+        }
+    return -1;
+  }
+};
 
 // Recursive bisection into batches of exactly kBatch points (one remainder).
 void bisect(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& z,
@@ -227,13 +260,13 @@ void grid_build(chx_grid* g, const std::string& grid_name, const std::string& da
   std::vector<std::pair<long, long>> parts;
   {
     const double rc2 = kCoreRadius * kCoreRadius;
+    const HeavyHash hh(m);
     std::vector<std::vector<long>> core(nat);
     std::vector<long> rest;
     rest.reserve(keep.size());
-    for (long i : keep) {
-      const int a = owner[i];
-      const double dx = px[i] - m.xyz[3 * a], dy = py[i] - m.xyz[3 * a + 1], dz = pz[i] - m.xyz[3 * a + 2];
-      if (m.Z[a] > 2 && dx * dx + dy * dy + dz * dz < rc2) core[a].push_back(i);
+    for (long i : keep) {  // any nucleus with Z > 2, not only the point's own atom's
+      const int a = hh.near(px[i], py[i], pz[i], rc2);
+      if (a >= 0) core[a].push_back(i);
       else rest.push_back(i);
     }
     keep.clear();
@@ -625,27 +658,22 @@ void grid_plan(chx_grid* g) {
       // core batches: any real point within core_radius of a nucleus with Z > 2
       std::vector<int> core_off(nb, -1), slot_core(nslot, -1);
       g->core_grps.clear();
+      g->main_grps.clear();
       g->ncore_batches = 0;
       int ncore_slots = 0;
       if (g->core_radius > 0) {
-        const double r2c = g->core_radius * g->core_radius;
-        std::vector<int> heavy;
-        for (int a = 0; a < g->mol.natoms; ++a)
-          if (g->mol.Z[a] > 2) heavy.push_back(a);
-        for (int b = 0; b < nb && !heavy.empty(); ++b) {
+        const double r2c = g->core_radius * g->core_radius;  // <= 1 bohr: the hash's reach
+        const HeavyHash hh(g->mol);
+        for (int b = 0; b < nb; ++b) {
           bool core = false;
           for (int q = 0; q < g->batch_n[b] && !core; ++q) {
             const size_t p = (size_t)b * kBatch + q;
-            for (int a : heavy) {
-              const double dx = g->hx[p] - g->mol.xyz[3 * a], dy = g->hy[p] - g->mol.xyz[3 * a + 1],
-                           dz = g->hz[p] - g->mol.xyz[3 * a + 2];
-              if (dx * dx + dy * dy + dz * dz < r2c) {
-                core = true;
-                break;
-              }
-            }
+            core = hh.near(g->hx[p], g->hy[p], g->hz[p], r2c) >= 0;
           }
-          if (!core) continue;
+          if (!core) {
+            for (int k = g->grp_off[b]; k < g->grp_off[b + 1]; ++k) g->main_grps.push_back(k);
+            continue;
+          }
           ++g->ncore_batches;
           core_off[b] = ncore_slots;
           for (int it = g->bt_off[b]; it < g->bt_off[b + 1]; ++it) slot_core[it] = ncore_slots++;
@@ -656,6 +684,8 @@ void grid_plan(chx_grid* g) {
       g->d_core_off.upload(core_off.empty() ? neg1 : core_off, st);
       g->d_slot_core.upload(slot_core.empty() ? neg1 : slot_core, st);
       g->d_core_grps.upload(g->core_grps.empty() ? zero1 : g->core_grps, st);
+      if (g->core_grps.empty()) g->main_grps.clear();  // no core batches: the launch takes every group
+      g->d_main_grps.upload(g->main_grps.empty() ? zero1 : g->main_grps, st);
       g->d_cmx3.alloc(std::max<size_t>(1, (size_t)ncore_slots * nT * kBatch * kTile));
       g->d_Ds3.alloc(ncore_slots ? (size_t)g->npad * g->npad : 1);
       launch_ao_cache_mx(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p,
@@ -715,6 +745,7 @@ void grid_plan(chx_grid* g) {
     m.Ds3 = g->d_Ds3.p;
     m.core_off = g->d_core_off.p;
     m.core_grps = g->d_core_grps.p;
+    m.main_grps = g->core_grps.empty() ? nullptr : g->d_main_grps.p;
     upload_mx_plan(g, st);
   }
   CHX_HIP(hipStreamSynchronize(st));

@@ -21,7 +21,7 @@ constexpr int kPadAO = 128;    // nbasis is padded to a multiple of this (4 tile
 constexpr int kCacheSuper = 4; // tiles per Vxc super-tile in the cached path
 constexpr int kCacheChunk = 16;  // batches per cached-path Vxc work item (FP64 form default)
 constexpr int kCoef = 5;       // Vxc integrand coefficients per point: a, c_x, c_y, c_z, t
-constexpr double kCoreRadius = 0.5;  // bohr: points this close to a nucleus with Z > 2 are batched apart
+constexpr double kCoreRadius = 0.3;  // bohr: points this close to a nucleus with Z > 2 are batched apart
 
 struct Error : std::runtime_error {
   int code;
@@ -230,9 +230,10 @@ struct MolDev {
   // density runs with operands split in three bf16 terms; cache3 holds the
   // third term of their slots' AOg blocks [core slot][c][g][mu], Ds3 that of D,
   // core_off the first core slot of each batch (-1: not a core batch),
-  // core_grps the density groups of the core batches
+  // core_grps the density groups of the core batches, main_grps those of the
+  // other batches (the (rho, grad rho) launch's groups; null: every group)
   const unsigned short *cache3, *Ds3;
-  const int *core_off, *core_grps;
+  const int *core_off, *core_grps, *main_grps;
 };
 constexpr int kMxMaxTiles = 512;  // alive tiles per batch the mixed density kernel stages (npad <= 16384)
 constexpr int kMxMaxChunk = 128;  // batches per Vxc work item in the mixed form
@@ -365,9 +366,9 @@ struct chx_grid {
   // mixed form: core batches (density with three-term operands)
   double core_radius = chx::kCoreRadius;  // bohr; 0 turns the core treatment off
   int ncore_batches = 0;
-  std::vector<int> core_grps;
+  std::vector<int> core_grps, main_grps;
   chx::DevBuf<unsigned short> d_cmx3, d_Ds3;
-  chx::DevBuf<int> d_core_off, d_core_grps, d_slot_core;
+  chx::DevBuf<int> d_core_off, d_core_grps, d_main_grps, d_slot_core;
   chx::DevBuf<double> d_cache, d_bmat, d_Mint;
   chx::DevBuf<int> d_slot_batch, d_slot_tile, d_slot_of, d_tile_nao, d_grp_slots, d_cpair_batches;
   chx::DevBuf<int4> d_cwork;

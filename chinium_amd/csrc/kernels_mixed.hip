@@ -409,7 +409,8 @@ __global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
   constexpr int NOUT = MODE == 0 ? 4 : 1;
   constexpr int grp_stride = (MODE == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
   __shared__ DensMxSmem<P3> sm;
-  const int grp = P3 ? m.core_grps[xcd_remap(blockIdx.x, gridDim.x)] : xcd_remap(blockIdx.x, gridDim.x);
+  const int r = xcd_remap(blockIdx.x, gridDim.x);
+  const int grp = P3 ? m.core_grps[r] : (MODE == 0 && m.main_grps) ? m.main_grps[r] : r;
   const u16* __restrict__ cache = m.cache + (size_t)MODE * kMxT;
   const u16* __restrict__ Ds = m.Ds;
   const size_t sstride = (size_t)m.sstride;

@@ -139,7 +139,8 @@ void enqueue_lane(chx_pack* p, int m0, int m1, hipStream_t s, hipEvent_t after, 
   if (after) CHX_HIP(hipStreamWaitEvent(s, after, 0));
   for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
-    launch_density_mx(s, p->type >= 2 ? 5 : 0, (int)g->grp_batch.size(), g->hmol);
+    const int ng = (int)g->grp_batch.size(), ncg = (int)g->core_grps.size();
+    launch_density_mx(s, p->type >= 2 ? 5 : 0, p->type >= 2 ? ng : ng - ncg, g->hmol);
     // core batches' rho and grad rho again with three-term operands
     launch_density_mx_core(s, 0, (int)g->core_grps.size(), g->hmol);
   }

@@ -150,9 +150,10 @@ int chx_grid_stream(const chx_grid* g, void** stream);
 int chx_grid_set_precision(chx_grid* g, int precision);
 /* Mixed form: batches with a point within `radius` bohr of a nucleus with
  * Z > 2 (the atomic cores, where the density is largest; the grid batches
- * the points within 0.5 bohr of such a nucleus apart) form rho and grad rho
+ * the points within 0.3 bohr of such a nucleus apart) form rho and grad rho
  * with operands split in three bf16 terms (FP32-level) instead of two; 0
- * turns this off. Default 0.5; set before getAO. No reference counterpart. */
+ * turns this off, at most 1. Default 0.3; set before getAO. No reference
+ * counterpart. */
 int chx_grid_set_core_radius(chx_grid* g, double radius);
 /* number of such core batches in the current plan (0 for the FP64 form) */
 int chx_grid_core_batches(const chx_grid* g, long* n);

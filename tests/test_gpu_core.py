@@ -18,19 +18,22 @@ def test_core_batches_cut_the_mixed_error():
     D = 2.0 * C[:, :5] @ C[:, :5].T
     xc = ca.ExchangeCorrelation()
     xc.Read("PBE")
-    E = {}
+    E, nc = {}, {}
     for name, prec, rc in (("fp64", ca.CHX_PREC_FP64, 0.0), ("mixed", ca.CHX_PREC_MIXED, 0.0),
-                           ("mixed_core", ca.CHX_PREC_MIXED, 0.5)):
+                           ("mixed_core", ca.CHX_PREC_MIXED, None), ("mixed_core_05", ca.CHX_PREC_MIXED, 0.5)):
         g = ca.Grid(mol, "SG-1", 1, 0)
         g.setPrecision(prec)
-        g.setCoreRadius(rc)
+        if rc is not None:  # None: the default radius, 0.3 bohr
+            g.setCoreRadius(rc)
         g.setType(1)
         g.getAO(0, 0)
         E[name], _ = ca.xc_build(g, xc, D)
-        if name == "mixed_core":
-            nc, nb = g.coreBatches(), g.stats()["batches"]
-    d0, d1 = abs(E["mixed"] - E["fp64"]), abs(E["mixed_core"] - E["fp64"])
-    print("  water RHF density, PBE: |mixed - FP64| %.2e Eh, with core batches %.2e Eh (%d of %d batches)" % (
-        d0, d1, nc, nb))
-    assert 0 < nc <= 0.1 * nb
-    assert d1 <= 3e-6 and d1 <= 0.2 * d0
+        nc[name], nb = g.coreBatches(), g.stats()["batches"]
+    d0 = abs(E["mixed"] - E["fp64"])
+    d1, d2 = abs(E["mixed_core"] - E["fp64"]), abs(E["mixed_core_05"] - E["fp64"])
+    print("  water RHF density, PBE: |mixed - FP64| %.2e Eh; with core batches (0.3 bohr) %.2e Eh (%d of %d "
+          "batches), 0.5 bohr %.2e Eh (%d)" % (d0, d1, nc["mixed_core"], nb, d2, nc["mixed_core_05"]))
+    assert nc["mixed"] == 0 and 0 < nc["mixed_core"] <= nc["mixed_core_05"] <= 0.1 * nb
+    # per atom: 1.5e-6 Eh at the default radius (the bar for converged densities is 3e-6), 5e-7 at 0.5 bohr
+    assert d1 / 3 <= 1.5e-6 and d1 <= 0.2 * d0
+    assert d2 / 3 <= 5e-7
