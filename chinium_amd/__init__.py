@@ -109,6 +109,7 @@ def lib():
         "chx_grid_energy_grad": (i, [_vp, _vp]),
         "chx_grid_get_density_u": (i, [_vp, _vp, i, i, i]),
         "chx_grid_fock_u": (i, [_vp, _vp, i, i, i]),
+        "chx_grid_fock_s": (i, [_vp, _vp, i, i]),
         "chx_xc_eval_points_f": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp]),
         "chx_pack_create": (i, [_vp, i, P(_vp)]),
         "chx_pack_xc": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -348,11 +349,18 @@ class Grid:
         self._nmats_u = len(Ds[0])
         _check(lib().chx_grid_get_density_u(self.h, _ptr(buf), len(Ds[0]), len(Ds), n))
 
-    def getFockU(self, nspin=1):
-        """Grid::getFockU<u_t> (src/Grid/Grid.cpp:193-216): response matrices [spin][mat]."""
-        n, m = self.nbasis, getattr(self, "_nmats_u", 0)
-        F = np.zeros(nspin * m * n * n)
-        _check(lib().chx_grid_fock_u(self.h, _ptr(F), m, nspin, n))
+    def getFockU(self, nspin=1, d_t="u"):
+        """Grid::getFockU<u_t> (src/Grid/Grid.cpp:193-216): response matrices [spin][mat]; d_t="s":
+        Grid::getFockU<s_t>, one matrix per nuclear coordinate [3 atom + t] from the skeleton density."""
+        n = self.nbasis
+        if d_t == "s":
+            m = 3 * self.natoms
+            F = np.zeros(nspin * m * n * n)
+            _check(lib().chx_grid_fock_s(self.h, _ptr(F), nspin, n))
+        else:
+            m = getattr(self, "_nmats_u", 0)
+            F = np.zeros(nspin * m * n * n)
+            _check(lib().chx_grid_fock_u(self.h, _ptr(F), m, nspin, n))
         return [[F[(k * m + j) * n * n:(k * m + j + 1) * n * n].reshape((n, n), order="F").copy() for j in range(m)]
                 for k in range(nspin)]
 

@@ -433,11 +433,13 @@ int chx_grid_set_type(chx_grid* g, int type) {
 
 int chx_grid_get_ao(chx_grid* g, int derivative, int output) {
   return guard([&] {
-    require(derivative == 0 || derivative == 1, CHX_ERR_UNSUPPORTED, "AO derivatives above first order are not built");
-    // derivative 1 (before the XC gradient, Restricted/Grad.cpp:73): the tables
-    // of a planned grid already serve the gradient, and the density and
-    // potentials of the last iteration must survive, as in the reference.
-    if (derivative == 1 && g->ao_ready) return;
+    require(derivative >= 0 && derivative <= 2, CHX_ERR_UNSUPPORTED, "AO derivatives above second order are not built");
+    // derivative 1 (before the XC gradient, Restricted/Grad.cpp:73) or 2
+    // (before the Hessian's skeleton terms, Restricted/Hess.cpp:78): the tables
+    // of a planned grid already serve them (the kernels that need second AO
+    // derivatives form them in place), and the density and potentials of the
+    // last iteration must survive, as in the reference.
+    if (derivative >= 1 && g->ao_ready) return;
     grid_plan(g);
     if (output)
       std::printf("AO screening: %.2f of %d tiles alive per batch, %.3e FLOP per XC build\n", g->mean_tiles, g->ntiles,
@@ -493,6 +495,33 @@ int chx_grid_get_density_u(chx_grid* g, const double* D, int nmats, int nspin, i
   });
 }
 
+// Response matrices of nmats perturbed densities held per point in rhou
+// ([mat][point][rho, grad rho], point = batch-major slot): per matrix the
+// response coefficients (k_response_coef), then the ground-state Vxc pipeline
+// on them (B = a phi + c . grad phi, M = Phi^T B, V = M + M^T), V to F (host).
+static void response_fock(chx_grid* g, const double* rhou, int nmats, double* F) {
+  const size_t nslots = (size_t)g->nbatches * kBatch, nb2 = (size_t)g->nbasis * g->nbasis;
+  double* coef = g->d_coef_u.alloc_get(std::max<size_t>(kCoef, kCoef * nslots));
+  for (int m = 0; m < nmats; ++m) {
+    launch_response_coef(g->stream, g->nbatches, rhou + (size_t)m * 4 * nslots, g->d_fx.p, g->d_w.p, g->fx_family,
+                         coef);
+    if (g->cached) {  // B = a phi + c . grad phi from the tables, M = Phi^T B, V = M + M^T
+      launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, coef, g->d_cache.p, g->d_bmat.p);
+      launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
+      launch_vxc_c64(g->stream, 1, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p,
+                     g->ntiles, g->d_cache.p, g->d_bmat.p, coef, g->d_Mint.p, g->npad);
+      launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, g->d_Vout.p);
+    } else {
+      launch_fill(g->stream, g->d_Vint.p, (size_t)g->npad * g->npad, 0.0);
+      launch_vxc_f64(g->stream, 1, (int)g->vwork.size(), g->d_vwork.p, g->d_pair_batches.p, g->d_alive.p,
+                     g->ntiles, g->d_x.p, g->d_y.p, g->d_z.p, coef, g->view(), g->d_Vint.p, g->npad);
+      launch_finalize_v(g->stream, g->nbasis, g->npad, g->d_Vint.p, g->d_ref2int.p, g->d_Vout.p);
+    }
+    CHX_HIP(hipMemcpyAsync(F + m * nb2, g->d_Vout.p, sizeof(double) * nb2, hipMemcpyDeviceToHost, g->stream));
+  }
+  CHX_HIP(hipStreamSynchronize(g->stream));
+}
+
 int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis) {
   return guard([&] {
     require(F != nullptr, CHX_ERR_ARG, "null output");
@@ -501,26 +530,53 @@ int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis) {
     require(g->fx_family >= 0, CHX_ERR_STATE, "Evaluate(\"f\") must precede Grid::getFockU");
     require(g->nmats_u > 0, CHX_ERR_STATE, "Grid::getDensityU must precede Grid::getFockU");
     require(nmats == g->nmats_u, CHX_ERR_ARG, "number of matrices differs from the last Grid::getDensityU");
-    const size_t nslots = (size_t)g->nbatches * kBatch, nb2 = (size_t)nbasis * nbasis;
-    double* coef = g->d_coef_u.alloc_get(std::max<size_t>(kCoef, kCoef * nslots));
-    for (int m = 0; m < nmats; ++m) {
-      launch_response_coef(g->stream, g->nbatches, g->d_rhou.p + (size_t)m * 4 * nslots, g->d_fx.p, g->d_w.p,
-                           g->fx_family, coef);
-      if (g->cached) {  // B = a phi + c . grad phi from the tables, M = Phi^T B, V = M + M^T
-        launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, coef, g->d_cache.p, g->d_bmat.p);
-        launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
-        launch_vxc_c64(g->stream, 1, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p,
-                       g->ntiles, g->d_cache.p, g->d_bmat.p, coef, g->d_Mint.p, g->npad);
-        launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, g->d_Vout.p);
-      } else {
-        launch_fill(g->stream, g->d_Vint.p, (size_t)g->npad * g->npad, 0.0);
-        launch_vxc_f64(g->stream, 1, (int)g->vwork.size(), g->d_vwork.p, g->d_pair_batches.p, g->d_alive.p,
-                       g->ntiles, g->d_x.p, g->d_y.p, g->d_z.p, coef, g->view(), g->d_Vint.p, g->npad);
-        launch_finalize_v(g->stream, g->nbasis, g->npad, g->d_Vint.p, g->d_ref2int.p, g->d_Vout.p);
-      }
-      CHX_HIP(hipMemcpyAsync(F + m * nb2, g->d_Vout.p, sizeof(double) * nb2, hipMemcpyDeviceToHost, g->stream));
+    response_fock(g, g->d_rhou.p, nmats, F);
+  });
+}
+
+// slot -> (density group, column tile in the group), for the per-slot epilogues
+static void ensure_slot_groups(chx_grid* g) {
+  const int ng = (int)g->grp_batch.size(), ns = (int)g->bt_list.size();
+  if (g->slot_grp.size() == (size_t)ns) return;
+  g->slot_grp.assign(ns, 0);
+  g->slot_q.assign(ns, 0);
+  for (int k = 0; k < ng; ++k)
+    for (int q = 0; q < g->grp_ntiles[k]; ++q) {
+      const int sl = g->grp_slots[(size_t)k * kGroup + q];
+      g->slot_grp[sl] = k;
+      g->slot_q[sl] = q;
     }
-    CHX_HIP(hipStreamSynchronize(g->stream));
+  g->d_slot_grp.upload(g->slot_grp, g->stream);
+  g->d_slot_q.upload(g->slot_q, g->stream);
+}
+
+int chx_grid_fock_s(chx_grid* g, double* F, int nspin, int nbasis) {
+  return guard([&] {
+    require(F != nullptr, CHX_ERR_ARG, "null output");
+    require(nspin == 1, CHX_ERR_UNSUPPORTED, "only the closed-shell path is built");
+    require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
+    require(g->have_skeleton, CHX_ERR_STATE, "Grid::getDensitySkeleton must precede Grid::getFockU<s_t>");
+    require(g->fx_family >= 0, CHX_ERR_STATE, "Evaluate(\"f\") must precede Grid::getFockU<s_t>");
+    require(g->precision == CHX_PREC_FP64 && g->cached, CHX_ERR_UNSUPPORTED,
+            "the skeleton response is built for the FP64 form with the AO tables in HBM");
+    const int na = g->mol.natoms, ng = (int)g->grp_batch.size(), ns = (int)g->bt_list.size();
+    const long npt = (long)g->nbatches * kBatch;
+    ensure_slot_groups(g);
+    // per point and nuclear coordinate: rho^{A,t}, grad rho^{A,t}
+    const size_t nxz = std::max<size_t>(1, (size_t)ng * 2 * kGroup * kTile * kBatch);
+    DevBuf<double> xz1, xz2;
+    xz1.alloc(nxz);
+    xz2.alloc(nxz);
+    double* rhos = g->d_rhou.alloc_get(std::max<size_t>(1, (size_t)3 * na * 4 * npt));
+    launch_fill(g->stream, rhos, (size_t)3 * na * 4 * npt, 0.0);
+    launch_xgrad_xz(g->stream, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
+                    g->d_bt_list.p, g->d_cache.p, g->d_bmat.p, g->d_Dskel.p, g->npad, 0, 1, xz1.p);
+    launch_xgrad_xz(g->stream, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
+                    g->d_bt_list.p, g->d_cache.p, g->d_bmat.p, g->d_Dskel.p, g->npad, 2, 3, xz2.p);
+    launch_skel_rho(g->stream, ns, g->d_slot_batch.p, g->d_slot_tile.p, g->d_slot_grp.p, g->d_slot_q.p, g->d_x.p,
+                    g->d_y.p, g->d_z.p, g->view(), xz1.p, xz2.p, npt, rhos);
+    g->nmats_u = 0;  // rhou now holds the skeleton densities, not getDensityU's
+    response_fock(g, rhos, 3 * na, F);
   });
 }
 
@@ -532,25 +588,14 @@ int chx_grid_energy_grad(chx_grid* g, double* grad) {
     require(g->v_family <= 1 && g->type <= 1, CHX_ERR_UNSUPPORTED, "the XC gradient is built for LDA and GGA");
     require(g->v_nspin == 1, CHX_ERR_UNSUPPORTED, "the XC gradient is built for the closed-shell path");
     const int na = g->mol.natoms, ng = (int)g->grp_batch.size(), ns = (int)g->bt_list.size();
-    if (g->slot_grp.size() != (size_t)ns) {  // slot -> (density group, column tile in the group)
-      g->slot_grp.assign(ns, 0);
-      g->slot_q.assign(ns, 0);
-      for (int k = 0; k < ng; ++k)
-        for (int q = 0; q < g->grp_ntiles[k]; ++q) {
-          const int sl = g->grp_slots[(size_t)k * kGroup + q];
-          g->slot_grp[sl] = k;
-          g->slot_q[sl] = q;
-        }
-      g->d_slot_grp.upload(g->slot_grp, g->stream);
-      g->d_slot_q.upload(g->slot_q, g->stream);
-    }
+    ensure_slot_groups(g);
     launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
     DevBuf<double> xz, dg;
     xz.alloc(std::max<size_t>(1, (size_t)ng * 2 * kGroup * kTile * kBatch));
     dg.alloc(3 * (size_t)na);
     launch_fill(g->stream, dg.p, 3 * (size_t)na, 0.0);
     launch_xgrad_xz(g->stream, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p, g->d_bt_list.p,
-                    g->d_cache.p, g->d_bmat.p, g->d_Dskel.p, g->npad, xz.p);
+                    g->d_cache.p, g->d_bmat.p, g->d_Dskel.p, g->npad, 0, -1, xz.p);
     launch_xgrad_epi(g->stream, ns, g->d_slot_batch.p, g->d_slot_tile.p, g->d_slot_grp.p, g->d_slot_q.p, g->d_x.p,
                      g->d_y.p, g->d_z.p, g->view(), g->d_coef.p, xz.p, dg.p);
     CHX_HIP(hipMemcpyAsync(grad, dg.p, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, g->stream));

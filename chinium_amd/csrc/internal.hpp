@@ -241,10 +241,15 @@ constexpr int kMxMaxChunk = 128;  // batches per Vxc work item in the mixed form
 // XC nuclear gradient (kernels_grad.hip), FP64 cached form.
 void launch_xgrad_xz(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
                      const int* bt_off, const int* bt_list, const double* cache, const double* bmat,
-                     const double* Dint, int npad, double* xz);
+                     const double* Dint, int npad, int c1, int c2, double* xz);
 void launch_xgrad_epi(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* slot_grp,
                       const int* slot_q, const double* bx, const double* by, const double* bz, BasisView bv,
                       const double* coef, const double* xz, double* grad);
+// skeleton density derivatives (rho, grad rho per nuclear coordinate) into
+// rhos[3 atom + t][npt points][4], accumulated (clear it first)
+void launch_skel_rho(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* slot_grp,
+                     const int* slot_q, const double* bx, const double* by, const double* bz, BasisView bv,
+                     const double* xz1, const double* xz2, long npt, double* rhos);
 
 // Mixed-precision form (kernels_mixed.hip): MFMA operands cached as bf16
 // hi/lo pairs, split-BF16 MFMA contractions with FP32 accumulation, FP32

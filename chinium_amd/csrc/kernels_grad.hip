@@ -11,6 +11,15 @@
 // two contractions of the density's shape (k_xgrad_xz, FP64 MFMA over the
 // cached phi and B tables) and a per-slot epilogue with the AO Hessian
 // (k_xgrad_epi, one wave per slot, lane = point).
+//
+// The skeleton XC response (Grid::getFockU<s_t>, src/Grid/Grid.cpp:193-216)
+// takes the same skeleton density derivatives per point, for every nuclear
+// coordinate (GridDensity.cpp:90-128):
+//   rho^{A,t}     = -2 sum_{mu in A} d_t phi_mu X_mu,
+//   d_r rho^{A,t} = -2 sum_{mu in A} (d_r d_t phi_mu X_mu + d_t phi_mu Y^r_mu),
+//   Y^r_mu = sum_nu D_mu,nu d_r phi_nu:
+// the contraction kernel run with (phi, d_x phi) and (d_y phi, d_z phi) as its
+// two A operands, then k_skel_rho per slot.
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -23,11 +32,14 @@ constexpr int kGSlotB = kBatch * kTile;
 constexpr int kGCols = kGroup * kTile;        // 128 columns per group
 
 // ---------------------------------------------------------------------------
-// X and Z of one density work group (batch, <= 4 column tiles), written as
-// xz[grp][0 = X, 1 = Z][column][g]. 4 waves, wave w owns rows 16w..16w+15.
+// Two products of one density work group (batch, <= 4 column tiles) with the
+// group's D block: A1 = cache component c1 (0 phi, r d_r phi), A2 = component
+// c2, or the B table when c2 < 0. Written as xz[grp][0 = A1 D, 1 = A2 D][column][g];
+// (c1, c2) = (0, -1) gives the gradient's X and Z. 4 waves, wave w owns rows
+// 16w..16w+15.
 struct XgradSmem {
-  double Ap[kBatch][kTile + 1];  // phi of the K tile, [g][nu]
-  double Ab[kBatch][kTile + 1];  // B of the K tile
+  double Ap[kBatch][kTile + 1];  // A1 of the K tile, [g][nu]
+  double Ab[kBatch][kTile + 1];  // A2 of the K tile
   double Bd[kTile][kGCols];      // D block [nu][mu]
 };
 
@@ -35,7 +47,7 @@ __global__ __launch_bounds__(256) void k_xgrad_xz(const int* __restrict__ grp_ba
                                                   const int* __restrict__ grp_ntiles, const int* __restrict__ bt_off,
                                                   const int* __restrict__ bt_list, const double* __restrict__ cache,
                                                   const double* __restrict__ bmat, const double* __restrict__ Dint,
-                                                  int npad, double* __restrict__ xz) {
+                                                  int npad, int c1, int c2, double* __restrict__ xz) {
   __shared__ XgradSmem sm;
   const int grp = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -51,8 +63,8 @@ __global__ __launch_bounds__(256) void k_xgrad_xz(const int* __restrict__ grp_ba
   for (int c = 0; c < 8; ++c) ax[c] = az[c] = dbl4{0, 0, 0, 0};
   for (int it = bt_off[b]; it < bt_off[b + 1]; ++it) {
     const int ti = bt_list[it];
-    const double* p = cache + (size_t)it * kGSlotAO;  // comp 0 = phi, [g][nu]
-    const double* q = bmat + (size_t)it * kGSlotB;    // B, [g][nu]
+    const double* p = cache + (size_t)it * kGSlotAO + (size_t)c1 * kBatch * kTile;  // [g][nu]
+    const double* q = c2 < 0 ? bmat + (size_t)it * kGSlotB : cache + (size_t)it * kGSlotAO + (size_t)c2 * kBatch * kTile;
     for (int e = tid; e < kBatch * kTile; e += 256) {
       sm.Ap[e / kTile][e % kTile] = p[e];
       sm.Ab[e / kTile][e % kTile] = q[e];
@@ -89,10 +101,10 @@ __global__ __launch_bounds__(256) void k_xgrad_xz(const int* __restrict__ grp_ba
 
 void launch_xgrad_xz(hipStream_t s, int ngroups, const int* grp_batch, const int* grp_tiles, const int* grp_ntiles,
                      const int* bt_off, const int* bt_list, const double* cache, const double* bmat,
-                     const double* Dint, int npad, double* xz) {
+                     const double* Dint, int npad, int c1, int c2, double* xz) {
   if (ngroups == 0) return;
   hipLaunchKernelGGL(k_xgrad_xz, dim3(ngroups), dim3(256), 0, s, grp_batch, grp_tiles, grp_ntiles, bt_off, bt_list,
-                     cache, bmat, Dint, npad, xz);
+                     cache, bmat, Dint, npad, c1, c2, xz);
   CHX_HIP(hipGetLastError());
 }
 
@@ -245,6 +257,111 @@ void launch_xgrad_epi(hipStream_t s, int nslots, const int* slot_batch, const in
   if (nslots == 0) return;
   hipLaunchKernelGGL(k_xgrad_epi, dim3((nslots + 3) / 4), dim3(256), 0, s, nslots, slot_batch, slot_tile, slot_grp,
                      slot_q, bx, by, bz, bv, coef, xz, grad);
+  CHX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Skeleton density derivatives of one shell's components at this lane's
+// point: acc[t] += (d_t phi X, sum over r of d_r d_t phi X + d_t phi Y^r for r = x, y, z).
+template <int TYPE>
+__device__ __forceinline__ void skel_shell(double x, double y, double z, double A0, const double* dA, const double* HA,
+                                           int first, int cnt, const double* c1, const double* c2,
+                                           double (*acc)[4]) {
+  constexpr int n = TYPE == 0 ? 1 : TYPE == -2 ? 5 : TYPE == -3 ? 7 : 3;
+  double q[n], g[n][3], h[n][6];
+  angular_h<TYPE>(x, y, z, q, g, h);
+  const int hi[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+  for (int m = 0; m < n; ++m) {
+    const int k = m - first;
+    if (k < 0 || k >= cnt) continue;
+    const double X = c1[k * kBatch];
+    const double Y[3] = {c1[(kGCols + k) * kBatch], c2[k * kBatch], c2[(kGCols + k) * kBatch]};
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const double dt = dA[t] * q[m] + A0 * g[m][t];  // d_t phi
+      acc[t][0] += dt * X;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        acc[t][1 + r] += (HA[hi[r][t]] * q[m] + dA[r] * g[m][t] + dA[t] * g[m][r] + A0 * h[m][hi[r][t]]) * X +
+                         dt * Y[r];
+    }
+  }
+}
+
+// One wave per slot (batch, tile), lane = point. xz1 / xz2: the group products
+// (phi D, d_x phi D) and (d_y phi D, d_z phi D). The sums of consecutive shells
+// of one atom are added, scaled by -2, to rhos[3 A + t][point][rho, d_x, d_y, d_z]
+// (npt points per coordinate) by FP64 atomics.
+__global__ __launch_bounds__(256) void k_skel_rho(int nslots, const int* __restrict__ slot_batch,
+                                                  const int* __restrict__ slot_tile, const int* __restrict__ slot_grp,
+                                                  const int* __restrict__ slot_q, const double* __restrict__ bx,
+                                                  const double* __restrict__ by, const double* __restrict__ bz,
+                                                  BasisView bv, const double* __restrict__ xz1,
+                                                  const double* __restrict__ xz2, long npt, double* __restrict__ rhos) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + wave;
+  if (slot >= nslots) return;
+  const int b = slot_batch[slot], t = slot_tile[slot];
+  const size_t p = (size_t)b * kBatch + g;
+  const double px = bx[p], py = by[p], pz = bz[p];
+  const size_t col0 = (size_t)slot_grp[slot] * 2 * kGCols * kBatch + (size_t)slot_q[slot] * kTile * kBatch + g;
+  const double* X1 = xz1 + col0;
+  const double* X2 = xz2 + col0;
+  double acc[3][4] = {};
+  int atom = -1;
+  auto flush = [&]() {
+    if (atom < 0) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        unsafeAtomicAdd(&rhos[((size_t)(3 * atom + d) * npt + p) * 4 + k], -2.0 * acc[d][k]);
+        acc[d][k] = 0.0;
+      }
+  };
+  const int s0 = bv.tile_seg_off[t], s1 = bv.tile_seg_off[t + 1];
+  for (int k = s0; k < s1; ++k) {
+    const int4 sg = bv.seg[k];
+    const int sh = sg.x;
+    if (bv.sh_atom[sh] != atom) {
+      flush();
+      atom = bv.sh_atom[sh];
+    }
+    const double x = px - bv.sh_xyz[3 * sh], y = py - bv.sh_xyz[3 * sh + 1], z = pz - bv.sh_xyz[3 * sh + 2];
+    const double r2 = x * x + y * y + z * z;
+    double A0 = 0, A1 = 0, A2 = 0;
+    for (int i = 0, po = bv.sh_poff[sh]; i < bv.sh_nprim[sh]; ++i) {
+      const double e = bv.p_exp[po + i];
+      const double v = bv.p_coef[po + i] * ::exp(-e * r2);
+      A0 += v;
+      A1 += e * v;
+      A2 += e * e * v;
+    }
+    const double d[3] = {x, y, z};
+    double dA[3], HA[6];
+    for (int r = 0; r < 3; ++r) dA[r] = -2 * d[r] * A1;
+    HA[0] = 4 * x * x * A2 - 2 * A1; HA[1] = 4 * x * y * A2; HA[2] = 4 * x * z * A2;
+    HA[3] = 4 * y * y * A2 - 2 * A1; HA[4] = 4 * y * z * A2; HA[5] = 4 * z * z * A2 - 2 * A1;
+    const double* c1 = X1 + (size_t)sg.w * kBatch;
+    const double* c2 = X2 + (size_t)sg.w * kBatch;
+    switch (bv.sh_type[sh]) {
+      case 0: skel_shell<0>(x, y, z, A0, dA, HA, sg.y, sg.z, c1, c2, acc); break;
+      case 1: skel_shell<1>(x, y, z, A0, dA, HA, sg.y, sg.z, c1, c2, acc); break;
+      case -1: skel_shell<-1>(x, y, z, A0, dA, HA, sg.y, sg.z, c1, c2, acc); break;
+      case -2: skel_shell<-2>(x, y, z, A0, dA, HA, sg.y, sg.z, c1, c2, acc); break;
+      default: skel_shell<-3>(x, y, z, A0, dA, HA, sg.y, sg.z, c1, c2, acc); break;
+    }
+  }
+  flush();
+}
+
+void launch_skel_rho(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* slot_grp,
+                     const int* slot_q, const double* bx, const double* by, const double* bz, BasisView bv,
+                     const double* xz1, const double* xz2, long npt, double* rhos) {
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_skel_rho, dim3((nslots + 3) / 4), dim3(256), 0, s, nslots, slot_batch, slot_tile, slot_grp,
+                     slot_q, bx, by, bz, bv, xz1, xz2, npt, rhos);
   CHX_HIP(hipGetLastError());
 }
 

@@ -57,6 +57,10 @@ inline void check(int rc) {
   if (rc != CHX_OK) throw std::runtime_error(chx_last_error());
 }
 
+// Which perturbed density Grid::getFockU contracts (src/Grid/Grid.h): the
+// skeleton derivatives of getDensitySkeleton (s_t) or getDensityU's (u_t).
+enum D_t { s_t, u_t };
+
 inline std::string data_root() {
   const char* p = std::getenv("CHINIUM_PATH");
   if (!p) throw std::runtime_error("CHINIUM_PATH is not set");
@@ -175,11 +179,19 @@ class Grid {
     check(chx_grid_get_density_u(h_, buf.data(), nmats, nspin, nbasis_));
     nmats_u_ = nmats;
   }
+  // D_t = u_t: the matrices of the last getDensityU. D_t = s_t (the Hessian's
+  // skeleton term, Restricted/Hess.cpp:96): one matrix per nuclear coordinate,
+  // [3 atom + t], from the last getDensitySkeleton.
+  template <D_t d_t = u_t>
   std::vector<std::vector<EigenMatrix>> getFockU() {
-    std::vector<double> buf((size_t)nmats_u_ * nbasis_ * nbasis_);
-    check(chx_grid_fock_u(h_, buf.data(), nmats_u_, 1, nbasis_));
+    const int nmats = d_t == s_t ? 3 * natoms_ : nmats_u_;
+    std::vector<double> buf((size_t)nmats * nbasis_ * nbasis_);
+    if (d_t == s_t)
+      check(chx_grid_fock_s(h_, buf.data(), 1, nbasis_));
+    else
+      check(chx_grid_fock_u(h_, buf.data(), nmats, 1, nbasis_));
     std::vector<std::vector<EigenMatrix>> F(1);
-    for (int m = 0; m < nmats_u_; ++m) {
+    for (int m = 0; m < nmats; ++m) {
       F[0].push_back(make_matrix(nbasis_, nbasis_));
       std::copy(buf.begin() + (size_t)m * nbasis_ * nbasis_, buf.begin() + (size_t)(m + 1) * nbasis_ * nbasis_,
                 F[0].back().data());

@@ -140,6 +140,16 @@ int chx_grid_energy_grad(chx_grid* g, double* grad);
  * nmats response matrices in the same order. */
 int chx_grid_get_density_u(chx_grid* g, const double* D, int nmats, int nspin, int nbasis);
 int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis);
+/* Grid::getFockU<s_t> (src/Grid/Grid.cpp:193-216, SubGrid::getFockU with the
+ * skeleton derivatives, GridFock.cpp:99-160), as the Hessian calls it
+ * (src/HartreeFockKohnSham/Restricted/Hess.cpp:78-97): after
+ * chx_xc_evaluate(xc, g, "f") and chx_grid_get_density_skeleton(g, D, ...),
+ * the XC response matrix of each nuclear coordinate's skeleton density
+ * derivative (rho^{A,t} = -2 sum_{mu in A} D d_t phi_mu phi, and its gradient,
+ * GridDensity.cpp:90-128). Writes 3 natoms column-major nbasis^2 matrices,
+ * [3 A + t] (the reference's Fs[0][jpert]). Closed shell, LDA/GGA, FP64 form
+ * with the AO tables in HBM. */
+int chx_grid_fock_s(chx_grid* g, double* F, int nspin, int nbasis);
 /* Supply v_rho directly, in the library's point order, as the SAP guess does
  * when it writes SubGrid::Eps1Rho before Grid::getFock
  * (src/HartreeFockKohnSham/GuessSCF.cpp:37-60). Other potentials are zeroed. */

@@ -851,6 +851,36 @@ int ox_fock(ox_system* s, double* V) {
   return 0;
 }
 
+// One perturbation's response matrix on one sub-grid (GridFock.cpp:99-160 at
+// nspin = 1), from the perturbed density rho^x, grad rho^x at its points
+// (rx[g], r1x[r][g]; may alias sg.rho / sg.rho1) and the ground state's
+// grad rho (rho10) and f_xc (fx[4g..]: v_sigma, v2rho2, v2rhosigma, v2sigma2):
+//   sigma^x = 2 grad rho . grad rho^x,
+//   F0 weight: v2rho2 rho^x + v2rhosigma sigma^x,
+//   F1 vector: (v2rhosigma rho^x + v2sigma2 sigma^x) grad rho + v_sigma grad rho^x,
+// evaluated through fock_subgrid (vrho := the F0 weight, vsigma := 1,
+// rho1 := the F1 vector); adds into Fout. Leaves sg.vrho, sg.vsigma, sg.rho1 changed.
+static void response_subgrid(const ox_system* s, SubGrid& sg, const std::vector<double>& fx,
+                             const std::vector<double>& rho10, const double* rx, const double* r1x, double* Fout,
+                             std::vector<double>& F, std::vector<double>& tmp, std::vector<double>& tmp3) {
+  const int ng = sg.npad;
+  std::vector<double> vr(ng), g1(s->family >= 1 ? (size_t)3 * ng : 0);
+  for (int g = 0; g < ng; ++g) {
+    double sx = 0.0;
+    if (s->family >= 1)
+      for (int r = 0; r < 3; ++r) sx += 2.0 * rho10[(size_t)r * ng + g] * r1x[(size_t)r * ng + g];
+    vr[g] = fx[4 * g + 1] * rx[g] + fx[4 * g + 2] * sx;
+    if (s->family >= 1) {
+      const double q = fx[4 * g + 2] * rx[g] + fx[4 * g + 3] * sx;
+      for (int r = 0; r < 3; ++r) g1[(size_t)r * ng + g] = q * rho10[(size_t)r * ng + g] + fx[4 * g] * r1x[(size_t)r * ng + g];
+    }
+  }
+  sg.vrho = vr;
+  sg.vsigma.assign(ng, 1.0);
+  if (s->family >= 1) sg.rho1 = g1;
+  fock_subgrid(s, sg, Fout, F, tmp, tmp3);
+}
+
 // Response Fock matrices (Evaluate("f") + Grid::getDensityU + Grid::getFockU<u_t>,
 // src/ExchangeCorrelation.cpp:101-104,126-133, GridDensity.cpp:47-77,
 // GridFock.cpp:99-160), closed shell, LDA/GGA, after ox_density at the
@@ -883,20 +913,7 @@ int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx) {
                               &fx[4 * g + 2], &fx[4 * g + 3]);
         for (int m = 0; m < nmats; ++m) {
           density_subgrid(s, sg, Dx + m * nb2, s->family, tmp);
-          sg.vrho.assign(ng, 0.0);
-          sg.vsigma.assign(ng, 1.0);
-          for (int g = 0; g < ng; ++g) {
-            double sx = 0.0;
-            if (s->family >= 1)
-              for (int r = 0; r < 3; ++r) sx += 2.0 * rho10[(size_t)r * ng + g] * sg.rho1[(size_t)r * ng + g];
-            sg.vrho[g] = fx[4 * g + 1] * sg.rho[g] + fx[4 * g + 2] * sx;
-            if (s->family >= 1) {
-              const double q = fx[4 * g + 2] * sg.rho[g] + fx[4 * g + 3] * sx;
-              for (int r = 0; r < 3; ++r)
-                sg.rho1[(size_t)r * ng + g] = q * rho10[(size_t)r * ng + g] + fx[4 * g] * sg.rho1[(size_t)r * ng + g];
-            }
-          }
-          fock_subgrid(s, sg, mine + m * nb2, F, tmp, tmp3);
+          response_subgrid(s, sg, fx, rho10, sg.rho.data(), sg.rho1.data(), mine + m * nb2, F, tmp, tmp3);
         }
         sg.rho = rho0; sg.rho1 = rho10; sg.sigma = sig0; sg.vrho = vr0; sg.vsigma = vs0;
       }
@@ -994,6 +1011,45 @@ int ox_eval_functional_pol(int ncodes, const int* codes, long n, const double* r
   return 0;
 }
 
+// phi, grad phi and the Hessian (xx, xy, xz, yy, yz, zz) of every AO at one
+// sub-grid's points: ao[mu][g], ao1[r][mu][g], ao2[hidx][mu][g] (the reference's
+// AO, AO1, AO2 of Grid::getAO(2, .), GridAO.cpp).
+static void ao_hessian_tables(const ox_system* s, const SubGrid& sg, std::vector<double>& ao, std::vector<double>& ao1,
+                              std::vector<double>& ao2) {
+  const int nb = s->nbasis, ng = sg.npad;
+  ao.assign((size_t)nb * ng, 0); ao1.assign((size_t)3 * nb * ng, 0); ao2.assign((size_t)6 * nb * ng, 0);
+  int mu0 = 0;
+  for (size_t is = 0; is < s->shells.size(); ++is) {
+    const Shell& sh = s->shells[is];
+    const double* c = &s->xyz[3 * sh.center];
+    for (int g = 0; g < ng; ++g) {
+      const double d[3] = {sg.x[g] - c[0], sg.y[g] - c[1], sg.z[g] - c[2]};
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double A0 = 0, A1 = 0, A2 = 0;
+      for (size_t p = 0; p < sh.exps.size(); ++p) {
+        const double t = sh.coefs[p] * std::exp(-sh.exps[p] * r2);
+        A0 += t; A1 += t * sh.exps[p]; A2 += t * sh.exps[p] * sh.exps[p];
+      }
+      double Rg[3], Rh[6];
+      for (int r = 0; r < 3; ++r) Rg[r] = -2 * d[r] * A1;
+      for (int r = 0; r < 3; ++r)
+        for (int t = r; t < 3; ++t) Rh[hidx(r, t)] = 4 * d[r] * d[t] * A2 - (r == t ? 2 * A1 : 0.0);
+      for (size_t m = 0; m < s->shell_ang[is].size(); ++m) {
+        double q, qg[3], qh[6];
+        poly_eval_h(s->shell_ang[is][m], d[0], d[1], d[2], &q, qg, qh);
+        const size_t mu = mu0 + m;
+        ao[mu * ng + g] = A0 * q;
+        for (int r = 0; r < 3; ++r) ao1[((size_t)r * nb + mu) * ng + g] = Rg[r] * q + A0 * qg[r];
+        for (int r = 0; r < 3; ++r)
+          for (int t = r; t < 3; ++t)
+            ao2[((size_t)hidx(r, t) * nb + mu) * ng + g] =
+                Rh[hidx(r, t)] * q + Rg[r] * qg[t] + Rg[t] * qg[r] + A0 * qh[hidx(r, t)];
+      }
+    }
+    mu0 += (int)s->shell_ang[is].size();
+  }
+}
+
 // XC nuclear gradient in the reference's skeleton form, the grid held fixed
 // (Restricted/Grad.cpp:72-84 -> SubGrid::getDensitySkeleton,
 // GridDensity.cpp:79-130 -> SubGrid::getEnergyGrad, GridEnergy.cpp:15-30):
@@ -1020,38 +1076,7 @@ int ox_xc_grad(ox_system* s, const double* D, double* grad) {
       for (int i = 0; i < n; ++i) {
         const SubGrid& sg = s->subgrids[s->active[i]];
         const int ng = sg.npad;
-        // phi, grad phi, Hessian of every AO at the sub-grid's points
-        ao.assign((size_t)nb * ng, 0); ao1.assign((size_t)3 * nb * ng, 0); ao2.assign((size_t)6 * nb * ng, 0);
-        int mu0 = 0;
-        for (size_t is = 0; is < s->shells.size(); ++is) {
-          const Shell& sh = s->shells[is];
-          const double* c = &s->xyz[3 * sh.center];
-          for (int g = 0; g < ng; ++g) {
-            const double d[3] = {sg.x[g] - c[0], sg.y[g] - c[1], sg.z[g] - c[2]};
-            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            double A0 = 0, A1 = 0, A2 = 0;
-            for (size_t p = 0; p < sh.exps.size(); ++p) {
-              const double t = sh.coefs[p] * std::exp(-sh.exps[p] * r2);
-              A0 += t; A1 += t * sh.exps[p]; A2 += t * sh.exps[p] * sh.exps[p];
-            }
-            double Rg[3], Rh[6];
-            for (int r = 0; r < 3; ++r) Rg[r] = -2 * d[r] * A1;
-            for (int r = 0; r < 3; ++r)
-              for (int t = r; t < 3; ++t) Rh[hidx(r, t)] = 4 * d[r] * d[t] * A2 - (r == t ? 2 * A1 : 0.0);
-            for (size_t m = 0; m < s->shell_ang[is].size(); ++m) {
-              double q, qg[3], qh[6];
-              poly_eval_h(s->shell_ang[is][m], d[0], d[1], d[2], &q, qg, qh);
-              const size_t mu = mu0 + m;
-              ao[mu * ng + g] = A0 * q;
-              for (int r = 0; r < 3; ++r) ao1[((size_t)r * nb + mu) * ng + g] = Rg[r] * q + A0 * qg[r];
-              for (int r = 0; r < 3; ++r)
-                for (int t = r; t < 3; ++t)
-                  ao2[((size_t)hidx(r, t) * nb + mu) * ng + g] =
-                      Rh[hidx(r, t)] * q + Rg[r] * qg[t] + Rg[t] * qg[r] + A0 * qh[hidx(r, t)];
-            }
-          }
-          mu0 += (int)s->shell_ang[is].size();
-        }
+        ao_hessian_tables(s, sg, ao, ao1, ao2);
         // X_mu = sum_nu D phi_nu, U_mu,r = sum_nu D d_r phi_nu
         X.assign((size_t)nb * ng, 0); U.assign((size_t)3 * nb * ng, 0);
         for (int mu = 0; mu < nb; ++mu)
@@ -1090,6 +1115,76 @@ int ox_xc_grad(ox_system* s, const double* D, double* grad) {
       for (size_t k = 0; k < G.size(); ++k) G[k] += Gl[k];
     }
     for (size_t k = 0; k < G.size(); ++k) grad[k] = G[k];
+  });
+}
+
+// XC response to the skeleton density derivatives (Restricted/Hess.cpp:79-97:
+// Evaluate("f"), getDensitySkeleton2({D}), then Grid::getFockU<s_t>,
+// Grid.cpp:193-216 with GridFock.cpp:99-160): for each nuclear coordinate
+// (atom A, direction t), the response matrix of GridFock.cpp:99-160 with the
+// perturbed density taken from getDensitySkeleton (GridDensity.cpp:90-128):
+//   RhoGrad[g,t,A]    = -2 sum_{mu in A, nu} D_mu,nu d_t phi_mu phi_nu,
+//   Rho1Grad[g,r,t,A] = -2 sum_{mu in A, nu} D_mu,nu (d_r d_t phi_mu phi_nu + d_t phi_mu d_r phi_nu),
+//   SigmaGrad = 2 grad rho . Rho1Grad (formed inside the response step).
+// D is the ground-state density (f_xc is evaluated at it). Fx: 3 natoms
+// column-major nbasis^2 matrices, [3 A + t]. Closed shell, LDA/GGA.
+int ox_fock_s(ox_system* s, const double* D, double* Fx) {
+  return guarded([&] {
+    if (s->family > 1) throw std::runtime_error("oracle: the skeleton response is LDA/GGA");
+    if (ox_density(s, D)) throw std::runtime_error(g_err);
+    const int nb = s->nbasis, na = s->natoms, n = (int)s->active.size(), nt = s->nthreads;
+    const size_t nb2 = (size_t)nb * nb, npert = (size_t)3 * na;
+    std::vector<int> ao_atom(nb);
+    for (size_t is = 0, mu = 0; is < s->shells.size(); ++is)
+      for (size_t m = 0; m < s->shell_ang[is].size(); ++m) ao_atom[mu++] = s->shells[is].center;
+    std::vector<std::vector<double>> part(nt, std::vector<double>(npert * nb2, 0.0));
+#pragma omp parallel num_threads(nt)
+    {
+      std::vector<double> F, tmp, tmp3, fx, rho0, rho10, sig0, vr0, vs0, ao, ao1, ao2, X, U, rx, r1x;
+      double* mine = part[omp_get_thread_num()].data();
+#pragma omp for schedule(dynamic, 1)
+      for (int i = 0; i < n; ++i) {
+        SubGrid& sg = s->subgrids[s->active[i]];
+        const int ng = sg.npad;
+        rho0 = sg.rho; rho10 = sg.rho1; sig0 = sg.sigma; vr0 = sg.vrho; vs0 = sg.vsigma;
+        fx.assign((size_t)4 * ng, 0.0);
+        for (int code : s->codes)
+          for (int g = 0; g < ng; ++g)
+            oxf::eval_point_f(code, rho0[g], s->family >= 1 ? sig0[g] : 0.0, &fx[4 * g], &fx[4 * g + 1],
+                              &fx[4 * g + 2], &fx[4 * g + 3]);
+        ao_hessian_tables(s, sg, ao, ao1, ao2);
+        // X_mu = sum_nu D phi_nu, U_mu,r = sum_nu D d_r phi_nu
+        X.assign((size_t)nb * ng, 0); U.assign((size_t)3 * nb * ng, 0);
+        for (int mu = 0; mu < nb; ++mu)
+          for (int nu = 0; nu < nb; ++nu) {
+            const double dd = D[(size_t)nu * nb + mu];
+            if (dd == 0.0) continue;
+            for (int g = 0; g < ng; ++g) X[(size_t)mu * ng + g] += dd * ao[(size_t)nu * ng + g];
+            for (int r = 0; r < 3; ++r)
+              for (int g = 0; g < ng; ++g)
+                U[((size_t)r * nb + mu) * ng + g] += dd * ao1[((size_t)r * nb + nu) * ng + g];
+          }
+        for (int A = 0; A < na; ++A)
+          for (int t = 0; t < 3; ++t) {
+            rx.assign(ng, 0.0); r1x.assign((size_t)3 * ng, 0.0);
+            for (int mu = 0; mu < nb; ++mu) {
+              if (ao_atom[mu] != A) continue;
+              for (int g = 0; g < ng; ++g) {
+                const double dt = ao1[((size_t)t * nb + mu) * ng + g], x = X[(size_t)mu * ng + g];
+                rx[g] += -2 * dt * x;
+                for (int r = 0; r < 3; ++r)
+                  r1x[(size_t)r * ng + g] += -2 * (ao2[((size_t)hidx(r, t) * nb + mu) * ng + g] * x +
+                                                   dt * U[((size_t)r * nb + mu) * ng + g]);
+              }
+            }
+            response_subgrid(s, sg, fx, rho10, rx.data(), r1x.data(), mine + (size_t)(3 * A + t) * nb2, F, tmp, tmp3);
+          }
+        sg.rho = rho0; sg.rho1 = rho10; sg.sigma = sig0; sg.vrho = vr0; sg.vsigma = vs0;
+      }
+    }
+    std::memset(Fx, 0, sizeof(double) * npert * nb2);
+    for (int t = 0; t < nt; ++t)
+      for (size_t k = 0; k < npert * nb2; ++k) Fx[k] += part[t][k];
   });
 }
 

@@ -97,6 +97,11 @@ int ox_eval_functional_f(int ncodes, const int* codes, long n, const double* rho
  * nbasis^2 each, back to back), closed shell, LDA/GGA, after ox_density at the
  * reference density: Evaluate("f"), getDensityU, getFockU<u_t>. */
 int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx);
+/* Skeleton XC response (getDensitySkeleton, Evaluate("f"), Grid::getFockU<s_t>):
+ * the response matrix of each nuclear coordinate's skeleton density derivative,
+ * closed shell, LDA/GGA; D is the ground-state density. Fx: 3 natoms
+ * column-major nbasis^2 matrices, [3 atom + t]. */
+int ox_fock_s(ox_system* s, const double* D, double* Fx);
 
 /* Density quantities on the active sub-grids after ox_density, in the
  * sub-grid order with padding removed: rho, grad (3 per point), sigma, tau. */

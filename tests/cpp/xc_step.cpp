@@ -52,6 +52,11 @@ int main(int argc, char** argv) {
       grid.getDensityU({{D}});
       chinium::EigenMatrix F = grid.getFockU()[0][0];
       out.write(reinterpret_cast<const char*>(F.data()), sizeof(double) * n * n);
+      // skeleton response per nuclear coordinate, in the Hessian's order (Restricted/Hess.cpp:78-96)
+      grid.getAO(2, 1);
+      grid.getDensitySkeleton({D});
+      const std::vector<std::vector<chinium::EigenMatrix>> Fs = grid.getFockU<chinium::s_t>();
+      for (const chinium::EigenMatrix& m : Fs[0]) out.write(reinterpret_cast<const char*>(m.data()), sizeof(double) * n * n);
       // spin-unrestricted call sequence (Unrestricted/SP.cpp:65-68) with Da = 0.6 D, Db = 0.4 D
       chinium::ExchangeCorrelation uxc;
       uxc.Spin = 2;

@@ -58,6 +58,7 @@ def lib():
         "ox_get_active_raw_weights": (ctypes.c_int, [_vp, _vp]),
         "ox_eval_functional_f": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long] + [_vp] * 6),
         "ox_fock_u": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+        "ox_fock_s": (ctypes.c_int, [_vp, _vp, _vp]),
         "ox_xc_pol": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_pol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
         "ox_mlp_forward": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp]),
@@ -174,6 +175,15 @@ class OracleSystem:
         F = np.zeros_like(X)
         _check(lib().ox_fock_u(self.h, len(Dxs), _ptr(X), _ptr(F)))
         return [F[m].reshape((n, n), order="F") for m in range(len(Dxs))]
+
+    def fock_s(self, D):
+        """Skeleton XC response matrices, one per nuclear coordinate [3 atom + t], at the
+        ground-state density D (getDensitySkeleton, Evaluate("f"), getFockU<s_t>)."""
+        n = self.nbasis
+        Df = np.asfortranarray(D, dtype=np.float64)
+        F = np.zeros((3 * len(self.Z), n * n))
+        _check(lib().ox_fock_s(self.h, _ptr(Df), _ptr(F)))
+        return [F[m].reshape((n, n), order="F") for m in range(F.shape[0])]
 
     def xc_grad(self, D):
         """XC nuclear gradient (natoms x 3), skeleton form with the grid fixed."""

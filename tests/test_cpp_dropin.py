@@ -40,9 +40,13 @@ def test_cpp_caller_matches_oracle(tmp_path):
     E, N = out[0], out[1]
     V = out[2:2 + n * n].reshape((n, n), order="F")
     F = out[2 + n * n:2 + 2 * n * n].reshape((n, n), order="F")  # response Fock matrix for D^x = D
-    Eu = out[2 + 2 * n * n]
-    Va = out[3 + 2 * n * n:3 + 3 * n * n].reshape((n, n), order="F")
-    Vb = out[3 + 3 * n * n:].reshape((n, n), order="F")
+    k = 2 + 2 * n * n
+    na3 = 3 * len(w["Z"])
+    Fs = [out[k + m * n * n:k + (m + 1) * n * n].reshape((n, n), order="F") for m in range(na3)]  # getFockU<s_t>
+    k += na3 * n * n
+    Eu = out[k]
+    Va = out[k + 1:k + 1 + n * n].reshape((n, n), order="F")
+    Vb = out[k + 1 + n * n:].reshape((n, n), order="F")
     orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], w["grid"], w["basis"])
     orc.set_functional(w["functional"])
     orc.get_ao()
@@ -52,6 +56,8 @@ def test_cpp_caller_matches_oracle(tmp_path):
     assert np.abs(V - Vo).max() <= 1e-9
     (Fo,) = orc.fock_u(D, [D])
     assert np.abs(F - Fo).max() <= 1e-9
+    Fso = orc.fock_s(D)
+    assert max(float(np.abs(a - b).max()) for a, b in zip(Fs, Fso)) <= 1e-9
     Euo, _, Vao, Vbo = orc.xc_pol(0.6 * D, 0.4 * D)
     assert abs(Eu - Euo) <= 1e-10
     assert np.abs(Va - Vao).max() <= 1e-9 and np.abs(Vb - Vbo).max() <= 1e-9

@@ -131,3 +131,72 @@ def test_response_protocol_errors():
     with pytest.raises(ca.ChxError) as e:  # FP64 form only
         xc.Evaluate("f", m)
     assert e.value.code == -5
+
+
+def _skeleton(mol, grid_name, func, D):
+    grid = ca.Grid(mol, grid_name, 1, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read(func)
+    grid.setType(xc.family)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    # the reference's Hessian order (Restricted/Hess.cpp:78-96)
+    grid.getAO(2, 1)
+    xc.Evaluate("f", grid)
+    grid.getDensitySkeleton([D])
+    return grid, xc, grid.getFockU(d_t="s")[0]
+
+
+@pytest.mark.parametrize("func", ["PBE", "B3LYP", "SVWN5"])
+def test_h2o_skeleton_response_against_oracle(func):
+    """Grid::getFockU<s_t>: one response matrix per nuclear coordinate from the skeleton density
+    derivatives, against the oracle (pinned by finite differences of V with one atom's basis moved)."""
+    w, mol, D = _h2o()
+    _say("skeleton response h2o/cc-pvdz/SG-1 %s, 9 coordinates" % func)
+    grid, xc, Fs = _skeleton(mol, "SG-1", func, D)
+    orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], "SG-1", w["basis"])
+    orc.set_functional(func)
+    orc.get_ao()
+    Fo = orc.fock_s(D)
+    assert len(Fs) == len(Fo) == 9
+    worst = max(float(np.abs(F - G).max()) for F, G in zip(Fs, Fo))
+    _say("  max |F - F_oracle| %.2e  max |F| %.3f" % (worst, max(float(np.abs(G).max()) for G in Fo)))
+    assert worst <= F_TOL
+    assert all(np.abs(F - F.T).max() <= 1e-12 for F in Fs)
+
+
+def test_water_cluster_skeleton_response():
+    Z, X = wl.water_cluster_angstrom(4, 5.0, 11)
+    xyz = X * ca.ANGSTROM_TO_BOHR
+    mol = ca.Molecule(Z, xyz, "def2-svp")
+    D, S = wl.workload_density(mol, 20, 3)
+    _say("skeleton response (H2O)4/def2-svp/SG-1 B3LYP, 36 coordinates")
+    grid, xc, Fs = _skeleton(mol, "SG-1", "B3LYP", D)
+    orc = ol.OracleSystem(Z, xyz, "SG-1", "def2-svp")
+    orc.set_functional("B3LYP")
+    orc.get_ao()
+    Fo = orc.fock_s(D)
+    worst = max(float(np.abs(F - G).max()) for F, G in zip(Fs, Fo))
+    _say("  max |F - F_oracle| %.2e" % worst)
+    assert worst <= F_TOL
+    # the same grid then still gives the ground-state build and the u_t response
+    E, V = ca.xc_build(grid, xc, D)
+    Eo, No, Vo = orc.xc(D)
+    assert abs(E - Eo) <= 1e-10 and np.abs(V - Vo).max() <= 1e-9
+
+
+def test_skeleton_response_protocol_errors():
+    w, mol, D = _h2o()
+    grid = ca.Grid(mol, "SG-1", 1, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    grid.setType(1)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    xc.Evaluate("f", grid)
+    with pytest.raises(ca.ChxError) as e:  # getDensitySkeleton has not run
+        grid.getFockU(d_t="s")
+    assert e.value.code == -3
+    with pytest.raises(ca.ChxError) as e:
+        grid.getAO(3, 0)
+    assert e.value.code == -5

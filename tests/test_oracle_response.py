@@ -5,7 +5,12 @@ by what the mathematics fixes, with no libxc:
     (central differences of the oracle's exact first derivatives);
   * the response Fock matrix is the linear response of V: F[D^x] = dV[D + t D^x]/dt
     at t = 0 (central differences of the oracle's full XC build);
-  * F is linear in D^x."""
+  * F is linear in D^x;
+  * the skeleton response (getFockU<s_t>) is the derivative of V[rho] with the
+    density taken from one atom's displaced basis functions (grid and the AO
+    values that assemble V held fixed): central differences of V assembled here
+    in numpy from the oracle's per-point functional, an assembly checked first
+    against the oracle's own build."""
 import numpy as np
 import pytest
 
@@ -104,3 +109,57 @@ def test_fock_u_rejects_meta_gga(h2o):
     o.get_ao()
     with pytest.raises(RuntimeError):
         o.fock_u(D, [D])
+
+
+def _df_codes(name):
+    with open("%s/DensityFunctionals/%s.df" % (ol.DATA_ROOT, name)) as f:
+        return [int(c) for c in f.readline().split()]
+
+
+@pytest.mark.parametrize("func", ["PBE", "B3LYP", "SVWN5"])
+def test_fock_s_is_skeleton_response_of_v(h2o, func):
+    o, D = h2o
+    o.set_functional(func)
+    o.get_ao()
+    codes = _df_codes(func)
+    gga = func != "SVWN5"
+    Fs = o.fock_s(D)
+    xyz, _, _, _, w = o.density_at(D)
+    ao0, g0, _ = o.ao_values(xyz)  # [mu, g], [r, mu, g]
+
+    def V_of(ao, g):  # V[rho] from the density of D built on (ao, g), assembled on the fixed (ao0, g0)
+        rho = np.einsum("mg,mn,ng->g", ao, D, ao)
+        grad = 2 * np.einsum("rmg,mn,ng->rg", g, D, ao)
+        sigma = np.sum(grad ** 2, axis=0)
+        _, vr, vs, _ = ol.eval_functional(codes, rho, sigma if gga else None)
+        V = (ao0 * (w * vr)) @ ao0.T
+        if gga:
+            Y = np.einsum("rg,rmg->mg", 2 * w * vs * grad, g0)
+            V += Y @ ao0.T + ao0 @ Y.T
+        return V
+
+    _, _, Vref = o.xc(D)
+    assert np.abs(V_of(ao0, g0) - Vref).max() <= 1e-12 * np.abs(Vref).max()
+    h = 1e-4
+    for A, t in ((0, 0), (0, 2), (1, 1), (2, 0)):
+        def V_at(eps):
+            d = np.zeros(3); d[t] = eps
+            o.shift_basis(A, d)
+            ao, g, _ = o.ao_values(xyz)
+            o.shift_basis(A, -d)
+            return V_of(ao, g)
+        d1 = (V_at(h) - V_at(-h)) / (2 * h)
+        d2 = (V_at(h / 2) - V_at(-h / 2)) / h
+        fd = (4 * d2 - d1) / 3
+        F = Fs[3 * A + t]
+        assert np.abs(F - F.T).max() <= 1e-13 * np.abs(F).max()
+        assert np.abs(F - fd).max() <= 1e-8 * np.abs(F).max(), (A, t, np.abs(F - fd).max(), np.abs(F).max())
+    assert all(np.abs(Fs[k]).max() > 0 for k in range(len(Fs)))
+
+
+def test_fock_s_rejects_meta_gga(h2o):
+    o, D = h2o
+    o.set_functional("R2SCAN")
+    o.get_ao()
+    with pytest.raises(RuntimeError):
+        o.fock_s(D)
