@@ -549,16 +549,18 @@ class ExchangeCorrelation:
 
     def eval_points(self, rho, sigma=None, tau=None, noalloc=False):
         """exc, v_rho, v_sigma, v_tau at the given points, evaluated on the device. Spin = 2:
-        rho (n x 2), sigma (n x 3) in libxc's polarised layout; returns exc, v_rho (n x 2), v_sigma (n x 3).
+        rho (n x 2), sigma (n x 3), tau (n x 2, meta-GGA) in libxc's polarised layout; returns exc,
+        v_rho (n x 2), v_sigma (n x 3) and, for a meta-GGA functional, v_tau (n x 2).
         noalloc: through chx_xc_eval_points_noalloc (fixed device staging, at most CHX_NOALLOC_MAX_POINTS)."""
         f = lib().chx_xc_eval_points_noalloc if noalloc else lib().chx_xc_eval_points
         if self.Spin == 2:
             rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
             n = rho.shape[0]
             sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1, 3)
-            e, vr, vs = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
-            _check(f(self.h, n, _ptr(rho), _ptr(sg), None, _ptr(e), _ptr(vr), _ptr(vs), None))
-            return e, vr, vs
+            ta = None if tau is None else np.ascontiguousarray(tau, dtype=np.float64).reshape(-1, 2)
+            e, vr, vs, vt = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3)), np.zeros((n, 2))
+            _check(f(self.h, n, _ptr(rho), _ptr(sg), _ptr(ta), _ptr(e), _ptr(vr), _ptr(vs), _ptr(vt)))
+            return (e, vr, vs, vt) if self.family >= 2 else (e, vr, vs)
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         n = len(rho)
         sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)

@@ -93,13 +93,15 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
   g->d_codes.upload(xc->codes, g->stream);
   hipEvent_t e = nullptr;
   g->surrogate_used = false;
-  if (g->nspin == 2) {  // spin-polarised LDA/GGA: alpha coefficients in d_coef, beta in d_coef_b
-    if (xc->family > 1) throw Error(CHX_ERR_UNSUPPORTED, "the spin-polarised path is built for LDA and GGA functionals");
+  if (g->nspin == 2) {  // spin-polarised: alpha coefficients in d_coef, beta in d_coef_b
+    if (xc->family > g->type)
+      throw Error(CHX_ERR_STATE, "the grid's type holds no tau for this meta-GGA functional (Grid::setType)");
     g->d_coef_b.alloc(g->d_coef.n);
     g->mark_begin(2, &e);
-    launch_functional_pol(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_b.p, g->d_w.p, g->d_codes.p,
-                          (int)xc->codes.size(), xc->family, want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p, g->d_coef.p,
-                          g->d_coef_b.p, g->d_en_part.p, g->en_blocks);
+    launch_functional_pol(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_b.p, g->d_part_tau.p,
+                          xc->family >= 2 ? g->d_part_tau_b.p : nullptr, g->d_w.p, g->d_codes.p, (int)xc->codes.size(),
+                          xc->family, want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p, g->d_coef.p, g->d_coef_b.p,
+                          g->d_en_part.p, g->en_blocks);
     launch_reduce_en(g->stream, g->en_blocks, g->d_en_part.p, g->d_en.p);
     g->mark_end(2, e);
     return;
@@ -237,11 +239,15 @@ void check_dm_args(const chx_grid* g, int nspin, int nbasis, int max_spin = 1) {
 // per spin index).
 void density_spin(chx_grid* g, const double* dD, int nspin) {
   if (nspin == 2) {
-    require(g->type <= 1, CHX_ERR_UNSUPPORTED, "the spin-polarised path is built for LDA and GGA grids");
     density_from_device(g, dD + (size_t)g->nbasis * g->nbasis);
     g->d_part_b.alloc(g->d_part.n);
     CHX_HIP(hipMemcpyAsync(g->d_part_b.p, g->d_part.p, sizeof(double) * g->d_part.n, hipMemcpyDeviceToDevice,
                            g->stream));
+    if (g->type >= 2) {  // the tau partials of beta too
+      g->d_part_tau_b.alloc(g->d_part_tau.n);
+      CHX_HIP(hipMemcpyAsync(g->d_part_tau_b.p, g->d_part_tau.p, sizeof(double) * g->d_part_tau.n,
+                             hipMemcpyDeviceToDevice, g->stream));
+    }
   }
   density_from_device(g, dD);
   g->nspin = nspin;
@@ -815,14 +821,20 @@ int chx_xc_create(int ncodes, const int* codes, int spin, chx_xc** out) {
         delete x;
         throw Error(CHX_ERR_UNSUPPORTED, "functional code " + std::to_string(codes[i]) + " is not implemented");
       }
-      if (spin == 2 && f > 1) {
+      if (spin == 2 && f > 1 && codes[i] != kMggaXR2scan && codes[i] != kMggaCR2scan) {
         delete x;
-        throw Error(CHX_ERR_UNSUPPORTED, "spin-polarised meta-GGA functionals are not built");
+        throw Error(CHX_ERR_UNSUPPORTED, "of the meta-GGA functionals, r2SCAN is built spin-polarised");
       }
       x->codes.push_back(codes[i]);
       x->family = std::max(x->family, f);
       x->exx = std::max(x->exx, code_exx(codes[i]));
     }
+    if (spin == 2 && x->family >= 2)
+      for (int c : x->codes)
+        if (c != kMggaXR2scan && c != kMggaCR2scan) {
+          delete x;
+          throw Error(CHX_ERR_UNSUPPORTED, "a spin-polarised meta-GGA functional is built from r2SCAN codes only");
+        }
     *out = x;
   });
 }
@@ -910,25 +922,28 @@ int chx_lebedev_rule(int npoints, double* xyzw) {
 int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
                        double* exc, double* vrho, double* vsigma, double* vtau) {
   if (xc && xc->spin == 2)
-    return guard([&] {  // libxc XC_POLARIZED layout: rho[2n], sigma[3n], vrho[2n], vsigma[3n]
+    return guard([&] {  // libxc XC_POLARIZED layout: rho[2n], sigma[3n], tau[2n], vrho[2n], vsigma[3n], vtau[2n]
       require(n >= 0 && rho && exc && vrho, CHX_ERR_ARG, "bad arguments");
+      const bool mgga = xc->family >= 2;
+      require(!mgga || (sigma && tau), CHX_ERR_ARG, "a meta-GGA functional needs sigma and tau");
       if (n == 0) return;
       hipStream_t s;
       CHX_HIP(hipStreamCreate(&s));
-      DevBuf<double> dr, ds, de, dvr, dvs;
+      DevBuf<double> dr, ds, dt, de, dvr, dvs, dvt;
       DevBuf<int> dc;
       dr.upload(std::vector<double>(rho, rho + 2 * n), s);
       if (sigma) ds.upload(std::vector<double>(sigma, sigma + 3 * n), s);
+      if (mgga) dt.upload(std::vector<double>(tau, tau + 2 * n), s);
       dc.upload(xc->codes, s);
-      de.alloc(n); dvr.alloc(2 * n); dvs.alloc(3 * n);
-      launch_eval_functional_pol(s, n, dc.p, (int)xc->codes.size(), dr.p, sigma ? ds.p : nullptr, de.p, dvr.p, dvs.p);
+      de.alloc(n); dvr.alloc(2 * n); dvs.alloc(3 * n); dvt.alloc(2 * n);
+      launch_eval_functional_pol(s, n, dc.p, (int)xc->codes.size(), dr.p, sigma ? ds.p : nullptr, mgga ? dt.p : nullptr,
+                                 de.p, dvr.p, dvs.p, dvt.p);
       CHX_HIP(hipMemcpyAsync(exc, de.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
       CHX_HIP(hipMemcpyAsync(vrho, dvr.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
       if (vsigma) CHX_HIP(hipMemcpyAsync(vsigma, dvs.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+      if (vtau && mgga) CHX_HIP(hipMemcpyAsync(vtau, dvt.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
       CHX_HIP(hipStreamSynchronize(s));
       CHX_HIP(hipStreamDestroy(s));
-      (void)tau;
-      (void)vtau;
     });
   return guard([&] {
     require(xc && n >= 0 && rho && exc && vrho, CHX_ERR_ARG, "bad arguments");

@@ -150,12 +150,15 @@ void launch_response_coef(hipStream_t s, int nbatches, const double* rhou, const
                           int family, double* coef);
 void launch_eval_functional_f(hipStream_t s, long n, const int* codes, int ncodes, const double* rho,
                               const double* sigma, double* v2rho2, double* v2rhosigma, double* v2sigma2);
-// spin-polarised (nspin = 2) LDA/GGA functional
+// spin-polarised (nspin = 2) functional, LDA/GGA and r2SCAN (part_tau / part_tau_b: meta-GGA)
 void launch_functional_pol(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* part_b,
-                           const double* bw, const int* codes, int ncodes, int family, int want_e, int want_v,
-                           double* rho_out, double* coef, double* coef_b, double* en_part, int en_blocks);
+                           const double* part_tau, const double* part_tau_b, const double* bw, const int* codes,
+                           int ncodes, int family, int want_e, int want_v, double* rho_out, double* coef,
+                           double* coef_b, double* en_part, int en_blocks);
+// at points, libxc's polarised layout; tau / vtau (meta-GGA) may be null
 void launch_eval_functional_pol(hipStream_t s, long n, const int* codes, int ncodes, const double* rho,
-                                const double* sigma, double* exc, double* vrho, double* vsigma);
+                                const double* sigma, const double* tau, double* exc, double* vrho, double* vsigma,
+                                double* vtau);
 // Surrogate network (kernels_mlp.hip): 3 -> 64 -> 64 -> 64 -> 1, tanh. Offsets
 // of the parameter block (floats); matrices are [out][in] (y = W x + b). The
 // run copies the kernels read hold W2 transposed ([in][out]) at the same offset.
@@ -415,7 +418,7 @@ struct chx_grid {
   // spin-polarised densities (nspin = 2): beta partials and coefficients; alpha in d_part / d_coef
   int nspin = 1;    // spin count of the density on the grid
   int v_nspin = 1;  // spin count of the potential on the grid
-  chx::DevBuf<double> d_part_b, d_coef_b, d_coef_save;
+  chx::DevBuf<double> d_part_b, d_part_tau_b, d_coef_b, d_coef_save;
   // surrogate functional: points that fell back to the analytic functional in the last Evaluate
   chx::DevBuf<int> d_nfall;
   bool surrogate_used = false;

@@ -355,16 +355,19 @@ void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const do
   CHX_HIP(hipGetLastError());
 }
 
-// Spin-polarised functional (LDA/GGA), one thread per slot: the density
-// partials of each spin's density matrix (part: alpha, part_b: beta), the
-// polarised functional, and the Vxc integrand coefficients of each spin
-// (coef: alpha, coef_b: beta):
+// Spin-polarised functional, one thread per slot: the density partials of
+// each spin's density matrix (part / part_tau: alpha, part_b / part_tau_b:
+// beta), the polarised functional, and the Vxc integrand coefficients of each
+// spin (coef: alpha, coef_b: beta):
 //   a_u = w v_rho_u / 2,  c_u = w (2 v_sigma_uu grad rho_u + v_sigma_ab grad rho_u'),
-// the scaled sigma derivatives of src/Grid/GridFock.cpp:21-30 at nspin = 2.
-// Block partial sums of E = sum w (rho_a + rho_b) eps and N.
+//   t_u = w v_tau_u / 4 (meta-GGA),
+// the scaled sigma derivatives of src/Grid/GridFock.cpp:21-30 and the tau
+// term of :32-40 at nspin = 2. Block partial sums of E = sum w (rho_a + rho_b) eps and N.
 __global__ __launch_bounds__(256) void k_functional_pol(int nslots, const int* __restrict__ grp_off,
                                                         const double* __restrict__ part,
                                                         const double* __restrict__ part_b,
+                                                        const double* __restrict__ part_tau,
+                                                        const double* __restrict__ part_tau_b,
                                                         const double* __restrict__ bw, const int* __restrict__ codes,
                                                         int ncodes, int family, int want_e, int want_v,
                                                         double* __restrict__ rho_out, double* __restrict__ coef,
@@ -374,7 +377,7 @@ __global__ __launch_bounds__(256) void k_functional_pol(int nslots, const int* _
   double e_acc = 0, n_acc = 0;
   if (slot < nslots) {
     const int b = slot / kBatch, g = slot % kBatch;
-    double ra = 0, rb = 0, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+    double ra = 0, rb = 0, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0}, ta = 0, tb = 0;
     for (int k = grp_off[b]; k < grp_off[b + 1]; ++k) {
       const double* p = part + (size_t)k * 4 * kBatch;
       const double* q = part_b + (size_t)k * 4 * kBatch;
@@ -385,12 +388,20 @@ __global__ __launch_bounds__(256) void k_functional_pol(int nslots, const int* _
         ga[r] += p[(r + 1) * kBatch + g];
         gb[r] += q[(r + 1) * kBatch + g];
       }
+      if (family >= 2) {
+        const double* pt = part_tau + (size_t)k * 3 * kBatch;
+        const double* qt = part_tau_b + (size_t)k * 3 * kBatch;
+        ta += pt[g] + pt[kBatch + g] + pt[2 * kBatch + g];
+        tb += qt[g] + qt[kBatch + g] + qt[2 * kBatch + g];
+      }
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       ga[r] *= 2;
       gb[r] *= 2;
     }
+    ta *= 0.5;
+    tb *= 0.5;
     const double w = bw[slot];
     double saa = 0, sab = 0, sbb = 0;
     if (family >= 1) {
@@ -401,8 +412,11 @@ __global__ __launch_bounds__(256) void k_functional_pol(int nslots, const int* _
         sbb += gb[r] * gb[r];
       }
     }
-    double eps, vr[2], vs[3];
-    eval_codes_pol(codes, ncodes, ra, rb, saa, sab, sbb, &eps, vr, vs);
+    double eps, vr[2], vs[3], vt[2] = {0, 0};
+    if (family >= 2)
+      eval_codes_mpol(codes, ncodes, ra, rb, saa, sab, sbb, ta, tb, &eps, vr, vs, vt);
+    else
+      eval_codes_pol(codes, ncodes, ra, rb, saa, sab, sbb, &eps, vr, vs);
     rho_out[slot] = ra + rb;
     if (want_v) {
       const double fa = family >= 1 ? w : 0.0;
@@ -413,8 +427,8 @@ __global__ __launch_bounds__(256) void k_functional_pol(int nslots, const int* _
         coef[kCoef * (size_t)slot + 1 + r] = fa * (2.0 * vs[0] * ga[r] + vs[1] * gb[r]);
         coef_b[kCoef * (size_t)slot + 1 + r] = fa * (2.0 * vs[2] * gb[r] + vs[1] * ga[r]);
       }
-      coef[kCoef * (size_t)slot + 4] = 0.0;
-      coef_b[kCoef * (size_t)slot + 4] = 0.0;
+      coef[kCoef * (size_t)slot + 4] = family >= 2 ? 0.25 * w * vt[0] : 0.0;
+      coef_b[kCoef * (size_t)slot + 4] = family >= 2 ? 0.25 * w * vt[1] : 0.0;
     }
     if (want_e) e_acc = w * (ra + rb) * eps;
     n_acc = w * (ra + rb);
@@ -436,35 +450,47 @@ __global__ __launch_bounds__(256) void k_functional_pol(int nslots, const int* _
 }
 
 void launch_functional_pol(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* part_b,
-                           const double* bw, const int* codes, int ncodes, int family, int want_e, int want_v,
-                           double* rho_out, double* coef, double* coef_b, double* en_part, int en_blocks) {
+                           const double* part_tau, const double* part_tau_b, const double* bw, const int* codes,
+                           int ncodes, int family, int want_e, int want_v, double* rho_out, double* coef,
+                           double* coef_b, double* en_part, int en_blocks) {
   const int nslots = nbatches * kBatch;
-  hipLaunchKernelGGL(k_functional_pol, dim3(en_blocks), dim3(256), 0, s, nslots, grp_off, part, part_b, bw, codes,
-                     ncodes, family, want_e, want_v, rho_out, coef, coef_b, en_part);
+  hipLaunchKernelGGL(k_functional_pol, dim3(en_blocks), dim3(256), 0, s, nslots, grp_off, part, part_b, part_tau,
+                     part_tau_b, bw, codes, ncodes, family, want_e, want_v, rho_out, coef, coef_b, en_part);
   CHX_HIP(hipGetLastError());
 }
 
-// Spin-polarised functional at arbitrary points, libxc XC_POLARIZED layout.
+// Spin-polarised functional at arbitrary points, libxc XC_POLARIZED layout;
+// meta-GGA (tau non-null): tau[2n] in, vtau[2n] out.
 __global__ void k_eval_functional_pol(long n, const int* __restrict__ codes, int ncodes, const double* __restrict__ rho,
-                                      const double* __restrict__ sigma, double* __restrict__ exc,
-                                      double* __restrict__ vrho, double* __restrict__ vsigma) {
+                                      const double* __restrict__ sigma, const double* __restrict__ tau,
+                                      double* __restrict__ exc, double* __restrict__ vrho, double* __restrict__ vsigma,
+                                      double* __restrict__ vtau) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double eps, vr[2], vs[3];
-  eval_codes_pol(codes, ncodes, rho[2 * i], rho[2 * i + 1], sigma ? sigma[3 * i] : 0.0, sigma ? sigma[3 * i + 1] : 0.0,
-                 sigma ? sigma[3 * i + 2] : 0.0, &eps, vr, vs);
+  double eps, vr[2], vs[3], vt[2] = {0, 0};
+  const double saa = sigma ? sigma[3 * i] : 0.0, sab = sigma ? sigma[3 * i + 1] : 0.0, sbb = sigma ? sigma[3 * i + 2] : 0.0;
+  if (tau)
+    eval_codes_mpol(codes, ncodes, rho[2 * i], rho[2 * i + 1], saa, sab, sbb, tau[2 * i], tau[2 * i + 1], &eps, vr, vs,
+                    vt);
+  else
+    eval_codes_pol(codes, ncodes, rho[2 * i], rho[2 * i + 1], saa, sab, sbb, &eps, vr, vs);
   exc[i] = eps;
   vrho[2 * i] = vr[0];
   vrho[2 * i + 1] = vr[1];
   if (vsigma)
     for (int k = 0; k < 3; ++k) vsigma[3 * i + k] = vs[k];
+  if (vtau) {
+    vtau[2 * i] = vt[0];
+    vtau[2 * i + 1] = vt[1];
+  }
 }
 
 void launch_eval_functional_pol(hipStream_t s, long n, const int* codes, int ncodes, const double* rho,
-                                const double* sigma, double* exc, double* vrho, double* vsigma) {
+                                const double* sigma, const double* tau, double* exc, double* vrho, double* vsigma,
+                                double* vtau) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_eval_functional_pol, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, codes, ncodes, rho,
-                     sigma, exc, vrho, vsigma);
+                     sigma, tau, exc, vrho, vsigma, vtau);
   CHX_HIP(hipGetLastError());
 }
 

@@ -73,6 +73,8 @@ int chx_xc_eval_points_noalloc(const chx_xc* xc, long n, const double* rho, cons
     const int ncodes = (int)xc->codes.size();
     if (ncodes > kNoAllocMaxCodes) throw Error(CHX_ERR_ARG, "too many functional codes for the scratch array");
     if (n == 0) return CHX_OK;
+    if (xc->spin == 2 && xc->family >= 2)
+      throw Error(CHX_ERR_UNSUPPORTED, "chx_xc_eval_points_noalloc takes spin-polarised LDA and GGA functionals");
     double* base = scratch_on_current_device();
     int* codes = reinterpret_cast<int*>(base);
     double* a = base + kNoAllocCodeDoubles;
@@ -82,7 +84,8 @@ int chx_xc_eval_points_noalloc(const chx_xc* xc, long n, const double* rho, cons
       double *r = a, *sg = a + 2 * m, *e = a + 5 * m, *vr = a + 6 * m, *vs = a + 8 * m;
       up(r, rho, 2 * m);
       if (sigma) up(sg, sigma, 3 * m);
-      launch_eval_functional_pol(nullptr, n, codes, ncodes, r, sigma ? sg : nullptr, e, vr, vsigma ? vs : nullptr);
+      launch_eval_functional_pol(nullptr, n, codes, ncodes, r, sigma ? sg : nullptr, nullptr, e, vr,
+                                 vsigma ? vs : nullptr, nullptr);
       down(exc, e, m);
       down(vrho, vr, 2 * m);
       down(vsigma, vs, 3 * m);

@@ -194,10 +194,11 @@ __device__ __forceinline__ T e_lyp_c(const T& rho, const T& rho13, const T& sigm
 // ---- r2SCAN (Furness, Kaplan, Ning, Perdew, Sun, JPCL 11, 8208 (2020) and
 // erratum), spin-unpolarised: p = s^2, alpha_bar = (tau - tau_W)/(tau_unif + eta tau_W).
 struct R2Poly { double c[8]; double c1, c2, d; };
-__device__ __forceinline__ D3 r2_interp(const D3& a, const R2Poly& f) {
+template <class T>
+__device__ __forceinline__ T r2_interp(const T& a, const R2Poly& f) {
   if (a.v <= 0.0) return dexp(-f.c1 * a / (1.0 - a));
   if (a.v <= 2.5) {
-    D3 r = dconst(f.c[7]);
+    T r = a * 0.0 + f.c[7];
 #pragma unroll
     for (int i = 6; i >= 0; --i) r = r * a + f.c[i];
     return r;
@@ -221,27 +222,29 @@ __device__ __forceinline__ R2Poly r2_fc() {
 constexpr double kR2Eta = 0.001;
 constexpr double kR2Dp2_4 = 0.361 * 0.361 * 0.361 * 0.361;
 
-__device__ __forceinline__ void r2_descriptors(const D3& rho, const D3& rho13, const D3& sigma, const D3& tau,
-                                               D3* p, D3* abar) {
+template <class T>
+__device__ __forceinline__ void r2_descriptors(const T& rho, const T& rho13, const T& sigma, const T& tau, T* p,
+                                               T* abar) {
   const double k2 = 9.570780000627305;  // (3 pi^2)^(2/3)
-  const D3 r53 = rho * rho13 * rho13;
+  const T r53 = rho * rho13 * rho13;
   *p = sigma / ((4.0 * k2) * rho * r53);
-  const D3 tw = sigma / (8.0 * rho);
-  const D3 tu = (0.3 * k2) * r53;
+  const T tw = sigma / (8.0 * rho);
+  const T tu = (0.3 * k2) * r53;
   *abar = (tau - tw) / (tu + kR2Eta * tw);
 }
 
-__device__ __forceinline__ D3 e_r2scan_x(const D3& rho, const D3& rho13, const D3& sigma, const D3& tau) {
+template <class T>
+__device__ __forceinline__ T e_r2scan_x(const T& rho, const T& rho13, const T& sigma, const T& tau) {
   const double h0x = 1.174, k1 = 0.065, muak = 10.0 / 81.0, a1 = 4.9479;
   const R2Poly fxp = r2_fx();
   const double c2 = -r2_slope(fxp) * (1.0 - h0x);
   const double ceta = 20.0 / 27.0 + kR2Eta * 5.0 / 3.0;
-  D3 p, ab;
+  T p, ab;
   r2_descriptors(rho, rho13, sigma, tau, &p, &ab);
-  const D3 x = (ceta * c2 * dexp(-(p * p) / kR2Dp2_4) + muak) * p;
-  const D3 h1x = (1.0 + k1) - k1 / (1.0 + x / k1);
-  const D3 fx = r2_interp(ab, fxp);
-  const D3 gx = p.v > 0.0 ? 1.0 - dexp(-a1 / dpow(p, 0.25)) : dconst(1.0);
+  const T x = (ceta * c2 * dexp(-(p * p) / kR2Dp2_4) + muak) * p;
+  const T h1x = (1.0 + k1) - k1 / (1.0 + x / k1);
+  const T fx = r2_interp(ab, fxp);
+  const T gx = p.v > 0.0 ? 1.0 - dexp(-a1 / dpow(p, 0.25)) : p * 0.0 + 1.0;
   return e_lda_x(rho, rho13) * (h1x + fx * (h0x - h1x)) * gx;
 }
 
@@ -448,7 +451,8 @@ __device__ __forceinline__ P5 dpowp(const P5& a, double p) {
 
 // f(zeta) of the spin interpolation and its curvature at 0, 4 / (9 (2^(1/3) - 1)).
 constexpr double kFpp0 = 1.709920934161365617563962776245;
-__device__ __forceinline__ P5 f_zeta(const P5& opz, const P5& omz) {  // opz = 1 + zeta, omz = 1 - zeta
+template <class T>
+__device__ __forceinline__ T f_zeta(const T& opz, const T& omz) {  // opz = 1 + zeta, omz = 1 - zeta
   return (opz * dcbrt(opz) + omz * dcbrt(omz) - 2.0) / (2.5198420997897464 - 2.0);  // 2^(4/3) - 2
 }
 
@@ -470,10 +474,19 @@ __device__ __forceinline__ P5 vwn_rpa_eps_pol(const P5& rs, const P5& opz, const
 }
 
 // PW92 G(rs) of Perdew-Wang 1992 eq. 10.
-__device__ __forceinline__ P5 pw92_g(const P5& rs, double A, double a1, double b1, double b2, double b3, double b4) {
-  const P5 srs = dsqrt(rs);
-  const P5 den = (2.0 * A) * (srs * (b1 + b3 * rs) + rs * (b2 + b4 * rs));
+template <class T>
+__device__ __forceinline__ T pw92_g(const T& rs, double A, double a1, double b1, double b2, double b3, double b4) {
+  const T srs = dsqrt(rs);
+  const T den = (2.0 * A) * (srs * (b1 + b3 * rs) + rs * (b2 + b4 * rs));
   return (-2.0 * A) * (1.0 + a1 * rs) * dlog(1.0 + 1.0 / den);
+}
+// dG/drs of the same G.
+template <class T>
+__device__ __forceinline__ T pw92_g_drs(const T& rs, double A, double a1, double b1, double b2, double b3, double b4) {
+  const T srs = dsqrt(rs);
+  const T den = (2.0 * A) * (srs * (b1 + b3 * rs) + rs * (b2 + b4 * rs));
+  const T dden = (2.0 * A) * ((0.5 * b1) / srs + b2 + (1.5 * b3) * srs + (2.0 * b4) * rs);
+  return (-2.0 * A * a1) * dlog(1.0 + 1.0 / den) + (2.0 * A) * (1.0 + a1 * rs) * dden / (den * den + den);
 }
 // PW92 with spin (eq. 8), libxc PW_MOD parameters; -alpha_c = G(rs; A_alpha, ...).
 __device__ __forceinline__ P5 pw92_eps_pol(const P5& rs, const P5& z, const P5& opz, const P5& omz) {
@@ -601,6 +614,178 @@ __device__ __forceinline__ void eval_codes_pol(const int* codes, int ncodes, dou
   vs[0] = d[2];
   vs[1] = d[3];
   vs[2] = d[4];
+}
+
+// ---- spin-polarised r2SCAN (libxc 497 / 498, XC_POLARIZED conventions) -----
+// Variables rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb, tau_a, tau_b; the
+// number type P7 carries the seven partials. Exchange by spin scaling,
+// E_x = (E_x[2 ra, 4 saa, 2 ta] + E_x[2 rb, 4 sbb, 2 tb]) / 2. Correlation with
+// SCAN's spin dependence (Sun, Ruzsinszky, Perdew, PRL 115, 036402 (2015)):
+// phi, d_s = ((1+z)^(5/3) + (1-z)^(5/3))/2 in tau_unif, G_c(z) =
+// (1 - 2.3631 (d_x - 1))(1 - z^12) on eps_c^0, the polarised PW92 in eps_c^1,
+// and r2SCAN's Delta y with eps_LSDA0 = eps_LDA0 G_c and 1 / (d_s phi^3).
+struct P7 {
+  double v, d[7];
+};
+__device__ __forceinline__ P7 qconst(double x) { return {x, {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}}; }
+__device__ __forceinline__ P7 qvar(double x, int k) {
+  P7 r = qconst(x);
+  r.d[k] = 1.0;
+  return r;
+}
+__device__ __forceinline__ P7 qchain(const P7& a, double f, double df) {
+  P7 r;
+  r.v = f;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) r.d[k] = df * a.d[k];
+  return r;
+}
+__device__ __forceinline__ P7 operator+(const P7& a, const P7& b) {
+  P7 r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+__device__ __forceinline__ P7 operator-(const P7& a) { return qchain(a, -a.v, -1.0); }
+__device__ __forceinline__ P7 operator-(const P7& a, const P7& b) { return a + (-b); }
+__device__ __forceinline__ P7 operator*(const P7& a, const P7& b) {
+  P7 r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+__device__ __forceinline__ P7 operator/(const P7& a, const P7& b) {
+  const double ib = 1.0 / b.v, q = a.v * ib;
+  P7 r;
+  r.v = q;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) r.d[k] = (a.d[k] - q * b.d[k]) * ib;
+  return r;
+}
+__device__ __forceinline__ P7 operator+(const P7& a, double b) { P7 r = a; r.v += b; return r; }
+__device__ __forceinline__ P7 operator+(double a, const P7& b) { return b + a; }
+__device__ __forceinline__ P7 operator-(const P7& a, double b) { return a + (-b); }
+__device__ __forceinline__ P7 operator-(double a, const P7& b) { return (-b) + a; }
+__device__ __forceinline__ P7 operator*(const P7& a, double b) { return qchain(a, a.v * b, b); }
+__device__ __forceinline__ P7 operator*(double a, const P7& b) { return b * a; }
+__device__ __forceinline__ P7 operator/(const P7& a, double b) { return a * (1.0 / b); }
+__device__ __forceinline__ P7 operator/(double a, const P7& b) { return qconst(a) / b; }
+__device__ __forceinline__ P7 dexp(const P7& a) { const double e = ::exp(a.v); return qchain(a, e, e); }
+__device__ __forceinline__ P7 dlog(const P7& a) { return qchain(a, ::log(a.v), 1.0 / a.v); }
+__device__ __forceinline__ P7 dsqrt(const P7& a) { const double s = ::sqrt(a.v); return qchain(a, s, 0.5 / s); }
+__device__ __forceinline__ P7 dcbrt(const P7& a) { const double c = ::cbrt(a.v); return qchain(a, c, c / (3.0 * a.v)); }
+__device__ __forceinline__ P7 dpow(const P7& a, double p) { const double f = ::pow(a.v, p); return qchain(a, f, p * f / a.v); }
+
+__device__ __forceinline__ P7 e_r2scan_c_pol(const P7& rho, const P7& rho13, const P7& sigma, const P7& tau,
+                                             const P7& z, const P7& opz, const P7& omz) {
+  const double gamma = 0.031090690869654895;
+  const double b1c = 0.0285764, b2c = 0.0889, b3c = 0.125541, chi_inf = 0.128026;
+  const double k2 = 9.570780000627305;  // (3 pi^2)^(2/3)
+  const R2Poly fcp = r2_fc();
+  const P7 c1 = dcbrt(opz), c2 = dcbrt(omz);
+  const P7 phi = 0.5 * (c1 * c1 + c2 * c2);
+  const P7 ds = 0.5 * (opz * c1 * c1 + omz * c2 * c2);
+  const P7 dx = 0.5 * (opz * c1 + omz * c2);
+  const P7 z2 = z * z, z4 = z2 * z2;
+  const P7 Gc = (1.0 - 2.3631 * (dx - 1.0)) * (1.0 - z4 * z4 * z4);
+  const P7 r53 = rho * rho13 * rho13;
+  const P7 p = sigma / ((4.0 * k2) * rho * r53);
+  const P7 tw = sigma / (8.0 * rho);
+  const P7 ab = (tau - tw) / ((0.3 * k2) * r53 * ds + kR2Eta * tw);
+  const P7 rs = rs_from_rho13(rho13);
+  const P7 srs = dsqrt(rs);
+  // polarised PW92 and its rs-derivative at fixed zeta
+  const double q0[6] = {0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294};
+  const double q1[6] = {0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517};
+  const double qa[6] = {0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671};
+  const P7 g0 = pw92_g(rs, q0[0], q0[1], q0[2], q0[3], q0[4], q0[5]);
+  const P7 g1 = pw92_g(rs, q1[0], q1[1], q1[2], q1[3], q1[4], q1[5]);
+  const P7 ga = pw92_g(rs, qa[0], qa[1], qa[2], qa[3], qa[4], qa[5]);
+  const P7 dg0 = pw92_g_drs(rs, q0[0], q0[1], q0[2], q0[3], q0[4], q0[5]);
+  const P7 dg1 = pw92_g_drs(rs, q1[0], q1[1], q1[2], q1[3], q1[4], q1[5]);
+  const P7 dga = pw92_g_drs(rs, qa[0], qa[1], qa[2], qa[3], qa[4], qa[5]);
+  const P7 f = f_zeta(opz, omz);
+  const P7 fs = f * (1.0 - z4) / kFpp0, fz = f * z4;
+  const P7 e1 = g0 - ga * fs + (g1 - g0) * fz;
+  const P7 de1 = dg0 - dga * fs + (dg1 - dg0) * fz;
+  const P7 den0 = 1.0 + b2c * srs + b3c * rs;
+  const P7 e0 = -b1c / den0;
+  const P7 de0 = b1c * ((0.5 * b2c) / srs + b3c) / (den0 * den0);
+  const P7 phi3 = phi * phi * phi;
+  const P7 beta = 0.066725 * (1.0 + 0.1 * rs) / (1.0 + 0.1778 * rs);
+  const P7 w1 = dexp(-e1 / (gamma * phi3)) - 1.0;
+  const P7 t2 = 1.5073033983379013 * p / (phi * phi * rs);  // (3 pi^2 / 16)^(2/3) p / (phi^2 rs)
+  const P7 y = beta * t2 / (gamma * w1);
+  const P7 dy = (r2_slope(fcp) / (27.0 * gamma)) / (ds * phi3 * w1) *
+                (20.0 * rs * (de0 * Gc - de1) - (45.0 * kR2Eta) * (e0 * Gc - e1)) * p * dexp(-(p * p) / kR2Dp2_4);
+  const P7 g = dpow(1.0 + 4.0 * (y - dy), -0.25);
+  const P7 ec1 = e1 + gamma * phi3 * dlog(1.0 + w1 * (1.0 - g));
+  const P7 w0 = dexp(-e0 / b1c) - 1.0;
+  const P7 ginf = dpow(1.0 + (4.0 * chi_inf) * p, -0.25);
+  const P7 ec0 = (e0 + b1c * dlog(1.0 + w0 * (1.0 - ginf))) * Gc;
+  const P7 fc = r2_interp(ab, fcp);
+  return rho * (ec1 + fc * (ec0 - ec1));
+}
+
+// Sum over meta-GGA codes at one point, spin-polarised: eps (per particle of
+// rho_a + rho_b), v_rho (a, b), v_sigma (aa, ab, bb), v_tau (a, b). Density
+// threshold on the total; an exchange channel below it contributes nothing;
+// per spin tau_s >= 1e-20, 1e-40 <= sigma_ss <= 8 rho_s tau_s, and
+// |sigma_ab| <= sqrt(sigma_aa sigma_bb).
+__device__ __forceinline__ void eval_codes_mpol(const int* codes, int ncodes, double ra, double rb, double saa,
+                                                double sab, double sbb, double ta, double tb, double* eps, double* vr,
+                                                double* vs, double* vt) {
+  double e = 0, d[7] = {0, 0, 0, 0, 0, 0, 0};
+  ra = ra > 0 ? ra : 0.0;
+  rb = rb > 0 ? rb : 0.0;
+  const double rho = ra + rb;
+  if (rho >= kDensThreshold) {
+    ta = ta > 1e-20 ? ta : 1e-20;
+    tb = tb > 1e-20 ? tb : 1e-20;
+    const double ca = 8.0 * ra * ta > 1e-40 ? 8.0 * ra * ta : 1e-40, cb = 8.0 * rb * tb > 1e-40 ? 8.0 * rb * tb : 1e-40;
+    saa = saa > 1e-40 ? (saa < ca ? saa : ca) : 1e-40;
+    sbb = sbb > 1e-40 ? (sbb < cb ? sbb : cb) : 1e-40;
+    const double cap = ::sqrt(saa * sbb);
+    sab = sab > cap ? cap : (sab < -cap ? -cap : sab);
+    const P7 pa = qvar(ra, 0), pb = qvar(rb, 1), paa = qvar(saa, 2), pab = qvar(sab, 3), pbb = qvar(sbb, 4);
+    const P7 pta = qvar(ta, 5), ptb = qvar(tb, 6);
+    const bool a_on = ra >= kDensThreshold, b_on = rb >= kDensThreshold;
+    const P7 prho = pa + pb;
+    const P7 r13 = dcbrt(prho);
+    P7 z = (pa - pb) / prho;
+    if (z.v > 1.0 - 1e-15) z = qconst(1.0 - 1e-15);  // zeta threshold: fully polarised limit
+    if (z.v < -1.0 + 1e-15) z = qconst(-1.0 + 1e-15);
+    const P7 opz = 1.0 + z, omz = 1.0 - z;
+    for (int k = 0; k < ncodes; ++k) {
+      P7 ed = qconst(0.0);
+      if (codes[k] == kMggaXR2scan) {
+        if (a_on) {
+          const P7 r2 = 2.0 * pa;
+          ed = ed + 0.5 * e_r2scan_x(r2, dcbrt(r2), 4.0 * paa, 2.0 * pta);
+        }
+        if (b_on) {
+          const P7 r2 = 2.0 * pb;
+          ed = ed + 0.5 * e_r2scan_x(r2, dcbrt(r2), 4.0 * pbb, 2.0 * ptb);
+        }
+      } else if (codes[k] == kMggaCR2scan) {
+        ed = e_r2scan_c_pol(prho, r13, paa + 2.0 * pab + pbb, pta + ptb, z, opz, omz);
+      }
+      e += ed.v;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) d[j] += ed.d[j];
+    }
+    e /= rho;
+  }
+  *eps = e;
+  vr[0] = d[0];
+  vr[1] = d[1];
+  vs[0] = d[2];
+  vs[1] = d[3];
+  vs[2] = d[4];
+  vt[0] = d[5];
+  vt[1] = d[6];
 }
 
 }  // namespace chx

@@ -112,7 +112,7 @@ class RestrictedSCF:
 
 class UnrestrictedSCF:
     """na alpha and nb beta electrons; xc = None gives UHF (EXX = 1). With xc set (Spin = 2 before
-    Read, LDA/GGA), grid must be a Grid of the molecule, typed and planned.
+    Read; LDA/GGA or r2SCAN), grid must be a Grid of the molecule, typed and planned.
 
     Mirrors the unrestricted update (src/HartreeFockKohnSham/Unrestricted/SP.cpp:42-80): orbitals of
     F_a and F_b, D_s = C_s,occ C_s,occ^T, J from D_a + D_b and K_s = EXX (mu la|nu si) D_s,la,si per

@@ -102,8 +102,9 @@ int chx_grid_set_type(chx_grid* g, int type);
 int chx_grid_get_ao(chx_grid* g, int derivative, int output);
 /* Grid::getDensity(std::vector<EigenMatrix> D) (src/Grid/Grid.cpp:35-49):
  * D is nspin column-major nbasis x nbasis matrices, back to back (host memory).
- * nspin = 2 (alpha, beta; Unrestricted/SP.cpp:65-68) is built for LDA and GGA
- * grids (type <= 1); the functional must then be polarised (spin = 2). */
+ * nspin = 2 (alpha, beta; Unrestricted/SP.cpp:65-68) is built for LDA, GGA
+ * and meta-GGA grids (meta-GGA: r2SCAN); the functional must then be
+ * polarised (spin = 2). */
 int chx_grid_get_density(chx_grid* g, const double* D, int nspin, int nbasis);
 /* Same, with D already in device memory (HBM); used to time the path with
  * inputs resident on the GPU. */
@@ -200,7 +201,8 @@ int chx_grid_kernel_times(chx_grid* g, int reset, double* ms /*[6]*/, long* laun
 int chx_xc_read(const char* data_root, const char* df_name, int spin, int output, chx_xc** out);
 /* Same from explicit libxc IDs (the codes a .df line holds). spin: 1
  * unpolarised, 2 polarised (ExchangeCorrelation::Spin, libxc XC_POLARIZED);
- * polarised meta-GGA codes are not built (CHX_ERR_UNSUPPORTED). */
+ * of the meta-GGA codes, r2SCAN (497, 498) is built polarised, the others
+ * return CHX_ERR_UNSUPPORTED. */
 int chx_xc_create(int ncodes, const int* codes, int spin, chx_xc** out);
 void chx_xc_destroy(chx_xc* xc);
 /* Members Family (0 LDA, 1 GGA, 2 mGGA), EXX, Codes (ExchangeCorrelation.h:9-12). */
@@ -220,8 +222,8 @@ int chx_lebedev_rule(int npoints, double* xyzw);
  * (src/ExchangeCorrelation.cpp:98,122,157): exc per particle and the summed
  * derivatives over the functional's codes. Host arrays; sigma/tau may be NULL.
  * A polarised functional (spin = 2) takes libxc's XC_POLARIZED layout: rho[2n]
- * (a, b), sigma[3n] (aa, ab, bb), and writes vrho[2n], vsigma[3n] (tau, vtau
- * unused). */
+ * (a, b), sigma[3n] (aa, ab, bb), tau[2n] (a, b; meta-GGA, else unused), and
+ * writes vrho[2n], vsigma[3n], vtau[2n] (meta-GGA). */
 int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
                        double* exc, double* vrho, double* vsigma, double* vtau);
 /* No counterpart in the reference. For callers that cannot allocate device
@@ -231,8 +233,9 @@ int chx_xc_eval_points(const chx_xc* xc, long n, const double* rho, const double
  * holds on each device until the process ends (about 5.8 MB) instead of
  * buffers allocated per call. It calls no hipMalloc / hipFree and creates no
  * stream (it runs on the null stream). At most CHX_NOALLOC_MAX_POINTS points
- * per call (CHX_ERR_ARG beyond). Calls that overlap on one device share that
- * array and race. */
+ * per call (CHX_ERR_ARG beyond); spin-polarised meta-GGA functionals are
+ * not taken (CHX_ERR_UNSUPPORTED). Calls that overlap on one device share
+ * that array and race. */
 #define CHX_NOALLOC_MAX_POINTS 65536
 int chx_xc_eval_points_noalloc(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau,
                                double* exc, double* vrho, double* vsigma, double* vtau);

@@ -363,9 +363,9 @@ inline void eval_point_f(int code, double rho, double sigma, double* vsigma, dou
 // spin interpolations.
 typedef Dual<5> P5d;
 
-inline P5d spin_f(const P5d& z) {  // f(zeta) = ((1+z)^(4/3) + (1-z)^(4/3) - 2) / (2^(4/3) - 2)
-  const P5d a = 1.0 + z, b = 1.0 - z;
-  auto p43 = [](const P5d& x) { return x.v > 0 ? pow(x, 4.0 / 3.0) : P5d(0.0); };
+template <class T> T spin_f(const T& z) {  // f(zeta) = ((1+z)^(4/3) + (1-z)^(4/3) - 2) / (2^(4/3) - 2)
+  const T a = 1.0 + z, b = 1.0 - z;
+  auto p43 = [](const T& x) { return x.v > 0 ? pow(x, 4.0 / 3.0) : T(0.0); };
   return (p43(a) + p43(b) - 2.0) / (std::pow(2.0, 4.0 / 3.0) - 2.0);
 }
 inline double spin_fpp0() { return 4.0 / (9.0 * (std::cbrt(2.0) - 1.0)); }
@@ -386,17 +386,29 @@ inline P5d vwnrpa_pol_eps(const P5d& rs, const P5d& z) {
 }
 
 // PW92 with spin, PW_MOD parameters (eqs. 8, 10).
-inline P5d pw92_G(const P5d& rs, const double* q) {  // q = A, a1, b1, b2, b3, b4
-  const P5d srs = sqrt(rs);
-  const P5d den = 2.0 * q[0] * (q[2] * srs + q[3] * rs + q[4] * rs * srs + q[5] * rs * rs);
+template <class T> T pw92_G(const T& rs, const double* q) {  // q = A, a1, b1, b2, b3, b4
+  const T srs = sqrt(rs);
+  const T den = 2.0 * q[0] * (q[2] * srs + q[3] * rs + q[4] * rs * srs + q[5] * rs * rs);
   return -2.0 * q[0] * (1.0 + q[1] * rs) * log(1.0 + 1.0 / den);
 }
-inline P5d pw92_pol_eps(const P5d& rs, const P5d& z) {
-  static const double q0[6] = {0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294};
-  static const double q1[6] = {0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517};
-  static const double qa[6] = {0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671};
-  const P5d e0 = pw92_G(rs, q0), e1 = pw92_G(rs, q1), alpha_c = -pw92_G(rs, qa);
-  const P5d f = spin_f(z), z4 = z * z * z * z;
+// dG/drs of the same G (closed form).
+template <class T> T pw92_G_drs(const T& rs, const double* q) {
+  const T srs = sqrt(rs);
+  const T den = 2.0 * q[0] * (q[2] * srs + q[3] * rs + q[4] * rs * srs + q[5] * rs * rs);
+  const T dden = 2.0 * q[0] * (0.5 * q[2] / srs + q[3] + 1.5 * q[4] * srs + 2.0 * q[5] * rs);
+  return -2.0 * q[0] * q[1] * log(1.0 + 1.0 / den) + 2.0 * q[0] * (1.0 + q[1] * rs) * dden / (den * den + den);
+}
+constexpr double kPw92Q0[6] = {0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294};
+constexpr double kPw92Q1[6] = {0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517};
+constexpr double kPw92Qa[6] = {0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671};
+// PW92 eps_c(rs, zeta); with drs, its partial derivative in rs at fixed zeta.
+template <class T> T pw92_pol_eps(const T& rs, const T& z, T* drs = nullptr) {
+  const T e0 = pw92_G(rs, kPw92Q0), e1 = pw92_G(rs, kPw92Q1), alpha_c = -pw92_G(rs, kPw92Qa);
+  const T f = spin_f(z), z4 = z * z * z * z;
+  if (drs) {
+    const T d0 = pw92_G_drs(rs, kPw92Q0), d1 = pw92_G_drs(rs, kPw92Q1), da = -pw92_G_drs(rs, kPw92Qa);
+    *drs = d0 + da * f * (1.0 - z4) / spin_fpp0() + (d1 - d0) * f * z4;
+  }
   return e0 + alpha_c * f * (1.0 - z4) / spin_fpp0() + (e1 - e0) * f * z4;
 }
 
@@ -494,6 +506,110 @@ inline void eval_point_pol(int code, double ra, double rb, double saa, double sa
   vsigma[0] += e.d[2];
   vsigma[1] += e.d[3];
   vsigma[2] += e.d[4];
+}
+
+// ---- Spin-polarised r2SCAN (libxc 497 / 498 with XC_POLARIZED): variables
+// rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb, tau_a, tau_b (Dual<7>).
+// Exchange by spin scaling, E_x[ra, rb, ta, tb] = (E_x[2 ra, 4 saa, 2 ta] +
+// E_x[2 rb, 4 sbb, 2 tb]) / 2. Correlation (Furness et al. 2020 with the
+// spin dependence of SCAN, Sun, Ruzsinszky, Perdew, PRL 115, 036402 (2015)):
+//   zeta = (ra - rb) / rho, phi = ((1+z)^(2/3) + (1-z)^(2/3)) / 2,
+//   d_s = ((1+z)^(5/3) + (1-z)^(5/3)) / 2, d_x = ((1+z)^(4/3) + (1-z)^(4/3)) / 2,
+//   alpha_bar = (tau - tau_W) / (tau_unif d_s + eta tau_W) on the totals,
+//   eps_c^1 = eps_PW92(rs, z) + gamma phi^3 ln(1 + w1 (1 - g(y - dy))),
+//     w1 = exp(-eps_PW92 / (gamma phi^3)) - 1, y = beta t^2 / (gamma w1),
+//     t^2 = (3 pi^2 / 16)^(2/3) p / (phi^2 rs),
+//     dy = Dfc2 / (27 gamma d_s phi^3 w1) (20 rs (d eps_0 / d rs - d eps_PW92 / d rs)
+//          - 45 eta (eps_0 - eps_PW92)) p exp(-p^2 / dp2^4), eps_0 = eps_LDA0 G_c,
+//   eps_c^0 = (eps_LDA0 + b1c ln(1 + w0 (1 - g_inf(p)))) G_c(z),
+//     G_c = (1 - 2.3631 (d_x - 1)) (1 - z^12),
+//   eps_c = eps_c^1 + f_c(alpha_bar) (eps_c^0 - eps_c^1).
+// At z = 0 this is the unpolarised mgga_c_r2scan above.
+typedef Dual<7> P7d;
+
+template <class T> T mgga_c_r2scan_pol(const T& rho, const T& z, const T& sigma, const T& tau) {
+  const double gamma = (1.0 - std::log(2.0)) / (kPi * kPi);
+  const double b1c = 0.0285764, b2c = 0.0889, b3c = 0.125541, chi_inf = 0.128026;
+  const double k2 = std::pow(3.0 * kPi * kPi, 2.0 / 3.0);
+  const T opz = 1.0 + z, omz = 1.0 - z;
+  auto pw = [](const T& x, double e) { return x.v > 0 ? pow(x, e) : T(0.0); };
+  const T phi = 0.5 * (pw(opz, 2.0 / 3.0) + pw(omz, 2.0 / 3.0));
+  const T ds = 0.5 * (pw(opz, 5.0 / 3.0) + pw(omz, 5.0 / 3.0));
+  const T dx = 0.5 * (pw(opz, 4.0 / 3.0) + pw(omz, 4.0 / 3.0));
+  const T z2 = z * z, z4 = z2 * z2, z12 = z4 * z4 * z4;
+  const T Gc = (1.0 - 2.3631 * (dx - 1.0)) * (1.0 - z12);
+  const T p = sigma / (4.0 * k2 * pow(rho, 8.0 / 3.0));
+  const T tw = sigma / (8.0 * rho);
+  const T ab = (tau - tw) / (0.3 * k2 * pow(rho, 5.0 / 3.0) * ds + kR2Eta * tw);
+  const T rs = rs_of(rho);
+  const T srs = sqrt(rs);
+  T de1;
+  const T e1 = pw92_pol_eps(rs, z, &de1);
+  const T den0 = 1.0 + b2c * srs + b3c * rs;
+  const T e0lda = -b1c / den0;
+  const T de0lda = b1c * (0.5 * b2c / srs + b3c) / (den0 * den0);
+  const T phi3 = phi * phi * phi;
+  const T beta = 0.066725 * (1.0 + 0.1 * rs) / (1.0 + 0.1778 * rs);
+  const T w1 = exp(-e1 / (gamma * phi3)) - 1.0;
+  const T t2 = std::pow(3.0 * kPi * kPi / 16.0, 2.0 / 3.0) * p / (phi * phi * rs);
+  const T y = beta * t2 / (gamma * w1);
+  const double dfc2 = r2_poly_slope(kR2Cc);
+  const T dy = dfc2 / (27.0 * gamma * ds * phi3 * w1) *
+               (20.0 * rs * (de0lda * Gc - de1) - 45.0 * kR2Eta * (e0lda * Gc - e1)) * p *
+               exp(-p * p / std::pow(kR2Dp2, 4));
+  const T g = pow(1.0 + 4.0 * (y - dy), -0.25);
+  const T ec1 = e1 + gamma * phi3 * log(1.0 + w1 * (1.0 - g));
+  const T w0 = exp(-e0lda / b1c) - 1.0;
+  const T ginf = pow(1.0 + 4.0 * chi_inf * p, -0.25);
+  const T ec0 = (e0lda + b1c * log(1.0 + w0 * (1.0 - ginf))) * Gc;
+  const T fc = r2_interp(ab, kR2Cc, 0.64, 1.5, 0.7);
+  return rho * (ec1 + fc * (ec0 - ec1));
+}
+
+// Energy density of one code with spin, meta-GGA variables; LDA/GGA codes take
+// the P5 path's forms (their energy does not depend on tau).
+inline P7d energy_density_mpol(int code, const P7d& ra, const P7d& rb, const P7d& saa, const P7d& sab,
+                               const P7d& sbb, const P7d& ta, const P7d& tb, bool a_on, bool b_on) {
+  const P7d rho = ra + rb;
+  P7d z = (ra - rb) / rho;
+  if (z.v > 1.0 - 1e-15) z = P7d(1.0 - 1e-15);  // zeta threshold
+  if (z.v < -1.0 + 1e-15) z = P7d(-1.0 + 1e-15);
+  switch (code) {
+    case 497: {
+      P7d e(0.0);
+      if (a_on) e = e + 0.5 * mgga_x_r2scan(2.0 * ra, 4.0 * saa, 2.0 * ta);
+      if (b_on) e = e + 0.5 * mgga_x_r2scan(2.0 * rb, 4.0 * sbb, 2.0 * tb);
+      return e;
+    }
+    case 498: return mgga_c_r2scan_pol(rho, z, saa + 2.0 * sab + sbb, ta + tb);
+    default: return P7d(0.0);
+  }
+}
+
+// One meta-GGA code at one point, spin-polarised, accumulating eps, vrho[2],
+// vsigma[3], vtau[2]. Density threshold on the total; an exchange channel
+// below it is skipped; per spin tau_s >= 1e-20 and sigma_ss <= 8 rho_s tau_s,
+// sigma_ss >= 1e-40, |sigma_ab| <= sqrt(sigma_aa sigma_bb).
+inline void eval_point_mpol(int code, double ra, double rb, double saa, double sab, double sbb, double ta, double tb,
+                            double* eps, double* vrho, double* vsigma, double* vtau) {
+  ra = ra > 0 ? ra : 0;
+  rb = rb > 0 ? rb : 0;
+  const double thr = density_threshold(code);
+  if (!(ra + rb >= thr)) return;
+  ta = std::max(ta, 1e-20);
+  tb = std::max(tb, 1e-20);
+  saa = std::min(std::max(saa, 1e-40), 8.0 * ra * ta > 1e-40 ? 8.0 * ra * ta : 1e-40);
+  sbb = std::min(std::max(sbb, 1e-40), 8.0 * rb * tb > 1e-40 ? 8.0 * rb * tb : 1e-40);
+  const double cap = std::sqrt(saa * sbb);
+  sab = std::min(std::max(sab, -cap), cap);
+  const P7d e = energy_density_mpol(code, P7d::var(ra, 0), P7d::var(rb, 1), P7d::var(saa, 2), P7d::var(sab, 3),
+                                    P7d::var(sbb, 4), P7d::var(ta, 5), P7d::var(tb, 6), ra >= thr, rb >= thr);
+  *eps += e.v / (ra + rb);
+  vrho[0] += e.d[0];
+  vrho[1] += e.d[1];
+  for (int k = 0; k < 3; ++k) vsigma[k] += e.d[2 + k];
+  vtau[0] += e.d[5];
+  vtau[1] += e.d[6];
 }
 
 }  // namespace oxf

@@ -925,24 +925,28 @@ int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx) {
 }
 
 // Spin-polarised XC build (Unrestricted/SP.cpp:65-68 -> Grid::getDensity({Da, Db}),
-// Evaluate("ev") with the functional polarised, getEnergy, getFock), LDA/GGA:
-// per sub-grid, rho_s and grad rho_s of each spin (GridDensity.cpp:22-31 at
-// nspin = 2, sigma packed aa, ab, bb), the polarised functional per point, then
-// per spin u (GridFock.cpp:16-30): F = 1/2 F0[w v_rho_u] + F1 with the scaled
-// sigma derivatives (2 on aa / bb, 1 on ab) against grad rho_v, F += F^T.
-// Evaluated through fock_subgrid with vrho := v_rho_u, vsigma := 1 and
-// rho1 := (2 v_sigma_uu grad rho_u + v_sigma_ab grad rho_other) / 2.
+// Evaluate("ev") with the functional polarised, getEnergy, getFock), LDA/GGA
+// and r2SCAN: per sub-grid, rho_s, grad rho_s (and tau_s) of each spin
+// (GridDensity.cpp:22-41 at nspin = 2, sigma packed aa, ab, bb), the polarised
+// functional per point, then per spin u (GridFock.cpp:16-40): F = 1/2 F0[w v_rho_u]
+// + F1 with the scaled sigma derivatives (2 on aa / bb, 1 on ab) against
+// grad rho_v (+ 1/2 F2[w v_tau_u / 2]), F += F^T. Evaluated through fock_subgrid
+// with vrho := v_rho_u, vsigma := 1, rho1 := (2 v_sigma_uu grad rho_u +
+// v_sigma_ab grad rho_other) / 2 and vtau := v_tau_u.
 int ox_xc_pol(ox_system* s, const double* Da, const double* Db, double* Va, double* Vb, double* Exc, double* Nel) {
   return guarded([&] {
-    if (s->family > 1) throw std::runtime_error("oracle: the spin-polarised path is LDA/GGA");
     if (!s->ao_ready) throw std::runtime_error("oracle: AO tables not ready");
+    const bool mgga = s->family >= 2;
+    for (int code : s->codes)
+      if (mgga && code != 497 && code != 498)
+        throw std::runtime_error("oracle: the spin-polarised meta-GGA path is r2SCAN");
     const int nb = s->nbasis, n = (int)s->active.size(), nt = s->nthreads;
     const size_t nb2 = (size_t)nb * nb;
     std::vector<std::vector<double>> part(nt, std::vector<double>(2 * nb2, 0.0));
     double E = 0, N = 0;
 #pragma omp parallel num_threads(nt) reduction(+ : E, N)
     {
-      std::vector<double> F, tmp, tmp3, ra, rb, ga, gb, eps, vr, vs;
+      std::vector<double> F, tmp, tmp3, ra, rb, ga, gb, ta, tb, eps, vr, vs, vt;
       double* mine = part[omp_get_thread_num()].data();
 #pragma omp for schedule(dynamic, 1)
       for (int i = 0; i < n; ++i) {
@@ -951,19 +955,27 @@ int ox_xc_pol(ox_system* s, const double* Da, const double* Db, double* Va, doub
         density_subgrid(s, sg, Da, s->family, tmp);
         ra = sg.rho;
         ga = s->family >= 1 ? sg.rho1 : std::vector<double>((size_t)3 * ng, 0.0);
+        if (mgga) ta = sg.tau;
         density_subgrid(s, sg, Db, s->family, tmp);
         rb = sg.rho;
         gb = s->family >= 1 ? sg.rho1 : std::vector<double>((size_t)3 * ng, 0.0);
+        if (mgga) tb = sg.tau;
         eps.assign(ng, 0.0);
         vr.assign((size_t)2 * ng, 0.0);
         vs.assign((size_t)3 * ng, 0.0);
+        vt.assign((size_t)2 * ng, 0.0);
         for (int g = 0; g < ng; ++g) {
           double saa = 0, sab = 0, sbb = 0;
           for (int r = 0; r < 3; ++r) {
             const double a = ga[(size_t)r * ng + g], b = gb[(size_t)r * ng + g];
             saa += a * a; sab += a * b; sbb += b * b;
           }
-          for (int code : s->codes) oxf::eval_point_pol(code, ra[g], rb[g], saa, sab, sbb, &eps[g], &vr[2 * g], &vs[3 * g]);
+          for (int code : s->codes)
+            if (mgga)
+              oxf::eval_point_mpol(code, ra[g], rb[g], saa, sab, sbb, ta[g], tb[g], &eps[g], &vr[2 * g], &vs[3 * g],
+                                   &vt[2 * g]);
+            else
+              oxf::eval_point_pol(code, ra[g], rb[g], saa, sab, sbb, &eps[g], &vr[2 * g], &vs[3 * g]);
           E += sg.w[g] * (ra[g] + rb[g]) * eps[g];
           N += sg.w[g] * (ra[g] + rb[g]);
         }
@@ -973,6 +985,11 @@ int ox_xc_pol(ox_system* s, const double* Da, const double* Db, double* Va, doub
           sg.vrho.assign(ng, 0.0);
           sg.vsigma.assign(ng, 1.0);
           sg.rho1.assign((size_t)3 * ng, 0.0);
+          if (mgga) {  // the tau term of spin u (GridFock.cpp:32-40 with Eps1Tau of spin u); no Laplacian
+            sg.vtau.assign(ng, 0.0);
+            sg.vlapl.assign(ng, 0.0);
+            for (int g = 0; g < ng; ++g) sg.vtau[g] = vt[2 * g + u];
+          }
           for (int g = 0; g < ng; ++g) {
             sg.vrho[g] = vr[2 * g + u];
             const double suu = vs[3 * g + (u == 0 ? 0 : 2)], sab = vs[3 * g + 1];
@@ -993,6 +1010,23 @@ int ox_xc_pol(ox_system* s, const double* Da, const double* Db, double* Va, doub
     if (Exc) *Exc = E;
     if (Nel) *Nel = N;
   });
+}
+
+int ox_eval_functional_mpol(int ncodes, const int* codes, long n, const double* rho, const double* sigma,
+                            const double* tau, double* exc, double* vrho, double* vsigma, double* vtau) {
+  for (long i = 0; i < n; ++i) {
+    double e = 0, vr[2] = {0, 0}, vs[3] = {0, 0, 0}, vt[2] = {0, 0};
+    for (int k = 0; k < ncodes; ++k) {
+      if (codes[k] != 497 && codes[k] != 498) { g_err = "oracle: the spin-polarised meta-GGA path is r2SCAN"; return -1; }
+      oxf::eval_point_mpol(codes[k], rho[2 * i], rho[2 * i + 1], sigma[3 * i], sigma[3 * i + 1], sigma[3 * i + 2],
+                           tau[2 * i], tau[2 * i + 1], &e, vr, vs, vt);
+    }
+    exc[i] = e;
+    vrho[2 * i] = vr[0]; vrho[2 * i + 1] = vr[1];
+    for (int k = 0; k < 3; ++k) vsigma[3 * i + k] = vs[k];
+    vtau[2 * i] = vt[0]; vtau[2 * i + 1] = vt[1];
+  }
+  return 0;
 }
 
 int ox_eval_functional_pol(int ncodes, const int* codes, long n, const double* rho, const double* sigma, double* exc,

@@ -112,11 +112,15 @@ int ox_get_active_xyz(const ox_system* s, double* xyz);
 /* radial x angular weights (before the Becke partition) of the active sub-grids' points */
 int ox_get_active_raw_weights(const ox_system* s, double* w);
 
-/* Spin-polarised XC build, LDA/GGA: getDensity({Da, Db}) -> Evaluate("ev")
- * (polarised functional) -> getEnergy -> getFock, Va / Vb per spin. */
+/* Spin-polarised XC build, LDA/GGA and r2SCAN: getDensity({Da, Db}) ->
+ * Evaluate("ev") (polarised functional) -> getEnergy -> getFock, Va / Vb per spin. */
 int ox_xc_pol(ox_system* s, const double* Da, const double* Db, double* Va, double* Vb, double* Exc, double* Nel);
 /* Spin-polarised functional at n points, libxc XC_POLARIZED layout: rho[2n]
  * (a, b), sigma[3n] (aa, ab, bb), vrho[2n], vsigma[3n]; LDA/GGA codes. */
+/* Spin-polarised r2SCAN (497, 498) at n points, libxc XC_POLARIZED layout:
+ * rho[2n], sigma[3n] (aa, ab, bb), tau[2n] -> exc[n], vrho[2n], vsigma[3n], vtau[2n]. */
+int ox_eval_functional_mpol(int ncodes, const int* codes, long n, const double* rho, const double* sigma,
+                            const double* tau, double* exc, double* vrho, double* vsigma, double* vtau);
 int ox_eval_functional_pol(int ncodes, const int* codes, long n, const double* rho, const double* sigma, double* exc,
                            double* vrho, double* vsigma);
 

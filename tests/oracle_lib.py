@@ -61,6 +61,8 @@ def lib():
         "ox_fock_s": (ctypes.c_int, [_vp, _vp, _vp]),
         "ox_xc_pol": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_pol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
+        "ox_eval_functional_mpol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                   _vp]),
         "ox_mlp_forward": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp]),
         "ox_mlp_loss_grad": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
         "ox_mlp_descriptors": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
@@ -318,3 +320,16 @@ def eval_functional_pol(codes, rho, sigma):
     e, vr, vs = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
     _check(lib().ox_eval_functional_pol(len(cc), _ptr(cc), n, _ptr(rho), _ptr(sigma), _ptr(e), _ptr(vr), _ptr(vs)))
     return e, vr, vs
+
+
+def eval_functional_mpol(codes, rho, sigma, tau):
+    """Spin-polarised r2SCAN: eps, vrho (n x 2), vsigma (n x 3), vtau (n x 2); rho, tau (n x 2), sigma (n x 3)."""
+    cc = np.ascontiguousarray(codes, dtype=np.int32)
+    rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1, 3)
+    tau = np.ascontiguousarray(tau, dtype=np.float64).reshape(-1, 2)
+    n = rho.shape[0]
+    e, vr, vs, vt = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3)), np.zeros((n, 2))
+    _check(lib().ox_eval_functional_mpol(len(cc), _ptr(cc), n, _ptr(rho), _ptr(sigma), _ptr(tau), _ptr(e), _ptr(vr),
+                                         _ptr(vs), _ptr(vt)))
+    return e, vr, vs, vt

@@ -111,9 +111,13 @@ def test_spin_protocol_errors():
     with pytest.raises(ca.ChxError) as e:
         pbe.Evaluate("ev", g)
     assert e.value.code == -3
-    with pytest.raises(ca.ChxError) as e:  # polarised meta-GGA is not built
-        _xc("R2SCAN")
+    with pytest.raises(ca.ChxError) as e:  # a polarised meta-GGA other than r2SCAN is not built
+        _xc(codes=[497, 131])
     assert e.value.code == -5
+    r2 = _xc("R2SCAN")
+    with pytest.raises(ca.ChxError) as e:  # the GGA grid holds no tau
+        r2.Evaluate("ev", g)
+    assert e.value.code == -3
     pol = _xc("PBE")
     with pytest.raises(ca.ChxError) as e:  # response path is closed shell
         pol.Evaluate("f", g)
