@@ -124,6 +124,8 @@ def lib():
         "chx_mlp_get_params": (i, [_vp, _vp]),
         "chx_mlp_set_params": (i, [_vp, _vp]),
         "chx_mlp_set_box": (i, [_vp, _vp, _vp, d]),
+        "chx_mlp_set_base": (i, [_vp, i, _vp]),
+        "chx_mlp_get_base": (i, [_vp, P(i), _vp]),
         "chx_mlp_get_box": (i, [_vp, _vp, _vp, P(d)]),
         "chx_mlp_features": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp]),
         "chx_mlp_set_data": (i, [_vp, l, _vp, _vp, _vp]),

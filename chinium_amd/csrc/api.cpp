@@ -115,7 +115,7 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
     g->d_nfall.alloc(1);
     launch_fill_u32(g->stream, (unsigned*)g->d_nfall.p, 1, 0u);
     g->mark_begin(2, &e);
-    launch_functional_mlp(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, m->Pinf.p,
+    launch_functional_mlp(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, m->Prun.p,
                           m->Wpk.p, m->box, g->d_codes.p, (int)xc->codes.size(), want_e ? 1 : 0, want_v ? 1 : 0, g->d_rho.p,
                           g->d_coef.p, g->d_en_part.p, g->d_nfall.p);
     launch_reduce_en(g->stream, g->nbatches, g->d_en_part.p, g->d_en.p);
@@ -1092,6 +1092,27 @@ int chx_mlp_set_box(chx_mlp* m, const float* lo, const float* hi, double rho_min
   });
 }
 
+int chx_mlp_set_base(chx_mlp* m, int ncodes, const int* codes) {
+  return guard([&] {
+    require(m && ncodes >= 0 && ncodes <= 2 && (ncodes == 0 || codes), CHX_ERR_ARG,
+            "the analytic base takes 0 to 2 functional codes");
+    for (int k = 0; k < ncodes; ++k)
+      require(code_family(codes[k]) == 0 || code_family(codes[k]) == 1, CHX_ERR_UNSUPPORTED,
+              "the analytic base is an LDA or GGA functional");
+    m->box.nbase = ncodes;
+    for (int k = 0; k < 2; ++k) m->box.base[k] = k < ncodes ? codes[k] : 0;
+  });
+}
+
+int chx_mlp_get_base(const chx_mlp* m, int* ncodes, int* codes) {
+  return guard([&] {
+    require(m && ncodes, CHX_ERR_ARG, "bad arguments");
+    *ncodes = m->box.nbase;
+    if (codes)
+      for (int k = 0; k < m->box.nbase; ++k) codes[k] = m->box.base[k];
+  });
+}
+
 int chx_mlp_get_box(const chx_mlp* m, float* lo, float* hi, double* rho_min) {
   return guard([&] {
     require(m && m->have_box, CHX_ERR_STATE, "the network has no descriptor box");
@@ -1221,7 +1242,9 @@ int chx_mlp_forward(const chx_mlp* m, long n, const float* U, int bf16, float* F
     dFv.alloc(n);
     if (dF) dG.alloc(3 * (size_t)n);
     CHX_HIP(hipMemcpyAsync(dU.p, U, sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->stream));
-    launch_mlp_forward(m->stream, n, dU.p, bf16 ? m->Pinf.p : m->Prun.p, bf16 == 2 ? m->Wpk.p : nullptr, dFv.p,
+    // 0: FP32 weights, per-lane network; 1: BF16-rounded weights, per-lane network; 2: the inference
+    // path (FP32 weights as split-BF16 MFMA operands)
+    launch_mlp_forward(m->stream, n, dU.p, bf16 == 1 ? m->Pinf.p : m->Prun.p, bf16 == 2 ? m->Wpk.p : nullptr, dFv.p,
                        dF ? dG.p : nullptr);
     CHX_HIP(hipMemcpyAsync(F, dFv.p, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
     if (dF) CHX_HIP(hipMemcpyAsync(dF, dG.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, m->stream));
@@ -1248,7 +1271,7 @@ int chx_mlp_eval_points(const chx_mlp* m, const chx_xc* xc, long n, const double
     de.alloc(n); dvr.alloc(n); dvs.alloc(n); dvt.alloc(n);
     dn.alloc(1);
     launch_fill_u32(s, (unsigned*)dn.p, 1, 0u);
-    launch_mlp_eval(s, n, m->Pinf.p, m->Wpk.p, m->box, dc.p, (int)xc->codes.size(), dr.p, ds.p, dt.p, de.p, dvr.p, dvs.p, dvt.p,
+    launch_mlp_eval(s, n, m->Prun.p, m->Wpk.p, m->box, dc.p, (int)xc->codes.size(), dr.p, ds.p, dt.p, de.p, dvr.p, dvs.p, dvt.p,
                     dn.p);
     int nf = 0;
     CHX_HIP(hipMemcpyAsync(exc, de.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));

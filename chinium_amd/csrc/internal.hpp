@@ -170,11 +170,13 @@ constexpr int kMlpW1 = 0, kMlpB1 = 192, kMlpW2 = 256, kMlpB2 = 4352, kMlpW3 = 44
 struct MlpBox {
   float lo[3], hi[3];
   double rho_min;  // below: analytic functional (low-density tail)
+  int nbase;       // the network predicts F - F_base of these analytic codes (0: the whole F)
+  int base[2];
 };
 void launch_mlp_pack(hipStream_t s, const float* P, float* Prun, float* Pinf);
 void launch_mlp_forward(hipStream_t s, long n, const float* U, const float* R, const unsigned* Wpk, float* F,
                         float* dF);
-constexpr int kMlpPacked = 4 * 4 * 2 * 64 * 4;  // u32 words of the packed BF16 MFMA weight fragments
+constexpr int kMlpPacked = 4 * 4 * 2 * 2 * 64 * 4;  // u32 words of the packed split-BF16 (hi, lo) MFMA weight fragments
 void launch_mlp_pack_mx(hipStream_t s, const float* P, unsigned* Wpk);
 void launch_mlp_grad(hipStream_t s, long n, const float* U, const float* T, const float* C, float cs, const float* R,
                      float* G, float* loss, int max_blocks);

@@ -3,8 +3,11 @@
 // factor F = e_xc / e_x^unif(rho); e_xc's rho / sigma / tau derivatives come
 // from the network's own input gradient. Trained on the GPU by the kernels
 // below (forward + backward with the weight gradient, Adam); evaluated with
-// BF16-rounded weights and FP32 arithmetic. Points outside the training box,
-// and the low-density tail, take the analytic functional (FP64). No reference
+// FP32 weights (split into two BF16 terms for the MFMA products) and FP32
+// arithmetic. With an analytic base (MlpBox::base, e.g. Slater + VWN5) the
+// network predicts F - F_base and the base is added back. Points outside the
+// training box, and the low-density tail, take the analytic functional
+// (FP64). No reference
 // counterpart: the reference evaluates functionals through libxc
 // (src/ExchangeCorrelation.cpp:86-163), whose result this replaces.
 //
@@ -90,9 +93,10 @@ __device__ __forceinline__ float mlp_value_grad(const float* R, const float u[3]
 // are permuted so that these registers are also the next product's B operand:
 // K step ks, slot 8 g + e carries unit U(ks, 8g+e) = 16 (2 ks + e/4) + 4 g + e%4,
 // i.e. B element e of (nt, ks) is X[2 ks + e/4][nt][e%4]. The packed weights
-// (k_mlp_pack_mx) hold A[m][slot] in that order, BF16 (the inference weights
-// are BF16 already). Activations enter as split BF16 (hi + lo), so each tile is
-// two MFMAs per K step. Forward: W2, W3; backward for dF/du: W3^T, W2^T.
+// (k_mlp_pack_mx) hold A[m][slot] in that order, split in two BF16 terms
+// (hi = bf16(w), lo = bf16(w - hi)), and activations enter split the same way,
+// so each tile is three MFMAs per K step (lo.hi, hi.lo, then hi.hi): FP32-level
+// products with FP32 accumulation. Forward: W2, W3; backward for dF/du: W3^T, W2^T.
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
@@ -128,22 +132,28 @@ __device__ __forceinline__ void mx_layer(const u32x4_t* __restrict__ Wpk, int ma
     for (int ks = 0; ks < 2; ++ks) pack_b(X, nt, ks, bh[nt][ks], bl[nt][ks]);
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
-    const bf16x8_t a0 = __builtin_bit_cast(bf16x8_t, Wpk[((mat * 4 + mt) * 2 + 0) * 64 + lane]);
-    const bf16x8_t a1 = __builtin_bit_cast(bf16x8_t, Wpk[((mat * 4 + mt) * 2 + 1) * 64 + lane]);
+    // fragment (mat, mt, ks, hl): Wpk[(((mat * 4 + mt) * 2 + ks) * 2 + hl) * 64 + lane]
+    const int f0 = ((mat * 4 + mt) * 2) * 2;
+    const bf16x8_t a0h = __builtin_bit_cast(bf16x8_t, Wpk[(f0 + 0) * 64 + lane]);
+    const bf16x8_t a0l = __builtin_bit_cast(bf16x8_t, Wpk[(f0 + 1) * 64 + lane]);
+    const bf16x8_t a1h = __builtin_bit_cast(bf16x8_t, Wpk[(f0 + 2) * 64 + lane]);
+    const bf16x8_t a1l = __builtin_bit_cast(bf16x8_t, Wpk[(f0 + 3) * 64 + lane]);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-      f32x4_t c = f32x4_t{0.f, 0.f, 0.f, 0.f};
-      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, __builtin_bit_cast(bf16x8_t, bl[nt][0]), c, 0, 0, 0);
-      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, __builtin_bit_cast(bf16x8_t, bl[nt][1]), c, 0, 0, 0);
-      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, __builtin_bit_cast(bf16x8_t, bh[nt][0]), c, 0, 0, 0);
-      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, __builtin_bit_cast(bf16x8_t, bh[nt][1]), c, 0, 0, 0);
+      f32x4_t c = f32x4_t{0.f, 0.f, 0.f, 0.f};  // the small terms first
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0l, __builtin_bit_cast(bf16x8_t, bh[nt][0]), c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1l, __builtin_bit_cast(bf16x8_t, bh[nt][1]), c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0h, __builtin_bit_cast(bf16x8_t, bl[nt][0]), c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1h, __builtin_bit_cast(bf16x8_t, bl[nt][1]), c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0h, __builtin_bit_cast(bf16x8_t, bh[nt][0]), c, 0, 0, 0);
+      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1h, __builtin_bit_cast(bf16x8_t, bh[nt][1]), c, 0, 0, 0);
     }
   }
 }
 
 // Network value and input gradient of the wave's 64 points (lane = point);
-// every lane of the wave must call it. R: inference parameters (BF16-rounded,
-// for W1, biases, w4); Wpk: packed BF16 W2, W3, W3^T, W2^T fragments.
+// every lane of the wave must call it. R: run parameters (FP32, for W1,
+// biases, w4); Wpk: packed split-BF16 W2, W3, W3^T, W2^T fragments.
 __device__ __forceinline__ float mlp_value_grad_mx(const u32x4_t* __restrict__ Wpk, const float* __restrict__ R,
                                                    const float u[3], float du[3]) {
   const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
@@ -313,10 +323,12 @@ __device__ __forceinline__ bool mlp_point(const float* R, const u32x4_t* Wpk, co
     double dF[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) dF[c] = g[0] * dt[0][c] + g[1] * dt[1][c] + g[2] * dt[2][c];
-    *eps = eu * (double)F / r;
-    *vr = (4.0 / 3.0) * kCxUnif * r13 * (double)F + eu * dF[0];
-    *vs = eu * dF[1];
-    *vt = eu * dF[2];
+    double e0 = 0.0, r0 = 0.0, s0 = 0.0, t0 = 0.0;  // the analytic base, if any
+    if (box.nbase > 0) eval_codes(box.base, box.nbase, rho, sigma, tau, &e0, &r0, &s0, &t0);
+    *eps = e0 + eu * (double)F / r;
+    *vr = r0 + (4.0 / 3.0) * kCxUnif * r13 * (double)F + eu * dF[0];
+    *vs = s0 + eu * dF[1];
+    *vt = t0 + eu * dF[2];
     return true;
   }
   eval_codes(codes, ncodes, rho, sigma, tau, eps, vr, vs, vt);
@@ -366,13 +378,15 @@ __global__ __launch_bounds__(64) void k_mlp_forward(long n, const float* __restr
   }
 }
 
-// Packed BF16 A-operand fragments of the MFMA network: for matrix mat (0 W2,
-// 1 W3, 2 W3^T, 3 W2^T), m tile mt, K step ks, lane l = 16 g + c, element e:
-// A[m = 16 mt + c][slot 8 g + e] with the slot's unit U = 16 (2 ks + e/4) + 4 g + e%4.
+// Packed split-BF16 A-operand fragments of the MFMA network: for matrix mat
+// (0 W2, 1 W3, 2 W3^T, 3 W2^T), m tile mt, K step ks, term hl (0 hi, 1 lo),
+// lane l = 16 g + c, element e: A[m = 16 mt + c][slot 8 g + e] with the slot's
+// unit U = 16 (2 ks + e/4) + 4 g + e%4.
 __global__ void k_mlp_pack_mx(const float* __restrict__ P, unsigned short* __restrict__ Wpk) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (((mat * 4 + mt) * 2 + ks) * 64 + lane) * 8 + e
-  if (idx >= 4 * 4 * 2 * 64 * 8) return;
-  const int e = idx & 7, lane = (idx >> 3) & 63, ks = (idx >> 9) & 1, mt = (idx >> 10) & 3, mat = idx >> 12;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // ((((mat * 4 + mt) * 2 + ks) * 2 + hl) * 64 + lane) * 8 + e
+  if (idx >= 4 * 4 * 2 * 2 * 64 * 8) return;
+  const int e = idx & 7, lane = (idx >> 3) & 63, hl = (idx >> 9) & 1, ks = (idx >> 10) & 1, mt = (idx >> 11) & 3,
+            mat = idx >> 13;
   const int g = lane >> 4, c = lane & 15;
   const int m = 16 * mt + c, un = 16 * (2 * ks + e / 4) + 4 * g + e % 4;
   float w;
@@ -382,7 +396,8 @@ __global__ void k_mlp_pack_mx(const float* __restrict__ P, unsigned short* __res
     case 2: w = P[kMlpW3 + kMlpH * un + m]; break;
     default: w = P[kMlpW2 + kMlpH * un + m]; break;
   }
-  Wpk[idx] = bf_bits(w);
+  const unsigned short h = bf_bits(w);
+  Wpk[idx] = hl ? bf_bits(w - bf_val(h)) : h;
 }
 
 // Loss L = cs sum_i C_i (y_i - T_i)^2 over the samples [0, n) (C_i = 1 when C
@@ -699,7 +714,7 @@ void launch_mlp_forward(hipStream_t st, long n, const float* U, const float* R, 
 }
 
 void launch_mlp_pack_mx(hipStream_t st, const float* P, unsigned* Wpk) {
-  hipLaunchKernelGGL(k_mlp_pack_mx, dim3(4 * 4 * 2 * 64 * 8 / 256), dim3(256), 0, st, P,
+  hipLaunchKernelGGL(k_mlp_pack_mx, dim3(4 * 4 * 2 * 2 * 64 * 8 / 256), dim3(256), 0, st, P,
                      reinterpret_cast<unsigned short*>(Wpk));
   CHX_HIP(hipGetLastError());
 }

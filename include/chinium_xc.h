@@ -314,8 +314,10 @@ void chx_pack_destroy(chx_pack* p);
  * rho < rho_min take the analytic functional (the out-of-distribution
  * fallback). Parameters: CHX_MLP_PARAMS floats, matrices [out][in]:
  * W1[64][3] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4. Training
- * runs in FP32 with Adam; inference uses BF16-rounded weights with split-BF16
- * activations on MFMA and FP32 accumulation. */
+ * runs in FP32 with Adam; inference runs the hidden products on MFMA with
+ * weights and activations split in two BF16 terms each (FP32-level products)
+ * and FP32 accumulation. With an analytic base (chx_mlp_set_base) the network
+ * predicts F - F_base, and Evaluate adds the base's own values back. */
 #define CHX_MLP_PARAMS 8641
 typedef struct chx_mlp chx_mlp;
 int chx_mlp_create(unsigned long long seed, chx_mlp** out); /* Glorot-uniform init, seeded */
@@ -324,6 +326,11 @@ int chx_mlp_get_params(const chx_mlp* m, float* P);
 int chx_mlp_set_params(chx_mlp* m, const float* P);
 int chx_mlp_set_box(chx_mlp* m, const float* lo, const float* hi, double rho_min);
 int chx_mlp_get_box(const chx_mlp* m, float* lo, float* hi, double* rho_min);
+/* Analytic base: 0 to 2 LDA / GGA codes (e.g. 1, 7: Slater + VWN5); 0 codes:
+ * the network predicts the whole F. Training targets must then be
+ * F_functional - F_base at the same points. */
+int chx_mlp_set_base(chx_mlp* m, int ncodes, const int* codes);
+int chx_mlp_get_base(const chx_mlp* m, int* ncodes, int* codes);
 /* Training samples from the analytic functional of xc at n points: t (n x 3)
  * and F (n). Host arrays. */
 int chx_mlp_features(const chx_xc* xc, long n, const double* rho, const double* sigma, const double* tau, float* t,

@@ -165,8 +165,8 @@ int ox_mlp_descriptors(long n, const double* rho, const double* sigma, const dou
 }
 
 int ox_mlp_point(const double* P, const double* lo, const double* hi, double rho_min, int ncodes, const int* codes,
-                 long n, const double* rho, const double* sigma, const double* tau, double* exc, double* vrho,
-                 double* vsigma, double* vtau, int* used_net) {
+                 int nbase, const int* base, long n, const double* rho, const double* sigma, const double* tau,
+                 double* exc, double* vrho, double* vsigma, double* vtau, int* used_net) {
   for (long i = 0; i < n; ++i) {
     bool in = rho[i] >= rho_min;
     double t[3], dt[3][3], u[3];
@@ -186,10 +186,13 @@ int ox_mlp_point(const double* P, const double* lo, const double* hi, double rho
       double dF[3] = {0, 0, 0};
       for (int k = 0; k < 3; ++k)
         for (int c = 0; c < 3; ++c) dF[c] += du[k] * 2.0 / (hi[k] - lo[k]) * dt[k][c];
-      exc[i] = eu * s.y / rho[i];
-      vrho[i] = 4.0 / 3.0 * kCx * std::cbrt(rho[i]) * s.y + eu * dF[0];
-      vsigma[i] = eu * dF[1];
-      vtau[i] = eu * dF[2];
+      // the analytic base (the network predicts F - F_base), then the network's part
+      double e0 = 0, r0 = 0, s0 = 0, t0 = 0;
+      for (int k = 0; k < nbase; ++k) oxf::eval_point(base[k], rho[i], sigma[i], tau[i], &e0, &r0, &s0, &t0);
+      exc[i] = e0 + eu * s.y / rho[i];
+      vrho[i] = r0 + 4.0 / 3.0 * kCx * std::cbrt(rho[i]) * s.y + eu * dF[0];
+      vsigma[i] = s0 + eu * dF[1];
+      vtau[i] = t0 + eu * dF[2];
     } else {
       double e = 0, vr = 0, vs = 0, vt = 0;
       for (int k = 0; k < ncodes; ++k) oxf::eval_point(codes[k], rho[i], sigma[i], tau[i], &e, &vr, &vs, &vt);

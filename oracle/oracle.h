@@ -134,9 +134,11 @@ int ox_mlp_forward(long n, const double* U, const double* P, double* F, double* 
 int ox_mlp_loss_grad(long n, const double* U, const double* T, const double* C, double cs, const double* P,
                      double* G, double* loss);
 int ox_mlp_descriptors(long n, const double* rho, const double* sigma, const double* tau, double* t, double* dt);
+/* The surrogate at n points (nbase analytic base codes added to the network's
+ * part; 0: the network is the whole F), with the analytic fallback of codes. */
 int ox_mlp_point(const double* P, const double* lo, const double* hi, double rho_min, int ncodes, const int* codes,
-                 long n, const double* rho, const double* sigma, const double* tau, double* exc, double* vrho,
-                 double* vsigma, double* vtau, int* used_net);
+                 int nbase, const int* base, long n, const double* rho, const double* sigma, const double* tau,
+                 double* exc, double* vrho, double* vsigma, double* vtau, int* used_net);
 
 #ifdef __cplusplus
 }

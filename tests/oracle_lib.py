@@ -66,8 +66,8 @@ def lib():
         "ox_mlp_forward": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp]),
         "ox_mlp_loss_grad": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
         "ox_mlp_descriptors": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
-        "ox_mlp_point": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _vp, ctypes.c_long]
-                         + [_vp] * 8),
+        "ox_mlp_point": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _vp, ctypes.c_int, _vp,
+                                        ctypes.c_long] + [_vp] * 8),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -298,16 +298,18 @@ def mlp_descriptors(rho, sigma, tau):
     return t, dt
 
 
-def mlp_point(P, lo, hi, rho_min, codes, rho, sigma, tau):
-    """Surrogate eps, v_rho, v_sigma, v_tau with the analytic fallback; used_net flags."""
+def mlp_point(P, lo, hi, rho_min, codes, rho, sigma, tau, base=()):
+    """Surrogate eps, v_rho, v_sigma, v_tau with the analytic fallback (base: the analytic codes the
+    network's part is added to); used_net flags."""
     P, lo, hi = _d(P), _d(lo), _d(hi)
     rho, sigma, tau = _d(rho), _d(sigma), _d(tau)
     c = np.ascontiguousarray(codes, dtype=np.int32)
+    b = np.ascontiguousarray(list(base) or [0], dtype=np.int32)
     n = rho.size
     out = [np.zeros(n) for _ in range(4)]
     used = np.zeros(n, dtype=np.int32)
-    _check(lib().ox_mlp_point(_ptr(P), _ptr(lo), _ptr(hi), rho_min, c.size, _ptr(c), n, _ptr(rho), _ptr(sigma),
-                              _ptr(tau), *[_ptr(o) for o in out], _ptr(used)))
+    _check(lib().ox_mlp_point(_ptr(P), _ptr(lo), _ptr(hi), rho_min, c.size, _ptr(c), len(base), _ptr(b), n,
+                              _ptr(rho), _ptr(sigma), _ptr(tau), *[_ptr(o) for o in out], _ptr(used)))
     return out[0], out[1], out[2], out[3], used.astype(bool)
 
 

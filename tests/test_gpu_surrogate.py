@@ -1,11 +1,12 @@
 """Learned functional surrogate on the GPU (chx_mlp_*): forward, input
 gradient and the loss / parameter gradient of the training kernels against the
 FP64 oracle (oracle/mlp.cpp, pinned by finite differences and known answers in
-test_oracle_mlp.py); the out-of-distribution fallback returns the analytic
-functional bit for bit; training on the workload grids lowers the loss and the
-trained network's E_xc error against the analytic r2SCAN is at least 20x below
-the untrained network's (the gain of training); Evaluate with the surrogate
-attached integrates the same per-point values the oracle gives.
+test_oracle_mlp.py), with and without the analytic base the network corrects;
+the out-of-distribution fallback returns the analytic functional bit for bit;
+training on the workload grids lowers the loss; on held-out cyclo[18]carbon the
+trained correction gains >= 100x in E_xc error over its base alone (what the
+network adds to the method without it); Evaluate with the surrogate attached
+integrates the same per-point values the oracle gives.
 
 Tolerances: FP32 network arithmetic against FP64 -> 2e-5 relative on F,
 1e-4 of the largest component on gradients."""
@@ -15,7 +16,7 @@ import pytest
 import chinium_amd as ca
 import oracle_lib as ol
 from chinium_amd import workloads as wl
-from chinium_amd.surrogate import Surrogate, workload_samples
+from chinium_amd.surrogate import LDA_BASE, Surrogate, workload_samples
 
 pytestmark = pytest.mark.gpu
 R2SCAN = [497, 498]
@@ -37,15 +38,15 @@ def _r2scan():
 
 @pytest.mark.parametrize("mode", ["fp32", "bf16", "mfma"])
 def test_forward_against_oracle(mode):
-    """fp32 / bf16: the per-lane FP32 network on the training / inference weights; mfma: the
-    inference path (BF16 weights, split-BF16 activations, transposed MFMA products, 1001 points
-    = 15 full waves and a ragged one). Bars: FP32 arithmetic 2e-5 / 1e-4; split-BF16 activations
-    (16 significant bits) 1e-4 / 1e-3."""
+    """fp32 / bf16: the per-lane FP32 network on the FP32 / BF16-rounded weights; mfma: the
+    inference path (FP32 weights and activations as split-BF16 MFMA operands, transposed products,
+    1001 points = 15 full waves and a ragged one) against the FP32 weights. Bars: FP32 arithmetic
+    2e-5 / 1e-4; split-BF16 operands (16 significant bits) 1e-4 / 1e-3."""
     s = Surrogate(3)
     s.params = _rand_params(1)
     U = np.random.default_rng(2).uniform(-1.2, 1.2, (1001, 3)).astype(np.float32)
     F, dF = s.forward(U, bf16=mode != "fp32", mfma=mode == "mfma")
-    P = s.params if mode == "fp32" else s.inference_params()
+    P = s.bf16_params() if mode == "bf16" else s.inference_params()
     Fo, dFo = ol.mlp_forward(P.astype(np.float64), U.astype(np.float64))
     ef = np.abs(F - Fo).max() / (1 + np.abs(Fo).max())
     eg = np.abs(dF - dFo).max() / np.abs(dFo).max()
@@ -73,8 +74,9 @@ def test_loss_gradient_against_oracle(n, weighted):
     assert eg <= 1e-4
 
 
-def test_eval_points_and_fallback():
-    s = Surrogate(6)
+@pytest.mark.parametrize("base", [(), (1, 7)])
+def test_eval_points_and_fallback(base):
+    s = Surrogate(6, base=base)
     s.params = _rand_params(7, 0.2)
     lo, hi = np.array([-12.0, 0.0, 0.02], np.float32), np.array([4.0, 0.8, 0.9], np.float32)
     s.set_box(lo, hi, 1e-8)
@@ -86,14 +88,14 @@ def test_eval_points_and_fallback():
     xc = _r2scan()
     e, vr, vs, vt, nf = s.eval_points(xc, rho, sigma, tau)
     eo, vro, vso, vto, used = ol.mlp_point(s.inference_params().astype(np.float64), lo.astype(np.float64),
-                                           hi.astype(np.float64), 1e-8, R2SCAN, rho, sigma, tau)
+                                           hi.astype(np.float64), 1e-8, R2SCAN, rho, sigma, tau, base=base)
     _say("  %d points, %d on the network, %d fallback" % (n, used.sum(), nf))
     assert nf == (~used).sum() and 0 < nf < n
     # fallback lanes: the device's analytic functional, bit for bit
     ea, vra, vsa, vta = xc.eval_points(rho, sigma, tau)
     for a, b in ((e, ea), (vr, vra), (vs, vsa), (vt, vta)):
         assert np.array_equal(a[~used], b[~used])
-    # network lanes against the FP64 oracle on the same BF16 weights; FP32 network errors are absolute
+    # network lanes against the FP64 oracle on the same FP32 weights; FP32 network errors are absolute
     # in F, so the scale is the uniform-gas exchange energy density |e_x^unif| (F = 1)
     eu = 0.7385587663820224 * rho ** (4.0 / 3.0)
     for a, b, sc in ((rho * e, rho * eo, eu), (vr, vro, eu / rho), (vs, vso, eu / sigma), (vt, vto, eu / tau)):
@@ -104,63 +106,80 @@ def test_eval_points_and_fallback():
 @pytest.fixture(scope="module")
 def trained():
     _say("collecting r2SCAN training samples on the workload grids")
-    rho, sigma, tau, w = workload_samples(n_alkanes=6)
+    rho, sigma, tau, w = workload_samples(n_alkanes=8)
     _say("  %d samples" % rho.size)
     xc = _r2scan()
-    s = Surrogate(11)
-    # E_xc of the untrained network on a held-out grid, for the gain check below
-    untrained = _surrogate_error(s, xc, box_from=(rho, sigma, tau))
-    loss = s.fit(xc, rho, sigma, tau, w, epochs=100, batch=16384, lr=3e-3, lr_final=3e-5)
+    s = Surrogate(11, base=LDA_BASE)
+    loss = s.fit(xc, rho, sigma, tau, w, epochs=150, batch=16384, lr=3e-3, lr_final=3e-5)
     _say("  loss: first epoch %.3e, last %.3e" % (loss[0], loss[-1]))
-    return s, xc, loss, untrained
+    return s, xc, loss
 
 
-def _heldout():
-    Z, X = wl.water_cluster_angstrom(4, 5.0, 11)
-    mol = ca.Molecule(Z, X * ca.ANGSTROM_TO_BOHR, "def2-svp")
-    D, _ = wl.workload_density(mol, 20, 3)
-    return mol, D, len(Z)
-
-
-def _surrogate_error(s, xc, box_from=None):
-    """|E_xc(surrogate) - E_xc(analytic)| per atom on the held-out (H2O)4 / def2-SVP / SG-1 grid."""
-    if box_from is not None:  # an untrained network needs a box to be attached
-        t, _ = Surrogate.features(xc, *box_from)
-        s.set_box(t.min(axis=0), t.max(axis=0), 1e-8)
-    mol, D, natoms = _heldout()
-    g = ca.Grid(mol, "SG-1", 1, 0)
+def _c18():
+    w = wl.build_workload("c18_def2tzvp_sg3_r2scan")
+    mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
+    D, _ = wl.workload_density(mol, w["nocc"], w["seed"])
+    g = ca.Grid(mol, w["grid"], 1, 0)
     g.setAOCache(1)
     g.setType(2)
     g.getAO(0, 0)
     g.getDensity([D])
-    xc.setSurrogate(None)
-    xc.Evaluate("ev", g)
-    Ea = g.getEnergy()
-    xc.setSurrogate(s)
-    xc.Evaluate("ev", g)
-    Es = g.getEnergy()
-    xc.setSurrogate(None)
-    return abs(Es - Ea) / natoms
+    return g, len(w["Z"])
+
+
+def _functional_ms(g, xc, reps=5):
+    g.profile(True)
+    g.kernel_times(reset=True)
+    for _ in range(reps):
+        xc.Evaluate("ev", g)
+    ms, n = g.kernel_times(reset=True)["functional"]
+    g.profile(False)
+    return ms / max(1, n)
 
 
 def test_training_lowers_loss(trained):
-    s, xc, loss, _ = trained
+    s, xc, loss = trained
     assert np.all(np.isfinite(loss))
-    assert loss[-1] < 1e-2 * loss[0]
+    assert loss[-1] < 1e-3 * loss[0]
 
 
-def test_trained_network_gain(trained):
-    s, xc, loss, untrained = trained
-    err = _surrogate_error(s, xc)
-    _say("  |dE_xc| per atom vs analytic r2SCAN, held-out (H2O)4: untrained %.3e, trained %.3e Eh"
-         % (untrained, err))
-    # the gradient sums use float atomics, so runs differ; measured 47x and 110x gains at 40 epochs
-    assert err < 0.05 * untrained
+def test_gain_over_the_method_without_the_network(trained):
+    """Held-out cyclo[18]carbon / def2-TZVP / SG-3 (configuration 3; no C18 sample is in the training
+    set). The method without the network is its analytic base, Slater + VWN5 (LDA): the trained
+    correction must cut the base's E_xc error against r2SCAN by >= 100x and stay within 2e-3 Eh/atom
+    (measured 2e-4 to 4e-4 with 2 M samples; runs differ, the gradient sums use float atomics). Also
+    recorded, not asserted: the functional stage of each, against the analytic r2SCAN it stands in for
+    (the network is slower and less accurate than that)."""
+    s, xc, _ = trained
+    g, natoms = _c18()
+    xc.Evaluate("ev", g)
+    Ea = g.getEnergy()
+    ta = _functional_ms(g, xc)
+    lda = ca.ExchangeCorrelation().from_codes(list(LDA_BASE))
+    lda.Evaluate("ev", g)
+    Eb = g.getEnergy()
+    tb = _functional_ms(g, lda)
+    xc.setSurrogate(s)
+    try:
+        xc.Evaluate("ev", g)
+        Es = g.getEnergy()
+        nf = g.fallbackCount()
+        ts = _functional_ms(g, xc)
+    finally:
+        xc.setSurrogate(None)
+    eb, es = abs(Eb - Ea) / natoms, abs(Es - Ea) / natoms
+    _say("  C18 |dE_xc| per atom vs analytic r2SCAN: LDA base alone %.3e, base + network %.3e Eh (gain %.0fx); "
+         "%d fallback points; functional stage: r2SCAN %.4f ms, LDA %.4f ms, base + network %.4f ms"
+         % (eb, es, eb / es, nf, ta, tb, ts))
+    assert es <= 2e-3
+    assert eb >= 100 * es
 
 
 def test_grid_evaluate_matches_points(trained):
-    s, xc, _, _ = trained
-    mol, D, natoms = _heldout()
+    s, xc, _ = trained
+    Z, X = wl.water_cluster_angstrom(4, 5.0, 11)
+    mol = ca.Molecule(Z, X * ca.ANGSTROM_TO_BOHR, "def2-svp")
+    D, _ = wl.workload_density(mol, 20, 3)
     g = ca.Grid(mol, "SG-1", 1, 0)
     g.setAOCache(1)
     g.setType(2)
@@ -177,7 +196,7 @@ def test_grid_evaluate_matches_points(trained):
     rho, sigma, tau, w = g.pointDensity()
     lo, hi, rmin = s.box
     eo, _, _, _, used = ol.mlp_point(s.inference_params().astype(np.float64), lo.astype(np.float64),
-                                     hi.astype(np.float64), rmin, R2SCAN, rho, sigma, tau)
+                                     hi.astype(np.float64), rmin, R2SCAN, rho, sigma, tau, base=s.base)
     live = (w != 0) & (rho >= 1e-15)
     Eo = float(np.sum(w * rho * eo))
     _say("  grid E_xc %.10f, from oracle points %.10f; fallback %d (oracle %d)"

@@ -143,3 +143,49 @@ def test_fallback_is_the_analytic_functional():
     assert np.array_equal(used, inside) and not used.all()
     for a, b in ((e, ea), (vr, vra), (vs, vsa), (vt, vta)):
         assert np.array_equal(a[~used], b[~used])
+
+
+def test_zero_network_on_a_base_is_the_base():
+    """With an analytic base (Slater + VWN5) the network adds to the base: a network that outputs 0
+    returns the base functional's values and derivatives; F = 1 adds Slater exchange once more."""
+    rng = np.random.default_rng(12)
+    rho = 10 ** rng.uniform(-4, 2, 50)
+    sigma = 10 ** rng.uniform(-2, 0, 50) * rho ** (8 / 3)
+    tau = sigma / (8 * rho) + rho ** (5 / 3)
+    lo, hi = np.array([-30.0, 0.0, 0.0]), np.array([10.0, 1.0, 1.0])
+    P = np.zeros(ol.MLP_PARAMS)
+    e, vr, vs, vt, used = ol.mlp_point(P, lo, hi, 1e-8, R2SCAN, rho, sigma, tau, base=[1, 7])
+    assert used.all()
+    eb, vb, _, _ = ol.eval_functional([1, 7], rho)
+    np.testing.assert_allclose(e, eb, rtol=1e-14)
+    np.testing.assert_allclose(vr, vb, rtol=1e-14)
+    assert np.all(vs == 0) and np.all(vt == 0)
+    P[8640] = 1.0
+    e1, vr1, _, _, _ = ol.mlp_point(P, lo, hi, 1e-8, R2SCAN, rho, sigma, tau, base=[1, 7])
+    ex, vx, _, _ = ol.eval_functional([1], rho)
+    np.testing.assert_allclose(e1, eb + ex, rtol=1e-13)
+    np.testing.assert_allclose(vr1, vb + vx, rtol=1e-13)
+
+
+def test_point_potentials_are_derivatives_with_base():
+    P = _params(13, 0.3)
+    rng = np.random.default_rng(14)
+    n = 30
+    rho = 10 ** rng.uniform(-3, 1, n)
+    sigma = 10 ** rng.uniform(-2, 0.5, n) * 38.28 * rho ** (8 / 3)
+    tau = sigma / (8 * rho) + 10 ** rng.uniform(-1, 1, n) * 2.87 * rho ** (5 / 3)
+    lo, hi = np.array([-20.0, 0.0, 0.0]), np.array([5.0, 1.0, 1.0])
+    base = [101, 130]  # a GGA base, so that v_sigma has a base part too
+    e, vr, vs, vt, used = ol.mlp_point(P, lo, hi, 1e-8, R2SCAN, rho, sigma, tau, base=base)
+    assert used.all()
+    for v, c, x in ((vr, 0, rho), (vs, 1, sigma), (vt, 2, tau)):
+        h = 1e-6 * x
+        args = [rho, sigma, tau]
+
+        def E(val):
+            a = list(args)
+            a[c] = val
+            return a[0] * ol.mlp_point(P, lo, hi, 1e-8, R2SCAN, *a, base=base)[0]
+        fd = (E(x + h) - E(x - h)) / (2 * h)
+        scale = np.abs(rho * e) / x
+        assert np.all(np.abs(v - fd) <= 1e-6 * (np.abs(fd) + scale)), c
