@@ -35,7 +35,7 @@ FP64_MFMA_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix
 BF16_SPLIT_PEAK_TFLOPS = 2500.0 / 3.0
 HBM_PEAK_GBS = 8000.0
 DEFAULT_WORKLOAD = "h2o128_def2svp_sg2_r2scan"
-CPU_SAMPLE_SUBGRIDS = 16
+CPU_SAMPLE_SUBGRIDS = 64
 
 
 def _say(msg):
@@ -179,7 +179,7 @@ def main():
         _say("%s form: grid ready in %.1f s (AO tables %s)" % ("mixed" if precision else "FP64", t_setup,
                                                                "cached" if grid.aoCached() else "recomputed"))
         dV = hip.DeviceArray(n * n)
-        tuned = grid.autotune(xc, dD.ptr, dV.ptr, reps=2) if tune and grid.aoCached() else None
+        tuned = grid.autotuneTiles(xc, dD.ptr, dV.ptr, reps=2) if tune and precision == ca.CHX_PREC_MIXED else None
         ms, E, wall = time_steps(grid, steps, warmup, dV)
         _say("%s form: median %.3f ms per build over %d steps" % ("mixed" if precision else "FP64",
                                                                    statistics.median(ms), steps))
@@ -254,7 +254,10 @@ def main():
         "host_io_note": "median wall time of the by-value calls getDensity({D}) / getFock() with D and V in host "
                         "memory (PCIe included); not the value",
         "speedup_vs_plain_fp64": round(plain["ms"] / ms_step, 3) if plain else None,
-        "autotune": {"vxc_chunk": chosen["tuned"][0], "ms": {str(k): round(v, 4) for k, v in chosen["tuned"][1].items()}}
+        "autotune": {"vxc_shape": chosen["tuned"][0], "vxc_chunk": chosen["tuned"][1],
+                     "shapes": {"0": "128 x 128 blocks over 128-AO super-tile pairs",
+                                "1": "256 x 256 blocks over 8-tile groups of each chunk's alive-tile union"},
+                     "ms": {"%d/%d" % k: round(v, 4) for k, v in chosen["tuned"][2].items()}}
         if chosen["tuned"] else None,
         "tflops": round(flops_build / (ms_step * 1e-3) / 1e12, 3),
         "gpoints_per_s": round(st["points"] / (ms_step * 1e-3) / 1e9, 4),
@@ -540,7 +543,7 @@ def bench_pack(args):
                    "name": "alkanes64", "molecules": len(items), "points": points,
                    "nbasis_total": sum(it["mol"].nbasis for it in items),
                    "form": "mixed, per-molecule kernels on staggered lanes, hipGraph replay",
-                   "lanes": pack.lanes(), "vxc_chunk": chunk},
+                   "lanes": pack.lanes(), "vxc_chunk": chunk, "vxc_wide_from_npad": pack.vxcWideFrom()},
         "sequential_ms_per_step": round(ms_seq, 4),
         "speedup_vs_sequential": round(ms_seq / ms_pack, 3),
         "autotune_ms": {str(k): round(v, 4) for k, v in tune_ms.items()},

@@ -106,6 +106,10 @@ def lib():
         "chx_xc_eval_points_noalloc": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "chx_grid_get_density_skeleton": (i, [_vp, _vp, i, i]),
         "chx_grid_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
+        "chx_grid_autotune_tiles": (i, [_vp, _vp, _vp, _vp, i, P(i), P(i), _vp]),
+        "chx_grid_set_vxc_shape": (i, [_vp, i]),
+        "chx_grid_vxc_shape": (i, [_vp, P(i)]),
+        "chx_pack_vxc_wide_from": (i, [_vp, P(i)]),
         "chx_grid_energy_grad": (i, [_vp, _vp]),
         "chx_grid_get_density_u": (i, [_vp, _vp, i, i, i]),
         "chx_grid_fock_u": (i, [_vp, _vp, i, i, i]),
@@ -339,6 +343,24 @@ class Grid:
         _check(lib().chx_grid_autotune(self.h, xc.h, D_ptr, V_ptr, int(reps), ctypes.byref(c), ms))
         return c.value, dict(zip((16, 32, 64, 128), list(ms)))
 
+    def autotuneTiles(self, xc, D_ptr, V_ptr, reps=3):
+        """Mixed form: times both Vxc block shapes (0: 128-AO super-tiles, 1: 256-AO union groups) at
+        each work-item length, keeps the fastest; returns (shape, chunk, {(shape, chunk): ms})."""
+        sh, c = ctypes.c_int(), ctypes.c_int()
+        ms = (ctypes.c_float * 8)()
+        _check(lib().chx_grid_autotune_tiles(self.h, xc.h, D_ptr, V_ptr, int(reps), ctypes.byref(sh),
+                                             ctypes.byref(c), ms))
+        keys = [(a, b) for a in (0, 1) for b in (16, 32, 64, 128)]
+        return sh.value, c.value, dict(zip(keys, list(ms)))
+
+    def setVxcShape(self, shape):
+        _check(lib().chx_grid_set_vxc_shape(self.h, int(shape)))
+
+    def vxcShape(self):
+        v = ctypes.c_int()
+        _check(lib().chx_grid_vxc_shape(self.h, ctypes.byref(v)))
+        return v.value
+
     def getDensityU(self, Ds):
         """Grid::getDensityU (src/Grid/Grid.cpp:63-78): Ds[spin][mat], closed shell."""
         Ds = [list(x) for x in Ds]
@@ -437,9 +459,8 @@ class Grid:
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
 
     def vxcKernel(self):
-        """Name of the kernel the mixed form runs for the Vxc contraction on this basis size."""
-        npad = -(-self.nbasis // 128) * 128
-        return "k_vxc_mx_wide" if npad >= 1024 else "k_vxc_mx"
+        """Name of the kernel the mixed form runs for the Vxc contraction (the block shape in use)."""
+        return "k_vxc_mx_wide" if self.vxcShape() == 1 else "k_vxc_mx"
 
     def stats(self):
         nb, npnt, mt, fl = ctypes.c_long(), ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
@@ -643,6 +664,12 @@ class Pack:
         _check(lib().chx_pack_autotune(self.h, xc.h, self._ptrs(D_ptrs), self._ptrs(V_ptrs), int(reps),
                                        ctypes.byref(c), ms))
         return c.value, dict(zip((16, 32, 64, 128), list(ms)))
+
+    def vxcWideFrom(self):
+        """The autotuned Vxc shape threshold: 256-AO union groups for npad >= it (-1: none)."""
+        v = ctypes.c_int()
+        _check(lib().chx_pack_vxc_wide_from(self.h, ctypes.byref(v)))
+        return v.value
 
     def stream(self):
         p = ctypes.c_void_p()

@@ -17,6 +17,7 @@ namespace chx {
 void grid_build(chx_grid* g, const std::string& grid_name, const std::string& data_root);
 void grid_plan(chx_grid* g);
 void set_vxc_chunk(chx_grid* g, int chunk);
+void set_vxc_shape(chx_grid* g, int shape);
 void ensure_stream(chx_grid* g);
 void launch_selftest(hipStream_t s, double* out);
 }  // namespace chx
@@ -652,6 +653,52 @@ int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double
     g->profile = prof;
     set_vxc_chunk(g, cand[best]);
     if (chunk_out) *chunk_out = cand[best];
+  });
+}
+
+int chx_grid_set_vxc_shape(chx_grid* g, int shape) {
+  return guard([&] {
+    require(g && shape >= -1 && shape <= 1, CHX_ERR_ARG, "Vxc shape must be -1 (by basis size), 0 or 1");
+    set_vxc_shape(g, shape);
+  });
+}
+
+int chx_grid_vxc_shape(const chx_grid* g, int* shape) {
+  return guard([&] {
+    require(g && shape, CHX_ERR_ARG, "bad arguments");
+    *shape = g->mx_wide;
+  });
+}
+
+int chx_grid_autotune_tiles(chx_grid* g, const chx_xc* xc, const double* D_dev, double* V_dev, int reps,
+                            int* shape_out, int* chunk_out, float* ms_out) {
+  return guard([&] {
+    require(g && xc && D_dev && V_dev && reps > 0, CHX_ERR_ARG, "bad arguments");
+    require(g->precision == CHX_PREC_MIXED && g->ao_ready, CHX_ERR_STATE,
+            "tile shapes are tuned for the mixed form after Grid::getAO");
+    int best_shape = -1, best_chunk = 0;
+    float best_ms = 0.f;
+    for (int shape = 0; shape < 2; ++shape) {
+      set_vxc_shape(g, shape);
+      int chunk = 0;
+      float ms[4];
+      const int rc = chx_grid_autotune(g, xc, D_dev, V_dev, reps, &chunk, ms);
+      if (rc != CHX_OK) throw Error(rc, g_last_error);
+      float m = ms[0];
+      for (int k = 0; k < 4; ++k) {
+        if (ms_out) ms_out[4 * shape + k] = ms[k];
+        m = std::min(m, ms[k]);
+      }
+      if (best_shape < 0 || m < best_ms) {
+        best_shape = shape;
+        best_chunk = chunk;
+        best_ms = m;
+      }
+    }
+    set_vxc_shape(g, best_shape);
+    set_vxc_chunk(g, best_chunk);
+    if (shape_out) *shape_out = best_shape;
+    if (chunk_out) *chunk_out = best_chunk;
   });
 }
 

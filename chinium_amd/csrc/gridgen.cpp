@@ -526,6 +526,15 @@ void set_vxc_chunk(chx_grid* g, int chunk) {
   CHX_HIP(hipStreamSynchronize(st));
 }
 
+// Vxc block shape of the mixed form (-1: 256-AO union groups from npad 1024
+// on, 128-AO super-tiles below; 0, 1: either), rebuilding the work items.
+void set_vxc_shape(chx_grid* g, int shape) {
+  g->vxc_shape = shape;
+  if (!g->ao_ready || g->precision != CHX_PREC_MIXED) return;
+  g->mx_wide = (shape == 1 || (shape < 0 && g->npad >= 1024)) ? 1 : 0;
+  set_vxc_chunk(g, g->vxc_chunk);
+}
+
 // Screening and work plans (Grid::getAO counterpart).
 void grid_plan(chx_grid* g) {
   hipStream_t st = g->stream;
@@ -633,7 +642,7 @@ void grid_plan(chx_grid* g) {
                                     " GB of device memory for its AO cache");
   g->cached = mixed || g->cache_mode == 1 || (g->cache_mode == -1 && cache_bytes < 0.8 * (double)free_b);
   // large bases: Vxc items over tile groups of batch chunks, 256 x 256 blocks
-  g->mx_wide = mixed && g->npad >= 1024 ? 1 : 0;
+  g->mx_wide = mixed && (g->vxc_shape == 1 || (g->vxc_shape < 0 && g->npad >= 1024)) ? 1 : 0;
   if (g->cached) {
     build_cwork(g);
     g->d_slot_batch.upload(g->slot_batch, st); g->d_slot_tile.upload(g->slot_tile, st);

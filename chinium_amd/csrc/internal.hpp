@@ -362,6 +362,7 @@ struct chx_grid {
   int cache_mode = -1;               // -1 automatic, 0 never, 1 always
   std::vector<int> slot_batch, slot_tile, slot_of, tile_nao, grp_slots;
   int vxc_chunk = 0;                 // batches per Vxc work item; 0 = the form's default
+  int vxc_shape = -1;                // mixed form's Vxc blocks: -1 by basis size, 0 128-AO super-tiles, 1 256-AO union groups
   int mx_wide = 0;                   // mixed form: Vxc over 256-AO super-tiles (large bases)
   std::vector<int4> cwork;
   std::vector<int> cpair_batches;

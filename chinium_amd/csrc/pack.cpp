@@ -25,6 +25,7 @@
 
 namespace chx {
 void set_vxc_chunk(chx_grid* g, int chunk);
+void set_vxc_shape(chx_grid* g, int shape);
 std::string& last_error();
 }
 using namespace chx;
@@ -45,6 +46,7 @@ struct chx_pack {
   std::vector<unsigned long> gen;  // each grid's plan generation when the lanes and graph were built
   std::vector<double> h_en;
   int lanes = 16;
+  int wide_from = -2;    // Vxc shape of the last autotune: 256-AO groups for npad >= wide_from (-1: none; -2: not tuned)
   std::vector<int> cut;  // lane k holds molecules [cut[k], cut[k + 1])
   std::vector<hipStream_t> lane_st;
   std::vector<hipEvent_t> ev_dens, ev_done;
@@ -307,6 +309,28 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
       }
     }
     for (chx_grid* g : p->grids) set_vxc_chunk(g, cand[best]);
+    // then the Vxc tile shape per molecule, by a basis-size threshold: 256-AO union groups for
+    // npad >= T, 128-AO super-tiles below (T = 0: all groups; -1: none)
+    static const int thr[] = {-1, 1024, 512, 0};
+    int best_thr = -1;
+    float best_thr_ms = 0.f;
+    for (int T : thr) {
+      for (chx_grid* g : p->grids) set_vxc_shape(g, T >= 0 && g->npad >= T ? 1 : 0);
+      build_lanes(p);
+      run(p, xc, D_dev, V_dev, nullptr, nullptr);
+      CHX_HIP(hipEventRecord(a, p->stream));
+      for (int r = 0; r < reps; ++r) run(p, xc, D_dev, V_dev, nullptr, nullptr);
+      CHX_HIP(hipEventRecord(b, p->stream));
+      CHX_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      CHX_HIP(hipEventElapsedTime(&ms, a, b));
+      if (best_thr_ms == 0.f || ms < best_thr_ms) {
+        best_thr = T;
+        best_thr_ms = ms;
+      }
+    }
+    for (chx_grid* g : p->grids) set_vxc_shape(g, best_thr >= 0 && g->npad >= best_thr ? 1 : 0);
+    p->wide_from = best_thr;
     // then the lane count at that length (4, 8, 16, at most one lane per molecule)
     const int n = (int)p->grids.size();
     int best_lanes = -1;
@@ -332,6 +356,13 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
     p->lanes = best_lanes;
     build_lanes(p);
     if (chunk_out) *chunk_out = cand[best];
+  });
+}
+
+int chx_pack_vxc_wide_from(const chx_pack* p, int* npad_min) {
+  return pguard([&] {
+    prequire(p && npad_min, CHX_ERR_ARG, "bad arguments");
+    *npad_min = p->wide_from;
   });
 }
 

@@ -178,6 +178,18 @@ int chx_grid_set_screening(chx_grid* g, double threshold);
  * ms_out[4] the times. Needs the AO tables in HBM (cached or mixed form). */
 int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double* V_dev, int reps, int* chunk_out,
                       float* ms_out);
+/* Mixed form's Vxc block shape (no reference counterpart): 0 = 128 x 128
+ * blocks over 128-AO super-tile pairs and the batches where both are alive;
+ * 1 = 256 x 256 blocks whose rows and columns are groups of 8 tiles of each
+ * batch chunk's alive-tile union, dead tiles skipped per batch; -1 (default)
+ * = 1 from npad 1024 on, else 0. chx_grid_vxc_shape reports the shape in use. */
+int chx_grid_set_vxc_shape(chx_grid* g, int shape);
+int chx_grid_vxc_shape(const chx_grid* g, int* shape);
+/* chx_grid_autotune over both shapes (mixed form): times the four work
+ * lengths for shape 0, then for shape 1, keeps the fastest pair;
+ * ms_out[8] = [shape][length]. */
+int chx_grid_autotune_tiles(chx_grid* g, const chx_xc* xc, const double* D_dev, double* V_dev, int reps,
+                            int* shape_out, int* chunk_out, float* ms_out);
 /* Statistics of the batch layout: batches, points, AO tiles per batch (mean),
  * algorithmic FLOPs of one XC build (dense-contraction count, k*2*sum g_b n_b^2). */
 int chx_grid_stats(const chx_grid* g, long* nbatches, long* npoints, double* mean_tiles, double* flops_per_build);
@@ -287,10 +299,14 @@ int chx_pack_xc(chx_pack* p, const chx_xc* xc, const double* const* D_dev, doubl
 /* 1 (default) replays a captured hipGraph, 0 launches eagerly. */
 int chx_pack_set_graph(chx_pack* p, int enable);
 /* Autotuner: times `reps` builds for each Vxc work-item length (16, 32, 64,
- * 128 batches), keeps the fastest (chunk_out gets it, ms_out[4] the times),
- * then times 4, 8 and 16 lanes at that length and keeps the fastest. */
+ * 128 batches), keeps the fastest (chunk_out gets it, ms_out[4] the times);
+ * then the Vxc tile shape per molecule (chx_grid_set_vxc_shape) by a
+ * basis-size threshold T (256-AO union groups for npad >= T: none, 1024, 512,
+ * all), keeps the fastest; then 4, 8 and 16 lanes, keeps the fastest. */
 int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev, double* const* V_dev, int reps,
                       int* chunk_out, float* ms_out);
+/* The threshold the last autotune chose (-1: no molecule on union groups; -2: not tuned). */
+int chx_pack_vxc_wide_from(const chx_pack* p, int* npad_min);
 /* Lanes (1..16, default 16, at most one per molecule): the molecules are
  * split into that many contiguous groups of about equal work, each on a stream
  * of its own; a group's density launches wait for the previous group's, so the

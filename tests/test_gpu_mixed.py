@@ -160,3 +160,45 @@ def test_mixed_meta_grid_after_gga_potentials():
     assert np.abs(V - Vref).max() <= V_TOL
     check("PBE")
     check("R2SCAN")
+
+
+@pytest.mark.parametrize("name", ["h2o16_def2svp_sg2_b3lyp", "c18_def2tzvp_sg3_r2scan"])
+def test_vxc_block_shapes_agree(name):
+    """Both Vxc block shapes of the mixed form (128-AO super-tile pairs; 256-AO groups of each batch
+    chunk's alive-tile union) on bases below the default switch (npad 384 and 640): the same E_xc (the
+    shape only changes Vxc), V within the mixed form's rounding of each other and of the FP64 form;
+    the tile autotuner keeps one of them."""
+    from chinium_amd import hip
+    w = wl.build_workload(name)
+    mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
+    D, _ = wl.workload_density(mol, w["nocc"], w["seed"])
+    xc = ca.ExchangeCorrelation()
+    xc.Read(w["functional"])
+    g64 = ca.Grid(mol, w["grid"], 1, 0)
+    g64.setType(xc.family)
+    g64.getAO(0, 0)
+    E64, V64 = ca.xc_build(g64, xc, D)
+    g = ca.Grid(mol, w["grid"], 1, 0)
+    g.setPrecision(ca.CHX_PREC_MIXED)
+    g.setType(xc.family)
+    g.getAO(0, 0)
+    out = {}
+    for shape in (0, 1):
+        g.setVxcShape(shape)
+        assert g.vxcShape() == shape
+        out[shape] = ca.xc_build(g, xc, D)
+    (E0, V0), (E1, V1) = out[0], out[1]
+    print("  %s: shape 0 vs 1: dE %.1e, max dV %.1e; vs FP64: %.1e, %.1e" % (
+        name, E1 - E0, np.abs(V1 - V0).max(), np.abs(V0 - V64).max(), np.abs(V1 - V64).max()), flush=True)
+    assert E0 == E1
+    assert np.abs(V1 - V0).max() <= 5e-6
+    assert np.abs(V0 - V64).max() <= 5e-5 and np.abs(V1 - V64).max() <= 5e-5
+    n = mol.nbasis
+    dD, dV = hip.DeviceArray(n * n), hip.DeviceArray(n * n)
+    dD.upload(np.asfortranarray(D).ravel(order="F"))
+    shape, chunk, ms = g.autotuneTiles(xc, dD.ptr, dV.ptr, reps=1)
+    print("  autotuned shape %d, length %d: %s" % (shape, chunk, {k: round(v, 3) for k, v in ms.items()}),
+          flush=True)
+    assert shape in (0, 1) and g.vxcShape() == shape and min(ms.values()) == ms[(shape, chunk)]
+    E2, V2 = ca.xc_build(g, xc, D)
+    assert E2 == E0 and np.abs(V2 - V64).max() <= 5e-5
