@@ -108,6 +108,8 @@ def lib():
         "chx_grid_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
         "chx_grid_autotune_tiles": (i, [_vp, _vp, _vp, _vp, i, P(i), P(i), _vp]),
         "chx_grid_set_vxc_shape": (i, [_vp, i]),
+        "chx_grid_set_vxc_chunk": (i, [_vp, i]),
+        "chx_grid_core_flops": (i, [_vp, P(d)]),
         "chx_grid_vxc_shape": (i, [_vp, P(i)]),
         "chx_pack_vxc_wide_from": (i, [_vp, P(i)]),
         "chx_grid_energy_grad": (i, [_vp, _vp]),
@@ -298,6 +300,11 @@ class Grid:
         """Mixed form: batches within r bohr of a heavy nucleus take three-term operands (0: off)."""
         _check(lib().chx_grid_set_core_radius(self.h, float(r)))
 
+    def coreFlops(self):
+        v = ctypes.c_double()
+        _check(lib().chx_grid_core_flops(self.h, ctypes.byref(v)))
+        return v.value
+
     def coreBatches(self):
         n = ctypes.c_long()
         _check(lib().chx_grid_core_batches(self.h, ctypes.byref(n)))
@@ -341,7 +348,7 @@ class Grid:
         c = ctypes.c_int()
         ms = (ctypes.c_float * 4)()
         _check(lib().chx_grid_autotune(self.h, xc.h, D_ptr, V_ptr, int(reps), ctypes.byref(c), ms))
-        return c.value, dict(zip((16, 32, 64, 128), list(ms)))
+        return c.value, dict(zip((8, 16, 32, 128), list(ms)))
 
     def autotuneTiles(self, xc, D_ptr, V_ptr, reps=3):
         """Mixed form: times both Vxc block shapes (0: 128-AO super-tiles, 1: 256-AO union groups) at
@@ -350,8 +357,11 @@ class Grid:
         ms = (ctypes.c_float * 8)()
         _check(lib().chx_grid_autotune_tiles(self.h, xc.h, D_ptr, V_ptr, int(reps), ctypes.byref(sh),
                                              ctypes.byref(c), ms))
-        keys = [(a, b) for a in (0, 1) for b in (16, 32, 64, 128)]
+        keys = [(a, b) for a in (0, 1) for b in (8, 16, 32, 128)]
         return sh.value, c.value, dict(zip(keys, list(ms)))
+
+    def setVxcChunk(self, chunk):
+        _check(lib().chx_grid_set_vxc_chunk(self.h, int(chunk)))
 
     def setVxcShape(self, shape):
         _check(lib().chx_grid_set_vxc_shape(self.h, int(shape)))
@@ -437,7 +447,7 @@ class Grid:
         _check(lib().chx_grid_stream(self.h, ctypes.byref(s)))
         return s.value
 
-    STAGES = ("permute_d", "density", "functional", "vxc", "finalize_v", "make_b")
+    STAGES = ("permute_d", "density", "functional", "vxc", "finalize_v", "make_b", "density_core")
 
     def setAOCache(self, mode):
         """-1 automatic (default), 0 recompute AO values in the kernels, 1 cache them in HBM."""
@@ -453,8 +463,8 @@ class Grid:
 
     def kernel_times(self, reset=True):
         """Accumulated HIP-event milliseconds and launch counts per stage."""
-        ms = np.zeros(6)
-        n = np.zeros(6, dtype=np.int64)
+        ms = np.zeros(7)
+        n = np.zeros(7, dtype=np.int64)
         _check(lib().chx_grid_kernel_times(self.h, int(bool(reset)), _ptr(ms), _ptr(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
 
@@ -663,7 +673,7 @@ class Pack:
         ms = (ctypes.c_float * 4)()
         _check(lib().chx_pack_autotune(self.h, xc.h, self._ptrs(D_ptrs), self._ptrs(V_ptrs), int(reps),
                                        ctypes.byref(c), ms))
-        return c.value, dict(zip((16, 32, 64, 128), list(ms)))
+        return c.value, dict(zip((8, 16, 32, 128), list(ms)))
 
     def vxcWideFrom(self):
         """The autotuned Vxc shape threshold: 256-AO union groups for npad >= it (-1: none)."""

@@ -133,7 +133,7 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
 }
 
 // Stage ids of chx_grid_kernel_times.
-enum { kStPermute = 0, kStDensity = 1, kStFunctional = 2, kStVxc = 3, kStFinalize = 4, kStMakeB = 5 };
+enum { kStPermute = 0, kStDensity = 1, kStFunctional = 2, kStVxc = 3, kStFinalize = 4, kStMakeB = 5, kStCore = 6 };
 
 void density_from_device(chx_grid* g, const double* dD) {
   hipEvent_t e = nullptr;
@@ -147,10 +147,14 @@ void density_from_device(chx_grid* g, const double* dD) {
     // meta-GGA: rho, grad rho and the tau partials (A = d_r phi) in one launch that stages D once
     // (rho, grad rho alone: the core batches' groups are left to the core launch)
     launch_density_mx(g->stream, g->type >= 2 ? 5 : 0, g->type >= 2 ? ng : ng - ncg, g->hmol);
-    // core batches' rho and grad rho again with three-term operands (overwriting their partials; tau
-    // keeps two terms: near the nuclei its error is far below rho's effect on E_xc)
-    launch_density_mx_core(g->stream, 0, ncg, g->hmol);
     g->mark_end(kStDensity, e);
+    // core batches' rho and grad rho with three-term operands (meta-GGA: overwriting their partials;
+    // tau keeps two terms: near the nuclei its error is far below rho's effect on E_xc)
+    if (ncg) {
+      g->mark_begin(kStCore, &e);
+      launch_density_mx_core(g->stream, 0, ncg, g->hmol);
+      g->mark_end(kStCore, e);
+    }
     g->have_density = true;
     g->have_eval = false;
     g->have_v = false;
@@ -616,7 +620,7 @@ int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double
     require(g && xc && D_dev && V_dev && reps > 0, CHX_ERR_ARG, "bad arguments");
     require(g->ao_ready && g->cached, CHX_ERR_STATE, "Grid::getAO (with the AO tables in HBM) must precede tuning");
     require(g->type >= xc->family, CHX_ERR_STATE, "grid type is below the functional family (call setType)");
-    static const int cand[] = {16, 32, 64, 128};
+    static const int cand[] = {8, 16, 32, 128};
     hipEvent_t a, b;
     CHX_HIP(hipEventCreate(&a));
     CHX_HIP(hipEventCreate(&b));
@@ -653,6 +657,13 @@ int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double
     g->profile = prof;
     set_vxc_chunk(g, cand[best]);
     if (chunk_out) *chunk_out = cand[best];
+  });
+}
+
+int chx_grid_set_vxc_chunk(chx_grid* g, int chunk) {
+  return guard([&] {
+    require(g && chunk >= 0 && chunk <= kMxMaxChunk, CHX_ERR_ARG, "Vxc work length must be 0 (default) to 128 batches");
+    set_vxc_chunk(g, chunk);
   });
 }
 
@@ -808,6 +819,13 @@ int chx_grid_core_batches(const chx_grid* g, long* n) {
   });
 }
 
+int chx_grid_core_flops(const chx_grid* g, double* flops) {
+  return guard([&] {
+    require(g && flops, CHX_ERR_ARG, "bad arguments");
+    *flops = g->precision == CHX_PREC_MIXED && g->ao_ready ? g->core_flops : 0.0;
+  });
+}
+
 int chx_grid_set_screening(chx_grid* g, double thr) {
   return guard([&] {
     require(thr >= 0, CHX_ERR_ARG, "screening threshold must be >= 0");
@@ -826,7 +844,7 @@ int chx_grid_profile(chx_grid* g, int enable) {
 int chx_grid_kernel_times(chx_grid* g, int reset, double* ms, long* launches) {
   return guard([&] {
     g->collect_times();
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 7; ++k) {
       if (ms) ms[k] = g->stage_ms[k];
       if (launches) launches[k] = g->stage_n[k];
       if (reset) { g->stage_ms[k] = 0; g->stage_n[k] = 0; }

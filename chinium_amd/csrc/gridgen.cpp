@@ -549,6 +549,7 @@ void grid_plan(chx_grid* g) {
   for (int t = 0; t < nt; ++t)
     for (int k = 0; k < kTile; ++k) tile_nao[t] += g->int2ref[t * kTile + k] >= 0 ? 1 : 0;
   g->bt_off.assign(nb + 1, 0);
+  g->batch_work.assign(nb, 0.0);
   g->bt_list.clear();
   g->grp_batch.clear(); g->grp_tiles.clear(); g->grp_ntiles.clear();
   g->grp_off.assign(nb + 1, 0);
@@ -568,7 +569,8 @@ void grid_plan(chx_grid* g) {
       for (int q = 0; q < kGroup; ++q) g->grp_tiles.push_back(q < cnt ? lst[k + q] : 0);
     }
     tiles += (double)lst.size();
-    flops += (double)g->batch_n[b] * (double)nao * (double)nao;
+    g->batch_work[b] = (double)g->batch_n[b] * (double)nao * (double)nao;
+    flops += g->batch_work[b];
   }
   g->bt_off[nb] = (int)g->bt_list.size();
   g->grp_off[nb] = (int)g->grp_batch.size();
@@ -669,6 +671,7 @@ void grid_plan(chx_grid* g) {
       g->core_grps.clear();
       g->main_grps.clear();
       g->ncore_batches = 0;
+      g->core_flops = 0.0;
       int ncore_slots = 0;
       if (g->core_radius > 0) {
         const double r2c = g->core_radius * g->core_radius;  // <= 1 bohr: the hash's reach
@@ -684,6 +687,7 @@ void grid_plan(chx_grid* g) {
             continue;
           }
           ++g->ncore_batches;
+          g->core_flops += 2.0 * g->batch_work[b];
           core_off[b] = ncore_slots;
           for (int it = g->bt_off[b]; it < g->bt_off[b + 1]; ++it) slot_core[it] = ncore_slots++;
           for (int k = g->grp_off[b]; k < g->grp_off[b + 1]; ++k) g->core_grps.push_back(k);

@@ -384,6 +384,8 @@ struct chx_grid {
   chx::DevBuf<int> d_slot_batch, d_slot_tile, d_slot_of, d_tile_nao, d_grp_slots, d_cpair_batches;
   chx::DevBuf<int4> d_cwork;
   double flops_build = 0, mean_tiles = 0;
+  std::vector<double> batch_work;  // g_b n_b^2 per batch (points x kept AOs squared)
+  double core_flops = 0;           // the core batches' three-term (rho, grad rho) launch: 2 sum g_b n_b^2
 
   // Device mirrors.
   chx::DevBuf<double> d_sh_xyz, d_p_exp, d_p_coef;

@@ -285,7 +285,7 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
                       int* chunk_out, float* ms_out) {
   return pguard([&] {
     prequire(p && xc && D_dev && V_dev && reps > 0, CHX_ERR_ARG, "bad arguments");
-    static const int cand[] = {16, 32, 64, 128};
+    static const int cand[] = {8, 16, 32, 128};
     hipEvent_t a, b;
     CHX_HIP(hipEventCreate(&a));
     CHX_HIP(hipEventCreate(&b));

@@ -168,13 +168,17 @@ int chx_grid_set_precision(chx_grid* g, int precision);
 int chx_grid_set_core_radius(chx_grid* g, double radius);
 /* number of such core batches in the current plan (0 for the FP64 form) */
 int chx_grid_core_batches(const chx_grid* g, long* n);
+/* algorithmic FLOPs of the core batches' launch per build (2 sum g_b n_b^2
+ * over those batches, the (rho, grad rho) contraction); the LDA/GGA density
+ * launch does the rest of that stage's work */
+int chx_grid_core_flops(const chx_grid* g, double* flops);
 /* AO screening threshold: an AO tile is dropped from a batch when every |phi|
  * and |grad phi| in it stays at or below the threshold at every point of the
  * batch. Takes effect at the next chx_grid_get_ao. Default 1e-14. */
 int chx_grid_set_screening(chx_grid* g, double threshold);
 /* Autotuner (no reference counterpart): times `reps` XC builds (D_dev ->
  * V_dev, device pointers, column-major nbasis^2) for each Vxc work-item
- * length (16, 32, 64, 128 batches), keeps the fastest; chunk_out gets it,
+ * length (8, 16, 32, 128 batches), keeps the fastest; chunk_out gets it,
  * ms_out[4] the times. Needs the AO tables in HBM (cached or mixed form). */
 int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double* V_dev, int reps, int* chunk_out,
                       float* ms_out);
@@ -184,6 +188,9 @@ int chx_grid_autotune(chx_grid* g, const chx_xc* xc, const double* D_dev, double
  * batch chunk's alive-tile union, dead tiles skipped per batch; -1 (default)
  * = 1 from npad 1024 on, else 0. chx_grid_vxc_shape reports the shape in use. */
 int chx_grid_set_vxc_shape(chx_grid* g, int shape);
+/* Vxc work-item length in batches (0: the form's default, at most 128); the
+ * autotuners set it from their timings. */
+int chx_grid_set_vxc_chunk(chx_grid* g, int chunk);
 int chx_grid_vxc_shape(const chx_grid* g, int* shape);
 /* chx_grid_autotune over both shapes (mixed form): times the four work
  * lengths for shape 0, then for shape 1, keeps the fastest pair;
@@ -202,10 +209,11 @@ int chx_grid_set_ao_cache(chx_grid* g, int mode);
 int chx_grid_ao_cached(const chx_grid* g, int* cached);
 /* Per-kernel HIP-event timing on the handle's stream (no reference counterpart).
  * Stages: 0 D permute, 1 density contraction, 2 functional + E/N sums,
- * 3 Vxc contraction, 4 V finalise, 5 Vxc integrand B (cached path).
+ * 3 Vxc contraction, 4 V finalise, 5 Vxc integrand B (cached path),
+ * 6 the core batches' three-term density launch (mixed form).
  * Times accumulate until read with reset. */
 int chx_grid_profile(chx_grid* g, int enable);
-int chx_grid_kernel_times(chx_grid* g, int reset, double* ms /*[6]*/, long* launches /*[6]*/);
+int chx_grid_kernel_times(chx_grid* g, int reset, double* ms /*[7]*/, long* launches /*[7]*/);
 
 /* ---- Functional ------------------------------------------------------------
  * ExchangeCorrelation::Read(std::string df, bool output)
@@ -298,7 +306,7 @@ int chx_pack_xc(chx_pack* p, const chx_xc* xc, const double* const* D_dev, doubl
                 double* N);
 /* 1 (default) replays a captured hipGraph, 0 launches eagerly. */
 int chx_pack_set_graph(chx_pack* p, int enable);
-/* Autotuner: times `reps` builds for each Vxc work-item length (16, 32, 64,
+/* Autotuner: times `reps` builds for each Vxc work-item length (8, 16, 32,
  * 128 batches), keeps the fastest (chunk_out gets it, ms_out[4] the times);
  * then the Vxc tile shape per molecule (chx_grid_set_vxc_shape) by a
  * basis-size threshold T (256-AO union groups for npad >= T: none, 1024, 512,

@@ -97,7 +97,7 @@ def test_pack_autotune(alkanes):
     E0, _, V0 = _run(alkanes)
     chunk, ms = pack.autotune(xc, [d.ptr for d in alkanes["dD"]], [v.ptr for v in alkanes["dV"]], reps=2)
     _say("  autotune: chunk %d, lanes %d, ms %s" % (chunk, pack.lanes(), {k: round(v, 4) for k, v in ms.items()}))
-    assert chunk in (16, 32, 64, 128) and all(v > 0 for v in ms.values())
+    assert chunk in (8, 16, 32, 128) and all(v > 0 for v in ms.values())
     assert 1 <= pack.lanes() <= len(alkanes["mols"])
     E, _, Vs = _run(alkanes)
     # the density side does not depend on the Vxc item length; M is summed in
