@@ -125,6 +125,15 @@ void launch_permute_d(hipStream_t s, int n, int npad, const double* D, const int
 void launch_density_f64(hipStream_t s, int mode, int ngroups, const int* grp_batch, const int* grp_tiles,
                         const int* grp_ntiles, const int* bt_off, const int* bt_list, const double* bx,
                         const double* by, const double* bz, BasisView bv, const double* Dint, int npad, double* part);
+// One molecule of a multi-molecule functional launch: byte offsets of its
+// buffers from the launch's base pointer, its slot count, its first block
+// (entry nmol is a sentinel whose blk0 is the total block count) and its E/N slot.
+struct FnMolecule {
+  long grp_off, part, part_tau, w, rho, coef, en_part;
+  int nslots, blk0, slot_en, pad;
+};
+void launch_functional_multi(hipStream_t s, const char* base, const FnMolecule* fm, int nmol, int nblocks,
+                             const int* codes, int ncodes, int family, double* en);
 void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* part_tau,
                        const double* bw, const int* codes, int ncodes, int family, int want_e, int want_v,
                        double* rho_out, double* coef, double* en_part, int en_blocks);

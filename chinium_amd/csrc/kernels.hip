@@ -289,60 +289,80 @@ void launch_density_f64(hipStream_t s, int mode, int ngroups, const int* grp_bat
 // Functional: one thread per slot. Sums the density partials of the batch's
 // work groups, evaluates the functional and writes the Vxc integrand
 // coefficients; block partial sums of E = sum w rho eps and N = sum w rho.
+__device__ __forceinline__ void functional_slot(int slot, int nslots, const int* __restrict__ grp_off,
+                                                const double* __restrict__ part, const double* __restrict__ part_tau,
+                                                const double* __restrict__ bw, const int* __restrict__ codes,
+                                                int ncodes, int family, int want_e, int want_v,
+                                                double* __restrict__ rho_out, double* __restrict__ coef, double& e_acc,
+                                                double& n_acc) {
+  e_acc = n_acc = 0.0;
+  if (slot >= nslots) return;
+  const int b = slot / kBatch, g = slot % kBatch;
+  double rho = 0, gx = 0, gy = 0, gz = 0, tau = 0;
+  for (int k = grp_off[b]; k < grp_off[b + 1]; ++k) {
+    const double* p = part + (size_t)k * 4 * kBatch;
+    rho += p[g];
+    gx += p[kBatch + g];
+    gy += p[2 * kBatch + g];
+    gz += p[3 * kBatch + g];
+    if (family >= 2) {
+      const double* q = part_tau + (size_t)k * 3 * kBatch;
+      tau += q[g] + q[kBatch + g] + q[2 * kBatch + g];
+    }
+  }
+  gx *= 2; gy *= 2; gz *= 2;
+  tau *= 0.5;
+  const double w = bw[slot];
+  double sigma = family >= 1 ? gx * gx + gy * gy + gz * gz : 0.0;
+  double eps, vr, vs, vt;
+  eval_codes(codes, ncodes, rho, sigma, tau, &eps, &vr, &vs, &vt);
+  rho_out[slot] = rho;
+  if (want_v) {
+    coef[kCoef * (size_t)slot + 0] = 0.5 * w * vr;
+    const double f = family >= 1 ? 2.0 * w * vs : 0.0;
+    coef[kCoef * (size_t)slot + 1] = f * gx;
+    coef[kCoef * (size_t)slot + 2] = f * gy;
+    coef[kCoef * (size_t)slot + 3] = f * gz;
+    coef[kCoef * (size_t)slot + 4] = family >= 2 ? 0.25 * w * vt : 0.0;
+  }
+  if (want_e) e_acc = w * rho * eps;
+  n_acc = w * rho;
+}
+
+// Sums of (e, n) over a 256-thread block: within each wave by xor shuffles,
+// then the four wave sums in a fixed order. Thread 0 holds the result.
+__device__ __forceinline__ void block_sum2(double& e, double& n) {
+  __shared__ double red[2][4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off, 64);
+    n += __shfl_xor(n, off, 64);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][wave] = e;
+    red[1][wave] = n;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    e = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    n = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_functional(int nslots, const int* __restrict__ grp_off,
                                                     const double* __restrict__ part, const double* __restrict__ part_tau,
                                                     const double* __restrict__ bw, const int* __restrict__ codes,
                                                     int ncodes, int family, int want_e, int want_v,
                                                     double* __restrict__ rho_out, double* __restrict__ coef,
                                                     double* __restrict__ en_part) {
-  __shared__ double red[2][256];
-  const int slot = blockIdx.x * 256 + threadIdx.x;
-  double e_acc = 0, n_acc = 0;
-  if (slot < nslots) {
-    const int b = slot / kBatch, g = slot % kBatch;
-    double rho = 0, gx = 0, gy = 0, gz = 0, tau = 0;
-    for (int k = grp_off[b]; k < grp_off[b + 1]; ++k) {
-      const double* p = part + (size_t)k * 4 * kBatch;
-      rho += p[g];
-      gx += p[kBatch + g];
-      gy += p[2 * kBatch + g];
-      gz += p[3 * kBatch + g];
-      if (family >= 2) {
-        const double* q = part_tau + (size_t)k * 3 * kBatch;
-        tau += q[g] + q[kBatch + g] + q[2 * kBatch + g];
-      }
-    }
-    gx *= 2; gy *= 2; gz *= 2;
-    tau *= 0.5;
-    const double w = bw[slot];
-    double sigma = family >= 1 ? gx * gx + gy * gy + gz * gz : 0.0;
-    double eps, vr, vs, vt;
-    eval_codes(codes, ncodes, rho, sigma, tau, &eps, &vr, &vs, &vt);
-    rho_out[slot] = rho;
-    if (want_v) {
-      coef[kCoef * (size_t)slot + 0] = 0.5 * w * vr;
-      const double f = family >= 1 ? 2.0 * w * vs : 0.0;
-      coef[kCoef * (size_t)slot + 1] = f * gx;
-      coef[kCoef * (size_t)slot + 2] = f * gy;
-      coef[kCoef * (size_t)slot + 3] = f * gz;
-      coef[kCoef * (size_t)slot + 4] = family >= 2 ? 0.25 * w * vt : 0.0;
-    }
-    if (want_e) e_acc = w * rho * eps;
-    n_acc = w * rho;
-  }
-  red[0][threadIdx.x] = e_acc;
-  red[1][threadIdx.x] = n_acc;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + st];
-      red[1][threadIdx.x] += red[1][threadIdx.x + st];
-    }
-    __syncthreads();
-  }
+  double e, n;
+  functional_slot(blockIdx.x * 256 + threadIdx.x, nslots, grp_off, part, part_tau, bw, codes, ncodes, family, want_e,
+                  want_v, rho_out, coef, e, n);
+  block_sum2(e, n);
   if (threadIdx.x == 0) {
-    en_part[2 * blockIdx.x] = red[0][0];
-    en_part[2 * blockIdx.x + 1] = red[1][0];
+    en_part[2 * blockIdx.x] = e;
+    en_part[2 * blockIdx.x + 1] = n;
   }
 }
 
@@ -352,6 +372,63 @@ void launch_functional(hipStream_t s, int nbatches, const int* grp_off, const do
   const int nslots = nbatches * kBatch;
   hipLaunchKernelGGL(k_functional, dim3(en_blocks), dim3(256), 0, s, nslots, grp_off, part, part_tau, bw, codes, ncodes,
                      family, want_e, want_v, rho_out, coef, en_part);
+  CHX_HIP(hipGetLastError());
+}
+
+// The functional of several molecules in one launch (packs): block g belongs
+// to molecule m with fm[m].blk0 <= g < fm[m + 1].blk0 and runs that molecule's
+// block g - blk0 exactly as k_functional does. Each molecule's buffers are
+// addressed as base + byte offset, base a kernel argument, so that every access
+// compiles to a global (not flat) memory instruction. Then one block per
+// molecule reduces its block partial sums into en[2 m], en[2 m + 1].
+__global__ __launch_bounds__(256) void k_functional_multi(const char* __restrict__ base,
+                                                          const FnMolecule* __restrict__ fm, int nmol,
+                                                          const int* __restrict__ codes, int ncodes, int family) {
+  const int gb = blockIdx.x;
+  int lo = 0, hi = nmol;  // the last molecule whose blk0 <= gb
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (fm[mid].blk0 <= gb) lo = mid; else hi = mid;
+  }
+  const FnMolecule m = fm[lo];
+  const int blk = gb - m.blk0;
+  double e, n;
+  functional_slot(blk * 256 + threadIdx.x, m.nslots, reinterpret_cast<const int*>(base + m.grp_off),
+                  reinterpret_cast<const double*>(base + m.part), reinterpret_cast<const double*>(base + m.part_tau),
+                  reinterpret_cast<const double*>(base + m.w), codes, ncodes, family, 1, 1,
+                  reinterpret_cast<double*>(const_cast<char*>(base) + m.rho),
+                  reinterpret_cast<double*>(const_cast<char*>(base) + m.coef), e, n);
+  block_sum2(e, n);
+  if (threadIdx.x == 0) {
+    double* en_part = reinterpret_cast<double*>(const_cast<char*>(base) + m.en_part);
+    en_part[2 * blk] = e;
+    en_part[2 * blk + 1] = n;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_en_multi(const char* __restrict__ base,
+                                                         const FnMolecule* __restrict__ fm, double* __restrict__ en) {
+  const FnMolecule m = fm[blockIdx.x];
+  const double* en_part = reinterpret_cast<const double*>(base + m.en_part);
+  const int nparts = fm[blockIdx.x + 1].blk0 - m.blk0;
+  double e = 0, n = 0;
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    e += en_part[2 * i];
+    n += en_part[2 * i + 1];
+  }
+  block_sum2(e, n);
+  if (threadIdx.x == 0) {
+    en[2 * m.slot_en] = e;
+    en[2 * m.slot_en + 1] = n;
+  }
+}
+
+void launch_functional_multi(hipStream_t s, const char* base, const FnMolecule* fm, int nmol, int nblocks,
+                             const int* codes, int ncodes, int family, double* en) {
+  if (nmol == 0) return;
+  hipLaunchKernelGGL(k_functional_multi, dim3(nblocks), dim3(256), 0, s, base, fm, nmol, codes, ncodes, family);
+  CHX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_en_multi, dim3(nmol), dim3(256), 0, s, base, fm, en);
   CHX_HIP(hipGetLastError());
 }
 
@@ -372,7 +449,6 @@ __global__ __launch_bounds__(256) void k_functional_pol(int nslots, const int* _
                                                         int ncodes, int family, int want_e, int want_v,
                                                         double* __restrict__ rho_out, double* __restrict__ coef,
                                                         double* __restrict__ coef_b, double* __restrict__ en_part) {
-  __shared__ double red[2][256];
   const int slot = blockIdx.x * 256 + threadIdx.x;
   double e_acc = 0, n_acc = 0;
   if (slot < nslots) {
@@ -433,19 +509,10 @@ __global__ __launch_bounds__(256) void k_functional_pol(int nslots, const int* _
     if (want_e) e_acc = w * (ra + rb) * eps;
     n_acc = w * (ra + rb);
   }
-  red[0][threadIdx.x] = e_acc;
-  red[1][threadIdx.x] = n_acc;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + st];
-      red[1][threadIdx.x] += red[1][threadIdx.x + st];
-    }
-    __syncthreads();
-  }
+  block_sum2(e_acc, n_acc);
   if (threadIdx.x == 0) {
-    en_part[2 * blockIdx.x] = red[0][0];
-    en_part[2 * blockIdx.x + 1] = red[1][0];
+    en_part[2 * blockIdx.x] = e_acc;
+    en_part[2 * blockIdx.x + 1] = n_acc;
   }
 }
 
@@ -644,25 +711,15 @@ void launch_eval_functional_f(hipStream_t s, long n, const int* codes, int ncode
 
 // Deterministic final sum of the block partials (one block).
 __global__ __launch_bounds__(256) void k_reduce_en(int nparts, const double* __restrict__ en_part, double* __restrict__ en) {
-  __shared__ double red[2][256];
   double e = 0, n = 0;
   for (int i = threadIdx.x; i < nparts; i += 256) {
     e += en_part[2 * i];
     n += en_part[2 * i + 1];
   }
-  red[0][threadIdx.x] = e;
-  red[1][threadIdx.x] = n;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + st];
-      red[1][threadIdx.x] += red[1][threadIdx.x + st];
-    }
-    __syncthreads();
-  }
+  block_sum2(e, n);
   if (threadIdx.x == 0) {
-    en[0] = red[0][0];
-    en[1] = red[1][0];
+    en[0] = e;
+    en[1] = n;
   }
 }
 

@@ -16,6 +16,7 @@
 //
 // No reference counterpart: Chinium runs one molecule per process
 // (src/HartreeFockKohnSham/SelfConsistentField.cpp:27-33).
+#include <array>
 #include <algorithm>
 #include <chrono>
 #include <string>
@@ -51,6 +52,12 @@ struct chx_pack {
   std::vector<hipStream_t> lane_st;
   std::vector<hipEvent_t> ev_dens, ev_done;
   hipEvent_t ev_fork = nullptr;
+  // per lane: the multi-molecule functional launch's table (molecules of the
+  // lane, then a sentinel) and base pointer, and its total block count
+  DevBuf<FnMolecule> fn_tab;  // every lane's table, back to back from fn_off[lane]
+  std::vector<int> fn_off;
+  std::vector<const char*> fn_base;
+  std::vector<int> fn_blocks;
   ~chx_pack() {
     drop_graph();
     free_lanes();
@@ -113,6 +120,48 @@ void build_lanes(chx_pack* p) {
   p->free_lanes();
   p->gen.resize(n);
   for (int m = 0; m < n; ++m) p->gen[m] = p->grids[m]->plan_gen;
+  // the functional of each lane's molecules in one launch: their buffers as byte
+  // offsets from the lowest address among them
+  std::vector<FnMolecule> all;
+  p->fn_off.assign(L, 0);
+  p->fn_base.assign(L, nullptr);
+  p->fn_blocks.assign(L, 0);
+  for (int k = 0; k < L; ++k) {
+    const int m0 = p->cut[k], m1 = p->cut[k + 1];
+    auto ptrs = [&](const chx_grid* g) {
+      return std::array<const char*, 7>{(const char*)g->d_grp_off.p, (const char*)g->d_part.p,
+                                        (const char*)g->d_part_tau.p, (const char*)g->d_w.p,
+                                        (const char*)g->d_rho.p, (const char*)g->d_coef.p,
+                                        (const char*)g->d_en_part.p};
+    };
+    const char* base = nullptr;
+    for (int m = m0; m < m1; ++m)
+      for (const char* q : ptrs(p->grids[m]))
+        if (!base || q < base) base = q;
+    std::vector<FnMolecule> tab;
+    int blk = 0;
+    for (int m = m0; m < m1; ++m) {
+      const chx_grid* g = p->grids[m];
+      const auto q = ptrs(g);
+      FnMolecule f{};
+      f.grp_off = q[0] - base; f.part = q[1] - base; f.part_tau = q[2] - base; f.w = q[3] - base;
+      f.rho = q[4] - base; f.coef = q[5] - base; f.en_part = q[6] - base;
+      f.nslots = g->nbatches * kBatch;
+      f.blk0 = blk;
+      f.slot_en = m;
+      blk += g->en_blocks;
+      tab.push_back(f);
+    }
+    FnMolecule end{};
+    end.blk0 = blk;
+    tab.push_back(end);
+    p->fn_off[k] = (int)all.size();
+    all.insert(all.end(), tab.begin(), tab.end());
+    p->fn_base[k] = base;
+    p->fn_blocks[k] = blk;
+  }
+  p->fn_tab.upload(all, p->stream);
+  CHX_HIP(hipStreamSynchronize(p->stream));  // the table's host vector goes out of scope
   if (L > 1) {
     p->lane_st.resize(L);
     p->ev_dens.resize(L);
@@ -126,12 +175,14 @@ void build_lanes(chx_pack* p) {
   }
 }
 
-// Molecules [m0, m1) on stream s: split D and clear M first; the density
-// launches wait for `after` (if set) and signal `dens_done` (if set); then per
-// molecule the functional, E/N sums, B^T, Vxc and V = M + M^T. The kernels and
-// their arguments are those of the single-grid build.
-void enqueue_lane(chx_pack* p, int m0, int m1, hipStream_t s, hipEvent_t after, hipEvent_t dens_done,
+// The molecules of a lane on stream s: split D and clear M first; the density
+// launches wait for `after` (if set) and signal `dens_done` (if set); then the
+// functional and E/N sums of all the lane's molecules in one launch each, and
+// per molecule B^T, Vxc and V = M + M^T. Apart from the functional, the kernels
+// and their arguments are those of the single-grid build.
+void enqueue_lane(chx_pack* p, int lane, hipStream_t s, hipEvent_t after, hipEvent_t dens_done,
                   const double* const* D, double* const* V) {
+  const int m0 = p->cut[lane], m1 = p->cut[lane + 1];
   for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
     launch_split_d(s, g->nbasis, g->npad, D[m], g->d_int2ref.p, g->d_Dsplit.p,
@@ -148,11 +199,11 @@ void enqueue_lane(chx_pack* p, int m0, int m1, hipStream_t s, hipEvent_t after, 
   }
   if (dens_done) CHX_HIP(hipEventRecord(dens_done, s));
   const int nc = p->family >= 2 ? 4 : 1;
+  // the functional and the E/N sums of all the lane's molecules: one launch each
+  launch_functional_multi(s, p->fn_base[lane], p->fn_tab.p + p->fn_off[lane], m1 - m0, p->fn_blocks[lane], p->d_codes.p,
+                          (int)p->codes.size(), p->family, p->d_en.p);
   for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
-    launch_functional(s, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_tau.p, g->d_w.p, p->d_codes.p,
-                      (int)p->codes.size(), p->family, 1, 1, g->d_rho.p, g->d_coef.p, g->d_en_part.p, g->en_blocks);
-    launch_reduce_en(s, g->en_blocks, g->d_en_part.p, p->d_en.p + 2 * m);
     launch_make_b_mx(s, nc, g->hmol);
     if (g->mx_wide)
       launch_vxc_mx_wide(s, nc, (int)g->cwork.size(), g->hmol);
@@ -167,13 +218,13 @@ void enqueue_lane(chx_pack* p, int m0, int m1, hipStream_t s, hipEvent_t after, 
 void enqueue(chx_pack* p, const double* const* D, double* const* V) {
   const int L = (int)p->cut.size() - 1;
   if (L == 1) {
-    enqueue_lane(p, 0, (int)p->grids.size(), p->stream, nullptr, nullptr, D, V);
+    enqueue_lane(p, 0, p->stream, nullptr, nullptr, D, V);
     return;
   }
   CHX_HIP(hipEventRecord(p->ev_fork, p->stream));
   for (int k = 0; k < L; ++k) {
     CHX_HIP(hipStreamWaitEvent(p->lane_st[k], p->ev_fork, 0));
-    enqueue_lane(p, p->cut[k], p->cut[k + 1], p->lane_st[k], k ? p->ev_dens[k - 1] : nullptr, p->ev_dens[k], D, V);
+    enqueue_lane(p, k, p->lane_st[k], k ? p->ev_dens[k - 1] : nullptr, p->ev_dens[k], D, V);
     CHX_HIP(hipEventRecord(p->ev_done[k], p->lane_st[k]));
   }
   for (int k = 0; k < L; ++k) CHX_HIP(hipStreamWaitEvent(p->stream, p->ev_done[k], 0));
