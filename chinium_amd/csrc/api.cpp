@@ -94,6 +94,8 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
   g->d_codes.upload(xc->codes, g->stream);
   hipEvent_t e = nullptr;
   g->surrogate_used = false;
+  g->vout_ready = false;
+  g->corr_pending = g->precision == CHX_PREC_MIXED && want_e && want_v;
   if (g->nspin == 2) {  // spin-polarised: alpha coefficients in d_coef, beta in d_coef_b
     if (xc->family > g->type)
       throw Error(CHX_ERR_STATE, "the grid's type holds no tau for this meta-GGA functional (Grid::setType)");
@@ -135,21 +137,24 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
 // Stage ids of chx_grid_kernel_times.
 enum { kStPermute = 0, kStDensity = 1, kStFunctional = 2, kStVxc = 3, kStFinalize = 4, kStMakeB = 5, kStCore = 6 };
 
-void density_from_device(chx_grid* g, const double* dD) {
+// spin: which of the (up to two) split remainders of D this density's is
+void density_from_device(chx_grid* g, const double* dD, int spin = 0) {
   hipEvent_t e = nullptr;
   const int ng = (int)g->grp_batch.size();
+  g->corr_pending = g->vout_ready = false;
   if (g->precision == CHX_PREC_MIXED) {
     g->mark_begin(kStPermute, &e);
     const int ncg = (int)g->core_grps.size();
-    launch_split_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dsplit.p, ncg ? g->d_Ds3.p : nullptr);
+    launch_split_d(g->stream, g->nbasis, g->npad, dD, g->d_int2ref.p, g->d_Dsplit.p,
+                   g->d_Dres.p + (size_t)spin * g->nbasis * g->nbasis);
     g->mark_end(kStPermute, e);
     g->mark_begin(kStDensity, &e);
     // meta-GGA: rho, grad rho and the tau partials (A = d_r phi) in one launch that stages D once
     // (rho, grad rho alone: the core batches' groups are left to the core launch)
     launch_density_mx(g->stream, g->type >= 2 ? 5 : 0, g->type >= 2 ? ng : ng - ncg, g->hmol);
     g->mark_end(kStDensity, e);
-    // core batches' rho and grad rho with three-term operands (meta-GGA: overwriting their partials;
-    // tau keeps two terms: near the nuclei its error is far below rho's effect on E_xc)
+    // core batches' rho and grad rho with three-term AO operands (meta-GGA: overwriting their
+    // partials; tau keeps two terms: near the nuclei its error is far below rho's effect on E_xc)
     if (ncg) {
       g->mark_begin(kStCore, &e);
       launch_density_mx_core(g->stream, 0, ncg, g->hmol);
@@ -187,7 +192,9 @@ void density_from_device(chx_grid* g, const double* dD) {
   g->have_v = false;
 }
 
-void fock_to_device(chx_grid* g, double* dV) {
+// spin: the potential's spin index, whose split remainder of D enters the
+// energy correction (mixed form, when the last Evaluate computed E and v)
+void fock_to_device(chx_grid* g, double* dV, int spin = 0) {
   hipEvent_t e = nullptr;
   if (g->precision == CHX_PREC_MIXED) {
     const int nc = g->v_family >= 2 ? 4 : 1;
@@ -203,7 +210,12 @@ void fock_to_device(chx_grid* g, double* dV) {
       launch_vxc_mx(g->stream, nc, (int)g->cwork.size(), g->hmol);
     g->mark_end(kStVxc, e);
     g->mark_begin(kStFinalize, &e);
-    launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);
+    if (g->corr_pending)
+      launch_finalize_corr(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV,
+                           g->d_Dres.p + (size_t)spin * g->nbasis * g->nbasis,
+                           g->d_cpart.p + (size_t)spin * corr_blocks(g->nbasis));
+    else
+      launch_finalize_sym(g->stream, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, dV);
     g->mark_end(kStFinalize, e);
     return;
   }
@@ -244,7 +256,7 @@ void check_dm_args(const chx_grid* g, int nspin, int nbasis, int max_spin = 1) {
 // per spin index).
 void density_spin(chx_grid* g, const double* dD, int nspin) {
   if (nspin == 2) {
-    density_from_device(g, dD + (size_t)g->nbasis * g->nbasis);
+    density_from_device(g, dD + (size_t)g->nbasis * g->nbasis, 1);
     g->d_part_b.alloc(g->d_part.n);
     CHX_HIP(hipMemcpyAsync(g->d_part_b.p, g->d_part.p, sizeof(double) * g->d_part.n, hipMemcpyDeviceToDevice,
                            g->stream));
@@ -259,18 +271,32 @@ void density_spin(chx_grid* g, const double* dD, int nspin) {
 }
 
 // Vxc of each spin: alpha from d_coef, then beta with d_coef_b swapped in
-// (the alpha coefficients are restored afterwards).
+// (the alpha coefficients are restored afterwards). Mixed form after an
+// energy evaluation: the energy's correction for the split of D,
+// sum_s V_s . (D_s - hi - lo), into d_en[2].
 void fock_spin(chx_grid* g, double* dV, int nspin) {
   require(nspin == g->v_nspin, CHX_ERR_ARG, "Fock spin count differs from the evaluated density's");
-  fock_to_device(g, dV);
+  fock_to_device(g, dV, 0);
   if (nspin == 2) {
     const size_t nc = g->d_coef.n, nb2 = (size_t)g->nbasis * g->nbasis;
     g->d_coef_save.alloc(nc);
     CHX_HIP(hipMemcpyAsync(g->d_coef_save.p, g->d_coef.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, g->stream));
     CHX_HIP(hipMemcpyAsync(g->d_coef.p, g->d_coef_b.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, g->stream));
-    fock_to_device(g, dV + nb2);
+    fock_to_device(g, dV + nb2, 1);
     CHX_HIP(hipMemcpyAsync(g->d_coef.p, g->d_coef_save.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, g->stream));
   }
+  if (g->precision == CHX_PREC_MIXED && g->corr_pending)
+    launch_reduce_corr(g->stream, nspin * corr_blocks(g->nbasis), g->d_cpart.p, g->d_en.p + 2);
+}
+
+// The potential of the last Evaluate into d_Vout (mixed form, before the
+// energy is read: the energy's correction needs it); later Fock calls copy it.
+void build_vout(chx_grid* g) {
+  const size_t n = (size_t)g->v_nspin * g->nbasis * g->nbasis;
+  if (g->d_Vout.n < n) g->d_Vout.alloc(n);
+  fock_spin(g, g->d_Vout.p, g->v_nspin);
+  g->vout_ready = true;
+  g->vout_nspin = g->v_nspin;
 }
 }  // namespace
 
@@ -512,6 +538,7 @@ int chx_grid_get_density_u(chx_grid* g, const double* D, int nmats, int nspin, i
 // on them (B = a phi + c . grad phi, M = Phi^T B, V = M + M^T), V to F (host).
 static void response_fock(chx_grid* g, const double* rhou, int nmats, double* F) {
   const size_t nslots = (size_t)g->nbatches * kBatch, nb2 = (size_t)g->nbasis * g->nbasis;
+  g->vout_ready = false;  // d_Vout stages the response matrices
   double* coef = g->d_coef_u.alloc_get(std::max<size_t>(kCoef, kCoef * nslots));
   for (int m = 0; m < nmats; ++m) {
     launch_response_coef(g->stream, g->nbatches, rhou + (size_t)m * 4 * nslots, g->d_fx.p, g->d_w.p, g->fx_family,
@@ -738,15 +765,20 @@ int chx_grid_num_electrons(chx_grid* g, double* N) {
   });
 }
 
+// Mixed form: E carries the first-order correction for the split of D
+// (sum_s V_s . (D_s - hi - lo)), which needs the potential; if Evaluate("ev")
+// was not yet followed by a Fock call, the potential is built here (into the
+// grid's own buffer, which the next Fock call returns without rebuilding).
 int chx_grid_energy(chx_grid* g, double* E) {
   return guard([&] {
     require(g->have_eval, CHX_ERR_STATE, "ExchangeCorrelation::Evaluate must precede Grid::getEnergy");
-    double en[2];
+    if (g->corr_pending && g->have_v && !g->vout_ready) build_vout(g);
+    double en[3];
     CHX_HIP(hipMemcpyAsync(en, g->d_en.p, sizeof(en), hipMemcpyDeviceToHost, g->stream));
     CHX_HIP(hipStreamSynchronize(g->stream));
-    g->last_E = en[0];
+    g->last_E = en[0] + en[2];
     g->last_N = en[1];
-    *E = en[0];
+    *E = en[0] + en[2];
   });
 }
 
@@ -754,9 +786,9 @@ int chx_grid_fock(chx_grid* g, double* F, int nspin, int nbasis) {
   return guard([&] {
     require(g->have_v, CHX_ERR_STATE, "no potential on the grid (Evaluate with \"v\" or \"ev\" first)");
     require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
+    require(nspin == g->v_nspin, CHX_ERR_ARG, "Fock spin count differs from the evaluated density's");
     const size_t n = (size_t)nspin * nbasis * nbasis;
-    if (g->d_Vout.n < n) g->d_Vout.alloc(n);
-    fock_spin(g, g->d_Vout.p, nspin);
+    if (!g->vout_ready) build_vout(g);
     CHX_HIP(hipMemcpyAsync(F, g->d_Vout.p, sizeof(double) * n, hipMemcpyDeviceToHost, g->stream));
     CHX_HIP(hipStreamSynchronize(g->stream));
   });
@@ -766,7 +798,12 @@ int chx_grid_fock_device(chx_grid* g, double* dF, int nspin, int nbasis) {
   return guard([&] {
     require(g->have_v, CHX_ERR_STATE, "no potential on the grid (Evaluate with \"v\" or \"ev\" first)");
     require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
-    fock_spin(g, dF, nspin);
+    require(nspin == g->v_nspin, CHX_ERR_ARG, "Fock spin count differs from the evaluated density's");
+    if (g->vout_ready)
+      CHX_HIP(hipMemcpyAsync(dF, g->d_Vout.p, sizeof(double) * (size_t)nspin * nbasis * nbasis,
+                             hipMemcpyDeviceToDevice, g->stream));
+    else
+      fock_spin(g, dF, nspin);
   });
 }
 
@@ -785,6 +822,7 @@ int chx_grid_set_vrho(chx_grid* g, const double* vrho, long npoints) {
     d.upload(vw, g->stream);
     launch_vrho_coef(g->stream, (long)vw.size(), d.p, g->d_coef.p);
     CHX_HIP(hipStreamSynchronize(g->stream));
+    g->corr_pending = g->vout_ready = false;  // a potential of its own: no energy correction
     g->have_v = true;
     g->v_family = 0;
     g->v_nspin = 1;

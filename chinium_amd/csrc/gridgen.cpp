@@ -700,7 +700,6 @@ void grid_plan(chx_grid* g) {
       if (g->core_grps.empty()) g->main_grps.clear();  // no core batches: the launch takes every group
       g->d_main_grps.upload(g->main_grps.empty() ? zero1 : g->main_grps, st);
       g->d_cmx3.alloc(std::max<size_t>(1, (size_t)ncore_slots * nT * kBatch * kTile));
-      g->d_Ds3.alloc(ncore_slots ? (size_t)g->npad * g->npad : 1);
       launch_ao_cache_mx(st, (int)nslot, g->d_slot_batch.p, g->d_slot_tile.p, g->d_tile_nao.p, g->d_x.p, g->d_y.p,
                          g->d_z.p, g->view(), nT, g->mx_sstride, g->d_cmx.p, ncore_slots ? g->d_slot_core.p : nullptr,
                          g->d_cmx3.p);
@@ -709,6 +708,8 @@ void grid_plan(chx_grid* g) {
       g->d_cmx.release();
       g->d_bmx.release();
       g->d_Dsplit.release();
+      g->d_Dres.release();
+      g->d_cpart.release();
       g->mx_nT = g->mx_sstride = g->bmx_comps = 0;
       g->d_cache.alloc(std::max<size_t>(1, nslot * (size_t)4 * kBatch * kTile));
       g->d_bmat.alloc(std::max<size_t>(1, nslot * (size_t)kBatch * kTile));
@@ -722,6 +723,8 @@ void grid_plan(chx_grid* g) {
     g->d_cmx.release();
     g->d_bmx.release();
     g->d_Dsplit.release();
+    g->d_Dres.release();
+    g->d_cpart.release();
   }
   g->d_bt_off.upload(g->bt_off, st); g->d_bt_list.upload(g->bt_list, st);
   g->d_grp_batch.upload(g->grp_batch, st); g->d_grp_tiles.upload(g->grp_tiles, st);
@@ -739,7 +742,12 @@ void grid_plan(chx_grid* g) {
   g->d_part_tau.alloc(g->type >= 2 ? std::max<size_t>(1, g->grp_batch.size() * 3 * kBatch) : 1);
   g->en_blocks = (int)std::max<size_t>(1, (nslots + 255) / 256);
   g->d_en_part.alloc(2 * (size_t)g->en_blocks);
-  g->d_en.alloc(2);
+  g->d_en.alloc(3);
+  launch_fill(st, g->d_en.p, 3, 0.0);
+  if (mixed) {  // split remainder of D per spin, and the partial sums of V . remainder
+    g->d_Dres.alloc(2 * (size_t)g->nbasis * g->nbasis);
+    g->d_cpart.alloc(2 * (size_t)corr_blocks(g->nbasis));
+  }
   if (mixed) {  // descriptor of this grid for the mixed kernels
     MolDev& m = g->hmol;
     m = MolDev{};
@@ -755,7 +763,6 @@ void grid_plan(chx_grid* g) {
     m.slot_of = g->d_slot_of.p;
     m.Mint = g->d_Mint.p;
     m.cache3 = g->d_cmx3.p;
-    m.Ds3 = g->d_Ds3.p;
     m.core_off = g->d_core_off.p;
     m.core_grps = g->d_core_grps.p;
     m.main_grps = g->core_grps.empty() ? nullptr : g->d_main_grps.p;
@@ -765,6 +772,7 @@ void grid_plan(chx_grid* g) {
   g->ao_ready = true;
   ++g->plan_gen;
   g->have_density = g->have_eval = g->have_v = g->have_skeleton = false;
+  g->corr_pending = g->vout_ready = false;
   g->slot_grp.clear();
   g->slot_q.clear();
 }

@@ -129,8 +129,8 @@ void launch_density_f64(hipStream_t s, int mode, int ngroups, const int* grp_bat
 // buffers from the launch's base pointer, its slot count, its first block
 // (entry nmol is a sentinel whose blk0 is the total block count) and its E/N slot.
 struct FnMolecule {
-  long grp_off, part, part_tau, w, rho, coef, en_part;
-  int nslots, blk0, slot_en, pad;
+  long grp_off, part, part_tau, w, rho, coef, en_part, cpart;
+  int nslots, blk0, slot_en, ncpart;
 };
 void launch_functional_multi(hipStream_t s, const char* base, const FnMolecule* fm, int nmol, int nblocks,
                              const int* codes, int ncodes, int family, double* en);
@@ -241,12 +241,12 @@ struct MolDev {
   const int* vtab;
   const int* vtile;
   // core batches (points within the core radius of a nucleus with Z > 2): their
-  // density runs with operands split in three bf16 terms; cache3 holds the
-  // third term of their slots' AOg blocks [core slot][c][g][mu], Ds3 that of D,
+  // density runs with the AO operands split in three bf16 terms; cache3 holds
+  // the third term of their slots' AOg blocks [core slot][c][g][mu],
   // core_off the first core slot of each batch (-1: not a core batch),
   // core_grps the density groups of the core batches, main_grps those of the
   // other batches (the (rho, grad rho) launch's groups; null: every group)
-  const unsigned short *cache3, *Ds3;
+  const unsigned short* cache3;
   const int *core_off, *core_grps, *main_grps;
 };
 constexpr int kMxMaxTiles = 512;  // alive tiles per batch the mixed density kernel stages (npad <= 16384)
@@ -274,9 +274,19 @@ constexpr int kMxComp = 2 * kTile * kBatch;  // elements per component and layou
 void launch_ao_cache_mx(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* tile_nao,
                         const double* bx, const double* by, const double* bz, BasisView bv, int nT, int sstride,
                         unsigned short* cache, const int* slot_core, unsigned short* cache3);
-// Ds3 (may be null): the third bf16 term of D, D - hi - lo
+// R (may be null): what the two terms leave of D, D - hi - lo, in FP32 and in
+// D's own (reference, column-major) order
 void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds,
-                    unsigned short* Ds3);
+                    float* R);
+// First-order energy correction for the split of D: V = M + M^T as
+// launch_finalize_sym, and per block the partial sum of V . R into cpart
+// (corr_blocks(n) blocks); launch_reduce_corr sums nparts partials into *out.
+int corr_blocks(int n);
+void launch_finalize_corr(hipStream_t s, int n, int npad, const double* Mint, const int* ref2int, double* V,
+                          const float* R, double* cpart);
+void launch_reduce_corr(hipStream_t s, int nparts, const double* cpart, double* out);
+// the packed form: per molecule of the table, en[2 slot_en] += its ncpart partials
+void launch_corr_multi(hipStream_t s, const char* base, const FnMolecule* fm, int nmol, double* en);
 void launch_bf16_check(hipStream_t s, unsigned n, unsigned* bad);
 void launch_fill_u32(hipStream_t s, unsigned* p, size_t n, unsigned v);
 // One launch over the grid `one` (descriptor by value): density work groups,
@@ -387,7 +397,7 @@ struct chx_grid {
   double core_radius = chx::kCoreRadius;  // bohr; 0 turns the core treatment off
   int ncore_batches = 0;
   std::vector<int> core_grps, main_grps;
-  chx::DevBuf<unsigned short> d_cmx3, d_Ds3;
+  chx::DevBuf<unsigned short> d_cmx3;
   chx::DevBuf<int> d_core_off, d_core_grps, d_main_grps, d_slot_core;
   chx::DevBuf<double> d_cache, d_bmat, d_Mint;
   chx::DevBuf<int> d_slot_batch, d_slot_tile, d_slot_of, d_tile_nao, d_grp_slots, d_cpair_batches;
@@ -416,7 +426,15 @@ struct chx_grid {
   chx::DevBuf<int> d_codes;
 
   // Per-iteration state.
+  // d_en: E, N and (mixed form) the first-order correction for the split of D
   chx::DevBuf<double> d_Din, d_Dint, d_part, d_part_tau, d_rho, d_coef, d_en_part, d_en, d_Vint, d_Vout;
+  // mixed form: D - hi - lo per spin (FP32, reference order) and the partial sums of V . (D - hi - lo)
+  chx::DevBuf<float> d_Dres;
+  chx::DevBuf<double> d_cpart;
+  // corr_pending: the last Evaluate computed E and v in the mixed form, so the potential's build
+  // adds the correction to E; vout_ready: d_Vout holds that potential (vout_nspin matrices)
+  bool corr_pending = false, vout_ready = false;
+  int vout_nspin = 0;
   int v_family = 0;    // family of the potential on the grid (set by Evaluate / set_vrho)
   int en_blocks = 0;
   // XC gradient (Grid::getDensitySkeleton / getEnergyGrad)

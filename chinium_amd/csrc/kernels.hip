@@ -709,7 +709,9 @@ void launch_eval_functional_f(hipStream_t s, long n, const int* codes, int ncode
   CHX_HIP(hipGetLastError());
 }
 
-// Deterministic final sum of the block partials (one block).
+// Deterministic final sum of the block partials (one block): E, N, and a
+// cleared slot en[2] for the mixed form's energy correction, which the
+// potential's build fills.
 __global__ __launch_bounds__(256) void k_reduce_en(int nparts, const double* __restrict__ en_part, double* __restrict__ en) {
   double e = 0, n = 0;
   for (int i = threadIdx.x; i < nparts; i += 256) {
@@ -720,6 +722,7 @@ __global__ __launch_bounds__(256) void k_reduce_en(int nparts, const double* __r
   if (threadIdx.x == 0) {
     en[0] = e;
     en[1] = n;
+    en[2] = 0.0;
   }
 }
 

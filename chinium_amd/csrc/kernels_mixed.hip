@@ -169,26 +169,101 @@ void launch_ao_cache_mx(hipStream_t s, int nslots, const int* slot_batch, const 
 
 // ---------------------------------------------------------------------------
 // D (reference order, column-major) -> internal order, split: Ds[hl][mu][nu].
+// R (if set): what the split leaves of D, D - hi - lo (its FP32 rounding
+// included), at D's own position. The products see hi + lo; sum V . R, with V
+// the potential of this density, is the first-order energy of the remainder
+// (dE/dD = V), which the energy takes back when the potential is built
+// (k_finalize_corr).
 __global__ void k_split_d(int n, int npad, const double* __restrict__ D, const int* __restrict__ int2ref,
-                          u16* __restrict__ Ds, u16* __restrict__ Ds3) {
+                          u16* __restrict__ Ds, float* __restrict__ R) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long np2 = (long)npad * npad;
   if (idx >= np2) return;
   const int r = (int)(idx / npad), c = (int)(idx % npad);
   const int rr = int2ref[r], cc = int2ref[c];
-  const double v = (rr >= 0 && cc >= 0) ? D[(size_t)cc * n + rr] : 0.0;
+  const bool real = rr >= 0 && cc >= 0;
+  const double v = real ? D[(size_t)cc * n + rr] : 0.0;
   u16 hi, lo;
   const float x = (float)v;
   split_bf(x, hi, lo);
   Ds[idx] = hi;
   Ds[np2 + idx] = lo;
-  if (Ds3) Ds3[idx] = f2bf((x - bf2f(hi)) - bf2f(lo));
+  if (R && real) R[(size_t)cc * n + rr] = (float)(v - (double)bf2f(hi) - (double)bf2f(lo));
 }
 
 void launch_split_d(hipStream_t s, int n, int npad, const double* D, const int* int2ref, unsigned short* Ds,
-                    unsigned short* Ds3) {
+                    float* R) {
   const long tot = (long)npad * npad;
-  hipLaunchKernelGGL(k_split_d, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, npad, D, int2ref, Ds, Ds3);
+  hipLaunchKernelGGL(k_split_d, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, npad, D, int2ref, Ds, R);
+  CHX_HIP(hipGetLastError());
+}
+
+// V = M + M^T (reference order, as k_finalize_sym) and the block's partial sum
+// of V . R, the split remainder of D: grid-stride over the n^2 elements with a
+// fixed block count, so the partials have one fixed order of summation.
+constexpr int kCorrBlocks = 1024;
+int corr_blocks(int n) { return (int)std::min<long>(kCorrBlocks, ((long)n * n + 255) / 256); }
+
+__device__ __forceinline__ double block_sum_d(double v) {
+  __shared__ double red[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void k_finalize_corr(int n, int npad, const double* __restrict__ Mint,
+                                                       const int* __restrict__ ref2int, double* __restrict__ V,
+                                                       const float* __restrict__ R, double* __restrict__ cpart) {
+  const long tot = (long)n * n;
+  double acc = 0.0;
+  for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < tot; idx += (long)gridDim.x * 256) {
+    const int col = (int)(idx / n), row = (int)(idx % n);
+    const int mu = ref2int[row], nu = ref2int[col];
+    const double v = Mint[(size_t)mu * npad + nu] + Mint[(size_t)nu * npad + mu];
+    V[idx] = v;
+    acc += v * (double)R[idx];
+  }
+  acc = block_sum_d(acc);
+  if (threadIdx.x == 0) cpart[blockIdx.x] = acc;
+}
+
+void launch_finalize_corr(hipStream_t s, int n, int npad, const double* Mint, const int* ref2int, double* V,
+                          const float* R, double* cpart) {
+  hipLaunchKernelGGL(k_finalize_corr, dim3((unsigned)corr_blocks(n)), dim3(256), 0, s, n, npad, Mint, ref2int, V, R,
+                     cpart);
+  CHX_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void k_reduce_corr(int nparts, const double* __restrict__ cpart,
+                                                     double* __restrict__ out) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) acc += cpart[i];
+  acc = block_sum_d(acc);
+  if (threadIdx.x == 0) *out = acc;
+}
+
+void launch_reduce_corr(hipStream_t s, int nparts, const double* cpart, double* out) {
+  hipLaunchKernelGGL(k_reduce_corr, dim3(1), dim3(256), 0, s, nparts, cpart, out);
+  CHX_HIP(hipGetLastError());
+}
+
+// One block per molecule of a pack lane: its partials (at base + cpart) added
+// to its energy, after the functional's sums have written it.
+__global__ __launch_bounds__(256) void k_corr_multi(const char* __restrict__ base, const FnMolecule* __restrict__ fm,
+                                                    double* __restrict__ en) {
+  const FnMolecule m = fm[blockIdx.x];
+  const double* cpart = reinterpret_cast<const double*>(base + m.cpart);
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < m.ncpart; i += 256) acc += cpart[i];
+  acc = block_sum_d(acc);
+  if (threadIdx.x == 0) en[2 * m.slot_en] += acc;
+}
+
+void launch_corr_multi(hipStream_t s, const char* base, const FnMolecule* fm, int nmol, double* en) {
+  if (nmol <= 0) return;
+  hipLaunchKernelGGL(k_corr_multi, dim3((unsigned)nmol), dim3(256), 0, s, base, fm, en);
   CHX_HIP(hipGetLastError());
 }
 
@@ -380,11 +455,10 @@ constexpr int kDxFrag = 64;  // u32x4 per fragment
 constexpr int kDxA = 8;      // A fragments per step: [row block][hl]
 constexpr int kDxB = 16;     // D fragments per step: [column block][hl]
 constexpr int kDxA3 = 4;     // P3: third-term A fragments per step: [row block]
-constexpr int kDxB3 = 8;     // P3: third-term D fragments per step: [column block]
 template <bool P3>
 struct DensMxSmem {
   union {
-    u32x4 F[(kDxA + kDxB + (P3 ? kDxA3 + kDxB3 : 0)) * kDxFrag];  // one K step's fragments
+    u32x4 F[(kDxA + kDxB + (P3 ? kDxA3 : 0)) * kDxFrag];  // one K step's fragments
     float X[kGroup * kTile][kBatch + 1];                          // epilogue: X^T [nu][g]
   };
   float red[kGroup][4][kBatch];
@@ -395,10 +469,12 @@ struct DensMxSmem {
 // MODE 0: A = phi; epilogue rho and grad rho / 2 (4 components into part).
 // MODE r = 1..3 (A = d phi / d x_r, epilogue sum X_r d_r phi, the tau partial
 // of direction r) is the single-direction form of k_density_mx_tau below.
-// P3 (core batches only): each operand carries a third bf16 term (x - hi - lo,
-// 24 significant bits in all), A's from the core slots' side cache and D's
-// from Ds3, and a product takes the six terms hi hi, hi lo, lo hi, hi lo2,
-// lo2 hi and lo lo: FP32-level operands where the density is largest.
+// P3 (core batches only): the AO operand carries a third bf16 term (x - hi -
+// lo, 24 significant bits in all) from the core slots' side cache, and a
+// product takes the five terms hi hi, lo hi, hi lo, lo2 hi and lo lo: the AO
+// values at FP32 level where the density is largest, and D as hi + lo exactly
+// (its remainder D - hi - lo, the same in every batch, is taken back in the
+// energy to first order once the potential is built).
 // Blocks walk the core group list; the others run the two-term kernel.
 // The grid's descriptor comes by value (a kernel argument: every pointer in it
 // is known to address global memory, and uniform table reads become scalar
@@ -440,11 +516,11 @@ __global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
   // 64 bytes; DMA lane i fetches row i >> 2, 16-byte chunk (i & 3) ^ sw(row)
   // (four neighbouring lanes read one contiguous 64-byte run: coalesced), and
   // the MFMA lane that needs (row, chunk) reads it back at frag_idx(row, chunk)
-  // (the XOR keeps those LDS reads free of bank conflicts). P3 adds the third
-  // terms: A3 row block = wave, D3 column blocks wave and wave + 4.
+  // (the XOR keeps those LDS reads free of bank conflicts). P3 adds the AO
+  // operand's third term: A3 row block = wave.
   const int drow = lane >> 2, dch = (lane & 3) ^ ((drow >> 2) & 3);
   const int fidx = 4 * lrow + (lk ^ ((lrow >> 2) & 3));  // this lane's MFMA operand in a fragment
-  int offa[2], offb[4], offb3[2];
+  int offa[2], offb[4];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int f = wave + 4 * j;
@@ -458,12 +534,6 @@ __global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
   }
   const u16* __restrict__ c3 = nullptr;
   if (P3) {
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int c = wave + 4 * j, q = c >> 1;
-      const int t = q == 1 ? jt[1] : q == 2 ? jt[2] : q == 3 ? jt[3] : jt[0];
-      offb3[j] = ((q < nt ? t : jt[0]) * kTile + 16 * (c & 1) + drow) * npad + 8 * dch;
-    }
     // this batch's core slots in the side cache: [core slot][c][g][mu]
     c3 = m.cache3 + ((size_t)m.core_off[b] * (m.toff / 2) + (size_t)MODE * kBatch * kTile) +
          (16 * wave + drow) * kTile + 8 * dch;
@@ -478,14 +548,9 @@ __global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       __builtin_amdgcn_global_load_lds(Ds + offb[j] + ti, (lds_ptr)(sm.F + (kDxA + wave + 4 * j) * kDxFrag), 16, 0, 0);
-    if (P3) {
+    if (P3)
       __builtin_amdgcn_global_load_lds(c3 + (size_t)k * (m.toff / 2),
                                        (lds_ptr)(sm.F + (kDxA + kDxB + wave) * kDxFrag), 16, 0, 0);
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        __builtin_amdgcn_global_load_lds(m.Ds3 + offb3[j] + ti,
-                                         (lds_ptr)(sm.F + (kDxA + kDxB + kDxA3 + wave + 4 * j) * kDxFrag), 16, 0, 0);
-    }
   };
 
   f32x4 acc[2][4];
@@ -498,7 +563,7 @@ __global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
   for (int k = 0; k < nk; ++k) {
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
     __syncthreads();                     // ... and every other wave's
-    u32x4 a[2][2], bq[4][2], a3[2], b3[4];
+    u32x4 a[2][2], bq[4][2], a3[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -510,8 +575,6 @@ __global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
     if (P3) {
 #pragma unroll
       for (int r = 0; r < 2; ++r) a3[r] = sm.F[(kDxA + kDxB + 2 * wr + r) * kDxFrag + fidx];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b3[c] = sm.F[(kDxA + kDxB + kDxA3 + 4 * wc + c) * kDxFrag + fidx];
     }
     __syncthreads();  // every wave holds its fragments: the buffer is free
     if (k + 1 < nk) dma(k + 1);
@@ -524,7 +587,6 @@ __global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
           if (P3) {  // the small terms first
             v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r][1]), as_bf(bq[c][1]), v, 0, 0, 0);
             v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a3[r]), as_bf(bq[c][0]), v, 0, 0, 0);
-            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r][0]), as_bf(b3[c]), v, 0, 0, 0);
           }
           acc[r][c] = mfma3(a[r][0], a[r][1], bq[c][0], bq[c][1], v);
         }

@@ -129,10 +129,10 @@ void build_lanes(chx_pack* p) {
   for (int k = 0; k < L; ++k) {
     const int m0 = p->cut[k], m1 = p->cut[k + 1];
     auto ptrs = [&](const chx_grid* g) {
-      return std::array<const char*, 7>{(const char*)g->d_grp_off.p, (const char*)g->d_part.p,
+      return std::array<const char*, 8>{(const char*)g->d_grp_off.p, (const char*)g->d_part.p,
                                         (const char*)g->d_part_tau.p, (const char*)g->d_w.p,
                                         (const char*)g->d_rho.p, (const char*)g->d_coef.p,
-                                        (const char*)g->d_en_part.p};
+                                        (const char*)g->d_en_part.p, (const char*)g->d_cpart.p};
     };
     const char* base = nullptr;
     for (int m = m0; m < m1; ++m)
@@ -145,7 +145,8 @@ void build_lanes(chx_pack* p) {
       const auto q = ptrs(g);
       FnMolecule f{};
       f.grp_off = q[0] - base; f.part = q[1] - base; f.part_tau = q[2] - base; f.w = q[3] - base;
-      f.rho = q[4] - base; f.coef = q[5] - base; f.en_part = q[6] - base;
+      f.rho = q[4] - base; f.coef = q[5] - base; f.en_part = q[6] - base; f.cpart = q[7] - base;
+      f.ncpart = corr_blocks(g->nbasis);
       f.nslots = g->nbatches * kBatch;
       f.blk0 = blk;
       f.slot_en = m;
@@ -177,16 +178,17 @@ void build_lanes(chx_pack* p) {
 
 // The molecules of a lane on stream s: split D and clear M first; the density
 // launches wait for `after` (if set) and signal `dens_done` (if set); then the
-// functional and E/N sums of all the lane's molecules in one launch each, and
-// per molecule B^T, Vxc and V = M + M^T. Apart from the functional, the kernels
-// and their arguments are those of the single-grid build.
+// functional and E/N sums of all the lane's molecules in one launch each, per
+// molecule B^T, Vxc and V = M + M^T (with the partial sums of V . (D - hi -
+// lo)), and last the energies' correction for the split of D, one launch for
+// the lane. Apart from the functional and the correction, the kernels and their
+// arguments are those of the single-grid build.
 void enqueue_lane(chx_pack* p, int lane, hipStream_t s, hipEvent_t after, hipEvent_t dens_done,
                   const double* const* D, double* const* V) {
   const int m0 = p->cut[lane], m1 = p->cut[lane + 1];
   for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
-    launch_split_d(s, g->nbasis, g->npad, D[m], g->d_int2ref.p, g->d_Dsplit.p,
-                   g->core_grps.empty() ? nullptr : g->d_Ds3.p);
+    launch_split_d(s, g->nbasis, g->npad, D[m], g->d_int2ref.p, g->d_Dsplit.p, g->d_Dres.p);
     launch_fill(s, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
   }
   if (after) CHX_HIP(hipStreamWaitEvent(s, after, 0));
@@ -194,7 +196,7 @@ void enqueue_lane(chx_pack* p, int lane, hipStream_t s, hipEvent_t after, hipEve
     chx_grid* g = p->grids[m];
     const int ng = (int)g->grp_batch.size(), ncg = (int)g->core_grps.size();
     launch_density_mx(s, p->type >= 2 ? 5 : 0, p->type >= 2 ? ng : ng - ncg, g->hmol);
-    // core batches' rho and grad rho again with three-term operands
+    // core batches' rho and grad rho again with three-term AO operands
     launch_density_mx_core(s, 0, (int)g->core_grps.size(), g->hmol);
   }
   if (dens_done) CHX_HIP(hipEventRecord(dens_done, s));
@@ -209,8 +211,9 @@ void enqueue_lane(chx_pack* p, int lane, hipStream_t s, hipEvent_t after, hipEve
       launch_vxc_mx_wide(s, nc, (int)g->cwork.size(), g->hmol);
     else
       launch_vxc_mx(s, nc, (int)g->cwork.size(), g->hmol);
-    launch_finalize_sym(s, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, V[m]);
+    launch_finalize_corr(s, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, V[m], g->d_Dres.p, g->d_cpart.p);
   }
+  launch_corr_multi(s, p->fn_base[lane], p->fn_tab.p + p->fn_off[lane], m1 - m0, p->d_en.p);
 }
 
 // One XC build of every molecule: one lane runs on the pack's stream; more

@@ -111,7 +111,15 @@ int chx_grid_get_density(chx_grid* g, const double* D, int nspin, int nbasis);
 int chx_grid_get_density_device(chx_grid* g, const double* D_dev, int nspin, int nbasis);
 /* Grid::getNumElectrons (src/Grid/Grid.cpp:51-61). */
 int chx_grid_num_electrons(chx_grid* g, double* N);
-/* Grid::getEnergy (src/Grid/Grid.cpp:112-122). */
+/* Grid::getEnergy (src/Grid/Grid.cpp:112-122). Mixed form after
+ * Evaluate("ev"): the contractions see each D_s as its two bf16 terms hi + lo,
+ * and E adds back the first-order energy of the remainder,
+ * sum_s V_s . (D_s - hi - lo) (dE/dD_s = V_s), which makes E a smooth function
+ * of D (without it E moves by up to ~2^-17 |V . D| between nearby densities,
+ * 5e-5 Eh at (H2O)128). The correction needs the potential: if no Fock call
+ * followed the Evaluate, this call builds it into the grid's own buffer, and
+ * the next chx_grid_fock / chx_grid_fock_device returns that copy. After
+ * Evaluate("e") (no potential) E is returned without the correction. */
 int chx_grid_energy(chx_grid* g, double* E);
 /* Grid::getFock (src/Grid/Grid.cpp:156-171): writes nspin matrices (host),
  * nspin equal to the evaluated density's. */
@@ -162,8 +170,9 @@ int chx_grid_set_precision(chx_grid* g, int precision);
 /* Mixed form: batches with a point within `radius` bohr of a nucleus with
  * Z > 2 (the atomic cores, where the density is largest; the grid batches
  * the points within 0.3 bohr of such a nucleus apart) form rho and grad rho
- * with operands split in three bf16 terms (FP32-level) instead of two; 0
- * turns this off, at most 1. Default 0.3; set before getAO. No reference
+ * with the AO operand split in three bf16 terms (FP32-level) instead of two
+ * (D keeps two: its remainder is taken back in the energy, chx_grid_energy);
+ * 0 turns this off, at most 1. Default 0.3; set before getAO. No reference
  * counterpart. */
 int chx_grid_set_core_radius(chx_grid* g, double radius);
 /* number of such core batches in the current plan (0 for the FP64 form) */
@@ -299,7 +308,8 @@ void chx_jfit_destroy(chx_jfit* j);
  * call: the lanes are cut again and the graph is recaptured. A member grid
  * switched to another form or type makes the pack call fail (CHX_ERR_STATE).
  * D_dev[m] / V_dev[m]: device arrays, column-major nbasis_m^2, reference
- * order. E, N (host, may be NULL): E_xc and N_e per molecule. */
+ * order. E, N (host, may be NULL): E_xc (with the correction for the split
+ * of D, as chx_grid_energy) and N_e per molecule. */
 typedef struct chx_pack chx_pack;
 int chx_pack_create(chx_grid* const* grids, int n, chx_pack** out);
 int chx_pack_xc(chx_pack* p, const chx_xc* xc, const double* const* D_dev, double* const* V_dev, double* E,

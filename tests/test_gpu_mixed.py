@@ -166,8 +166,10 @@ def test_mixed_meta_grid_after_gga_potentials():
 def test_vxc_block_shapes_agree(name):
     """Both Vxc block shapes of the mixed form (128-AO super-tile pairs; 256-AO groups of each batch
     chunk's alive-tile union) on bases below the default switch (npad 384 and 640): the same E_xc (the
-    shape only changes Vxc), V within the mixed form's rounding of each other and of the FP64 form;
-    the tile autotuner keeps one of them."""
+    shape only changes Vxc and, through it, E's first-order correction for the split of D,
+    sum V . (D - hi - lo), whose size is about 2^-17 |V . D|: the shapes' V differ by their rounding,
+    which moves E by far less than 1e-9 Eh), V within the mixed form's rounding of each other and of
+    the FP64 form; the tile autotuner keeps one of them."""
     from chinium_amd import hip
     w = wl.build_workload(name)
     mol = ca.Molecule(w["Z"], w["xyz_bohr"], w["basis"])
@@ -190,7 +192,7 @@ def test_vxc_block_shapes_agree(name):
     (E0, V0), (E1, V1) = out[0], out[1]
     print("  %s: shape 0 vs 1: dE %.1e, max dV %.1e; vs FP64: %.1e, %.1e" % (
         name, E1 - E0, np.abs(V1 - V0).max(), np.abs(V0 - V64).max(), np.abs(V1 - V64).max()), flush=True)
-    assert E0 == E1
+    assert abs(E1 - E0) <= 1e-9
     assert np.abs(V1 - V0).max() <= 5e-6
     assert np.abs(V0 - V64).max() <= 5e-5 and np.abs(V1 - V64).max() <= 5e-5
     n = mol.nbasis
@@ -201,4 +203,4 @@ def test_vxc_block_shapes_agree(name):
           flush=True)
     assert shape in (0, 1) and g.vxcShape() == shape and min(ms.values()) == ms[(shape, chunk)]
     E2, V2 = ca.xc_build(g, xc, D)
-    assert E2 == E0 and np.abs(V2 - V64).max() <= 5e-5
+    assert abs(E2 - E0) <= 1e-9 and np.abs(V2 - V64).max() <= 5e-5
