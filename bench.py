@@ -79,6 +79,9 @@ def main():
                     help="instead of the build timing: the full Kohn-Sham SCF of the workload (r2SCAN: fitted J from "
                          "the device Coulomb engine, ADIIS -> CDIIS), mixed form from the core guess, then the plain "
                          "FP64 form from the mixed form's converged Fock matrix; one JSON line with both energies")
+    ap.add_argument("--scf-mixed-tol", type=float, default=1e-6,
+                    help="--scf: the mixed form's CDIIS tolerance on max|G| (the reference's is 1e-6)")
+    ap.add_argument("--scf-max-iter", type=int, default=300, help="--scf: iterations per DIIS phase")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -431,7 +434,7 @@ def bench_scf(args):
                       "primitive_pairs_unscreened": info["pairs_unscreened"]},
            "integrals_s": round(t_ints, 2)}
     F0 = None
-    for prec, tol, name in ((ca.CHX_PREC_MIXED, 1e-5, "mixed"), (ca.CHX_PREC_FP64, 1e-6, "fp64")):
+    for prec, tol, name in ((ca.CHX_PREC_MIXED, args.scf_mixed_tol, "mixed"), (ca.CHX_PREC_FP64, 1e-6, "fp64")):
         t0 = time.perf_counter()
         grid = ca.Grid(mol, w["grid"], 1, 0)
         grid.setPrecision(prec)
@@ -440,13 +443,15 @@ def bench_scf(args):
         t_grid = time.perf_counter() - t0
         scf = RestrictedDFSCF(mol, w["nocc"], xc, grid, coulomb=jf)
         t0 = time.perf_counter()
-        r = scf.run(F0=F0, tol=tol, log=_say)
+        r = scf.run(F0=F0, tol=tol, max_iter=args.scf_max_iter, log=_say)
         t_scf = time.perf_counter() - t0
         F0 = r["F"]
         hist = r["history"]
         out[name] = {"E": r["E"], "E_xc": r["E_xc"], "iterations": r["iterations"], "diis_error": r["diis_error"],
                      "tol": tol, "start": "core guess" if name == "mixed" else "the mixed form's converged Fock matrix",
                      "last_dE": hist[-1][1] - hist[-2][1] if len(hist) > 1 else None,
+                     "E_spread_last3": (max(h[1] for h in hist[-3:]) - min(h[1] for h in hist[-3:])
+                                        if len(hist) >= 3 else None),
                      "scf_s": round(t_scf, 2), "grid_s": round(t_grid, 2),
                      "s_per_iter": {k: round(float(np.median(v)), 4) for k, v in scf.timing.items()}}
         _say("%s: E %.10f in %d iterations (%.1f s)" % (name, r["E"], r["iterations"], t_scf))
