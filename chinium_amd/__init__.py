@@ -603,11 +603,14 @@ class ExchangeCorrelation:
         return e, vr, vs, vt
 
     def eval_points_f(self, rho, sigma=None):
-        """v2rho2, v2rhosigma, v2sigma2 at the given points (LDA/GGA), on the device."""
-        rho = np.ascontiguousarray(rho, dtype=np.float64)
-        n = len(rho)
-        sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
-        f0, f1, f2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        """v2rho2, v2rhosigma, v2sigma2 at the given points (LDA/GGA), on the device. Spin = 2: rho (n x 2),
+        sigma (n x 3) in libxc's polarised layout; returns (n x 3), (n x 6), (n x 6)."""
+        pol = self.Spin == 2
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape((-1, 2) if pol else (-1,))
+        n = rho.shape[0]
+        sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64).reshape((-1, 3) if pol else (-1,))
+        shp = ((n, 3), (n, 6), (n, 6)) if pol else ((n,), (n,), (n,))
+        f0, f1, f2 = (np.zeros(x) for x in shp)
         _check(lib().chx_xc_eval_points_f(self.h, n, _ptr(rho), _ptr(sg), _ptr(f0), _ptr(f1), _ptr(f2)))
         return f0, f1, f2
 

@@ -509,14 +509,14 @@ int chx_grid_get_density_skeleton(chx_grid* g, const double* D, int nspin, int n
 
 int chx_grid_get_density_u(chx_grid* g, const double* D, int nmats, int nspin, int nbasis) {
   return guard([&] {
-    check_dm_args(g, nspin, nbasis);
+    check_dm_args(g, nspin, nbasis, 2);
     require(D != nullptr && nmats >= 1, CHX_ERR_ARG, "bad perturbed density matrices");
     require(g->precision == CHX_PREC_FP64, CHX_ERR_UNSUPPORTED, "the response path is built for the FP64 form");
     const size_t nslots = (size_t)g->nbatches * kBatch, nb2 = (size_t)nbasis * nbasis;
     const int ng = (int)g->grp_batch.size();
     double* part = g->d_part_u.alloc_get(std::max<size_t>(1, (size_t)ng * 4 * kBatch));
-    double* rhou = g->d_rhou.alloc_get(std::max<size_t>(1, (size_t)nmats * 4 * nslots));
-    for (int m = 0; m < nmats; ++m) {  // rho^x and grad rho^x of each matrix, per point
+    double* rhou = g->d_rhou.alloc_get(std::max<size_t>(1, (size_t)nspin * nmats * 4 * nslots));
+    for (int m = 0; m < nspin * nmats; ++m) {  // rho^x and grad rho^x of each matrix ([spin][mat]), per point
       CHX_HIP(hipMemcpyAsync(g->d_Din.p, D + m * nb2, sizeof(double) * nb2, hipMemcpyHostToDevice, g->stream));
       launch_permute_d(g->stream, g->nbasis, g->npad, g->d_Din.p, g->d_int2ref.p, g->d_Dint.p);
       if (g->cached)
@@ -529,20 +529,28 @@ int chx_grid_get_density_u(chx_grid* g, const double* D, int nmats, int nspin, i
     }
     CHX_HIP(hipStreamSynchronize(g->stream));
     g->nmats_u = nmats;
+    g->nspin_u = nspin;
   });
 }
 
 // Response matrices of nmats perturbed densities held per point in rhou
-// ([mat][point][rho, grad rho], point = batch-major slot): per matrix the
-// response coefficients (k_response_coef), then the ground-state Vxc pipeline
-// on them (B = a phi + c . grad phi, M = Phi^T B, V = M + M^T), V to F (host).
-static void response_fock(chx_grid* g, const double* rhou, int nmats, double* F) {
+// ([spin][mat][point][rho, grad rho], point = batch-major slot): per matrix
+// (and spin, nspin = 2) the response coefficients (k_response_coef /
+// k_response_coef_pol), then the ground-state Vxc pipeline on them
+// (B = a phi + c . grad phi, M = Phi^T B, V = M + M^T), V to F (host,
+// [spin][mat]).
+static void response_fock(chx_grid* g, const double* rhou, int nmats, double* F, int nspin = 1) {
   const size_t nslots = (size_t)g->nbatches * kBatch, nb2 = (size_t)g->nbasis * g->nbasis;
   g->vout_ready = false;  // d_Vout stages the response matrices
   double* coef = g->d_coef_u.alloc_get(std::max<size_t>(kCoef, kCoef * nslots));
-  for (int m = 0; m < nmats; ++m) {
-    launch_response_coef(g->stream, g->nbatches, rhou + (size_t)m * 4 * nslots, g->d_fx.p, g->d_w.p, g->fx_family,
-                         coef);
+  for (int m = 0; m < nspin * nmats; ++m) {
+    if (nspin == 2)  // output m = spin u, perturbation m % nmats (both spins' rho^x enter)
+      launch_response_coef_pol(g->stream, g->nbatches, rhou + (size_t)(m % nmats) * 4 * nslots,
+                               rhou + (size_t)(nmats + m % nmats) * 4 * nslots, g->d_fx.p, g->d_w.p, g->fx_family,
+                               m / nmats, coef);
+    else
+      launch_response_coef(g->stream, g->nbatches, rhou + (size_t)m * 4 * nslots, g->d_fx.p, g->d_w.p, g->fx_family,
+                           coef);
     if (g->cached) {  // B = a phi + c . grad phi from the tables, M = Phi^T B, V = M + M^T
       launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, coef, g->d_cache.p, g->d_bmat.p);
       launch_fill(g->stream, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
@@ -563,12 +571,14 @@ static void response_fock(chx_grid* g, const double* rhou, int nmats, double* F)
 int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis) {
   return guard([&] {
     require(F != nullptr, CHX_ERR_ARG, "null output");
-    require(nspin == 1, CHX_ERR_UNSUPPORTED, "only the closed-shell path is built");
+    require(nspin == 1 || nspin == 2, CHX_ERR_ARG, "Wrong number of spin types!");
     require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
     require(g->fx_family >= 0, CHX_ERR_STATE, "Evaluate(\"f\") must precede Grid::getFockU");
     require(g->nmats_u > 0, CHX_ERR_STATE, "Grid::getDensityU must precede Grid::getFockU");
     require(nmats == g->nmats_u, CHX_ERR_ARG, "number of matrices differs from the last Grid::getDensityU");
-    response_fock(g, g->d_rhou.p, nmats, F);
+    require(nspin == g->nspin_u && nspin == g->fx_nspin, CHX_ERR_ARG,
+            "spin count differs from the last Grid::getDensityU / Evaluate(\"f\")");
+    response_fock(g, g->d_rhou.p, nmats, F, nspin);
   });
 }
 
@@ -595,6 +605,7 @@ int chx_grid_fock_s(chx_grid* g, double* F, int nspin, int nbasis) {
     require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
     require(g->have_skeleton, CHX_ERR_STATE, "Grid::getDensitySkeleton must precede Grid::getFockU<s_t>");
     require(g->fx_family >= 0, CHX_ERR_STATE, "Evaluate(\"f\") must precede Grid::getFockU<s_t>");
+    require(g->fx_nspin == 1, CHX_ERR_UNSUPPORTED, "the skeleton response is built for the closed-shell path");
     require(g->precision == CHX_PREC_FP64 && g->cached, CHX_ERR_UNSUPPORTED,
             "the skeleton response is built for the FP64 form with the AO tables in HBM");
     const int na = g->mol.natoms, ng = (int)g->grp_batch.size(), ns = (int)g->bt_list.size();
@@ -994,14 +1005,21 @@ int chx_xc_evaluate(const chx_xc* xc, chx_grid* g, const char* order) {
     require(g->type >= xc->family, CHX_ERR_STATE, "grid type is below the functional family (call setType)");
     if (o == "f") {  // f_xc at the ground-state density, for the response path
       require(xc->family <= 1, CHX_ERR_UNSUPPORTED, "Evaluate(\"f\") is built for LDA and GGA functionals");
-      require(g->nspin == 1 && xc->spin == 1, CHX_ERR_UNSUPPORTED, "the response path is built for nspin = 1");
+      require(g->nspin == xc->spin, CHX_ERR_STATE,
+              "the functional's spin count differs from the density's (ExchangeCorrelation::Spin)");
       require(g->precision == CHX_PREC_FP64, CHX_ERR_UNSUPPORTED, "the response path is built for the FP64 form");
       g->d_codes.upload(xc->codes, g->stream);
       const size_t nslots = (size_t)g->nbatches * kBatch;
-      launch_functional_f(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_codes.p, (int)xc->codes.size(),
-                          xc->family, g->d_fx.alloc_get(std::max<size_t>(1, kFx * nslots)));
+      if (g->nspin == 2)  // alpha partials in d_part, beta in d_part_b
+        launch_functional_fpol(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_part_b.p, g->d_codes.p,
+                               (int)xc->codes.size(), xc->family,
+                               g->d_fx.alloc_get(std::max<size_t>(1, kFxPol * nslots)));
+      else
+        launch_functional_f(g->stream, g->nbatches, g->d_grp_off.p, g->d_part.p, g->d_codes.p, (int)xc->codes.size(),
+                            xc->family, g->d_fx.alloc_get(std::max<size_t>(1, kFx * nslots)));
       CHX_HIP(hipStreamSynchronize(g->stream));
       g->fx_family = xc->family;
+      g->fx_nspin = g->nspin;
       return;
     }
     const bool e = o.find('e') != std::string::npos, v = o.find('v') != std::string::npos;
@@ -1075,18 +1093,24 @@ int chx_xc_eval_points_f(const chx_xc* xc, long n, const double* rho, const doub
   return guard([&] {
     require(xc && n >= 0 && rho && v2rho2, CHX_ERR_ARG, "bad arguments");
     require(xc->family <= 1, CHX_ERR_UNSUPPORTED, "second derivatives are built for LDA and GGA functionals");
+    const bool pol = xc->spin == 2;  // libxc XC_POLARIZED: rho 2, sigma 3; v2rho2 3, v2rhosigma 6, v2sigma2 6 per point
+    const long nr = pol ? 2 : 1, ns = pol ? 3 : 1, o0 = pol ? 3 : 1, o1 = pol ? 6 : 1, o2 = pol ? 6 : 1;
     hipStream_t s;
     CHX_HIP(hipStreamCreate(&s));
-    const std::vector<double> hr(rho, rho + n), hs(sigma ? std::vector<double>(sigma, sigma + n) : std::vector<double>(n, 0.0));
+    const std::vector<double> hr(rho, rho + nr * n),
+        hs(sigma ? std::vector<double>(sigma, sigma + ns * n) : std::vector<double>(ns * n, 0.0));
     DevBuf<double> dr, ds, f0, f1, f2;
     DevBuf<int> dc;
     dr.upload(hr, s); ds.upload(hs, s); dc.upload(xc->codes, s);
-    f0.alloc(std::max<long>(1, n)); f1.alloc(std::max<long>(1, n)); f2.alloc(std::max<long>(1, n));
-    launch_eval_functional_f(s, n, dc.p, (int)xc->codes.size(), dr.p, ds.p, f0.p, f1.p, f2.p);
-    auto down = [&](double* h, const DevBuf<double>& d) {
-      if (h && n) CHX_HIP(hipMemcpyAsync(h, d.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    f0.alloc(std::max<long>(1, o0 * n)); f1.alloc(std::max<long>(1, o1 * n)); f2.alloc(std::max<long>(1, o2 * n));
+    if (pol)
+      launch_eval_functional_fpol(s, n, dc.p, (int)xc->codes.size(), dr.p, sigma ? ds.p : nullptr, f0.p, f1.p, f2.p);
+    else
+      launch_eval_functional_f(s, n, dc.p, (int)xc->codes.size(), dr.p, ds.p, f0.p, f1.p, f2.p);
+    auto down = [&](double* h, const DevBuf<double>& d, long k) {
+      if (h && n) CHX_HIP(hipMemcpyAsync(h, d.p, sizeof(double) * k * n, hipMemcpyDeviceToHost, s));
     };
-    down(v2rho2, f0); down(v2rhosigma, f1); down(v2sigma2, f2);
+    down(v2rho2, f0, o0); down(v2rhosigma, f1, o1); down(v2sigma2, f2, o2);
     CHX_HIP(hipStreamSynchronize(s));
     CHX_HIP(hipStreamDestroy(s));
   });

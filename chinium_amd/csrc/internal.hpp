@@ -152,6 +152,14 @@ void launch_ao_cache(hipStream_t s, int nslots, const int* slot_batch, const int
                      const double* bx, const double* by, const double* bz, BasisView bv, double* cache);
 // response path (closed shell, LDA/GGA)
 constexpr int kFx = 8;  // per slot: v2rho2, v2rhosigma, v2sigma2, v_sigma, grad rho (3), -
+// polarised (nspin = 2): vsigma (3), v2rho2 (3), v2rhosigma (6), v2sigma2 (6), grad rho_a (3), grad rho_b (3)
+constexpr int kFxPol = 24;
+void launch_functional_fpol(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* part_b,
+                            const int* codes, int ncodes, int family, double* fx);
+void launch_response_coef_pol(hipStream_t s, int nbatches, const double* rhou_a, const double* rhou_b,
+                              const double* fx, const double* bw, int family, int spin, double* coef);
+void launch_eval_functional_fpol(hipStream_t s, long n, const int* codes, int ncodes, const double* rho,
+                                 const double* sigma, double* v2rho2, double* v2rhosigma, double* v2sigma2);
 void launch_functional_f(hipStream_t s, int nbatches, const int* grp_off, const double* part, const int* codes,
                          int ncodes, int family, double* fx);
 void launch_point_rho(hipStream_t s, int nbatches, const int* grp_off, const double* part, double* rhou);
@@ -446,7 +454,9 @@ struct chx_grid {
   // response path: f_xc at the ground-state density, perturbed densities
   chx::DevBuf<double> d_fx, d_part_u, d_rhou, d_coef_u;
   int fx_family = -1;  // family of the f_xc on the grid (-1: none)
-  int nmats_u = 0;     // perturbed densities held in d_rhou
+  int fx_nspin = 1;    // spin count of that f_xc (2: kFxPol values per slot)
+  int nmats_u = 0;     // perturbed densities held in d_rhou (per spin)
+  int nspin_u = 1;     // spin count of those densities ([spin][mat] in d_rhou)
   // spin-polarised densities (nspin = 2): beta partials and coefficients; alpha in d_part / d_coef
   int nspin = 1;    // spin count of the density on the grid
   int v_nspin = 1;  // spin count of the potential on the grid

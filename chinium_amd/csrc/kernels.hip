@@ -667,6 +667,163 @@ __global__ __launch_bounds__(256) void k_response_coef(int nslots, const double*
   c[4] = 0.0;
 }
 
+// ---- Spin-polarised response (nspin = 2, LDA/GGA): Evaluate("f") with the
+// functional polarised, getDensityU / getFockU<u_t> at nspin = 2.
+
+// Polarised f_xc at the grid's ground-state density (alpha partials in part,
+// beta in part_b): per slot kFxPol values, eval_codes_fpol's 18 (vsigma,
+// v2rho2, v2rhosigma, v2sigma2), then grad rho_a (3) and grad rho_b (3).
+__global__ __launch_bounds__(256) void k_functional_fpol(int nslots, const int* __restrict__ grp_off,
+                                                         const double* __restrict__ part,
+                                                         const double* __restrict__ part_b,
+                                                         const int* __restrict__ codes, int ncodes, int family,
+                                                         double* __restrict__ fx) {
+  const int slot = blockIdx.x * 256 + threadIdx.x;
+  if (slot >= nslots) return;
+  const int b = slot / kBatch, g = slot % kBatch;
+  double ra = 0, rb = 0, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+  for (int k = grp_off[b]; k < grp_off[b + 1]; ++k) {
+    const double* p = part + (size_t)k * 4 * kBatch;
+    const double* q = part_b + (size_t)k * 4 * kBatch;
+    ra += p[g];
+    rb += q[g];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      ga[r] += p[(r + 1) * kBatch + g];
+      gb[r] += q[(r + 1) * kBatch + g];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    ga[r] = family >= 1 ? 2 * ga[r] : 0.0;
+    gb[r] = family >= 1 ? 2 * gb[r] : 0.0;
+  }
+  const double saa = ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2];
+  const double sab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+  const double sbb = gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2];
+  double* f = fx + (size_t)kFxPol * slot;
+  eval_codes_fpol(codes, ncodes, ra, rb, saa, sab, sbb, f);
+  if (family < 1) {  // LDA: no sigma terms
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f[k] = 0.0;
+#pragma unroll
+    for (int k = 6; k < 18; ++k) f[k] = 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    f[18 + r] = ga[r];
+    f[21 + r] = gb[r];
+  }
+}
+
+// Coefficients of spin u's response matrix for one perturbation (rho^x_s,
+// grad rho^x_s of both spins per slot in rhou_a / rhou_b), in k_functional's
+// layout (a phi + c . grad phi, then M + M^T):
+//   sigma^x = (2 grad rho_a . grad rho^x_a, grad rho^x_a . grad rho_b + grad rho_a . grad rho^x_b,
+//              2 grad rho_b . grad rho^x_b),
+//   dv_rho_u = sum_v f_{rho_u rho_v} rho^x_v + sum_x f_{rho_u sigma_x} sigma^x_x,
+//   dv_sigma_x = sum_w f_{rho_w sigma_x} rho^x_w + sum_y f_{sigma_x sigma_y} sigma^x_y,
+//   a = 1/2 w dv_rho_u,
+//   c = w (2 dv_sigma_uu grad rho_u + dv_sigma_ab grad rho_o + 2 v_sigma_uu grad rho^x_u
+//          + v_sigma_ab grad rho^x_o)
+// (the derivative of the polarised ground-state potential along D^x).
+__global__ __launch_bounds__(256) void k_response_coef_pol(int nslots, const double* __restrict__ rhou_a,
+                                                           const double* __restrict__ rhou_b,
+                                                           const double* __restrict__ fx,
+                                                           const double* __restrict__ bw, int family, int spin,
+                                                           double* __restrict__ coef) {
+  const int slot = blockIdx.x * 256 + threadIdx.x;
+  if (slot >= nslots) return;
+  const double* f = fx + (size_t)kFxPol * slot;
+  const double* xa = rhou_a + 4 * (size_t)slot;
+  const double* xb = rhou_b + 4 * (size_t)slot;
+  const double* ga = f + 18;
+  const double* gb = f + 21;
+  const double w = bw[slot];
+  double sx[3] = {0, 0, 0};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    sx[0] += 2.0 * ga[r] * xa[1 + r];
+    sx[1] += xa[1 + r] * gb[r] + ga[r] * xb[1 + r];
+    sx[2] += 2.0 * gb[r] * xb[1 + r];
+  }
+  const double rx[2] = {xa[0], xb[0]};
+  const int u = spin;
+  // f layout: vsigma 0..2, v2rho2 3..5 (aa, ab, bb), v2rhosigma 6 + 3 w + x, v2sigma2 12.. (upper triangle)
+  double dvr = f[3 + u] * rx[0] + f[4 + u] * rx[1];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) dvr += f[6 + 3 * u + x] * sx[x];
+  double* c = coef + kCoef * (size_t)slot;
+  c[0] = 0.5 * w * dvr;
+  if (family >= 1) {
+    const int fss[3][3] = {{12, 13, 14}, {13, 15, 16}, {14, 16, 17}};
+    double dvs[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      dvs[x] = f[6 + x] * rx[0] + f[9 + x] * rx[1];
+#pragma unroll
+      for (int y = 0; y < 3; ++y) dvs[x] += f[fss[x][y]] * sx[y];
+    }
+    const int uu = u == 0 ? 0 : 2;
+    const double* gu = u == 0 ? ga : gb;
+    const double* go = u == 0 ? gb : ga;
+    const double* xu = u == 0 ? xa : xb;
+    const double* xo = u == 0 ? xb : xa;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      c[1 + r] = w * (2.0 * dvs[uu] * gu[r] + dvs[1] * go[r] + 2.0 * f[uu] * xu[1 + r] + f[1] * xo[1 + r]);
+  } else {
+    c[1] = c[2] = c[3] = 0.0;
+  }
+  c[4] = 0.0;
+}
+
+// Polarised f_xc at arbitrary points (libxc XC_POLARIZED layout): rho[2n],
+// sigma[3n] (null: LDA) -> v2rho2[3n], v2rhosigma[6n], v2sigma2[6n].
+__global__ void k_eval_functional_fpol(long n, const int* __restrict__ codes, int ncodes,
+                                       const double* __restrict__ rho, const double* __restrict__ sigma,
+                                       double* __restrict__ v2rho2, double* __restrict__ v2rhosigma,
+                                       double* __restrict__ v2sigma2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[18];
+  eval_codes_fpol(codes, ncodes, rho[2 * i], rho[2 * i + 1], sigma ? sigma[3 * i] : 0.0, sigma ? sigma[3 * i + 1] : 0.0,
+                  sigma ? sigma[3 * i + 2] : 0.0, f);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v2rho2[3 * i + k] = f[3 + k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    v2rhosigma[6 * i + k] = sigma ? f[6 + k] : 0.0;
+    v2sigma2[6 * i + k] = sigma ? f[12 + k] : 0.0;
+  }
+}
+
+void launch_functional_fpol(hipStream_t s, int nbatches, const int* grp_off, const double* part, const double* part_b,
+                            const int* codes, int ncodes, int family, double* fx) {
+  const int nslots = nbatches * kBatch;
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_functional_fpol, dim3((nslots + 255) / 256), dim3(256), 0, s, nslots, grp_off, part, part_b,
+                     codes, ncodes, family, fx);
+  CHX_HIP(hipGetLastError());
+}
+
+void launch_response_coef_pol(hipStream_t s, int nbatches, const double* rhou_a, const double* rhou_b,
+                              const double* fx, const double* bw, int family, int spin, double* coef) {
+  const int nslots = nbatches * kBatch;
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_response_coef_pol, dim3((nslots + 255) / 256), dim3(256), 0, s, nslots, rhou_a, rhou_b, fx, bw,
+                     family, spin, coef);
+  CHX_HIP(hipGetLastError());
+}
+
+void launch_eval_functional_fpol(hipStream_t s, long n, const int* codes, int ncodes, const double* rho,
+                                 const double* sigma, double* v2rho2, double* v2rhosigma, double* v2sigma2) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_eval_functional_fpol, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, n, codes, ncodes,
+                     rho, sigma, v2rho2, v2rhosigma, v2sigma2);
+  CHX_HIP(hipGetLastError());
+}
+
 // f_xc at arbitrary (rho, sigma), as libxc's xc_{lda,gga}_fxc returns it.
 __global__ void k_eval_functional_f(long n, const int* __restrict__ codes, int ncodes, const double* __restrict__ rho,
                                     const double* __restrict__ sigma, double* __restrict__ v2rho2,

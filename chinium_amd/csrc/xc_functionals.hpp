@@ -449,6 +449,104 @@ __device__ __forceinline__ P5 dpowp(const P5& a, double p) {
   return pchain(a, f, p * f / a.v);
 }
 
+// Second-order forward mode in the five polarised variables (rho_a, rho_b,
+// sigma_aa, sigma_ab, sigma_bb): value, gradient and the Hessian's upper
+// triangle (row-major, h[hix(i, j)] for i <= j), for the polarised f_xc of the
+// response path. The polarised energy densities below are templates on the
+// number type, so f_xc is the exact second derivative of the same code.
+struct H5 {
+  double v, g[5], h[15];
+};
+__device__ __forceinline__ constexpr int hix(int i, int j) { return i * 5 - i * (i - 1) / 2 + (j - i); }
+__device__ __forceinline__ H5 h5const(double x) {
+  H5 r;
+  r.v = x;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.g[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) r.h[k] = 0.0;
+  return r;
+}
+__device__ __forceinline__ H5 h5var(double x, int k) {
+  H5 r = h5const(x);
+  r.g[k] = 1.0;
+  return r;
+}
+__device__ __forceinline__ H5 h5chain(const H5& a, double f, double f1, double f2) {
+  H5 r;
+  r.v = f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) r.g[i] = f1 * a.g[i];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = i; j < 5; ++j) r.h[hix(i, j)] = f1 * a.h[hix(i, j)] + f2 * a.g[i] * a.g[j];
+  return r;
+}
+__device__ __forceinline__ H5 operator+(const H5& a, const H5& b) {
+  H5 r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.g[k] = a.g[k] + b.g[k];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) r.h[k] = a.h[k] + b.h[k];
+  return r;
+}
+__device__ __forceinline__ H5 operator-(const H5& a) { return h5chain(a, -a.v, -1.0, 0.0); }
+__device__ __forceinline__ H5 operator-(const H5& a, const H5& b) { return a + (-b); }
+__device__ __forceinline__ H5 operator*(const H5& a, const H5& b) {
+  H5 r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) r.g[i] = a.g[i] * b.v + a.v * b.g[i];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = i; j < 5; ++j)
+      r.h[hix(i, j)] = a.h[hix(i, j)] * b.v + a.g[i] * b.g[j] + a.g[j] * b.g[i] + a.v * b.h[hix(i, j)];
+  return r;
+}
+__device__ __forceinline__ H5 h5inv(const H5& a) {
+  const double q = 1.0 / a.v;
+  return h5chain(a, q, -q * q, 2.0 * q * q * q);
+}
+__device__ __forceinline__ H5 operator/(const H5& a, const H5& b) { return a * h5inv(b); }
+__device__ __forceinline__ H5 operator+(const H5& a, double b) { H5 r = a; r.v += b; return r; }
+__device__ __forceinline__ H5 operator+(double a, const H5& b) { return b + a; }
+__device__ __forceinline__ H5 operator-(const H5& a, double b) { return a + (-b); }
+__device__ __forceinline__ H5 operator-(double a, const H5& b) { return (-b) + a; }
+__device__ __forceinline__ H5 operator*(const H5& a, double b) { return h5chain(a, a.v * b, b, 0.0); }
+__device__ __forceinline__ H5 operator*(double a, const H5& b) { return b * a; }
+__device__ __forceinline__ H5 operator/(const H5& a, double b) { return a * (1.0 / b); }
+__device__ __forceinline__ H5 operator/(double a, const H5& b) { return h5inv(b) * a; }
+__device__ __forceinline__ H5 dexp(const H5& a) { const double e = ::exp(a.v); return h5chain(a, e, e, e); }
+__device__ __forceinline__ H5 dlog(const H5& a) { const double q = 1.0 / a.v; return h5chain(a, ::log(a.v), q, -q * q); }
+__device__ __forceinline__ H5 dsqrt(const H5& a) {
+  const double q = ::sqrt(a.v);
+  return h5chain(a, q, 0.5 / q, -0.25 / (q * a.v));
+}
+__device__ __forceinline__ H5 datan(const H5& a) {
+  const double q = 1.0 / (1.0 + a.v * a.v);
+  return h5chain(a, ::atan(a.v), q, -2.0 * a.v * q * q);
+}
+__device__ __forceinline__ H5 dasinh(const H5& a) {
+  const double q = 1.0 / ::sqrt(1.0 + a.v * a.v);
+  return h5chain(a, ::asinh(a.v), q, -a.v * q * q * q);
+}
+__device__ __forceinline__ H5 dcbrt(const H5& a) {
+  const double c = ::cbrt(a.v), q = 1.0 / a.v;
+  return h5chain(a, c, c * q / 3.0, -2.0 * c * q * q / 9.0);
+}
+__device__ __forceinline__ H5 dpowp(const H5& a, double p) {
+  const double f = ::pow(a.v, p), q = 1.0 / a.v;
+  return h5chain(a, f, p * f * q, p * (p - 1.0) * f * q * q);
+}
+
+// Constants in either polarised number type.
+template <class T> __device__ __forceinline__ T tconst(double x);
+template <> __device__ __forceinline__ P5 tconst<P5>(double x) { return pconst(x); }
+template <> __device__ __forceinline__ H5 tconst<H5>(double x) { return h5const(x); }
+
 // f(zeta) of the spin interpolation and its curvature at 0, 4 / (9 (2^(1/3) - 1)).
 constexpr double kFpp0 = 1.709920934161365617563962776245;
 template <class T>
@@ -458,18 +556,20 @@ __device__ __forceinline__ T f_zeta(const T& opz, const T& omz) {  // opz = 1 + 
 
 // VWN with spin (Vosko-Wilk-Nusair 1980, eq. 4.4 with their spin-stiffness
 // interpolation, as libxc's lda_c_vwn): eps = eP + a f (1 - z^4)/f''(0) + (eF - eP) f z^4.
-__device__ __forceinline__ P5 vwn5_eps_pol(const P5& rs, const P5& z, const P5& opz, const P5& omz) {
-  const P5 eP = vwn_eps(rs, 0.0310907, 3.72744, 12.9352, -0.10498);
-  const P5 eF = vwn_eps(rs, 0.01554535, 7.06042, 18.0578, -0.32500);
-  const P5 ea = vwn_eps(rs, -1.0 / (6.0 * kPi * kPi), 1.13107, 13.0045, -0.0047584);
-  const P5 f = f_zeta(opz, omz);
-  const P5 z4 = z * z * z * z;
+template <class T>
+__device__ __forceinline__ T vwn5_eps_pol(const T& rs, const T& z, const T& opz, const T& omz) {
+  const T eP = vwn_eps(rs, 0.0310907, 3.72744, 12.9352, -0.10498);
+  const T eF = vwn_eps(rs, 0.01554535, 7.06042, 18.0578, -0.32500);
+  const T ea = vwn_eps(rs, -1.0 / (6.0 * kPi * kPi), 1.13107, 13.0045, -0.0047584);
+  const T f = f_zeta(opz, omz);
+  const T z4 = z * z * z * z;
   return eP + ea * f * (1.0 - z4) / kFpp0 + (eF - eP) * f * z4;
 }
 // VWN RPA fit with the plain f(zeta) interpolation (libxc lda_c_vwn_rpa).
-__device__ __forceinline__ P5 vwn_rpa_eps_pol(const P5& rs, const P5& opz, const P5& omz) {
-  const P5 eP = vwn_eps(rs, 0.0310907, 13.0720, 42.7198, -0.409286);
-  const P5 eF = vwn_eps(rs, 0.01554535, 20.1231, 101.578, -0.743294);
+template <class T>
+__device__ __forceinline__ T vwn_rpa_eps_pol(const T& rs, const T& opz, const T& omz) {
+  const T eP = vwn_eps(rs, 0.0310907, 13.0720, 42.7198, -0.409286);
+  const T eF = vwn_eps(rs, 0.01554535, 20.1231, 101.578, -0.743294);
   return eP + (eF - eP) * f_zeta(opz, omz);
 }
 
@@ -489,50 +589,53 @@ __device__ __forceinline__ T pw92_g_drs(const T& rs, double A, double a1, double
   return (-2.0 * A * a1) * dlog(1.0 + 1.0 / den) + (2.0 * A) * (1.0 + a1 * rs) * dden / (den * den + den);
 }
 // PW92 with spin (eq. 8), libxc PW_MOD parameters; -alpha_c = G(rs; A_alpha, ...).
-__device__ __forceinline__ P5 pw92_eps_pol(const P5& rs, const P5& z, const P5& opz, const P5& omz) {
-  const P5 e0 = pw92_g(rs, 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
-  const P5 e1 = pw92_g(rs, 0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
-  const P5 ma = pw92_g(rs, 0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
-  const P5 f = f_zeta(opz, omz);
-  const P5 z4 = z * z * z * z;
+template <class T>
+__device__ __forceinline__ T pw92_eps_pol(const T& rs, const T& z, const T& opz, const T& omz) {
+  const T e0 = pw92_g(rs, 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  const T e1 = pw92_g(rs, 0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  const T ma = pw92_g(rs, 0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
+  const T f = f_zeta(opz, omz);
+  const T z4 = z * z * z * z;
   return e0 - ma * f * (1.0 - z4) / kFpp0 + (e1 - e0) * f * z4;
 }
 
 // PBE correlation with spin: phi = ((1+z)^(2/3) + (1-z)^(2/3))/2,
 // t^2 = |grad rho|^2 / (4 phi^2 ks^2 rho^2), H = gamma phi^3 ln(1 + beta/gamma t^2 (1+At^2)/(1+At^2+A^2t^4)).
-__device__ __forceinline__ P5 e_pbe_c_pol(const P5& rho, const P5& rho13, const P5& gg, const P5& z, const P5& opz,
-                                          const P5& omz, double beta) {
+template <class T>
+__device__ __forceinline__ T e_pbe_c_pol(const T& rho, const T& rho13, const T& gg, const T& z, const T& opz,
+                                         const T& omz, double beta) {
   const double gamma = 0.031090690869654895;
-  const P5 rs = rs_from_rho13(rho13);
-  const P5 ec = pw92_eps_pol(rs, z, opz, omz);
-  const P5 c1 = dcbrt(opz), c2 = dcbrt(omz);
-  const P5 phi = 0.5 * (c1 * c1 + c2 * c2);
-  const P5 phi3 = phi * phi * phi;
-  const P5 kf = 3.0936677262801355 * rho13;
-  const P5 t2 = gg / ((16.0 / kPi) * kf * rho * rho * phi * phi);
-  const P5 A = (beta / gamma) / (dexp(-ec / (gamma * phi3)) - 1.0);
-  const P5 At2 = A * t2;
-  const P5 H = gamma * phi3 * dlog(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  const T rs = rs_from_rho13(rho13);
+  const T ec = pw92_eps_pol(rs, z, opz, omz);
+  const T c1 = dcbrt(opz), c2 = dcbrt(omz);
+  const T phi = 0.5 * (c1 * c1 + c2 * c2);
+  const T phi3 = phi * phi * phi;
+  const T kf = 3.0936677262801355 * rho13;
+  const T t2 = gg / ((16.0 / kPi) * kf * rho * rho * phi * phi);
+  const T A = (beta / gamma) / (dexp(-ec / (gamma * phi3)) - 1.0);
+  const T At2 = A * t2;
+  const T H = gamma * phi3 * dlog(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
   return rho * (ec + H);
 }
 
 // LYP correlation, general spin form (Miehlich, Savin, Stoll, Preuss 1989, eq. 2).
-__device__ __forceinline__ P5 e_lyp_c_pol(const P5& ra, const P5& rb, const P5& rho, const P5& rho13, const P5& gaa,
-                                          const P5& gab, const P5& gbb) {
+template <class T>
+__device__ __forceinline__ T e_lyp_c_pol(const T& ra, const T& rb, const T& rho, const T& rho13, const T& gaa,
+                                         const T& gab, const T& gbb) {
   const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
   const double CF = 2.8712340001881915;
-  const P5 rm13 = 1.0 / rho13;
-  const P5 inv_den = 1.0 / (1.0 + d * rm13);
-  const P5 delta = rm13 * (c + d * inv_den);
-  const P5 omega = dexp(-c * rm13) * inv_den * (rm13 * rm13 / (rho * rho * rho));  // rho^(-11/3)
-  const P5 gg = gaa + 2.0 * gab + gbb;
-  auto p83 = [](const P5& x) {  // x^(8/3), 0 for an empty channel
-    if (!(x.v > 0.0)) return pconst(0.0);
-    const P5 c13 = dcbrt(x);
+  const T rm13 = 1.0 / rho13;
+  const T inv_den = 1.0 / (1.0 + d * rm13);
+  const T delta = rm13 * (c + d * inv_den);
+  const T omega = dexp(-c * rm13) * inv_den * (rm13 * rm13 / (rho * rho * rho));  // rho^(-11/3)
+  const T gg = gaa + 2.0 * gab + gbb;
+  auto p83 = [](const T& x) {  // x^(8/3), 0 for an empty channel
+    if (!(x.v > 0.0)) return tconst<T>(0.0);
+    const T c13 = dcbrt(x);
     return x * x * c13 * c13;
   };
-  const P5 ra83 = p83(ra), rb83 = p83(rb);
-  const P5 inner = ra * rb * (12.699208415745595 * CF * (ra83 + rb83) + (47.0 / 18.0 - (7.0 / 18.0) * delta) * gg -
+  const T ra83 = p83(ra), rb83 = p83(rb);
+  const T inner = ra * rb * (12.699208415745595 * CF * (ra83 + rb83) + (47.0 / 18.0 - (7.0 / 18.0) * delta) * gg -
                               (2.5 - delta / 18.0) * (gaa + gbb) - (delta - 11.0) / 9.0 * (ra / rho * gaa + rb / rho * gbb)) -
                    (2.0 / 3.0) * rho * rho * gg + ((2.0 / 3.0) * rho * rho - ra * ra) * gbb +
                    ((2.0 / 3.0) * rho * rho - rb * rb) * gaa;
@@ -540,33 +643,35 @@ __device__ __forceinline__ P5 e_lyp_c_pol(const P5& ra, const P5& rb, const P5& 
 }
 
 // Exchange by spin scaling: E_x[ra, rb] = (E_x[2 ra] + E_x[2 rb]) / 2.
-__device__ __forceinline__ P5 e_x_channel(int code, const P5& r2, const P5& s4) {
-  const P5 r13 = dcbrt(r2);
+template <class T>
+__device__ __forceinline__ T e_x_channel(int code, const T& r2, const T& s4) {
+  const T r13 = dcbrt(r2);
   switch (code) {
     case kLdaX: return 0.5 * e_lda_x(r2, r13);
     case kGgaXPbe: return 0.5 * e_pbe_x(r2, r13, s4, 0.804, 0.2195149727645171);
     case kGgaXPbeG: return 0.5 * e_pbe_x(r2, r13, s4, 0.804, 0.219510);
     case kGgaXB88: return 0.5 * e_b88_x(r2, r13, s4);
-    default: return pconst(0.0);
+    default: return tconst<T>(0.0);
   }
 }
 
 // Energy density of one LDA/GGA code with spin. ra_x / rb_x: the exchange
 // channels (zero below the threshold, then skipped).
-__device__ __forceinline__ P5 energy_density_pol(int code, const P5& ra, const P5& rb, const P5& saa, const P5& sab,
-                                                 const P5& sbb, bool a_on, bool b_on) {
+template <class T>
+__device__ __forceinline__ T energy_density_pol(int code, const T& ra, const T& rb, const T& saa, const T& sab,
+                                                const T& sbb, bool a_on, bool b_on) {
   auto ex = [&](int xc) {
-    P5 e = pconst(0.0);
+    T e = tconst<T>(0.0);
     if (a_on) e = e + e_x_channel(xc, 2.0 * ra, 4.0 * saa);
     if (b_on) e = e + e_x_channel(xc, 2.0 * rb, 4.0 * sbb);
     return e;
   };
-  const P5 rho = ra + rb;
-  const P5 rho13 = dcbrt(rho);
-  const P5 z = (ra - rb) / rho;
-  P5 opz = 1.0 + z, omz = 1.0 - z;
-  if (opz.v < 1e-15) opz = pconst(1e-15);  // zeta threshold: fully polarised limit
-  if (omz.v < 1e-15) omz = pconst(1e-15);
+  const T rho = ra + rb;
+  const T rho13 = dcbrt(rho);
+  const T z = (ra - rb) / rho;
+  T opz = 1.0 + z, omz = 1.0 - z;
+  if (opz.v < 1e-15) opz = tconst<T>(1e-15);  // zeta threshold: fully polarised limit
+  if (omz.v < 1e-15) omz = tconst<T>(1e-15);
   switch (code) {
     case kLdaX: case kGgaXPbe: case kGgaXPbeG: case kGgaXB88: return ex(code);
     case kLdaCVwn: return rho * vwn5_eps_pol(rs_from_rho13(rho13), z, opz, omz);
@@ -577,7 +682,7 @@ __device__ __forceinline__ P5 energy_density_pol(int code, const P5& ra, const P
     case kHybB3lyp:
       return 0.08 * ex(kLdaX) + 0.72 * ex(kGgaXB88) + 0.19 * (rho * vwn_rpa_eps_pol(rs_from_rho13(rho13), opz, omz)) +
              0.81 * e_lyp_c_pol(ra, rb, rho, rho13, saa, sab, sbb);
-    default: return pconst(0.0);
+    default: return tconst<T>(0.0);
   }
 }
 
@@ -614,6 +719,45 @@ __device__ __forceinline__ void eval_codes_pol(const int* codes, int ncodes, dou
   vs[0] = d[2];
   vs[1] = d[3];
   vs[2] = d[4];
+}
+
+// Second derivatives of the summed polarised LDA/GGA energy density at one
+// point (libxc xc_{lda,gga}_fxc with XC_POLARIZED, as Evaluate("f") copies them,
+// src/ExchangeCorrelation.cpp:126-133,232-245), with the same thresholds,
+// floors and caps as eval_codes_pol. f[18]: vsigma (aa, ab, bb), v2rho2 (a a,
+// a b, b b), v2rhosigma (a aa, a ab, a bb, b aa, b ab, b bb), v2sigma2 (aa aa,
+// aa ab, aa bb, ab ab, ab bb, bb bb).
+__device__ __forceinline__ void eval_codes_fpol(const int* codes, int ncodes, double ra, double rb, double saa,
+                                                double sab, double sbb, double* f) {
+#pragma unroll
+  for (int k = 0; k < 18; ++k) f[k] = 0.0;
+  ra = ra > 0 ? ra : 0.0;
+  rb = rb > 0 ? rb : 0.0;
+  if (!(ra + rb >= kDensThreshold)) return;
+  const double sfl = 1e-40;
+  saa = saa > sfl ? saa : sfl;
+  sbb = sbb > sfl ? sbb : sfl;
+  const double cap = ::sqrt(saa * sbb);
+  sab = sab > cap ? cap : (sab < -cap ? -cap : sab);
+  const H5 pa = h5var(ra, 0), pb = h5var(rb, 1), paa = h5var(saa, 2), pab = h5var(sab, 3), pbb = h5var(sbb, 4);
+  const bool a_on = ra >= kDensThreshold, b_on = rb >= kDensThreshold;
+  for (int k = 0; k < ncodes; ++k) {
+    const H5 e = energy_density_pol(codes[k], pa, pb, paa, pab, pbb, a_on, b_on);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) f[x] += e.g[2 + x];
+    f[3] += e.h[hix(0, 0)];
+    f[4] += e.h[hix(0, 1)];
+    f[5] += e.h[hix(1, 1)];
+#pragma unroll
+    for (int w = 0; w < 2; ++w)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) f[6 + 3 * w + x] += e.h[hix(w, 2 + x)];
+    int q = 12;
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int y = x; y < 3; ++y) f[q++] += e.h[hix(2 + x, 2 + y)];
+  }
 }
 
 // ---- spin-polarised r2SCAN (libxc 497 / 498, XC_POLARIZED conventions) -----

@@ -138,7 +138,8 @@ int chx_grid_fock_device(chx_grid* g, double* F_dev, int nspin, int nbasis);
 int chx_grid_get_density_skeleton(chx_grid* g, const double* D, int nspin, int nbasis);
 int chx_grid_energy_grad(chx_grid* g, double* grad);
 /* Response path (coupled-perturbed KS, src/HartreeFockKohnSham/CoupledPerturbed.cpp:133-134,
- * Universal.h:92-118), closed shell, LDA/GGA, FP64 form:
+ * Universal.h:92-118), LDA/GGA, FP64 form, nspin = 1 or 2 (the polarised
+ * f_xc from a polarised Evaluate("f") after getDensity({Da, Db})):
  * chx_xc_evaluate(xc, g, "f") after the ground-state density takes f_xc
  * (Evaluate("f"), src/ExchangeCorrelation.cpp:101-104,126-133);
  * chx_grid_get_density_u replaces Grid::getDensityU (src/Grid/Grid.cpp:63-78,
@@ -146,7 +147,14 @@ int chx_grid_energy_grad(chx_grid* g, double* grad);
  * column-major nbasis^2 matrices, back to back, [spin][mat] (host memory);
  * chx_grid_fock_u replaces Grid::getFockU<u_t> (src/Grid/Grid.cpp:193-216,
  * SubGrid::getFockU, src/Grid/GridFock.cpp:99-160) and writes the nspin x
- * nmats response matrices in the same order. */
+ * nmats response matrices in the same order. At nspin = 2 the matrix of spin
+ * u for perturbation m is the derivative of the polarised ground-state Fock
+ * matrix of spin u along (Dx_a[m], Dx_b[m]): GridFock.cpp:99-160 with the
+ * sigma-sigma term scaled by the factor of the differentiated sigma (2 on aa
+ * and bb, 1 on ab), where the reference applies the factor of the contracted
+ * sigma (FockEinSum/S_x...Eps2Sigma2_g,w,x---ScaledEps2Sigma2_g,w,x.hpp); the
+ * two agree for LDA and whenever f_{sigma sigma} has no aa-ab or ab-bb
+ * coupling (B88, LYP), and differ for PBE correlation. */
 int chx_grid_get_density_u(chx_grid* g, const double* D, int nmats, int nspin, int nbasis);
 int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis);
 /* Grid::getFockU<s_t> (src/Grid/Grid.cpp:193-216, SubGrid::getFockU with the
@@ -270,7 +278,11 @@ int chx_xc_eval_points_noalloc(const chx_xc* xc, long n, const double* rho, cons
                                double* exc, double* vrho, double* vsigma, double* vtau);
 /* Second derivatives at n points, as xc_{lda,gga}_fxc returns them
  * (src/ExchangeCorrelation.cpp:101-104,126-133): v2rho2, v2rhosigma, v2sigma2
- * summed over the codes. LDA/GGA; host arrays; sigma may be NULL (LDA). */
+ * summed over the codes. LDA/GGA; host arrays; sigma may be NULL (LDA). A
+ * polarised functional (spin 2) takes libxc's XC_POLARIZED layout: rho[2n]
+ * (a, b), sigma[3n] (aa, ab, bb) in; v2rho2[3n] (a a, a b, b b),
+ * v2rhosigma[6n] (a aa, a ab, a bb, b aa, b ab, b bb), v2sigma2[6n]
+ * (aa aa, aa ab, aa bb, ab ab, ab bb, bb bb) out. */
 int chx_xc_eval_points_f(const chx_xc* xc, long n, const double* rho, const double* sigma, double* v2rho2,
                          double* v2rhosigma, double* v2sigma2);
 

@@ -363,9 +363,16 @@ inline void eval_point_f(int code, double rho, double sigma, double* vsigma, dou
 // spin interpolations.
 typedef Dual<5> P5d;
 
+// 1 + zeta and 1 - zeta enter each power clamped at kZetaThr from below (a
+// constant there, as libxc's piecewise opz_pow_n), so a fully polarised point
+// keeps the derivatives of the other term.
+constexpr double kZetaThr = 1e-15;
+template <class T> T opz_pow(const T& x, double n) {
+  return x.v > kZetaThr ? pow(x, n) : T(std::pow(kZetaThr, n));
+}
 template <class T> T spin_f(const T& z) {  // f(zeta) = ((1+z)^(4/3) + (1-z)^(4/3) - 2) / (2^(4/3) - 2)
   const T a = 1.0 + z, b = 1.0 - z;
-  auto p43 = [](const T& x) { return x.v > 0 ? pow(x, 4.0 / 3.0) : T(0.0); };
+  auto p43 = [](const T& x) { return opz_pow(x, 4.0 / 3.0); };
   return (p43(a) + p43(b) - 2.0) / (std::pow(2.0, 4.0 / 3.0) - 2.0);
 }
 inline double spin_fpp0() { return 4.0 / (9.0 * (std::cbrt(2.0) - 1.0)); }
@@ -374,14 +381,14 @@ inline double spin_fpp0() { return 4.0 / (9.0 * (std::cbrt(2.0) - 1.0)); }
 // interpolation, libxc 8 (RPA) the plain f(zeta) one.
 constexpr VwnPar kVwn5F = {0.01554535, 7.06042, 18.0578, -0.32500};
 constexpr VwnPar kVwnRpaF = {0.01554535, 20.1231, 101.578, -0.743294};
-inline P5d vwn5_pol_eps(const P5d& rs, const P5d& z) {
+template <class T> T vwn5_pol_eps(const T& rs, const T& z) {
   const VwnPar stiff = {-1.0 / (6.0 * kPi * kPi), 1.13107, 13.0045, -0.0047584};
-  const P5d eP = vwn_eps(rs, kVwn5), eF = vwn_eps(rs, kVwn5F), ac = vwn_eps(rs, stiff);
-  const P5d f = spin_f(z), z4 = z * z * z * z;
+  const T eP = vwn_eps(rs, kVwn5), eF = vwn_eps(rs, kVwn5F), ac = vwn_eps(rs, stiff);
+  const T f = spin_f(z), z4 = z * z * z * z;
   return eP + ac * f * (1.0 - z4) / spin_fpp0() + (eF - eP) * f * z4;
 }
-inline P5d vwnrpa_pol_eps(const P5d& rs, const P5d& z) {
-  const P5d eP = vwn_eps(rs, kVwnRpa), eF = vwn_eps(rs, kVwnRpaF);
+template <class T> T vwnrpa_pol_eps(const T& rs, const T& z) {
+  const T eP = vwn_eps(rs, kVwnRpa), eF = vwn_eps(rs, kVwnRpaF);
   return eP + (eF - eP) * spin_f(z);
 }
 
@@ -413,32 +420,32 @@ template <class T> T pw92_pol_eps(const T& rs, const T& z, T* drs = nullptr) {
 }
 
 // PBE correlation with spin (PRL 77, 3865, eqs. 3, 7-8).
-inline P5d pbe_c_pol(const P5d& rho, const P5d& z, const P5d& gg, double beta) {
+template <class T> T pbe_c_pol(const T& rho, const T& z, const T& gg, double beta) {
   const double gamma = (1.0 - std::log(2.0)) / (kPi * kPi);
-  const P5d ec = pw92_pol_eps(rs_of(rho), z);
-  auto p23 = [](const P5d& x) { return x.v > 0 ? pow(x, 2.0 / 3.0) : P5d(0.0); };
-  const P5d phi = 0.5 * (p23(1.0 + z) + p23(1.0 - z));
-  const P5d kf = pow(3.0 * kPi * kPi * rho, 1.0 / 3.0);
-  const P5d ks2 = 4.0 * kf / kPi;
-  const P5d t2 = gg / (4.0 * phi * phi * ks2 * rho * rho);
-  const P5d g3 = gamma * phi * phi * phi;
-  const P5d A = (beta / gamma) / (exp(-ec / g3) - 1.0);
-  const P5d At2 = A * t2;
+  const T ec = pw92_pol_eps(rs_of(rho), z);
+  auto p23 = [](const T& x) { return opz_pow(x, 2.0 / 3.0); };
+  const T phi = 0.5 * (p23(1.0 + z) + p23(1.0 - z));
+  const T kf = pow(3.0 * kPi * kPi * rho, 1.0 / 3.0);
+  const T ks2 = 4.0 * kf / kPi;
+  const T t2 = gg / (4.0 * phi * phi * ks2 * rho * rho);
+  const T g3 = gamma * phi * phi * phi;
+  const T A = (beta / gamma) / (exp(-ec / g3) - 1.0);
+  const T At2 = A * t2;
   return rho * (ec + g3 * log(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2)));
 }
 
 // LYP, general spin form (CPL 157, 200, eq. 2).
-inline P5d lyp_pol(const P5d& ra, const P5d& rb, const P5d& gaa, const P5d& gab, const P5d& gbb) {
+template <class T> T lyp_pol(const T& ra, const T& rb, const T& gaa, const T& gab, const T& gbb) {
   const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
   const double CF = 0.3 * std::pow(3.0 * kPi * kPi, 2.0 / 3.0);
-  const P5d rho = ra + rb;
-  const P5d rm13 = pow(rho, -1.0 / 3.0);
-  const P5d den = 1.0 + d * rm13;
-  const P5d omega = exp(-c * rm13) / den * pow(rho, -11.0 / 3.0);
-  const P5d delta = c * rm13 + d * rm13 / den;
-  const P5d g = gaa + gbb + 2.0 * gab;
-  auto p83 = [](const P5d& x) { return x.v > 0 ? pow(x, 8.0 / 3.0) : P5d(0.0); };
-  const P5d bracket = ra * rb * (std::pow(2.0, 11.0 / 3.0) * CF * (p83(ra) + p83(rb)) +
+  const T rho = ra + rb;
+  const T rm13 = pow(rho, -1.0 / 3.0);
+  const T den = 1.0 + d * rm13;
+  const T omega = exp(-c * rm13) / den * pow(rho, -11.0 / 3.0);
+  const T delta = c * rm13 + d * rm13 / den;
+  const T g = gaa + gbb + 2.0 * gab;
+  auto p83 = [](const T& x) { return x.v > 0 ? pow(x, 8.0 / 3.0) : T(0.0); };
+  const T bracket = ra * rb * (std::pow(2.0, 11.0 / 3.0) * CF * (p83(ra) + p83(rb)) +
                                  (47.0 / 18.0 - 7.0 / 18.0 * delta) * g - (2.5 - delta / 18.0) * (gaa + gbb) -
                                  (delta - 11.0) / 9.0 * (ra / rho * gaa + rb / rho * gbb)) -
                       2.0 / 3.0 * rho * rho * g + (2.0 / 3.0 * rho * rho - ra * ra) * gbb +
@@ -447,30 +454,28 @@ inline P5d lyp_pol(const P5d& ra, const P5d& rb, const P5d& gaa, const P5d& gab,
 }
 
 // One exchange channel of spin density r_s with |grad r_s|^2 = g_s: E_x[2 r_s, 4 g_s] / 2.
-inline P5d x_channel(int code, const P5d& r_s, const P5d& g_s) {
-  const P5d r2 = 2.0 * r_s, g4 = 4.0 * g_s;
+template <class T> T x_channel(int code, const T& r_s, const T& g_s) {
+  const T r2 = 2.0 * r_s, g4 = 4.0 * g_s;
   switch (code) {
     case 1: return 0.5 * lda_x(r2);
     case 101: return 0.5 * gga_x_pbe(r2, g4, 0.804, 0.2195149727645171);
     case 321: return 0.5 * gga_x_pbe(r2, g4, 0.804, 0.219510);
     case 106: return 0.5 * gga_x_b88(r2, g4);
-    default: return P5d(0.0);
+    default: return T(0.0);
   }
 }
 
-inline P5d energy_density_pol(int code, const P5d& ra, const P5d& rb, const P5d& gaa, const P5d& gab,
-                              const P5d& gbb, bool a_on, bool b_on) {
+template <class T> T energy_density_pol(int code, const T& ra, const T& rb, const T& gaa, const T& gab, const T& gbb,
+                                        bool a_on, bool b_on) {
   auto xs = [&](int c) {
-    P5d e(0.0);
+    T e(0.0);
     if (a_on) e = e + x_channel(c, ra, gaa);
     if (b_on) e = e + x_channel(c, rb, gbb);
     return e;
   };
-  const P5d rho = ra + rb;
-  P5d z = (ra - rb) / rho;
-  if (z.v > 1.0 - 1e-15) z = P5d(1.0 - 1e-15);   // zeta threshold
-  if (z.v < -1.0 + 1e-15) z = P5d(-1.0 + 1e-15);
-  const P5d g = gaa + gbb + 2.0 * gab;
+  const T rho = ra + rb;
+  const T z = (ra - rb) / rho;  // |z| <= 1; each power of 1 +- z is clamped (opz_pow)
+  const T g = gaa + gbb + 2.0 * gab;
   switch (code) {
     case 1: case 101: case 321: case 106: return xs(code);
     case 7: return rho * vwn5_pol_eps(rs_of(rho), z);
@@ -480,7 +485,7 @@ inline P5d energy_density_pol(int code, const P5d& ra, const P5d& rb, const P5d&
     case 131: return lyp_pol(ra, rb, gaa, gab, gbb);
     case 402:
       return 0.08 * xs(1) + 0.72 * xs(106) + 0.19 * (rho * vwnrpa_pol_eps(rs_of(rho), z)) + 0.81 * lyp_pol(ra, rb, gaa, gab, gbb);
-    default: return P5d(0.0);
+    default: return T(0.0);
   }
 }
 
@@ -506,6 +511,35 @@ inline void eval_point_pol(int code, double ra, double rb, double saa, double sa
   vsigma[0] += e.d[2];
   vsigma[1] += e.d[3];
   vsigma[2] += e.d[4];
+}
+
+// Second derivatives of one code at one point, spin-polarised (libxc
+// xc_{lda,gga}_fxc with XC_POLARIZED, the layout Evaluate("f") copies,
+// src/ExchangeCorrelation.cpp:126-133,232-245): accumulates vsigma[3] (aa, ab,
+// bb), v2rho2[3] (a a, a b, b b), v2rhosigma[6] (a aa, a ab, a bb, b aa, b ab,
+// b bb) and v2sigma2[6] (aa aa, aa ab, aa bb, ab ab, ab bb, bb bb). Same
+// thresholds, floors and caps as eval_point_pol.
+inline void eval_point_fpol(int code, double ra, double rb, double saa, double sab, double sbb, double* vsigma,
+                            double* v2rho2, double* v2rhosigma, double* v2sigma2) {
+  ra = ra > 0 ? ra : 0;
+  rb = rb > 0 ? rb : 0;
+  const double thr = density_threshold(code);
+  if (!(ra + rb >= thr)) return;
+  saa = std::max(saa, 1e-40);
+  sbb = std::max(sbb, 1e-40);
+  const double cap = std::sqrt(saa * sbb);
+  sab = std::min(std::max(sab, -cap), cap);
+  typedef Hyper<5> H5;
+  const H5 e = energy_density_pol(code, H5::var(ra, 0), H5::var(rb, 1), H5::var(saa, 2), H5::var(sab, 3),
+                                  H5::var(sbb, 4), ra >= thr, rb >= thr);
+  for (int x = 0; x < 3; ++x) vsigma[x] += e.g[2 + x];
+  v2rho2[0] += e.h[0][0];
+  v2rho2[1] += e.h[0][1];
+  v2rho2[2] += e.h[1][1];
+  for (int w = 0; w < 2; ++w)
+    for (int x = 0; x < 3; ++x) v2rhosigma[3 * w + x] += e.h[w][2 + x];
+  for (int x = 0, k = 0; x < 3; ++x)
+    for (int y = x; y < 3; ++y, ++k) v2sigma2[k] += e.h[2 + x][2 + y];
 }
 
 // ---- Spin-polarised r2SCAN (libxc 497 / 498 with XC_POLARIZED): variables

@@ -1029,6 +1029,146 @@ int ox_eval_functional_mpol(int ncodes, const int* codes, long n, const double* 
   return 0;
 }
 
+// Spin-polarised response (Evaluate("f") with the functional polarised, then
+// Grid::getDensityU / Grid::getFockU<u_t> at nspin = 2: GridDensity.cpp:47-77,
+// GridFock.cpp:99-160), LDA/GGA, ground state Da, Db. Per perturbation m and
+// spin u, from rho^x_s, grad rho^x_s of Dxa[m], Dxb[m]:
+//   sigma^x_aa = 2 grad rho_a . grad rho^x_a, sigma^x_ab = grad rho^x_a . grad rho_b
+//     + grad rho_a . grad rho^x_b, sigma^x_bb = 2 grad rho_b . grad rho^x_b (SigmaU);
+//   dv_rho_u = sum_v f_{rho_u rho_v} rho^x_v + sum_x f_{rho_u sigma_x} sigma^x_x;
+//   dv_sigma_x = sum_w f_{rho_w sigma_x} rho^x_w + sum_y f_{sigma_x sigma_y} sigma^x_y;
+//   F^x_u = 1/2 F0[w dv_rho_u] + F1[w (S(uu) dv_sigma_uu grad rho_u + dv_sigma_ab grad rho_o
+//           + S(uu) v_sigma_uu grad rho^x_u + v_sigma_ab grad rho^x_o)], F += F^T,
+// with S = 2 on aa and bb and 1 on ab: the derivative of the polarised
+// ground-state Fock matrix (GridFock.cpp:16-41 at nspin = 2) along D^x. The
+// reference scales f_{sigma sigma} by S of the index it contracts with
+// SigmaU (FockEinSum/S_x...Eps2Sigma2_g,w,x---ScaledEps2Sigma2_g,w,x.hpp with
+// ScaledEps2Sigma2_g,u,v...SigmaG_g,mat,v---TMP1_g,mat,u.hpp), which differs
+// from the derivative in the aa-ab and ab-bb cross terms; this follows the
+// derivative. Evaluated through fock_subgrid (vrho := dv_rho_u, vsigma := 1,
+// rho1 := the F1 vector / 2).
+int ox_fock_u_pol(ox_system* s, const double* Da, const double* Db, int nmats, const double* Dxa, const double* Dxb,
+                  double* Fa, double* Fb) {
+  return guarded([&] {
+    if (!s->ao_ready) throw std::runtime_error("oracle: AO tables not ready");
+    if (s->family > 1) throw std::runtime_error("oracle: the response path is LDA/GGA");
+    const bool gga = s->family >= 1;
+    const int nb = s->nbasis, n = (int)s->active.size(), nt = s->nthreads;
+    const size_t nb2 = (size_t)nb * nb;
+    std::vector<std::vector<double>> part(nt, std::vector<double>((size_t)2 * nmats * nb2, 0.0));
+#pragma omp parallel num_threads(nt)
+    {
+      std::vector<double> F, tmp, tmp3, ga, gb, fx, xa, xb, gxa, gxb;
+      double* mine = part[omp_get_thread_num()].data();
+#pragma omp for schedule(dynamic, 1)
+      for (int i = 0; i < n; ++i) {
+        SubGrid& sg = s->subgrids[s->active[i]];
+        const int ng = sg.npad;
+        const std::vector<double> zero3((size_t)3 * ng, 0.0);
+        density_subgrid(s, sg, Da, s->family, tmp);
+        const std::vector<double> ra = sg.rho;
+        ga = gga ? sg.rho1 : zero3;
+        density_subgrid(s, sg, Db, s->family, tmp);
+        const std::vector<double> rb = sg.rho;
+        gb = gga ? sg.rho1 : zero3;
+        // per point: vsigma[3], v2rho2[3], v2rhosigma[6], v2sigma2[6]
+        fx.assign((size_t)18 * ng, 0.0);
+        for (int g = 0; g < ng; ++g) {
+          double saa = 0, sab = 0, sbb = 0;
+          for (int r = 0; r < 3; ++r) {
+            const double a = ga[(size_t)r * ng + g], b = gb[(size_t)r * ng + g];
+            saa += a * a; sab += a * b; sbb += b * b;
+          }
+          double* f = &fx[(size_t)18 * g];
+          for (int code : s->codes) oxf::eval_point_fpol(code, ra[g], rb[g], saa, sab, sbb, f, f + 3, f + 6, f + 12);
+          if (!gga) std::fill(f, f + 3, 0.0), std::fill(f + 6, f + 18, 0.0);
+        }
+        for (int m = 0; m < nmats; ++m) {
+          density_subgrid(s, sg, Dxa + m * nb2, s->family, tmp);
+          xa = sg.rho;
+          gxa = gga ? sg.rho1 : zero3;
+          density_subgrid(s, sg, Dxb + m * nb2, s->family, tmp);
+          xb = sg.rho;
+          gxb = gga ? sg.rho1 : zero3;
+          std::vector<double> dvr((size_t)2 * ng), vec((size_t)2 * 3 * ng, 0.0);
+          for (int g = 0; g < ng; ++g) {
+            const double* f = &fx[(size_t)18 * g];
+            double sx[3] = {0, 0, 0};
+            for (int r = 0; r < 3; ++r) {
+              const size_t k = (size_t)r * ng + g;
+              sx[0] += 2.0 * ga[k] * gxa[k];
+              sx[1] += gxa[k] * gb[k] + ga[k] * gxb[k];
+              sx[2] += 2.0 * gb[k] * gxb[k];
+            }
+            const double rx[2] = {xa[g], xb[g]};
+            auto frr = [&](int u, int v) { return f[3 + (u + v)]; };  // aa, ab, bb
+            auto frs = [&](int w, int x) { return f[6 + 3 * w + x]; };
+            auto fss = [&](int x, int y) {
+              const int a = std::min(x, y), b = std::max(x, y);
+              static const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+              return f[12 + idx[a][b]];
+            };
+            double dvs[3];
+            for (int x = 0; x < 3; ++x) {
+              dvs[x] = frs(0, x) * rx[0] + frs(1, x) * rx[1];
+              for (int y = 0; y < 3; ++y) dvs[x] += fss(x, y) * sx[y];
+            }
+            for (int u = 0; u < 2; ++u) {
+              double d = frr(u, 0) * rx[0] + frr(u, 1) * rx[1];
+              for (int x = 0; x < 3; ++x) d += frs(u, x) * sx[x];
+              dvr[(size_t)u * ng + g] = d;
+              if (!gga) continue;
+              const int uu = u == 0 ? 0 : 2;
+              const std::vector<double>& gu = u == 0 ? ga : gb;
+              const std::vector<double>& go = u == 0 ? gb : ga;
+              const std::vector<double>& gxu = u == 0 ? gxa : gxb;
+              const std::vector<double>& gxo = u == 0 ? gxb : gxa;
+              for (int r = 0; r < 3; ++r) {
+                const size_t k = (size_t)r * ng + g;
+                vec[(size_t)u * 3 * ng + k] =
+                    0.5 * (2.0 * dvs[uu] * gu[k] + dvs[1] * go[k] + 2.0 * f[uu] * gxu[k] + f[1] * gxo[k]);
+              }
+            }
+          }
+          for (int u = 0; u < 2; ++u) {
+            sg.vrho.assign(dvr.begin() + (size_t)u * ng, dvr.begin() + (size_t)(u + 1) * ng);
+            sg.vsigma.assign(ng, 1.0);
+            if (gga) sg.rho1.assign(vec.begin() + (size_t)u * 3 * ng, vec.begin() + (size_t)(u + 1) * 3 * ng);
+            fock_subgrid(s, sg, mine + ((size_t)u * nmats + m) * nb2, F, tmp, tmp3);
+          }
+        }
+      }
+    }
+    std::memset(Fa, 0, sizeof(double) * nmats * nb2);
+    std::memset(Fb, 0, sizeof(double) * nmats * nb2);
+    for (int t = 0; t < nt; ++t)
+      for (size_t k = 0; k < (size_t)nmats * nb2; ++k) {
+        Fa[k] += part[t][k];
+        Fb[k] += part[t][(size_t)nmats * nb2 + k];
+      }
+  });
+}
+
+// Polarised f_xc of the summed LDA/GGA codes at n points (libxc XC_POLARIZED
+// layout): rho[2n], sigma[3n] in; vsigma[3n], v2rho2[3n], v2rhosigma[6n],
+// v2sigma2[6n] out (each may be null).
+int ox_eval_functional_fpol(int ncodes, const int* codes, long n, const double* rho, const double* sigma,
+                            double* vsigma, double* v2rho2, double* v2rhosigma, double* v2sigma2) {
+  for (long i = 0; i < n; ++i) {
+    double f[18] = {0};
+    const double s0 = sigma ? sigma[3 * i] : 0.0, s1 = sigma ? sigma[3 * i + 1] : 0.0, s2 = sigma ? sigma[3 * i + 2] : 0.0;
+    for (int k = 0; k < ncodes; ++k) {
+      if (oxf::family_of(codes[k]) > 1) { g_err = "oracle: the spin-polarised f_xc is LDA/GGA"; return -1; }
+      oxf::eval_point_fpol(codes[k], rho[2 * i], rho[2 * i + 1], s0, s1, s2, f, f + 3, f + 6, f + 12);
+    }
+    if (vsigma) for (int k = 0; k < 3; ++k) vsigma[3 * i + k] = f[k];
+    if (v2rho2) for (int k = 0; k < 3; ++k) v2rho2[3 * i + k] = f[3 + k];
+    if (v2rhosigma) for (int k = 0; k < 6; ++k) v2rhosigma[6 * i + k] = f[6 + k];
+    if (v2sigma2) for (int k = 0; k < 6; ++k) v2sigma2[6 * i + k] = f[12 + k];
+  }
+  return 0;
+}
+
 int ox_eval_functional_pol(int ncodes, const int* codes, long n, const double* rho, const double* sigma, double* exc,
                            double* vrho, double* vsigma) {
   for (long i = 0; i < n; ++i) {

@@ -121,6 +121,14 @@ int ox_xc_pol(ox_system* s, const double* Da, const double* Db, double* Va, doub
  * rho[2n], sigma[3n] (aa, ab, bb), tau[2n] -> exc[n], vrho[2n], vsigma[3n], vtau[2n]. */
 int ox_eval_functional_mpol(int ncodes, const int* codes, long n, const double* rho, const double* sigma,
                             const double* tau, double* exc, double* vrho, double* vsigma, double* vtau);
+/* Spin-polarised response, LDA/GGA: ground state Da, Db; nmats perturbed matrices per spin
+ * (Dxa, Dxb, each nmats x nbasis^2, column-major); response Fock matrices Fa, Fb in the same
+ * layout. The derivative of the polarised Fock matrix along D^x. */
+int ox_fock_u_pol(ox_system* s, const double* Da, const double* Db, int nmats, const double* Dxa, const double* Dxb,
+                  double* Fa, double* Fb);
+/* Polarised f_xc at n points: rho[2n], sigma[3n] -> vsigma[3n], v2rho2[3n], v2rhosigma[6n], v2sigma2[6n]. */
+int ox_eval_functional_fpol(int ncodes, const int* codes, long n, const double* rho, const double* sigma,
+                            double* vsigma, double* v2rho2, double* v2rhosigma, double* v2sigma2);
 int ox_eval_functional_pol(int ncodes, const int* codes, long n, const double* rho, const double* sigma, double* exc,
                            double* vrho, double* vsigma);
 

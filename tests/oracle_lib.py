@@ -61,6 +61,8 @@ def lib():
         "ox_fock_s": (ctypes.c_int, [_vp, _vp, _vp]),
         "ox_xc_pol": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_pol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
+        "ox_eval_functional_fpol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "ox_fock_u_pol": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_mpol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
                                                    _vp]),
         "ox_mlp_forward": (ctypes.c_int, [ctypes.c_long, _vp, _vp, _vp, _vp]),
@@ -177,6 +179,19 @@ class OracleSystem:
         F = np.zeros_like(X)
         _check(lib().ox_fock_u(self.h, len(Dxs), _ptr(X), _ptr(F)))
         return [F[m].reshape((n, n), order="F") for m in range(len(Dxs))]
+
+    def fock_u_pol(self, Da, Db, Dxa, Dxb):
+        """Spin-polarised response matrices (Fa[m], Fb[m]) for the perturbed pairs (Dxa[m], Dxb[m])
+        at the ground state (Da, Db): Evaluate("f") polarised, getDensityU, getFockU<u_t>."""
+        n, m = self.nbasis, len(Dxa)
+        A = np.asfortranarray(Da, dtype=np.float64)
+        B = np.asfortranarray(Db, dtype=np.float64)
+        XA = np.stack([np.asfortranarray(d, dtype=np.float64).ravel(order="F") for d in Dxa])
+        XB = np.stack([np.asfortranarray(d, dtype=np.float64).ravel(order="F") for d in Dxb])
+        FA, FB = np.zeros_like(XA), np.zeros_like(XB)
+        _check(lib().ox_fock_u_pol(self.h, _ptr(A), _ptr(B), m, _ptr(XA), _ptr(XB), _ptr(FA), _ptr(FB)))
+        return ([FA[k].reshape((n, n), order="F") for k in range(m)],
+                [FB[k].reshape((n, n), order="F") for k in range(m)])
 
     def fock_s(self, D):
         """Skeleton XC response matrices, one per nuclear coordinate [3 atom + t], at the
@@ -322,6 +337,19 @@ def eval_functional_pol(codes, rho, sigma):
     e, vr, vs = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
     _check(lib().ox_eval_functional_pol(len(cc), _ptr(cc), n, _ptr(rho), _ptr(sigma), _ptr(e), _ptr(vr), _ptr(vs)))
     return e, vr, vs
+
+
+def eval_functional_fpol(codes, rho, sigma):
+    """Spin-polarised f_xc: vsigma (n x 3), v2rho2 (n x 3), v2rhosigma (n x 6), v2sigma2 (n x 6) in libxc's
+    XC_POLARIZED order; rho (n x 2), sigma (n x 3)."""
+    cc = np.ascontiguousarray(codes, dtype=np.int32)
+    rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1, 3)
+    n = rho.shape[0]
+    vs, frr, frs, fss = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 6)), np.zeros((n, 6))
+    _check(lib().ox_eval_functional_fpol(len(cc), _ptr(cc), n, _ptr(rho), _ptr(sigma), _ptr(vs), _ptr(frr), _ptr(frs),
+                                         _ptr(fss)))
+    return vs, frr, frs, fss
 
 
 def eval_functional_mpol(codes, rho, sigma, tau):

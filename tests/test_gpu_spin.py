@@ -118,9 +118,12 @@ def test_spin_protocol_errors():
     with pytest.raises(ca.ChxError) as e:  # the GGA grid holds no tau
         r2.Evaluate("ev", g)
     assert e.value.code == -3
-    pol = _xc("PBE")
-    with pytest.raises(ca.ChxError) as e:  # response path is closed shell
-        pol.Evaluate("f", g)
+    g2 = ca.Grid(mol, "SG-1", 1, 0)
+    g2.setType(2)
+    g2.getAO(0, 0)
+    g2.getDensity([Da, Db])
+    with pytest.raises(ca.ChxError) as e:  # f_xc is built for LDA and GGA (the reference's mGGA has no "f")
+        r2.Evaluate("f", g2)
     assert e.value.code == -5
     with pytest.raises(ca.ChxError) as e:  # three spin matrices
         g.getDensity([Da, Db, Da])
