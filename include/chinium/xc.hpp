@@ -167,17 +167,20 @@ class Grid {
     check(chx_grid_energy_grad(h_, g.data()));
     return g;
   }
-  // Response path, closed shell, LDA/GGA, FP64 form: Grid::getDensityU
+  // Response path, LDA/GGA, FP64 form, nspin 1 or 2: Grid::getDensityU
   // (src/Grid/Grid.cpp:63-78) takes D[spin][mat]; Grid::getFockU<u_t>
   // (src/Grid/Grid.cpp:193-216) returns F[spin][mat]. Evaluate("f") first.
   void getDensityU(std::vector<std::vector<EigenMatrix>> D) {
     const int nspin = (int)D.size(), nmats = nspin ? (int)D[0].size() : 0;
+    for (const auto& spin : D)
+      if ((int)spin.size() != nmats) throw std::runtime_error("Inconsistent number of spin types!");
     std::vector<double> buf;
     buf.reserve((size_t)nspin * nmats * nbasis_ * nbasis_);
     for (const auto& spin : D)
       for (const EigenMatrix& m : spin) buf.insert(buf.end(), m.data(), m.data() + (size_t)nbasis_ * nbasis_);
     check(chx_grid_get_density_u(h_, buf.data(), nmats, nspin, nbasis_));
     nmats_u_ = nmats;
+    nspin_u_ = nspin;
   }
   // D_t = u_t: the matrices of the last getDensityU. D_t = s_t (the Hessian's
   // skeleton term, Restricted/Hess.cpp:96): one matrix per nuclear coordinate,
@@ -185,17 +188,20 @@ class Grid {
   template <D_t d_t = u_t>
   std::vector<std::vector<EigenMatrix>> getFockU() {
     const int nmats = d_t == s_t ? 3 * natoms_ : nmats_u_;
-    std::vector<double> buf((size_t)nmats * nbasis_ * nbasis_);
+    const int nspin = d_t == s_t ? 1 : nspin_u_;
+    std::vector<double> buf((size_t)nspin * nmats * nbasis_ * nbasis_);
     if (d_t == s_t)
       check(chx_grid_fock_s(h_, buf.data(), 1, nbasis_));
     else
-      check(chx_grid_fock_u(h_, buf.data(), nmats, 1, nbasis_));
-    std::vector<std::vector<EigenMatrix>> F(1);
-    for (int m = 0; m < nmats; ++m) {
-      F[0].push_back(make_matrix(nbasis_, nbasis_));
-      std::copy(buf.begin() + (size_t)m * nbasis_ * nbasis_, buf.begin() + (size_t)(m + 1) * nbasis_ * nbasis_,
-                F[0].back().data());
-    }
+      check(chx_grid_fock_u(h_, buf.data(), nmats, nspin, nbasis_));
+    std::vector<std::vector<EigenMatrix>> F(nspin);
+    for (int u = 0; u < nspin; ++u)
+      for (int m = 0; m < nmats; ++m) {
+        const size_t k = (size_t)u * nmats + m;
+        F[u].push_back(make_matrix(nbasis_, nbasis_));
+        std::copy(buf.begin() + k * nbasis_ * nbasis_, buf.begin() + (k + 1) * nbasis_ * nbasis_,
+                  F[u].back().data());
+      }
     return F;
   }
   // GPU options (no reference counterpart): CHX_PREC_FP64 or CHX_PREC_MIXED.
@@ -206,7 +212,7 @@ class Grid {
   chx_grid* h_ = nullptr;
   int nbasis_ = 0;
   int natoms_ = 0;
-  int nmats_u_ = 0;
+  int nmats_u_ = 0, nspin_u_ = 1;
   int nspin_ = 1;  // spin count of the last getDensity
 };
 
