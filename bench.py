@@ -493,16 +493,15 @@ def bench_pack(args):
     points = sum(it["stats"]["points"] for it in items)
     flops = sum(it["stats"]["flops_per_build"] for it in items)
 
-    # one after another: each grid through the single-grid C ABI, D and V in HBM
+    # one after another: each grid through the single-grid C ABI, D and V in HBM; every grid's build
+    # is enqueued (on the grid's own stream) before any energy is read, so the grids overlap
     def seq_step():
-        E = []
         for it in items:
             nb = it["mol"].nbasis
             it["grid"].getDensityDevice(it["dD"].ptr, 1, nb)
             xc.Evaluate("ev", it["grid"])
-            E.append(it["grid"].getEnergy())
             it["grid"].getFockDevice(it["dV"].ptr, 1, nb)
-        return E
+        return [it["grid"].getEnergy() for it in items]
 
     for _ in range(args.warmup):
         seq_step()
@@ -517,19 +516,32 @@ def bench_pack(args):
     pack = ca.Pack([it["grid"] for it in items])
     Dp, Vp = [it["dD"].ptr for it in items], [it["dV"].ptr for it in items]
     chunk, tune_ms = pack.autotune(xc, Dp, Vp, reps=2)
-    for _ in range(args.warmup):
-        pack.xc(xc, Dp, Vp)
-    hip.device_synchronize()
-    e0, e1 = hip.Event(), hip.Event()
+    form, lanes = pack.form(), pack.lanes()
     stream = pack.stream()
-    w0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        E_pack, N_pack = pack.xc(xc, Dp, Vp)
-    e1.record(stream)
-    hip.device_synchronize()
-    wall_pack = 1e3 * (time.perf_counter() - w0) / args.steps
-    ms_pack = e0.elapsed_ms(e1) / args.steps
+
+    def time_pack():
+        for _ in range(args.warmup):
+            pack.xc(xc, Dp, Vp)
+        hip.device_synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        w0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            E, N = pack.xc(xc, Dp, Vp)
+        e1.record(stream)
+        hip.device_synchronize()
+        return e0.elapsed_ms(e1) / args.steps, 1e3 * (time.perf_counter() - w0) / args.steps, E, N
+
+    # every form at the tuned length and lanes for the record, then the autotuner's choice (the value)
+    modes = {}
+    for f in (0, 1, 2):
+        pack.set_lanes(lanes)
+        pack.set_form(f)
+        modes["%s%s" % (ca.Pack.FORMS[f], "" if f == 1 else ", %d lanes" % lanes)] = round(time_pack()[0], 4)
+    pack.set_lanes(lanes)
+    pack.set_form(form)
+    ms_pack, wall_pack, E_pack, N_pack = time_pack()
+    _say("pack: autotuned %s; %s" % (ca.Pack.FORMS[form], modes))
     dV = max(float(np.abs(it["dV"].download() - v).max()) for it, v in zip(items, V_seq))
     dE = max(abs(a - b) / it["natoms"] for a, b, it in zip(E_pack, E_seq, items))
 
@@ -549,8 +561,13 @@ def bench_pack(args):
         "config": {"workload": "64 alkanes C_nH_2n+2, n in 1..40 / def2-SVP / SG-1 / PBE, one pack",
                    "name": "alkanes64", "molecules": len(items), "points": points,
                    "nbasis_total": sum(it["mol"].nbasis for it in items),
-                   "form": "mixed, per-molecule kernels on staggered lanes, hipGraph replay",
-                   "lanes": pack.lanes(), "vxc_chunk": chunk, "vxc_wide_from_npad": pack.vxcWideFrom()},
+                   "form": "mixed, " + {0: "per-molecule kernels on staggered lanes",
+                                        1: "one persistent launch per stage over every molecule's items",
+                                        2: "one launch per stage per lane, lanes staggered"}[form] +
+                           ", hipGraph replay (autotuned)",
+                   "lanes": None if form == 1 else lanes, "vxc_chunk": chunk,
+                   "vxc_wide_from_npad": pack.vxcWideFrom()},
+        "pack_forms_ms": modes,
         "sequential_ms_per_step": round(ms_seq, 4),
         "speedup_vs_sequential": round(ms_seq / ms_pack, 3),
         "autotune_ms": {str(k): round(v, 4) for k, v in tune_ms.items()},

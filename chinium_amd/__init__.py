@@ -122,6 +122,8 @@ def lib():
         "chx_pack_set_graph": (i, [_vp, i]),
         "chx_pack_set_lanes": (i, [_vp, i]),
         "chx_pack_lanes": (i, [_vp, P(i)]),
+        "chx_pack_set_form": (i, [_vp, i]),
+        "chx_pack_form": (i, [_vp, P(i)]),
         "chx_pack_autotune": (i, [_vp, _vp, _vp, _vp, i, P(i), _vp]),
         "chx_pack_stream": (i, [_vp, P(_vp)]),
         "chx_pack_destroy": (None, [_vp]),
@@ -667,6 +669,18 @@ class Pack:
     def lanes(self):
         v = ctypes.c_int()
         _check(lib().chx_pack_lanes(self.h, ctypes.byref(v)))
+        return v.value
+
+    FORMS = {0: "lanes", 1: "persistent", 2: "lane stages"}
+
+    def set_form(self, form):
+        """0: per-molecule kernels on lanes; 1: one persistent launch per stage over every molecule's
+        items; 2: one launch per stage per lane (1 and 2: LDA/GGA grids)."""
+        _check(lib().chx_pack_set_form(self.h, int(form)))
+
+    def form(self):
+        v = ctypes.c_int()
+        _check(lib().chx_pack_form(self.h, ctypes.byref(v)))
         return v.value
 
     def autotune(self, xc, D_ptrs, V_ptrs, reps=3):

@@ -94,7 +94,7 @@ void run_functional(chx_grid* g, const chx_xc* xc, bool want_e, bool want_v) {
   g->d_codes.upload(xc->codes, g->stream);
   hipEvent_t e = nullptr;
   g->surrogate_used = false;
-  g->vout_ready = false;
+  g->vout_ready = g->corr_done = false;
   g->corr_pending = g->precision == CHX_PREC_MIXED && want_e && want_v;
   if (g->nspin == 2) {  // spin-polarised: alpha coefficients in d_coef, beta in d_coef_b
     if (xc->family > g->type)
@@ -141,7 +141,7 @@ enum { kStPermute = 0, kStDensity = 1, kStFunctional = 2, kStVxc = 3, kStFinaliz
 void density_from_device(chx_grid* g, const double* dD, int spin = 0) {
   hipEvent_t e = nullptr;
   const int ng = (int)g->grp_batch.size();
-  g->corr_pending = g->vout_ready = false;
+  g->corr_pending = g->corr_done = g->vout_ready = false;
   if (g->precision == CHX_PREC_MIXED) {
     g->mark_begin(kStPermute, &e);
     const int ncg = (int)g->core_grps.size();
@@ -285,8 +285,10 @@ void fock_spin(chx_grid* g, double* dV, int nspin) {
     fock_to_device(g, dV + nb2, 1);
     CHX_HIP(hipMemcpyAsync(g->d_coef.p, g->d_coef_save.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, g->stream));
   }
-  if (g->precision == CHX_PREC_MIXED && g->corr_pending)
+  if (g->precision == CHX_PREC_MIXED && g->corr_pending) {
     launch_reduce_corr(g->stream, nspin * corr_blocks(g->nbasis), g->d_cpart.p, g->d_en.p + 2);
+    g->corr_done = true;
+  }
 }
 
 // The potential of the last Evaluate into d_Vout (mixed form, before the
@@ -779,11 +781,12 @@ int chx_grid_num_electrons(chx_grid* g, double* N) {
 // Mixed form: E carries the first-order correction for the split of D
 // (sum_s V_s . (D_s - hi - lo)), which needs the potential; if Evaluate("ev")
 // was not yet followed by a Fock call, the potential is built here (into the
-// grid's own buffer, which the next Fock call returns without rebuilding).
+// grid's own buffer, which the next Fock call returns without rebuilding);
+// after a Fock call the correction is already in place.
 int chx_grid_energy(chx_grid* g, double* E) {
   return guard([&] {
     require(g->have_eval, CHX_ERR_STATE, "ExchangeCorrelation::Evaluate must precede Grid::getEnergy");
-    if (g->corr_pending && g->have_v && !g->vout_ready) build_vout(g);
+    if (g->corr_pending && g->have_v && !g->corr_done) build_vout(g);
     double en[3];
     CHX_HIP(hipMemcpyAsync(en, g->d_en.p, sizeof(en), hipMemcpyDeviceToHost, g->stream));
     CHX_HIP(hipStreamSynchronize(g->stream));
@@ -833,7 +836,7 @@ int chx_grid_set_vrho(chx_grid* g, const double* vrho, long npoints) {
     d.upload(vw, g->stream);
     launch_vrho_coef(g->stream, (long)vw.size(), d.p, g->d_coef.p);
     CHX_HIP(hipStreamSynchronize(g->stream));
-    g->corr_pending = g->vout_ready = false;  // a potential of its own: no energy correction
+    g->corr_pending = g->corr_done = g->vout_ready = false;  // a potential of its own: no energy correction
     g->have_v = true;
     g->v_family = 0;
     g->v_nspin = 1;

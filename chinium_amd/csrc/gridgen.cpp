@@ -772,7 +772,7 @@ void grid_plan(chx_grid* g) {
   g->ao_ready = true;
   ++g->plan_gen;
   g->have_density = g->have_eval = g->have_v = g->have_skeleton = false;
-  g->corr_pending = g->vout_ready = false;
+  g->corr_pending = g->corr_done = g->vout_ready = false;
   g->slot_grp.clear();
   g->slot_q.clear();
 }

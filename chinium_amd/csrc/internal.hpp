@@ -306,6 +306,12 @@ void launch_density_mx_core(hipStream_t s, int mode, int ncore_groups, const Mol
 // B^T of every slot plus the all-zero slot (index nslots) for ncomp components
 void launch_make_b_mx(hipStream_t s, int ncomp, const MolDev& one);
 void launch_vxc_mx(hipStream_t s, int ncomp, int nitems, const MolDev& one);
+// Persistent stage launches over many molecules (packs, LDA/GGA): tab holds one
+// descriptor per molecule with byte offsets from base in place of pointers;
+// starts (nmol + 1) the first item of each molecule in the stage's flat list.
+enum { kStageDensity = 0, kStageCore = 1, kStageMakeB = 2, kStageVxc = 3, kStageVxcWide = 4 };
+void launch_stage_multi(hipStream_t s, int stage, int ncomp, const char* base, const MolDev* tab, const int* starts,
+                        int nmol, int total);
 // 256 x 256 output blocks over groups of 8 tiles of each batch chunk's alive-tile
 // union (vchunk / vtab / vtile); items (group I, group J, chunk)
 void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one);
@@ -440,8 +446,9 @@ struct chx_grid {
   chx::DevBuf<float> d_Dres;
   chx::DevBuf<double> d_cpart;
   // corr_pending: the last Evaluate computed E and v in the mixed form, so the potential's build
-  // adds the correction to E; vout_ready: d_Vout holds that potential (vout_nspin matrices)
-  bool corr_pending = false, vout_ready = false;
+  // adds the correction to E; corr_done: a build since then has; vout_ready: d_Vout holds that
+  // potential (vout_nspin matrices)
+  bool corr_pending = false, corr_done = false, vout_ready = false;
   int vout_nspin = 0;
   int v_family = 0;    // family of the potential on the grid (set by Evaluate / set_vrho)
   int en_blocks = 0;

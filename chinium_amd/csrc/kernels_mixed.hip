@@ -479,13 +479,13 @@ struct DensMxSmem {
 // The grid's descriptor comes by value (a kernel argument: every pointer in it
 // is known to address global memory, and uniform table reads become scalar
 // loads; a descriptor read from device memory would leave them flat).
+// r: the block's item (after the XCD-aware remap) within the molecule's launch.
 template <int MODE, bool P3>
-__global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
+__device__ __forceinline__ void density_mx_body(const MolDev& m, int r) {
   using lds_ptr = __attribute__((address_space(3))) void*;
   constexpr int NOUT = MODE == 0 ? 4 : 1;
   constexpr int grp_stride = (MODE == 0 ? 4 : 3) * kBatch;  // part: [group][4][64]; tau partials: [group][3][64]
   __shared__ DensMxSmem<P3> sm;
-  const int r = xcd_remap(blockIdx.x, gridDim.x);
   const int grp = P3 ? m.core_grps[r] : (MODE == 0 && m.main_grps) ? m.main_grps[r] : r;
   const u16* __restrict__ cache = m.cache + (size_t)MODE * kMxT;
   const u16* __restrict__ Ds = m.Ds;
@@ -617,6 +617,10 @@ __global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
     for (int q = 0; q < nt; ++q) v += (double)sm.red[q][c][g];
     part[(size_t)grp * grp_stride + c * kBatch + g] = v;
   }
+}
+template <int MODE, bool P3>
+__global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx(MolDev m) {
+  density_mx_body<MODE, P3>(m, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // The three tau partials in one block: per K step the three A = d_r phi
@@ -979,10 +983,7 @@ struct MakeB {
 // components, so its place moves with the potential's component count and is
 // rewritten on every build.
 template <int NC>
-__global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev m) {
-  constexpr int W = NC == 4 ? 1 : 4;  // waves per block
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
-  const int slot = blockIdx.x * W + wave;
+__device__ __forceinline__ void make_b_slot(const MolDev& m, int slot, int g) {
   if (slot > m.nslots) return;
   u16* __restrict__ col0 = m.bt + (size_t)slot * NC * kMxT + g;  // [c][hl][nu][g], this lane's point
   if (slot == m.nslots) {
@@ -1019,6 +1020,12 @@ __global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev m) {
       col0[c * kMxT + kTile * kBatch + col * kBatch] = 0;
     }
 }
+template <int NC>
+__global__ __launch_bounds__(NC == 4 ? 64 : 256) void k_make_b_mx(MolDev m) {
+  constexpr int W = NC == 4 ? 1 : 4;  // waves per block
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  make_b_slot<NC>(m, blockIdx.x * W + wave, threadIdx.x & 63);
+}
 
 void launch_make_b_mx(hipStream_t s, int ncomp, const MolDev& one) {
   const int waves = one.nslots + 1;  // every slot and the all-zero slot
@@ -1049,10 +1056,9 @@ struct VxcMxSmem {
 };
 
 template <int NC>
-__global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev m) {
+__device__ __forceinline__ void vxc_mx_body(const MolDev& m, int item) {
   using lds_ptr = __attribute__((address_space(3))) void*;
   __shared__ VxcMxSmem sm;
-  const int item = xcd_remap(blockIdx.x, gridDim.x);
   const int4 wk = m.cwork[item];
   const int I = wk.x, J = wk.y;
   const u16* __restrict__ cache = m.cache + m.toff;
@@ -1142,6 +1148,10 @@ __global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev m) {
         unsafeAtomicAdd(&Mint[(size_t)mu * npad + nu], (double)acc[r][cb][i]);
       }
 }
+template <int NC>
+__global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev m) {
+  vxc_mx_body<NC>(m, xcd_remap(blockIdx.x, gridDim.x));
+}
 
 // Vxc for large bases: M_IJ += Phi_I^T B_J over 256 x 256 blocks whose rows
 // and columns are groups of 8 tiles of a batch chunk's alive-tile union, so
@@ -1169,10 +1179,9 @@ struct VxcMxWideSmem {
 };
 
 template <int NC>
-__global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
+__device__ __forceinline__ void vxc_mx_wide_body(const MolDev& m, int item) {
   using lds_ptr = __attribute__((address_space(3))) void*;
   __shared__ VxcMxWideSmem sm;
-  const int item = xcd_remap(blockIdx.x, gridDim.x);
   const int4 wk = m.cwork[item];
   const int I = wk.x, J = wk.y;    // I <= J
   const int4 ch = m.vchunk[wk.z];  // (table offset, batches, table width, tile-list offset)
@@ -1314,6 +1323,166 @@ __global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
   }
   if (I < J) flush(8, 0);
   else flush(0, 8);
+}
+template <int NC>
+__global__ __launch_bounds__(512, 1) void k_vxc_mx_wide(MolDev m) {
+  vxc_mx_wide_body<NC>(m, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// ---------------------------------------------------------------------------
+// Persistent stage kernels over many molecules (packs). A table holds one
+// descriptor per molecule whose pointer fields are byte offsets from a base
+// kernel argument; rebase() adds the base, so every table address the kernels
+// form derives from a kernel argument and stays a global-memory access (a
+// descriptor read from device memory with its own pointers would leave them
+// flat). starts[k] is the first item of molecule k in the stage's flat item
+// list (starts[nmol] = total). A fixed grid of blocks walks the list: virtual
+// block v = blockIdx.x + j gridDim.x takes item xcd_remap(v, total) (gridDim.x
+// a multiple of 8, so each XCD keeps a contiguous range), finds its molecule by
+// bisection, and runs the single-molecule body on it.
+template <class T>
+__device__ __forceinline__ T* rb(const char* base, T* off) {
+  return reinterpret_cast<T*>(const_cast<char*>(base) + reinterpret_cast<long>(off));
+}
+__device__ __forceinline__ MolDev rebase(const char* base, const MolDev& o) {
+  MolDev m = o;
+  m.grp_batch = rb(base, o.grp_batch); m.grp_tiles = rb(base, o.grp_tiles); m.grp_ntiles = rb(base, o.grp_ntiles);
+  m.bt_off = rb(base, o.bt_off); m.bt_list = rb(base, o.bt_list);
+  m.bx = rb(base, o.bx); m.by = rb(base, o.by); m.bz = rb(base, o.bz);
+  m.bv.sh_xyz = rb(base, o.bv.sh_xyz); m.bv.sh_type = rb(base, o.bv.sh_type);
+  m.bv.sh_nprim = rb(base, o.bv.sh_nprim); m.bv.sh_poff = rb(base, o.bv.sh_poff);
+  m.bv.p_exp = rb(base, o.bv.p_exp); m.bv.p_coef = rb(base, o.bv.p_coef);
+  m.bv.tile_seg_off = rb(base, o.bv.tile_seg_off); m.bv.seg = rb(base, o.bv.seg);
+  m.bv.segrec = rb(base, o.bv.segrec); m.bv.sh_atom = rb(base, o.bv.sh_atom);
+  m.cache = rb(base, o.cache); m.Ds = rb(base, o.Ds);
+  m.part = rb(base, o.part); m.part_tau = rb(base, o.part_tau);
+  m.slot_batch = rb(base, o.slot_batch); m.slot_tile = rb(base, o.slot_tile);
+  m.coef = rb(base, o.coef); m.bt = rb(base, o.bt);
+  m.cwork = rb(base, o.cwork); m.pair_batches = rb(base, o.pair_batches); m.slot_of = rb(base, o.slot_of);
+  m.cslot = rb(base, o.cslot); m.Mint = rb(base, o.Mint);
+  m.vchunk = rb(base, o.vchunk); m.vtab = rb(base, o.vtab); m.vtile = rb(base, o.vtile);
+  m.cache3 = rb(base, o.cache3);
+  m.core_off = rb(base, o.core_off); m.core_grps = rb(base, o.core_grps); m.main_grps = rb(base, o.main_grps);
+  return m;
+}
+__device__ __forceinline__ int item_owner(const int* __restrict__ starts, int nmol, int item) {
+  int lo = 0, hi = nmol;  // the last molecule whose first item is <= item
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (starts[mid] <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+// A block's items come in increasing order (xcd_remap keeps each block's XCD),
+// so after the first bisection its molecule cursor only moves forward.
+__device__ __forceinline__ int item_owner_from(const int* __restrict__ starts, int nmol, int item, int k) {
+  if (k < 0) return item_owner(starts, nmol, item);
+  while (k + 1 < nmol && starts[k + 1] <= item) ++k;
+  return k;
+}
+
+template <int MODE, bool P3>
+__global__ __launch_bounds__(256, P3 ? 2 : 4) void k_density_mx_multi(const char* base, const MolDev* __restrict__ tab,
+                                                                      const int* __restrict__ starts, int nmol,
+                                                                      int total) {
+  int k = -1;
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int item = xcd_remap(v, total);
+    k = item_owner_from(starts, nmol, item, k);
+    density_mx_body<MODE, P3>(rebase(base, tab[k]), item - starts[k]);
+    __syncthreads();  // the next item reuses the block's LDS
+  }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_make_b_mx_multi(const char* base, const MolDev* __restrict__ tab,
+                                                         const int* __restrict__ starts, int nmol, int total) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int k = -1;
+  for (int v = blockIdx.x * 4 + wave; v < total; v += gridDim.x * 4) {  // one wave per slot, no LDS
+    k = item_owner_from(starts, nmol, v, k);
+    make_b_slot<NC>(rebase(base, tab[k]), v - starts[k], threadIdx.x & 63);
+  }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256, 3) void k_vxc_mx_multi(const char* base, const MolDev* __restrict__ tab,
+                                                         const int* __restrict__ starts, int nmol, int total) {
+  int k = -1;
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int item = xcd_remap(v, total);
+    k = item_owner_from(starts, nmol, item, k);
+    vxc_mx_body<NC>(rebase(base, tab[k]), item - starts[k]);
+    __syncthreads();
+  }
+}
+
+template <int NC>
+__global__ __launch_bounds__(512, 1) void k_vxc_mx_wide_multi(const char* base, const MolDev* __restrict__ tab,
+                                                              const int* __restrict__ starts, int nmol, int total) {
+  int k = -1;
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int item = xcd_remap(v, total);
+    k = item_owner_from(starts, nmol, item, k);
+    vxc_mx_wide_body<NC>(rebase(base, tab[k]), item - starts[k]);
+    __syncthreads();
+  }
+}
+
+// Blocks of a persistent launch: the kernel's resident blocks per CU times the
+// CUs, a multiple of 8, at most the items (rounded up to 8).
+template <class K>
+static int persistent_blocks(K kernel, int threads, int total) {
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    CHX_HIP(hipGetDevice(&dev));
+    CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  int per_cu = 0;
+  CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+  const long full = (long)std::max(1, per_cu) * ncu;
+  const long want = std::min<long>(full, ((long)total + 7) / 8 * 8);
+  return (int)std::max<long>(8, want / 8 * 8);
+}
+
+void launch_stage_multi(hipStream_t s, int stage, int ncomp, const char* base, const MolDev* tab, const int* starts,
+                        int nmol, int total) {
+  if (total <= 0) return;
+  switch (stage) {
+    case kStageDensity: {
+      auto k = k_density_mx_multi<0, false>;
+      hipLaunchKernelGGL(k, dim3(persistent_blocks(k, 256, total)), dim3(256), 0, s, base, tab, starts, nmol, total);
+      break;
+    }
+    case kStageCore: {
+      auto k = k_density_mx_multi<0, true>;
+      hipLaunchKernelGGL(k, dim3(persistent_blocks(k, 256, total)), dim3(256), 0, s, base, tab, starts, nmol, total);
+      break;
+    }
+    case kStageMakeB: {
+      if (ncomp != 1) throw Error(CHX_ERR_UNSUPPORTED, "persistent packs are built for LDA and GGA");
+      auto k = k_make_b_mx_multi<1>;
+      const int blocks = persistent_blocks(k, 256, (total + 3) / 4);
+      hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, s, base, tab, starts, nmol, total);
+      break;
+    }
+    case kStageVxc: {
+      if (ncomp != 1) throw Error(CHX_ERR_UNSUPPORTED, "persistent packs are built for LDA and GGA");
+      auto k = k_vxc_mx_multi<1>;
+      hipLaunchKernelGGL(k, dim3(persistent_blocks(k, 256, total)), dim3(256), 0, s, base, tab, starts, nmol, total);
+      break;
+    }
+    case kStageVxcWide: {
+      if (ncomp != 1) throw Error(CHX_ERR_UNSUPPORTED, "persistent packs are built for LDA and GGA");
+      auto k = k_vxc_mx_wide_multi<1>;
+      hipLaunchKernelGGL(k, dim3(persistent_blocks(k, 512, total)), dim3(512), 0, s, base, tab, starts, nmol, total);
+      break;
+    }
+    default:
+      throw Error(CHX_ERR_ARG, "unknown pack stage");
+  }
+  CHX_HIP(hipGetLastError());
 }
 
 void launch_vxc_mx_wide(hipStream_t s, int ncomp, int nitems, const MolDev& one) {

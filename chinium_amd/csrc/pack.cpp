@@ -11,14 +11,20 @@
 // small per-molecule launches. The whole build is captured once as a hipGraph
 // (streams forked from and joined back to the pack's stream) and replayed each
 // SCF iteration; a capture is redone when the D/V pointers, the functional,
-// the lanes or the Vxc work length change. chx_pack_autotune picks the Vxc
-// work-item length by timing.
+// the lanes or the Vxc work length change. LDA/GGA packs can instead run each
+// stage as one persistent launch over every molecule's items (the molecules'
+// descriptors in a table of offsets from a base pointer, so the kernels keep
+// their global-memory instructions). chx_pack_autotune picks the Vxc work-item
+// length, the Vxc block shape by basis size, and lanes or persistent stages by
+// timing.
 //
 // No reference counterpart: Chinium runs one molecule per process
 // (src/HartreeFockKohnSham/SelfConsistentField.cpp:27-33).
 #include <array>
 #include <algorithm>
 #include <chrono>
+#include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -58,6 +64,24 @@ struct chx_pack {
   std::vector<int> fn_off;
   std::vector<const char*> fn_base;
   std::vector<int> fn_blocks;
+  // persistent stages (LDA/GGA packs): one launch per stage over every
+  // molecule's items; descriptors with byte offsets from mol_base, per stage the
+  // first item of each molecule (kStages x (n + 1)), and the functional's table
+  // over all molecules
+  // form: 0 per-molecule kernels on the lanes, 1 persistent stages over every
+  // molecule, 2 one launch per stage per lane (each over the lane's molecules)
+  int form = 0;
+  DevBuf<MolDev> mol_tab;
+  const char* mol_base = nullptr;
+  DevBuf<int> d_starts, d_ident;
+  std::vector<int> totals;
+  // form 2: per lane, the stages' first items of its molecules (5 x (molecules + 1), from lane_soff)
+  DevBuf<int> d_lane_starts;
+  std::vector<int> lane_soff;
+  std::vector<std::array<int, 5>> lane_tot;
+  DevBuf<FnMolecule> fn_all;
+  const char* fn_all_base = nullptr;
+  int fn_all_blocks = 0;
   ~chx_pack() {
     drop_graph();
     free_lanes();
@@ -99,6 +123,117 @@ void prequire(bool ok, int code, const std::string& msg) {
   if (!ok) throw Error(code, msg);
 }
 
+// The functional launch's table for molecules [m0, m1): their buffers as byte
+// offsets from the lowest address among them, then a sentinel; its block count.
+void fn_table(const chx_pack* p, int m0, int m1, std::vector<FnMolecule>& tab, const char*& base, int& blocks) {
+  auto ptrs = [&](const chx_grid* g) {
+    return std::array<const char*, 8>{(const char*)g->d_grp_off.p, (const char*)g->d_part.p,
+                                      (const char*)g->d_part_tau.p, (const char*)g->d_w.p,
+                                      (const char*)g->d_rho.p, (const char*)g->d_coef.p,
+                                      (const char*)g->d_en_part.p, (const char*)g->d_cpart.p};
+  };
+  base = nullptr;
+  for (int m = m0; m < m1; ++m)
+    for (const char* q : ptrs(p->grids[m]))
+      if (!base || q < base) base = q;
+  tab.clear();
+  int blk = 0;
+  for (int m = m0; m < m1; ++m) {
+    const chx_grid* g = p->grids[m];
+    const auto q = ptrs(g);
+    FnMolecule f{};
+    f.grp_off = q[0] - base; f.part = q[1] - base; f.part_tau = q[2] - base; f.w = q[3] - base;
+    f.rho = q[4] - base; f.coef = q[5] - base; f.en_part = q[6] - base; f.cpart = q[7] - base;
+    f.ncpart = corr_blocks(g->nbasis);
+    f.nslots = g->nbatches * kBatch;
+    f.blk0 = blk;
+    f.slot_en = m;
+    blk += g->en_blocks;
+    tab.push_back(f);
+  }
+  FnMolecule end{};
+  end.blk0 = blk;
+  tab.push_back(end);
+  blocks = blk;
+}
+
+// Tables of the persistent stages (LDA/GGA packs): each molecule's descriptor
+// with its pointers as byte offsets from the lowest of them (a molecule without
+// core batches gets an identity list for the (rho, grad rho) launch's groups),
+// the items of each stage per molecule, and the functional's table over all.
+void build_persistent(chx_pack* p) {
+  const int n = (int)p->grids.size();
+  int maxng = 1;
+  for (const chx_grid* g : p->grids) maxng = std::max(maxng, (int)g->grp_batch.size());
+  std::vector<int> ident(maxng);
+  for (int k = 0; k < maxng; ++k) ident[k] = k;
+  p->d_ident.upload(ident, p->stream);
+  std::vector<MolDev> mols(n);
+  for (int m = 0; m < n; ++m) {
+    mols[m] = p->grids[m]->hmol;
+    if (!mols[m].main_grps) mols[m].main_grps = p->d_ident.p;
+  }
+  // every pointer field of a descriptor (the fields rebase() offsets), read and
+  // written as addresses through memcpy
+  auto fields = [](MolDev& d) {
+    return std::array<void*, 38>{
+        &d.grp_batch, &d.grp_tiles, &d.grp_ntiles, &d.bt_off, &d.bt_list, &d.bx, &d.by, &d.bz,
+        &d.bv.sh_xyz, &d.bv.sh_type, &d.bv.sh_nprim, &d.bv.sh_poff, &d.bv.p_exp, &d.bv.p_coef,
+        &d.bv.tile_seg_off, &d.bv.seg, &d.bv.segrec, &d.bv.sh_atom, &d.cache, &d.Ds, &d.part, &d.part_tau,
+        &d.slot_batch, &d.slot_tile, &d.coef, &d.bt, &d.cwork, &d.pair_batches, &d.slot_of, &d.cslot, &d.Mint,
+        &d.vchunk, &d.vtab, &d.vtile, &d.cache3, &d.core_off, &d.core_grps, &d.main_grps};
+  };
+  auto get = [](void* f) {
+    uintptr_t v;
+    std::memcpy(&v, f, sizeof v);
+    return v;
+  };
+  uintptr_t base = 0;
+  for (MolDev& d : mols)
+    for (void* f : fields(d)) {
+      const uintptr_t v = get(f);
+      if (v && (!base || v < base)) base = v;
+    }
+  for (MolDev& d : mols)  // null fields (never read by that molecule's items) map to the base
+    for (void* f : fields(d)) {
+      const uintptr_t v = get(f), off = v ? v - base : 0;
+      std::memcpy(f, &off, sizeof off);
+    }
+  p->mol_base = (const char*)base;
+  p->mol_tab.upload(mols, p->stream);
+  // per stage: density (the groups outside the core batches), core groups, B^T slots (+ the zero
+  // slot), Vxc items of 128-AO molecules, Vxc items of 256-AO molecules
+  std::vector<int> starts(5 * (size_t)(n + 1), 0);
+  p->totals.assign(5, 0);
+  for (int m = 0; m < n; ++m) {
+    const chx_grid* g = p->grids[m];
+    const int ng = (int)g->grp_batch.size(), ncg = (int)g->core_grps.size(), nw = (int)g->cwork.size();
+    const int cnt[5] = {ng - ncg, ncg, (int)g->bt_list.size() + 1, g->mx_wide ? 0 : nw, g->mx_wide ? nw : 0};
+    for (int st = 0; st < 5; ++st) starts[(size_t)st * (n + 1) + m + 1] = starts[(size_t)st * (n + 1) + m] + cnt[st];
+  }
+  for (int st = 0; st < 5; ++st) p->totals[st] = starts[(size_t)st * (n + 1) + n];
+  p->d_starts.upload(starts, p->stream);
+  // the same per lane (form 2)
+  const int L = (int)p->cut.size() - 1;
+  std::vector<int> ls;
+  p->lane_soff.assign(std::max(L, 0), 0);
+  p->lane_tot.assign(std::max(L, 0), std::array<int, 5>{0, 0, 0, 0, 0});
+  for (int k = 0; k < L; ++k) {
+    const int m0 = p->cut[k], m1 = p->cut[k + 1], nl = m1 - m0;
+    p->lane_soff[k] = (int)ls.size();
+    for (int st = 0; st < 5; ++st) {
+      const int* a = &starts[(size_t)st * (n + 1)];
+      for (int i = 0; i <= nl; ++i) ls.push_back(a[m0 + i] - a[m0]);
+      p->lane_tot[k][st] = a[m1] - a[m0];
+    }
+  }
+  p->d_lane_starts.upload(ls.empty() ? std::vector<int>{0} : ls, p->stream);
+  std::vector<FnMolecule> tab;
+  fn_table(p, 0, n, tab, p->fn_all_base, p->fn_all_blocks);
+  p->fn_all.upload(tab, p->stream);
+  CHX_HIP(hipStreamSynchronize(p->stream));  // the host vectors go out of scope
+}
+
 // Lane cut points (equal shares of density groups + Vxc work items, at least
 // one molecule per lane) and the lanes' streams and events.
 void build_lanes(chx_pack* p) {
@@ -120,49 +255,20 @@ void build_lanes(chx_pack* p) {
   p->free_lanes();
   p->gen.resize(n);
   for (int m = 0; m < n; ++m) p->gen[m] = p->grids[m]->plan_gen;
-  // the functional of each lane's molecules in one launch: their buffers as byte
-  // offsets from the lowest address among them
+  // the functional of each lane's molecules in one launch
   std::vector<FnMolecule> all;
   p->fn_off.assign(L, 0);
   p->fn_base.assign(L, nullptr);
   p->fn_blocks.assign(L, 0);
   for (int k = 0; k < L; ++k) {
-    const int m0 = p->cut[k], m1 = p->cut[k + 1];
-    auto ptrs = [&](const chx_grid* g) {
-      return std::array<const char*, 8>{(const char*)g->d_grp_off.p, (const char*)g->d_part.p,
-                                        (const char*)g->d_part_tau.p, (const char*)g->d_w.p,
-                                        (const char*)g->d_rho.p, (const char*)g->d_coef.p,
-                                        (const char*)g->d_en_part.p, (const char*)g->d_cpart.p};
-    };
-    const char* base = nullptr;
-    for (int m = m0; m < m1; ++m)
-      for (const char* q : ptrs(p->grids[m]))
-        if (!base || q < base) base = q;
     std::vector<FnMolecule> tab;
-    int blk = 0;
-    for (int m = m0; m < m1; ++m) {
-      const chx_grid* g = p->grids[m];
-      const auto q = ptrs(g);
-      FnMolecule f{};
-      f.grp_off = q[0] - base; f.part = q[1] - base; f.part_tau = q[2] - base; f.w = q[3] - base;
-      f.rho = q[4] - base; f.coef = q[5] - base; f.en_part = q[6] - base; f.cpart = q[7] - base;
-      f.ncpart = corr_blocks(g->nbasis);
-      f.nslots = g->nbatches * kBatch;
-      f.blk0 = blk;
-      f.slot_en = m;
-      blk += g->en_blocks;
-      tab.push_back(f);
-    }
-    FnMolecule end{};
-    end.blk0 = blk;
-    tab.push_back(end);
+    fn_table(p, p->cut[k], p->cut[k + 1], tab, p->fn_base[k], p->fn_blocks[k]);
     p->fn_off[k] = (int)all.size();
     all.insert(all.end(), tab.begin(), tab.end());
-    p->fn_base[k] = base;
-    p->fn_blocks[k] = blk;
   }
   p->fn_tab.upload(all, p->stream);
   CHX_HIP(hipStreamSynchronize(p->stream));  // the table's host vector goes out of scope
+  if (p->type <= 1) build_persistent(p);  // after the cut: form 2's tables are per lane
   if (L > 1) {
     p->lane_st.resize(L);
     p->ev_dens.resize(L);
@@ -183,8 +289,44 @@ void build_lanes(chx_pack* p) {
 // lo)), and last the energies' correction for the split of D, one launch for
 // the lane. Apart from the functional and the correction, the kernels and their
 // arguments are those of the single-grid build.
+bool stages_ok(const chx_pack* p) { return p->type <= 1 && p->family <= 1; }
+
+// Form 2: the lane's molecules in one launch per stage (the persistent stage
+// kernels over the lane's part of the descriptor table).
+void enqueue_lane_stages(chx_pack* p, int lane, hipStream_t s, hipEvent_t after, hipEvent_t dens_done,
+                         const double* const* D, double* const* V) {
+  const int m0 = p->cut[lane], m1 = p->cut[lane + 1], nl = m1 - m0;
+  for (int m = m0; m < m1; ++m) {
+    chx_grid* g = p->grids[m];
+    launch_split_d(s, g->nbasis, g->npad, D[m], g->d_int2ref.p, g->d_Dsplit.p, g->d_Dres.p);
+    launch_fill(s, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
+  }
+  if (after) CHX_HIP(hipStreamWaitEvent(s, after, 0));
+  const int* st = p->d_lane_starts.p + p->lane_soff[lane];
+  auto stage = [&](int k) {
+    launch_stage_multi(s, k, 1, p->mol_base, p->mol_tab.p + m0, st + (size_t)k * (nl + 1), nl, p->lane_tot[lane][k]);
+  };
+  stage(kStageDensity);
+  stage(kStageCore);
+  if (dens_done) CHX_HIP(hipEventRecord(dens_done, s));
+  launch_functional_multi(s, p->fn_base[lane], p->fn_tab.p + p->fn_off[lane], nl, p->fn_blocks[lane], p->d_codes.p,
+                          (int)p->codes.size(), p->family, p->d_en.p);
+  stage(kStageMakeB);
+  stage(kStageVxc);
+  stage(kStageVxcWide);
+  for (int m = m0; m < m1; ++m) {
+    chx_grid* g = p->grids[m];
+    launch_finalize_corr(s, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, V[m], g->d_Dres.p, g->d_cpart.p);
+  }
+  launch_corr_multi(s, p->fn_base[lane], p->fn_tab.p + p->fn_off[lane], nl, p->d_en.p);
+}
+
 void enqueue_lane(chx_pack* p, int lane, hipStream_t s, hipEvent_t after, hipEvent_t dens_done,
                   const double* const* D, double* const* V) {
+  if (p->form == 2 && stages_ok(p)) {
+    enqueue_lane_stages(p, lane, s, after, dens_done, D, V);
+    return;
+  }
   const int m0 = p->cut[lane], m1 = p->cut[lane + 1];
   for (int m = m0; m < m1; ++m) {
     chx_grid* g = p->grids[m];
@@ -216,9 +358,46 @@ void enqueue_lane(chx_pack* p, int lane, hipStream_t s, hipEvent_t after, hipEve
   launch_corr_multi(s, p->fn_base[lane], p->fn_tab.p + p->fn_off[lane], m1 - m0, p->d_en.p);
 }
 
-// One XC build of every molecule: one lane runs on the pack's stream; more
-// lanes fork from it, are staggered by their density launches and join back.
+// One XC build of every molecule as persistent stages on the pack's stream
+// (LDA/GGA): per molecule split D and clear M; then one launch per stage over
+// every molecule's items (density, core batches, functional, B^T, Vxc of each
+// block shape); per molecule V = M + M^T with the partial sums of
+// V . (D - hi - lo), and one launch for the energies' corrections.
+void enqueue_persistent(chx_pack* p, const double* const* D, double* const* V) {
+  const int n = (int)p->grids.size();
+  hipStream_t s = p->stream;
+  for (int m = 0; m < n; ++m) {
+    chx_grid* g = p->grids[m];
+    launch_split_d(s, g->nbasis, g->npad, D[m], g->d_int2ref.p, g->d_Dsplit.p, g->d_Dres.p);
+    launch_fill(s, g->d_Mint.p, (size_t)g->npad * g->npad, 0.0);
+  }
+  const int* st = p->d_starts.p;
+  auto stage = [&](int k) {
+    launch_stage_multi(s, k, 1, p->mol_base, p->mol_tab.p, st + (size_t)k * (n + 1), n, p->totals[k]);
+  };
+  stage(kStageDensity);
+  stage(kStageCore);
+  launch_functional_multi(s, p->fn_all_base, p->fn_all.p, n, p->fn_all_blocks, p->d_codes.p, (int)p->codes.size(),
+                          p->family, p->d_en.p);
+  stage(kStageMakeB);
+  stage(kStageVxc);
+  stage(kStageVxcWide);
+  for (int m = 0; m < n; ++m) {
+    chx_grid* g = p->grids[m];
+    launch_finalize_corr(s, g->nbasis, g->npad, g->d_Mint.p, g->d_ref2int.p, V[m], g->d_Dres.p, g->d_cpart.p);
+  }
+  launch_corr_multi(s, p->fn_all_base, p->fn_all.p, n, p->d_en.p);
+}
+
+
+// One XC build of every molecule: persistent stages, or lanes (one lane runs on
+// the pack's stream; more lanes fork from it, are staggered by their density
+// launches and join back).
 void enqueue(chx_pack* p, const double* const* D, double* const* V) {
+  if (p->form == 1 && stages_ok(p)) {
+    enqueue_persistent(p, D, V);
+    return;
+  }
   const int L = (int)p->cut.size() - 1;
   if (L == 1) {
     enqueue_lane(p, 0, p->stream, nullptr, nullptr, D, V);
@@ -385,13 +564,18 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
     }
     for (chx_grid* g : p->grids) set_vxc_shape(g, best_thr >= 0 && g->npad >= best_thr ? 1 : 0);
     p->wide_from = best_thr;
-    // then the lane count at that length (4, 8, 16, at most one lane per molecule)
+    // then the form and lane count at that length: per-molecule kernels or per-lane stage launches
+    // on 4, 8 or 16 lanes (at most one lane per molecule), or persistent stages (LDA/GGA)
     const int n = (int)p->grids.size();
-    int best_lanes = -1;
-    float best_lanes_ms = 0.f;
-    for (int lanes : {4, 8, 16}) {
-      if (lanes > n && lanes != 4) continue;
-      p->lanes = std::min(lanes, n);
+    const bool st_ok = p->type <= 1 && xc->family <= 1;
+    int best_form = -1, best_lanes = 0;
+    float best_form_ms = 0.f;
+    const int cands[7][2] = {{0, 4}, {0, 8}, {0, 16}, {2, 4}, {2, 8}, {2, 16}, {1, 16}};
+    for (const auto& c : cands) {
+      if (c[0] != 0 && !st_ok) continue;
+      if (c[0] != 1 && c[1] > n && c[1] != 4) continue;
+      p->form = c[0];
+      p->lanes = std::min(c[1], n);
       build_lanes(p);
       run(p, xc, D_dev, V_dev, nullptr, nullptr);
       CHX_HIP(hipEventRecord(a, p->stream));
@@ -400,13 +584,15 @@ int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev,
       CHX_HIP(hipEventSynchronize(b));
       float ms = 0.f;
       CHX_HIP(hipEventElapsedTime(&ms, a, b));
-      if (best_lanes < 0 || ms < best_lanes_ms) {
+      if (best_form < 0 || ms < best_form_ms) {
+        best_form = p->form;
         best_lanes = p->lanes;
-        best_lanes_ms = ms;
+        best_form_ms = ms;
       }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    p->form = best_form;
     p->lanes = best_lanes;
     build_lanes(p);
     if (chunk_out) *chunk_out = cand[best];
@@ -433,7 +619,26 @@ int chx_pack_set_lanes(chx_pack* p, int lanes) {
     CHX_HIP(hipStreamSynchronize(p->stream));
     for (hipStream_t t : p->lane_st) CHX_HIP(hipStreamSynchronize(t));
     p->lanes = lanes;
+    if (p->form == 1) p->form = 0;  // lanes, not persistent stages
     build_lanes(p);
+  });
+}
+
+int chx_pack_set_form(chx_pack* p, int form) {
+  return pguard([&] {
+    prequire(p != nullptr && form >= 0 && form <= 2, CHX_ERR_ARG, "pack form must be 0, 1 or 2");
+    prequire(form == 0 || p->type <= 1, CHX_ERR_UNSUPPORTED, "stage launches are built for LDA and GGA grids");
+    CHX_HIP(hipStreamSynchronize(p->stream));
+    for (hipStream_t t : p->lane_st) CHX_HIP(hipStreamSynchronize(t));
+    p->form = form;
+    p->drop_graph();
+  });
+}
+
+int chx_pack_form(const chx_pack* p, int* form) {
+  return pguard([&] {
+    prequire(p && form, CHX_ERR_ARG, "bad arguments");
+    *form = p->form;
   });
 }
 

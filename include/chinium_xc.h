@@ -332,9 +332,26 @@ int chx_pack_set_graph(chx_pack* p, int enable);
  * 128 batches), keeps the fastest (chunk_out gets it, ms_out[4] the times);
  * then the Vxc tile shape per molecule (chx_grid_set_vxc_shape) by a
  * basis-size threshold T (256-AO union groups for npad >= T: none, 1024, 512,
- * all), keeps the fastest; then 4, 8 and 16 lanes, keeps the fastest. */
+ * all), keeps the fastest; then the form and lanes (per-molecule kernels or
+ * per-lane stage launches on 4, 8 or 16 lanes, or persistent stages; the last
+ * two for LDA/GGA), keeps the fastest. */
 int chx_pack_autotune(chx_pack* p, const chx_xc* xc, const double* const* D_dev, double* const* V_dev, int reps,
                       int* chunk_out, float* ms_out);
+/* How the build is launched. CHX_PACK_LANES (default): every molecule's
+ * single-grid kernels on the lanes. CHX_PACK_PERSISTENT: one launch per stage
+ * over every molecule's work items (density, core batches, functional, B^T,
+ * Vxc of each block shape), each a fixed grid of resident blocks that walks
+ * the flat (molecule, item) list, with each molecule's descriptor read from a
+ * table of offsets. CHX_PACK_LANE_STAGES: the same stage launches per lane,
+ * over the lane's molecules, the lanes staggered as with CHX_PACK_LANES. The
+ * last two are built for LDA/GGA grids (CHX_ERR_UNSUPPORTED on meta-GGA grids;
+ * a meta-GGA functional on the pack runs CHX_PACK_LANES). The whole build is
+ * one hipGraph in every form; results do not depend on the form. */
+#define CHX_PACK_LANES 0
+#define CHX_PACK_PERSISTENT 1
+#define CHX_PACK_LANE_STAGES 2
+int chx_pack_set_form(chx_pack* p, int form);
+int chx_pack_form(const chx_pack* p, int* form);
 /* The threshold the last autotune chose (-1: no molecule on union groups; -2: not tuned). */
 int chx_pack_vxc_wide_from(const chx_pack* p, int* npad_min);
 /* Lanes (1..16, default 16, at most one per molecule): the molecules are
@@ -342,7 +359,7 @@ int chx_pack_vxc_wide_from(const chx_pack* p, int* npad_min);
  * of its own; a group's density launches wait for the previous group's, so the
  * later stages of one group run beside the density of the next. Results do not
  * depend on it. chx_pack_stream stays the stream the whole build is ordered
- * on. */
+ * on. Setting lanes leaves CHX_PACK_PERSISTENT for CHX_PACK_LANES. */
 int chx_pack_set_lanes(chx_pack* p, int lanes);
 /* Lanes in use (after clamping to the molecule count). */
 int chx_pack_lanes(const chx_pack* p, int* lanes);

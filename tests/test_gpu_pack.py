@@ -96,7 +96,8 @@ def test_pack_autotune(alkanes):
     xc, pack = alkanes["xc"], alkanes["pack"]
     E0, _, V0 = _run(alkanes)
     chunk, ms = pack.autotune(xc, [d.ptr for d in alkanes["dD"]], [v.ptr for v in alkanes["dV"]], reps=2)
-    _say("  autotune: chunk %d, lanes %d, ms %s" % (chunk, pack.lanes(), {k: round(v, 4) for k, v in ms.items()}))
+    _say("  autotune: chunk %d, lanes %d, form %d, ms %s" % (chunk, pack.lanes(), pack.form(),
+                                                                   {k: round(v, 4) for k, v in ms.items()}))
     assert chunk in (8, 16, 32, 128) and all(v > 0 for v in ms.values())
     assert 1 <= pack.lanes() <= len(alkanes["mols"])
     E, _, Vs = _run(alkanes)
@@ -133,6 +134,37 @@ def test_pack_lanes(alkanes):
         pack.set_lanes(16)
 
 
+def test_pack_forms(alkanes):
+    """Persistent stages (one launch per stage over every molecule's items) and per-lane stage
+    launches give the per-molecule kernels' results, eager and replayed; setting lanes leaves the
+    persistent form."""
+    pack = alkanes["pack"]
+    pack.set_lanes(16)
+    pack.set_form(0)
+    E0, N0, V0 = _run(alkanes)
+    try:
+        for form, lanes in ((1, 16), (2, 2), (2, 4)):
+            pack.set_lanes(lanes)
+            pack.set_form(form)
+            assert pack.form() == form
+            for graph in (False, True, True):
+                pack.set_graph(graph)
+                E, N, Vs = _run(alkanes)
+                dV = max(np.abs(a - b).max() for a, b in zip(Vs, V0))
+                _say("  form %d lanes %d graph %d: max dE %.1e max dN %.1e max dV %.1e"
+                     % (form, lanes, graph, np.abs(E - E0).max(), np.abs(N - N0).max(), dV))
+                assert np.all(np.abs(E - E0) <= 1e-12 * np.abs(E0))
+                assert np.all(np.abs(N - N0) <= 1e-12 * N0)
+                assert dV <= 1e-12
+        pack.set_form(1)
+        pack.set_lanes(16)
+        assert pack.form() == 0
+    finally:
+        pack.set_graph(True)
+        pack.set_form(0)
+        pack.set_lanes(16)
+
+
 def test_pack_with_wide_member(alkanes):
     """A pack member large enough for the 256-AO Vxc super-tiles ((H2O)40 /
     def2-SVP, 960 AOs, npad 1024) keeps its own plan inside the pack: the pack
@@ -161,6 +193,15 @@ def test_pack_with_wide_member(alkanes):
     assert np.abs(V - V1).max() <= 1e-12
     assert abs(E[1] - Es1) <= 1e-12 * abs(Es1)
     assert np.abs(Vs - Vs1).max() <= 1e-12
+    for form in (1, 2):  # the wide member's items go to the 256-AO stage launch
+        pack.set_form(form)
+        E, N = pack.xc(xc, [dD.ptr, alkanes["dD"][1].ptr], [dV.ptr, alkanes["dV"][1].ptr])
+        V = dV.download().reshape((nb, nb), order="F")
+        Vs = alkanes["dV"][1].download().reshape((small["mol"].nbasis,) * 2, order="F")
+        _say("  form %d: (H2O)40 dE %.1e dV %.1e; C%d dE %.1e dV %.1e"
+             % (form, abs(E[0] - E1), np.abs(V - V1).max(), small["n"], abs(E[1] - Es1), np.abs(Vs - Vs1).max()))
+        assert abs(E[0] - E1) <= 1e-12 * abs(E1) and np.abs(V - V1).max() <= 1e-12
+        assert abs(E[1] - Es1) <= 1e-12 * abs(Es1) and np.abs(Vs - Vs1).max() <= 1e-12
 
 
 def test_pack_meta_gga():
@@ -184,6 +225,10 @@ def test_pack_meta_gga():
         dD.upload(np.asfortranarray(D).ravel(order="F"))
         mols.append(dict(n=n, Z=Z, xyz=xyz, mol=mol, D=D, grid=grid, E1=E1, V1=V1, dD=dD, dV=dV))
     pack = ca.Pack([m["grid"] for m in mols])
+    for form in (1, 2):
+        with pytest.raises(ca.ChxError) as e:  # stage launches are built for LDA / GGA grids
+            pack.set_form(form)
+        assert e.value.code == -5
     pack.set_lanes(2)
     for rep in range(2):
         E, N = pack.xc(xc, [m["dD"].ptr for m in mols], [m["dV"].ptr for m in mols])
