@@ -1171,13 +1171,19 @@ __global__ __launch_bounds__(256, 3) void k_vxc_mx(MolDev m) {
 // screening.
 constexpr int kMxSupW = 8;   // tiles per group (256 AOs)
 constexpr int kVxAW = 32;    // Phi^T fragments per step: [row block (16)][hl]
+constexpr int kVxWStep = 2 * kVxAW * kDxFrag;  // u32x4 per K step: Phi^T of the row group, then B^T of the column group
 struct VxcMxWideSmem {
-  u32x4 F[2 * kVxAW * kDxFrag];  // one K step's fragments: Phi^T of the row group, then B^T of the column group
+  u32x4 F[2 * kVxWStep];  // two K steps' fragments: step k in buffer k & 1
   int sl[kMxMaxChunk][16];       // per batch: slots of I's 8 tiles, then J's
   unsigned mk[kMxMaxChunk];      // per batch: alive bits of those 16 tiles
   int tile[16];                  // I's tiles, then J's (-1: padding)
 };
 
+// Two LDS buffers and one barrier per K step: after it, step k's fragments
+// are in LDS and every wave has read step k - 1's buffer, so the DMA of step
+// k + 1 goes there before step k's fragment reads and products (measured at
+// (H2O)128 / r2SCAN: 114.5 ms per launch against 125.5 ms with one buffer and
+// a second barrier after the fragment reads).
 template <int NC>
 __device__ __forceinline__ void vxc_mx_wide_body(const MolDev& m, int item) {
   using lds_ptr = __attribute__((address_space(3))) void*;
@@ -1235,22 +1241,24 @@ __device__ __forceinline__ void vxc_mx_wide_body(const MolDev& m, int item) {
       it = q >> 1; h = q & 1; c = 0;
     }
   };
-  auto dma = [&](int k) {
+  // issue step k's loads into buffer buf
+  auto dma = [&](int k, int buf) {
     int it, h, c;
     bool mir;
     decode(k, it, h, c, mir);
     const int ra = mir ? 8 : 0, cb = mir ? 0 : 8;  // table columns of the row and column groups
     const unsigned mk = __builtin_amdgcn_readfirstlane(sm.mk[it]);
+    u32x4* F = sm.F + buf * kVxWStep;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int t = t0 + 2 * j;
       if ((mk >> (ra + t)) & 1u) {
         const u16* pa = cache + (size_t)sm.sl[it][ra + t] * sstride + c * kMxT + h * 32;
-        __builtin_amdgcn_global_load_lds(pa + off, (lds_ptr)(sm.F + (wave + 8 * j) * kDxFrag), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(pa + off, (lds_ptr)(F + (wave + 8 * j) * kDxFrag), 16, 0, 0);
       }
       if ((mk >> (cb + t)) & 1u) {
         const u16* pb = bt + ((size_t)sm.sl[it][cb + t] * NC + c) * kMxT + h * 32;
-        __builtin_amdgcn_global_load_lds(pb + off, (lds_ptr)(sm.F + (kVxAW + wave + 8 * j) * kDxFrag), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(pb + off, (lds_ptr)(F + (kVxAW + wave + 8 * j) * kDxFrag), 16, 0, 0);
       }
     }
   };
@@ -1279,7 +1287,7 @@ __device__ __forceinline__ void vxc_mx_wide_body(const MolDev& m, int item) {
       }
     }
   };
-  dma(0);
+  dma(0, 0);
   for (int k = 0; k < nstage; ++k) {
     if (NC == 4 && k == ntau && I < J) {  // 2 M_tau: the item's (I, J) block and, by symmetry, (J, I)
 #pragma unroll
@@ -1301,18 +1309,18 @@ __device__ __forceinline__ void vxc_mx_wide_body(const MolDev& m, int item) {
     const unsigned mk = __builtin_amdgcn_readfirstlane(sm.mk[it]);
     const unsigned rmk = (mk >> ((mir ? 8 : 0) + 4 * wr)) & 15u, cmk = (mk >> ((mir ? 0 : 8) + 2 * wc)) & 3u;
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's fragments of step k are in LDS
-    __syncthreads();                     // ... and every other wave's
+    __syncthreads();                     // ... and every other wave's, and step k - 1's buffer is read
+    if (k + 1 < nstage) dma(k + 1, (k + 1) & 1);
+    const u32x4* F = sm.F + (k & 1) * kVxWStep;
     u32x4 a[8][2], bq[4][2];
 #pragma unroll
     for (int r = 0; r < 8; ++r)
 #pragma unroll
-      for (int hh = 0; hh < 2; ++hh) a[r][hh] = sm.F[((8 * wr + r) * 2 + hh) * kDxFrag + fidx];
+      for (int hh = 0; hh < 2; ++hh) a[r][hh] = F[((8 * wr + r) * 2 + hh) * kDxFrag + fidx];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int hh = 0; hh < 2; ++hh) bq[r][hh] = sm.F[(kVxAW + (4 * wc + r) * 2 + hh) * kDxFrag + fidx];
-    __syncthreads();  // every wave holds its fragments: the buffer is free
-    if (k + 1 < nstage) dma(k + 1);
+      for (int hh = 0; hh < 2; ++hh) bq[r][hh] = F[(kVxAW + (4 * wc + r) * 2 + hh) * kDxFrag + fidx];
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc)
       if ((cmk >> (cc >> 1)) & 1u) {
