@@ -116,6 +116,7 @@ def lib():
         "chx_grid_get_density_u": (i, [_vp, _vp, i, i, i]),
         "chx_grid_fock_u": (i, [_vp, _vp, i, i, i]),
         "chx_grid_fock_s": (i, [_vp, _vp, i, i]),
+        "chx_grid_fock_skeleton": (i, [_vp, _vp, i, i]),
         "chx_xc_eval_points_f": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp]),
         "chx_pack_create": (i, [_vp, i, P(_vp)]),
         "chx_pack_xc": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -384,6 +385,14 @@ class Grid:
         buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for x in Ds for D in x])
         self._nmats_u = len(Ds[0])
         _check(lib().chx_grid_get_density_u(self.h, _ptr(buf), len(Ds[0]), len(Ds), n))
+
+    def getFockSkeleton(self):
+        """Grid::getFockSkeleton (src/Grid/Grid.cpp:173-191): the XC Fock matrix's derivative along each
+        nuclear coordinate with the potential held fixed, [spin][3 atom + t] (closed shell)."""
+        n, m = self.nbasis, 3 * self.natoms
+        F = np.zeros(m * n * n)
+        _check(lib().chx_grid_fock_skeleton(self.h, _ptr(F), 1, n))
+        return [[F[j * n * n:(j + 1) * n * n].reshape((n, n), order="F").copy() for j in range(m)]]
 
     def getFockU(self, nspin=1, d_t="u"):
         """Grid::getFockU<u_t> (src/Grid/Grid.cpp:193-216): response matrices [spin][mat]; d_t="s":

@@ -631,6 +631,51 @@ int chx_grid_fock_s(chx_grid* g, double* F, int nspin, int nbasis) {
   });
 }
 
+int chx_grid_fock_skeleton(chx_grid* g, double* F, int nspin, int nbasis) {
+  return guard([&] {
+    require(F != nullptr, CHX_ERR_ARG, "null output");
+    require(nspin == 1, CHX_ERR_UNSUPPORTED, "only the closed-shell path is built");
+    require(nbasis == g->nbasis, CHX_ERR_ARG, "Fock matrix size does not match the basis");
+    require(g->have_v, CHX_ERR_STATE, "ExchangeCorrelation::Evaluate(\"ev\") must precede Grid::getFockSkeleton");
+    require(g->v_family <= 1 && g->type <= 1, CHX_ERR_UNSUPPORTED,
+            "the skeleton Fock derivative is built for LDA and GGA");
+    require(g->v_nspin == 1, CHX_ERR_UNSUPPORTED, "the skeleton Fock derivative is built for the closed-shell path");
+    require(g->precision == CHX_PREC_FP64 && g->cached, CHX_ERR_UNSUPPORTED,
+            "the skeleton Fock derivative is built for the FP64 form with the AO tables in HBM");
+    const int na = g->mol.natoms, ns = (int)g->bt_list.size(), n = g->nbasis;
+    const size_t pp = (size_t)g->npad * g->npad, sb = (size_t)kBatch * kTile;
+    // atom of each AO, reference order
+    std::vector<int> ao_atom(n, 0);
+    for (const chx::ShellH& sh : g->mol.shells)
+      for (int k = 0; k < chx::shell_size(sh.type); ++k) ao_atom[sh.ao_off + k] = sh.center;
+    DevBuf<int> d_atom;
+    d_atom.upload(ao_atom, g->stream);
+    launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
+    DevBuf<double> etab, qp, out;
+    etab.alloc(std::max<size_t>(1, 3 * (size_t)ns * sb));
+    launch_fill(g->stream, etab.p, 3 * (size_t)ns * sb, 0.0);
+    launch_skel_etab(g->stream, ns, g->d_slot_batch.p, g->d_slot_tile.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(),
+                     g->d_coef.p, etab.p);
+    qp.alloc(6 * pp);
+    launch_fill(g->stream, qp.p, 6 * pp, 0.0);
+    for (int t = 0; t < 3; ++t) {
+      // Q^t = (d_t Phi)^T B: the cached component t + 1 as the row operand
+      launch_vxc_c64(g->stream, 1, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p,
+                     g->ntiles, g->d_cache.p + (size_t)(t + 1) * sb, g->d_bmat.p, g->d_coef.p, qp.p + t * pp,
+                     g->npad);
+      // P^t = Phi^T E^t: E^t in the B table's place
+      launch_vxc_c64(g->stream, 1, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p,
+                     g->ntiles, g->d_cache.p, etab.p + (size_t)t * ns * sb, g->d_coef.p, qp.p + (3 + t) * pp,
+                     g->npad);
+    }
+    const size_t tot = (size_t)3 * na * n * n;
+    out.alloc(std::max<size_t>(1, tot));
+    launch_fock_skel_fin(g->stream, n, g->npad, na, qp.p, g->d_ref2int.p, d_atom.p, out.p);
+    CHX_HIP(hipMemcpyAsync(F, out.p, sizeof(double) * tot, hipMemcpyDeviceToHost, g->stream));
+    CHX_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
 int chx_grid_energy_grad(chx_grid* g, double* grad) {
   return guard([&] {
     require(grad != nullptr, CHX_ERR_ARG, "null output");

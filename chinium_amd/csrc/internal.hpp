@@ -269,6 +269,13 @@ void launch_xgrad_epi(hipStream_t s, int nslots, const int* slot_batch, const in
                       const double* coef, const double* xz, double* grad);
 // skeleton density derivatives (rho, grad rho per nuclear coordinate) into
 // rhos[3 atom + t][npt points][4], accumulated (clear it first)
+// Skeleton Fock derivative: E^t = a d_t phi + (c . grad) d_t phi per slot into
+// etab[t][slot][g][column] (zero it first), then F[3 A + t] in the reference's
+// order from Q^t = (d_t Phi)^T B (qp[t]) and P^t = Phi^T E^t (qp[3 + t]).
+void launch_skel_etab(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const double* bx,
+                      const double* by, const double* bz, BasisView bv, const double* coef, double* etab);
+void launch_fock_skel_fin(hipStream_t s, int n, int npad, int natoms, const double* qp, const int* ref2int,
+                          const int* ao_atom, double* F);
 void launch_skel_rho(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* slot_grp,
                      const int* slot_q, const double* bx, const double* by, const double* bz, BasisView bv,
                      const double* xz1, const double* xz2, long npt, double* rhos);

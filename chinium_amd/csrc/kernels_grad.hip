@@ -365,4 +365,122 @@ void launch_skel_rho(hipStream_t s, int nslots, const int* slot_batch, const int
   CHX_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Skeleton derivative of the XC Fock matrix with the potential held fixed
+// (Grid::getFockSkeleton, src/Grid/Grid.cpp:173-191 -> GridFock.cpp:51-97).
+// With a, c and B = a phi + c . grad phi as above, for direction t:
+//   N^t[mu][nu] = sum_g (d_t phi_mu B_nu + E^t_mu phi_nu),  E^t = a d_t phi + (c . grad) d_t phi,
+//   F[3 A + t] = -(P_A N^t + (P_A N^t)^T), P_A keeping the rows of atom A's AOs.
+// The two products run through the cached Vxc contraction: Q^t = (d_t Phi)^T B
+// and P^t = Phi^T E^t, so N^t[mu][nu] = Q^t[mu][nu] + P^t[nu][mu].
+template <int TYPE>
+__device__ __forceinline__ void etab_shell(double x, double y, double z, double A0, const double* dA, const double* HA,
+                                           int first, int cnt, double a, const double* c, double* out,
+                                           size_t tstride) {
+  constexpr int n = TYPE == 0 ? 1 : TYPE == -2 ? 5 : TYPE == -3 ? 7 : 3;
+  double q[n], g[n][3], h[n][6];
+  angular_h<TYPE>(x, y, z, q, g, h);
+  const int hi[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+  for (int m = 0; m < n; ++m) {
+    const int k = m - first;
+    if (k < 0 || k >= cnt) continue;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const double dt = dA[t] * q[m] + A0 * g[m][t];  // d_t phi
+      double cd = 0;                                  // (c . grad) d_t phi
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        cd += c[r] * (HA[hi[r][t]] * q[m] + dA[r] * g[m][t] + dA[t] * g[m][r] + A0 * h[m][hi[r][t]]);
+      out[t * tstride + k] = a * dt + cd;
+    }
+  }
+}
+
+// One wave per slot (batch, tile), lane = point: E^t of the tile's AOs into
+// etab[t][slot][g][column] (columns past the tile's AOs keep their zeros).
+__global__ __launch_bounds__(256) void k_skel_etab(int nslots, const int* __restrict__ slot_batch,
+                                                   const int* __restrict__ slot_tile, const double* __restrict__ bx,
+                                                   const double* __restrict__ by, const double* __restrict__ bz,
+                                                   BasisView bv, const double* __restrict__ coef,
+                                                   double* __restrict__ etab) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + wave;
+  if (slot >= nslots) return;
+  const int b = slot_batch[slot], t = slot_tile[slot];
+  const size_t p = (size_t)b * kBatch + g;
+  const double px = bx[p], py = by[p], pz = bz[p];
+  const double a = coef[kCoef * p];
+  const double c[3] = {coef[kCoef * p + 1], coef[kCoef * p + 2], coef[kCoef * p + 3]};
+  double* out0 = etab + (size_t)slot * kGSlotB + (size_t)g * kTile;
+  const size_t tstride = (size_t)nslots * kGSlotB;
+  const int s0 = bv.tile_seg_off[t], s1 = bv.tile_seg_off[t + 1];
+  for (int k = s0; k < s1; ++k) {
+    const int4 sg = bv.seg[k];
+    const int sh = sg.x;
+    const double x = px - bv.sh_xyz[3 * sh], y = py - bv.sh_xyz[3 * sh + 1], z = pz - bv.sh_xyz[3 * sh + 2];
+    const double r2 = x * x + y * y + z * z;
+    double A0 = 0, A1 = 0, A2 = 0;
+    for (int i = 0, po = bv.sh_poff[sh]; i < bv.sh_nprim[sh]; ++i) {
+      const double e = bv.p_exp[po + i];
+      const double v = bv.p_coef[po + i] * ::exp(-e * r2);
+      A0 += v;
+      A1 += e * v;
+      A2 += e * e * v;
+    }
+    const double d[3] = {x, y, z};
+    double dA[3], HA[6];
+    for (int r = 0; r < 3; ++r) dA[r] = -2 * d[r] * A1;
+    HA[0] = 4 * x * x * A2 - 2 * A1; HA[1] = 4 * x * y * A2; HA[2] = 4 * x * z * A2;
+    HA[3] = 4 * y * y * A2 - 2 * A1; HA[4] = 4 * y * z * A2; HA[5] = 4 * z * z * A2 - 2 * A1;
+    double* cols = out0 + sg.w;  // the segment's first column
+    switch (bv.sh_type[sh]) {
+      case 0: etab_shell<0>(x, y, z, A0, dA, HA, sg.y, sg.z, a, c, cols, tstride); break;
+      case 1: etab_shell<1>(x, y, z, A0, dA, HA, sg.y, sg.z, a, c, cols, tstride); break;
+      case -1: etab_shell<-1>(x, y, z, A0, dA, HA, sg.y, sg.z, a, c, cols, tstride); break;
+      case -2: etab_shell<-2>(x, y, z, A0, dA, HA, sg.y, sg.z, a, c, cols, tstride); break;
+      default: etab_shell<-3>(x, y, z, A0, dA, HA, sg.y, sg.z, a, c, cols, tstride); break;
+    }
+  }
+}
+
+void launch_skel_etab(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const double* bx,
+                      const double* by, const double* bz, BasisView bv, const double* coef, double* etab) {
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_skel_etab, dim3((nslots + 3) / 4), dim3(256), 0, s, nslots, slot_batch, slot_tile, bx, by, bz,
+                     bv, coef, etab);
+  CHX_HIP(hipGetLastError());
+}
+
+// F[3 A + t] in the reference's order (column-major), from the internal Q^t
+// (qp[t]) and P^t (qp[3 + t]), npad x npad row-major [mu][nu]:
+//   F[row, col] = -( [atom(row) = A] N^t[mu][nu] + [atom(col) = A] N^t[nu][mu] ),
+//   N^t[mu][nu] = Q^t[mu][nu] + P^t[nu][mu], mu = ref2int[row], nu = ref2int[col].
+__global__ void k_fock_skel_fin(int n, int npad, int natoms, const double* __restrict__ qp,
+                                const int* __restrict__ ref2int, const int* __restrict__ ao_atom,
+                                double* __restrict__ F) {
+  const size_t nn = (size_t)n * n, tot = (size_t)3 * natoms * nn, pp = (size_t)npad * npad;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(idx / nn), A = m / 3, t = m % 3;
+    const size_t e = idx % nn;
+    const int col = (int)(e / n), row = (int)(e % n);
+    const size_t mu = ref2int[row], nu = ref2int[col];
+    const double* Q = qp + (size_t)t * pp;
+    const double* P = qp + (size_t)(3 + t) * pp;
+    double v = 0.0;
+    if (ao_atom[row] == A) v += Q[mu * npad + nu] + P[nu * npad + mu];
+    if (ao_atom[col] == A) v += Q[nu * npad + mu] + P[mu * npad + nu];
+    F[idx] = -v;
+  }
+}
+
+void launch_fock_skel_fin(hipStream_t s, int n, int npad, int natoms, const double* qp, const int* ref2int,
+                          const int* ao_atom, double* F) {
+  const size_t tot = (size_t)3 * natoms * n * n;
+  if (tot == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_fock_skel_fin, dim3(blocks), dim3(256), 0, s, n, npad, natoms, qp, ref2int, ao_atom, F);
+  CHX_HIP(hipGetLastError());
+}
+
 }  // namespace chx

@@ -182,6 +182,20 @@ class Grid {
     nmats_u_ = nmats;
     nspin_u_ = nspin;
   }
+  // Grid::getFockSkeleton (Restricted/Hess.cpp:94): one matrix per nuclear
+  // coordinate, [spin][3 atom + t], the potential held fixed.
+  std::vector<std::vector<EigenMatrix>> getFockSkeleton() {
+    const int nmats = 3 * natoms_;
+    std::vector<double> buf((size_t)nmats * nbasis_ * nbasis_);
+    check(chx_grid_fock_skeleton(h_, buf.data(), 1, nbasis_));
+    std::vector<std::vector<EigenMatrix>> F(1);
+    for (int m = 0; m < nmats; ++m) {
+      F[0].push_back(make_matrix(nbasis_, nbasis_));
+      std::copy(buf.begin() + (size_t)m * nbasis_ * nbasis_, buf.begin() + (size_t)(m + 1) * nbasis_ * nbasis_,
+                F[0].back().data());
+    }
+    return F;
+  }
   // D_t = u_t: the matrices of the last getDensityU. D_t = s_t (the Hessian's
   // skeleton term, Restricted/Hess.cpp:96): one matrix per nuclear coordinate,
   // [3 atom + t], from the last getDensitySkeleton.

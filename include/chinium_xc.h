@@ -167,6 +167,15 @@ int chx_grid_fock_u(chx_grid* g, double* F, int nmats, int nspin, int nbasis);
  * [3 A + t] (the reference's Fs[0][jpert]). Closed shell, LDA/GGA, FP64 form
  * with the AO tables in HBM. */
 int chx_grid_fock_s(chx_grid* g, double* F, int nspin, int nbasis);
+/* Grid::getFockSkeleton (src/Grid/Grid.cpp:173-191, SubGrid::getFockSkeleton,
+ * src/Grid/GridFock.cpp:51-97), as the Hessian calls it
+ * (src/HartreeFockKohnSham/Restricted/Hess.cpp:94): the derivative of the XC
+ * Fock matrix along each nuclear coordinate with the potential (v_rho,
+ * v_sigma, grad rho of the last getDensity / Evaluate("ev")) and the grid
+ * held fixed; with chx_grid_fock_s it makes the total derivative. Writes 3
+ * natoms column-major nbasis^2 matrices, [3 A + t] (the reference's
+ * Fs[0][jpert]). Closed shell, LDA/GGA, FP64 form with the AO tables in HBM. */
+int chx_grid_fock_skeleton(chx_grid* g, double* F, int nspin, int nbasis);
 /* Supply v_rho directly, in the library's point order, as the SAP guess does
  * when it writes SubGrid::Eps1Rho before Grid::getFock
  * (src/HartreeFockKohnSham/GuessSCF.cpp:37-60). Other potentials are zeroed. */

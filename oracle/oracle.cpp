@@ -1362,6 +1362,81 @@ int ox_fock_s(ox_system* s, const double* D, double* Fx) {
   });
 }
 
+// Skeleton derivative of the XC Fock matrix, the potential held fixed
+// (Restricted/Hess.cpp:94 -> Grid::getFockSkeleton, Grid.cpp:173-191 ->
+// SubGrid::getFockSkeleton, GridFock.cpp:51-97): for each atom A and direction t,
+//   M[mu,nu] = sum_g W (Eps1Rho d_t phi_mu phi_nu
+//              + 2 Eps1Sigma sum_r Rho1[g,r] (d_r d_t phi_mu phi_nu + d_t phi_mu d_r phi_nu))
+// over mu in A (GridFock.cpp:57-84; 2 Eps1Sigma is ScaledEps1Sigma at nspin = 1),
+// F[3 A + t] = -(M + M^T) (GridFock.cpp:64, 83, 86). Eps1Rho, Eps1Sigma and Rho1
+// come from the density of D and its evaluation. Closed shell, LDA/GGA.
+int ox_fock_skeleton(ox_system* s, const double* D, double* Fx) {
+  return guarded([&] {
+    if (s->family > 1) throw std::runtime_error("oracle: the skeleton Fock derivative is LDA/GGA");
+    if (ox_density(s, D) || ox_evaluate(s)) throw std::runtime_error(g_err);
+    const int nb = s->nbasis, na = s->natoms, n = (int)s->active.size(), nt = s->nthreads;
+    const size_t nb2 = (size_t)nb * nb, npert = (size_t)3 * na;
+    std::vector<int> ao_atom(nb);
+    for (size_t is = 0, mu = 0; is < s->shells.size(); ++is)
+      for (size_t m = 0; m < s->shell_ang[is].size(); ++m) ao_atom[mu++] = s->shells[is].center;
+    // M^t[mu][nu] for every mu (each AO belongs to one atom), summed over the sub-grids
+    std::vector<std::vector<double>> part(nt, std::vector<double>(3 * nb2, 0.0));
+#pragma omp parallel num_threads(nt)
+    {
+      std::vector<double> ao, ao1, ao2, c;
+      double* M = part[omp_get_thread_num()].data();
+#pragma omp for schedule(dynamic, 1)
+      for (int i = 0; i < n; ++i) {
+        const SubGrid& sg = s->subgrids[s->active[i]];
+        const int ng = sg.npad;
+        ao_hessian_tables(s, sg, ao, ao1, ao2);
+        for (int t = 0; t < 3; ++t)
+          for (int mu = 0; mu < nb; ++mu) {
+            // per point: the coefficient of phi_nu (c0) and of d_r phi_nu (c1..3)
+            c.assign(4 * (size_t)ng, 0.0);
+            for (int g = 0; g < ng; ++g) {
+              const double dt = ao1[((size_t)t * nb + mu) * ng + g];
+              double c0 = sg.vrho[g] * dt;
+              if (s->family >= 1)
+                for (int r = 0; r < 3; ++r) {
+                  const double s2 = 2 * sg.vsigma[g] * sg.rho1[(size_t)r * ng + g];
+                  c0 += s2 * ao2[((size_t)hidx(r, t) * nb + mu) * ng + g];
+                  c[4 * (size_t)g + 1 + r] = s2 * dt;
+                }
+              c[4 * (size_t)g] = c0;
+            }
+            for (int nu = 0; nu < nb; ++nu) {
+              double acc = 0;
+              for (int g = 0; g < ng; ++g) {
+                double v = c[4 * (size_t)g] * ao[(size_t)nu * ng + g];
+                if (s->family >= 1)
+                  for (int r = 0; r < 3; ++r) v += c[4 * (size_t)g + 1 + r] * ao1[((size_t)r * nb + nu) * ng + g];
+                acc += sg.w[g] * v;
+              }
+              M[(size_t)t * nb2 + (size_t)nu * nb + mu] += acc;
+            }
+          }
+      }
+    }
+    std::vector<double> Mt(3 * nb2, 0.0);
+    for (int k = 0; k < nt; ++k)
+      for (size_t q = 0; q < Mt.size(); ++q) Mt[q] += part[k][q];
+    std::memset(Fx, 0, sizeof(double) * npert * nb2);
+    for (int A = 0; A < na; ++A)
+      for (int t = 0; t < 3; ++t) {
+        double* F = Fx + (size_t)(3 * A + t) * nb2;
+        const double* Ms = Mt.data() + (size_t)t * nb2;  // column-major: Ms[nu * nb + mu] = M[mu][nu]
+        for (int nu = 0; nu < nb; ++nu)
+          for (int mu = 0; mu < nb; ++mu) {
+            double v = 0;
+            if (ao_atom[mu] == A) v += Ms[(size_t)nu * nb + mu];
+            if (ao_atom[nu] == A) v += Ms[(size_t)mu * nb + nu];
+            F[(size_t)nu * nb + mu] = -v;
+          }
+      }
+  });
+}
+
 // Moves the basis functions of one atom (not its grid); the AO tables must be
 // rebuilt with ox_get_ao. Used by finite-difference checks of ox_xc_grad.
 int ox_shift_basis(ox_system* s, int atom, double dx, double dy, double dz) {

@@ -102,6 +102,10 @@ int ox_fock_u(ox_system* s, int nmats, const double* Dx, double* Fx);
  * closed shell, LDA/GGA; D is the ground-state density. Fx: 3 natoms
  * column-major nbasis^2 matrices, [3 atom + t]. */
 int ox_fock_s(ox_system* s, const double* D, double* Fx);
+/* Skeleton derivative of the XC Fock matrix with the potential held fixed
+ * (Grid::getFockSkeleton), closed shell, LDA/GGA, at the density D. Fx: 3 natoms
+ * column-major nbasis^2 matrices, [3 atom + t]. */
+int ox_fock_skeleton(ox_system* s, const double* D, double* Fx);
 
 /* Density quantities on the active sub-grids after ox_density, in the
  * sub-grid order with padding removed: rho, grad (3 per point), sigma, tau. */

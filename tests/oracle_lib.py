@@ -59,6 +59,7 @@ def lib():
         "ox_eval_functional_f": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long] + [_vp] * 6),
         "ox_fock_u": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
         "ox_fock_s": (ctypes.c_int, [_vp, _vp, _vp]),
+        "ox_fock_skeleton": (ctypes.c_int, [_vp, _vp, _vp]),
         "ox_xc_pol": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_pol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_fpol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -200,6 +201,15 @@ class OracleSystem:
         Df = np.asfortranarray(D, dtype=np.float64)
         F = np.zeros((3 * len(self.Z), n * n))
         _check(lib().ox_fock_s(self.h, _ptr(Df), _ptr(F)))
+        return [F[m].reshape((n, n), order="F") for m in range(F.shape[0])]
+
+    def fock_skeleton(self, D):
+        """Skeleton derivative of the XC Fock matrix with the potential held fixed, one matrix per
+        nuclear coordinate [3 atom + t] (Grid::getFockSkeleton)."""
+        n = self.nbasis
+        Df = np.asfortranarray(D, dtype=np.float64)
+        F = np.zeros((3 * len(self.Z), n * n))
+        _check(lib().ox_fock_skeleton(self.h, _ptr(Df), _ptr(F)))
         return [F[m].reshape((n, n), order="F") for m in range(F.shape[0])]
 
     def xc_grad(self, D):

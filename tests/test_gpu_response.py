@@ -200,3 +200,81 @@ def test_skeleton_response_protocol_errors():
     with pytest.raises(ca.ChxError) as e:
         grid.getAO(3, 0)
     assert e.value.code == -5
+
+
+def _skeleton_fock(mol, grid_name, func, D):
+    grid = ca.Grid(mol, grid_name, 1, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read(func)
+    grid.setType(xc.family)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    xc.Evaluate("ev", grid)  # the SCF's last build
+    # the reference's Hessian order (Restricted/Hess.cpp:78-96)
+    grid.getAO(2, 1)
+    xc.Evaluate("f", grid)
+    grid.getDensitySkeleton([D])
+    Fk = grid.getFockSkeleton()[0]
+    Fs = grid.getFockU(d_t="s")[0]
+    return grid, xc, Fk, Fs
+
+
+@pytest.mark.parametrize("func", ["PBE", "B3LYP", "SVWN5"])
+def test_h2o_fock_skeleton_against_oracle(func):
+    """Grid::getFockSkeleton: the XC Fock matrix's derivative along each nuclear coordinate with the
+    potential held fixed, against the oracle (pinned by finite differences of V assembled on one atom's
+    moved basis functions); the sum with getFockU<s_t> against the oracle's sum."""
+    w, mol, D = _h2o()
+    _say("skeleton Fock derivative h2o/cc-pvdz/SG-1 %s, 9 coordinates" % func)
+    grid, xc, Fk, Fs = _skeleton_fock(mol, "SG-1", func, D)
+    orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], "SG-1", w["basis"])
+    orc.set_functional(func)
+    orc.get_ao()
+    Fo = orc.fock_skeleton(D)
+    Fso = orc.fock_s(D)
+    assert len(Fk) == len(Fo) == 9
+    scale = max(float(np.abs(G).max()) for G in Fo)
+    worst = max(float(np.abs(F - G).max()) for F, G in zip(Fk, Fo))
+    worst_sum = max(float(np.abs(F + S - G - T).max()) for F, S, G, T in zip(Fk, Fs, Fo, Fso))
+    _say("  max |F - F_oracle| %.2e, of the total derivative %.2e, max |F| %.3f" % (worst, worst_sum, scale))
+    assert worst <= F_TOL and worst_sum <= F_TOL
+    assert all(np.abs(F - F.T).max() <= 1e-12 * scale for F in Fk)
+
+
+def test_water_cluster_fock_skeleton():
+    Z, X = wl.water_cluster_angstrom(4, 5.0, 11)
+    xyz = X * ca.ANGSTROM_TO_BOHR
+    mol = ca.Molecule(Z, xyz, "def2-svp")
+    D, S = wl.workload_density(mol, 20, 3)
+    _say("skeleton Fock derivative (H2O)4/def2-svp/SG-1 B3LYP, 36 coordinates")
+    grid, xc, Fk, Fs = _skeleton_fock(mol, "SG-1", "B3LYP", D)
+    orc = ol.OracleSystem(Z, xyz, "SG-1", "def2-svp")
+    orc.set_functional("B3LYP")
+    orc.get_ao()
+    Fo = orc.fock_skeleton(D)
+    worst = max(float(np.abs(F - G).max()) for F, G in zip(Fk, Fo))
+    _say("  max |F - F_oracle| %.2e" % worst)
+    assert len(Fk) == 36 and worst <= F_TOL
+
+
+def test_fock_skeleton_protocol_errors():
+    w, mol, D = _h2o()
+    grid = ca.Grid(mol, "SG-1", 1, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    grid.setType(1)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    with pytest.raises(ca.ChxError) as e:  # no potential yet
+        grid.getFockSkeleton()
+    assert e.value.code == -3
+    m = ca.Grid(mol, "SG-1", 1, 0)
+    r2 = ca.ExchangeCorrelation()
+    r2.Read("R2SCAN")
+    m.setType(2)
+    m.getAO(0, 0)
+    m.getDensity([D])
+    r2.Evaluate("ev", m)
+    with pytest.raises(ca.ChxError) as e:  # meta-GGA: not built (the reference's has no tau term)
+        m.getFockSkeleton()
+    assert e.value.code == -5

@@ -163,3 +163,66 @@ def test_fock_s_rejects_meta_gga(h2o):
     o.get_ao()
     with pytest.raises(RuntimeError):
         o.fock_s(D)
+
+
+@pytest.mark.parametrize("func", ["PBE", "B3LYP", "SVWN5"])
+def test_fock_skeleton_is_explicit_derivative_of_v(h2o, func):
+    """Grid::getFockSkeleton (GridFock.cpp:51-97) is the derivative of V along a shift of one atom's
+    basis functions with the potential (v_rho, v_sigma, grad rho of D) and the grid held fixed; with
+    getFockU<s_t> (the response of the potential to the same shift) it makes the total derivative
+    of V[D] at fixed D."""
+    o, D = h2o
+    o.set_functional(func)
+    o.get_ao()
+    codes = _df_codes(func)
+    gga = func != "SVWN5"
+    Fk = o.fock_skeleton(D)
+    Fs = o.fock_s(D)
+    xyz, _, _, _, w = o.density_at(D)
+    ao0, g0, _ = o.ao_values(xyz)
+
+    def dens(ao, g):
+        rho = np.einsum("mg,mn,ng->g", ao, D, ao)
+        grad = 2 * np.einsum("rmg,mn,ng->rg", g, D, ao)
+        return rho, grad
+
+    rho0, grad0 = dens(ao0, g0)
+    _, vr0, vs0, _ = ol.eval_functional(codes, rho0, np.sum(grad0 ** 2, axis=0) if gga else None)
+
+    def V_on(ao, g, vr, vs, grad):  # V of the potential (vr, vs, grad) assembled on the AOs (ao, g)
+        V = (ao * (w * vr)) @ ao.T
+        if gga:
+            Y = np.einsum("rg,rmg->mg", 2 * w * vs * grad, g)
+            V += Y @ ao.T + ao @ Y.T
+        return V
+
+    def V_full(ao, g):
+        rho, grad = dens(ao, g)
+        _, vr, vs, _ = ol.eval_functional(codes, rho, np.sum(grad ** 2, axis=0) if gga else None)
+        return V_on(ao, g, vr, vs, grad)
+
+    h = 1e-4
+    for A, t in ((0, 0), (0, 2), (1, 1), (2, 0)):
+        def at(eps, fn):
+            d = np.zeros(3); d[t] = eps
+            o.shift_basis(A, d)
+            ao, g, _ = o.ao_values(xyz)
+            o.shift_basis(A, -d)
+            return fn(ao, g)
+
+        def fd(fn):
+            d1 = (at(h, fn) - at(-h, fn)) / (2 * h)
+            d2 = (at(h / 2, fn) - at(-h / 2, fn)) / h
+            return (4 * d2 - d1) / 3
+
+        F = Fk[3 * A + t]
+        ref = fd(lambda ao, g: V_on(ao, g, vr0, vs0, grad0))
+        assert np.abs(F - F.T).max() <= 1e-13 * np.abs(F).max()
+        assert np.abs(F - ref).max() <= 1e-8 * np.abs(F).max(), (A, t, np.abs(F - ref).max(), np.abs(F).max())
+        tot = fd(V_full)
+        S = F + Fs[3 * A + t]
+        assert np.abs(S - tot).max() <= 1e-8 * np.abs(S).max()
+    with pytest.raises(RuntimeError):
+        o.set_functional("R2SCAN")
+        o.get_ao()
+        o.fock_skeleton(D)
