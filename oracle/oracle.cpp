@@ -107,6 +107,37 @@ void poly_eval_h(const Poly& p, double x, double y, double z, double* q, double*
   for (int k = 0; k < 6; ++k) qh[k] = h[k];
 }
 
+// Third derivatives of a polynomial, (xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz).
+void poly_eval_3(const Poly& p, double x, double y, double z, double* q3) {
+  static const int idx[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                                 {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+  const double b[3] = {x, y, z};
+  for (int k = 0; k < 10; ++k) {
+    double v = 0;
+    for (const Mono& m : p) {
+      int e[3] = {m.i, m.j, m.k};
+      double c = m.c;
+      for (int d = 0; d < 3; ++d) {  // differentiate once along each index
+        c *= e[idx[k][d]];
+        --e[idx[k][d]];
+      }
+      if (c == 0) continue;
+      v += c * ipow(b[0], e[0]) * ipow(b[1], e[1]) * ipow(b[2], e[2]);
+    }
+    q3[k] = v;
+  }
+}
+
+// (r, s, t) -> position in (xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz)
+inline int tidx(int r, int s, int t) {
+  int a[3] = {r, s, t};
+  std::sort(a, a + 3);
+  static const int map[3][3][3] = {{{0, 1, 2}, {1, 3, 4}, {2, 4, 5}},
+                                   {{1, 3, 4}, {3, 6, 7}, {4, 7, 8}},
+                                   {{2, 4, 5}, {4, 7, 8}, {5, 8, 9}}};
+  return map[a[0]][a[1]][a[2]];
+}
+
 inline int hidx(int r, int t) {  // (r, t) -> position in (xx, xy, xz, yy, yz, zz)
   static const int map[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
   return map[r][t];
@@ -1359,6 +1390,175 @@ int ox_fock_s(ox_system* s, const double* D, double* Fx) {
     std::memset(Fx, 0, sizeof(double) * npert * nb2);
     for (int t = 0; t < nt; ++t)
       for (size_t k = 0; k < npert * nb2; ++k) Fx[k] += part[t][k];
+  });
+}
+
+// Third derivatives of every AO at a sub-grid's points, ao3[tidx][mu][g]:
+// phi = R Q with R = sum c exp(-alpha r^2), the product rule over R's and Q's
+// derivatives, R_ijk = -8 x_i x_j x_k A3 + 4 (d_ij x_k + d_ik x_j + d_jk x_i) A2.
+static void ao_third_tables(const ox_system* s, const SubGrid& sg, std::vector<double>& ao3) {
+  const int nb = s->nbasis, ng = sg.npad;
+  ao3.assign((size_t)10 * nb * ng, 0);
+  static const int idx[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                                 {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+  int mu0 = 0;
+  for (size_t is = 0; is < s->shells.size(); ++is) {
+    const Shell& sh = s->shells[is];
+    const double* c = &s->xyz[3 * sh.center];
+    for (int g = 0; g < ng; ++g) {
+      const double d[3] = {sg.x[g] - c[0], sg.y[g] - c[1], sg.z[g] - c[2]};
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double A[4] = {0, 0, 0, 0};
+      for (size_t p = 0; p < sh.exps.size(); ++p) {
+        const double t = sh.coefs[p] * std::exp(-sh.exps[p] * r2);
+        A[0] += t; A[1] += t * sh.exps[p]; A[2] += t * sh.exps[p] * sh.exps[p];
+        A[3] += t * sh.exps[p] * sh.exps[p] * sh.exps[p];
+      }
+      auto R1 = [&](int i) { return -2 * d[i] * A[1]; };
+      auto R2 = [&](int i, int j) { return 4 * d[i] * d[j] * A[2] - (i == j ? 2 * A[1] : 0.0); };
+      auto R3 = [&](int i, int j, int k) {
+        return -8 * d[i] * d[j] * d[k] * A[3] +
+               4 * ((i == j ? d[k] : 0.0) + (i == k ? d[j] : 0.0) + (j == k ? d[i] : 0.0)) * A[2];
+      };
+      for (size_t m = 0; m < s->shell_ang[is].size(); ++m) {
+        double q, qg[3], qh[6], q3[10];
+        poly_eval_h(s->shell_ang[is][m], d[0], d[1], d[2], &q, qg, qh);
+        poly_eval_3(s->shell_ang[is][m], d[0], d[1], d[2], q3);
+        const size_t mu = mu0 + m;
+        for (int k = 0; k < 10; ++k) {
+          const int i = idx[k][0], j = idx[k][1], l = idx[k][2];
+          const double v = R3(i, j, l) * q + R2(i, j) * qg[l] + R2(i, l) * qg[j] + R2(j, l) * qg[i] +
+                           R1(i) * qh[hidx(j, l)] + R1(j) * qh[hidx(i, l)] + R1(l) * qh[hidx(i, j)] + A[0] * q3[k];
+          ao3[((size_t)k * nb + mu) * ng + g] = v;
+        }
+      }
+    }
+    mu0 += (int)s->shell_ang[is].size();
+  }
+}
+
+// XC energy Hessian in the reference's skeleton form, the grid held fixed
+// (Restricted/Hess.cpp:78-90 -> SubGrid::getDensitySkeleton2, GridDensity.cpp:132-209,
+// and SubGrid::getEnergyHess, GridEnergy.cpp:32-45), closed shell, LDA/GGA: with
+// a = (A, t), b = (B, s) and the skeleton density derivatives
+//   rho^a = -2 sum_{mu in A} d_t phi_mu X_mu,   X = D phi, Y^r = D d_r phi,
+//   rho^ab = 2 [ [A = B] sum_{mu in A} d_t d_s phi_mu X_mu
+//                + sum_{mu in A, nu in B} D_mu,nu d_t phi_mu d_s phi_nu ]
+// and their gradients, sigma^a = 2 grad rho . grad rho^a,
+// sigma^ab = 2 (grad rho . grad rho^ab + grad rho^a . grad rho^b):
+//   H[a][b] = sum_g W (Eps2Rho2 rho^a rho^b + Eps2RhoSigma (rho^a sigma^b + sigma^a rho^b)
+//             + Eps2Sigma2 sigma^a sigma^b + Eps1Rho rho^ab + Eps1Sigma sigma^ab).
+// H: 3 natoms x 3 natoms, row-major, [3 A + t][3 B + s].
+int ox_xc_hess(ox_system* s, const double* D, double* H) {
+  return guarded([&] {
+    if (s->family > 1) throw std::runtime_error("oracle: the XC Hessian is built for LDA and GGA");
+    if (ox_density(s, D) || ox_evaluate(s)) throw std::runtime_error(g_err);
+    const int nb = s->nbasis, na = s->natoms, n = (int)s->active.size(), nt = s->nthreads, np = 3 * na;
+    const bool gga = s->family >= 1;
+    std::vector<int> ao_atom(nb);
+    for (size_t is = 0, mu = 0; is < s->shells.size(); ++is)
+      for (size_t m = 0; m < s->shell_ang[is].size(); ++m) ao_atom[mu++] = s->shells[is].center;
+    std::vector<std::vector<double>> part(nt, std::vector<double>((size_t)np * np, 0.0));
+#pragma omp parallel num_threads(nt)
+    {
+      std::vector<double> ao, ao1, ao2, ao3, X, U;
+      std::vector<double> r1((size_t)np * 4), r2((size_t)np * np * 4);
+      double* Hl = part[omp_get_thread_num()].data();
+#pragma omp for schedule(dynamic, 1)
+      for (int i = 0; i < n; ++i) {
+        const SubGrid& sg = s->subgrids[s->active[i]];
+        const int ng = sg.npad;
+        ao_hessian_tables(s, sg, ao, ao1, ao2);
+        if (gga) ao_third_tables(s, sg, ao3);
+        X.assign((size_t)nb * ng, 0); U.assign((size_t)3 * nb * ng, 0);
+        for (int mu = 0; mu < nb; ++mu)
+          for (int nu = 0; nu < nb; ++nu) {
+            const double dd = D[(size_t)nu * nb + mu];
+            if (dd == 0.0) continue;
+            for (int g = 0; g < ng; ++g) X[(size_t)mu * ng + g] += dd * ao[(size_t)nu * ng + g];
+            for (int r = 0; r < 3; ++r)
+              for (int g = 0; g < ng; ++g)
+                U[((size_t)r * nb + mu) * ng + g] += dd * ao1[((size_t)r * nb + nu) * ng + g];
+          }
+        for (int g = 0; g < ng; ++g) {
+          if (sg.w[g] == 0.0) continue;
+          auto A1 = [&](int t, int mu) { return ao1[((size_t)t * nb + mu) * ng + g]; };
+          auto A2 = [&](int r, int t, int mu) { return ao2[((size_t)hidx(r, t) * nb + mu) * ng + g]; };
+          auto A3 = [&](int r, int t, int u, int mu) { return ao3[((size_t)tidx(r, t, u) * nb + mu) * ng + g]; };
+          std::fill(r1.begin(), r1.end(), 0.0);
+          std::fill(r2.begin(), r2.end(), 0.0);
+          // first order: r1[a][0] = rho^a, r1[a][1 + r] = d_r rho^a
+          for (int mu = 0; mu < nb; ++mu) {
+            const int A = ao_atom[mu];
+            const double x = X[(size_t)mu * ng + g];
+            for (int t = 0; t < 3; ++t) {
+              double* q = &r1[(size_t)(3 * A + t) * 4];
+              q[0] += -2 * A1(t, mu) * x;
+              if (gga)
+                for (int r = 0; r < 3; ++r)
+                  q[1 + r] += -2 * (A2(r, t, mu) * x + A1(t, mu) * U[((size_t)r * nb + mu) * ng + g]);
+            }
+          }
+          // second order, diagonal atom blocks
+          for (int mu = 0; mu < nb; ++mu) {
+            const int A = ao_atom[mu];
+            const double x = X[(size_t)mu * ng + g];
+            for (int t = 0; t < 3; ++t)
+              for (int u = 0; u < 3; ++u) {
+                double* q = &r2[((size_t)(3 * A + t) * np + 3 * A + u) * 4];
+                q[0] += 2 * A2(t, u, mu) * x;
+                if (gga)
+                  for (int r = 0; r < 3; ++r)
+                    q[1 + r] += 2 * (A3(r, t, u, mu) * x + A2(t, u, mu) * U[((size_t)r * nb + mu) * ng + g]);
+              }
+          }
+          // second order, D_mu,nu d_t phi_mu d_s phi_nu
+          for (int mu = 0; mu < nb; ++mu)
+            for (int nu = 0; nu < nb; ++nu) {
+              const double dd = D[(size_t)nu * nb + mu];
+              if (dd == 0.0) continue;
+              const int A = ao_atom[mu], B = ao_atom[nu];
+              for (int t = 0; t < 3; ++t)
+                for (int u = 0; u < 3; ++u) {
+                  double* q = &r2[((size_t)(3 * A + t) * np + 3 * B + u) * 4];
+                  q[0] += 2 * dd * A1(t, mu) * A1(u, nu);
+                  if (gga)
+                    for (int r = 0; r < 3; ++r)
+                      q[1 + r] += 2 * dd * (A2(r, t, mu) * A1(u, nu) + A1(t, mu) * A2(r, u, nu));
+                }
+            }
+          // the functional's derivatives at the point (summed over the codes)
+          double vs = 0, frr = 0, frs = 0, fss = 0;
+          for (int code : s->codes) {
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            oxf::eval_point_f(code, sg.rho[g], gga ? sg.sigma[g] : 0.0, &a0, &a1, &a2, &a3);
+            vs += a0; frr += a1; frs += a2; fss += a3;
+          }
+          (void)vs;
+          const double gr[3] = {sg.rho1[g], sg.rho1[(size_t)ng + g], sg.rho1[(size_t)2 * ng + g]};
+          const double w = sg.w[g], vr = sg.vrho[g], vsg = gga ? sg.vsigma[g] : 0.0;
+          for (int a = 0; a < np; ++a) {
+            const double* qa = &r1[(size_t)a * 4];
+            const double sa = gga ? 2 * (gr[0] * qa[1] + gr[1] * qa[2] + gr[2] * qa[3]) : 0.0;
+            for (int b = 0; b < np; ++b) {
+              const double* qb = &r1[(size_t)b * 4];
+              const double* qab = &r2[((size_t)a * np + b) * 4];
+              double e = frr * qa[0] * qb[0] + vr * qab[0];
+              if (gga) {
+                const double sb = 2 * (gr[0] * qb[1] + gr[1] * qb[2] + gr[2] * qb[3]);
+                const double sab = 2 * (gr[0] * qab[1] + gr[1] * qab[2] + gr[2] * qab[3] + qa[1] * qb[1] +
+                                        qa[2] * qb[2] + qa[3] * qb[3]);
+                e += frs * (qa[0] * sb + sa * qb[0]) + fss * sa * sb + vsg * sab;
+              }
+              Hl[(size_t)a * np + b] += w * e;
+            }
+          }
+        }
+      }
+    }
+    std::memset(H, 0, sizeof(double) * np * np);
+    for (int t = 0; t < nt; ++t)
+      for (size_t k = 0; k < (size_t)np * np; ++k) H[k] += part[t][k];
   });
 }
 

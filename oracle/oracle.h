@@ -106,6 +106,10 @@ int ox_fock_s(ox_system* s, const double* D, double* Fx);
  * (Grid::getFockSkeleton), closed shell, LDA/GGA, at the density D. Fx: 3 natoms
  * column-major nbasis^2 matrices, [3 atom + t]. */
 int ox_fock_skeleton(ox_system* s, const double* D, double* Fx);
+/* XC energy Hessian in the skeleton form (getDensitySkeleton2, getEnergyHess), the
+ * grid held fixed, closed shell, LDA/GGA, at the density D: H[3 A + t][3 B + s],
+ * row-major 3 natoms x 3 natoms. */
+int ox_xc_hess(ox_system* s, const double* D, double* H);
 
 /* Density quantities on the active sub-grids after ox_density, in the
  * sub-grid order with padding removed: rho, grad (3 per point), sigma, tau. */

@@ -60,6 +60,7 @@ def lib():
         "ox_fock_u": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
         "ox_fock_s": (ctypes.c_int, [_vp, _vp, _vp]),
         "ox_fock_skeleton": (ctypes.c_int, [_vp, _vp, _vp]),
+        "ox_xc_hess": (ctypes.c_int, [_vp, _vp, _vp]),
         "ox_xc_pol": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_pol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp]),
         "ox_eval_functional_fpol": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -211,6 +212,14 @@ class OracleSystem:
         F = np.zeros((3 * len(self.Z), n * n))
         _check(lib().ox_fock_skeleton(self.h, _ptr(Df), _ptr(F)))
         return [F[m].reshape((n, n), order="F") for m in range(F.shape[0])]
+
+    def xc_hess(self, D):
+        """XC energy Hessian (3 natoms x 3 natoms), skeleton form with the grid fixed."""
+        Df = np.asfortranarray(D, dtype=np.float64)
+        na = len(self.Z)
+        H = np.zeros(9 * na * na)
+        _check(lib().ox_xc_hess(self.h, _ptr(Df), _ptr(H)))
+        return H.reshape(3 * na, 3 * na)
 
     def xc_grad(self, D):
         """XC nuclear gradient (natoms x 3), skeleton form with the grid fixed."""

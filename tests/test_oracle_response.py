@@ -226,3 +226,34 @@ def test_fock_skeleton_is_explicit_derivative_of_v(h2o, func):
         o.set_functional("R2SCAN")
         o.get_ao()
         o.fock_skeleton(D)
+
+
+@pytest.mark.parametrize("func", ["PBE", "B3LYP", "SVWN5"])
+def test_xc_hess_is_derivative_of_gradient(h2o, func):
+    """getDensitySkeleton2 + getEnergyHess (GridDensity.cpp:132-209, GridEnergy.cpp:32-45): the
+    skeleton XC Hessian is the derivative of the skeleton XC gradient (itself pinned as the derivative
+    of E_xc) along a shift of one atom's basis functions, D and the grid fixed; it is symmetric."""
+    o, D = h2o
+    o.set_functional(func)
+    o.get_ao()
+    H = o.xc_hess(D)
+    na = len(o.Z)
+    assert H.shape == (3 * na, 3 * na)
+    assert np.abs(H - H.T).max() <= 1e-10 * np.abs(H).max()
+    h = 1e-4
+
+    def grad_at(B, s, eps):
+        d = np.zeros(3); d[s] = eps
+        o.shift_basis(B, d)
+        o.get_ao()
+        G = o.xc_grad(D).ravel()
+        o.shift_basis(B, -d)
+        o.get_ao()
+        return G
+
+    for B, s in ((0, 0), (1, 2), (2, 1)):
+        d1 = (grad_at(B, s, h) - grad_at(B, s, -h)) / (2 * h)
+        d2 = (grad_at(B, s, h / 2) - grad_at(B, s, -h / 2)) / h
+        fd = (4 * d2 - d1) / 3
+        col = H[:, 3 * B + s]
+        assert np.abs(col - fd).max() <= 1e-7 * np.abs(H).max(), (B, s, np.abs(col - fd).max(), np.abs(H).max())
