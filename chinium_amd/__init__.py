@@ -117,6 +117,8 @@ def lib():
         "chx_grid_fock_u": (i, [_vp, _vp, i, i, i]),
         "chx_grid_fock_s": (i, [_vp, _vp, i, i]),
         "chx_grid_fock_skeleton": (i, [_vp, _vp, i, i]),
+        "chx_grid_get_density_skeleton2": (i, [_vp, _vp, i, i]),
+        "chx_grid_energy_hess": (i, [_vp, _vp]),
         "chx_xc_eval_points_f": (i, [_vp, l, _vp, _vp, _vp, _vp, _vp]),
         "chx_pack_create": (i, [_vp, i, P(_vp)]),
         "chx_pack_xc": (i, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -385,6 +387,20 @@ class Grid:
         buf = np.concatenate([np.asfortranarray(D, dtype=np.float64).ravel(order="F") for x in Ds for D in x])
         self._nmats_u = len(Ds[0])
         _check(lib().chx_grid_get_density_u(self.h, _ptr(buf), len(Ds[0]), len(Ds), n))
+
+    def getDensitySkeleton2(self, Ds):
+        """Grid::getDensitySkeleton2 (src/Grid/Grid.cpp:96-111): D for the XC Hessian (closed shell)."""
+        D = np.asfortranarray(Ds[0], dtype=np.float64)
+        assert len(Ds) == 1 and D.shape == (self.nbasis, self.nbasis)
+        _check(lib().chx_grid_get_density_skeleton2(self.h, _ptr(D), 1, self.nbasis))
+
+    def getEnergyHess(self):
+        """Grid::getEnergyHess (src/Grid/Grid.cpp:139-154): the skeleton XC energy Hessian,
+        (3 natoms) x (3 natoms), [3 A + t][3 B + s]."""
+        m = 3 * self.natoms
+        H = np.zeros(m * m)
+        _check(lib().chx_grid_energy_hess(self.h, _ptr(H)))
+        return H.reshape(m, m)
 
     def getFockSkeleton(self):
         """Grid::getFockSkeleton (src/Grid/Grid.cpp:173-191): the XC Fock matrix's derivative along each

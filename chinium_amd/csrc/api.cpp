@@ -676,6 +676,70 @@ int chx_grid_fock_skeleton(chx_grid* g, double* F, int nspin, int nbasis) {
   });
 }
 
+int chx_grid_get_density_skeleton2(chx_grid* g, const double* D, int nspin, int nbasis) {
+  // the second-order skeleton densities are formed inside chx_grid_energy_hess from the same D
+  return chx_grid_get_density_skeleton(g, D, nspin, nbasis);
+}
+
+int chx_grid_energy_hess(chx_grid* g, double* H) {
+  return guard([&] {
+    require(H != nullptr, CHX_ERR_ARG, "null output");
+    require(g->have_skeleton, CHX_ERR_STATE, "Grid::getDensitySkeleton2 must precede Grid::getEnergyHess");
+    require(g->have_v, CHX_ERR_STATE, "ExchangeCorrelation::Evaluate(\"ev\") must precede Grid::getEnergyHess");
+    require(g->fx_family >= 0, CHX_ERR_STATE, "Evaluate(\"f\") must precede Grid::getEnergyHess");
+    require(g->v_family <= 1 && g->type <= 1 && g->fx_family <= 1, CHX_ERR_UNSUPPORTED,
+            "the XC Hessian is built for LDA and GGA");
+    require(g->v_nspin == 1 && g->fx_nspin == 1, CHX_ERR_UNSUPPORTED, "the XC Hessian is built for the closed-shell path");
+    require(g->precision == CHX_PREC_FP64 && g->cached, CHX_ERR_UNSUPPORTED,
+            "the XC Hessian is built for the FP64 form with the AO tables in HBM");
+    const int na = g->mol.natoms, np = 3 * na, ng = (int)g->grp_batch.size(), ns = (int)g->bt_list.size();
+    const long npt = (long)g->nbatches * kBatch;
+    const size_t pp = (size_t)g->npad * g->npad, sb = (size_t)kBatch * kTile;
+    ensure_slot_groups(g);
+    // atom of each internal AO (-1: padding)
+    std::vector<int> ao_atom(g->nbasis, 0), atom_int(g->npad, -1);
+    for (const chx::ShellH& sh : g->mol.shells)
+      for (int k = 0; k < chx::shell_size(sh.type); ++k) ao_atom[sh.ao_off + k] = sh.center;
+    for (int i = 0; i < g->npad; ++i)
+      if (g->int2ref[i] >= 0) atom_int[i] = ao_atom[g->int2ref[i]];
+    DevBuf<int> d_atom;
+    d_atom.upload(atom_int, g->stream);
+    // X = D phi, Y^x = D d_x phi (xz1); Y^y, Y^z (xz2); first-order skeleton densities
+    const size_t nxz = std::max<size_t>(1, (size_t)ng * 2 * kGroup * kTile * kBatch);
+    DevBuf<double> xz1, xz2, rhos, etab, rr, dH;
+    xz1.alloc(nxz);
+    xz2.alloc(nxz);
+    launch_xgrad_xz(g->stream, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
+                    g->d_bt_list.p, g->d_cache.p, g->d_bmat.p, g->d_Dskel.p, g->npad, 0, 1, xz1.p);
+    launch_xgrad_xz(g->stream, ng, g->d_grp_batch.p, g->d_grp_tiles.p, g->d_grp_ntiles.p, g->d_bt_off.p,
+                    g->d_bt_list.p, g->d_cache.p, g->d_bmat.p, g->d_Dskel.p, g->npad, 2, 3, xz2.p);
+    rhos.alloc(std::max<size_t>(1, (size_t)np * 4 * npt));
+    launch_fill(g->stream, rhos.p, (size_t)np * 4 * npt, 0.0);
+    launch_skel_rho(g->stream, ns, g->d_slot_batch.p, g->d_slot_tile.p, g->d_slot_grp.p, g->d_slot_q.p, g->d_x.p,
+                    g->d_y.p, g->d_z.p, g->view(), xz1.p, xz2.p, npt, rhos.p);
+    // rr[3 t + s] = (d_t Phi)^T E^s over the cached contraction
+    launch_make_b(g->stream, g->nbatches, g->d_bt_off.p, g->d_coef.p, g->d_cache.p, g->d_bmat.p);
+    etab.alloc(std::max<size_t>(1, 3 * (size_t)ns * sb));
+    launch_fill(g->stream, etab.p, 3 * (size_t)ns * sb, 0.0);
+    launch_skel_etab(g->stream, ns, g->d_slot_batch.p, g->d_slot_tile.p, g->d_x.p, g->d_y.p, g->d_z.p, g->view(),
+                     g->d_coef.p, etab.p);
+    rr.alloc(9 * pp);
+    launch_fill(g->stream, rr.p, 9 * pp, 0.0);
+    for (int t = 0; t < 3; ++t)
+      for (int u = 0; u < 3; ++u)
+        launch_vxc_c64(g->stream, 1, (int)g->cwork.size(), g->d_cwork.p, g->d_cpair_batches.p, g->d_slot_of.p,
+                       g->ntiles, g->d_cache.p + (size_t)(t + 1) * sb, etab.p + (size_t)u * ns * sb, g->d_coef.p,
+                       rr.p + (size_t)(3 * t + u) * pp, g->npad);
+    dH.alloc((size_t)np * np);
+    launch_fill(g->stream, dH.p, (size_t)np * np, 0.0);
+    launch_hess_terms(g->stream, ns, g->d_slot_batch.p, g->d_slot_tile.p, g->d_slot_grp.p, g->d_slot_q.p, g->d_x.p,
+                      g->d_y.p, g->d_z.p, g->view(), g->d_coef.p, xz1.p, xz2.p, npt, rhos.p, g->d_fx.p, g->d_w.p,
+                      g->fx_family, g->npad, g->d_Dskel.p, rr.p, d_atom.p, np, dH.p);
+    CHX_HIP(hipMemcpyAsync(H, dH.p, sizeof(double) * np * np, hipMemcpyDeviceToHost, g->stream));
+    CHX_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
 int chx_grid_energy_grad(chx_grid* g, double* grad) {
   return guard([&] {
     require(grad != nullptr, CHX_ERR_ARG, "null output");

@@ -276,6 +276,15 @@ void launch_skel_etab(hipStream_t s, int nslots, const int* slot_batch, const in
                       const double* by, const double* bz, BasisView bv, const double* coef, double* etab);
 void launch_fock_skel_fin(hipStream_t s, int n, int npad, int natoms, const double* qp, const int* ref2int,
                           const int* ao_atom, double* F);
+// XC energy Hessian terms added to H (np x np, np = 3 natoms; clear it first):
+// the per-slot diagonal-atom term (AO third derivatives, X and Y^r from xz1 /
+// xz2), the f_xc term of the first-order skeleton densities rhos, and the D
+// term from rr[3 t + s] = (d_t Phi)^T E^s (npad^2 each, internal order).
+void launch_hess_terms(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* slot_grp,
+                       const int* slot_q, const double* bx, const double* by, const double* bz, BasisView bv,
+                       const double* coef, const double* xz1, const double* xz2, long npt, const double* rhos,
+                       const double* fx, const double* bw, int family, int npad, const double* Dint,
+                       const double* rr, const int* atom, int np, double* H);
 void launch_skel_rho(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* slot_grp,
                      const int* slot_q, const double* bx, const double* by, const double* bz, BasisView bv,
                      const double* xz1, const double* xz2, long npt, double* rhos);

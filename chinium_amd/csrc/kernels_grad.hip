@@ -483,4 +483,224 @@ void launch_fock_skel_fin(hipStream_t s, int n, int npad, int natoms, const doub
   CHX_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// XC energy Hessian, skeleton form with the grid held fixed
+// (Restricted/Hess.cpp:78-90 -> getDensitySkeleton2, GridDensity.cpp:132-209,
+// getEnergyHess, GridEnergy.cpp:32-45). With a = (A, t), b = (B, s):
+//   H[a][b] = H1 + H2 + H3,
+//   H1 = sum_g (2 c0^a rho^b + c^a . grad rho^b): the response coefficients of
+//        the first-order skeleton density (rho^a, grad rho^a) contracted with rho^b;
+//   H2 = 2 sum_{mu in A, nu in B} D_mu,nu K^{ts}_mu,nu,
+//        K^{ts} = sum_g (d_t phi_mu E^s_nu + E^t_mu d_s phi_nu), E^s = a d_s phi + (c . grad) d_s phi;
+//   H3 = [A = B] 2 sum_{mu in A} sum_g (2 a d_t d_s phi_mu X_mu
+//        + sum_r c_r (d_r d_t d_s phi_mu X_mu + d_t d_s phi_mu Y^r_mu)).
+
+// Third derivatives of the angular parts (xxx, xxy, xxz, xyy, xyz, xzz, yyy,
+// yyz, yzz, zzz): zero up to d; constants for pure f (angular_h's order).
+template <int TYPE>
+__device__ __forceinline__ void angular_3(double (*k)[10]) {
+  constexpr int n = TYPE == 0 ? 1 : TYPE == -2 ? 5 : TYPE == -3 ? 7 : 3;
+#pragma unroll
+  for (int m = 0; m < n; ++m)
+#pragma unroll
+    for (int j = 0; j < 10; ++j) k[m][j] = 0.0;
+  if (TYPE == -3) {
+    const double r15 = 3.872983346207417, c10 = 0.25 * 3.1622776601683795, c6 = 0.25 * 2.449489742783178;
+    k[0][1] = 6 * c10; k[0][6] = -6 * c10;                    // y (3x^2 - y^2)
+    k[1][4] = r15;                                            // xyz
+    k[2][1] = -2 * c6; k[2][6] = -6 * c6; k[2][8] = 8 * c6;   // y (4z^2 - x^2 - y^2)
+    k[3][9] = 6; k[3][2] = -3; k[3][7] = -3;                  // z (z^2 - 3/2 (x^2 + y^2))
+    k[4][5] = 8 * c6; k[4][0] = -6 * c6; k[4][3] = -2 * c6;   // x (4z^2 - x^2 - y^2)
+    k[5][2] = r15; k[5][7] = -r15;                            // (x^2 - y^2) z
+    k[6][0] = 6 * c10; k[6][3] = -6 * c10;                    // x (x^2 - 3y^2)
+  }
+}
+
+// H3 of components first..first+cnt-1 of one shell at this lane's point:
+// acc[tu] += 2 (2 a phi_tu X + sum_r c_r (phi_rtu X + phi_tu Y^r)), tu over (xx, xy, xz, yy, yz, zz).
+template <int TYPE>
+__device__ __forceinline__ void hess3_shell(double x, double y, double z, double A0, const double* dA, const double* HA,
+                                            const double* TA, int first, int cnt, const double* c1, const double* c2,
+                                            double a, const double* c, double* acc) {
+  constexpr int n = TYPE == 0 ? 1 : TYPE == -2 ? 5 : TYPE == -3 ? 7 : 3;
+  double q[n], g[n][3], h[n][6], k3[n][10];
+  angular_h<TYPE>(x, y, z, q, g, h);
+  angular_3<TYPE>(k3);
+  const int hi[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  const int ti[3][3][3] = {{{0, 1, 2}, {1, 3, 4}, {2, 4, 5}}, {{1, 3, 4}, {3, 6, 7}, {4, 7, 8}},
+                           {{2, 4, 5}, {4, 7, 8}, {5, 8, 9}}};
+  const int pt[6] = {0, 0, 0, 1, 1, 2}, pu[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int m = 0; m < n; ++m) {
+    const int kk = m - first;
+    if (kk < 0 || kk >= cnt) continue;
+    const double X = c1[kk * kBatch];
+    const double Y[3] = {c1[(kGCols + kk) * kBatch], c2[kk * kBatch], c2[(kGCols + kk) * kBatch]};
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      const int t = pt[p], u = pu[p], tu = hi[t][u];
+      const double ptu = HA[tu] * q[m] + dA[t] * g[m][u] + dA[u] * g[m][t] + A0 * h[m][tu];
+      double v = 2 * a * ptu * X;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double prtu = TA[ti[r][t][u]] * q[m] + HA[hi[r][t]] * g[m][u] + HA[hi[r][u]] * g[m][t] +
+                            HA[tu] * g[m][r] + dA[r] * h[m][tu] + dA[t] * h[m][hi[r][u]] +
+                            dA[u] * h[m][hi[r][t]] + A0 * k3[m][ti[r][t][u]];
+        v += c[r] * (prtu * X + ptu * Y[r]);
+      }
+      acc[p] += 2 * v;
+    }
+  }
+}
+
+// One wave per slot, lane = point: H3 summed over the wave per run of one
+// atom's shells and added to H[3 A + t][3 A + u] (np = 3 natoms).
+__global__ __launch_bounds__(256) void k_hess_diag(int nslots, const int* __restrict__ slot_batch,
+                                                   const int* __restrict__ slot_tile, const int* __restrict__ slot_grp,
+                                                   const int* __restrict__ slot_q, const double* __restrict__ bx,
+                                                   const double* __restrict__ by, const double* __restrict__ bz,
+                                                   BasisView bv, const double* __restrict__ coef,
+                                                   const double* __restrict__ xz1, const double* __restrict__ xz2,
+                                                   int np, double* __restrict__ H) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + wave;
+  if (slot >= nslots) return;
+  const int b = slot_batch[slot], t = slot_tile[slot];
+  const size_t p = (size_t)b * kBatch + g;
+  const double px = bx[p], py = by[p], pz = bz[p];
+  const double a = coef[kCoef * p];
+  const double c[3] = {coef[kCoef * p + 1], coef[kCoef * p + 2], coef[kCoef * p + 3]};
+  const size_t col0 = (size_t)slot_grp[slot] * 2 * kGCols * kBatch + (size_t)slot_q[slot] * kTile * kBatch + g;
+  const double* X1 = xz1 + col0;
+  const double* X2 = xz2 + col0;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  int atom = -1;
+  const int pt[6] = {0, 0, 0, 1, 1, 2}, pu[6] = {0, 1, 2, 1, 2, 2};
+  auto flush = [&]() {
+    if (atom < 0) return;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      double v = acc[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      acc[k] = 0.0;
+      if (g == 0) {
+        const int r0 = 3 * atom + pt[k], r1 = 3 * atom + pu[k];
+        unsafeAtomicAdd(&H[(size_t)r0 * np + r1], v);
+        if (r0 != r1) unsafeAtomicAdd(&H[(size_t)r1 * np + r0], v);
+      }
+    }
+  };
+  const int s0 = bv.tile_seg_off[t], s1 = bv.tile_seg_off[t + 1];
+  for (int k = s0; k < s1; ++k) {
+    const int4 sg = bv.seg[k];
+    const int sh = sg.x;
+    if (bv.sh_atom[sh] != atom) {
+      flush();
+      atom = bv.sh_atom[sh];
+    }
+    const double x = px - bv.sh_xyz[3 * sh], y = py - bv.sh_xyz[3 * sh + 1], z = pz - bv.sh_xyz[3 * sh + 2];
+    const double r2 = x * x + y * y + z * z;
+    double A0 = 0, A1 = 0, A2 = 0, A3 = 0;
+    for (int i = 0, po = bv.sh_poff[sh]; i < bv.sh_nprim[sh]; ++i) {
+      const double e = bv.p_exp[po + i];
+      const double v = bv.p_coef[po + i] * ::exp(-e * r2);
+      A0 += v;
+      A1 += e * v;
+      A2 += e * e * v;
+      A3 += e * e * e * v;
+    }
+    const double d[3] = {x, y, z};
+    double dA[3], HA[6], TA[10];
+    for (int r = 0; r < 3; ++r) dA[r] = -2 * d[r] * A1;
+    HA[0] = 4 * x * x * A2 - 2 * A1; HA[1] = 4 * x * y * A2; HA[2] = 4 * x * z * A2;
+    HA[3] = 4 * y * y * A2 - 2 * A1; HA[4] = 4 * y * z * A2; HA[5] = 4 * z * z * A2 - 2 * A1;
+    {
+      const int ix[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                             {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+#pragma unroll
+      for (int j = 0; j < 10; ++j) {
+        const int i0 = ix[j][0], i1 = ix[j][1], i2 = ix[j][2];
+        TA[j] = -8 * d[i0] * d[i1] * d[i2] * A3 +
+                4 * ((i0 == i1 ? d[i2] : 0.0) + (i0 == i2 ? d[i1] : 0.0) + (i1 == i2 ? d[i0] : 0.0)) * A2;
+      }
+    }
+    const double* c1 = X1 + (size_t)sg.w * kBatch;
+    const double* c2 = X2 + (size_t)sg.w * kBatch;
+    switch (bv.sh_type[sh]) {
+      case 0: hess3_shell<0>(x, y, z, A0, dA, HA, TA, sg.y, sg.z, c1, c2, a, c, acc); break;
+      case 1: hess3_shell<1>(x, y, z, A0, dA, HA, TA, sg.y, sg.z, c1, c2, a, c, acc); break;
+      case -1: hess3_shell<-1>(x, y, z, A0, dA, HA, TA, sg.y, sg.z, c1, c2, a, c, acc); break;
+      case -2: hess3_shell<-2>(x, y, z, A0, dA, HA, TA, sg.y, sg.z, c1, c2, a, c, acc); break;
+      default: hess3_shell<-3>(x, y, z, A0, dA, HA, TA, sg.y, sg.z, c1, c2, a, c, acc); break;
+    }
+  }
+  flush();
+}
+
+// H1: block (16 a's x 16 b's, chunk of points), thread (a, b); the response
+// coefficients of rho^a (rhos[a][p][0..3]) with fx (v2rho2, v2rhosigma,
+// v2sigma2, v_sigma, grad rho) contracted with (2 rho^b, grad rho^b).
+__global__ __launch_bounds__(256) void k_hess_fx(long npt, int np, const double* __restrict__ rhos,
+                                                 const double* __restrict__ fx, const double* __restrict__ bw,
+                                                 int family, long chunk, double* __restrict__ H) {
+  const int a = blockIdx.x * 16 + (threadIdx.x >> 4), b = blockIdx.y * 16 + (threadIdx.x & 15);
+  if (a >= np || b >= np) return;
+  const long p0 = (long)blockIdx.z * chunk, p1 = p0 + chunk < npt ? p0 + chunk : npt;
+  double e = 0.0;
+  for (long p = p0; p < p1; ++p) {
+    const double w = bw[p];
+    if (w == 0.0) continue;
+    const double* f = fx + (size_t)kFx * p;
+    const double* u = rhos + ((size_t)a * npt + p) * 4;
+    const double* v = rhos + ((size_t)b * npt + p) * 4;
+    const double sx = 2.0 * (f[4] * u[1] + f[5] * u[2] + f[6] * u[3]);
+    double t = w * (f[0] * u[0] + f[1] * sx) * v[0];  // 2 c0 rho^b
+    if (family >= 1) {
+      const double q = 2.0 * w * (f[1] * u[0] + f[2] * sx), r = 2.0 * w * f[3];
+      t += (q * f[4] + r * u[1]) * v[1] + (q * f[5] + r * u[2]) * v[2] + (q * f[6] + r * u[3]) * v[3];
+    }
+    e += t;
+  }
+  unsafeAtomicAdd(&H[(size_t)a * np + b], e);
+}
+
+// H2: one thread per internal element (mu, nu) of D (npad^2, internal order),
+// nine products added to H[3 A + t][3 B + s] (atom: atom of each internal AO, -1 padding).
+__global__ void k_hess_dk(int npad, int np, const double* __restrict__ Dint, const double* __restrict__ rr,
+                          const int* __restrict__ atom, double* __restrict__ H) {
+  const size_t pp = (size_t)npad * npad;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < pp; idx += (size_t)gridDim.x * blockDim.x) {
+    const int mu = (int)(idx / npad), nu = (int)(idx % npad);
+    const int A = atom[mu], B = atom[nu];
+    if (A < 0 || B < 0) continue;
+    const double dd = Dint[idx];
+    if (dd == 0.0) continue;
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const double k = rr[(size_t)(3 * t + s) * pp + (size_t)mu * npad + nu] +
+                         rr[(size_t)(3 * s + t) * pp + (size_t)nu * npad + mu];
+        unsafeAtomicAdd(&H[(size_t)(3 * A + t) * np + 3 * B + s], 2.0 * dd * k);
+      }
+  }
+}
+
+void launch_hess_terms(hipStream_t s, int nslots, const int* slot_batch, const int* slot_tile, const int* slot_grp,
+                       const int* slot_q, const double* bx, const double* by, const double* bz, BasisView bv,
+                       const double* coef, const double* xz1, const double* xz2, long npt, const double* rhos,
+                       const double* fx, const double* bw, int family, int npad, const double* Dint,
+                       const double* rr, const int* atom, int np, double* H) {
+  if (nslots > 0)
+    hipLaunchKernelGGL(k_hess_diag, dim3((nslots + 3) / 4), dim3(256), 0, s, nslots, slot_batch, slot_tile, slot_grp,
+                       slot_q, bx, by, bz, bv, coef, xz1, xz2, np, H);
+  const long chunk = 4096;
+  const dim3 gf((np + 15) / 16, (np + 15) / 16, (unsigned)((npt + chunk - 1) / chunk));
+  if (npt > 0) hipLaunchKernelGGL(k_hess_fx, gf, dim3(256), 0, s, npt, np, rhos, fx, bw, family, chunk, H);
+  const size_t pp = (size_t)npad * npad;
+  const unsigned blocks = (unsigned)std::min<size_t>((pp + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_hess_dk, dim3(blocks), dim3(256), 0, s, npad, np, Dint, rr, atom, H);
+  CHX_HIP(hipGetLastError());
+}
+
 }  // namespace chx

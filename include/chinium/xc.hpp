@@ -182,6 +182,18 @@ class Grid {
     nmats_u_ = nmats;
     nspin_u_ = nspin;
   }
+  // Grid::getDensitySkeleton2 / getEnergyHess (Restricted/Hess.cpp:83-88).
+  void getDensitySkeleton2(std::vector<EigenMatrix> D) {
+    check(chx_grid_get_density_skeleton2(h_, D.at(0).data(), 1, nbasis_));
+  }
+  std::vector<std::vector<double>> getEnergyHess() {
+    const int m = 3 * natoms_;
+    std::vector<double> buf((size_t)m * m);
+    check(chx_grid_energy_hess(h_, buf.data()));
+    std::vector<std::vector<double>> H(m);
+    for (int k = 0; k < m; ++k) H[k].assign(buf.begin() + (size_t)k * m, buf.begin() + (size_t)(k + 1) * m);
+    return H;
+  }
   // Grid::getFockSkeleton (Restricted/Hess.cpp:94): one matrix per nuclear
   // coordinate, [spin][3 atom + t], the potential held fixed.
   std::vector<std::vector<EigenMatrix>> getFockSkeleton() {

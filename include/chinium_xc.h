@@ -176,6 +176,17 @@ int chx_grid_fock_s(chx_grid* g, double* F, int nspin, int nbasis);
  * natoms column-major nbasis^2 matrices, [3 A + t] (the reference's
  * Fs[0][jpert]). Closed shell, LDA/GGA, FP64 form with the AO tables in HBM. */
 int chx_grid_fock_skeleton(chx_grid* g, double* F, int nspin, int nbasis);
+/* Grid::getDensitySkeleton2 (src/Grid/Grid.cpp:96-111, GridDensity.cpp:132-209)
+ * and Grid::getEnergyHess (Grid.cpp:139-154, GridEnergy.cpp:32-45), as the
+ * Hessian calls them (src/HartreeFockKohnSham/Restricted/Hess.cpp:78-90): the
+ * XC energy's second derivatives along pairs of nuclear coordinates with D and
+ * the grid held fixed. chx_grid_get_density_skeleton2 takes D (as
+ * chx_grid_get_density_skeleton); chx_grid_energy_hess, after it,
+ * chx_xc_evaluate(xc, g, "ev") at D and chx_xc_evaluate(xc, g, "f"), writes H
+ * (3 natoms x 3 natoms, row-major, [3 A + t][3 B + s], the reference's
+ * hess[pert]). Closed shell, LDA/GGA, FP64 form with the AO tables in HBM. */
+int chx_grid_get_density_skeleton2(chx_grid* g, const double* D, int nspin, int nbasis);
+int chx_grid_energy_hess(chx_grid* g, double* H);
 /* Supply v_rho directly, in the library's point order, as the SAP guess does
  * when it writes SubGrid::Eps1Rho before Grid::getFock
  * (src/HartreeFockKohnSham/GuessSCF.cpp:37-60). Other potentials are zeroed. */

@@ -1535,7 +1535,9 @@ int ox_xc_hess(ox_system* s, const double* D, double* H) {
             vs += a0; frr += a1; frs += a2; fss += a3;
           }
           (void)vs;
-          const double gr[3] = {sg.rho1[g], sg.rho1[(size_t)ng + g], sg.rho1[(size_t)2 * ng + g]};
+          double gr[3] = {0, 0, 0};  // rho1 is filled for GGA only
+          if (gga)
+            for (int r = 0; r < 3; ++r) gr[r] = sg.rho1[(size_t)r * ng + g];
           const double w = sg.w[g], vr = sg.vrho[g], vsg = gga ? sg.vsigma[g] : 0.0;
           for (int a = 0; a < np; ++a) {
             const double* qa = &r1[(size_t)a * 4];

@@ -278,3 +278,49 @@ def test_fock_skeleton_protocol_errors():
     with pytest.raises(ca.ChxError) as e:  # meta-GGA: not built (the reference's has no tau term)
         m.getFockSkeleton()
     assert e.value.code == -5
+
+
+@pytest.mark.parametrize("func", ["PBE", "B3LYP", "SVWN5"])
+def test_h2o_energy_hess_against_oracle(func):
+    """Grid::getDensitySkeleton2 + Grid::getEnergyHess: the skeleton XC Hessian against the oracle
+    (pinned by finite differences of the pinned skeleton gradient); symmetric."""
+    w, mol, D = _h2o()
+    _say("XC energy Hessian h2o/cc-pvdz/SG-1 %s" % func)
+    grid = ca.Grid(mol, "SG-1", 1, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read(func)
+    grid.setType(xc.family)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    xc.Evaluate("ev", grid)
+    grid.getAO(2, 1)
+    xc.Evaluate("f", grid)
+    grid.getDensitySkeleton2([D])
+    H = grid.getEnergyHess()
+    orc = ol.OracleSystem(w["Z"], w["xyz_bohr"], "SG-1", w["basis"])
+    orc.set_functional(func)
+    orc.get_ao()
+    Ho = orc.xc_hess(D)
+    err = float(np.abs(H - Ho).max())
+    _say("  max |H - H_oracle| %.2e, max |H| %.3f, asymmetry %.1e" % (err, np.abs(Ho).max(), np.abs(H - H.T).max()))
+    assert H.shape == (9, 9)
+    assert err <= 1e-9 * np.abs(Ho).max()
+    assert np.abs(H - H.T).max() <= 1e-11 * np.abs(Ho).max()
+
+
+def test_energy_hess_protocol_errors():
+    w, mol, D = _h2o()
+    grid = ca.Grid(mol, "SG-1", 1, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    grid.setType(1)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    xc.Evaluate("ev", grid)
+    with pytest.raises(ca.ChxError) as e:  # getDensitySkeleton2 has not run
+        grid.getEnergyHess()
+    assert e.value.code == -3
+    grid.getDensitySkeleton2([D])
+    with pytest.raises(ca.ChxError) as e:  # Evaluate("f") has not run
+        grid.getEnergyHess()
+    assert e.value.code == -3
