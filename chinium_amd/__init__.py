@@ -390,9 +390,8 @@ class Grid:
 
     def getDensitySkeleton2(self, Ds):
         """Grid::getDensitySkeleton2 (src/Grid/Grid.cpp:96-111): D for the XC Hessian (closed shell)."""
-        D = np.asfortranarray(Ds[0], dtype=np.float64)
-        assert len(Ds) == 1 and D.shape == (self.nbasis, self.nbasis)
-        _check(lib().chx_grid_get_density_skeleton2(self.h, _ptr(D), 1, self.nbasis))
+        buf, ns = self._stack(Ds)
+        _check(lib().chx_grid_get_density_skeleton2(self.h, _ptr(buf), ns, self.nbasis))
 
     def getEnergyHess(self):
         """Grid::getEnergyHess (src/Grid/Grid.cpp:139-154): the skeleton XC energy Hessian,

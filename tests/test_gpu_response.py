@@ -280,6 +280,33 @@ def test_fock_skeleton_protocol_errors():
     assert e.value.code == -5
 
 
+def test_water_dimer_energy_hess():
+    """The skeleton XC Hessian on (H2O)2 / def2-SVP / SG-1 / PBE (six atoms: atom pairs across
+    molecules) against the oracle."""
+    Z, X = wl.water_cluster_angstrom(2, 4.0, 5)
+    xyz = X * ca.ANGSTROM_TO_BOHR
+    mol = ca.Molecule(Z, xyz, "def2-svp")
+    D, S = wl.workload_density(mol, 10, 7)
+    grid = ca.Grid(mol, "SG-1", 1, 0)
+    xc = ca.ExchangeCorrelation()
+    xc.Read("PBE")
+    grid.setType(1)
+    grid.getAO(0, 0)
+    grid.getDensity([D])
+    xc.Evaluate("ev", grid)
+    grid.getAO(2, 1)
+    xc.Evaluate("f", grid)
+    grid.getDensitySkeleton2([D])
+    H = grid.getEnergyHess()
+    orc = ol.OracleSystem(Z, xyz, "SG-1", "def2-svp")
+    orc.set_functional("PBE")
+    orc.get_ao()
+    Ho = orc.xc_hess(D)
+    err = float(np.abs(H - Ho).max())
+    _say("  (H2O)2 PBE: max |H - H_oracle| %.2e, max |H| %.3f" % (err, np.abs(Ho).max()))
+    assert H.shape == (18, 18) and err <= 1e-9 * np.abs(Ho).max()
+
+
 @pytest.mark.parametrize("func", ["PBE", "B3LYP", "SVWN5"])
 def test_h2o_energy_hess_against_oracle(func):
     """Grid::getDensitySkeleton2 + Grid::getEnergyHess: the skeleton XC Hessian against the oracle
