@@ -57,6 +57,12 @@ int main(int argc, char** argv) {
       grid.getDensitySkeleton({D});
       const std::vector<std::vector<chinium::EigenMatrix>> Fs = grid.getFockU<chinium::s_t>();
       for (const chinium::EigenMatrix& m : Fs[0]) out.write(reinterpret_cast<const char*>(m.data()), sizeof(double) * n * n);
+      // the Hessian's other XC terms (Restricted/Hess.cpp:83-94)
+      grid.getDensitySkeleton2({D});
+      const std::vector<std::vector<double>> H = grid.getEnergyHess();
+      for (const std::vector<double>& row : H) out.write(reinterpret_cast<const char*>(row.data()), sizeof(double) * row.size());
+      const std::vector<std::vector<chinium::EigenMatrix>> Fk = grid.getFockSkeleton();
+      for (const chinium::EigenMatrix& m : Fk[0]) out.write(reinterpret_cast<const char*>(m.data()), sizeof(double) * n * n);
       // spin-unrestricted call sequence (Unrestricted/SP.cpp:65-68) with Da = 0.6 D, Db = 0.4 D
       chinium::ExchangeCorrelation uxc;
       uxc.Spin = 2;

@@ -44,6 +44,10 @@ def test_cpp_caller_matches_oracle(tmp_path):
     na3 = 3 * len(w["Z"])
     Fs = [out[k + m * n * n:k + (m + 1) * n * n].reshape((n, n), order="F") for m in range(na3)]  # getFockU<s_t>
     k += na3 * n * n
+    H = out[k:k + na3 * na3].reshape(na3, na3)  # getEnergyHess
+    k += na3 * na3
+    Fk = [out[k + m * n * n:k + (m + 1) * n * n].reshape((n, n), order="F") for m in range(na3)]  # getFockSkeleton
+    k += na3 * n * n
     Eu = out[k]
     Va = out[k + 1:k + 1 + n * n].reshape((n, n), order="F")
     Vb = out[k + 1 + n * n:].reshape((n, n), order="F")
@@ -58,6 +62,10 @@ def test_cpp_caller_matches_oracle(tmp_path):
     assert np.abs(F - Fo).max() <= 1e-9
     Fso = orc.fock_s(D)
     assert max(float(np.abs(a - b).max()) for a, b in zip(Fs, Fso)) <= 1e-9
+    Ho = orc.xc_hess(D)
+    assert np.abs(H - Ho).max() <= 1e-9 * np.abs(Ho).max()
+    Fko = orc.fock_skeleton(D)
+    assert max(float(np.abs(a - b).max()) for a, b in zip(Fk, Fko)) <= 1e-9
     Euo, _, Vao, Vbo = orc.xc_pol(0.6 * D, 0.4 * D)
     assert abs(Eu - Euo) <= 1e-10
     assert np.abs(Va - Vao).max() <= 1e-9 and np.abs(Vb - Vbo).max() <= 1e-9
