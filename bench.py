@@ -72,7 +72,7 @@ def main():
     ap.add_argument("--ao-cache", type=int, default=-1, help="-1 auto, 0 recompute AO values, 1 cache them")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="meta-GGA workloads: skip the learned-functional leg (training + builds with it)")
-    ap.add_argument("--surrogate-epochs", type=int, default=100)
+    ap.add_argument("--surrogate-epochs", type=int, default=400)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg")
     ap.add_argument("--no-host-io", action="store_true", help="skip the host-memory D/V leg")
     ap.add_argument("--scf", action="store_true",
